@@ -1,0 +1,169 @@
+/*
+ * gwn_hip.h - C ABI of the MI355X (gfx950) geometric-query library libgwn_hip.so.
+ *
+ * Each entry point replaces one function of the reference's header-only gwn_* C++ API
+ * (kririae/smallgwn, paths relative to its repository root).  Objects are opaque handles
+ * that own stream-ordered device memory, like the reference's gwn_*_object classes; batch
+ * queries take plain device pointers and a count, like its gwn_device_span arguments.
+ *
+ * Every function returns a gwn_error_code.  On failure gwn_last_error_message() returns the
+ * diagnostic of the calling thread's last failure.  Status categories mirror
+ * include/gwn/gwn_utils.cuh:213-218 (success, invalid_argument, cuda_runtime_error -> HIP
+ * runtime error, internal_error).  Batch queries enqueue on the given stream and do not
+ * synchronize (include/gwn/gwn_query.cuh:10-18).  Build and refit synchronize internally,
+ * as the reference's do (include/gwn/detail/gwn_bvh_build_impl.cuh:230-240).
+ *
+ * The C++ template surface that mirrors the reference's names is include/gwn/gwn.hpp.
+ */
+#ifndef GWN_HIP_H
+#define GWN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GWN_HIP_ABI_VERSION 1
+
+typedef enum gwn_error_code {
+    GWN_SUCCESS = 0,
+    GWN_ERROR_INVALID_ARGUMENT = 1,
+    GWN_ERROR_HIP_RUNTIME = 2,
+    GWN_ERROR_INTERNAL = 3
+} gwn_error_code;
+
+/* include/gwn/gwn_bvh_build.cuh:16-27 (enum order kept: H-PLOC first). */
+typedef enum gwn_bvh_build_method {
+    GWN_BVH_BUILD_HPLOC = 0,
+    GWN_BVH_BUILD_LBVH = 1
+} gwn_bvh_build_method;
+
+/*
+ * Arithmetic of a winding query.  GWN_PRECISION_PLAIN is the reference's float arithmetic.
+ * GWN_PRECISION_SPLIT keeps traversal and the Taylor far field in FP32 and evaluates every
+ * near-field (exact) triangle term in FP64.  GWN_PRECISION_FP64 evaluates every term in FP64
+ * (the oracle mode of the exact batch).
+ */
+typedef enum gwn_precision {
+    GWN_PRECISION_PLAIN = 0,
+    GWN_PRECISION_SPLIT = 1,
+    GWN_PRECISION_FP64 = 2
+} gwn_precision;
+
+typedef struct gwn_geometry_handle *gwn_geometry_t;
+typedef struct gwn_bvh_handle *gwn_bvh_t;
+typedef struct gwn_moment_handle *gwn_moment_t;
+
+/* Reference gwn_query_batch_config (include/gwn/gwn_query.cuh:41-61): block size 256 and
+ * stack capacity 64 by default.  sort_queries: -1 auto, 0 off, 1 on (queries are traversed in
+ * Morton order of their positions; results are written in input order). */
+typedef struct gwn_query_config {
+    int32_t block_size;
+    int32_t stack_capacity;
+    int32_t sort_queries;
+    int32_t reserved;
+} gwn_query_config;
+
+/* Host-side description of a built BVH (the scalar members of the reference's
+ * gwn_bvh_accessor, include/gwn/gwn_bvh.cuh:209-223). */
+typedef struct gwn_bvh_info {
+    uint64_t node_count;
+    uint64_t triangle_count;
+    uint64_t revision;
+    uint32_t internal_stack_bound;
+    uint32_t packed_stack_bound;
+    uint32_t max_depth;
+    uint32_t width;
+    float root_bounds[6];
+    uint64_t root_reference;
+} gwn_bvh_info;
+
+int gwn_abi_version(void);
+const char *gwn_last_error_message(void);
+
+/* ---- geometry: replaces gwn_geometry_object + gwn_upload_geometry / gwn_update_geometry
+ *      (include/gwn/gwn_geometry.cuh:19-149, detail/gwn_geometry_impl.cuh:102-219) ---- */
+int gwn_geometry_create(gwn_geometry_t *out);
+int gwn_geometry_destroy(gwn_geometry_t geometry);
+/* Host SoA positions and oriented triangle indices; validates every index < vertex_count. */
+int gwn_upload_geometry_f32(
+    gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
+    hipStream_t stream);
+/* Replace positions (same vertex count) from host memory. */
+int gwn_update_geometry_f32(
+    gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count,
+    hipStream_t stream);
+/* Replace positions from device memory (enqueued device-to-device copies, no host sync). */
+int gwn_update_geometry_device_f32(
+    gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count,
+    hipStream_t stream);
+int gwn_geometry_counts(gwn_geometry_t geometry, uint64_t *vertex_count, uint64_t *triangle_count);
+
+/* ---- BVH: replaces gwn_bvh_object<4, float, uint32_t> and gwn_build_bvh
+ *      (include/gwn/gwn_bvh.cuh:331-380, include/gwn/gwn_bvh_build.cuh:29-39) ---- */
+int gwn_bvh_create(gwn_bvh_t *out);
+int gwn_bvh_destroy(gwn_bvh_t bvh);
+/* Width-4 build.  hploc_search_radius must be in [1, 8] for H-PLOC (build_impl.cuh:66-72). */
+int gwn_build_bvh4_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, int32_t method, uint32_t hploc_search_radius,
+    hipStream_t stream);
+int gwn_bvh_get_info(gwn_bvh_t bvh, gwn_bvh_info *info);
+/* Copies the canonical arrays to host memory (any pointer may be NULL) and synchronizes:
+ * nodes in the reference's gwn_bvh_node<4, float> layout (128 B each), primitive order,
+ * and leaf-ordered triangle records as 9 floats (v0, v1, v2). */
+int gwn_bvh_download(
+    gwn_bvh_t bvh, void *nodes, uint32_t *primitive_indices, float *triangles, hipStream_t stream);
+
+/* ---- Taylor moments: replaces gwn_bvh_moment_object<4, Order, float> and
+ *      gwn_refit_bvh_moment<Order> (include/gwn/gwn_bvh_refit.cuh:31-51) ---- */
+int gwn_moment_create(gwn_moment_t *out, int32_t order);
+int gwn_moment_destroy(gwn_moment_t moment);
+int gwn_refit_bvh_moment_f32(gwn_bvh_t bvh, gwn_moment_t moment, hipStream_t stream);
+/* Copies coefficients in the reference's field-SoA gwn_bvh_moment_node<4, Order, float>
+ * layout (node-major, then field, then child slot) and synchronizes. */
+int gwn_moment_download(gwn_moment_t moment, float *fields, hipStream_t stream);
+int gwn_moment_order(gwn_moment_t moment, int32_t *order);
+
+/* ---- winding-number batches: replace gwn_compute_winding_number_exact_batch and
+ *      gwn_compute_winding_number_taylor_batch<Order> (include/gwn/gwn_query.cuh:308-445) ---- */
+int gwn_compute_winding_number_exact_batch_f32(
+    gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, float *out, uint64_t count,
+    const gwn_query_config *config, hipStream_t stream);
+int gwn_compute_winding_number_taylor_batch_f32(
+    gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz,
+    float *out, uint64_t count, float accuracy_scale, const gwn_query_config *config,
+    hipStream_t stream);
+
+/* Precision-split and FP64 forms (no reference counterpart: the reference computes in the
+ * geometry's Real).  The exact form writes double results for every precision. */
+int gwn_compute_winding_number_exact_batch_ex_f32(
+    gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, double *out, uint64_t count,
+    int32_t precision, const gwn_query_config *config, hipStream_t stream);
+/* counters (optional, device, 3 x uint32 per query): child tests, accepted expansions,
+ * near-field triangle evaluations -- the work counts used for the FLOP roofline. */
+int gwn_compute_winding_number_taylor_batch_ex_f32(
+    gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz,
+    float *out, uint64_t count, float accuracy_scale, int32_t precision,
+    const gwn_query_config *config, uint32_t *counters, hipStream_t stream);
+
+/* ---- internal building block exported for tests: stable LSD radix sort of device
+ *      (uint64 key, uint32 value) pairs on bits [begin_bit, end_bit) (replaces the CUB
+ *      DeviceRadixSort::SortPairs call of include/gwn/detail/gwn_bvh_build_common.cuh:325-347).
+ *      Sorted output overwrites keys/values; scratch is allocated stream-ordered. ---- */
+int gwn_radix_sort_pairs_u64(
+    uint64_t *keys, uint32_t *values, uint64_t count, int32_t begin_bit, int32_t end_bit,
+    hipStream_t stream);
+int gwn_radix_sort_pairs_u32(
+    uint32_t *keys, uint32_t *values, uint64_t count, int32_t begin_bit, int32_t end_bit,
+    hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GWN_HIP_H */

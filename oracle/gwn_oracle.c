@@ -1,0 +1,739 @@
+/*
+ * gwn_oracle.c - CPU oracle for the smallgwn geometric-query path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see gwn_oracle.h).  Plain C11 + pthreads, built
+ * with -ffp-contract=off.  Each function cites the reference file:line it
+ * restates.
+ */
+#include "gwn_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define GWN_ORACLE_OFFSET_MASK ((((uint64_t)1) << 47) - 1)
+#define GWN_ORACLE_VALID_BIT (((uint64_t)1) << 47)
+#define GWN_ORACLE_INVALID_INDEX 0xffffffffu
+
+int gwn_oracle_moment_field_count(int order)
+{
+    /* gwn_bvh.cuh:144-202: 7, 13 and 23 scalars per child slot. */
+    switch (order) {
+    case 0: return 7;
+    case 1: return 13;
+    case 2: return 23;
+    default: return -1;
+    }
+}
+
+#define REAL float
+#define SFX(name) name##f32
+#include "gwn_oracle_real.inc"
+#undef REAL
+#undef SFX
+
+#define REAL double
+#define SFX(name) name##f64
+#include "gwn_oracle_real.inc"
+#undef REAL
+#undef SFX
+
+/* ---------------------------------------------------------------------------
+ * Morton codes: gwn_bvh_build_common.cuh:35-88
+ * ------------------------------------------------------------------------- */
+
+static uint32_t expand_bits_u32(uint32_t value)
+{
+    uint32_t x = value & 0x000003ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+static uint64_t expand_bits_u64(uint64_t value)
+{
+    uint64_t x = value & 0x1fffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+static float clamp01(float v)
+{
+    if (v < 0.0f)
+        return 0.0f;
+    if (v > 1.0f)
+        return 1.0f;
+    return v;
+}
+
+uint32_t gwn_oracle_morton_u32(float nx, float ny, float nz)
+{
+    uint32_t x = (uint32_t)(clamp01(nx) * 1023.0f);
+    uint32_t y = (uint32_t)(clamp01(ny) * 1023.0f);
+    uint32_t z = (uint32_t)(clamp01(nz) * 1023.0f);
+    return (expand_bits_u32(x) << 2) | (expand_bits_u32(y) << 1) | expand_bits_u32(z);
+}
+
+uint64_t gwn_oracle_morton_u64(float nx, float ny, float nz)
+{
+    uint64_t x = (uint64_t)(clamp01(nx) * 2097151.0f);
+    uint64_t y = (uint64_t)(clamp01(ny) * 2097151.0f);
+    uint64_t z = (uint64_t)(clamp01(nz) * 2097151.0f);
+    return (expand_bits_u64(x) << 2) | (expand_bits_u64(y) << 1) | expand_bits_u64(z);
+}
+
+/* ---------------------------------------------------------------------------
+ * Stable LSD radix order (CUB DeviceRadixSort::SortPairs is stable:
+ * gwn_bvh_build_common.cuh:325-347).
+ * ------------------------------------------------------------------------- */
+
+void gwn_oracle_stable_order_u64(const uint64_t *keys, uint64_t n, uint32_t *out_order)
+{
+    uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t *src = out_order, *dst = tmp;
+    for (uint64_t i = 0; i < n; ++i)
+        src[i] = (uint32_t)i;
+    for (int shift = 0; shift < 64; shift += 8) {
+        uint64_t count[257];
+        memset(count, 0, sizeof(count));
+        for (uint64_t i = 0; i < n; ++i)
+            count[((keys[src[i]] >> shift) & 0xffu) + 1]++;
+        for (int d = 0; d < 256; ++d)
+            count[d + 1] += count[d];
+        for (uint64_t i = 0; i < n; ++i)
+            dst[count[(keys[src[i]] >> shift) & 0xffu]++] = src[i];
+        uint32_t *t = src;
+        src = dst;
+        dst = t;
+    }
+    /* 8 passes: the result is back in out_order. */
+    free(tmp);
+}
+
+/* ---------------------------------------------------------------------------
+ * LBVH (Karras 2012) with the reference's equal-code tie-break:
+ * gwn_bvh_build_binary.cuh:37-128
+ * ------------------------------------------------------------------------- */
+
+static int clz64(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
+
+static int lbvh_delta(const uint64_t *codes, int64_t n, int64_t i, int64_t j)
+{
+    if (j < 0 || j >= n)
+        return -1;
+    uint64_t ci = codes[i], cj = codes[j];
+    if (ci == cj) {
+        uint64_t diff = (uint64_t)i ^ (uint64_t)j;
+        if (diff == 0)
+            return 64 + 64;
+        return 64 + clz64(diff);
+    }
+    return clz64(ci ^ cj);
+}
+
+static void lbvh_node(const uint64_t *codes, int64_t n, int64_t i, gwn_oracle_binary_node *out,
+                      int64_t *first_out)
+{
+    int dir = (lbvh_delta(codes, n, i, i + 1) - lbvh_delta(codes, n, i, i - 1) >= 0) ? 1 : -1;
+    int dmin = lbvh_delta(codes, n, i, i - dir);
+    int64_t lmax = 2;
+    while (lbvh_delta(codes, n, i, i + lmax * dir) > dmin)
+        lmax <<= 1;
+    int64_t len = 0;
+    for (int64_t step = lmax >> 1; step > 0; step >>= 1)
+        if (lbvh_delta(codes, n, i, i + (len + step) * dir) > dmin)
+            len += step;
+    int64_t j = i + len * dir;
+    int64_t first = i < j ? i : j;
+    int64_t last = i < j ? j : i;
+    int dnode = lbvh_delta(codes, n, first, last);
+    int64_t split = first;
+    int64_t step = last - first;
+    do {
+        step = (step + 1) >> 1;
+        int64_t cand = split + step;
+        if (cand < last && lbvh_delta(codes, n, first, cand) > dnode)
+            split = cand;
+    } while (step > 1);
+    out->left_index = (uint32_t)split;
+    out->left_kind = (split == first) ? 2u : 1u;
+    out->right_index = (uint32_t)(split + 1);
+    out->right_kind = ((split + 1) == last) ? 2u : 1u;
+    *first_out = first;
+}
+
+/* Half-area used by the collapse expansion rule, gwn_bvh_build_common.cuh:98-104. */
+static float half_area(const float *b)
+{
+    float dx = b[3] - b[0];
+    float dy = b[4] - b[1];
+    float dz = b[5] - b[2];
+    dx = dx < 0.0f ? 0.0f : dx;
+    dy = dy < 0.0f ? 0.0f : dy;
+    dz = dz < 0.0f ? 0.0f : dz;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+static void aabb_union(float *dst, const float *a, const float *b)
+{
+    for (int k = 0; k < 3; ++k) {
+        dst[k] = (b[k] < a[k]) ? b[k] : a[k];
+        dst[3 + k] = (a[3 + k] < b[3 + k]) ? b[3 + k] : a[3 + k];
+    }
+}
+
+typedef struct {
+    uint32_t binary;     /* binary internal node that roots this wide node */
+    uint32_t depth;
+    uint32_t first_leaf; /* smallest sorted leaf index under the wide node */
+    uint32_t prefix_internal;
+    uint32_t prefix_packed;
+} wide_task;
+
+static int cmp_wide_key(const void *pa, const void *pb)
+{
+    const uint64_t *a = (const uint64_t *)pa, *b = (const uint64_t *)pb;
+    return (a[0] < b[0]) ? -1 : (a[0] > b[0]);
+}
+
+int gwn_oracle_build_bvh4_f32(
+    const float *vx, const float *vy, const float *vz, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
+    gwn_oracle_bvh_f32 *out)
+{
+    uint64_t T = triangle_count;
+    memset(&out->root, 0, sizeof(out->root));
+    out->node_count = 0;
+    out->internal_stack_bound = 0;
+    out->packed_stack_bound = 0;
+    if (T == 0)
+        return GWN_ORACLE_OK;
+    if (vertex_count == 0 || T > 0x7fffffffull)
+        return GWN_ORACLE_INVALID_ARGUMENT;
+
+    /* Scene bounds over every vertex and the host safe inverse (build_common.cuh:243-274). */
+    float smin[3] = {vx[0], vy[0], vz[0]}, smax[3] = {vx[0], vy[0], vz[0]};
+    for (uint64_t v = 0; v < vertex_count; ++v) {
+        float p[3] = {vx[v], vy[v], vz[v]};
+        for (int k = 0; k < 3; ++k) {
+            if (p[k] < smin[k]) smin[k] = p[k];
+            if (p[k] > smax[k]) smax[k] = p[k];
+        }
+    }
+    float inv[3];
+    for (int k = 0; k < 3; ++k)
+        inv[k] = (smax[k] == smin[k]) ? 1.0f : 1.0f / (smax[k] - smin[k]);
+    for (int k = 0; k < 3; ++k) {
+        out->scene[k] = smin[k];
+        out->scene[3 + k] = smax[k];
+        out->scene[6 + k] = inv[k];
+    }
+
+    /* Primitive AABBs + Morton codes (build_common.cuh:170-213). */
+    float *aabb = (float *)malloc(sizeof(float) * 6 * T);
+    uint64_t *codes = (uint64_t *)malloc(sizeof(uint64_t) * T);
+    uint32_t *order = out->primitive_indices;
+    for (uint64_t t = 0; t < T; ++t) {
+        uint32_t a = i0[t], b = i1[t], c = i2[t];
+        float *bb = aabb + 6 * t;
+        if (a >= vertex_count || b >= vertex_count || c >= vertex_count) {
+            memset(bb, 0, sizeof(float) * 6);
+            codes[t] = 0;
+            continue;
+        }
+        float X[3] = {vx[a], vx[b], vx[c]}, Y[3] = {vy[a], vy[b], vy[c]}, Z[3] = {vz[a], vz[b], vz[c]};
+        const float *P[3] = {X, Y, Z};
+        for (int k = 0; k < 3; ++k) {
+            const float *q = P[k];
+            float lo = (q[2] < q[1]) ? q[2] : q[1];
+            lo = (lo < q[0]) ? lo : q[0];
+            float hi = (q[1] < q[2]) ? q[2] : q[1];
+            hi = (q[0] < hi) ? hi : q[0];
+            bb[k] = lo;
+            bb[3 + k] = hi;
+        }
+        float cx = (bb[0] + bb[3]) * 0.5f, cy = (bb[1] + bb[4]) * 0.5f, cz = (bb[2] + bb[5]) * 0.5f;
+        codes[t] = gwn_oracle_morton_u64((cx - smin[0]) * inv[0], (cy - smin[1]) * inv[1],
+                                         (cz - smin[2]) * inv[2]);
+    }
+    gwn_oracle_stable_order_u64(codes, T, order);
+    float *saabb = (float *)malloc(sizeof(float) * 6 * T);
+    for (uint64_t i = 0; i < T; ++i) {
+        out->sorted_morton[i] = codes[order[i]];
+        memcpy(saabb + 6 * i, aabb + 6 * (uint64_t)order[i], sizeof(float) * 6);
+    }
+
+    /* Leaf-ordered records (gwn_bvh_triangle_impl.cuh:23-67, build_impl.cuh:23-41). */
+    int status = GWN_ORACLE_OK;
+    for (uint64_t i = 0; i < T; ++i) {
+        uint32_t p = order[i];
+        uint32_t a = i0[p], b = i1[p], c = i2[p];
+        if (a >= vertex_count || b >= vertex_count || c >= vertex_count) {
+            status = GWN_ORACLE_INTERNAL_ERROR;
+            continue;
+        }
+        float *r = out->triangles + 9 * i;
+        r[0] = vx[a]; r[1] = vy[a]; r[2] = vz[a];
+        r[3] = vx[b]; r[4] = vy[b]; r[5] = vz[b];
+        r[6] = vx[c]; r[7] = vy[c]; r[8] = vz[c];
+    }
+    if (status != GWN_ORACLE_OK) {
+        free(aabb); free(codes); free(saabb);
+        return status;
+    }
+
+    if (T == 1) {
+        memcpy(out->root.bounds, saabb, sizeof(float) * 6);
+        out->root.reference = GWN_ORACLE_VALID_BIT | (((uint64_t)1) << 48);
+        free(aabb); free(codes); free(saabb);
+        return GWN_ORACLE_OK;
+    }
+
+    uint64_t NI = T - 1;
+    gwn_oracle_binary_node *bn = (gwn_oracle_binary_node *)malloc(sizeof(*bn) * NI);
+    int64_t *bfirst = (int64_t *)malloc(sizeof(int64_t) * NI);
+    float *bbounds = (float *)malloc(sizeof(float) * 6 * NI);
+    uint32_t *bparent = (uint32_t *)malloc(sizeof(uint32_t) * NI);
+    for (uint64_t i = 0; i < NI; ++i)
+        bparent[i] = GWN_ORACLE_INVALID_INDEX;
+    for (uint64_t i = 0; i < NI; ++i) {
+        lbvh_node(out->sorted_morton, (int64_t)T, (int64_t)i, &bn[i], &bfirst[i]);
+        if (bn[i].left_kind == 1u) bparent[bn[i].left_index] = (uint32_t)i;
+        if (bn[i].right_kind == 1u) bparent[bn[i].right_index] = (uint32_t)i;
+    }
+
+    /* Bottom-up bounds (build_binary.cuh:131-192): post-order over the tree. */
+    {
+        uint32_t *stack = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (NI + 1));
+        uint8_t *visited = (uint8_t *)calloc(NI, 1);
+        int64_t sp = 0;
+        stack[sp++] = 0;
+        while (sp > 0) {
+            uint32_t node = stack[sp - 1];
+            if (!visited[node]) {
+                visited[node] = 1;
+                if (bn[node].right_kind == 1u) stack[sp++] = bn[node].right_index;
+                if (bn[node].left_kind == 1u) stack[sp++] = bn[node].left_index;
+                continue;
+            }
+            --sp;
+            const float *lb = bn[node].left_kind == 1u ? bbounds + 6 * (uint64_t)bn[node].left_index
+                                                       : saabb + 6 * (uint64_t)bn[node].left_index;
+            const float *rb = bn[node].right_kind == 1u ? bbounds + 6 * (uint64_t)bn[node].right_index
+                                                        : saabb + 6 * (uint64_t)bn[node].right_index;
+            aabb_union(bbounds + 6 * (uint64_t)node, lb, rb);
+        }
+        free(stack);
+        free(visited);
+    }
+    if (out->binary_nodes)
+        memcpy(out->binary_nodes, bn, sizeof(*bn) * NI);
+    if (out->binary_bounds)
+        memcpy(out->binary_bounds, bbounds, sizeof(float) * 6 * NI);
+
+    /* BVH4 collapse (collapse.cuh:163-296), breadth-first over a FIFO of tasks. */
+    wide_task *tasks = (wide_task *)malloc(sizeof(wide_task) * NI);
+    gwn_oracle_node4_f32 *tmp_nodes = (gwn_oracle_node4_f32 *)calloc(NI, sizeof(gwn_oracle_node4_f32));
+    uint32_t *child_task = (uint32_t *)malloc(sizeof(uint32_t) * 4 * NI);
+    uint64_t head = 0, tail = 0;
+    tasks[tail++] = (wide_task){0u, 0u, (uint32_t)bfirst[0], 0u, 0u};
+    uint32_t max_int = 0, max_packed = 0;
+    while (head < tail) {
+        wide_task w = tasks[head];
+        uint64_t self = head++;
+        gwn_oracle_binary_node root = bn[w.binary];
+        uint32_t cidx[4], ckind[4];
+        int cnt = 2;
+        cidx[0] = root.left_index; ckind[0] = root.left_kind;
+        cidx[1] = root.right_index; ckind[1] = root.right_kind;
+        while (cnt < 4) {
+            int best = -1;
+            float best_area = -INFINITY;
+            uint32_t best_internal = GWN_ORACLE_INVALID_INDEX;
+            for (int s = 0; s < cnt; ++s) {
+                if (ckind[s] != 1u)
+                    continue;
+                float area = half_area(bbounds + 6 * (uint64_t)cidx[s]);
+                if (best >= 0 && area < best_area)
+                    continue;
+                if (best >= 0 && area == best_area && cidx[s] >= best_internal)
+                    continue;
+                best = s;
+                best_area = area;
+                best_internal = cidx[s];
+            }
+            if (best < 0)
+                break;
+            gwn_oracle_binary_node ex = bn[cidx[best]];
+            cidx[best] = ex.left_index; ckind[best] = ex.left_kind;
+            cidx[cnt] = ex.right_index; ckind[cnt] = ex.right_kind;
+            ++cnt;
+        }
+        uint32_t n_int = 0;
+        for (int s = 0; s < cnt; ++s)
+            n_int += (ckind[s] == 1u);
+        uint32_t node_int = w.prefix_internal + (n_int ? n_int - 1 : 0);
+        uint32_t node_packed = w.prefix_packed + (uint32_t)(cnt - 1);
+        if (node_int > max_int) max_int = node_int;
+        if (node_packed > max_packed) max_packed = node_packed;
+        for (int s = 0; s < 4; ++s)
+            child_task[4 * self + s] = GWN_ORACLE_INVALID_INDEX;
+        for (int s = 0; s < cnt; ++s) {
+            gwn_oracle_child_f32 *ch = &tmp_nodes[self].children[s];
+            if (ckind[s] == 2u) {
+                memcpy(ch->bounds, saabb + 6 * (uint64_t)cidx[s], sizeof(float) * 6);
+                ch->reference = GWN_ORACLE_VALID_BIT | (uint64_t)cidx[s] | (((uint64_t)1) << 48);
+            } else {
+                memcpy(ch->bounds, bbounds + 6 * (uint64_t)cidx[s], sizeof(float) * 6);
+                child_task[4 * self + s] = (uint32_t)tail;
+                tasks[tail++] = (wide_task){cidx[s], w.depth + 1, (uint32_t)bfirst[cidx[s]],
+                                            node_int, node_packed};
+            }
+        }
+    }
+    uint64_t NW = tail;
+
+    /* Canonical numbering: sort by (depth, first sorted leaf). */
+    uint64_t *keyed = (uint64_t *)malloc(sizeof(uint64_t) * 2 * NW);
+    for (uint64_t i = 0; i < NW; ++i) {
+        keyed[2 * i] = ((uint64_t)tasks[i].depth << 32) | tasks[i].first_leaf;
+        keyed[2 * i + 1] = i;
+    }
+    qsort(keyed, NW, 2 * sizeof(uint64_t), cmp_wide_key);
+    uint32_t *newid = (uint32_t *)malloc(sizeof(uint32_t) * NW);
+    for (uint64_t i = 0; i < NW; ++i)
+        newid[keyed[2 * i + 1]] = (uint32_t)i;
+    for (uint64_t i = 0; i < NW; ++i) {
+        uint64_t old = keyed[2 * i + 1];
+        out->nodes[i] = tmp_nodes[old];
+        for (int s = 0; s < 4; ++s) {
+            uint32_t ct = child_task[4 * old + s];
+            if (ct != GWN_ORACLE_INVALID_INDEX)
+                out->nodes[i].children[s].reference = GWN_ORACLE_VALID_BIT | (uint64_t)newid[ct];
+        }
+    }
+    memcpy(out->root.bounds, bbounds, sizeof(float) * 6);
+    out->root.reference = GWN_ORACLE_VALID_BIT;
+    out->node_count = NW;
+    out->internal_stack_bound = max_int;
+    out->packed_stack_bound = max_packed;
+
+    free(keyed); free(newid); free(tasks); free(tmp_nodes); free(child_task);
+    free(bn); free(bfirst); free(bbounds); free(bparent);
+    free(aabb); free(codes); free(saabb);
+    return GWN_ORACLE_OK;
+}
+
+int gwn_oracle_refit_moment_f32(const gwn_oracle_node4_f32 *nodes, uint64_t node_count,
+                                const float *triangles, uint64_t triangle_count, int order,
+                                float *out)
+{
+    return refit_moment_impl_f32(nodes, node_count, triangles, triangle_count, order, out);
+}
+
+int gwn_oracle_refit_moment_f64(const gwn_oracle_node4_f32 *nodes, uint64_t node_count,
+                                const float *triangles, uint64_t triangle_count, int order,
+                                double *out)
+{
+    return refit_moment_impl_f64(nodes, node_count, triangles, triangle_count, order, out);
+}
+
+void gwn_oracle_triangle_payload_f64(const float *record, int order, double *out)
+{
+    payload_f64 p;
+    triangle_payload_f64(record, order, &p);
+    out[0] = p.area; out[1] = p.apx; out[2] = p.apy; out[3] = p.apz;
+    out[4] = p.nx; out[5] = p.ny; out[6] = p.nz;
+    for (int k = 0; k < 9; ++k) out[7 + k] = p.n1[k];
+    for (int k = 0; k < 18; ++k) out[16 + k] = p.raw[k];
+}
+
+void gwn_oracle_triangle_payload_f32(const float *record, int order, float *out)
+{
+    payload_f32 p;
+    triangle_payload_f32(record, order, &p);
+    out[0] = p.area; out[1] = p.apx; out[2] = p.apy; out[3] = p.apz;
+    out[4] = p.nx; out[5] = p.ny; out[6] = p.nz;
+    for (int k = 0; k < 9; ++k) out[7 + k] = p.n1[k];
+    for (int k = 0; k < 18; ++k) out[16 + k] = p.raw[k];
+}
+
+double gwn_oracle_solid_angle_f64(const double *a, const double *b, const double *c,
+                                  const double *q)
+{
+    return solid_angle_f64(a, b, c, q);
+}
+
+float gwn_oracle_solid_angle_f32(const float *a, const float *b, const float *c, const float *q)
+{
+    return solid_angle_f32(a, b, c, q);
+}
+
+/* ---------------------------------------------------------------------------
+ * Query-parallel helpers
+ * ------------------------------------------------------------------------- */
+
+typedef void (*query_range_fn)(void *ctx, uint64_t begin, uint64_t end);
+
+typedef struct {
+    query_range_fn fn;
+    void *ctx;
+    uint64_t begin, end;
+} range_job;
+
+static void *range_thread(void *arg)
+{
+    range_job *job = (range_job *)arg;
+    job->fn(job->ctx, job->begin, job->end);
+    return NULL;
+}
+
+static void parallel_ranges(query_range_fn fn, void *ctx, uint64_t n, int threads)
+{
+    if (threads < 1)
+        threads = 1;
+    if ((uint64_t)threads > n)
+        threads = n ? (int)n : 1;
+    if (threads == 1) {
+        fn(ctx, 0, n);
+        return;
+    }
+    pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+    range_job *jobs = (range_job *)malloc(sizeof(range_job) * threads);
+    for (int t = 0; t < threads; ++t) {
+        jobs[t].fn = fn;
+        jobs[t].ctx = ctx;
+        jobs[t].begin = n * (uint64_t)t / (uint64_t)threads;
+        jobs[t].end = n * (uint64_t)(t + 1) / (uint64_t)threads;
+        pthread_create(&tid[t], NULL, range_thread, &jobs[t]);
+    }
+    for (int t = 0; t < threads; ++t)
+        pthread_join(tid[t], NULL);
+    free(tid);
+    free(jobs);
+}
+
+/* Exact winding over records (gwn_query_winding_impl.cuh:377-389). */
+typedef struct {
+    const float *tri;
+    uint64_t nt;
+    const float *qx, *qy, *qz;
+    float *out;
+} exact_rec_ctx;
+
+static void exact_rec_range(void *vctx, uint64_t begin, uint64_t end)
+{
+    exact_rec_ctx *c = (exact_rec_ctx *)vctx;
+    const float k_pi = (float)3.141592653589793238462643383279502884L;
+    for (uint64_t q = begin; q < end; ++q) {
+        float qq[3] = {c->qx[q], c->qy[q], c->qz[q]};
+        float sum = 0.0f;
+        for (uint64_t t = 0; t < c->nt; ++t) {
+            float a[3], b[3], cc[3];
+            record_vertices_f32(c->tri + 9 * t, a, b, cc);
+            sum += solid_angle_f32(a, b, cc, qq);
+        }
+        c->out[q] = sum / (4.0f * k_pi);
+    }
+}
+
+void gwn_oracle_winding_exact_records_f32(const float *triangles, uint64_t triangle_count,
+                                          const float *qx, const float *qy, const float *qz,
+                                          uint64_t query_count, float *out, int threads)
+{
+    exact_rec_ctx ctx = {triangles, triangle_count, qx, qy, qz, out};
+    parallel_ranges(exact_rec_range, &ctx, query_count, threads);
+}
+
+/* Exact FP64 winding over the mesh (tests/reference_cpu.cuh:51-82). */
+typedef struct {
+    const float *vx, *vy, *vz;
+    const uint32_t *i0, *i1, *i2;
+    uint64_t nt;
+    const float *qx, *qy, *qz;
+    double *out;
+} exact_mesh_ctx;
+
+static void exact_mesh_range(void *vctx, uint64_t begin, uint64_t end)
+{
+    exact_mesh_ctx *c = (exact_mesh_ctx *)vctx;
+    const double k_pi = 3.141592653589793238462643383279502884;
+    for (uint64_t q = begin; q < end; ++q) {
+        double qq[3] = {c->qx[q], c->qy[q], c->qz[q]};
+        double sum = 0.0;
+        for (uint64_t t = 0; t < c->nt; ++t) {
+            uint32_t ia = c->i0[t], ib = c->i1[t], ic = c->i2[t];
+            double a[3] = {c->vx[ia], c->vy[ia], c->vz[ia]};
+            double b[3] = {c->vx[ib], c->vy[ib], c->vz[ib]};
+            double cc[3] = {c->vx[ic], c->vy[ic], c->vz[ic]};
+            sum += solid_angle_f64(a, b, cc, qq);
+        }
+        c->out[q] = sum / (4.0 * k_pi);
+    }
+}
+
+void gwn_oracle_winding_exact_mesh_f64(const float *vx, const float *vy, const float *vz,
+                                       const uint32_t *i0, const uint32_t *i1, const uint32_t *i2,
+                                       uint64_t triangle_count, const float *qx, const float *qy,
+                                       const float *qz, uint64_t query_count, double *out,
+                                       int threads)
+{
+    exact_mesh_ctx ctx = {vx, vy, vz, i0, i1, i2, triangle_count, qx, qy, qz, out};
+    parallel_ranges(exact_mesh_range, &ctx, query_count, threads);
+}
+
+/* ---------------------------------------------------------------------------
+ * Taylor traversal replay, gwn_query_winding_impl.cuh:102-374 (ComputeGradient = false).
+ * ------------------------------------------------------------------------- */
+
+typedef struct {
+    const gwn_oracle_node4_f32 *nodes;
+    uint64_t node_count;
+    gwn_oracle_child_f32 root;
+    const float *tri;
+    uint64_t nt;
+    const float *mom;
+    int order, F, cap;
+    float beta;
+    const float *qx, *qy, *qz;
+    float *out;
+    uint64_t *counters;
+} taylor_ctx;
+
+static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint64_t *cnt)
+{
+    const float k_pi = (float)3.141592653589793238462643383279502884L;
+    float q[3] = {qx, qy, qz};
+    float omega = 0.0f;
+    float beta2 = c->beta * c->beta;
+    uint32_t stack[1024];
+    int sp = 0;
+    uint64_t tests = 0, accepted = 0, leaves = 0;
+#define LEAF(child_ptr)                                                                            \
+    do {                                                                                           \
+        uint64_t off_ = (child_ptr)->reference & GWN_ORACLE_OFFSET_MASK;                          \
+        uint32_t n_ = (uint32_t)((child_ptr)->reference >> 48);                                   \
+        for (uint32_t k_ = 0; k_ < n_; ++k_) {                                                    \
+            if (off_ + k_ >= c->nt)                                                               \
+                continue;                                                                         \
+            float a_[3], b_[3], c_[3];                                                            \
+            record_vertices_f32(c->tri + 9 * (off_ + k_), a_, b_, c_);                            \
+            omega += solid_angle_f32(a_, b_, c_, q);                                              \
+            ++leaves;                                                                             \
+        }                                                                                         \
+    } while (0)
+    if ((c->root.reference & GWN_ORACLE_VALID_BIT) && (c->root.reference >> 48) != 0) {
+        LEAF(&c->root);
+        if (cnt) { cnt[0] = tests; cnt[1] = accepted; cnt[2] = leaves; }
+        return omega / (4.0f * k_pi);
+    }
+    uint32_t node = (uint32_t)(c->root.reference & GWN_ORACLE_OFFSET_MASK);
+    for (;;) {
+        if (node >= c->node_count)
+            break;
+        const gwn_oracle_node4_f32 *nd = &c->nodes[node];
+        const float *t = c->mom + (size_t)node * 4 * c->F;
+        uint32_t next = GWN_ORACLE_INVALID_INDEX;
+        for (int s = 0; s < 4; ++s) {
+            const gwn_oracle_child_f32 *ch = &nd->children[s];
+            if (!(ch->reference & GWN_ORACLE_VALID_BIT))
+                continue;
+            ++tests;
+#define T(f) t[(f) * 4 + s]
+            float qrx = qx - T(1), qry = qy - T(2), qrz = qz - T(3);
+            float ql2 = qrx * qrx + qry * qry + qrz * qrz;
+            int descend = !(ql2 > 0.0f);
+            if (!descend)
+                descend = ql2 <= T(0) * beta2;
+            if (!descend) {
+                float m1 = 1.0f / sqrtf(ql2);
+                float m2 = m1 * m1;
+                float m3 = m2 * m1;
+                float qnx = qrx * m1, qny = qry * m1, qnz = qrz * m1;
+                float Nx = T(4), Ny = T(5), Nz = T(6);
+                float rdotN = qnx * Nx + qny * Ny + qnz * Nz;
+                float w = -m2 * rdotN;
+                if (c->order >= 1) {
+                    float qxx = qnx * qnx, qyy = qny * qny, qzz = qnz * qnz;
+                    float Txx = T(7), Tyy = T(8), Tzz = T(9);
+                    float Sxy = T(10), Syz = T(11), Szx = T(12);
+                    float Tr = Txx + Tyy + Tzz;
+                    float S = qxx * Txx + qyy * Tyy + qzz * Tzz + qnx * qny * Sxy +
+                              qnx * qnz * Szx + qny * qnz * Syz;
+                    w += m3 * (Tr - 3.0f * S);
+                }
+                if (c->order >= 2) {
+                    float m4 = m2 * m2;
+                    float qnx2 = qnx * qnx, qny2 = qny * qny, qnz2 = qnz * qnz;
+                    float qnx3 = qnx2 * qnx, qny3 = qny2 * qny, qnz3 = qnz2 * qnz;
+                    float xxx = T(13), yyy = T(14), zzz = T(15), spn = T(16);
+                    float Axy = T(17), Axz = T(18), Ayz = T(19), Ayx = T(20), Azx = T(21), Azy = T(22);
+                    float t0x = Ayx + Azx, t0y = Azy + Axy, t0z = Axz + Ayz;
+                    float t1x = qny * Axy + qnz * Axz;
+                    float t1y = qnz * Ayz + qnx * Ayx;
+                    float t1z = qnx * Azx + qny * Azy;
+                    w += m4 * (1.5f * (qnx * (3.0f * xxx + t0x) + qny * (3.0f * yyy + t0y) +
+                                       qnz * (3.0f * zzz + t0z)) -
+                               7.5f * (qnx3 * xxx + qny3 * yyy + qnz3 * zzz + qnx * qny * qnz * spn +
+                                       qnx2 * t1x + qny2 * t1y + qnz2 * t1z));
+                }
+#undef T
+                if (isfinite(w)) {
+                    omega += w;
+                    ++accepted;
+                    continue;
+                }
+            }
+            uint64_t ref = ch->reference;
+            if ((ref >> 48) == 0) {
+                if (next != GWN_ORACLE_INVALID_INDEX) {
+                    if (sp >= c->cap || sp >= 1024) {
+                        if (cnt) { cnt[0] = tests; cnt[1] = accepted; cnt[2] = leaves; }
+                        return NAN;
+                    }
+                    stack[sp++] = next;
+                }
+                next = (uint32_t)(ref & GWN_ORACLE_OFFSET_MASK);
+                continue;
+            }
+            LEAF(ch);
+        }
+        if (next != GWN_ORACLE_INVALID_INDEX) {
+            node = next;
+            continue;
+        }
+        if (sp == 0)
+            break;
+        node = stack[--sp];
+    }
+#undef LEAF
+    if (cnt) { cnt[0] = tests; cnt[1] = accepted; cnt[2] = leaves; }
+    return omega / (4.0f * k_pi);
+}
+
+static void taylor_range(void *vctx, uint64_t begin, uint64_t end)
+{
+    taylor_ctx *c = (taylor_ctx *)vctx;
+    for (uint64_t q = begin; q < end; ++q)
+        c->out[q] = taylor_one(c, c->qx[q], c->qy[q], c->qz[q],
+                               c->counters ? c->counters + 3 * q : NULL);
+}
+
+void gwn_oracle_winding_taylor_f32(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
+    const float *triangles, uint64_t triangle_count, const float *moments, int order,
+    float accuracy_scale, int stack_capacity, const float *qx, const float *qy,
+    const float *qz, uint64_t query_count, float *out, uint64_t *counters, int threads)
+{
+    taylor_ctx ctx = {nodes, node_count, root, triangles, triangle_count, moments, order,
+                      gwn_oracle_moment_field_count(order), stack_capacity, accuracy_scale,
+                      qx, qy, qz, out, counters};
+    parallel_ranges(taylor_range, &ctx, query_count, threads);
+}

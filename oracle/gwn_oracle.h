@@ -1,0 +1,152 @@
+/*
+ * gwn_oracle.h - CPU oracle for the smallgwn geometric-query path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * implementation in smallgwn_amd/ is compared against.  Only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it;
+ * the product path never calls into it.
+ *
+ * It is a plain-C restatement of the reference algorithms (citations are
+ * relative to the reference repository root):
+ *   - Morton encoding: include/gwn/detail/gwn_bvh_build_common.cuh:35-88, 157-214
+ *   - scene bounds / safe inverse: gwn_bvh_build_common.cuh:243-274
+ *   - stable key sort: gwn_bvh_build_common.cuh:325-347 (CUB LSD radix sort is stable)
+ *   - Karras LBVH: include/gwn/detail/gwn_bvh_build_binary.cuh:29-129
+ *   - bottom-up bounds: gwn_bvh_build_binary.cuh:131-192
+ *   - BVH4 collapse rule and stack bounds: include/gwn/detail/gwn_bvh_build_collapse.cuh:163-296
+ *   - leaf-ordered triangle records: include/gwn/detail/gwn_bvh_triangle_impl.cuh:23-67
+ *   - Taylor moments: include/gwn/detail/gwn_bvh_refit_common.cuh:66-387, 445-607
+ *   - moment accumulation order: include/gwn/detail/gwn_bvh_refit_moment_impl.cuh:33-47
+ *   - solid angle: include/gwn/detail/gwn_query_geometry_impl.cuh:16-45
+ *   - exact winding: include/gwn/detail/gwn_query_winding_impl.cuh:377-389
+ *   - Taylor traversal: include/gwn/detail/gwn_query_winding_impl.cuh:102-374
+ *
+ * Deliberate, documented differences from the reference that the HIP path
+ * shares:
+ *   - wide nodes are numbered breadth-first by (depth, first sorted leaf of
+ *     the subtree) instead of (depth, nondeterministic parent slot); the node
+ *     set, child slot order, bounds and stack bounds are unchanged.
+ *   - triangle records store the three vertices (v0, v1, v2).  The plain
+ *     ("reference form") arithmetic reconstructs e1 = v1 - v0,
+ *     b = v0 + e1 exactly as the reference's (v0, e1, e2) record does.
+ *
+ * Everything is compiled with -ffp-contract=off so float results are
+ * reproducible operation by operation.
+ */
+#ifndef GWN_ORACLE_H
+#define GWN_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Reference gwn_bvh_child<float> layout (include/gwn/gwn_bvh.cuh:55-103): 32 bytes. */
+typedef struct {
+    float bounds[6];      /* min_x min_y min_z max_x max_y max_z */
+    uint64_t reference;   /* offset bits 0-46, valid bit 47, primitive count bits 48-63 */
+} gwn_oracle_child_f32;
+
+typedef struct {
+    gwn_oracle_child_f32 children[4];
+} gwn_oracle_node4_f32;
+
+/* Binary LBVH node: kind 1 = internal, 2 = leaf (gwn_bvh_build_binary.cuh:13-27). */
+typedef struct {
+    uint32_t left_index;
+    uint32_t right_index;
+    uint32_t left_kind;
+    uint32_t right_kind;
+} gwn_oracle_binary_node;
+
+typedef struct {
+    /* caller-allocated outputs; capacities: nodes >= max(T-1,1), the rest >= T */
+    gwn_oracle_node4_f32 *nodes;
+    uint32_t *primitive_indices;
+    float *triangles;              /* 9 floats per record: v0 xyz, v1 xyz, v2 xyz */
+    uint64_t *sorted_morton;
+    gwn_oracle_binary_node *binary_nodes; /* optional (may be NULL): T-1 entries */
+    float *binary_bounds;          /* optional: 6 floats per binary internal node */
+    /* scalar outputs */
+    gwn_oracle_child_f32 root;
+    uint64_t node_count;
+    uint32_t internal_stack_bound;
+    uint32_t packed_stack_bound;
+    float scene[9];                /* min xyz, max xyz, inv xyz */
+} gwn_oracle_bvh_f32;
+
+enum {
+    GWN_ORACLE_OK = 0,
+    GWN_ORACLE_INVALID_ARGUMENT = 1,
+    GWN_ORACLE_INTERNAL_ERROR = 3
+};
+
+uint32_t gwn_oracle_morton_u32(float nx, float ny, float nz);
+uint64_t gwn_oracle_morton_u64(float nx, float ny, float nz);
+
+/* Stable ascending sort of 64-bit keys: out_order[i] = input position of the i-th smallest key. */
+void gwn_oracle_stable_order_u64(const uint64_t *keys, uint64_t n, uint32_t *out_order);
+
+int gwn_oracle_build_bvh4_f32(
+    const float *vx, const float *vy, const float *vz, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
+    gwn_oracle_bvh_f32 *out);
+
+/* Number of float fields per child slot for Taylor order 0/1/2 (gwn_bvh.cuh:144-202). */
+int gwn_oracle_moment_field_count(int order);
+
+/*
+ * Moment refit on a canonical BVH4.  Output layout per node is the reference's
+ * field-SoA gwn_bvh_moment_node<4, Order, Real>: field f of child slot s is at
+ * out[node * 4 * F + f * 4 + s].
+ */
+int gwn_oracle_refit_moment_f32(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, const float *triangles,
+    uint64_t triangle_count, int order, float *out);
+int gwn_oracle_refit_moment_f64(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, const float *triangles,
+    uint64_t triangle_count, int order, double *out);
+
+/*
+ * Additive payload of one leaf record (gwn_bvh_refit_common.cuh:122-387), 34 values:
+ * area, area*P (3), N (3), N_i P_j (9, i-major), raw second moments (18: x,y,z-major of
+ * xx yy zz xy yz zx).  Entries beyond the order are zero.
+ */
+void gwn_oracle_triangle_payload_f64(const float *record, int order, double *out);
+void gwn_oracle_triangle_payload_f32(const float *record, int order, float *out);
+
+/* Exact winding over leaf-ordered records, reference-form FP32 arithmetic. */
+void gwn_oracle_winding_exact_records_f32(
+    const float *triangles, uint64_t triangle_count, const float *qx, const float *qy,
+    const float *qz, uint64_t query_count, float *out, int threads);
+
+/* Exact winding in FP64 over the original mesh (exact vertex values). */
+void gwn_oracle_winding_exact_mesh_f64(
+    const float *vx, const float *vy, const float *vz, const uint32_t *i0, const uint32_t *i1,
+    const uint32_t *i2, uint64_t triangle_count, const float *qx, const float *qy,
+    const float *qz, uint64_t query_count, double *out, int threads);
+
+/* Signed solid angle of one triangle (a, b, c) seen from q. */
+double gwn_oracle_solid_angle_f64(const double *a, const double *b, const double *c,
+                                  const double *q);
+float gwn_oracle_solid_angle_f32(const float *a, const float *b, const float *c,
+                                 const float *q);
+
+/*
+ * Taylor traversal replay (FP32, reference-form leaves).  counters (optional,
+ * 3 x uint64 per query or NULL): child tests, accepted expansions, leaf
+ * triangle evaluations.  Returns NaN for a query whose stack would exceed
+ * stack_capacity.
+ */
+void gwn_oracle_winding_taylor_f32(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
+    const float *triangles, uint64_t triangle_count, const float *moments, int order,
+    float accuracy_scale, int stack_capacity, const float *qx, const float *qy,
+    const float *qz, uint64_t query_count, float *out, uint64_t *counters, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

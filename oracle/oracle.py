@@ -1,0 +1,279 @@
+"""ctypes bindings for the CPU oracle (libgwn_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` as the checker.  The product path in
+``smallgwn_amd`` never imports this module.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libgwn_oracle.so")
+
+CHILD_DTYPE = np.dtype([("bounds", np.float32, (6,)), ("reference", np.uint64)])
+NODE_DTYPE = np.dtype([("children", CHILD_DTYPE, (4,))])
+assert CHILD_DTYPE.itemsize == 32 and NODE_DTYPE.itemsize == 128
+
+OFFSET_MASK = (1 << 47) - 1
+VALID_BIT = 1 << 47
+
+
+class _Child(ctypes.Structure):
+    _fields_ = [("bounds", ctypes.c_float * 6), ("reference", ctypes.c_uint64)]
+
+
+class _Bvh(ctypes.Structure):
+    _fields_ = [
+        ("nodes", ctypes.c_void_p),
+        ("primitive_indices", ctypes.c_void_p),
+        ("triangles", ctypes.c_void_p),
+        ("sorted_morton", ctypes.c_void_p),
+        ("binary_nodes", ctypes.c_void_p),
+        ("binary_bounds", ctypes.c_void_p),
+        ("root", _Child),
+        ("node_count", ctypes.c_uint64),
+        ("internal_stack_bound", ctypes.c_uint32),
+        ("packed_stack_bound", ctypes.c_uint32),
+        ("scene", ctypes.c_float * 9),
+    ]
+
+
+def build_library(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
+    if force or not os.path.exists(_LIB_PATH):
+        cmd = ["make", "-s", "-C", _HERE]
+        if force:
+            cmd.append("-B")
+        subprocess.run(cmd, check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build_library()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, u64, i32, f32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_float
+        L.gwn_oracle_morton_u32.restype = ctypes.c_uint32
+        L.gwn_oracle_morton_u32.argtypes = [f32, f32, f32]
+        L.gwn_oracle_morton_u64.restype = ctypes.c_uint64
+        L.gwn_oracle_morton_u64.argtypes = [f32, f32, f32]
+        L.gwn_oracle_stable_order_u64.argtypes = [vp, u64, vp]
+        L.gwn_oracle_build_bvh4_f32.restype = i32
+        L.gwn_oracle_build_bvh4_f32.argtypes = [vp, vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(_Bvh)]
+        L.gwn_oracle_moment_field_count.restype = i32
+        L.gwn_oracle_moment_field_count.argtypes = [i32]
+        L.gwn_oracle_refit_moment_f32.restype = i32
+        L.gwn_oracle_refit_moment_f32.argtypes = [vp, u64, vp, u64, i32, vp]
+        L.gwn_oracle_refit_moment_f64.restype = i32
+        L.gwn_oracle_refit_moment_f64.argtypes = [vp, u64, vp, u64, i32, vp]
+        L.gwn_oracle_winding_exact_records_f32.argtypes = [vp, u64, vp, vp, vp, u64, vp, i32]
+        L.gwn_oracle_winding_exact_mesh_f64.argtypes = [vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, u64, vp, i32]
+        L.gwn_oracle_solid_angle_f64.restype = ctypes.c_double
+        L.gwn_oracle_solid_angle_f64.argtypes = [vp, vp, vp, vp]
+        L.gwn_oracle_solid_angle_f32.restype = ctypes.c_float
+        L.gwn_oracle_solid_angle_f32.argtypes = [vp, vp, vp, vp]
+        L.gwn_oracle_triangle_payload_f64.argtypes = [vp, i32, vp]
+        L.gwn_oracle_triangle_payload_f32.argtypes = [vp, i32, vp]
+        L.gwn_oracle_winding_taylor_f32.argtypes = [
+            vp, u64, _Child, vp, u64, vp, i32, f32, i32, vp, vp, vp, u64, vp, vp, i32,
+        ]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _c(a, dtype) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def default_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def morton_u64(nx: float, ny: float, nz: float) -> int:
+    return int(lib().gwn_oracle_morton_u64(nx, ny, nz))
+
+
+def morton_u32(nx: float, ny: float, nz: float) -> int:
+    return int(lib().gwn_oracle_morton_u32(nx, ny, nz))
+
+
+def stable_order_u64(keys) -> np.ndarray:
+    k = _c(keys, np.uint64)
+    out = np.empty(k.shape[0], dtype=np.uint32)
+    lib().gwn_oracle_stable_order_u64(_ptr(k), k.shape[0], _ptr(out))
+    return out
+
+
+@dataclass
+class OracleBvh:
+    nodes: np.ndarray              # NODE_DTYPE[node_count]
+    primitive_indices: np.ndarray  # uint32[T]
+    triangles: np.ndarray          # float32[T, 9]  (v0, v1, v2)
+    sorted_morton: np.ndarray      # uint64[T]
+    binary_nodes: np.ndarray       # uint32[T-1, 4]  left, right, left_kind, right_kind
+    binary_bounds: np.ndarray      # float32[T-1, 6]
+    root_bounds: np.ndarray        # float32[6]
+    root_reference: int
+    internal_stack_bound: int
+    packed_stack_bound: int
+    scene: np.ndarray              # float32[9]
+
+    @property
+    def node_count(self) -> int:
+        return int(self.nodes.shape[0])
+
+
+def build_bvh4(mesh) -> OracleBvh:
+    vx, vy, vz = _c(mesh.vx, np.float32), _c(mesh.vy, np.float32), _c(mesh.vz, np.float32)
+    i0, i1, i2 = _c(mesh.i0, np.uint32), _c(mesh.i1, np.uint32), _c(mesh.i2, np.uint32)
+    T = i0.shape[0]
+    nodes = np.zeros(max(T - 1, 1), dtype=NODE_DTYPE)
+    prim = np.zeros(max(T, 1), dtype=np.uint32)
+    tris = np.zeros((max(T, 1), 9), dtype=np.float32)
+    morton = np.zeros(max(T, 1), dtype=np.uint64)
+    bnodes = np.zeros((max(T - 1, 1), 4), dtype=np.uint32)
+    bbounds = np.zeros((max(T - 1, 1), 6), dtype=np.float32)
+    out = _Bvh()
+    out.nodes, out.primitive_indices, out.triangles = _ptr(nodes), _ptr(prim), _ptr(tris)
+    out.sorted_morton, out.binary_nodes, out.binary_bounds = _ptr(morton), _ptr(bnodes), _ptr(bbounds)
+    st = lib().gwn_oracle_build_bvh4_f32(
+        _ptr(vx), _ptr(vy), _ptr(vz), vx.shape[0], _ptr(i0), _ptr(i1), _ptr(i2), T, ctypes.byref(out)
+    )
+    if st != 0:
+        raise RuntimeError(f"oracle build failed with status {st}")
+    n = int(out.node_count)
+    return OracleBvh(
+        nodes=nodes[:n].copy(),
+        primitive_indices=prim[:T].copy(),
+        triangles=tris[:T].copy(),
+        sorted_morton=morton[:T].copy(),
+        binary_nodes=bnodes[: max(T - 1, 0)].copy(),
+        binary_bounds=bbounds[: max(T - 1, 0)].copy(),
+        root_bounds=np.array(list(out.root.bounds), dtype=np.float32),
+        root_reference=int(out.root.reference),
+        internal_stack_bound=int(out.internal_stack_bound),
+        packed_stack_bound=int(out.packed_stack_bound),
+        scene=np.array(list(out.scene), dtype=np.float32),
+    )
+
+
+def field_count(order: int) -> int:
+    return int(lib().gwn_oracle_moment_field_count(order))
+
+
+def refit_moment(bvh: OracleBvh, order: int, dtype=np.float32) -> np.ndarray:
+    """Moments in the reference's field-SoA node layout: array[node, field, slot]."""
+    F = field_count(order)
+    out = np.zeros((bvh.node_count, F, 4), dtype=dtype)
+    nodes = _c(bvh.nodes, NODE_DTYPE)
+    tris = _c(bvh.triangles, np.float32)
+    fn = lib().gwn_oracle_refit_moment_f32 if dtype == np.float32 else lib().gwn_oracle_refit_moment_f64
+    st = fn(_ptr(nodes), bvh.node_count, _ptr(tris), tris.shape[0], order, _ptr(out))
+    if st != 0:
+        raise RuntimeError(f"oracle moment refit failed with status {st}")
+    return out
+
+
+def winding_exact_records_f32(triangles, qx, qy, qz, threads: int | None = None) -> np.ndarray:
+    tris = _c(triangles, np.float32)
+    qx, qy, qz = _c(qx, np.float32), _c(qy, np.float32), _c(qz, np.float32)
+    out = np.empty(qx.shape[0], dtype=np.float32)
+    lib().gwn_oracle_winding_exact_records_f32(
+        _ptr(tris), tris.shape[0], _ptr(qx), _ptr(qy), _ptr(qz), qx.shape[0], _ptr(out),
+        threads or default_threads(),
+    )
+    return out
+
+
+def winding_exact_mesh_f64(mesh, qx, qy, qz, threads: int | None = None) -> np.ndarray:
+    vx, vy, vz = _c(mesh.vx, np.float32), _c(mesh.vy, np.float32), _c(mesh.vz, np.float32)
+    i0, i1, i2 = _c(mesh.i0, np.uint32), _c(mesh.i1, np.uint32), _c(mesh.i2, np.uint32)
+    qx, qy, qz = _c(qx, np.float32), _c(qy, np.float32), _c(qz, np.float32)
+    out = np.empty(qx.shape[0], dtype=np.float64)
+    lib().gwn_oracle_winding_exact_mesh_f64(
+        _ptr(vx), _ptr(vy), _ptr(vz), _ptr(i0), _ptr(i1), _ptr(i2), i0.shape[0],
+        _ptr(qx), _ptr(qy), _ptr(qz), qx.shape[0], _ptr(out), threads or default_threads(),
+    )
+    return out
+
+
+def solid_angle_f64(a, b, c, q) -> float:
+    a, b, c, q = (_c(v, np.float64) for v in (a, b, c, q))
+    return float(lib().gwn_oracle_solid_angle_f64(_ptr(a), _ptr(b), _ptr(c), _ptr(q)))
+
+
+def solid_angle_f32(a, b, c, q) -> float:
+    a, b, c, q = (_c(v, np.float32) for v in (a, b, c, q))
+    return float(lib().gwn_oracle_solid_angle_f32(_ptr(a), _ptr(b), _ptr(c), _ptr(q)))
+
+
+def triangle_payload(record, order: int, dtype=np.float64) -> np.ndarray:
+    """34 additive payload values of one (v0, v1, v2) record (see gwn_oracle.h)."""
+    rec = _c(record, np.float32)
+    out = np.zeros(34, dtype=dtype)
+    fn = lib().gwn_oracle_triangle_payload_f64 if dtype == np.float64 else lib().gwn_oracle_triangle_payload_f32
+    fn(_ptr(rec), order, _ptr(out))
+    return out
+
+
+def winding_taylor_f32(bvh: OracleBvh, moments: np.ndarray, order: int, qx, qy, qz,
+                       accuracy_scale: float = 2.0, stack_capacity: int = 64,
+                       counters: bool = False, threads: int | None = None):
+    nodes = _c(bvh.nodes, NODE_DTYPE)
+    tris = _c(bvh.triangles, np.float32)
+    mom = _c(moments, np.float32)
+    qx, qy, qz = _c(qx, np.float32), _c(qy, np.float32), _c(qz, np.float32)
+    out = np.empty(qx.shape[0], dtype=np.float32)
+    cnt = np.zeros((qx.shape[0], 3), dtype=np.uint64) if counters else None
+    root = _Child()
+    for k in range(6):
+        root.bounds[k] = float(bvh.root_bounds[k])
+    root.reference = bvh.root_reference
+    lib().gwn_oracle_winding_taylor_f32(
+        _ptr(nodes), bvh.node_count, root, _ptr(tris), tris.shape[0], _ptr(mom), order,
+        accuracy_scale, stack_capacity, _ptr(qx), _ptr(qy), _ptr(qz), qx.shape[0], _ptr(out),
+        _ptr(cnt) if cnt is not None else None, threads or default_threads(),
+    )
+    return (out, cnt) if counters else out
+
+
+def canonical_dfs(nodes: np.ndarray, root_reference: int):
+    """Slot-order DFS serialization: (kind, offset-or-leaf, count, bounds) with internal offsets erased."""
+    out = []
+    if not (root_reference & VALID_BIT):
+        return out
+    if (root_reference >> 48) != 0:
+        return [("leaf", root_reference & OFFSET_MASK, root_reference >> 48)]
+    stack = [root_reference & OFFSET_MASK]
+    while stack:
+        n = stack.pop()
+        kids = []
+        for s in range(4):
+            ch = nodes[n]["children"][s]
+            ref = int(ch["reference"])
+            b = tuple(np.asarray(ch["bounds"]).view(np.uint32).tolist())
+            if not ref & VALID_BIT:
+                out.append(("invalid", s))
+                continue
+            if (ref >> 48) != 0:
+                out.append(("leaf", s, ref & OFFSET_MASK, ref >> 48, b))
+            else:
+                out.append(("internal", s, b))
+                kids.append(ref & OFFSET_MASK)
+        stack.extend(reversed(kids))
+    return out

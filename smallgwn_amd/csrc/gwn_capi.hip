@@ -1,0 +1,467 @@
+// gwn_capi.hip - extern "C" surface of libgwn_hip.so (declared in include/gwn_hip.h).
+//
+// Every entry point validates its arguments before enqueuing work, with the reference's
+// diagnostics (include/gwn/gwn_query.cuh:140-445, include/gwn/detail/gwn_bvh_build_impl.cuh:55-95,
+// include/gwn/detail/gwn_geometry_impl.cuh:80-130), and translates exceptions into status codes
+// the way gwn_try_translate_status does (include/gwn/detail/gwn_bvh_status_helpers.cuh:43-74).
+// Builds and refits run into a staging object that replaces the target only on success (strong
+// guarantee); the replaced storage is released on the stream it was allocated on.
+#include "gwn_core.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, std::string const &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <class F> int guarded(char const *fn, F &&f) noexcept {
+    try {
+        f();
+        return GWN_SUCCESS;
+    } catch (gwn::error const &e) {
+        return fail(e.code, e.what());
+    } catch (std::bad_alloc const &) {
+        return fail(GWN_ERROR_INTERNAL, std::string("Host allocation failed in ") + fn + ".");
+    } catch (std::exception const &e) {
+        return fail(GWN_ERROR_INTERNAL, std::string("Unhandled std::exception in ") + fn + ": " + e.what());
+    } catch (...) {
+        return fail(GWN_ERROR_INTERNAL, std::string("Unhandled unknown exception in ") + fn + ".");
+    }
+}
+
+gwn_query_config resolve_config(gwn_query_config const *cfg) {
+    gwn_query_config c{256, 64, -1, 0};
+    if (cfg != nullptr)
+        c = *cfg;
+    return c;
+}
+
+void validate_config(gwn_query_config const &c, bool traversal) {
+    if (c.block_size <= 0 || c.block_size > 1024)
+        gwn::throw_invalid("Query batch block size must be in the range [1, 1024].");
+    if (traversal && c.stack_capacity <= 0)
+        gwn::throw_invalid("Traversal stack capacity must be positive.");
+}
+
+gwn::query_launch to_launch(gwn_query_config const &c) {
+    gwn::query_launch q;
+    q.block_size = c.block_size;
+    q.stack_capacity = c.stack_capacity;
+    q.sort_queries = c.sort_queries;
+    return q;
+}
+
+void require_bvh(gwn_bvh_handle const *bvh) {
+    if (bvh == nullptr || !bvh->valid())
+        gwn::throw_invalid("BVH object contains no queryable data.");
+}
+
+void require_stack(gwn_bvh_handle const &bvh, int capacity) {
+    if (gwn::ref_internal(bvh.root.reference) && bvh.internal_stack_bound > static_cast<std::uint32_t>(capacity))
+        gwn::throw_invalid("Traversal stack capacity is below the BVH internal stack bound.");
+}
+
+void require_moment(gwn_moment_handle const *m, gwn_bvh_handle const &bvh) {
+    bool ok = m != nullptr && m->bvh_revision != 0 && m->bvh_revision == bvh.revision;
+    if (ok) {
+        if (gwn::ref_leaf(bvh.root.reference))
+            ok = m->node_count == 0;
+        else
+            ok = m->node_count == bvh.node_count && m->records.ptr != nullptr;
+    }
+    if (!ok)
+        gwn::throw_invalid("Moment object is not aligned with the BVH.");
+}
+
+void require_query_storage(std::uint64_t count, void const *a, void const *b, void const *c, void const *o) {
+    if (count != 0 && (a == nullptr || b == nullptr || c == nullptr || o == nullptr))
+        gwn::throw_invalid("Query/output spans must use non-null storage when non-empty.");
+}
+
+} // namespace
+
+extern "C" {
+
+int gwn_abi_version(void) { return GWN_HIP_ABI_VERSION; }
+
+const char *gwn_last_error_message(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------------------------------
+// geometry
+// ------------------------------------------------------------------------------------------
+
+int gwn_geometry_create(gwn_geometry_t *out) {
+    return guarded("gwn_geometry_create", [&] {
+        if (out == nullptr)
+            gwn::throw_invalid("Output handle pointer is null.");
+        *out = new gwn_geometry_handle();
+    });
+}
+
+int gwn_geometry_destroy(gwn_geometry_t geometry) {
+    return guarded("gwn_geometry_destroy", [&] { delete geometry; });
+}
+
+int gwn_upload_geometry_f32(
+    gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count, hipStream_t stream
+) {
+    return guarded("gwn_upload_geometry", [&] {
+        if (geometry == nullptr)
+            gwn::throw_invalid("Geometry handle is null.");
+        if (vertex_count > 0xffffffffull)
+            gwn::throw_invalid("Vertex count exceeds index type capacity.");
+        if (triangle_count > 0xffffffffull)
+            gwn::throw_invalid("Triangle count exceeds index type capacity.");
+        if ((vertex_count != 0 && (x == nullptr || y == nullptr || z == nullptr)) ||
+            (triangle_count != 0 && (i0 == nullptr || i1 == nullptr || i2 == nullptr)))
+            gwn::throw_invalid("Geometry spans must use non-null storage when non-empty.");
+        for (std::uint64_t t = 0; t < triangle_count; ++t)
+            if (i0[t] >= vertex_count || i1[t] >= vertex_count || i2[t] >= vertex_count)
+                gwn::throw_invalid("Triangle indices must be in [0, vertex_count).");
+        gwn_geometry_handle staging;
+        staging.stream = stream;
+        staging.vertex_count = vertex_count;
+        staging.triangle_count = triangle_count;
+        staging.x.allocate(vertex_count, stream);
+        staging.y.allocate(vertex_count, stream);
+        staging.z.allocate(vertex_count, stream);
+        staging.i0.allocate(triangle_count, stream);
+        staging.i1.allocate(triangle_count, stream);
+        staging.i2.allocate(triangle_count, stream);
+        std::size_t const vb = vertex_count * sizeof(float), tb = triangle_count * sizeof(std::uint32_t);
+        if (vb) {
+            GWN_HIP(hipMemcpyAsync(staging.x.ptr, x, vb, hipMemcpyHostToDevice, stream));
+            GWN_HIP(hipMemcpyAsync(staging.y.ptr, y, vb, hipMemcpyHostToDevice, stream));
+            GWN_HIP(hipMemcpyAsync(staging.z.ptr, z, vb, hipMemcpyHostToDevice, stream));
+        }
+        if (tb) {
+            GWN_HIP(hipMemcpyAsync(staging.i0.ptr, i0, tb, hipMemcpyHostToDevice, stream));
+            GWN_HIP(hipMemcpyAsync(staging.i1.ptr, i1, tb, hipMemcpyHostToDevice, stream));
+            GWN_HIP(hipMemcpyAsync(staging.i2.ptr, i2, tb, hipMemcpyHostToDevice, stream));
+        }
+        // Host sources may be pageable and released by the caller after return.
+        GWN_HIP(hipStreamSynchronize(stream));
+        staging.generation = geometry->generation + 1;
+        std::swap(staging.x, geometry->x);
+        std::swap(staging.y, geometry->y);
+        std::swap(staging.z, geometry->z);
+        std::swap(staging.i0, geometry->i0);
+        std::swap(staging.i1, geometry->i1);
+        std::swap(staging.i2, geometry->i2);
+        geometry->vertex_count = staging.vertex_count;
+        geometry->triangle_count = staging.triangle_count;
+        geometry->stream = stream;
+        geometry->generation = staging.generation;
+    });
+}
+
+static void update_positions(
+    gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count,
+    hipStream_t stream, hipMemcpyKind kind
+) {
+    if (geometry == nullptr || (geometry->vertex_count != 0 && geometry->x.ptr == nullptr))
+        gwn::throw_invalid("Geometry accessor is invalid for update.");
+    if (vertex_count != geometry->vertex_count)
+        gwn::throw_invalid("Updated vertex count must match geometry.");
+    if (vertex_count != 0 && (x == nullptr || y == nullptr || z == nullptr))
+        gwn::throw_invalid("Geometry update spans must use non-null storage when non-empty.");
+    geometry->stream = stream;
+    std::size_t const vb = vertex_count * sizeof(float);
+    if (vb) {
+        GWN_HIP(hipMemcpyAsync(geometry->x.ptr, x, vb, kind, stream));
+        GWN_HIP(hipMemcpyAsync(geometry->y.ptr, y, vb, kind, stream));
+        GWN_HIP(hipMemcpyAsync(geometry->z.ptr, z, vb, kind, stream));
+    }
+    geometry->generation += 1;
+}
+
+int gwn_update_geometry_f32(
+    gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count, hipStream_t stream
+) {
+    return guarded("gwn_update_geometry", [&] {
+        update_positions(geometry, x, y, z, vertex_count, stream, hipMemcpyHostToDevice);
+        GWN_HIP(hipStreamSynchronize(stream));
+    });
+}
+
+int gwn_update_geometry_device_f32(
+    gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count, hipStream_t stream
+) {
+    return guarded("gwn_update_geometry", [&] {
+        update_positions(geometry, x, y, z, vertex_count, stream, hipMemcpyDeviceToDevice);
+    });
+}
+
+int gwn_geometry_counts(gwn_geometry_t geometry, uint64_t *vertex_count, uint64_t *triangle_count) {
+    return guarded("gwn_geometry_counts", [&] {
+        if (geometry == nullptr)
+            gwn::throw_invalid("Geometry handle is null.");
+        if (vertex_count)
+            *vertex_count = geometry->vertex_count;
+        if (triangle_count)
+            *triangle_count = geometry->triangle_count;
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// BVH
+// ------------------------------------------------------------------------------------------
+
+int gwn_bvh_create(gwn_bvh_t *out) {
+    return guarded("gwn_bvh_create", [&] {
+        if (out == nullptr)
+            gwn::throw_invalid("Output handle pointer is null.");
+        *out = new gwn_bvh_handle();
+    });
+}
+
+int gwn_bvh_destroy(gwn_bvh_t bvh) {
+    return guarded("gwn_bvh_destroy", [&] { delete bvh; });
+}
+
+int gwn_build_bvh4_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, int32_t method, uint32_t hploc_search_radius, hipStream_t stream
+) {
+    return guarded("gwn_build_bvh", [&] {
+        if (bvh == nullptr)
+            gwn::throw_invalid("bvh.build: BVH handle is null.");
+        if (geometry == nullptr || (geometry->triangle_count != 0 && geometry->i0.ptr == nullptr))
+            gwn::throw_invalid("bvh.build: Geometry accessor is invalid for BVH build.");
+        if (method != GWN_BVH_BUILD_HPLOC && method != GWN_BVH_BUILD_LBVH)
+            gwn::throw_invalid("bvh.build: BVH build method is invalid.");
+        if (method == GWN_BVH_BUILD_HPLOC && (hploc_search_radius == 0 || hploc_search_radius > 8))
+            gwn::throw_invalid("bvh.build: H-PLOC search radius must be in the range [1, 8].");
+        if (method == GWN_BVH_BUILD_HPLOC)
+            gwn::throw_invalid("bvh.build: the H-PLOC builder is not available in this library version; use the LBVH method.");
+        if (geometry->triangle_count != 0 && geometry->vertex_count == 0)
+            gwn::throw_invalid("bvh.build: Cannot build a BVH with triangles but no vertices.");
+        if (geometry->triangle_count > 0x7fffffffull)
+            gwn::throw_invalid("bvh.build: Triangle count exceeds the native wide-collapse counter range.");
+        gwn_bvh_handle staging;
+        staging.stream = stream;
+        gwn::build_bvh4(*geometry, staging, stream);
+        if (geometry->triangle_count != 0) {
+            staging.revision = gwn::reserve_revision();
+            if (staging.revision == 0)
+                gwn::throw_internal("bvh.build: BVH geometry-derived state revision is exhausted.");
+        }
+        std::swap(*bvh, staging);
+        // staging now owns the replaced storage; its buffers free on their own streams.
+    });
+}
+
+int gwn_bvh_get_info(gwn_bvh_t bvh, gwn_bvh_info *info) {
+    return guarded("gwn_bvh_get_info", [&] {
+        if (bvh == nullptr || info == nullptr)
+            gwn::throw_invalid("BVH handle or info pointer is null.");
+        std::memset(info, 0, sizeof(*info));
+        info->node_count = bvh->node_count;
+        info->triangle_count = bvh->revision ? bvh->triangle_count : 0;
+        info->revision = bvh->revision;
+        info->internal_stack_bound = bvh->internal_stack_bound;
+        info->packed_stack_bound = bvh->packed_stack_bound;
+        info->max_depth = bvh->max_depth;
+        info->width = 4;
+        std::memcpy(info->root_bounds, &bvh->root.bounds, sizeof(info->root_bounds));
+        info->root_reference = bvh->root.reference;
+    });
+}
+
+int gwn_bvh_download(gwn_bvh_t bvh, void *nodes, uint32_t *primitive_indices, float *triangles, hipStream_t stream) {
+    return guarded("gwn_bvh_download", [&] {
+        require_bvh(bvh);
+        if (nodes && bvh->node_count)
+            GWN_HIP(hipMemcpyAsync(nodes, bvh->nodes.ptr, bvh->node_count * sizeof(gwn::bvh_node4), hipMemcpyDeviceToHost, stream));
+        if (primitive_indices)
+            GWN_HIP(hipMemcpyAsync(primitive_indices, bvh->primitive_indices.ptr, bvh->triangle_count * sizeof(std::uint32_t), hipMemcpyDeviceToHost, stream));
+        if (triangles)
+            GWN_HIP(hipMemcpyAsync(triangles, bvh->triangles.ptr, bvh->triangle_count * sizeof(gwn::tri_record), hipMemcpyDeviceToHost, stream));
+        GWN_HIP(hipStreamSynchronize(stream));
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// moments
+// ------------------------------------------------------------------------------------------
+
+int gwn_moment_create(gwn_moment_t *out, int32_t order) {
+    return guarded("gwn_moment_create", [&] {
+        if (out == nullptr)
+            gwn::throw_invalid("Output handle pointer is null.");
+        if (order < 0 || order > 2)
+            gwn::throw_invalid("Taylor order must be 0, 1, or 2.");
+        auto *m = new gwn_moment_handle();
+        m->order = order;
+        *out = m;
+    });
+}
+
+int gwn_moment_destroy(gwn_moment_t moment) {
+    return guarded("gwn_moment_destroy", [&] { delete moment; });
+}
+
+int gwn_moment_order(gwn_moment_t moment, int32_t *order) {
+    return guarded("gwn_moment_order", [&] {
+        if (moment == nullptr || order == nullptr)
+            gwn::throw_invalid("Moment handle or output pointer is null.");
+        *order = moment->order;
+    });
+}
+
+int gwn_refit_bvh_moment_f32(gwn_bvh_t bvh, gwn_moment_t moment, hipStream_t stream) {
+    return guarded("gwn_refit_bvh_moment", [&] {
+        if (bvh == nullptr || !bvh->valid())
+            gwn::throw_invalid("bvh.refit.moment: BVH accessor is invalid for moment refit.");
+        if (moment == nullptr)
+            gwn::throw_invalid("bvh.refit.moment: Moment handle is null.");
+        gwn_moment_handle staging;
+        staging.order = moment->order;
+        staging.stream = stream;
+        gwn::refit_moment(*bvh, staging, stream);
+        std::swap(*moment, staging);
+    });
+}
+
+int gwn_moment_download(gwn_moment_t moment, float *fields, hipStream_t stream) {
+    return guarded("gwn_moment_download", [&] {
+        if (moment == nullptr || fields == nullptr)
+            gwn::throw_invalid("Moment handle or output pointer is null.");
+        std::size_t const N = moment->node_count;
+        int const F = gwn::moment_field_count(moment->order);
+        int const RV = gwn::moment_record_vec4(moment->order);
+        if (N == 0 || moment->records.ptr == nullptr)
+            return;
+        std::vector<float> host(N * RV * 4);
+        GWN_HIP(hipMemcpyAsync(host.data(), moment->records.ptr, host.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
+        GWN_HIP(hipStreamSynchronize(stream));
+        for (std::size_t n = 0; n < N; ++n)
+            std::memcpy(fields + n * F * 4, host.data() + n * RV * 4 + 4, sizeof(float) * F * 4);
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// winding-number batches
+// ------------------------------------------------------------------------------------------
+
+int gwn_compute_winding_number_exact_batch_f32(
+    gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, float *out, uint64_t count,
+    const gwn_query_config *config, hipStream_t stream
+) {
+    return guarded("gwn_compute_winding_number_exact_batch", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, false);
+        require_bvh(bvh);
+        require_query_storage(count, qx, qy, qz, out);
+        if (count == 0)
+            return;
+        gwn::winding_exact(*bvh, qx, qy, qz, out, count, -1, to_launch(c), stream);
+    });
+}
+
+int gwn_compute_winding_number_exact_batch_ex_f32(
+    gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, double *out, uint64_t count,
+    int32_t precision, const gwn_query_config *config, hipStream_t stream
+) {
+    return guarded("gwn_compute_winding_number_exact_batch", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, false);
+        require_bvh(bvh);
+        if (precision != GWN_PRECISION_PLAIN && precision != GWN_PRECISION_SPLIT && precision != GWN_PRECISION_FP64)
+            gwn::throw_invalid("Unknown winding precision mode.");
+        require_query_storage(count, qx, qy, qz, out);
+        if (count == 0)
+            return;
+        gwn::winding_exact(*bvh, qx, qy, qz, out, count, precision, to_launch(c), stream);
+    });
+}
+
+static void taylor_common(
+    gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz, float *out,
+    uint64_t count, float accuracy_scale, int32_t precision, const gwn_query_config *config,
+    uint32_t *counters, hipStream_t stream
+) {
+    gwn_query_config const c = resolve_config(config);
+    validate_config(c, true);
+    require_bvh(bvh);
+    require_stack(*bvh, c.stack_capacity);
+    require_moment(moment, *bvh);
+    if (!(accuracy_scale >= 0.0f))
+        gwn::throw_invalid("Taylor accuracy scale must be nonnegative.");
+    if (precision != GWN_PRECISION_PLAIN && precision != GWN_PRECISION_SPLIT)
+        gwn::throw_invalid("Taylor winding supports the plain and split precision modes.");
+    require_query_storage(count, qx, qy, qz, out);
+    if (count == 0)
+        return;
+    gwn::winding_taylor(*bvh, *moment, qx, qy, qz, out, count, accuracy_scale, precision, to_launch(c), counters, stream);
+}
+
+int gwn_compute_winding_number_taylor_batch_f32(
+    gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz, float *out,
+    uint64_t count, float accuracy_scale, const gwn_query_config *config, hipStream_t stream
+) {
+    return guarded("gwn_compute_winding_number_taylor_batch", [&] {
+        taylor_common(bvh, moment, qx, qy, qz, out, count, accuracy_scale, GWN_PRECISION_PLAIN, config, nullptr, stream);
+    });
+}
+
+int gwn_compute_winding_number_taylor_batch_ex_f32(
+    gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz, float *out,
+    uint64_t count, float accuracy_scale, int32_t precision, const gwn_query_config *config,
+    uint32_t *counters, hipStream_t stream
+) {
+    return guarded("gwn_compute_winding_number_taylor_batch", [&] {
+        taylor_common(bvh, moment, qx, qy, qz, out, count, accuracy_scale, precision, config, counters, stream);
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// sort building block
+// ------------------------------------------------------------------------------------------
+
+int gwn_radix_sort_pairs_u64(uint64_t *keys, uint32_t *values, uint64_t count, int32_t begin_bit, int32_t end_bit, hipStream_t stream) {
+    return guarded("gwn_radix_sort_pairs_u64", [&] {
+        if (count != 0 && (keys == nullptr || values == nullptr))
+            gwn::throw_invalid("Sort spans must use non-null storage when non-empty.");
+        if (begin_bit < 0 || end_bit > 64 || begin_bit > end_bit)
+            gwn::throw_invalid("Sort bit range must satisfy 0 <= begin_bit <= end_bit <= 64.");
+        gwn::device_buffer<std::uint64_t> kalt(count, stream);
+        gwn::device_buffer<std::uint32_t> valt(count, stream);
+        std::uint64_t *ko = nullptr;
+        std::uint32_t *vo = nullptr;
+        gwn::radix_sort_pairs_u64(keys, kalt.ptr, values, valt.ptr, count, begin_bit, end_bit, &ko, &vo, stream);
+        if (ko != keys) {
+            GWN_HIP(hipMemcpyAsync(keys, ko, count * sizeof(std::uint64_t), hipMemcpyDeviceToDevice, stream));
+            GWN_HIP(hipMemcpyAsync(values, vo, count * sizeof(std::uint32_t), hipMemcpyDeviceToDevice, stream));
+        }
+    });
+}
+
+int gwn_radix_sort_pairs_u32(uint32_t *keys, uint32_t *values, uint64_t count, int32_t begin_bit, int32_t end_bit, hipStream_t stream) {
+    return guarded("gwn_radix_sort_pairs_u32", [&] {
+        if (count != 0 && (keys == nullptr || values == nullptr))
+            gwn::throw_invalid("Sort spans must use non-null storage when non-empty.");
+        if (begin_bit < 0 || end_bit > 32 || begin_bit > end_bit)
+            gwn::throw_invalid("Sort bit range must satisfy 0 <= begin_bit <= end_bit <= 32.");
+        gwn::device_buffer<std::uint32_t> kalt(count, stream);
+        gwn::device_buffer<std::uint32_t> valt(count, stream);
+        std::uint32_t *ko = nullptr;
+        std::uint32_t *vo = nullptr;
+        gwn::radix_sort_pairs_u32(keys, kalt.ptr, values, valt.ptr, count, begin_bit, end_bit, &ko, &vo, stream);
+        if (ko != keys) {
+            GWN_HIP(hipMemcpyAsync(keys, ko, count * sizeof(std::uint32_t), hipMemcpyDeviceToDevice, stream));
+            GWN_HIP(hipMemcpyAsync(values, vo, count * sizeof(std::uint32_t), hipMemcpyDeviceToDevice, stream));
+        }
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,255 @@
+// gwn_core.hpp - shared host/device definitions of the MI355X (gfx950) geometric-query library.
+//
+// The public surface is the C ABI in include/gwn_hip.h; everything here is internal.
+// Host code throws gwn::error; the C ABI layer translates it into a status code and a
+// thread-local message, mirroring the reference's gwn_status contract
+// (include/gwn/gwn_utils.cuh:213-283, include/gwn/detail/gwn_bvh_status_helpers.cuh:43-74).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/gwn_hip.h"
+
+namespace gwn {
+
+// ---------------------------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------------------------
+
+struct error : std::runtime_error {
+    int code;
+    error(int c, std::string const &msg) : std::runtime_error(msg), code(c) {}
+};
+
+[[noreturn]] inline void throw_invalid(std::string const &msg) {
+    throw error(GWN_ERROR_INVALID_ARGUMENT, msg);
+}
+[[noreturn]] inline void throw_internal(std::string const &msg) {
+    throw error(GWN_ERROR_INTERNAL, msg);
+}
+
+inline void hip_check(hipError_t const e, char const *what) {
+    if (e != hipSuccess) {
+        throw error(
+            GWN_ERROR_HIP_RUNTIME, std::string(what) + " failed [hip_error=" +
+                                       std::to_string(static_cast<int>(e)) + " (" +
+                                       hipGetErrorName(e) + "): \"" + hipGetErrorString(e) + "\"]"
+        );
+    }
+}
+
+#define GWN_HIP(call) ::gwn::hip_check((call), #call)
+#define GWN_LAUNCH_CHECK() ::gwn::hip_check(hipGetLastError(), "kernel launch")
+
+// ---------------------------------------------------------------------------------------------
+// Stream-ordered device memory (hipMallocAsync pools; the reference uses cudaMallocAsync,
+// include/gwn/gwn_utils.cuh:364-392).
+// ---------------------------------------------------------------------------------------------
+
+void *device_alloc(std::size_t bytes, hipStream_t stream);
+void device_free(void *ptr, hipStream_t stream) noexcept;
+
+template <class T> struct device_buffer {
+    T *ptr = nullptr;
+    std::size_t count = 0;
+    hipStream_t stream = nullptr;
+
+    device_buffer() = default;
+    device_buffer(std::size_t n, hipStream_t s) { allocate(n, s); }
+    device_buffer(device_buffer const &) = delete;
+    device_buffer &operator=(device_buffer const &) = delete;
+    device_buffer(device_buffer &&o) noexcept { swap(o); }
+    device_buffer &operator=(device_buffer &&o) noexcept {
+        device_buffer tmp(static_cast<device_buffer &&>(o));
+        swap(tmp);
+        return *this;
+    }
+    ~device_buffer() { release(); }
+
+    void swap(device_buffer &o) noexcept {
+        std::swap(ptr, o.ptr);
+        std::swap(count, o.count);
+        std::swap(stream, o.stream);
+    }
+    void allocate(std::size_t n, hipStream_t s) {
+        release();
+        stream = s;
+        count = n;
+        ptr = n ? static_cast<T *>(device_alloc(n * sizeof(T), s)) : nullptr;
+    }
+    void release() noexcept {
+        if (ptr != nullptr)
+            device_free(ptr, stream);
+        ptr = nullptr;
+        count = 0;
+    }
+    T *release_ownership() noexcept {
+        T *p = ptr;
+        ptr = nullptr;
+        count = 0;
+        return p;
+    }
+    std::size_t bytes() const noexcept { return count * sizeof(T); }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Canonical BVH layout (kept byte-compatible with the reference's gwn_bvh_child<float> and
+// gwn_bvh_node<4, float>: include/gwn/gwn_bvh.cuh:25-130).
+// ---------------------------------------------------------------------------------------------
+
+constexpr int k_offset_bits = 47;
+constexpr std::uint64_t k_offset_mask = (std::uint64_t(1) << k_offset_bits) - 1;
+constexpr std::uint64_t k_valid_mask = std::uint64_t(1) << k_offset_bits;
+constexpr int k_count_shift = k_offset_bits + 1;
+
+struct aabb {
+    float min_x, min_y, min_z, max_x, max_y, max_z;
+};
+
+struct bvh_child {
+    aabb bounds;
+    std::uint64_t reference;
+};
+static_assert(sizeof(bvh_child) == 32, "child must be one 32-byte record");
+
+struct bvh_node4 {
+    bvh_child children[4];
+};
+static_assert(sizeof(bvh_node4) == 128, "BVH4 node must be 128 bytes");
+
+__host__ __device__ constexpr std::uint64_t encode_internal(std::uint64_t node) {
+    return k_valid_mask | node;
+}
+__host__ __device__ constexpr std::uint64_t encode_leaf(std::uint64_t offset, std::uint64_t count) {
+    return k_valid_mask | offset | (count << k_count_shift);
+}
+__host__ __device__ constexpr bool ref_valid(std::uint64_t r) { return (r & k_valid_mask) != 0; }
+__host__ __device__ constexpr std::uint32_t ref_count(std::uint64_t r) {
+    return static_cast<std::uint32_t>(r >> k_count_shift);
+}
+__host__ __device__ constexpr bool ref_internal(std::uint64_t r) {
+    return ref_valid(r) && ref_count(r) == 0;
+}
+__host__ __device__ constexpr bool ref_leaf(std::uint64_t r) { return ref_valid(r) && ref_count(r) != 0; }
+__host__ __device__ constexpr std::uint64_t ref_offset(std::uint64_t r) { return r & k_offset_mask; }
+
+// Compact 32-bit child reference used by the traversal records of the moment object:
+// bit 31 set = leaf (sorted primitive offset in bits 0-30), otherwise an internal node index.
+constexpr std::uint32_t k_cref_invalid = 0xffffffffu;
+constexpr std::uint32_t k_cref_leaf = 0x80000000u;
+
+// Leaf-ordered triangle record: the three vertices, 36 bytes.  The reference's
+// (v0, e1, e2) form is recovered exactly as e1 = v1 - v0 (include/gwn/detail/
+// gwn_bvh_triangle_impl.cuh:56-59), and the exact vertices feed the FP64 near field.
+struct tri_record {
+    float v0x, v0y, v0z, v1x, v1y, v1z, v2x, v2y, v2z;
+};
+static_assert(sizeof(tri_record) == 36, "triangle record must be 36 bytes");
+
+// Number of float fields per child slot of a Taylor moment node (include/gwn/gwn_bvh.cuh:144-202).
+__host__ __device__ constexpr int moment_field_count(int order) {
+    return order == 0 ? 7 : (order == 1 ? 13 : 23);
+}
+// Traversal record: one float4 of compact child references, then the fields as float4 (slot-SoA),
+// padded to a whole number of 128-byte cache lines: 128 B / 256 B / 384 B for order 0 / 1 / 2.
+__host__ __device__ constexpr int moment_record_vec4(int order) {
+    return order == 0 ? 8 : (order == 1 ? 16 : 24);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Library objects behind the opaque C handles.
+// ---------------------------------------------------------------------------------------------
+
+std::uint64_t reserve_revision() noexcept;
+
+} // namespace gwn
+
+struct gwn_geometry_handle {
+    gwn::device_buffer<float> x, y, z;
+    gwn::device_buffer<std::uint32_t> i0, i1, i2;
+    std::size_t vertex_count = 0;
+    std::size_t triangle_count = 0;
+    hipStream_t stream = nullptr;
+    std::uint64_t generation = 0;  // bumps on every upload/update
+};
+
+struct gwn_bvh_handle {
+    gwn::device_buffer<gwn::bvh_node4> nodes;
+    gwn::device_buffer<std::uint32_t> primitive_indices;
+    gwn::device_buffer<gwn::tri_record> triangles;
+    gwn::device_buffer<std::uint32_t> parent;          // (parent << 2 | slot), root = ~0u
+    gwn::device_buffer<std::uint8_t> internal_children; // internal child count per node
+    gwn::bvh_child root{};
+    std::size_t node_count = 0;
+    std::size_t triangle_count = 0;
+    std::uint32_t internal_stack_bound = 0;
+    std::uint32_t packed_stack_bound = 0;
+    std::uint32_t max_depth = 0;
+    std::uint64_t revision = 0;
+    hipStream_t stream = nullptr;
+
+    bool valid() const noexcept {
+        if (revision == 0 || !gwn::ref_valid(root.reference) || triangle_count == 0)
+            return false;
+        if (gwn::ref_internal(root.reference))
+            return gwn::ref_offset(root.reference) < node_count;
+        return gwn::ref_offset(root.reference) + gwn::ref_count(root.reference) <= triangle_count;
+    }
+};
+
+struct gwn_moment_handle {
+    int order = 0;
+    gwn::device_buffer<float4> records; // node_count * moment_record_vec4(order)
+    std::size_t node_count = 0;
+    std::uint64_t bvh_revision = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace gwn {
+
+// Radix sort of (key, value) pairs on bits [begin_bit, end_bit).  Keys are uint32 or uint64;
+// values are uint32.  The result ends in (keys_out, vals_out); the inputs are clobbered.
+void radix_sort_pairs_u64(
+    std::uint64_t *keys, std::uint64_t *keys_alt, std::uint32_t *vals, std::uint32_t *vals_alt,
+    std::size_t n, int begin_bit, int end_bit, std::uint64_t **keys_out,
+    std::uint32_t **vals_out, hipStream_t stream
+);
+void radix_sort_pairs_u32(
+    std::uint32_t *keys, std::uint32_t *keys_alt, std::uint32_t *vals, std::uint32_t *vals_alt,
+    std::size_t n, int begin_bit, int end_bit, std::uint32_t **keys_out,
+    std::uint32_t **vals_out, hipStream_t stream
+);
+
+void build_bvh4(gwn_geometry_handle const &geometry, gwn_bvh_handle &staging, hipStream_t stream);
+void refit_moment(gwn_bvh_handle const &bvh, gwn_moment_handle &staging, hipStream_t stream);
+
+struct query_launch {
+    int block_size = 256;
+    int stack_capacity = 64;
+    int sort_queries = -1; // -1 auto, 0 off, 1 on
+};
+
+void winding_exact(
+    gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, void *out,
+    std::size_t n, int precision, query_launch const &cfg, hipStream_t stream
+);
+void winding_taylor(
+    gwn_bvh_handle const &bvh, gwn_moment_handle const &moment, float const *qx, float const *qy,
+    float const *qz, float *out, std::size_t n, float accuracy_scale, int precision,
+    query_launch const &cfg, std::uint32_t *counters, hipStream_t stream
+);
+
+// Zero-fill helper (a kernel, not hipMemsetAsync, so that captured graphs replay it safely).
+void fill_u32(std::uint32_t *ptr, std::uint32_t value, std::size_t n, hipStream_t stream);
+
+inline unsigned grid_for(std::size_t n, int block) {
+    std::size_t g = (n + static_cast<std::size_t>(block) - 1) / static_cast<std::size_t>(block);
+    return static_cast<unsigned>(g == 0 ? 1 : g);
+}
+
+} // namespace gwn

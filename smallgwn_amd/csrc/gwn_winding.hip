@@ -1,0 +1,452 @@
+// gwn_winding.hip - exact and Taylor winding-number batches on gfx950.
+//
+// Reference: include/gwn/detail/gwn_query_winding_impl.cuh:102-443,
+// include/gwn/detail/gwn_query_geometry_impl.cuh:16-45, include/gwn/gwn_query.cuh:296-445.
+//
+// Exact batch: one query per lane; the block streams the leaf-ordered triangle records through
+// LDS in 256-record tiles, so every record is read from HBM/L2 once per block and broadcast to
+// all 256 lanes.  VALU bound (solid angle: 3 sqrt, 3 div, 1 atan2 per pair).
+//
+// Taylor batch: one query per lane, queries optionally visited in Morton order (results are
+// written back in input order).  Per-lane stack of internal node ids; the last internal child
+// of a node is entered directly, the others are pushed, leaves are evaluated in slot order --
+// the reference's traversal order, so float sums are formed in the same sequence.  The node
+// data is one traversal record per node (compact child refs + field-SoA coefficients), so a
+// node visit touches 2-3 consecutive 128-byte lines and no separate hierarchy array.
+//
+// Precision split: far-field Taylor terms in FP32; every near-field (exact) triangle term in
+// FP64 on the exact vertices, accumulated in FP64.
+#include "gwn_core.hpp"
+
+#include <cmath>
+
+namespace gwn {
+namespace {
+
+constexpr float k_pi_f = 3.14159265358979323846f;
+constexpr double k_pi_d = 3.14159265358979323846;
+constexpr std::uint32_t k_invalid = 0xffffffffu;
+
+// Reference-form FP32 solid angle: rebuild b = v0 + (v1 - v0), c = v0 + (v2 - v0) as the
+// reference's (v0, e1, e2) record does, then Van Oosterom-Strackee.
+__device__ inline float solid_angle_plain(tri_record const &t, float qx, float qy, float qz) {
+    float const e1x = t.v1x - t.v0x, e1y = t.v1y - t.v0y, e1z = t.v1z - t.v0z;
+    float const e2x = t.v2x - t.v0x, e2y = t.v2y - t.v0y, e2z = t.v2z - t.v0z;
+    float ax = t.v0x - qx, ay = t.v0y - qy, az = t.v0z - qz;
+    float bx = (t.v0x + e1x) - qx, by = (t.v0y + e1y) - qy, bz = (t.v0z + e1z) - qz;
+    float cx = (t.v0x + e2x) - qx, cy = (t.v0y + e2y) - qy, cz = (t.v0z + e2z) - qz;
+    float const al = sqrtf(ax * ax + ay * ay + az * az);
+    float const bl = sqrtf(bx * bx + by * by + bz * bz);
+    float const cl = sqrtf(cx * cx + cy * cy + cz * cz);
+    if (al == 0.0f || bl == 0.0f || cl == 0.0f)
+        return 0.0f;
+    ax /= al; ay /= al; az /= al;
+    bx /= bl; by /= bl; bz /= bl;
+    cx /= cl; cy /= cl; cz /= cl;
+    float const ux = bx - ax, uy = by - ay, uz = bz - az;
+    float const vx = cx - ax, vy = cy - ay, vz = cz - az;
+    float const num = ax * (uy * vz - uz * vy) + ay * (uz * vx - ux * vz) + az * (ux * vy - uy * vx);
+    if (num == 0.0f)
+        return 0.0f;
+    float const den = 1.0f + (ax * bx + ay * by + az * bz) + (ax * cx + ay * cy + az * cz) +
+                      (bx * cx + by * cy + bz * cz);
+    return 2.0f * atan2f(num, den);
+}
+
+// FP64 solid angle on the exact vertices.
+__device__ inline double solid_angle_f64(tri_record const &t, double qx, double qy, double qz) {
+    double ax = double(t.v0x) - qx, ay = double(t.v0y) - qy, az = double(t.v0z) - qz;
+    double bx = double(t.v1x) - qx, by = double(t.v1y) - qy, bz = double(t.v1z) - qz;
+    double cx = double(t.v2x) - qx, cy = double(t.v2y) - qy, cz = double(t.v2z) - qz;
+    double const al = sqrt(ax * ax + ay * ay + az * az);
+    double const bl = sqrt(bx * bx + by * by + bz * bz);
+    double const cl = sqrt(cx * cx + cy * cy + cz * cz);
+    if (al == 0.0 || bl == 0.0 || cl == 0.0)
+        return 0.0;
+    double const ia = 1.0 / al, ib = 1.0 / bl, ic = 1.0 / cl;
+    ax *= ia; ay *= ia; az *= ia;
+    bx *= ib; by *= ib; bz *= ib;
+    cx *= ic; cy *= ic; cz *= ic;
+    double const ux = bx - ax, uy = by - ay, uz = bz - az;
+    double const vx = cx - ax, vy = cy - ay, vz = cz - az;
+    double const num = ax * (uy * vz - uz * vy) + ay * (uz * vx - ux * vz) + az * (ux * vy - uy * vx);
+    if (num == 0.0)
+        return 0.0;
+    double const den = 1.0 + (ax * bx + ay * by + az * bz) + (ax * cx + ay * cy + az * cz) +
+                       (bx * cx + by * cy + bz * cz);
+    return 2.0 * atan2(num, den);
+}
+
+// ------------------------------------------------------------------------------------------
+// Exact batch
+// ------------------------------------------------------------------------------------------
+
+constexpr int k_exact_block = 256;
+// Near-field radius of the precision split, in units of the triangle's longest edge from v0.
+constexpr float k_split_near_scale2 = 16.0f; // (4 * edge)^2
+
+template <int Prec, class Out>
+__global__ __launch_bounds__(k_exact_block) void exact_kernel(
+    tri_record const *__restrict__ tris, std::size_t nt, float const *__restrict__ qx,
+    float const *__restrict__ qy, float const *__restrict__ qz, Out *__restrict__ out, std::size_t nq
+) {
+    __shared__ tri_record tile[k_exact_block];
+    std::size_t const qi = static_cast<std::size_t>(blockIdx.x) * k_exact_block + threadIdx.x;
+    bool const active = qi < nq;
+    float const x = active ? qx[qi] : 0.0f;
+    float const y = active ? qy[qi] : 0.0f;
+    float const z = active ? qz[qi] : 0.0f;
+    float sum_f = 0.0f;   // plain: the reference's running float sum
+    double sum_d = 0.0;   // split / fp64
+    for (std::size_t base = 0; base < nt; base += k_exact_block) {
+        std::size_t const count = (nt - base) < k_exact_block ? (nt - base) : k_exact_block;
+        __syncthreads();
+        {
+            float const *src = reinterpret_cast<float const *>(tris + base);
+            float *dst = reinterpret_cast<float *>(tile);
+            std::size_t const words = count * 9;
+            for (std::size_t w = threadIdx.x; w < words; w += k_exact_block)
+                dst[w] = src[w];
+        }
+        __syncthreads();
+        if constexpr (Prec == GWN_PRECISION_PLAIN) {
+            for (std::size_t k = 0; k < count; ++k)
+                sum_f += solid_angle_plain(tile[k], x, y, z);
+        } else if constexpr (Prec == GWN_PRECISION_FP64) {
+            for (std::size_t k = 0; k < count; ++k)
+                sum_d += solid_angle_f64(tile[k], x, y, z);
+        } else {
+            float far = 0.0f;
+            for (std::size_t k = 0; k < count; ++k) {
+                tri_record const &t = tile[k];
+                float const e1 = (t.v1x - t.v0x) * (t.v1x - t.v0x) + (t.v1y - t.v0y) * (t.v1y - t.v0y) +
+                                 (t.v1z - t.v0z) * (t.v1z - t.v0z);
+                float const e2 = (t.v2x - t.v0x) * (t.v2x - t.v0x) + (t.v2y - t.v0y) * (t.v2y - t.v0y) +
+                                 (t.v2z - t.v0z) * (t.v2z - t.v0z);
+                float const dq = (x - t.v0x) * (x - t.v0x) + (y - t.v0y) * (y - t.v0y) + (z - t.v0z) * (z - t.v0z);
+                if (!(dq > k_split_near_scale2 * fmaxf(e1, e2)))
+                    sum_d += solid_angle_f64(t, x, y, z);
+                else
+                    far += solid_angle_plain(t, x, y, z);
+            }
+            sum_d += double(far);
+        }
+    }
+    if (!active)
+        return;
+    if constexpr (Prec == GWN_PRECISION_PLAIN)
+        out[qi] = static_cast<Out>(sum_f / (4.0f * k_pi_f));
+    else
+        out[qi] = static_cast<Out>(sum_d / (4.0 * k_pi_d));
+}
+
+// ------------------------------------------------------------------------------------------
+// Taylor batch
+// ------------------------------------------------------------------------------------------
+
+constexpr int k_taylor_block = 256;
+
+__device__ inline std::uint32_t spread10(std::uint32_t v) {
+    std::uint32_t x = v & 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void query_morton_kernel(
+    float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
+    std::size_t n, aabb box, std::uint32_t *__restrict__ keys, std::uint32_t *__restrict__ ids
+) {
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    float const sx = box.max_x > box.min_x ? 1.0f / (box.max_x - box.min_x) : 1.0f;
+    float const sy = box.max_y > box.min_y ? 1.0f / (box.max_y - box.min_y) : 1.0f;
+    float const sz = box.max_z > box.min_z ? 1.0f / (box.max_z - box.min_z) : 1.0f;
+    // NaN coordinates compare false and land in cell 0; order only affects performance.
+    float const fx = fminf(fmaxf((qx[i] - box.min_x) * sx, 0.0f), 1.0f) * 1023.0f;
+    float const fy = fminf(fmaxf((qy[i] - box.min_y) * sy, 0.0f), 1.0f) * 1023.0f;
+    float const fz = fminf(fmaxf((qz[i] - box.min_z) * sz, 0.0f), 1.0f) * 1023.0f;
+    keys[i] = (spread10(static_cast<std::uint32_t>(fx)) << 2) | (spread10(static_cast<std::uint32_t>(fy)) << 1) |
+              spread10(static_cast<std::uint32_t>(fz));
+    ids[i] = static_cast<std::uint32_t>(i);
+}
+
+template <int Order>
+__device__ inline float taylor_term(float const *f, int s, float qrx, float qry, float qrz, float ql2) {
+#define GWN_F(k) f[(k) * 4 + s]
+    float const m1 = __frsqrt_rn(ql2);
+    float const m2 = m1 * m1;
+    float const m3 = m2 * m1;
+    float const qnx = qrx * m1, qny = qry * m1, qnz = qrz * m1;
+    float const Nx = GWN_F(4), Ny = GWN_F(5), Nz = GWN_F(6);
+    float const rdotN = qnx * Nx + qny * Ny + qnz * Nz;
+    float w = -m2 * rdotN;
+    if constexpr (Order >= 1) {
+        float const qxx = qnx * qnx, qyy = qny * qny, qzz = qnz * qnz;
+        float const Txx = GWN_F(7), Tyy = GWN_F(8), Tzz = GWN_F(9);
+        float const Sxy = GWN_F(10), Syz = GWN_F(11), Szx = GWN_F(12);
+        float const Tr = Txx + Tyy + Tzz;
+        float const S = qxx * Txx + qyy * Tyy + qzz * Tzz + qnx * qny * Sxy + qnx * qnz * Szx + qny * qnz * Syz;
+        w += m3 * (Tr - 3.0f * S);
+    }
+    if constexpr (Order >= 2) {
+        float const m4 = m2 * m2;
+        float const qnx2 = qnx * qnx, qny2 = qny * qny, qnz2 = qnz * qnz;
+        float const qnx3 = qnx2 * qnx, qny3 = qny2 * qny, qnz3 = qnz2 * qnz;
+        float const xxx = GWN_F(13), yyy = GWN_F(14), zzz = GWN_F(15), spn = GWN_F(16);
+        float const Axy = GWN_F(17), Axz = GWN_F(18), Ayz = GWN_F(19), Ayx = GWN_F(20), Azx = GWN_F(21), Azy = GWN_F(22);
+        float const t0x = Ayx + Azx, t0y = Azy + Axy, t0z = Axz + Ayz;
+        float const t1x = qny * Axy + qnz * Axz;
+        float const t1y = qnz * Ayz + qnx * Ayx;
+        float const t1z = qnx * Azx + qny * Azy;
+        w += m4 * (1.5f * (qnx * (3.0f * xxx + t0x) + qny * (3.0f * yyy + t0y) + qnz * (3.0f * zzz + t0z)) -
+                   7.5f * (qnx3 * xxx + qny3 * yyy + qnz3 * zzz + qnx * qny * qnz * spn + qnx2 * t1x +
+                           qny2 * t1y + qnz2 * t1z));
+    }
+#undef GWN_F
+    return w;
+}
+
+template <int Order, int Prec, int StackCap>
+__global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
+    float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node,
+    float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
+    std::uint32_t const *__restrict__ order, float *__restrict__ out, std::size_t n, float beta2,
+    std::uint32_t *__restrict__ counters
+) {
+    constexpr int RV = moment_record_vec4(Order);
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_taylor_block + threadIdx.x;
+    if (i >= n)
+        return;
+    std::size_t const q = order ? order[i] : i;
+    float const x = qx[q], y = qy[q], z = qz[q];
+    std::uint32_t stack[StackCap];
+    int sp = 0;
+    float omega = 0.0f;
+    double near = 0.0;
+    std::uint32_t n_tests = 0, n_accept = 0, n_leaf = 0;
+    std::uint32_t node = root_node;
+    bool overflow = false;
+    for (;;) {
+        float4 const *rec = records + static_cast<std::size_t>(node) * RV;
+        float4 const refs = rec[0];
+        float4 const mp = rec[1], px = rec[2], py = rec[3], pz = rec[4];
+        float const *fields = reinterpret_cast<float const *>(rec + 1);
+        std::uint32_t const cref[4] = {__float_as_uint(refs.x), __float_as_uint(refs.y),
+                                       __float_as_uint(refs.z), __float_as_uint(refs.w)};
+        float const mpd[4] = {mp.x, mp.y, mp.z, mp.w};
+        float const cx[4] = {px.x, px.y, px.z, px.w};
+        float const cy[4] = {py.x, py.y, py.z, py.w};
+        float const cz[4] = {pz.x, pz.y, pz.z, pz.w};
+        std::uint32_t next = k_invalid;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            std::uint32_t const c = cref[s];
+            if (c == k_invalid)
+                continue;
+            ++n_tests;
+            float const qrx = x - cx[s], qry = y - cy[s], qrz = z - cz[s];
+            float const ql2 = qrx * qrx + qry * qry + qrz * qrz;
+            bool const descend = !(ql2 > 0.0f) || ql2 <= mpd[s] * beta2;
+            if (!descend) {
+                float const w = taylor_term<Order>(fields, s, qrx, qry, qrz, ql2);
+                if (isfinite(w)) {
+                    omega += w;
+                    ++n_accept;
+                    continue;
+                }
+            }
+            if (c & k_cref_leaf) {
+                tri_record const t = tris[c & ~k_cref_leaf];
+                ++n_leaf;
+                if constexpr (Prec == GWN_PRECISION_PLAIN)
+                    omega += solid_angle_plain(t, x, y, z);
+                else
+                    near += solid_angle_f64(t, x, y, z);
+                continue;
+            }
+            if (next != k_invalid) {
+                if (sp >= StackCap) {
+                    overflow = true;
+                    break;
+                }
+                stack[sp++] = next;
+            }
+            next = c;
+        }
+        if (overflow)
+            break;
+        if (next != k_invalid) {
+            node = next;
+            continue;
+        }
+        if (sp == 0)
+            break;
+        node = stack[--sp];
+    }
+    float result;
+    if (overflow)
+        result = __builtin_nanf("");
+    else if constexpr (Prec == GWN_PRECISION_PLAIN)
+        result = omega / (4.0f * k_pi_f);
+    else
+        result = static_cast<float>((double(omega) + near) / (4.0 * k_pi_d));
+    out[q] = result;
+    if (counters != nullptr) {
+        counters[3 * q + 0] = n_tests;
+        counters[3 * q + 1] = n_accept;
+        counters[3 * q + 2] = n_leaf;
+    }
+}
+
+template <int Prec>
+__global__ __launch_bounds__(256) void leaf_root_kernel(
+    tri_record const *__restrict__ tris, std::uint64_t offset, std::uint32_t count,
+    float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
+    float *__restrict__ out, std::size_t n, std::uint32_t *__restrict__ counters
+) {
+    std::size_t const q = static_cast<std::size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (q >= n)
+        return;
+    float const x = qx[q], y = qy[q], z = qz[q];
+    float omega = 0.0f;
+    double near = 0.0;
+    for (std::uint32_t k = 0; k < count; ++k) {
+        if constexpr (Prec == GWN_PRECISION_PLAIN)
+            omega += solid_angle_plain(tris[offset + k], x, y, z);
+        else
+            near += solid_angle_f64(tris[offset + k], x, y, z);
+    }
+    if constexpr (Prec == GWN_PRECISION_PLAIN)
+        out[q] = omega / (4.0f * k_pi_f);
+    else
+        out[q] = static_cast<float>(near / (4.0 * k_pi_d));
+    if (counters != nullptr) {
+        counters[3 * q + 0] = 0;
+        counters[3 * q + 1] = 0;
+        counters[3 * q + 2] = count;
+    }
+}
+
+template <int Order, int Prec, int Cap>
+void launch_taylor_cap(
+    gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy,
+    float const *qz, std::uint32_t const *order, float *out, std::size_t n, float beta2,
+    std::uint32_t *counters, hipStream_t stream
+) {
+    taylor_kernel<Order, Prec, Cap><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(
+        m.records.ptr, bvh.triangles.ptr, static_cast<std::uint32_t>(ref_offset(bvh.root.reference)), qx, qy,
+        qz, order, out, n, beta2, counters
+    );
+    GWN_LAUNCH_CHECK();
+}
+
+template <int Order, int Prec>
+void launch_taylor(
+    gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy,
+    float const *qz, std::uint32_t const *order, float *out, std::size_t n, float beta2, int cap,
+    std::uint32_t *counters, hipStream_t stream
+) {
+    // The stack only ever holds internal_stack_bound entries; the smallest compiled capacity that
+    // covers both the bound and nothing more keeps the per-lane footprint small.
+    int const need = static_cast<int>(bvh.internal_stack_bound);
+    (void)cap;
+    if (need <= 32)
+        launch_taylor_cap<Order, Prec, 32>(bvh, m, qx, qy, qz, order, out, n, beta2, counters, stream);
+    else if (need <= 48)
+        launch_taylor_cap<Order, Prec, 48>(bvh, m, qx, qy, qz, order, out, n, beta2, counters, stream);
+    else if (need <= 64)
+        launch_taylor_cap<Order, Prec, 64>(bvh, m, qx, qy, qz, order, out, n, beta2, counters, stream);
+    else if (need <= 96)
+        launch_taylor_cap<Order, Prec, 96>(bvh, m, qx, qy, qz, order, out, n, beta2, counters, stream);
+    else if (need <= 128)
+        launch_taylor_cap<Order, Prec, 128>(bvh, m, qx, qy, qz, order, out, n, beta2, counters, stream);
+    else
+        throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
+}
+
+} // namespace
+
+void winding_exact(
+    gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, void *out,
+    std::size_t n, int precision, query_launch const &, hipStream_t stream
+) {
+    unsigned const grid = grid_for(n, k_exact_block);
+    switch (precision) {
+    case -1: // reference-form float output
+        exact_kernel<GWN_PRECISION_PLAIN, float><<<grid, k_exact_block, 0, stream>>>(
+            bvh.triangles.ptr, bvh.triangle_count, qx, qy, qz, static_cast<float *>(out), n);
+        break;
+    case GWN_PRECISION_PLAIN:
+        exact_kernel<GWN_PRECISION_PLAIN, double><<<grid, k_exact_block, 0, stream>>>(
+            bvh.triangles.ptr, bvh.triangle_count, qx, qy, qz, static_cast<double *>(out), n);
+        break;
+    case GWN_PRECISION_SPLIT:
+        exact_kernel<GWN_PRECISION_SPLIT, double><<<grid, k_exact_block, 0, stream>>>(
+            bvh.triangles.ptr, bvh.triangle_count, qx, qy, qz, static_cast<double *>(out), n);
+        break;
+    case GWN_PRECISION_FP64:
+        exact_kernel<GWN_PRECISION_FP64, double><<<grid, k_exact_block, 0, stream>>>(
+            bvh.triangles.ptr, bvh.triangle_count, qx, qy, qz, static_cast<double *>(out), n);
+        break;
+    default: throw_invalid("Unknown winding precision mode.");
+    }
+    GWN_LAUNCH_CHECK();
+}
+
+void winding_taylor(
+    gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy,
+    float const *qz, float *out, std::size_t n, float accuracy_scale, int precision,
+    query_launch const &cfg, std::uint32_t *counters, hipStream_t stream
+) {
+    if (n == 0)
+        return;
+    if (precision != GWN_PRECISION_PLAIN && precision != GWN_PRECISION_SPLIT)
+        throw_invalid("Taylor winding supports the plain and split precision modes.");
+    if (ref_leaf(bvh.root.reference)) {
+        unsigned const grid = grid_for(n, 256);
+        if (precision == GWN_PRECISION_PLAIN)
+            leaf_root_kernel<GWN_PRECISION_PLAIN><<<grid, 256, 0, stream>>>(
+                bvh.triangles.ptr, ref_offset(bvh.root.reference), ref_count(bvh.root.reference), qx, qy, qz, out, n, counters);
+        else
+            leaf_root_kernel<GWN_PRECISION_SPLIT><<<grid, 256, 0, stream>>>(
+                bvh.triangles.ptr, ref_offset(bvh.root.reference), ref_count(bvh.root.reference), qx, qy, qz, out, n, counters);
+        GWN_LAUNCH_CHECK();
+        return;
+    }
+    bool const sort = cfg.sort_queries > 0 || (cfg.sort_queries < 0 && n >= (1u << 16));
+    device_buffer<std::uint32_t> keys, keys_alt, ids, ids_alt;
+    std::uint32_t *order = nullptr;
+    if (sort) {
+        keys.allocate(n, stream);
+        keys_alt.allocate(n, stream);
+        ids.allocate(n, stream);
+        ids_alt.allocate(n, stream);
+        query_morton_kernel<<<grid_for(n, 256), 256, 0, stream>>>(qx, qy, qz, n, bvh.root.bounds, keys.ptr, ids.ptr);
+        GWN_LAUNCH_CHECK();
+        std::uint32_t *sk = nullptr;
+        radix_sort_pairs_u32(keys.ptr, keys_alt.ptr, ids.ptr, ids_alt.ptr, n, 0, 30, &sk, &order, stream);
+    }
+    float const beta2 = accuracy_scale * accuracy_scale;
+    int const cap = cfg.stack_capacity;
+#define GWN_TAYLOR_CASE(O, P) launch_taylor<O, P>(bvh, m, qx, qy, qz, order, out, n, beta2, cap, counters, stream)
+    if (precision == GWN_PRECISION_PLAIN) {
+        switch (m.order) {
+        case 0: GWN_TAYLOR_CASE(0, GWN_PRECISION_PLAIN); break;
+        case 1: GWN_TAYLOR_CASE(1, GWN_PRECISION_PLAIN); break;
+        default: GWN_TAYLOR_CASE(2, GWN_PRECISION_PLAIN); break;
+        }
+    } else {
+        switch (m.order) {
+        case 0: GWN_TAYLOR_CASE(0, GWN_PRECISION_SPLIT); break;
+        case 1: GWN_TAYLOR_CASE(1, GWN_PRECISION_SPLIT); break;
+        default: GWN_TAYLOR_CASE(2, GWN_PRECISION_SPLIT); break;
+        }
+    }
+#undef GWN_TAYLOR_CASE
+}
+
+} // namespace gwn

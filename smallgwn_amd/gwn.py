@@ -1,0 +1,424 @@
+"""Python mirror of the reference's gwn_* API over libgwn_hip.so (ctypes, C ABI).
+
+Names, argument meaning and error behaviour follow the reference's header-only C++ API
+(kririae/smallgwn include/gwn/*.cuh): objects own device storage, functions return a
+``Status`` instead of raising (``gwn_status``), batch queries take device arrays and
+enqueue on a stream without synchronizing.  Device arrays are torch tensors on a
+ROCm device (any object exposing ``data_ptr()`` works); streams are torch streams or raw
+``hipStream_t`` integers.
+
+The HIP library must be built (``__graft_entry__.build()``); importing this module
+without it raises ImportError -- there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgwn_hip.so")
+
+DEFAULT_BLOCK_SIZE = 256          # include/gwn/gwn_query.cuh:42
+DEFAULT_STACK_CAPACITY = 64       # include/gwn/gwn_query.cuh:45
+
+EXPORTED_SYMBOLS = (
+    "gwn_abi_version", "gwn_last_error_message",
+    "gwn_geometry_create", "gwn_geometry_destroy", "gwn_upload_geometry_f32",
+    "gwn_update_geometry_f32", "gwn_update_geometry_device_f32", "gwn_geometry_counts",
+    "gwn_bvh_create", "gwn_bvh_destroy", "gwn_build_bvh4_f32", "gwn_bvh_get_info", "gwn_bvh_download",
+    "gwn_moment_create", "gwn_moment_destroy", "gwn_refit_bvh_moment_f32", "gwn_moment_download",
+    "gwn_moment_order",
+    "gwn_compute_winding_number_exact_batch_f32", "gwn_compute_winding_number_taylor_batch_f32",
+    "gwn_compute_winding_number_exact_batch_ex_f32", "gwn_compute_winding_number_taylor_batch_ex_f32",
+    "gwn_radix_sort_pairs_u64", "gwn_radix_sort_pairs_u32",
+)
+
+
+class Error(enum.IntEnum):
+    """gwn_error (include/gwn/gwn_utils.cuh:213-218); cuda_runtime_error maps to the HIP runtime."""
+
+    SUCCESS = 0
+    INVALID_ARGUMENT = 1
+    HIP_RUNTIME_ERROR = 2
+    INTERNAL_ERROR = 3
+
+
+class BuildMethod(enum.IntEnum):
+    """gwn_bvh_build_method (include/gwn/gwn_bvh_build.cuh:16-20)."""
+
+    HPLOC = 0
+    LBVH = 1
+
+
+class Precision(enum.IntEnum):
+    PLAIN = 0   # the reference's float arithmetic
+    SPLIT = 1   # FP32 traversal / far field, FP64 near-field triangle terms
+    FP64 = 2    # every term in FP64 (exact batch only)
+
+
+@dataclass(frozen=True)
+class Status:
+    """gwn_status: an error category and a diagnostic message."""
+
+    error: Error = Error.SUCCESS
+    message: str = ""
+
+    def is_ok(self) -> bool:
+        return self.error == Error.SUCCESS
+
+    def raise_if_error(self) -> "Status":
+        if not self.is_ok():
+            raise GwnError(self)
+        return self
+
+
+class GwnError(RuntimeError):
+    def __init__(self, status: Status):
+        super().__init__(f"{status.error.name}: {status.message}")
+        self.status = status
+
+
+@dataclass(frozen=True)
+class BuildOptions:
+    """gwn_bvh_build_options (include/gwn/gwn_bvh_build.cuh:22-27)."""
+
+    method: BuildMethod = BuildMethod.HPLOC
+    hploc_search_radius: int = 8
+
+
+@dataclass(frozen=True)
+class QueryConfig:
+    """gwn_query_batch_config (include/gwn/gwn_query.cuh:48-52) plus the query presort switch."""
+
+    block_size: int = DEFAULT_BLOCK_SIZE
+    stack_capacity: int = DEFAULT_STACK_CAPACITY
+    sort_queries: int = -1
+
+
+class _CConfig(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_int32), ("stack_capacity", ctypes.c_int32),
+                ("sort_queries", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class _CBvhInfo(ctypes.Structure):
+    _fields_ = [
+        ("node_count", ctypes.c_uint64), ("triangle_count", ctypes.c_uint64), ("revision", ctypes.c_uint64),
+        ("internal_stack_bound", ctypes.c_uint32), ("packed_stack_bound", ctypes.c_uint32),
+        ("max_depth", ctypes.c_uint32), ("width", ctypes.c_uint32),
+        ("root_bounds", ctypes.c_float * 6), ("root_reference", ctypes.c_uint64),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)"
+        )
+    # torch first, so the library binds to the HIP runtime torch already loaded.
+    import torch  # noqa: F401
+
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u64, i32, u32, f32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_uint32, ctypes.c_float
+    sigs = {
+        "gwn_abi_version": [],
+        "gwn_geometry_create": [vp],
+        "gwn_geometry_destroy": [vp],
+        "gwn_upload_geometry_f32": [vp, vp, vp, vp, u64, vp, vp, vp, u64, vp],
+        "gwn_update_geometry_f32": [vp, vp, vp, vp, u64, vp],
+        "gwn_update_geometry_device_f32": [vp, vp, vp, vp, u64, vp],
+        "gwn_geometry_counts": [vp, vp, vp],
+        "gwn_bvh_create": [vp],
+        "gwn_bvh_destroy": [vp],
+        "gwn_build_bvh4_f32": [vp, vp, i32, u32, vp],
+        "gwn_bvh_get_info": [vp, vp],
+        "gwn_bvh_download": [vp, vp, vp, vp, vp],
+        "gwn_moment_create": [vp, i32],
+        "gwn_moment_destroy": [vp],
+        "gwn_refit_bvh_moment_f32": [vp, vp, vp],
+        "gwn_moment_download": [vp, vp, vp],
+        "gwn_moment_order": [vp, vp],
+        "gwn_compute_winding_number_exact_batch_f32": [vp, vp, vp, vp, vp, u64, vp, vp],
+        "gwn_compute_winding_number_taylor_batch_f32": [vp, vp, vp, vp, vp, vp, u64, f32, vp, vp],
+        "gwn_compute_winding_number_exact_batch_ex_f32": [vp, vp, vp, vp, vp, u64, i32, vp, vp],
+        "gwn_compute_winding_number_taylor_batch_ex_f32": [vp, vp, vp, vp, vp, vp, u64, f32, i32, vp, vp, vp],
+        "gwn_radix_sort_pairs_u64": [vp, vp, u64, i32, i32, vp],
+        "gwn_radix_sort_pairs_u32": [vp, vp, u64, i32, i32, vp],
+    }
+    for name, args in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    L.gwn_last_error_message.argtypes = []
+    L.gwn_last_error_message.restype = ctypes.c_char_p
+    return L
+
+
+_lib = _load()
+
+
+def library() -> ctypes.CDLL:
+    return _lib
+
+
+def _status(code: int) -> Status:
+    if code == 0:
+        return Status()
+    msg = _lib.gwn_last_error_message()
+    return Status(Error(code), msg.decode() if msg else "")
+
+
+def _stream(stream) -> int | None:
+    if stream is None:
+        import torch
+
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def _dptr(t) -> int | None:
+    if t is None:
+        return None
+    return int(t.data_ptr())
+
+
+def _config(cfg: QueryConfig | None) -> _CConfig:
+    cfg = cfg or QueryConfig()
+    return _CConfig(cfg.block_size, cfg.stack_capacity, cfg.sort_queries, 0)
+
+
+class _Handle:
+    _create = None
+    _destroy = None
+
+    def __init__(self, *args):
+        h = ctypes.c_void_p()
+        _status(getattr(_lib, self._create)(ctypes.byref(h), *args)).raise_if_error()
+        self._h = h
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            getattr(_lib, self._destroy)(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GeometryObject(_Handle):
+    """gwn_geometry_object<float, uint32_t> (include/gwn/gwn_geometry.cuh:97-149)."""
+
+    _create = "gwn_geometry_create"
+    _destroy = "gwn_geometry_destroy"
+
+    def counts(self) -> tuple[int, int]:
+        v, t = ctypes.c_uint64(), ctypes.c_uint64()
+        _status(_lib.gwn_geometry_counts(self._h, ctypes.byref(v), ctypes.byref(t))).raise_if_error()
+        return int(v.value), int(t.value)
+
+    def vertex_count(self) -> int:
+        return self.counts()[0]
+
+    def triangle_count(self) -> int:
+        return self.counts()[1]
+
+    def has_data(self) -> bool:
+        v, t = self.counts()
+        return v != 0 or t != 0
+
+
+class BvhObject(_Handle):
+    """gwn_bvh_object<4, float, uint32_t> (include/gwn/gwn_bvh.cuh:331-380)."""
+
+    _create = "gwn_bvh_create"
+    _destroy = "gwn_bvh_destroy"
+
+    def info(self) -> dict:
+        info = _CBvhInfo()
+        _status(_lib.gwn_bvh_get_info(self._h, ctypes.byref(info))).raise_if_error()
+        return {
+            "node_count": int(info.node_count),
+            "triangle_count": int(info.triangle_count),
+            "revision": int(info.revision),
+            "internal_stack_bound": int(info.internal_stack_bound),
+            "packed_stack_bound": int(info.packed_stack_bound),
+            "max_depth": int(info.max_depth),
+            "width": int(info.width),
+            "root_bounds": np.array(list(info.root_bounds), dtype=np.float32),
+            "root_reference": int(info.root_reference),
+        }
+
+    def has_data(self) -> bool:
+        return self.info()["revision"] != 0
+
+    def download(self, stream=None):
+        """Host copies: nodes (reference gwn_bvh_node<4,float> bytes), primitive order, records."""
+        inf = self.info()
+        n, t = inf["node_count"], inf["triangle_count"]
+        nodes = np.zeros(n * 128, dtype=np.uint8)
+        prim = np.zeros(t, dtype=np.uint32)
+        tris = np.zeros((t, 9), dtype=np.float32)
+        _status(_lib.gwn_bvh_download(
+            self._h, nodes.ctypes.data if n else None, prim.ctypes.data if t else None,
+            tris.ctypes.data if t else None, _stream(stream),
+        )).raise_if_error()
+        return nodes, prim, tris
+
+
+class MomentObject(_Handle):
+    """gwn_bvh_moment_object<4, Order, float, uint32_t> (include/gwn/gwn_bvh.cuh:383-433)."""
+
+    _create = "gwn_moment_create"
+    _destroy = "gwn_moment_destroy"
+
+    def __init__(self, order: int):
+        super().__init__(int(order))
+        self.order = int(order)
+
+    def download(self, node_count: int, stream=None) -> np.ndarray:
+        """Coefficients in the reference field-SoA layout: array[node, field, slot]."""
+        F = {0: 7, 1: 13, 2: 23}[self.order]
+        out = np.zeros((node_count, F, 4), dtype=np.float32)
+        if node_count:
+            _status(_lib.gwn_moment_download(self._h, out.ctypes.data, _stream(stream))).raise_if_error()
+        return out
+
+
+# ---------------------------------------------------------------------------------------------
+# free functions with the reference's names
+# ---------------------------------------------------------------------------------------------
+
+def _host(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def gwn_upload_geometry(geometry: GeometryObject, x, y, z, i0, i1, i2, stream=None) -> Status:
+    """gwn_upload_geometry (include/gwn/gwn_geometry.cuh:66-71): host SoA arrays."""
+    x, y, z = _host(x, np.float32), _host(y, np.float32), _host(z, np.float32)
+    i0, i1, i2 = _host(i0, np.uint32), _host(i1, np.uint32), _host(i2, np.uint32)
+    if not (x.shape == y.shape == z.shape):
+        return Status(Error.INVALID_ARGUMENT, "Vertex SoA spans must have identical lengths.")
+    if not (i0.shape == i1.shape == i2.shape):
+        return Status(Error.INVALID_ARGUMENT, "Triangle SoA spans must have identical lengths.")
+    nv, nt = x.shape[0], i0.shape[0]
+    return _status(_lib.gwn_upload_geometry_f32(
+        geometry.handle, x.ctypes.data if nv else None, y.ctypes.data if nv else None,
+        z.ctypes.data if nv else None, nv, i0.ctypes.data if nt else None,
+        i1.ctypes.data if nt else None, i2.ctypes.data if nt else None, nt, _stream(stream),
+    ))
+
+
+def gwn_update_geometry(geometry: GeometryObject, x, y, z, stream=None) -> Status:
+    """gwn_update_geometry (include/gwn/gwn_geometry.cuh:84-90); device tensors are accepted too."""
+    if hasattr(x, "data_ptr") and getattr(x, "is_cuda", False):
+        if not (x.numel() == y.numel() == z.numel()):
+            return Status(Error.INVALID_ARGUMENT, "Vertex SoA spans must have identical lengths.")
+        return _status(_lib.gwn_update_geometry_device_f32(
+            geometry.handle, _dptr(x), _dptr(y), _dptr(z), x.numel(), _stream(stream)))
+    x, y, z = _host(x, np.float32), _host(y, np.float32), _host(z, np.float32)
+    if not (x.shape == y.shape == z.shape):
+        return Status(Error.INVALID_ARGUMENT, "Vertex SoA spans must have identical lengths.")
+    n = x.shape[0]
+    return _status(_lib.gwn_update_geometry_f32(
+        geometry.handle, x.ctypes.data if n else None, y.ctypes.data if n else None,
+        z.ctypes.data if n else None, n, _stream(stream)))
+
+
+def gwn_build_bvh(geometry: GeometryObject, bvh: BvhObject, options: BuildOptions = BuildOptions(),
+                  stream=None) -> Status:
+    """gwn_build_bvh<4, float, uint32_t> (include/gwn/gwn_bvh_build.cuh:34-39)."""
+    return _status(_lib.gwn_build_bvh4_f32(
+        geometry.handle, bvh.handle, int(options.method), int(options.hploc_search_radius), _stream(stream)))
+
+
+def gwn_refit_bvh_moment(bvh: BvhObject, moment: MomentObject, stream=None) -> Status:
+    """gwn_refit_bvh_moment<Order> (include/gwn/gwn_bvh_refit.cuh:36-51)."""
+    return _status(_lib.gwn_refit_bvh_moment_f32(bvh.handle, moment.handle, _stream(stream)))
+
+
+def _numel(t) -> int:
+    return 0 if t is None else int(t.numel())
+
+
+def _check_soa(qx, qy, qz, outs) -> Status | None:
+    n = _numel(qx)
+    if _numel(qy) != n or _numel(qz) != n:
+        return Status(Error.INVALID_ARGUMENT, "Query SoA spans must have identical lengths.")
+    for o in outs:
+        if _numel(o) != n:
+            return Status(Error.INVALID_ARGUMENT, "Output span size must match query count.")
+    return None
+
+
+def gwn_compute_winding_number_exact_batch(bvh: BvhObject, qx, qy, qz, out,
+                                           config: QueryConfig | None = None, stream=None) -> Status:
+    """gwn_compute_winding_number_exact_batch (include/gwn/gwn_query.cuh:308-346)."""
+    bad = _check_soa(qx, qy, qz, [out])
+    if bad:
+        return bad
+    cfg = _config(config)
+    return _status(_lib.gwn_compute_winding_number_exact_batch_f32(
+        bvh.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx), ctypes.byref(cfg), _stream(stream)))
+
+
+def gwn_compute_winding_number_exact_batch_ex(bvh: BvhObject, qx, qy, qz, out_f64, precision: Precision,
+                                              config: QueryConfig | None = None, stream=None) -> Status:
+    """Exact batch with a precision mode; ``out_f64`` is a float64 device array."""
+    bad = _check_soa(qx, qy, qz, [out_f64])
+    if bad:
+        return bad
+    cfg = _config(config)
+    return _status(_lib.gwn_compute_winding_number_exact_batch_ex_f32(
+        bvh.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out_f64), _numel(qx), int(precision),
+        ctypes.byref(cfg), _stream(stream)))
+
+
+def gwn_compute_winding_number_taylor_batch(bvh: BvhObject, moment: MomentObject, qx, qy, qz, out,
+                                            accuracy_scale: float = 2.0, config: QueryConfig | None = None,
+                                            stream=None, precision: Precision = Precision.PLAIN,
+                                            counters=None) -> Status:
+    """gwn_compute_winding_number_taylor_batch<Order> (include/gwn/gwn_query.cuh:384-445).
+
+    ``precision=Precision.SPLIT`` evaluates near-field triangles in FP64.  ``counters`` (optional
+    uint32 device array of 3 * count) receives per-query child tests, accepted expansions and
+    near-field triangle evaluations.
+    """
+    bad = _check_soa(qx, qy, qz, [out])
+    if bad:
+        return bad
+    cfg = _config(config)
+    if precision == Precision.PLAIN and counters is None:
+        return _status(_lib.gwn_compute_winding_number_taylor_batch_f32(
+            bvh.handle, moment.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx),
+            float(accuracy_scale), ctypes.byref(cfg), _stream(stream)))
+    return _status(_lib.gwn_compute_winding_number_taylor_batch_ex_f32(
+        bvh.handle, moment.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx),
+        float(accuracy_scale), int(precision), ctypes.byref(cfg), _dptr(counters), _stream(stream)))
+
+
+def radix_sort_pairs(keys, values, begin_bit: int = 0, end_bit: int | None = None, stream=None) -> Status:
+    """Stable device sort of (uint64 or uint32 key, uint32 value) pairs in place."""
+    import torch
+
+    n = _numel(keys)
+    if _numel(values) != n:
+        return Status(Error.INVALID_ARGUMENT, "Key and value spans must have identical lengths.")
+    if keys.dtype in (torch.int64, torch.uint64):
+        fn, width = _lib.gwn_radix_sort_pairs_u64, 64
+    else:
+        fn, width = _lib.gwn_radix_sort_pairs_u32, 32
+    end_bit = width if end_bit is None else end_bit
+    return _status(fn(_dptr(keys), _dptr(values), n, int(begin_bit), int(end_bit), _stream(stream)))

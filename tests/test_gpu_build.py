@@ -1,0 +1,158 @@
+"""GPU parity of the build path: onesweep sort, Morton/LBVH/BVH4 build and Taylor-moment refit.
+
+Integer and index results (sorted order, node arrays, primitive order, records, stack bounds)
+must be bit-identical to the oracle; moments too, because both sides evaluate the same unfused
+IEEE float sequence.
+"""
+
+import numpy as np
+import pytest
+
+from smallgwn_amd import synth
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 100, 4095, 4096, 4097, 100_003, 1 << 20])
+def test_radix_sort_u64_stable(gwn, dev, n):
+    torch = _torch()
+    rng = np.random.default_rng(n)
+    keys = (rng.integers(0, 1 << 20, size=n, dtype=np.uint64) << np.uint64(40)) | rng.integers(0, 7, size=n, dtype=np.uint64)
+    vals = np.arange(n, dtype=np.uint32)
+    dk = torch.from_numpy(keys.view(np.int64)).to(dev)
+    dv = torch.from_numpy(vals.view(np.int32)).to(dev)
+    gwn.radix_sort_pairs(dk, dv, 0, 64).raise_if_error()
+    torch.cuda.synchronize()
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(dk.cpu().numpy().view(np.uint64), keys[order])
+    assert np.array_equal(dv.cpu().numpy().view(np.uint32), vals[order])
+
+
+@pytest.mark.parametrize("bits", [(0, 32), (8, 24), (0, 30)])
+def test_radix_sort_u32_bit_ranges(gwn, dev, bits):
+    torch = _torch()
+    rng = np.random.default_rng(bits[0] + bits[1])
+    n = 300_001
+    keys = rng.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    dk = torch.from_numpy(keys.view(np.int32)).to(dev)
+    dv = torch.from_numpy(vals.view(np.int32)).to(dev)
+    gwn.radix_sort_pairs(dk, dv, bits[0], bits[1]).raise_if_error()
+    torch.cuda.synchronize()
+    width = bits[1] - bits[0]
+    sub = (keys >> np.uint32(bits[0])) & np.uint32((1 << width) - 1 if width < 32 else 0xFFFFFFFF)
+    order = np.argsort(sub, kind="stable")
+    assert np.array_equal(dv.cpu().numpy().view(np.uint32), vals[order])
+    assert np.array_equal(dk.cpu().numpy().view(np.uint32), keys[order])
+
+
+def _meshes():
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=(3000, 3)).astype(np.float32)
+    soup = synth._mesh_from_arrays(v, rng.integers(0, 3000, size=(2000, 3)))
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=np.float32)
+    return {
+        "single": synth._mesh_from_arrays(tri, [[0, 1, 2]]),
+        "two": synth._mesh_from_arrays(
+            np.stack([[0, 1, 0, 4, 6, 4], [0, 0, 2, 0, 0, 3], [1, 1, 1, -2, -2, -2]], 1), [[0, 1, 2], [3, 4, 5]]),
+        "asym_octahedron": synth.asymmetric_octahedron(),
+        "coincident33": synth._mesh_from_arrays(tri, np.tile([[0, 1, 2]], (33, 1))),
+        "icosphere4": synth.icosphere(4),
+        "torus": synth.torus(64, 32),
+        "soup": soup,
+        "displaced_256x128": synth.displaced_sphere(256, 128),
+    }
+
+
+MESHES = _meshes()
+
+
+def build_gpu(gwn, mesh, method=None):
+    geom = gwn.GeometryObject()
+    gwn.gwn_upload_geometry(geom, mesh.vx, mesh.vy, mesh.vz, mesh.i0, mesh.i1, mesh.i2).raise_if_error()
+    bvh = gwn.BvhObject()
+    gwn.gwn_build_bvh(geom, bvh, gwn.BuildOptions(method=method or gwn.BuildMethod.LBVH)).raise_if_error()
+    return geom, bvh
+
+
+def assert_bvh_matches_oracle(gwn, orc, mesh, bvh):
+    ref = orc.build_bvh4(mesh)
+    info = bvh.info()
+    nodes, prim, tris = bvh.download()
+    assert info["node_count"] == ref.node_count
+    assert info["root_reference"] == ref.root_reference
+    assert np.array_equal(info["root_bounds"].view(np.uint32), ref.root_bounds.view(np.uint32))
+    assert info["internal_stack_bound"] == ref.internal_stack_bound
+    assert info["packed_stack_bound"] == ref.packed_stack_bound
+    assert np.array_equal(prim, ref.primitive_indices)
+    assert np.array_equal(tris.view(np.uint32), ref.triangles.view(np.uint32))
+    assert nodes.tobytes() == ref.nodes.tobytes()
+    return ref
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_build_matches_oracle(gwn, orc, name):
+    mesh = MESHES[name]
+    _, bvh = build_gpu(gwn, mesh)
+    ref = assert_bvh_matches_oracle(gwn, orc, mesh, bvh)
+    nodes, prim, tris = bvh.download()
+    helpers.check_bvh4(nodes, bvh.info()["root_reference"], prim, tris, mesh,
+                       ref.internal_stack_bound, ref.packed_stack_bound)
+
+
+def test_build_large_mesh_matches_oracle(gwn, orc):
+    """The 1,048,576-triangle displaced sphere: node arrays bit-identical at full size."""
+    mesh = synth.config_large_mesh()
+    _, bvh = build_gpu(gwn, mesh)
+    assert_bvh_matches_oracle(gwn, orc, mesh, bvh)
+
+
+def test_rebuild_is_deterministic_and_revisions_advance(gwn):
+    mesh = MESHES["torus"]
+    geom, bvh = build_gpu(gwn, mesh)
+    a = bvh.download()
+    r1 = bvh.info()["revision"]
+    gwn.gwn_build_bvh(geom, bvh, gwn.BuildOptions(method=gwn.BuildMethod.LBVH)).raise_if_error()
+    b = bvh.download()
+    assert bvh.info()["revision"] > r1
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def test_empty_geometry_builds_empty_bvh(gwn):
+    geom = gwn.GeometryObject()
+    gwn.gwn_upload_geometry(geom, [], [], [], [], [], []).raise_if_error()
+    bvh = gwn.BvhObject()
+    gwn.gwn_build_bvh(geom, bvh, gwn.BuildOptions(method=gwn.BuildMethod.LBVH)).raise_if_error()
+    assert not bvh.has_data()
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("name", ["two", "asym_octahedron", "coincident33", "icosphere4", "soup", "displaced_256x128"])
+def test_moment_matches_oracle_bitwise(gwn, orc, name, order):
+    mesh = MESHES[name]
+    _, bvh = build_gpu(gwn, mesh)
+    mom = gwn.MomentObject(order)
+    gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+    n = bvh.info()["node_count"]
+    got = mom.download(n)
+    ref = orc.refit_moment(orc.build_bvh4(mesh), order)
+    assert got.shape == ref.shape
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_moment_large_mesh_matches_oracle_bitwise(gwn, orc):
+    mesh = synth.config_large_mesh()
+    _, bvh = build_gpu(gwn, mesh)
+    ref_bvh = orc.build_bvh4(mesh)
+    for order in (1, 2):
+        mom = gwn.MomentObject(order)
+        gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+        got = mom.download(bvh.info()["node_count"])
+        assert np.array_equal(got.view(np.uint32), orc.refit_moment(ref_bvh, order).view(np.uint32))
