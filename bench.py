@@ -1,0 +1,272 @@
+"""Throughput of the geometric-query hot path on one MI355X.
+
+Workload (large single mesh): radially displaced sphere r = 1 + 0.05 * (8 random-phase real
+spherical harmonics of degree <= 12), 1024 x 512 latitude-longitude quads = 1,048,576
+triangles; 16,777,216 query points, half uniform in [-1.5, 1.5]^3 and half within 0.01 of the
+surface along the face normal.  Seeded synthetic data (splitmix64 from 0xB3A9E5D4).
+
+One step = one Taylor order-2 winding batch over all 16M queries (Morton presort + traversal),
+FP32 traversal and far field, FP64 near-field triangle terms, at an accuracy scale that keeps
+max |dW| <= 1e-4 against the FP64 exact winding number on a seeded query subsample.  Inputs,
+BVH and moments are resident in HBM before the timed region.
+
+Also reported (outside the timed steps): the exact winding batch with the same FP64 near field
+(the plain form the Taylor path is compared with), BVH build and moment refit throughput, the
+roofline of the traversal kernel from its own work counters, and the CPU oracle's replay of the
+same algorithm on a bounded query sample.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+if REPO not in sys.path:
+    sys.path.insert(0, REPO)
+
+METRIC = "Mqueries/s Taylor vs exact at |dW|<=1e-4 vs FP64; adjoint steps/s; build Mtri/s"
+
+# Algorithmic FLOP weights per unit of traversal work (special functions count 1 each).
+FLOPS_CHILD_TEST = 10
+FLOPS_ACCEPT = {0: 14, 1: 34, 2: 79}
+FLOPS_TRIANGLE = 87            # solid angle: 80 FLOP + 3 sqrt + 3 div + 1 atan2
+PEAK_FP32_TFLOPS = 157.3       # MI355X dense FP32 vector
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse_args():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--queries", type=int, default=16_777_216)
+    p.add_argument("--order", type=int, default=2)
+    p.add_argument("--beta", type=float, default=4.0, help="Taylor accuracy scale")
+    p.add_argument("--check-queries", type=int, default=32_768, help="FP64 subsample for |dW|")
+    p.add_argument("--exact-queries", type=int, default=32_768, help="subsample timed for the exact form")
+    p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline duration")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def main() -> int:
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (queries shard by "
+              "contiguous ranges with a replicated BVH, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from smallgwn_amd import gwn, synth
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+
+    t0 = time.time()
+    mesh = synth.config_large_mesh()
+    qx, qy, qz = synth.mixed_queries(mesh, args.queries, near_fraction=0.5, band=0.01)
+    log(f"inputs: {mesh.triangle_count} tris, {qx.shape[0]} queries ({time.time() - t0:.1f}s)")
+
+    geom = gwn.GeometryObject()
+    gwn.gwn_upload_geometry(geom, mesh.vx, mesh.vy, mesh.vz, mesh.i0, mesh.i1, mesh.i2).raise_if_error()
+    bvh = gwn.BvhObject()
+    opts = gwn.BuildOptions(method=gwn.BuildMethod.LBVH)
+    gwn.gwn_build_bvh(geom, bvh, opts).raise_if_error()
+    mom = gwn.MomentObject(args.order)
+    gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+    info = bvh.info()
+    d = [torch.from_numpy(a).to(dev) for a in (qx, qy, qz)]
+    n = d[0].numel()
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    order = torch.empty(n, dtype=torch.int32, device=dev)
+    cfg = gwn.QueryConfig(stack_capacity=max(64, info["internal_stack_bound"]))
+    torch.cuda.synchronize()
+
+    def step():
+        gwn.gwn_compute_winding_number_taylor_batch(
+            bvh, mom, *d, out, accuracy_scale=args.beta, config=cfg, precision=gwn.Precision.SPLIT,
+        ).raise_if_error()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(stream)
+    for _ in range(args.steps):
+        step()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    ms_per_step = ev[0].elapsed_time(ev[1]) / args.steps
+    value = n / (ms_per_step * 1e-3) / 1e6
+    log(f"taylor o{args.order} split beta={args.beta}: {ms_per_step:.3f} ms/step, {value:.1f} Mq/s")
+
+    # Stage split: presort vs traversal kernel, timed with events on the launch stream.
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    reps = max(3, args.steps)
+    t_order = t_trav = 0.0
+    for _ in range(reps):
+        e[0].record(stream)
+        gwn.gwn_compute_query_order(bvh, *d, order).raise_if_error()
+        e[1].record(stream)
+        gwn.gwn_compute_winding_number_taylor_batch(
+            bvh, mom, *d, out, accuracy_scale=args.beta, config=cfg, precision=gwn.Precision.SPLIT, order=order,
+        ).raise_if_error()
+        e[2].record(stream)
+        torch.cuda.synchronize()
+        t_order += e[0].elapsed_time(e[1]) / reps
+        t_trav += e[1].elapsed_time(e[2]) / reps
+
+    # Work counters of the traversal (one untimed pass) -> algorithmic FLOPs.
+    cnt = torch.zeros(3 * n, dtype=torch.int32, device=dev)
+    gwn.gwn_compute_winding_number_taylor_batch(
+        bvh, mom, *d, out, accuracy_scale=args.beta, config=cfg, precision=gwn.Precision.SPLIT,
+        order=order, counters=cnt,
+    ).raise_if_error()
+    c = cnt.view(n, 3).to(torch.int64).sum(0).cpu().numpy()
+    tests, accepts, leaves = (int(v) for v in c)
+    fp32_flops = tests * FLOPS_CHILD_TEST + accepts * FLOPS_ACCEPT[args.order]
+    fp64_flops = leaves * FLOPS_TRIANGLE
+    fp32_equiv = fp32_flops + 2 * fp64_flops  # FP64 vector rate is half the FP32 rate
+    achieved_tflops = fp32_equiv / (t_trav * 1e-3) / 1e12
+
+    # Accuracy against FP64 exact on a seeded, strided subsample (both query halves).
+    taylor_host = out.cpu().numpy()
+    m = min(args.check_queries, n)
+    idx = np.unique(np.linspace(0, n - 1, m).astype(np.int64))
+    sub = [t[torch.from_numpy(idx).to(dev)].contiguous() for t in d]
+    ref = torch.empty(idx.size, dtype=torch.float64, device=dev)
+    gwn.gwn_compute_winding_number_exact_batch_ex(bvh, *sub, ref, gwn.Precision.FP64).raise_if_error()
+    torch.cuda.synchronize()
+    err = np.abs(taylor_host[idx].astype(np.float64) - ref.cpu().numpy())
+    max_err, p99_err = float(err.max()), float(np.quantile(err, 0.99))
+    log(f"accuracy on {idx.size} queries: max |dW| {max_err:.3e}, p99 {p99_err:.3e}")
+
+    # The exact form with the same FP64 near field, on a subsample (cost is linear in queries).
+    me = min(args.exact_queries, n)
+    eidx = torch.from_numpy(np.linspace(0, n - 1, me).astype(np.int64)).to(dev)
+    esub = [t[eidx].contiguous() for t in d]
+    eout = torch.empty(me, dtype=torch.float64, device=dev)
+    gwn.gwn_compute_winding_number_exact_batch_ex(bvh, *esub, eout, gwn.Precision.SPLIT).raise_if_error()
+    torch.cuda.synchronize()
+    ee = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ee[0].record(stream)
+    gwn.gwn_compute_winding_number_exact_batch_ex(bvh, *esub, eout, gwn.Precision.SPLIT).raise_if_error()
+    ee[1].record(stream)
+    torch.cuda.synchronize()
+    exact_ms = ee[0].elapsed_time(ee[1])
+    exact_rate = me / (exact_ms * 1e-3) / 1e6
+    log(f"exact split: {exact_rate:.4f} Mq/s on {me} queries ({exact_ms:.1f} ms)")
+
+    # Build and refit throughput.
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    build_s = timed(lambda: gwn.gwn_build_bvh(geom, bvh, opts).raise_if_error(), 5)
+    mom = gwn.MomentObject(args.order)
+    refit_s = timed(lambda: gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error(), 5)
+    T = mesh.triangle_count
+    log(f"build {build_s * 1e3:.2f} ms ({T / build_s / 1e6:.1f} Mtri/s), refit o{args.order} {refit_s * 1e3:.2f} ms")
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mqueries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "taylor_winding_large_mesh",
+            "triangles": T,
+            "queries": n,
+            "near_surface_fraction": 0.5,
+            "taylor_order": args.order,
+            "accuracy_scale": args.beta,
+            "precision": "fp32 traversal/far field, fp64 near field",
+            "builder": "lbvh",
+            "bvh_width": 4,
+        },
+        "accuracy": {"max_abs_dw": max_err, "p99_abs_dw": p99_err, "checked_queries": int(idx.size),
+                     "target": 1e-4, "meets_target": bool(max_err <= 1e-4)},
+        "exact_fp64_near_field": {"mqueries_per_s": exact_rate, "sampled_queries": me,
+                                  "taylor_speedup": value / exact_rate},
+        "stages_ms": {"query_presort": round(t_order, 4), "traversal": round(t_trav, 4)},
+        "build": {"method": "lbvh", "ms": build_s * 1e3, "mtri_per_s": T / build_s / 1e6,
+                  "refit_moment_ms": refit_s * 1e3, "refit_mtri_per_s": T / refit_s / 1e6,
+                  "nodes": info["node_count"], "internal_stack_bound": info["internal_stack_bound"]},
+        "work": {"child_tests": tests, "accepted": accepts, "near_field_triangles": leaves},
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved_tflops, 3),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
+            "traffic": None,
+            "kernel": "taylor_kernel<2, split>",
+            "kernel_ms": round(t_trav, 4),
+            "flops_per_launch": fp32_equiv,
+            "note": "FP32-equivalent FLOPs (FP64 counted twice) from per-query work counters",
+        },
+    }
+
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(mesh, qx, qy, qz, args)
+
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def cpu_baseline(mesh, qx, qy, qz, args) -> dict:
+    """The oracle's C replay of the same split Taylor traversal, on a bounded query sample."""
+    from oracle import oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    bvh = oracle.build_bvh4(mesh)
+    mom = oracle.refit_moment(bvh, args.order)
+    rng_idx = np.linspace(0, qx.shape[0] - 1, 4096).astype(np.int64)
+
+    def run(idx):
+        t = time.perf_counter()
+        oracle.winding_taylor_f32(bvh, mom, args.order, qx[idx], qy[idx], qz[idx], accuracy_scale=args.beta,
+                                  stack_capacity=1024, threads=threads, near_fp64=True)
+        return time.perf_counter() - t
+
+    probe = run(rng_idx)
+    m = int(min(qx.shape[0], max(4096, 4096 * args.cpu_seconds / max(probe, 1e-3))))
+    idx = np.linspace(0, qx.shape[0] - 1, m).astype(np.int64)
+    sec = run(idx)
+    log(f"cpu baseline: {m} queries in {sec:.1f}s on {threads} threads")
+    return {
+        "value": round(m / sec / 1e6, 5),
+        "unit": "Mqueries/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{m} strided queries of the same workload, Taylor o{args.order} split, beta={args.beta}",
+    }
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -144,12 +144,21 @@ int gwn_compute_winding_number_taylor_batch_f32(
 int gwn_compute_winding_number_exact_batch_ex_f32(
     gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, double *out, uint64_t count,
     int32_t precision, const gwn_query_config *config, hipStream_t stream);
-/* counters (optional, device, 3 x uint32 per query): child tests, accepted expansions,
+/* order (optional, device, count entries): visit queries in this permutation (from
+ * gwn_compute_query_order_f32); when NULL the batch presorts by itself as config says.
+ * counters (optional, device, 3 x uint32 per query): child tests, accepted expansions,
  * near-field triangle evaluations -- the work counts used for the FLOP roofline. */
 int gwn_compute_winding_number_taylor_batch_ex_f32(
     gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz,
     float *out, uint64_t count, float accuracy_scale, int32_t precision,
-    const gwn_query_config *config, uint32_t *counters, hipStream_t stream);
+    const gwn_query_config *config, const uint32_t *order, uint32_t *counters, hipStream_t stream);
+
+/* Morton-order permutation of query points within the BVH root bounds (30-bit keys, stable
+ * onesweep sort): order[i] is the input index of the i-th query to visit.  Reusable across
+ * batches over the same points (no reference counterpart; an MI355X traversal aid). */
+int gwn_compute_query_order_f32(
+    gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, uint32_t *order,
+    uint64_t count, hipStream_t stream);
 
 /* ---- internal building block exported for tests: stable LSD radix sort of device
  *      (uint64 key, uint32 value) pairs on bits [begin_bit, end_bit) (replaces the CUB

@@ -605,17 +605,22 @@ typedef struct {
     const float *qx, *qy, *qz;
     float *out;
     uint64_t *counters;
+    int near_fp64;
 } taylor_ctx;
 
 static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint64_t *cnt)
 {
     const float k_pi = (float)3.141592653589793238462643383279502884L;
+    const double k_pi_d = 3.141592653589793238462643383279502884;
     float q[3] = {qx, qy, qz};
+    double qd[3] = {qx, qy, qz};
     float omega = 0.0f;
+    double near = 0.0;
     float beta2 = c->beta * c->beta;
     uint32_t stack[1024];
     int sp = 0;
     uint64_t tests = 0, accepted = 0, leaves = 0;
+    /* Precision split: leaves in FP64 on the exact vertices, accumulated in FP64. */
 #define LEAF(child_ptr)                                                                            \
     do {                                                                                           \
         uint64_t off_ = (child_ptr)->reference & GWN_ORACLE_OFFSET_MASK;                          \
@@ -623,16 +628,24 @@ static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint6
         for (uint32_t k_ = 0; k_ < n_; ++k_) {                                                    \
             if (off_ + k_ >= c->nt)                                                               \
                 continue;                                                                         \
-            float a_[3], b_[3], c_[3];                                                            \
-            record_vertices_f32(c->tri + 9 * (off_ + k_), a_, b_, c_);                            \
-            omega += solid_angle_f32(a_, b_, c_, q);                                              \
+            const float *r_ = c->tri + 9 * (off_ + k_);                                           \
+            if (c->near_fp64) {                                                                   \
+                double a_[3] = {r_[0], r_[1], r_[2]}, b_[3] = {r_[3], r_[4], r_[5]},              \
+                       c_[3] = {r_[6], r_[7], r_[8]};                                             \
+                near += solid_angle_f64(a_, b_, c_, qd);                                          \
+            } else {                                                                              \
+                float a_[3], b_[3], c_[3];                                                        \
+                record_vertices_f32(r_, a_, b_, c_);                                              \
+                omega += solid_angle_f32(a_, b_, c_, q);                                          \
+            }                                                                                     \
             ++leaves;                                                                             \
         }                                                                                         \
     } while (0)
+#define RESULT() (c->near_fp64 ? (float)(((double)omega + near) / (4.0 * k_pi_d)) : omega / (4.0f * k_pi))
     if ((c->root.reference & GWN_ORACLE_VALID_BIT) && (c->root.reference >> 48) != 0) {
         LEAF(&c->root);
         if (cnt) { cnt[0] = tests; cnt[1] = accepted; cnt[2] = leaves; }
-        return omega / (4.0f * k_pi);
+        return RESULT();
     }
     uint32_t node = (uint32_t)(c->root.reference & GWN_ORACLE_OFFSET_MASK);
     for (;;) {
@@ -715,7 +728,8 @@ static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint6
     }
 #undef LEAF
     if (cnt) { cnt[0] = tests; cnt[1] = accepted; cnt[2] = leaves; }
-    return omega / (4.0f * k_pi);
+    return RESULT();
+#undef RESULT
 }
 
 static void taylor_range(void *vctx, uint64_t begin, uint64_t end)
@@ -730,10 +744,11 @@ void gwn_oracle_winding_taylor_f32(
     const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
     const float *triangles, uint64_t triangle_count, const float *moments, int order,
     float accuracy_scale, int stack_capacity, const float *qx, const float *qy,
-    const float *qz, uint64_t query_count, float *out, uint64_t *counters, int threads)
+    const float *qz, uint64_t query_count, float *out, uint64_t *counters, int near_fp64,
+    int threads)
 {
     taylor_ctx ctx = {nodes, node_count, root, triangles, triangle_count, moments, order,
                       gwn_oracle_moment_field_count(order), stack_capacity, accuracy_scale,
-                      qx, qy, qz, out, counters};
+                      qx, qy, qz, out, counters, near_fp64};
     parallel_ranges(taylor_range, &ctx, query_count, threads);
 }

@@ -137,13 +137,15 @@ float gwn_oracle_solid_angle_f32(const float *a, const float *b, const float *c,
  * Taylor traversal replay (FP32, reference-form leaves).  counters (optional,
  * 3 x uint64 per query or NULL): child tests, accepted expansions, leaf
  * triangle evaluations.  Returns NaN for a query whose stack would exceed
- * stack_capacity.
+ * stack_capacity.  near_fp64 != 0 evaluates every leaf triangle in FP64 on the
+ * exact vertices and accumulates it in FP64 (the precision split).
  */
 void gwn_oracle_winding_taylor_f32(
     const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
     const float *triangles, uint64_t triangle_count, const float *moments, int order,
     float accuracy_scale, int stack_capacity, const float *qx, const float *qy,
-    const float *qz, uint64_t query_count, float *out, uint64_t *counters, int threads);
+    const float *qz, uint64_t query_count, float *out, uint64_t *counters, int near_fp64,
+    int threads);
 
 #ifdef __cplusplus
 }

@@ -86,7 +86,7 @@ def lib():
         L.gwn_oracle_triangle_payload_f64.argtypes = [vp, i32, vp]
         L.gwn_oracle_triangle_payload_f32.argtypes = [vp, i32, vp]
         L.gwn_oracle_winding_taylor_f32.argtypes = [
-            vp, u64, _Child, vp, u64, vp, i32, f32, i32, vp, vp, vp, u64, vp, vp, i32,
+            vp, u64, _Child, vp, u64, vp, i32, f32, i32, vp, vp, vp, u64, vp, vp, i32, i32,
         ]
         _lib = L
     return _lib
@@ -233,7 +233,8 @@ def triangle_payload(record, order: int, dtype=np.float64) -> np.ndarray:
 
 def winding_taylor_f32(bvh: OracleBvh, moments: np.ndarray, order: int, qx, qy, qz,
                        accuracy_scale: float = 2.0, stack_capacity: int = 64,
-                       counters: bool = False, threads: int | None = None):
+                       counters: bool = False, threads: int | None = None, near_fp64: bool = False):
+    """Replay of the Taylor traversal; ``near_fp64`` evaluates leaves in FP64 (precision split)."""
     nodes = _c(bvh.nodes, NODE_DTYPE)
     tris = _c(bvh.triangles, np.float32)
     mom = _c(moments, np.float32)
@@ -247,7 +248,7 @@ def winding_taylor_f32(bvh: OracleBvh, moments: np.ndarray, order: int, qx, qy, 
     lib().gwn_oracle_winding_taylor_f32(
         _ptr(nodes), bvh.node_count, root, _ptr(tris), tris.shape[0], _ptr(mom), order,
         accuracy_scale, stack_capacity, _ptr(qx), _ptr(qy), _ptr(qz), qx.shape[0], _ptr(out),
-        _ptr(cnt) if cnt is not None else None, threads or default_threads(),
+        _ptr(cnt) if cnt is not None else None, 1 if near_fp64 else 0, threads or default_threads(),
     )
     return (out, cnt) if counters else out
 

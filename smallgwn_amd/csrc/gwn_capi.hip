@@ -388,7 +388,7 @@ int gwn_compute_winding_number_exact_batch_ex_f32(
 static void taylor_common(
     gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz, float *out,
     uint64_t count, float accuracy_scale, int32_t precision, const gwn_query_config *config,
-    uint32_t *counters, hipStream_t stream
+    const uint32_t *order, uint32_t *counters, hipStream_t stream
 ) {
     gwn_query_config const c = resolve_config(config);
     validate_config(c, true);
@@ -402,7 +402,7 @@ static void taylor_common(
     require_query_storage(count, qx, qy, qz, out);
     if (count == 0)
         return;
-    gwn::winding_taylor(*bvh, *moment, qx, qy, qz, out, count, accuracy_scale, precision, to_launch(c), counters, stream);
+    gwn::winding_taylor(*bvh, *moment, qx, qy, qz, out, count, accuracy_scale, precision, to_launch(c), order, counters, stream);
 }
 
 int gwn_compute_winding_number_taylor_batch_f32(
@@ -410,17 +410,30 @@ int gwn_compute_winding_number_taylor_batch_f32(
     uint64_t count, float accuracy_scale, const gwn_query_config *config, hipStream_t stream
 ) {
     return guarded("gwn_compute_winding_number_taylor_batch", [&] {
-        taylor_common(bvh, moment, qx, qy, qz, out, count, accuracy_scale, GWN_PRECISION_PLAIN, config, nullptr, stream);
+        taylor_common(bvh, moment, qx, qy, qz, out, count, accuracy_scale, GWN_PRECISION_PLAIN, config, nullptr, nullptr, stream);
     });
 }
 
 int gwn_compute_winding_number_taylor_batch_ex_f32(
     gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz, float *out,
     uint64_t count, float accuracy_scale, int32_t precision, const gwn_query_config *config,
-    uint32_t *counters, hipStream_t stream
+    const uint32_t *order, uint32_t *counters, hipStream_t stream
 ) {
     return guarded("gwn_compute_winding_number_taylor_batch", [&] {
-        taylor_common(bvh, moment, qx, qy, qz, out, count, accuracy_scale, precision, config, counters, stream);
+        taylor_common(bvh, moment, qx, qy, qz, out, count, accuracy_scale, precision, config, order, counters, stream);
+    });
+}
+
+int gwn_compute_query_order_f32(
+    gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, uint32_t *order, uint64_t count,
+    hipStream_t stream
+) {
+    return guarded("gwn_compute_query_order", [&] {
+        require_bvh(bvh);
+        require_query_storage(count, qx, qy, qz, order);
+        if (count > 0x3fffffffull)
+            gwn::throw_invalid("Query count exceeds the ordering range (2^30 - 1).");
+        gwn::query_order(*bvh, qx, qy, qz, order, count, stream);
     });
 }
 

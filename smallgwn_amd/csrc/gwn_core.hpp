@@ -241,7 +241,12 @@ void winding_exact(
 void winding_taylor(
     gwn_bvh_handle const &bvh, gwn_moment_handle const &moment, float const *qx, float const *qy,
     float const *qz, float *out, std::size_t n, float accuracy_scale, int precision,
-    query_launch const &cfg, std::uint32_t *counters, hipStream_t stream
+    query_launch const &cfg, std::uint32_t const *order, std::uint32_t *counters, hipStream_t stream
+);
+// Morton-order visiting permutation of query points within the BVH root bounds.
+void query_order(
+    gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz,
+    std::uint32_t *order, std::size_t n, hipStream_t stream
 );
 
 // Zero-fill helper (a kernel, not hipMemsetAsync, so that captured graphs replay it safely).

@@ -37,7 +37,7 @@ __global__ void fill_u32_kernel(std::uint32_t *p, std::uint32_t v, std::size_t n
 
 template <class Key>
 __global__ __launch_bounds__(k_sort_block) void histogram_kernel(
-    Key const *__restrict__ keys, std::size_t n, int begin_bit, int passes,
+    Key const *__restrict__ keys, std::size_t n, int begin_bit, int end_bit, int passes,
     std::uint32_t *__restrict__ hist
 ) {
     __shared__ std::uint32_t s_hist[k_max_passes][k_radix];
@@ -49,7 +49,9 @@ __global__ __launch_bounds__(k_sort_block) void histogram_kernel(
     for (; i < n; i += stride) {
         Key const k = keys[i];
         for (int p = 0; p < passes; ++p) {
-            unsigned d = static_cast<unsigned>((k >> (begin_bit + 8 * p)) & Key(0xff));
+            int const sh = begin_bit + 8 * p;
+            int const width = end_bit - sh < 8 ? end_bit - sh : 8;
+            unsigned d = static_cast<unsigned>((k >> sh) & Key((1u << width) - 1u));
             atomicAdd(&s_hist[p][d], 1u);
         }
     }
@@ -112,8 +114,8 @@ template <class Key>
 __global__ __launch_bounds__(k_sort_block) void onesweep_kernel(
     Key const *__restrict__ keys_in, Key *__restrict__ keys_out,
     std::uint32_t const *__restrict__ vals_in, std::uint32_t *__restrict__ vals_out, std::size_t n,
-    int shift, std::uint32_t const *__restrict__ pass_offsets, std::uint32_t *tile_status,
-    std::uint32_t *tile_counter
+    int shift, unsigned digit_mask, std::uint32_t const *__restrict__ pass_offsets,
+    std::uint32_t *tile_status, std::uint32_t *tile_counter
 ) {
     __shared__ onesweep_smem<Key> s;
     int const lane = threadIdx.x & 63;
@@ -149,7 +151,7 @@ __global__ __launch_bounds__(k_sort_block) void onesweep_kernel(
     for (int it = 0; it < k_items; ++it) {
         int const local = wave_base + it * 64 + lane;
         bool const valid = local < tile_count;
-        unsigned const d = static_cast<unsigned>((key[it] >> shift) & Key(0xff));
+        unsigned const d = static_cast<unsigned>((key[it] >> shift) & Key(digit_mask));
         std::uint64_t match = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(k_sort_block) void onesweep_kernel(
     for (int it = 0; it < k_items; ++it) {
         int const local = wave_base + it * 64 + lane;
         if (local < tile_count) {
-            unsigned const d = static_cast<unsigned>((key[it] >> shift) & Key(0xff));
+            unsigned const d = static_cast<unsigned>((key[it] >> shift) & Key(digit_mask));
             std::uint32_t const pos = s.tile_start[d] + s.wave_count[wave][d] + rank[it];
             s.keys[pos] = key[it];
             s.vals[pos] = val[it];
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(k_sort_block) void onesweep_kernel(
     __syncthreads();
     for (int j = threadIdx.x; j < tile_count; j += k_sort_block) {
         Key const k = s.keys[j];
-        unsigned const d = static_cast<unsigned>((k >> shift) & Key(0xff));
+        unsigned const d = static_cast<unsigned>((k >> shift) & Key(digit_mask));
         std::size_t const dst = static_cast<std::size_t>(s.digit_base[d]) + static_cast<std::size_t>(j);
         keys_out[dst] = k;
         vals_out[dst] = s.vals[j];
@@ -259,7 +261,7 @@ void radix_sort_pairs(
     std::uint32_t *pass_state = scratch.ptr + static_cast<std::size_t>(passes) * k_radix;
 
     unsigned const hist_grid = static_cast<unsigned>(tiles < 2048 ? tiles : 2048);
-    histogram_kernel<Key><<<hist_grid, k_sort_block, 0, stream>>>(keys, n, begin_bit, passes, hist);
+    histogram_kernel<Key><<<hist_grid, k_sort_block, 0, stream>>>(keys, n, begin_bit, end_bit, passes, hist);
     GWN_LAUNCH_CHECK();
     scan_histogram_kernel<<<1, k_sort_block, 0, stream>>>(hist, passes);
     GWN_LAUNCH_CHECK();
@@ -268,8 +270,10 @@ void radix_sort_pairs(
     std::uint32_t *vin = vals, *vout = vals_alt;
     for (int p = 0; p < passes; ++p) {
         std::uint32_t *counter = pass_state + static_cast<std::size_t>(p) * (1 + status_words);
+        int const shift = begin_bit + 8 * p;
+        int const width = end_bit - shift < 8 ? end_bit - shift : 8;
         onesweep_kernel<Key><<<static_cast<unsigned>(tiles), k_sort_block, 0, stream>>>(
-            kin, kout, vin, vout, n, begin_bit + 8 * p, hist + p * k_radix, counter + 1, counter
+            kin, kout, vin, vout, n, shift, (1u << width) - 1u, hist + p * k_radix, counter + 1, counter
         );
         GWN_LAUNCH_CHECK();
         std::swap(kin, kout);

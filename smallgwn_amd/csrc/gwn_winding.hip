@@ -174,6 +174,18 @@ __global__ __launch_bounds__(256) void query_morton_kernel(
     ids[i] = static_cast<std::uint32_t>(i);
 }
 
+// Block-scoped contract(off): the IR operations carry no contraction flag, so they stay
+// separate multiplies and adds even after inlining into FMA-friendly code.
+__device__ inline float squared_norm_unfused(float x, float y, float z) {
+#pragma clang fp contract(off)
+    return x * x + y * y + z * z;
+}
+
+__device__ inline float product_unfused(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+
 template <int Order>
 __device__ inline float taylor_term(float const *f, int s, float qrx, float qry, float qrz, float ql2) {
 #define GWN_F(k) f[(k) * 4 + s]
@@ -249,8 +261,10 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
                 continue;
             ++n_tests;
             float const qrx = x - cx[s], qry = y - cy[s], qrz = z - cz[s];
-            float const ql2 = qrx * qrx + qry * qry + qrz * qrz;
-            bool const descend = !(ql2 > 0.0f) || ql2 <= mpd[s] * beta2;
+            // The acceptance test is evaluated unfused, operation by operation as the reference
+            // writes it, so every implementation takes the same accept/descend decisions.
+            float const ql2 = squared_norm_unfused(qrx, qry, qrz);
+            bool const descend = !(ql2 > 0.0f) || ql2 <= product_unfused(mpd[s], beta2);
             if (!descend) {
                 float const w = taylor_term<Order>(fields, s, qrx, qry, qrz, ql2);
                 if (isfinite(w)) {
@@ -397,10 +411,26 @@ void winding_exact(
     GWN_LAUNCH_CHECK();
 }
 
+void query_order(
+    gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz,
+    std::uint32_t *order, std::size_t n, hipStream_t stream
+) {
+    if (n == 0)
+        return;
+    device_buffer<std::uint32_t> keys(n, stream), keys_alt(n, stream), ids_alt(n, stream);
+    query_morton_kernel<<<grid_for(n, 256), 256, 0, stream>>>(qx, qy, qz, n, bvh.root.bounds, keys.ptr, order);
+    GWN_LAUNCH_CHECK();
+    std::uint32_t *sk = nullptr;
+    std::uint32_t *sorted = nullptr;
+    radix_sort_pairs_u32(keys.ptr, keys_alt.ptr, order, ids_alt.ptr, n, 0, 30, &sk, &sorted, stream);
+    if (sorted != order)
+        GWN_HIP(hipMemcpyAsync(order, sorted, n * sizeof(std::uint32_t), hipMemcpyDeviceToDevice, stream));
+}
+
 void winding_taylor(
     gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy,
     float const *qz, float *out, std::size_t n, float accuracy_scale, int precision,
-    query_launch const &cfg, std::uint32_t *counters, hipStream_t stream
+    query_launch const &cfg, std::uint32_t const *given_order, std::uint32_t *counters, hipStream_t stream
 ) {
     if (n == 0)
         return;
@@ -417,18 +447,14 @@ void winding_taylor(
         GWN_LAUNCH_CHECK();
         return;
     }
-    bool const sort = cfg.sort_queries > 0 || (cfg.sort_queries < 0 && n >= (1u << 16));
-    device_buffer<std::uint32_t> keys, keys_alt, ids, ids_alt;
-    std::uint32_t *order = nullptr;
+    bool const sort = given_order == nullptr &&
+                      (cfg.sort_queries > 0 || (cfg.sort_queries < 0 && n >= (1u << 16)));
+    device_buffer<std::uint32_t> own_order;
+    std::uint32_t const *order = given_order;
     if (sort) {
-        keys.allocate(n, stream);
-        keys_alt.allocate(n, stream);
-        ids.allocate(n, stream);
-        ids_alt.allocate(n, stream);
-        query_morton_kernel<<<grid_for(n, 256), 256, 0, stream>>>(qx, qy, qz, n, bvh.root.bounds, keys.ptr, ids.ptr);
-        GWN_LAUNCH_CHECK();
-        std::uint32_t *sk = nullptr;
-        radix_sort_pairs_u32(keys.ptr, keys_alt.ptr, ids.ptr, ids_alt.ptr, n, 0, 30, &sk, &order, stream);
+        own_order.allocate(n, stream);
+        query_order(bvh, qx, qy, qz, own_order.ptr, n, stream);
+        order = own_order.ptr;
     }
     float const beta2 = accuracy_scale * accuracy_scale;
     int const cap = cfg.stack_capacity;

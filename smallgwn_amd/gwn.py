@@ -35,7 +35,7 @@ EXPORTED_SYMBOLS = (
     "gwn_moment_order",
     "gwn_compute_winding_number_exact_batch_f32", "gwn_compute_winding_number_taylor_batch_f32",
     "gwn_compute_winding_number_exact_batch_ex_f32", "gwn_compute_winding_number_taylor_batch_ex_f32",
-    "gwn_radix_sort_pairs_u64", "gwn_radix_sort_pairs_u32",
+    "gwn_compute_query_order_f32", "gwn_radix_sort_pairs_u64", "gwn_radix_sort_pairs_u32",
 )
 
 
@@ -145,7 +145,8 @@ def _load():
         "gwn_compute_winding_number_exact_batch_f32": [vp, vp, vp, vp, vp, u64, vp, vp],
         "gwn_compute_winding_number_taylor_batch_f32": [vp, vp, vp, vp, vp, vp, u64, f32, vp, vp],
         "gwn_compute_winding_number_exact_batch_ex_f32": [vp, vp, vp, vp, vp, u64, i32, vp, vp],
-        "gwn_compute_winding_number_taylor_batch_ex_f32": [vp, vp, vp, vp, vp, vp, u64, f32, i32, vp, vp, vp],
+        "gwn_compute_winding_number_taylor_batch_ex_f32": [vp, vp, vp, vp, vp, vp, u64, f32, i32, vp, vp, vp, vp],
+        "gwn_compute_query_order_f32": [vp, vp, vp, vp, vp, u64, vp],
         "gwn_radix_sort_pairs_u64": [vp, vp, u64, i32, i32, vp],
         "gwn_radix_sort_pairs_u32": [vp, vp, u64, i32, i32, vp],
     }
@@ -389,24 +390,37 @@ def gwn_compute_winding_number_exact_batch_ex(bvh: BvhObject, qx, qy, qz, out_f6
 def gwn_compute_winding_number_taylor_batch(bvh: BvhObject, moment: MomentObject, qx, qy, qz, out,
                                             accuracy_scale: float = 2.0, config: QueryConfig | None = None,
                                             stream=None, precision: Precision = Precision.PLAIN,
-                                            counters=None) -> Status:
+                                            counters=None, order=None) -> Status:
     """gwn_compute_winding_number_taylor_batch<Order> (include/gwn/gwn_query.cuh:384-445).
 
-    ``precision=Precision.SPLIT`` evaluates near-field triangles in FP64.  ``counters`` (optional
-    uint32 device array of 3 * count) receives per-query child tests, accepted expansions and
-    near-field triangle evaluations.
+    ``precision=Precision.SPLIT`` evaluates near-field triangles in FP64.  ``order`` (optional
+    uint32 device array from ``gwn_compute_query_order``) fixes the visiting order.  ``counters``
+    (optional uint32 device array of 3 * count) receives per-query child tests, accepted
+    expansions and near-field triangle evaluations.
     """
     bad = _check_soa(qx, qy, qz, [out])
     if bad:
         return bad
+    if order is not None and _numel(order) != _numel(qx):
+        return Status(Error.INVALID_ARGUMENT, "Query order span size must match query count.")
     cfg = _config(config)
-    if precision == Precision.PLAIN and counters is None:
+    if precision == Precision.PLAIN and counters is None and order is None:
         return _status(_lib.gwn_compute_winding_number_taylor_batch_f32(
             bvh.handle, moment.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx),
             float(accuracy_scale), ctypes.byref(cfg), _stream(stream)))
     return _status(_lib.gwn_compute_winding_number_taylor_batch_ex_f32(
         bvh.handle, moment.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx),
-        float(accuracy_scale), int(precision), ctypes.byref(cfg), _dptr(counters), _stream(stream)))
+        float(accuracy_scale), int(precision), ctypes.byref(cfg), _dptr(order), _dptr(counters),
+        _stream(stream)))
+
+
+def gwn_compute_query_order(bvh: BvhObject, qx, qy, qz, order, stream=None) -> Status:
+    """Morton visiting order of query points (uint32 device array, one entry per query)."""
+    bad = _check_soa(qx, qy, qz, [order])
+    if bad:
+        return bad
+    return _status(_lib.gwn_compute_query_order_f32(
+        bvh.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(order), _numel(qx), _stream(stream)))
 
 
 def radix_sort_pairs(keys, values, begin_bit: int = 0, end_bit: int | None = None, stream=None) -> Status:
