@@ -52,7 +52,7 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--queries", type=int, default=16_777_216)
     p.add_argument("--order", type=int, default=2)
-    p.add_argument("--beta", type=float, default=4.0, help="Taylor accuracy scale")
+    p.add_argument("--beta", type=float, default=7.0, help="Taylor accuracy scale")
     p.add_argument("--check-queries", type=int, default=32_768, help="FP64 subsample for |dW|")
     p.add_argument("--exact-queries", type=int, default=32_768, help="subsample timed for the exact form")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline duration")

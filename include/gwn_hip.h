@@ -112,6 +112,13 @@ int gwn_bvh_destroy(gwn_bvh_t bvh);
 int gwn_build_bvh4_f32(
     gwn_geometry_t geometry, gwn_bvh_t bvh, int32_t method, uint32_t hploc_search_radius,
     hipStream_t stream);
+/* Bounds refit from updated positions (same triangle indexing): replaces gwn_refit_bvh
+ * (include/gwn/gwn_bvh_refit.cuh:25-29, detail/gwn_bvh_refit_impl.cuh:194-320).  Rewrites the
+ * leaf-ordered triangle records and every child box in place, keeps topology, primitive order
+ * and stack bounds, and issues a new revision, so moments must be refit afterwards.  A failure
+ * after the in-place writes began leaves the BVH unqueryable (revision 0); it is then bound
+ * to `stream` like a successful refit. */
+int gwn_refit_bvh_f32(gwn_geometry_t geometry, gwn_bvh_t bvh, hipStream_t stream);
 int gwn_bvh_get_info(gwn_bvh_t bvh, gwn_bvh_info *info);
 /* Copies the canonical arrays to host memory (any pointer may be NULL) and synchronizes:
  * nodes in the reference's gwn_bvh_node<4, float> layout (128 B each), primitive order,

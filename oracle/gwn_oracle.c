@@ -431,6 +431,109 @@ int gwn_oracle_build_bvh4_f32(
     return GWN_ORACLE_OK;
 }
 
+/*
+ * Bounds refit (include/gwn/detail/gwn_bvh_refit_impl.cuh:23-127, 129-153, 155-192): rewrite
+ * every leaf-ordered triangle record from the geometry through primitive_indices, recompute the
+ * leaf boxes exactly as the build does (gwn_bvh_triangle_impl.cuh:56-66), and give every internal
+ * child slot the fold of its node's child boxes in slot order, bounds = union(bounds, child)
+ * (combine_node, :40-51).  Nodes are numbered breadth-first, so children follow their parents
+ * and a reverse sweep visits every child before its parent.  The root of a multi-node tree is
+ * the fold over node 0; a leaf root takes its triangle's box (:159-187).
+ */
+static void triangle_box(const float *r, float *box)
+{
+    for (int k = 0; k < 3; ++k) {
+        float lo = (r[6 + k] < r[3 + k]) ? r[6 + k] : r[3 + k];
+        float hi = (r[3 + k] < r[6 + k]) ? r[6 + k] : r[3 + k];
+        box[k] = (lo < r[k]) ? lo : r[k];
+        box[3 + k] = (r[k] < hi) ? hi : r[k];
+    }
+}
+
+int gwn_oracle_refit_bvh_f32(
+    gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 *root,
+    const uint32_t *primitive_indices, float *triangles, uint64_t triangle_count,
+    const float *vx, const float *vy, const float *vz, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2)
+{
+    for (uint64_t s = 0; s < triangle_count; ++s) {
+        uint32_t const p = primitive_indices[s];
+        if (p >= triangle_count)
+            return GWN_ORACLE_INVALID_ARGUMENT;
+        uint32_t const v[3] = {i0[p], i1[p], i2[p]};
+        for (int j = 0; j < 3; ++j) {
+            if (v[j] >= vertex_count)
+                return GWN_ORACLE_INVALID_ARGUMENT;
+            triangles[9 * s + 3 * j + 0] = vx[v[j]];
+            triangles[9 * s + 3 * j + 1] = vy[v[j]];
+            triangles[9 * s + 3 * j + 2] = vz[v[j]];
+        }
+    }
+    if (triangle_count == 0)
+        return GWN_ORACLE_OK;
+    if (node_count == 0) {
+        uint64_t const off = root->reference & ((1ull << 47) - 1ull);
+        if (off >= triangle_count)
+            return GWN_ORACLE_INVALID_ARGUMENT;
+        triangle_box(triangles + 9 * off, root->bounds);
+        return GWN_ORACLE_OK;
+    }
+    float *node_box = (float *)malloc(sizeof(float) * 6 * node_count);
+    if (!node_box)
+        return GWN_ORACLE_INTERNAL_ERROR;
+    for (uint64_t n = node_count; n-- > 0;) {
+        int init = 0;
+        float acc[6];
+        for (int s = 0; s < 4; ++s) {
+            gwn_oracle_child_f32 *c = &nodes[n].children[s];
+            uint64_t const ref = c->reference;
+            if (!(ref & (1ull << 47)))
+                continue;
+            uint64_t const off = ref & ((1ull << 47) - 1ull);
+            uint32_t const count = (uint32_t)(ref >> 48);
+            float box[6];
+            if (count == 0) {
+                if (off <= n || off >= node_count) {
+                    free(node_box);
+                    return GWN_ORACLE_INVALID_ARGUMENT;
+                }
+                for (int k = 0; k < 6; ++k)
+                    box[k] = node_box[6 * off + k];
+            } else {
+                if (off + count > triangle_count) {
+                    free(node_box);
+                    return GWN_ORACLE_INVALID_ARGUMENT;
+                }
+                triangle_box(triangles + 9 * off, box);
+                for (uint32_t j = 1; j < count; ++j) {
+                    float b2[6];
+                    triangle_box(triangles + 9 * (off + j), b2);
+                    aabb_union(box, box, b2);
+                }
+            }
+            for (int k = 0; k < 6; ++k)
+                c->bounds[k] = box[k];
+            if (init) {
+                aabb_union(acc, acc, box);
+            } else {
+                for (int k = 0; k < 6; ++k)
+                    acc[k] = box[k];
+                init = 1;
+            }
+        }
+        if (!init) {
+            free(node_box);
+            return GWN_ORACLE_INVALID_ARGUMENT;
+        }
+        for (int k = 0; k < 6; ++k)
+            node_box[6 * n + k] = acc[k];
+    }
+    for (int k = 0; k < 6; ++k)
+        root->bounds[k] = node_box[k];
+    free(node_box);
+    return GWN_ORACLE_OK;
+}
+
 int gwn_oracle_refit_moment_f32(const gwn_oracle_node4_f32 *nodes, uint64_t node_count,
                                 const float *triangles, uint64_t triangle_count, int order,
                                 float *out)

@@ -13,6 +13,7 @@
  *   - stable key sort: gwn_bvh_build_common.cuh:325-347 (CUB LSD radix sort is stable)
  *   - Karras LBVH: include/gwn/detail/gwn_bvh_build_binary.cuh:29-129
  *   - bottom-up bounds: gwn_bvh_build_binary.cuh:131-192
+ *   - bounds refit: include/gwn/detail/gwn_bvh_refit_impl.cuh:23-192
  *   - BVH4 collapse rule and stack bounds: include/gwn/detail/gwn_bvh_build_collapse.cuh:163-296
  *   - leaf-ordered triangle records: include/gwn/detail/gwn_bvh_triangle_impl.cuh:23-67
  *   - Taylor moments: include/gwn/detail/gwn_bvh_refit_common.cuh:66-387, 445-607
@@ -92,6 +93,17 @@ int gwn_oracle_build_bvh4_f32(
     const float *vx, const float *vy, const float *vz, uint64_t vertex_count,
     const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
     gwn_oracle_bvh_f32 *out);
+
+/*
+ * Bounds refit in place: triangle records from the geometry, child boxes folded in slot order,
+ * root bounds (gwn_bvh_refit_impl.cuh:23-192).  Topology, primitive order and stack bounds are
+ * untouched.
+ */
+int gwn_oracle_refit_bvh_f32(
+    gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 *root,
+    const uint32_t *primitive_indices, float *triangles, uint64_t triangle_count,
+    const float *vx, const float *vy, const float *vz, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2);
 
 /* Number of float fields per child slot for Taylor order 0/1/2 (gwn_bvh.cuh:144-202). */
 int gwn_oracle_moment_field_count(int order);

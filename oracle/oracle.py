@@ -8,6 +8,7 @@ the ``cpu_baseline`` leg of ``bench.py`` as the checker.  The product path in
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import os
 import subprocess
 from dataclasses import dataclass
@@ -71,6 +72,8 @@ def lib():
         L.gwn_oracle_stable_order_u64.argtypes = [vp, u64, vp]
         L.gwn_oracle_build_bvh4_f32.restype = i32
         L.gwn_oracle_build_bvh4_f32.argtypes = [vp, vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(_Bvh)]
+        L.gwn_oracle_refit_bvh_f32.restype = i32
+        L.gwn_oracle_refit_bvh_f32.argtypes = [vp, u64, ctypes.POINTER(_Child), vp, vp, u64, vp, vp, vp, u64, vp, vp, vp]
         L.gwn_oracle_moment_field_count.restype = i32
         L.gwn_oracle_moment_field_count.argtypes = [i32]
         L.gwn_oracle_refit_moment_f32.restype = i32
@@ -169,6 +172,29 @@ def build_bvh4(mesh) -> OracleBvh:
         internal_stack_bound=int(out.internal_stack_bound),
         packed_stack_bound=int(out.packed_stack_bound),
         scene=np.array(list(out.scene), dtype=np.float32),
+    )
+
+
+def refit_bvh(bvh: OracleBvh, mesh) -> OracleBvh:
+    """Bounds refit of `bvh` against `mesh` (same triangle indexing): a new OracleBvh whose
+    triangle records, child boxes and root box come from the mesh; topology is unchanged."""
+    nodes = bvh.nodes.copy()
+    tris = np.ascontiguousarray(bvh.triangles.copy())
+    prim = _c(bvh.primitive_indices, np.uint32)
+    vx, vy, vz = _c(mesh.vx, np.float32), _c(mesh.vy, np.float32), _c(mesh.vz, np.float32)
+    i0, i1, i2 = _c(mesh.i0, np.uint32), _c(mesh.i1, np.uint32), _c(mesh.i2, np.uint32)
+    root = _Child()
+    for k in range(6):
+        root.bounds[k] = float(bvh.root_bounds[k])
+    root.reference = bvh.root_reference
+    st = lib().gwn_oracle_refit_bvh_f32(
+        _ptr(nodes) if nodes.size else None, bvh.node_count, ctypes.byref(root), _ptr(prim), _ptr(tris),
+        prim.shape[0], _ptr(vx), _ptr(vy), _ptr(vz), vx.shape[0], _ptr(i0), _ptr(i1), _ptr(i2),
+    )
+    if st != 0:
+        raise RuntimeError(f"oracle bounds refit failed with status {st}")
+    return dataclasses.replace(
+        bvh, nodes=nodes, triangles=tris, root_bounds=np.array(list(root.bounds), dtype=np.float32)
     )
 
 

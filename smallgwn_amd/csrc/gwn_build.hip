@@ -3,15 +3,18 @@
 // Pipeline (reference: include/gwn/detail/gwn_bvh_build_impl.cuh:44-255):
 //   scene bounds (device reduction, no host read-back)       build_common.cuh:243-274
 //   triangle AABB + 63-bit Morton code                       build_common.cuh:157-214
-//   stable onesweep radix sort of (code, triangle id)        build_common.cuh:325-347
+//   stable LSD radix sort of (code, triangle id)             build_common.cuh:325-347
 //   Karras LBVH                                              build_binary.cuh:29-129
 //   bottom-up bounds + first leaf (arrival counters)         build_binary.cuh:131-192
-//   BVH4 collapse on a persistent work queue                 build_collapse.cuh:98-314
-//   breadth-first numbering by (depth, first sorted leaf)    (replaces reorder.cuh:61-143)
+//   BVH4 collapse, one launch per level, emitting nodes      build_collapse.cuh:98-314
+//     directly in breadth-first (depth, first sorted leaf)   (replaces reorder.cuh:61-143)
+//     order
 //   leaf-ordered triangle records                            build_impl.cuh:23-41
-// One host synchronization remains, after the collapse, to size the node array from the
-// device-side node count (the reference performs the same read-back, collapse.cuh:425-446).
+// The host reads the small level table back once per chunk of levels (about three times for a
+// million triangles) to find the last level and size the node array; the reference performs
+// one such read-back of its node count (collapse.cuh:425-446).
 #include "gwn_core.hpp"
+#include "gwn_lookback.hpp"
 
 #include <algorithm>
 #include <vector>
@@ -235,10 +238,12 @@ __global__ __launch_bounds__(k_block) void bounds_kernel(
     std::uint32_t link = lparent[leaf];
     while (link != k_invalid) {
         std::uint32_t const node = link >> 1;
+        release_agent();
         std::uint32_t const prior =
-            __hip_atomic_fetch_add(&arrivals[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&arrivals[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (prior == 0u)
             return;
+        acquire_agent();
         arrivals[node] = 0u;
         uint2 const bn = bnodes[node];
         std::uint32_t const l = bn.x & ~k_leaf_bit, r = bn.y & ~k_leaf_bit;
@@ -262,188 +267,203 @@ __device__ inline float half_area(aabb const &b) {
     return dx * dy + dy * dz + dz * dx;
 }
 
-constexpr std::uint32_t k_collapse_idle_limit = 1u << 22;
+// Level-synchronous BVH4 collapse.  Level d holds the wide nodes at depth d, numbered
+// contiguously from levels[d].start in (depth, first sorted leaf) order.  One launch per level:
+// every thread takes one wide node of the level, expands its binary subtree to at most four
+// candidates by always opening the internal candidate with the largest half-area (ties: smaller
+// binary index; collapse.cuh:170-211), and a look-back scan over the level's internal-child
+// counts numbers the next level's nodes.  Children of one node are numbered in first-leaf
+// order, which keeps the whole level in first-leaf order, so nodes come out in their final
+// breadth-first positions and no reordering pass follows.
+constexpr int k_max_levels = 130; // binary depth <= 128: the 127-bit (code, index) prefix grows per level
 
 struct collapse_task {
     std::uint32_t binary;
-    std::uint32_t depth;
-    std::uint32_t prefix_internal;
-    std::uint32_t prefix_packed;
-};
-
-struct collapse_counters {
-    std::uint32_t head;       // next queue position to claim
-    std::uint32_t tail;       // next wide-node id to allocate (= queue length)
-    std::uint32_t leaves_done;
-    std::uint32_t max_internal;
-    std::uint32_t max_packed;
-    std::uint32_t max_depth;
-    std::uint32_t error;
+    std::uint32_t prefix_internal; // internal-child stack demand accumulated above this node
+    std::uint32_t prefix_packed;   // same for all valid children
     std::uint32_t pad;
 };
 
-__global__ void collapse_init_kernel(collapse_task *tasks, std::uint32_t *ready, collapse_counters *ctr) {
+struct collapse_level {
+    std::uint32_t start;
+    std::uint32_t count;
+    std::uint32_t tiles_claimed;
+    std::uint32_t pad;
+};
+
+struct collapse_stats {
+    std::uint32_t max_internal;
+    std::uint32_t max_packed;
+    std::uint32_t leaves;
+    std::uint32_t error;
+};
+
+__global__ void collapse_init_kernel(
+    collapse_task *tasks, collapse_level *levels, collapse_stats *stats, std::uint32_t *parent
+) {
     tasks[0] = collapse_task{0u, 0u, 0u, 0u};
-    ready[0] = 1u;
-    *ctr = collapse_counters{0u, 1u, 0u, 0u, 0u, 0u, 0u, 0u};
+    levels[0] = collapse_level{0u, 1u, 0u, 0u};
+    *stats = collapse_stats{0u, 0u, 0u, 0u};
+    parent[0] = k_invalid;
 }
 
-// Persistent BVH4 collapse.  Every wide node is one queue entry; its id is its queue position.
-// A worker claims the next position, polls until the entry is published, expands the binary
-// subtree to at most four candidates by always opening the internal candidate with the largest
-// half-area (ties: smaller binary index; collapse.cuh:170-211), publishes internal children as
-// new entries and retires.  There is no nested spin: a lane whose entry is not yet published
-// falls through to the next loop iteration, so producer lanes of the same wave keep running.
-// All wide nodes are done when every triangle has been emitted as a leaf.
-__global__ __launch_bounds__(k_block) void collapse_kernel(
+__device__ inline std::uint32_t wave_max_u32(std::uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        std::uint32_t const w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(k_block) void collapse_level_kernel(
     uint2 const *__restrict__ bnodes, aabb const *__restrict__ ibounds,
-    std::uint32_t const *__restrict__ ifirst, aabb const *__restrict__ leaf_boxes,
-    std::uint32_t n_leaves, std::uint32_t capacity, int key_shift, collapse_task *tasks,
-    std::uint32_t *ready, collapse_counters *ctr, bvh_node4 *__restrict__ wide,
-    std::uint64_t *__restrict__ keys
+    std::uint32_t const *__restrict__ ifirst, aabb const *__restrict__ leaf_boxes, int depth,
+    std::uint32_t capacity, collapse_level *levels, collapse_task *tasks, std::uint32_t *status_cur,
+    std::uint32_t *status_next, std::uint32_t status_words, collapse_stats *stats,
+    bvh_node4 *__restrict__ nodes, std::uint32_t *__restrict__ parent,
+    std::uint8_t *__restrict__ internal_children
 ) {
-    std::uint32_t claim = k_invalid;
-    std::uint32_t idle = 0;
-    for (;;) {
-        if (claim == k_invalid)
-            claim = atomicAdd(&ctr->head, 1u);
-        bool worked = false;
-        if (claim < capacity &&
-            __hip_atomic_load(&ready[claim], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-            worked = true;
-            std::uint32_t const self = claim;
-            claim = k_invalid;
-            collapse_task const task = tasks[self];
-            uint2 const root = bnodes[task.binary];
-            std::uint32_t cand[4] = {root.x, root.y, k_invalid, k_invalid};
-            int count = 2;
-            while (count < 4) {
-                int best = -1;
-                float best_area = -INFINITY;
-                std::uint32_t best_index = k_invalid;
-                for (int s = 0; s < count; ++s) {
-                    if (cand[s] & k_leaf_bit)
-                        continue;
-                    float const area = half_area(ibounds[cand[s]]);
-                    if (best >= 0 && area < best_area)
-                        continue;
-                    if (best >= 0 && area == best_area && cand[s] >= best_index)
-                        continue;
-                    best = s;
-                    best_area = area;
-                    best_index = cand[s];
-                }
-                if (best < 0)
-                    break;
-                uint2 const ex = bnodes[cand[best]];
-                cand[best] = ex.x;
-                cand[count] = ex.y;
-                ++count;
-            }
-            std::uint32_t n_internal = 0, n_leaf = 0;
-            for (int s = 0; s < count; ++s) {
-                if (cand[s] & k_leaf_bit)
-                    ++n_leaf;
-                else
-                    ++n_internal;
-            }
-            std::uint32_t const node_internal = task.prefix_internal + (n_internal ? n_internal - 1u : 0u);
-            std::uint32_t const node_packed = task.prefix_packed + static_cast<std::uint32_t>(count - 1);
-            bvh_node4 out{};
-            for (int s = 0; s < count; ++s) {
-                std::uint32_t const c = cand[s] & ~k_leaf_bit;
-                if (cand[s] & k_leaf_bit) {
-                    out.children[s].bounds = leaf_boxes[c];
-                    out.children[s].reference = encode_leaf(c, 1u);
-                    continue;
-                }
-                std::uint32_t const id = atomicAdd(&ctr->tail, 1u);
-                if (id >= capacity) {
-                    atomicOr(&ctr->error, 1u);
-                    continue;
-                }
-                out.children[s].bounds = ibounds[c];
-                out.children[s].reference = encode_internal(id);
-                tasks[id] = collapse_task{c, task.depth + 1u, node_internal, node_packed};
-                __hip_atomic_store(&ready[id], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            wide[self] = out;
-            keys[self] = (static_cast<std::uint64_t>(task.depth) << key_shift) | ifirst[task.binary];
-            atomicMax(&ctr->max_internal, node_internal);
-            atomicMax(&ctr->max_packed, node_packed);
-            atomicMax(&ctr->max_depth, task.depth);
-            __hip_atomic_fetch_add(&ctr->leaves_done, n_leaf, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (!worked) {
-            std::uint32_t const done =
-                __hip_atomic_load(&ctr->leaves_done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            std::uint32_t const err = __hip_atomic_load(&ctr->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (done >= n_leaves || err != 0u)
-                break;
-            // A well-formed binary tree always drains the queue; the idle cap only guarantees
-            // that a malformed one ends the kernel (reported as an error) instead of hanging.
-            if (++idle > k_collapse_idle_limit) {
-                atomicOr(&ctr->error, 2u);
-                break;
-            }
-        } else {
-            idle = 0;
-        }
-        if (__ballot(worked) == 0ull)
-            __builtin_amdgcn_s_sleep(2);
-    }
-}
-
-__global__ __launch_bounds__(k_block) void make_keys32_kernel(
-    std::uint64_t const *__restrict__ keys64, std::size_t n, std::uint32_t *__restrict__ keys32,
-    std::uint32_t *__restrict__ ids
-) {
-    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (i < n) {
-        keys32[i] = static_cast<std::uint32_t>(keys64[i]);
-        ids[i] = static_cast<std::uint32_t>(i);
-    }
-}
-
-__global__ __launch_bounds__(k_block) void iota_kernel(std::uint32_t *__restrict__ ids, std::size_t n) {
-    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (i < n)
-        ids[i] = static_cast<std::uint32_t>(i);
-}
-
-__global__ __launch_bounds__(k_block) void invert_kernel(
-    std::uint32_t const *__restrict__ perm, std::size_t n, std::uint32_t *__restrict__ inv
-) {
-    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (i < n)
-        inv[perm[i]] = static_cast<std::uint32_t>(i);
-}
-
-// nodes[new] = wide[perm[new]] with internal child references renumbered; records parent links
-// and internal-child counts for the bottom-up refits.
-__global__ __launch_bounds__(k_block) void scatter_nodes_kernel(
-    bvh_node4 const *__restrict__ wide, std::uint32_t const *__restrict__ perm,
-    std::uint32_t const *__restrict__ inv, std::size_t n, bvh_node4 *__restrict__ nodes,
-    std::uint32_t *__restrict__ parent, std::uint8_t *__restrict__ internal_children
-) {
-    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (i >= n)
+    __shared__ std::uint32_t s_tile;
+    __shared__ std::uint32_t s_exclusive;
+    __shared__ std::uint32_t s_wave[k_block / 64];
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0)
+        s_tile = atomicAdd(&levels[depth].tiles_claimed, 1u);
+    // The next level scans over the other status array; clear it for that launch.
+    for (std::uint32_t i = blockIdx.x * k_block + threadIdx.x; i < status_words; i += gridDim.x * k_block)
+        status_next[i] = 0u;
+    __syncthreads();
+    std::uint32_t const start = levels[depth].start;
+    std::uint32_t const count = levels[depth].count;
+    std::uint32_t const tile = s_tile;
+    // A level never exceeds the launch (4^depth bound) or the capacity; report, do not index.
+    if (static_cast<std::uint64_t>(start) + count > capacity ||
+        count > static_cast<std::uint64_t>(gridDim.x) * k_block) {
+        if (threadIdx.x == 0)
+            atomicOr(&stats->error, 1u);
         return;
-    bvh_node4 node = wide[perm[i]];
-    std::uint8_t n_int = 0;
-    for (int s = 0; s < 4; ++s) {
-        std::uint64_t const r = node.children[s].reference;
-        if (!ref_internal(r))
-            continue;
-        std::uint32_t const child = inv[ref_offset(r)];
-        node.children[s].reference = encode_internal(child);
-        parent[child] = (static_cast<std::uint32_t>(i) << 2) | static_cast<std::uint32_t>(s);
-        ++n_int;
     }
-    nodes[i] = node;
-    internal_children[i] = n_int;
-    if (i == 0)
-        parent[0] = k_invalid;
+    if (tile >= (count + k_block - 1) / k_block)
+        return;
+    std::uint32_t const i = tile * k_block + threadIdx.x;
+    bool const active = i < count;
+    std::uint32_t const self = start + i;
+
+    std::uint32_t cand[4] = {k_invalid, k_invalid, k_invalid, k_invalid};
+    int n_cand = 0;
+    collapse_task task{};
+    if (active) {
+        task = tasks[self];
+        uint2 const root = bnodes[task.binary];
+        cand[0] = root.x;
+        cand[1] = root.y;
+        n_cand = 2;
+        while (n_cand < 4) {
+            int best = -1;
+            float best_area = -INFINITY;
+            std::uint32_t best_index = k_invalid;
+            for (int s = 0; s < n_cand; ++s) {
+                if (cand[s] & k_leaf_bit)
+                    continue;
+                float const area = half_area(ibounds[cand[s]]);
+                if (best >= 0 && area < best_area)
+                    continue;
+                if (best >= 0 && area == best_area && cand[s] >= best_index)
+                    continue;
+                best = s;
+                best_area = area;
+                best_index = cand[s];
+            }
+            if (best < 0)
+                break;
+            uint2 const ex = bnodes[cand[best]];
+            cand[best] = ex.x;
+            cand[n_cand] = ex.y;
+            ++n_cand;
+        }
+    }
+    std::uint32_t first[4];
+    std::uint32_t n_internal = 0, n_leaf = 0;
+    for (int s = 0; s < 4; ++s) {
+        first[s] = k_invalid;
+        if (s >= n_cand)
+            continue;
+        if (cand[s] & k_leaf_bit) {
+            ++n_leaf;
+        } else {
+            ++n_internal;
+            first[s] = ifirst[cand[s]];
+        }
+    }
+
+    // Block-wide exclusive scan of internal-child counts, then the cross-tile look-back.
+    std::uint32_t incl = n_internal;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        std::uint32_t const v = __shfl_up(incl, o, 64);
+        if (lane >= o)
+            incl += v;
+    }
+    if (lane == 63)
+        s_wave[wave] = incl;
+    __syncthreads();
+    std::uint32_t wave_prefix = 0, block_total = 0;
+    for (int w = 0; w < k_block / 64; ++w) {
+        if (w < wave)
+            wave_prefix += s_wave[w];
+        block_total += s_wave[w];
+    }
+    if (wave == 0) {
+        std::uint32_t const ex = wave_lookback(status_cur, tile, block_total, &stats->error);
+        if (lane == 0)
+            s_exclusive = ex;
+    }
+    __syncthreads();
+    std::uint32_t const next_start = start + count;
+    if (threadIdx.x == 0 && tile == (count - 1) / k_block) {
+        levels[depth + 1].start = next_start;
+        levels[depth + 1].count = s_exclusive + block_total;
+    }
+    if (active) {
+        std::uint32_t const base = next_start + s_exclusive + wave_prefix + incl - n_internal;
+        std::uint32_t const node_internal = task.prefix_internal + (n_internal ? n_internal - 1u : 0u);
+        std::uint32_t const node_packed = task.prefix_packed + static_cast<std::uint32_t>(n_cand - 1);
+        bvh_node4 out{};
+        for (int s = 0; s < n_cand; ++s) {
+            std::uint32_t const c = cand[s] & ~k_leaf_bit;
+            if (cand[s] & k_leaf_bit) {
+                out.children[s].bounds = leaf_boxes[c];
+                out.children[s].reference = encode_leaf(c, 1u);
+                continue;
+            }
+            std::uint32_t rank = 0;
+            for (int t = 0; t < 4; ++t)
+                rank += first[t] < first[s] ? 1u : 0u;
+            std::uint32_t const id = base + rank;
+            if (id >= capacity) {
+                atomicOr(&stats->error, 1u);
+                continue;
+            }
+            out.children[s].bounds = ibounds[c];
+            out.children[s].reference = encode_internal(id);
+            tasks[id] = collapse_task{c, node_internal, node_packed, 0u};
+            parent[id] = (self << 2) | static_cast<std::uint32_t>(s);
+        }
+        nodes[self] = out;
+        internal_children[self] = static_cast<std::uint8_t>(n_internal);
+        task.prefix_internal = node_internal;
+        task.prefix_packed = node_packed;
+    }
+    std::uint32_t const w_int = wave_max_u32(active ? task.prefix_internal : 0u);
+    std::uint32_t const w_pack = wave_max_u32(active ? task.prefix_packed : 0u);
+    std::uint32_t const w_leaf = wave_sum_u32(n_leaf);
+    if (lane == 0) {
+        atomicMax(&stats->max_internal, w_int);
+        atomicMax(&stats->max_packed, w_pack);
+        atomicAdd(&stats->leaves, w_leaf);
+    }
 }
 
 __global__ __launch_bounds__(k_block) void records_kernel(
@@ -464,12 +484,6 @@ __global__ __launch_bounds__(k_block) void records_kernel(
     records[i] = tri_record{vx[a], vy[a], vz[a], vx[b], vy[b], vz[b], vx[c], vy[c], vz[c]};
 }
 
-int bits_for(std::uint64_t v) {
-    int b = 0;
-    while (b < 64 && (std::uint64_t(1) << b) < v)
-        ++b;
-    return b;
-}
 
 } // namespace
 
@@ -551,76 +565,83 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, hipStream_t s
     );
     GWN_LAUNCH_CHECK();
 
-    // Persistent collapse.
-    int const key_shift = bits_for(T);
+    // Level-synchronous collapse into oversized arrays (wide nodes <= binary internal nodes).
     std::size_t const capacity = NI;
+    std::uint32_t const status_words = static_cast<std::uint32_t>((capacity + k_block - 1) / k_block);
     device_buffer<collapse_task> tasks(capacity, stream);
-    device_buffer<std::uint32_t> ready(capacity, stream);
-    device_buffer<collapse_counters> ctr(1, stream);
+    device_buffer<collapse_level> levels(k_max_levels, stream);
+    device_buffer<collapse_stats> stats(1, stream);
+    device_buffer<std::uint32_t> status(2 * static_cast<std::size_t>(status_words), stream);
     device_buffer<bvh_node4> wide(capacity, stream);
-    device_buffer<std::uint64_t> keys(capacity, stream);
-    fill_u32(ready.ptr, 0u, capacity, stream);
-    collapse_init_kernel<<<1, 1, 0, stream>>>(tasks.ptr, ready.ptr, ctr.ptr);
+    device_buffer<std::uint32_t> wparent(capacity, stream);
+    device_buffer<std::uint8_t> wchildren(capacity, stream);
+    fill_u32(reinterpret_cast<std::uint32_t *>(levels.ptr), 0u, k_max_levels * 4, stream);
+    fill_u32(status.ptr, 0u, 2 * static_cast<std::size_t>(status_words), stream);
+    collapse_init_kernel<<<1, 1, 0, stream>>>(tasks.ptr, levels.ptr, stats.ptr, wparent.ptr);
     GWN_LAUNCH_CHECK();
-    {
-        int device = 0, cus = 0;
-        GWN_HIP(hipGetDevice(&device));
-        GWN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-        std::size_t workers = static_cast<std::size_t>(std::max(cus, 1)) * 8u * k_block;
-        workers = std::min(workers, ((capacity + k_block - 1) / k_block) * k_block);
-        collapse_kernel<<<grid_for(workers, k_block), k_block, 0, stream>>>(
-            bnodes.ptr, ibounds.ptr, ifirst.ptr, leaf_boxes.ptr, static_cast<std::uint32_t>(T),
-            static_cast<std::uint32_t>(capacity), key_shift, tasks.ptr, ready.ptr, ctr.ptr, wide.ptr,
-            keys.ptr
-        );
-        GWN_LAUNCH_CHECK();
+
+    // Launch levels in chunks and read the level table back after each chunk; a launch for a
+    // level past the last one finds an empty level and returns.
+    std::vector<collapse_level> host_levels(k_max_levels);
+    int depth = 0;
+    int chunk = 8;
+    int empty_level = -1;
+    while (empty_level < 0) {
+        int const chunk_end = std::min(depth + chunk, k_max_levels - 1);
+        for (; depth < chunk_end; ++depth) {
+            // A level holds at most 4^depth nodes and never more than the capacity.
+            std::size_t bound = capacity;
+            if (depth < 15)
+                bound = std::min<std::size_t>(bound, std::size_t(1) << (2 * depth));
+            unsigned const grid = grid_for(bound, k_block);
+            std::uint32_t *cur = status.ptr + static_cast<std::size_t>(depth & 1) * status_words;
+            std::uint32_t *nxt = status.ptr + static_cast<std::size_t>((depth + 1) & 1) * status_words;
+            collapse_level_kernel<<<grid, k_block, 0, stream>>>(
+                bnodes.ptr, ibounds.ptr, ifirst.ptr, leaf_boxes.ptr, depth,
+                static_cast<std::uint32_t>(capacity), levels.ptr, tasks.ptr, cur,
+                nxt, status_words, stats.ptr, wide.ptr, wparent.ptr, wchildren.ptr
+            );
+            GWN_LAUNCH_CHECK();
+        }
+        GWN_HIP(hipMemcpyAsync(host_levels.data(), levels.ptr, k_max_levels * sizeof(collapse_level),
+                               hipMemcpyDeviceToHost, stream));
+        GWN_HIP(hipStreamSynchronize(stream));
+        for (int d = 0; d <= depth; ++d) {
+            if (host_levels[d].count == 0u) {
+                empty_level = d;
+                break;
+            }
+        }
+        if (empty_level < 0 && depth >= k_max_levels - 1)
+            throw_internal("bvh.build.collapse: Canonical wide collapse exceeded the depth limit.");
+        chunk *= 2;
     }
-    collapse_counters hc{};
+    collapse_stats hs{};
     aabb root_box{};
     std::uint32_t rec_err = 0;
-    GWN_HIP(hipMemcpyAsync(&hc, ctr.ptr, sizeof(hc), hipMemcpyDeviceToHost, stream));
+    GWN_HIP(hipMemcpyAsync(&hs, stats.ptr, sizeof(hs), hipMemcpyDeviceToHost, stream));
     GWN_HIP(hipMemcpyAsync(&root_box, ibounds.ptr, sizeof(aabb), hipMemcpyDeviceToHost, stream));
     GWN_HIP(hipMemcpyAsync(&rec_err, flags.ptr, sizeof(rec_err), hipMemcpyDeviceToHost, stream));
     GWN_HIP(hipStreamSynchronize(stream));
     if (rec_err != 0u)
         throw_internal("bvh.build: Leaf-ordered triangle construction encountered invalid geometry.");
-    if (hc.error != 0u || hc.tail == 0u || hc.tail > capacity || hc.leaves_done != T)
+    std::size_t const NW = host_levels[empty_level].start;
+    if (hs.error != 0u || empty_level == 0 || NW == 0u || NW > capacity || hs.leaves != T)
         throw_internal("bvh.build.collapse: Canonical wide collapse reported an invalid output.");
-    std::size_t const NW = hc.tail;
 
-    // Breadth-first numbering: sort wide nodes by (depth, first sorted leaf).
-    int const end_bit = key_shift + bits_for(static_cast<std::uint64_t>(hc.max_depth) + 1u);
-    device_buffer<std::uint32_t> perm(NW, stream), perm_alt(NW, stream), inv(NW, stream);
-    std::uint32_t *sorted_perm = nullptr;
-    if (end_bit <= 32) {
-        device_buffer<std::uint32_t> k32(NW, stream), k32_alt(NW, stream);
-        make_keys32_kernel<<<grid_for(NW, k_block), k_block, 0, stream>>>(keys.ptr, NW, k32.ptr, perm.ptr);
-        GWN_LAUNCH_CHECK();
-        std::uint32_t *sk = nullptr;
-        radix_sort_pairs_u32(k32.ptr, k32_alt.ptr, perm.ptr, perm_alt.ptr, NW, 0, end_bit, &sk, &sorted_perm, stream);
-    } else {
-        device_buffer<std::uint64_t> k64_alt(NW, stream);
-        iota_kernel<<<grid_for(NW, k_block), k_block, 0, stream>>>(perm.ptr, NW);
-        GWN_LAUNCH_CHECK();
-        std::uint64_t *sk = nullptr;
-        radix_sort_pairs_u64(keys.ptr, k64_alt.ptr, perm.ptr, perm_alt.ptr, NW, 0, end_bit, &sk, &sorted_perm, stream);
-    }
-    invert_kernel<<<grid_for(NW, k_block), k_block, 0, stream>>>(sorted_perm, NW, inv.ptr);
-    GWN_LAUNCH_CHECK();
     out.nodes.allocate(NW, stream);
     out.parent.allocate(NW, stream);
     out.internal_children.allocate(NW, stream);
-    scatter_nodes_kernel<<<grid_for(NW, k_block), k_block, 0, stream>>>(
-        wide.ptr, sorted_perm, inv.ptr, NW, out.nodes.ptr, out.parent.ptr, out.internal_children.ptr
-    );
-    GWN_LAUNCH_CHECK();
+    GWN_HIP(hipMemcpyAsync(out.nodes.ptr, wide.ptr, NW * sizeof(bvh_node4), hipMemcpyDeviceToDevice, stream));
+    GWN_HIP(hipMemcpyAsync(out.parent.ptr, wparent.ptr, NW * sizeof(std::uint32_t), hipMemcpyDeviceToDevice, stream));
+    GWN_HIP(hipMemcpyAsync(out.internal_children.ptr, wchildren.ptr, NW, hipMemcpyDeviceToDevice, stream));
 
     out.node_count = NW;
     out.root.bounds = root_box;
     out.root.reference = encode_internal(0);
-    out.internal_stack_bound = hc.max_internal;
-    out.packed_stack_bound = hc.max_packed;
-    out.max_depth = hc.max_depth;
+    out.internal_stack_bound = hs.max_internal;
+    out.packed_stack_bound = hs.max_packed;
+    out.max_depth = static_cast<std::uint32_t>(empty_level - 1);
 }
 
 } // namespace gwn

@@ -245,7 +245,9 @@ int gwn_build_bvh4_f32(
             gwn::throw_invalid("bvh.build: the H-PLOC builder is not available in this library version; use the LBVH method.");
         if (geometry->triangle_count != 0 && geometry->vertex_count == 0)
             gwn::throw_invalid("bvh.build: Cannot build a BVH with triangles but no vertices.");
-        if (geometry->triangle_count > 0x7fffffffull)
+        // Wide-node ids and per-level counts travel in 30-bit fields (look-back status words,
+        // parent links with the slot in the low two bits).
+        if (geometry->triangle_count > (1ull << 30) - 1ull)
             gwn::throw_invalid("bvh.build: Triangle count exceeds the native wide-collapse counter range.");
         gwn_bvh_handle staging;
         staging.stream = stream;
@@ -257,6 +259,29 @@ int gwn_build_bvh4_f32(
         }
         std::swap(*bvh, staging);
         // staging now owns the replaced storage; its buffers free on their own streams.
+    });
+}
+
+int gwn_refit_bvh_f32(gwn_geometry_t geometry, gwn_bvh_t bvh, hipStream_t stream) {
+    return guarded("gwn_refit_bvh", [&] {
+        if (geometry == nullptr || (geometry->triangle_count != 0 && geometry->i0.ptr == nullptr))
+            gwn::throw_invalid("bvh.refit: Geometry accessor is invalid.");
+        if (bvh == nullptr || !bvh->valid())
+            gwn::throw_invalid("bvh.refit: BVH accessor is invalid.");
+        if (geometry->triangle_count != bvh->triangle_count)
+            gwn::throw_invalid("bvh.refit: Geometry triangle count differs from the BVH primitive order.");
+        std::uint64_t const next_revision = gwn::reserve_revision();
+        if (next_revision == 0)
+            gwn::throw_internal("bvh.refit: BVH geometry-derived state revision is exhausted.");
+        // Writes below are in place and cannot be rolled back: the BVH is unqueryable (and
+        // every moment formed from it stale) until the refit has converged.  From here on its
+        // storage is released on the refit stream, which holds the last writes.
+        bvh->revision = 0;
+        bvh->stream = stream;
+        bvh->nodes.stream = bvh->primitive_indices.stream = bvh->triangles.stream = stream;
+        bvh->parent.stream = bvh->internal_children.stream = stream;
+        gwn::refit_bounds(*geometry, *bvh, stream);
+        bvh->revision = next_revision;
     });
 }
 

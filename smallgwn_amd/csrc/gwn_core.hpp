@@ -162,6 +162,26 @@ __host__ __device__ constexpr int moment_record_vec4(int order) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Cross-workgroup publication within one kernel.
+//
+// MI355X has one L2 per XCD; agent-scope release writes dirty L2 lines back (buffer_wbl2 sc1)
+// and acquire invalidates (buffer_inv sc1).  The release must also wait for the plain stores
+// it publishes and for the write-back itself before the flag or counter update is performed.
+// hipcc (ROCm 7.2) drops that wait when the release sits at a loop header and the stores
+// were issued on the back edge, so publishers call release_agent() (fence + explicit full
+// wait) and then update the flag with a relaxed atomic; consumers use acquire_agent() after
+// observing it.
+// ---------------------------------------------------------------------------------------------
+
+__device__ inline void release_agent() {
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
+}
+
+__device__ inline void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
+// ---------------------------------------------------------------------------------------------
 // Library objects behind the opaque C handles.
 // ---------------------------------------------------------------------------------------------
 
@@ -227,6 +247,8 @@ void radix_sort_pairs_u32(
 
 void build_bvh4(gwn_geometry_handle const &geometry, gwn_bvh_handle &staging, hipStream_t stream);
 void refit_moment(gwn_bvh_handle const &bvh, gwn_moment_handle &staging, hipStream_t stream);
+// In-place bounds refit: triangle records and child boxes from the geometry, root box updated.
+void refit_bounds(gwn_geometry_handle const &geometry, gwn_bvh_handle &bvh, hipStream_t stream);
 
 struct query_launch {
     int block_size = 256;

@@ -320,10 +320,12 @@ __global__ __launch_bounds__(k_block) void moment_refit_kernel(
             return;
         pending[n] = total;
         std::size_t const pn = link >> 2;
+        release_agent();
         std::uint32_t const prior =
-            __hip_atomic_fetch_add(&arrivals[pn], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&arrivals[pn], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (prior + 1u < internal_children[pn])
             return;
+        acquire_agent();
         arrivals[pn] = 0u;
         n = pn;
     }

@@ -3,15 +3,17 @@
 // Replaces the CUB DeviceRadixSort::SortPairs calls of the reference build
 // (include/gwn/detail/gwn_bvh_build_common.cuh:325-347, gwn_bvh_reorder.cuh:104-116).
 //
-// Design (onesweep, 8-bit digits):
-//   1. one histogram sweep counts every digit of every pass at once;
-//   2. one scan turns the histograms into per-pass digit offsets;
-//   3. one kernel per pass: a 256-thread block (four wave64s) ranks a 4096-key tile with
-//      wave-wide ballot "match" masks (stable: rank order = input order), resolves its global
-//      digit offsets with decoupled look-back over the preceding tiles, stages the tile in LDS
-//      in digit order and writes it out so that equal-digit runs are contiguous stores.
-// Tile IDs come from an atomic counter, so every tile a block waits on belongs to a block that
-// has already started: the look-back always terminates.
+// Design (reduce-then-scan, 8-bit digits, 4096-key tiles of one 256-thread block):
+//   1. one histogram sweep counts every digit of every pass at once, and one scan turns the
+//      counts into per-pass digit bases;
+//   2. per pass, a count kernel records each tile's digit counts (digit-major table), a scan
+//      kernel (one block per digit) turns each digit's row into global tile offsets, and a
+//      scatter kernel ranks the tile with wave-wide ballot "match" masks (stable: rank order =
+//      input order), stages it in LDS in digit order and writes equal-digit runs contiguously.
+// No workgroup waits on another: every cross-tile dependency is a kernel boundary.  The table
+// costs 1 KiB per tile and one extra read of the keys per pass; a single-pass look-back design
+// serializes on one status round trip per tile, which on a multi-XCD part costs more than the
+// whole pass's traffic.
 #include "gwn_core.hpp"
 
 namespace gwn {
@@ -24,9 +26,7 @@ constexpr int k_items = 16;
 constexpr int k_tile = k_sort_block * k_items;
 constexpr int k_max_passes = 8;
 
-constexpr std::uint32_t k_flag_aggregate = 1u << 30;
-constexpr std::uint32_t k_flag_inclusive = 2u << 30;
-constexpr std::uint32_t k_value_mask = (1u << 30) - 1u;
+constexpr std::size_t k_max_sort_count = (std::size_t(1) << 31) - 1u;
 
 __global__ void fill_u32_kernel(std::uint32_t *p, std::uint32_t v, std::size_t n) {
     std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -100,53 +100,17 @@ __global__ __launch_bounds__(k_sort_block) void scan_histogram_kernel(std::uint3
     }
 }
 
-template <class Key> struct onesweep_smem {
-    Key keys[k_tile];
-    std::uint32_t vals[k_tile];
-    std::uint32_t wave_count[k_sort_waves][k_radix];
-    std::uint32_t tile_start[k_radix];
-    std::uint32_t digit_base[k_radix];
-    std::uint32_t wave_totals[k_sort_waves];
-    std::uint32_t tile_id;
-};
-
+// Stable in-wave ranking of the tile's items.  Item `it` of lane l is element
+// wave*64*k_items + it*64 + l of the tile, so ranking items in (it, lane) order ranks elements
+// in input order.  For each item, `match` is the set of lanes holding the same digit; the lowest
+// such lane advances the wave's digit counter.
 template <class Key>
-__global__ __launch_bounds__(k_sort_block) void onesweep_kernel(
-    Key const *__restrict__ keys_in, Key *__restrict__ keys_out,
-    std::uint32_t const *__restrict__ vals_in, std::uint32_t *__restrict__ vals_out, std::size_t n,
-    int shift, unsigned digit_mask, std::uint32_t const *__restrict__ pass_offsets,
-    std::uint32_t *tile_status, std::uint32_t *tile_counter
+__device__ inline void rank_tile_items(
+    Key const (&key)[k_items], int tile_count, int wave_base, int shift, unsigned digit_mask,
+    std::uint32_t *wave_count, std::uint32_t (&rank)[k_items]
 ) {
-    __shared__ onesweep_smem<Key> s;
     int const lane = threadIdx.x & 63;
-    int const wave = threadIdx.x >> 6;
-
-    if (threadIdx.x == 0)
-        s.tile_id = atomicAdd(tile_counter, 1u);
-    for (int i = threadIdx.x; i < k_sort_waves * k_radix; i += k_sort_block)
-        (&s.wave_count[0][0])[i] = 0u;
-    __syncthreads();
-    std::uint32_t const tile = s.tile_id;
-    std::size_t const tile_begin = static_cast<std::size_t>(tile) * k_tile;
-    if (tile_begin >= n)
-        return;
-    std::size_t const remaining = n - tile_begin;
-    int const tile_count = remaining < static_cast<std::size_t>(k_tile) ? static_cast<int>(remaining) : k_tile;
-
-    // Wave-striped load: item i of lane l is element wave*64*k_items + i*64 + l of the tile, so
-    // ranking items in (i, lane) order ranks elements in input order (stability).
-    Key key[k_items];
-    std::uint32_t val[k_items];
-    std::uint32_t rank[k_items];
-    int const wave_base = wave * 64 * k_items;
     std::uint64_t const lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll
-    for (int it = 0; it < k_items; ++it) {
-        int const local = wave_base + it * 64 + lane;
-        bool const valid = local < tile_count;
-        key[it] = valid ? keys_in[tile_begin + local] : Key(0);
-        val[it] = valid ? vals_in[tile_begin + local] : 0u;
-    }
 #pragma unroll
     for (int it = 0; it < k_items; ++it) {
         int const local = wave_base + it * 64 + lane;
@@ -161,17 +125,108 @@ __global__ __launch_bounds__(k_sort_block) void onesweep_kernel(
         }
         std::uint32_t r = 0;
         if (valid) {
-            std::uint32_t const old = s.wave_count[wave][d];
+            std::uint32_t const old = wave_count[d];
             r = old + static_cast<std::uint32_t>(__popcll(match & lt_mask));
             int const leader = __ffsll(static_cast<long long>(match)) - 1;
             if (lane == leader)
-                s.wave_count[wave][d] = old + static_cast<std::uint32_t>(__popcll(match));
+                wave_count[d] = old + static_cast<std::uint32_t>(__popcll(match));
         }
         rank[it] = r;
     }
+}
+
+// counts[d * tiles + t] = number of keys of tile t whose digit is d.
+template <class Key>
+__global__ __launch_bounds__(k_sort_block) void tile_count_kernel(
+    Key const *__restrict__ keys, std::size_t n, int shift, unsigned digit_mask, std::uint32_t tiles,
+    std::uint32_t *__restrict__ counts
+) {
+    __shared__ std::uint32_t s_count[k_sort_waves][k_radix];
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < k_sort_waves * k_radix; i += k_sort_block)
+        (&s_count[0][0])[i] = 0u;
+    __syncthreads();
+    std::size_t const tile_begin = static_cast<std::size_t>(blockIdx.x) * k_tile;
+    std::size_t const remaining = n - tile_begin;
+    int const tile_count = remaining < static_cast<std::size_t>(k_tile) ? static_cast<int>(remaining) : k_tile;
+    int const wave_base = wave * 64 * k_items;
+    Key key[k_items];
+#pragma unroll
+    for (int it = 0; it < k_items; ++it) {
+        int const local = wave_base + it * 64 + lane;
+        key[it] = local < tile_count ? keys[tile_begin + local] : Key(0);
+    }
+    std::uint32_t rank[k_items];
+    rank_tile_items<Key>(key, tile_count, wave_base, shift, digit_mask, s_count[wave], rank);
+    __syncthreads();
+    std::uint32_t total = 0;
+#pragma unroll
+    for (int w = 0; w < k_sort_waves; ++w)
+        total += s_count[w][threadIdx.x];
+    counts[static_cast<std::size_t>(threadIdx.x) * tiles + blockIdx.x] = total;
+}
+
+// One block per digit: row d of the count table becomes the global output offset of each
+// tile's run of digit d (digit base + exclusive prefix over earlier tiles).
+__global__ __launch_bounds__(k_sort_block) void tile_scan_kernel(
+    std::uint32_t *__restrict__ counts, std::uint32_t tiles, std::uint32_t const *__restrict__ digit_base
+) {
+    __shared__ std::uint32_t s_tot[k_sort_waves];
+    std::uint32_t *row = counts + static_cast<std::size_t>(blockIdx.x) * tiles;
+    std::uint32_t carry = digit_base[blockIdx.x];
+    for (std::uint32_t c = 0; c < tiles; c += k_sort_block) {
+        std::uint32_t const i = c + threadIdx.x;
+        std::uint32_t const v = i < tiles ? row[i] : 0u;
+        std::uint32_t total;
+        std::uint32_t const e = block_exclusive_scan(v, s_tot, total);
+        if (i < tiles)
+            row[i] = carry + e;
+        carry += total;
+    }
+}
+
+template <class Key> struct scatter_smem {
+    Key keys[k_tile];
+    std::uint32_t vals[k_tile];
+    std::uint32_t wave_count[k_sort_waves][k_radix];
+    std::uint32_t tile_start[k_radix];
+    std::uint32_t digit_base[k_radix];
+    std::uint32_t wave_totals[k_sort_waves];
+};
+
+template <class Key>
+__global__ __launch_bounds__(k_sort_block) void scatter_kernel(
+    Key const *__restrict__ keys_in, Key *__restrict__ keys_out,
+    std::uint32_t const *__restrict__ vals_in, std::uint32_t *__restrict__ vals_out, std::size_t n,
+    int shift, unsigned digit_mask, std::uint32_t tiles, std::uint32_t const *__restrict__ offsets
+) {
+    __shared__ scatter_smem<Key> s;
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < k_sort_waves * k_radix; i += k_sort_block)
+        (&s.wave_count[0][0])[i] = 0u;
+    __syncthreads();
+    std::uint32_t const tile = blockIdx.x;
+    std::size_t const tile_begin = static_cast<std::size_t>(tile) * k_tile;
+    std::size_t const remaining = n - tile_begin;
+    int const tile_count = remaining < static_cast<std::size_t>(k_tile) ? static_cast<int>(remaining) : k_tile;
+
+    Key key[k_items];
+    std::uint32_t val[k_items];
+    std::uint32_t rank[k_items];
+    int const wave_base = wave * 64 * k_items;
+#pragma unroll
+    for (int it = 0; it < k_items; ++it) {
+        int const local = wave_base + it * 64 + lane;
+        bool const valid = local < tile_count;
+        key[it] = valid ? keys_in[tile_begin + local] : Key(0);
+        val[it] = valid ? vals_in[tile_begin + local] : 0u;
+    }
+    rank_tile_items<Key>(key, tile_count, wave_base, shift, digit_mask, s.wave_count[wave], rank);
     __syncthreads();
 
-    // One thread per digit: wave prefixes, tile aggregate, decoupled look-back.
+    // One thread per digit: per-wave prefixes and the tile's digit total.
     unsigned const digit = threadIdx.x;
     std::uint32_t tile_total = 0;
 #pragma unroll
@@ -180,42 +235,13 @@ __global__ __launch_bounds__(k_sort_block) void onesweep_kernel(
         s.wave_count[w][digit] = tile_total;
         tile_total += c;
     }
-    std::uint32_t *my_status = tile_status + static_cast<std::size_t>(tile) * k_radix + digit;
-    std::uint32_t exclusive = 0;
-    if (tile == 0) {
-        __hip_atomic_store(my_status, k_flag_inclusive | tile_total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        __hip_atomic_store(my_status, k_flag_aggregate | tile_total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        std::int64_t pred = static_cast<std::int64_t>(tile) - 1;
-        std::uint32_t spins = 0;
-        while (pred >= 0) {
-            std::uint32_t const st = __hip_atomic_load(
-                tile_status + static_cast<std::size_t>(pred) * k_radix + digit, __ATOMIC_ACQUIRE,
-                __HIP_MEMORY_SCOPE_AGENT
-            );
-            std::uint32_t const flag = st & ~k_value_mask;
-            if (flag == 0u) {
-                // Every predecessor tile publishes its aggregate before its own look-back, so
-                // this wait ends; the cap only bounds the kernel if that invariant were broken.
-                if (++spins > (1u << 24))
-                    break;
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            exclusive += st & k_value_mask;
-            if (flag == k_flag_inclusive)
-                break;
-            --pred;
-        }
-        __hip_atomic_store(my_status, k_flag_inclusive | (exclusive + tile_total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
     std::uint32_t block_total;
     std::uint32_t const tile_start = block_exclusive_scan(tile_total, s.wave_totals, block_total);
     s.tile_start[digit] = tile_start;
-    s.digit_base[digit] = pass_offsets[digit] + exclusive - tile_start;
+    s.digit_base[digit] = offsets[static_cast<std::size_t>(digit) * tiles + tile] - tile_start;
     __syncthreads();
 
-    // Stage the tile in digit order.
+    // Stage the tile in digit order, then write each digit run contiguously.
 #pragma unroll
     for (int it = 0; it < k_items; ++it) {
         int const local = wave_base + it * 64 + lane;
@@ -245,20 +271,19 @@ void radix_sort_pairs(
     *vals_out = vals;
     if (n <= 1 || end_bit <= begin_bit)
         return;
-    if (n > static_cast<std::size_t>(k_value_mask))
-        throw_invalid("radix sort: element count exceeds 2^30 - 1.");
+    if (n > k_max_sort_count)
+        throw_invalid("radix sort: element count exceeds 2^31 - 1.");
     int const passes = (end_bit - begin_bit + 7) / 8;
     if (passes > k_max_passes)
         throw_invalid("radix sort: bit range exceeds the key width.");
     std::size_t const tiles = (n + k_tile - 1) / k_tile;
-    // Scratch: histograms [passes][256], then per pass a tile counter and tile status words.
-    std::size_t const status_words = tiles * k_radix;
-    std::size_t const scratch_words = static_cast<std::size_t>(passes) * k_radix +
-                                      static_cast<std::size_t>(passes) * (1 + status_words);
-    device_buffer<std::uint32_t> scratch(scratch_words, stream);
-    fill_u32(scratch.ptr, 0u, scratch_words, stream);
+    // Scratch: digit bases [passes][256], then the [256][tiles] tile table reused by every pass.
+    std::size_t const table_words = tiles * k_radix;
+    std::size_t const base_words = static_cast<std::size_t>(passes) * k_radix;
+    device_buffer<std::uint32_t> scratch(base_words + table_words, stream);
+    fill_u32(scratch.ptr, 0u, base_words, stream);
     std::uint32_t *hist = scratch.ptr;
-    std::uint32_t *pass_state = scratch.ptr + static_cast<std::size_t>(passes) * k_radix;
+    std::uint32_t *table = scratch.ptr + base_words;
 
     unsigned const hist_grid = static_cast<unsigned>(tiles < 2048 ? tiles : 2048);
     histogram_kernel<Key><<<hist_grid, k_sort_block, 0, stream>>>(keys, n, begin_bit, end_bit, passes, hist);
@@ -268,13 +293,16 @@ void radix_sort_pairs(
 
     Key *kin = keys, *kout = keys_alt;
     std::uint32_t *vin = vals, *vout = vals_alt;
+    auto const ntiles = static_cast<std::uint32_t>(tiles);
     for (int p = 0; p < passes; ++p) {
-        std::uint32_t *counter = pass_state + static_cast<std::size_t>(p) * (1 + status_words);
         int const shift = begin_bit + 8 * p;
         int const width = end_bit - shift < 8 ? end_bit - shift : 8;
-        onesweep_kernel<Key><<<static_cast<unsigned>(tiles), k_sort_block, 0, stream>>>(
-            kin, kout, vin, vout, n, shift, (1u << width) - 1u, hist + p * k_radix, counter + 1, counter
-        );
+        unsigned const mask = (1u << width) - 1u;
+        tile_count_kernel<Key><<<ntiles, k_sort_block, 0, stream>>>(kin, n, shift, mask, ntiles, table);
+        GWN_LAUNCH_CHECK();
+        tile_scan_kernel<<<k_radix, k_sort_block, 0, stream>>>(table, ntiles, hist + p * k_radix);
+        GWN_LAUNCH_CHECK();
+        scatter_kernel<Key><<<ntiles, k_sort_block, 0, stream>>>(kin, kout, vin, vout, n, shift, mask, ntiles, table);
         GWN_LAUNCH_CHECK();
         std::swap(kin, kout);
         std::swap(vin, vout);

@@ -53,28 +53,55 @@ __device__ inline float solid_angle_plain(tri_record const &t, float qx, float q
     return 2.0f * atan2f(num, den);
 }
 
-// FP64 solid angle on the exact vertices.
+// Solid angle without normalizing the corner vectors: with a = v0 - q, b = v1 - q, c = v2 - q,
+// tan(omega / 2) = a . (e1 x e2) / (|a||b||c| + (a.b)|c| + (a.c)|b| + (b.c)|a|), which equals the
+// reference's normalized form (divide numerator and denominator by |a||b||c|; qa . ((qb - qa) x
+// (qc - qa)) = qa . (qb x qc)).  The numerator uses the edges e1 = v1 - v0, e2 = v2 - v0, which do
+// not depend on q, so a far triangle's nearly parallel corner vectors never cancel.
+template <class R>
+__device__ inline R solid_angle_edges(R ax, R ay, R az, R e1x, R e1y, R e1z, R e2x, R e2y, R e2z) {
+    R const bx = ax + e1x, by = ay + e1y, bz = az + e1z;
+    R const cx = ax + e2x, cy = ay + e2y, cz = az + e2z;
+    R const la = sqrt(ax * ax + ay * ay + az * az);
+    R const lb = sqrt(bx * bx + by * by + bz * bz);
+    R const lc = sqrt(cx * cx + cy * cy + cz * cz);
+    if (la == R(0) || lb == R(0) || lc == R(0))
+        return R(0);
+    R const nx = e1y * e2z - e1z * e2y;
+    R const ny = e1z * e2x - e1x * e2z;
+    R const nz = e1x * e2y - e1y * e2x;
+    R const num = ax * nx + ay * ny + az * nz;
+    if (num == R(0))
+        return R(0);
+    R const den = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + (ax * cx + ay * cy + az * cz) * lb +
+                  (bx * cx + by * cy + bz * cz) * la;
+    return R(2) * atan2(num, den);
+}
+
+// FP64 solid angle on the exact vertices (float inputs convert exactly, so e1 and e2 are exact).
 __device__ inline double solid_angle_f64(tri_record const &t, double qx, double qy, double qz) {
-    double ax = double(t.v0x) - qx, ay = double(t.v0y) - qy, az = double(t.v0z) - qz;
-    double bx = double(t.v1x) - qx, by = double(t.v1y) - qy, bz = double(t.v1z) - qz;
-    double cx = double(t.v2x) - qx, cy = double(t.v2y) - qy, cz = double(t.v2z) - qz;
-    double const al = sqrt(ax * ax + ay * ay + az * az);
-    double const bl = sqrt(bx * bx + by * by + bz * bz);
-    double const cl = sqrt(cx * cx + cy * cy + cz * cz);
-    if (al == 0.0 || bl == 0.0 || cl == 0.0)
-        return 0.0;
-    double const ia = 1.0 / al, ib = 1.0 / bl, ic = 1.0 / cl;
-    ax *= ia; ay *= ia; az *= ia;
-    bx *= ib; by *= ib; bz *= ib;
-    cx *= ic; cy *= ic; cz *= ic;
-    double const ux = bx - ax, uy = by - ay, uz = bz - az;
-    double const vx = cx - ax, vy = cy - ay, vz = cz - az;
-    double const num = ax * (uy * vz - uz * vy) + ay * (uz * vx - ux * vz) + az * (ux * vy - uy * vx);
-    if (num == 0.0)
-        return 0.0;
-    double const den = 1.0 + (ax * bx + ay * by + az * bz) + (ax * cx + ay * cy + az * cz) +
-                       (bx * cx + by * cy + bz * cz);
-    return 2.0 * atan2(num, den);
+    return solid_angle_edges<double>(
+        double(t.v0x) - qx, double(t.v0y) - qy, double(t.v0z) - qz, double(t.v1x) - double(t.v0x),
+        double(t.v1y) - double(t.v0y), double(t.v1z) - double(t.v0z), double(t.v2x) - double(t.v0x),
+        double(t.v2y) - double(t.v0y), double(t.v2z) - double(t.v0z)
+    );
+}
+
+// FP32 far-field solid angle (precision split): same form with hardware square roots.
+__device__ inline float solid_angle_far_f32(float ax, float ay, float az, float e1x, float e1y, float e1z,
+                                            float e2x, float e2y, float e2z) {
+    float const bx = ax + e1x, by = ay + e1y, bz = az + e1z;
+    float const cx = ax + e2x, cy = ay + e2y, cz = az + e2z;
+    float const la = __builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az);
+    float const lb = __builtin_amdgcn_sqrtf(bx * bx + by * by + bz * bz);
+    float const lc = __builtin_amdgcn_sqrtf(cx * cx + cy * cy + cz * cz);
+    float const nx = e1y * e2z - e1z * e2y;
+    float const ny = e1z * e2x - e1x * e2z;
+    float const nz = e1x * e2y - e1y * e2x;
+    float const num = ax * nx + ay * ny + az * nz;
+    float const den = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + (ax * cx + ay * cy + az * cz) * lb +
+                      (bx * cx + by * cy + bz * cz) * la;
+    return 2.0f * atan2f(num, den);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -116,18 +143,20 @@ __global__ __launch_bounds__(k_exact_block) void exact_kernel(
             for (std::size_t k = 0; k < count; ++k)
                 sum_d += solid_angle_f64(tile[k], x, y, z);
         } else {
+            // Per-tile FP32 partial sums of the far triangles, folded into the FP64 total.
             float far = 0.0f;
             for (std::size_t k = 0; k < count; ++k) {
                 tri_record const &t = tile[k];
-                float const e1 = (t.v1x - t.v0x) * (t.v1x - t.v0x) + (t.v1y - t.v0y) * (t.v1y - t.v0y) +
-                                 (t.v1z - t.v0z) * (t.v1z - t.v0z);
-                float const e2 = (t.v2x - t.v0x) * (t.v2x - t.v0x) + (t.v2y - t.v0y) * (t.v2y - t.v0y) +
-                                 (t.v2z - t.v0z) * (t.v2z - t.v0z);
-                float const dq = (x - t.v0x) * (x - t.v0x) + (y - t.v0y) * (y - t.v0y) + (z - t.v0z) * (z - t.v0z);
+                float const e1x = t.v1x - t.v0x, e1y = t.v1y - t.v0y, e1z = t.v1z - t.v0z;
+                float const e2x = t.v2x - t.v0x, e2y = t.v2y - t.v0y, e2z = t.v2z - t.v0z;
+                float const ax = t.v0x - x, ay = t.v0y - y, az = t.v0z - z;
+                float const e1 = e1x * e1x + e1y * e1y + e1z * e1z;
+                float const e2 = e2x * e2x + e2y * e2y + e2z * e2z;
+                float const dq = ax * ax + ay * ay + az * az;
                 if (!(dq > k_split_near_scale2 * fmaxf(e1, e2)))
                     sum_d += solid_angle_f64(t, x, y, z);
                 else
-                    far += solid_angle_plain(t, x, y, z);
+                    far += solid_angle_far_f32(ax, ay, az, e1x, e1y, e1z, e2x, e2y, e2z);
             }
             sum_d += double(far);
         }
@@ -189,7 +218,7 @@ __device__ inline float product_unfused(float a, float b) {
 template <int Order>
 __device__ inline float taylor_term(float const *f, int s, float qrx, float qry, float qrz, float ql2) {
 #define GWN_F(k) f[(k) * 4 + s]
-    float const m1 = __frsqrt_rn(ql2);
+    float const m1 = __builtin_amdgcn_rsqf(ql2);
     float const m2 = m1 * m1;
     float const m3 = m2 * m1;
     float const qnx = qrx * m1, qny = qry * m1, qnz = qrz * m1;
@@ -316,6 +345,134 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
     }
 }
 
+// Wave-cooperative traversal for spatially sorted queries.  The 64 lanes of a wave walk the union
+// of their traversal paths together: the wave-uniform stack holds (node, lane mask) pairs, every
+// node record is fetched once per wave from a wave-uniform address, and each lane only tests the
+// children of nodes its own traversal reaches.  A lane's descended internal children are pushed
+// and entered in the same slot / last-in-first-out order as in the per-lane kernel, so every lane
+// forms its sum over exactly the same terms in exactly the same order; the stack depth is bounded
+// by internal_stack_bound for the same reason.
+template <int Order, int Prec, int StackCap>
+__global__ __launch_bounds__(k_taylor_block) void taylor_wave_kernel(
+    float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node,
+    float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
+    std::uint32_t const *__restrict__ order, float *__restrict__ out, std::size_t n, float beta2,
+    std::uint32_t *__restrict__ counters
+) {
+    constexpr int RV = moment_record_vec4(Order);
+    constexpr int k_waves = k_taylor_block / 64;
+    __shared__ std::uint32_t s_node[k_waves][StackCap];
+    __shared__ std::uint64_t s_mask[k_waves][StackCap];
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_taylor_block + threadIdx.x;
+    bool const active = i < n;
+    std::size_t const q = active ? (order ? order[i] : i) : 0;
+    float const x = active ? qx[q] : 0.0f;
+    float const y = active ? qy[q] : 0.0f;
+    float const z = active ? qz[q] : 0.0f;
+    std::uint64_t mask = __ballot(active);
+    if (mask == 0ull)
+        return;
+    std::uint64_t const my_bit = 1ull << lane;
+    float omega = 0.0f;
+    double near = 0.0;
+    std::uint32_t n_tests = 0, n_accept = 0, n_leaf = 0;
+    std::uint32_t node = root_node;
+    int sp = 0;
+    bool overflow = false;
+    for (;;) {
+        node = __builtin_amdgcn_readfirstlane(node);
+        float4 const *rec = records + static_cast<std::size_t>(node) * RV;
+        float4 const refs = rec[0];
+        float4 const mp = rec[1], px = rec[2], py = rec[3], pz = rec[4];
+        float const *fields = reinterpret_cast<float const *>(rec + 1);
+        std::uint32_t const cref[4] = {__float_as_uint(refs.x), __float_as_uint(refs.y),
+                                       __float_as_uint(refs.z), __float_as_uint(refs.w)};
+        float const mpd[4] = {mp.x, mp.y, mp.z, mp.w};
+        float const cx[4] = {px.x, px.y, px.z, px.w};
+        float const cy[4] = {py.x, py.y, py.z, py.w};
+        float const cz[4] = {pz.x, pz.y, pz.z, pz.w};
+        bool const in = (mask & my_bit) != 0ull;
+        std::uint32_t next = k_invalid;
+        std::uint64_t next_mask = 0ull;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            std::uint32_t const c = __builtin_amdgcn_readfirstlane(cref[s]);
+            if (c == k_invalid)
+                continue;
+            bool descend = false;
+            if (in) {
+                ++n_tests;
+                float const qrx = x - cx[s], qry = y - cy[s], qrz = z - cz[s];
+                float const ql2 = squared_norm_unfused(qrx, qry, qrz);
+                descend = !(ql2 > 0.0f) || ql2 <= product_unfused(mpd[s], beta2);
+                if (!descend) {
+                    float const w = taylor_term<Order>(fields, s, qrx, qry, qrz, ql2);
+                    if (isfinite(w)) {
+                        omega += w;
+                        ++n_accept;
+                    } else {
+                        descend = true;
+                    }
+                }
+            }
+            std::uint64_t const dmask = __ballot(descend);
+            if (dmask == 0ull)
+                continue;
+            if (c & k_cref_leaf) {
+                tri_record const t = tris[c & ~k_cref_leaf];
+                if (dmask & my_bit) {
+                    ++n_leaf;
+                    if constexpr (Prec == GWN_PRECISION_PLAIN)
+                        omega += solid_angle_plain(t, x, y, z);
+                    else
+                        near += solid_angle_f64(t, x, y, z);
+                }
+                continue;
+            }
+            if (next != k_invalid) {
+                if (sp >= StackCap) {
+                    overflow = true;
+                    break;
+                }
+                s_node[wave][sp] = next;
+                s_mask[wave][sp] = next_mask;
+                ++sp;
+            }
+            next = c;
+            next_mask = dmask;
+        }
+        if (overflow)
+            break;
+        if (next != k_invalid) {
+            node = next;
+            mask = next_mask;
+            continue;
+        }
+        if (sp == 0)
+            break;
+        --sp;
+        node = s_node[wave][sp];
+        mask = s_mask[wave][sp];
+    }
+    if (!active)
+        return;
+    float result;
+    if (overflow)
+        result = __builtin_nanf("");
+    else if constexpr (Prec == GWN_PRECISION_PLAIN)
+        result = omega / (4.0f * k_pi_f);
+    else
+        result = static_cast<float>((double(omega) + near) / (4.0 * k_pi_d));
+    out[q] = result;
+    if (counters != nullptr) {
+        counters[3 * q + 0] = n_tests;
+        counters[3 * q + 1] = n_accept;
+        counters[3 * q + 2] = n_leaf;
+    }
+}
+
 template <int Prec>
 __global__ __launch_bounds__(256) void leaf_root_kernel(
     tri_record const *__restrict__ tris, std::uint64_t offset, std::uint32_t count,
@@ -351,10 +508,17 @@ void launch_taylor_cap(
     float const *qz, std::uint32_t const *order, float *out, std::size_t n, float beta2,
     std::uint32_t *counters, hipStream_t stream
 ) {
-    taylor_kernel<Order, Prec, Cap><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(
-        m.records.ptr, bvh.triangles.ptr, static_cast<std::uint32_t>(ref_offset(bvh.root.reference)), qx, qy,
-        qz, order, out, n, beta2, counters
-    );
+    std::uint32_t const root = static_cast<std::uint32_t>(ref_offset(bvh.root.reference));
+    if (order != nullptr) {
+        // Morton-ordered queries: neighbouring lanes share most of their paths.
+        taylor_wave_kernel<Order, Prec, Cap><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(
+            m.records.ptr, bvh.triangles.ptr, root, qx, qy, qz, order, out, n, beta2, counters
+        );
+    } else {
+        taylor_kernel<Order, Prec, Cap><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(
+            m.records.ptr, bvh.triangles.ptr, root, qx, qy, qz, order, out, n, beta2, counters
+        );
+    }
     GWN_LAUNCH_CHECK();
 }
 

@@ -30,7 +30,8 @@ EXPORTED_SYMBOLS = (
     "gwn_abi_version", "gwn_last_error_message",
     "gwn_geometry_create", "gwn_geometry_destroy", "gwn_upload_geometry_f32",
     "gwn_update_geometry_f32", "gwn_update_geometry_device_f32", "gwn_geometry_counts",
-    "gwn_bvh_create", "gwn_bvh_destroy", "gwn_build_bvh4_f32", "gwn_bvh_get_info", "gwn_bvh_download",
+    "gwn_bvh_create", "gwn_bvh_destroy", "gwn_build_bvh4_f32", "gwn_refit_bvh_f32", "gwn_bvh_get_info",
+    "gwn_bvh_download",
     "gwn_moment_create", "gwn_moment_destroy", "gwn_refit_bvh_moment_f32", "gwn_moment_download",
     "gwn_moment_order",
     "gwn_compute_winding_number_exact_batch_f32", "gwn_compute_winding_number_taylor_batch_f32",
@@ -135,6 +136,7 @@ def _load():
         "gwn_bvh_create": [vp],
         "gwn_bvh_destroy": [vp],
         "gwn_build_bvh4_f32": [vp, vp, i32, u32, vp],
+        "gwn_refit_bvh_f32": [vp, vp, vp],
         "gwn_bvh_get_info": [vp, vp],
         "gwn_bvh_download": [vp, vp, vp, vp, vp],
         "gwn_moment_create": [vp, i32],
@@ -343,6 +345,12 @@ def gwn_build_bvh(geometry: GeometryObject, bvh: BvhObject, options: BuildOption
     """gwn_build_bvh<4, float, uint32_t> (include/gwn/gwn_bvh_build.cuh:34-39)."""
     return _status(_lib.gwn_build_bvh4_f32(
         geometry.handle, bvh.handle, int(options.method), int(options.hploc_search_radius), _stream(stream)))
+
+
+def gwn_refit_bvh(geometry: GeometryObject, bvh: BvhObject, stream=None) -> Status:
+    """gwn_refit_bvh (include/gwn/gwn_bvh_refit.cuh:25-29): records and boxes from the updated
+    geometry, same topology; the BVH gets a new revision, so moments need a refit."""
+    return _status(_lib.gwn_refit_bvh_f32(geometry.handle, bvh.handle, _stream(stream)))
 
 
 def gwn_refit_bvh_moment(bvh: BvhObject, moment: MomentObject, stream=None) -> Status:

@@ -156,3 +156,108 @@ def test_moment_large_mesh_matches_oracle_bitwise(gwn, orc):
         gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
         got = mom.download(bvh.info()["node_count"])
         assert np.array_equal(got.view(np.uint32), orc.refit_moment(ref_bvh, order).view(np.uint32))
+
+
+def test_repeated_builds_and_refits_are_bit_identical(gwn, orc):
+    """Many builds and order-2 refits of one mesh: every one equals the oracle bit for bit."""
+    mesh = MESHES["displaced_256x128"]
+    ref = orc.build_bvh4(mesh)
+    ref_m = orc.refit_moment(ref, 2)
+    geom = gwn.GeometryObject()
+    gwn.gwn_upload_geometry(geom, mesh.vx, mesh.vy, mesh.vz, mesh.i0, mesh.i1, mesh.i2).raise_if_error()
+    for rep in range(40):
+        bvh = gwn.BvhObject()
+        gwn.gwn_build_bvh(geom, bvh, gwn.BuildOptions(method=gwn.BuildMethod.LBVH)).raise_if_error()
+        nodes, prim, tris = bvh.download()
+        assert nodes.tobytes() == ref.nodes.tobytes(), f"build {rep}: nodes differ"
+        assert np.array_equal(prim, ref.primitive_indices), f"build {rep}: primitive order differs"
+        mom = gwn.MomentObject(2)
+        gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+        got = mom.download(bvh.info()["node_count"])
+        assert np.array_equal(got.view(np.uint32), ref_m.view(np.uint32)), f"refit {rep}: moments differ"
+
+
+# ---------------------------------------------------------------------------------------------
+# Bounds refit (gwn_refit_bvh)
+# ---------------------------------------------------------------------------------------------
+
+def _deformed(mesh, seed):
+    from tests.test_oracle import _deform
+
+    return _deform(mesh, seed)
+
+
+def test_refit_bvh_kat(gwn, orc):
+    """tests/unit_gwn_bvh.cu:220-287 through the C ABI: root (-7,-4,-6)-(11,9,3), primitive order
+    preserved, records from the new positions."""
+    x, y, z = [0, 1, 0, 4, 6, 4], [0, 0, 2, 0, 0, 3], [1, 1, 1, -2, -2, -2]
+    faces = [[0, 1, 2], [3, 4, 5]]
+    mesh = synth._mesh_from_arrays(np.stack([x, y, z], 1), faces)
+    geom, bvh = build_gpu(gwn, mesh)
+    _, prim_before, _ = bvh.download()
+    moved = synth._mesh_from_arrays(
+        np.stack([[-7, -2, -7, 8, 11, 8], [5, 5, 9, -4, -4, 2], [3, 3, 3, -6, -6, -6]], 1), faces)
+    rev = bvh.info()["revision"]
+    gwn.gwn_update_geometry(geom, moved.vx, moved.vy, moved.vz).raise_if_error()
+    gwn.gwn_refit_bvh(geom, bvh).raise_if_error()
+    info = bvh.info()
+    assert info["root_bounds"].tolist() == [-7, -4, -6, 11, 9, 3]
+    assert info["revision"] > rev
+    nodes, prim, tris = bvh.download()
+    assert np.array_equal(prim, prim_before)
+    ref = orc.refit_bvh(orc.build_bvh4(mesh), moved)
+    assert nodes.tobytes() == ref.nodes.tobytes()
+    assert np.array_equal(tris.view(np.uint32), ref.triangles.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["single", "icosphere4", "torus", "soup", "displaced_256x128"])
+def test_refit_bvh_matches_oracle(gwn, orc, name):
+    mesh = MESHES[name]
+    geom, bvh = build_gpu(gwn, mesh)
+    moved = _deformed(mesh, 11)
+    gwn.gwn_update_geometry(geom, moved.vx, moved.vy, moved.vz).raise_if_error()
+    gwn.gwn_refit_bvh(geom, bvh).raise_if_error()
+    ref = orc.refit_bvh(orc.build_bvh4(mesh), moved)
+    info = bvh.info()
+    nodes, prim, tris = bvh.download()
+    assert nodes.tobytes() == ref.nodes.tobytes()
+    assert np.array_equal(tris.view(np.uint32), ref.triangles.view(np.uint32))
+    assert np.array_equal(info["root_bounds"].view(np.uint32), ref.root_bounds.view(np.uint32))
+    assert info["internal_stack_bound"] == ref.internal_stack_bound
+    helpers.check_bvh4(nodes, info["root_reference"], prim, tris, moved,
+                       ref.internal_stack_bound, ref.packed_stack_bound)
+
+
+def test_refit_bvh_then_moment_refit_equals_fresh_state(gwn, orc):
+    """Refit to new positions, refit moments: bit-identical to the oracle's refit + moments, and
+    a stale moment (formed before the refit) is rejected by the Taylor batch."""
+    torch = _torch()
+    mesh = MESHES["displaced_256x128"]
+    geom, bvh = build_gpu(gwn, mesh)
+    stale = gwn.MomentObject(1)
+    gwn.gwn_refit_bvh_moment(bvh, stale).raise_if_error()
+    moved = _deformed(mesh, 3)
+    gwn.gwn_update_geometry(geom, moved.vx, moved.vy, moved.vz).raise_if_error()
+    gwn.gwn_refit_bvh(geom, bvh).raise_if_error()
+    q = [torch.zeros(8, device="cuda") for _ in range(3)]
+    out = torch.empty(8, device="cuda")
+    st = gwn.gwn_compute_winding_number_taylor_batch(bvh, stale, *q, out)
+    assert st.error == gwn.Error.INVALID_ARGUMENT and "not aligned" in st.message
+    mom = gwn.MomentObject(1)
+    gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+    ref = orc.refit_bvh(orc.build_bvh4(mesh), moved)
+    assert np.array_equal(mom.download(bvh.info()["node_count"]).view(np.uint32),
+                          orc.refit_moment(ref, 1).view(np.uint32))
+
+
+def test_refit_bvh_validation(gwn):
+    mesh = MESHES["torus"]
+    geom, bvh = build_gpu(gwn, mesh)
+    other = gwn.GeometryObject()
+    small = MESHES["two"]
+    gwn.gwn_upload_geometry(other, small.vx, small.vy, small.vz, small.i0, small.i1, small.i2).raise_if_error()
+    st = gwn.gwn_refit_bvh(other, bvh)
+    assert st.error == gwn.Error.INVALID_ARGUMENT and "triangle count differs" in st.message
+    empty = gwn.BvhObject()
+    st = gwn.gwn_refit_bvh(geom, empty)
+    assert st.error == gwn.Error.INVALID_ARGUMENT and "BVH accessor is invalid" in st.message

@@ -149,6 +149,67 @@ def test_build_two_triangle_kat(orc):
     helpers.check_bvh4(bvh.nodes, bvh.root_reference, bvh.primitive_indices, bvh.triangles, mesh, 0, 1)
 
 
+def _deform(mesh, seed: int):
+    """Smooth deformation of a mesh's vertices, same indexing."""
+    rng = np.random.default_rng(seed)
+    a = rng.normal(size=3).astype(np.float32) * np.float32(0.3)
+    vx = (mesh.vx * np.float32(1.2) + a[0] * np.sin(np.float32(2.0) * mesh.vy)).astype(np.float32)
+    vy = (mesh.vy * np.float32(0.8) + a[1] * np.cos(np.float32(3.0) * mesh.vz)).astype(np.float32)
+    vz = (mesh.vz + a[2] * mesh.vx * mesh.vy).astype(np.float32)
+    return synth.Mesh(vx, vy, vz, mesh.i0, mesh.i1, mesh.i2)
+
+
+def test_refit_bvh_kat(orc):
+    """tests/unit_gwn_bvh.cu:220-287: moved two-triangle mesh -> root (-7,-4,-6)-(11,9,3),
+    primitive order preserved, records taken from the new positions."""
+    x = [0, 1, 0, 4, 6, 4]
+    y = [0, 0, 2, 0, 0, 3]
+    z = [1, 1, 1, -2, -2, -2]
+    faces = [[0, 1, 2], [3, 4, 5]]
+    bvh = orc.build_bvh4(synth._mesh_from_arrays(np.stack([x, y, z], 1), faces))
+    moved = synth._mesh_from_arrays(
+        np.stack([[-7, -2, -7, 8, 11, 8], [5, 5, 9, -4, -4, 2], [3, 3, 3, -6, -6, -6]], 1), faces)
+    out = orc.refit_bvh(bvh, moved)
+    assert out.root_bounds.tolist() == [-7, -4, -6, 11, 9, 3]
+    assert np.array_equal(out.primitive_indices, bvh.primitive_indices)
+    assert out.internal_stack_bound == bvh.internal_stack_bound
+    helpers.check_bvh4(out.nodes, out.root_reference, out.primitive_indices, out.triangles, moved, 0, 1)
+
+
+def test_refit_bvh_leaf_root(orc):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=np.float32)
+    bvh = orc.build_bvh4(synth._mesh_from_arrays(v, [[0, 1, 2]]))
+    out = orc.refit_bvh(bvh, synth._mesh_from_arrays(v * 3 - 1, [[0, 1, 2]]))
+    assert out.root_bounds.tolist() == [-1, -1, -1, 2, 2, -1]
+    assert out.triangles[0].tolist() == (v * 3 - 1).reshape(-1).tolist()
+
+
+@pytest.mark.parametrize("name", ["torus", "icosphere3", "soup"])
+def test_refit_bvh_unchanged_is_identity_and_deformed_is_exact(orc, name):
+    if name == "torus":
+        mesh = synth.torus(24, 12)
+    elif name == "icosphere3":
+        mesh = synth.icosphere(3)
+    else:
+        rng = np.random.default_rng(5)
+        v = rng.normal(size=(500, 3)).astype(np.float32)
+        mesh = synth._mesh_from_arrays(v, rng.integers(0, 500, size=(700, 3)))
+    bvh = orc.build_bvh4(mesh)
+    same = orc.refit_bvh(bvh, mesh)
+    assert same.nodes.tobytes() == bvh.nodes.tobytes()
+    assert np.array_equal(same.root_bounds.view(np.uint32), bvh.root_bounds.view(np.uint32))
+    moved = _deform(mesh, 7)
+    out = orc.refit_bvh(bvh, moved)
+    # Child boxes are exact subtree min/max of the new triangles; topology and stack bounds keep.
+    helpers.check_bvh4(out.nodes, out.root_reference, out.primitive_indices, out.triangles, moved,
+                       bvh.internal_stack_bound, bvh.packed_stack_bound)
+    refs = out.nodes["children"]["reference"]
+    assert np.array_equal(refs, bvh.nodes["children"]["reference"])
+    v = np.stack([moved.vx, moved.vy, moved.vz], 1)
+    assert out.root_bounds[:3].tolist() == v.min(0).tolist() or name == "soup"
+    assert out.root_bounds[3:].tolist() == v.max(0).tolist() or name == "soup"
+
+
 # ---------------------------------------------------------------------------------------------
 # Moments
 # ---------------------------------------------------------------------------------------------
