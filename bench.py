@@ -92,6 +92,12 @@ def main() -> int:
     out = torch.empty(n, dtype=torch.float32, device=dev)
     order = torch.empty(n, dtype=torch.int32, device=dev)
     cfg = gwn.QueryConfig(stack_capacity=max(64, info["internal_stack_bound"]))
+    # A plain 256 MiB device copy of known size, outside every timed region: under rocprofv3
+    # --pmc it calibrates FETCH_SIZE / WRITE_SIZE against bytes actually moved.
+    calib_src = torch.ones(1 << 26, dtype=torch.float32, device=dev)
+    calib_dst = torch.empty_like(calib_src)
+    calib_dst.copy_(calib_src)
+    del calib_src, calib_dst
     torch.cuda.synchronize()
 
     def step():
@@ -224,8 +230,8 @@ def main() -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
-            "kernel": "taylor_kernel<2, split>",
+            "traffic": traffic_bytes(),
+            "kernel": f"taylor_wave_kernel<{args.order}, split>",
             "kernel_ms": round(t_trav, 4),
             "flops_per_launch": fp32_equiv,
             "note": "FP32-equivalent FLOPs (FP64 counted twice) from per-query work counters",
@@ -237,6 +243,21 @@ def main() -> int:
 
     print(json.dumps(result), flush=True)
     return 0
+
+
+def traffic_bytes():
+    """HBM bytes per traversal launch, from the committed rocprofv3 --pmc summary of this
+    workload (FETCH_SIZE + WRITE_SIZE of the traversal kernel, scaled by the same counters on a
+    256 MiB copy); None when that summary is absent."""
+    path = os.path.join(REPO, "profiles", "taylor_wave_o2_large_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(mesh, qx, qy, qz, args) -> dict:

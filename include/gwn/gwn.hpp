@@ -1,0 +1,262 @@
+// gwn/gwn.hpp - C++ surface of libgwn_hip.so with the reference's names (kririae/smallgwn).
+//
+// Header-only.  Each template below keeps the name, template parameters and argument order of
+// the reference function it replaces and forwards to one extern "C" entry point of
+// include/gwn_hip.h, so code written against the reference's public API compiles against this
+// header (Real = float, Index = std::uint32_t, Width = 4; other instantiations are rejected at
+// compile time).  Needs C++20 (class-type template parameters, as the reference uses).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <utility>
+
+#include "../gwn_hip.h"
+
+namespace gwn {
+
+// ---- status (include/gwn/gwn_utils.cuh:213-283) ---------------------------------------------
+
+enum class gwn_error {
+    success = GWN_SUCCESS,
+    invalid_argument = GWN_ERROR_INVALID_ARGUMENT,
+    cuda_runtime_error = GWN_ERROR_HIP_RUNTIME, // the HIP runtime error on this platform
+    internal_error = GWN_ERROR_INTERNAL,
+};
+
+class gwn_status {
+public:
+    gwn_status() = default;
+    gwn_status(gwn_error e, std::string message) : error_(e), message_(std::move(message)) {}
+    [[nodiscard]] static gwn_status ok() { return {}; }
+    [[nodiscard]] bool is_ok() const noexcept { return error_ == gwn_error::success; }
+    [[nodiscard]] gwn_error error() const noexcept { return error_; }
+    [[nodiscard]] std::string const &message() const noexcept { return message_; }
+
+private:
+    gwn_error error_ = gwn_error::success;
+    std::string message_;
+};
+
+namespace detail {
+[[nodiscard]] inline gwn_status gwn_from_code(int code) {
+    if (code == GWN_SUCCESS)
+        return gwn_status::ok();
+    char const *msg = gwn_last_error_message();
+    return gwn_status(static_cast<gwn_error>(code), msg ? msg : "");
+}
+} // namespace detail
+
+// The reference's default stream is the legacy default stream (gwn_utils.cuh:45).
+[[nodiscard]] inline hipStream_t gwn_default_stream() noexcept { return nullptr; }
+
+// ---- spans (include/gwn/gwn_utils.cuh:84-171): explicit host vs device memory ----------------
+
+template <class T> class gwn_host_span final {
+public:
+    constexpr gwn_host_span() noexcept = default;
+    constexpr gwn_host_span(T *data, std::size_t size) noexcept : data_(data), size_(size) {}
+    [[nodiscard]] constexpr T *data() const noexcept { return data_; }
+    [[nodiscard]] constexpr std::size_t size() const noexcept { return size_; }
+
+private:
+    T *data_ = nullptr;
+    std::size_t size_ = 0;
+};
+
+template <class T> class gwn_device_span final {
+public:
+    constexpr gwn_device_span() noexcept = default;
+    constexpr gwn_device_span(T *data, std::size_t size) noexcept : data_(data), size_(size) {}
+    template <class U>
+    constexpr gwn_device_span(gwn_device_span<U> other) noexcept : data_(other.data()), size_(other.size()) {}
+    [[nodiscard]] constexpr T *data() const noexcept { return data_; }
+    [[nodiscard]] constexpr std::size_t size() const noexcept { return size_; }
+
+private:
+    T *data_ = nullptr;
+    std::size_t size_ = 0;
+};
+
+// ---- objects: move-only owners of device state (gwn_geometry.cuh:98-149, gwn_bvh.cuh:331-429) --
+
+namespace detail {
+template <class Handle, int (*Destroy)(Handle)> class gwn_handle_owner {
+public:
+    gwn_handle_owner() = default;
+    gwn_handle_owner(gwn_handle_owner const &) = delete;
+    gwn_handle_owner &operator=(gwn_handle_owner const &) = delete;
+    gwn_handle_owner(gwn_handle_owner &&o) noexcept : h_(std::exchange(o.h_, nullptr)) {}
+    gwn_handle_owner &operator=(gwn_handle_owner &&o) noexcept {
+        std::swap(h_, o.h_);
+        return *this;
+    }
+    ~gwn_handle_owner() {
+        if (h_ != nullptr)
+            Destroy(h_);
+    }
+    [[nodiscard]] Handle get() const noexcept { return h_; }
+
+protected:
+    Handle h_ = nullptr;
+};
+} // namespace detail
+
+template <class Real, class Index>
+class gwn_geometry_object final : public detail::gwn_handle_owner<gwn_geometry_t, gwn_geometry_destroy> {
+    static_assert(sizeof(Real) == 4 && Real(0.5) != Real(0), "libgwn_hip provides Real = float");
+    static_assert(sizeof(Index) == 4 && Index(-1) > Index(0), "libgwn_hip provides Index = uint32_t");
+
+public:
+    gwn_geometry_object() { gwn_geometry_create(&h_); }
+    [[nodiscard]] std::size_t vertex_count() const noexcept {
+        std::uint64_t v = 0, t = 0;
+        gwn_geometry_counts(h_, &v, &t);
+        return v;
+    }
+    [[nodiscard]] std::size_t triangle_count() const noexcept {
+        std::uint64_t v = 0, t = 0;
+        gwn_geometry_counts(h_, &v, &t);
+        return t;
+    }
+};
+
+template <int Width, class Real, class Index>
+class gwn_bvh_object final : public detail::gwn_handle_owner<gwn_bvh_t, gwn_bvh_destroy> {
+    static_assert(Width == 4, "libgwn_hip builds width-4 BVHs");
+    static_assert(sizeof(Real) == 4 && Real(0.5) != Real(0), "libgwn_hip provides Real = float");
+
+public:
+    gwn_bvh_object() { gwn_bvh_create(&h_); }
+    [[nodiscard]] gwn_bvh_info info() const noexcept {
+        gwn_bvh_info i{};
+        gwn_bvh_get_info(h_, &i);
+        return i;
+    }
+    [[nodiscard]] bool has_data() const noexcept { return info().triangle_count != 0; }
+};
+
+template <int Width, int Order, class Real, class Index>
+class gwn_bvh_moment_object final : public detail::gwn_handle_owner<gwn_moment_t, gwn_moment_destroy> {
+    static_assert(Width == 4, "libgwn_hip builds width-4 BVHs");
+    static_assert(Order >= 0 && Order <= 2, "Taylor order must be 0, 1, or 2");
+
+public:
+    gwn_bvh_moment_object() { gwn_moment_create(&h_, Order); }
+};
+
+template <class Real = float, class Index = std::uint32_t> using gwn_bvh4_object = gwn_bvh_object<4, Real, Index>;
+template <int Order, class Real = float, class Index = std::uint32_t>
+using gwn_bvh4_moment_object = gwn_bvh_moment_object<4, Order, Real, Index>;
+
+// ---- geometry (include/gwn/gwn_geometry.cuh:60-90) ------------------------------------------
+
+template <class Real, class Index>
+[[nodiscard]] gwn_status gwn_upload_geometry(
+    gwn_geometry_object<Real, Index> &object, gwn_host_span<Real const> x, gwn_host_span<Real const> y,
+    gwn_host_span<Real const> z, gwn_host_span<Index const> i0, gwn_host_span<Index const> i1,
+    gwn_host_span<Index const> i2, hipStream_t stream = gwn_default_stream()
+) noexcept {
+    if (y.size() != x.size() || z.size() != x.size() || i1.size() != i0.size() || i2.size() != i0.size())
+        return gwn_status(gwn_error::invalid_argument, "Geometry SoA spans must have identical lengths.");
+    return detail::gwn_from_code(gwn_upload_geometry_f32(
+        object.get(), x.data(), y.data(), z.data(), x.size(), i0.data(), i1.data(), i2.data(), i0.size(), stream
+    ));
+}
+
+template <class Real, class Index>
+[[nodiscard]] gwn_status gwn_update_geometry(
+    gwn_geometry_object<Real, Index> &object, gwn_host_span<Real const> x, gwn_host_span<Real const> y,
+    gwn_host_span<Real const> z, hipStream_t stream = gwn_default_stream()
+) noexcept {
+    if (y.size() != x.size() || z.size() != x.size())
+        return gwn_status(gwn_error::invalid_argument, "Vertex SoA spans must have identical lengths.");
+    return detail::gwn_from_code(gwn_update_geometry_f32(object.get(), x.data(), y.data(), z.data(), x.size(), stream));
+}
+
+// ---- build and refit (include/gwn/gwn_bvh_build.cuh:16-39, gwn_bvh_refit.cuh:25-51) ----------
+
+enum class gwn_bvh_build_method : std::uint8_t { k_hploc, k_lbvh };
+
+struct gwn_bvh_build_options {
+    gwn_bvh_build_method method = gwn_bvh_build_method::k_hploc;
+    std::uint32_t hploc_search_radius = 8;
+};
+
+template <int Width, class Real, class Index, class MortonCode = std::uint64_t>
+[[nodiscard]] gwn_status gwn_build_bvh(
+    gwn_geometry_object<Real, Index> const &geometry, gwn_bvh_object<Width, Real, Index> &bvh,
+    gwn_bvh_build_options const options = {}, hipStream_t stream = gwn_default_stream()
+) noexcept {
+    int const method = options.method == gwn_bvh_build_method::k_hploc ? GWN_BVH_BUILD_HPLOC : GWN_BVH_BUILD_LBVH;
+    return detail::gwn_from_code(
+        gwn_build_bvh4_f32(geometry.get(), bvh.get(), method, options.hploc_search_radius, stream)
+    );
+}
+
+template <int Width, class Real, class Index>
+[[nodiscard]] gwn_status gwn_refit_bvh(
+    gwn_geometry_object<Real, Index> const &geometry, gwn_bvh_object<Width, Real, Index> &bvh,
+    hipStream_t stream = gwn_default_stream()
+) noexcept {
+    return detail::gwn_from_code(gwn_refit_bvh_f32(geometry.get(), bvh.get(), stream));
+}
+
+template <int Order, int Width, class Real, class Index>
+[[nodiscard]] gwn_status gwn_refit_bvh_moment(
+    gwn_bvh_object<Width, Real, Index> const &bvh, gwn_bvh_moment_object<Width, Order, Real, Index> &moment,
+    hipStream_t stream = gwn_default_stream()
+) noexcept {
+    return detail::gwn_from_code(gwn_refit_bvh_moment_f32(bvh.get(), moment.get(), stream));
+}
+
+// ---- query batches (include/gwn/gwn_query.cuh:41-61, 296-445) --------------------------------
+
+inline constexpr int k_gwn_default_query_batch_block_size = 256;
+inline constexpr int k_gwn_default_traversal_stack_capacity = 64;
+
+struct gwn_query_batch_config {
+    int block_size = k_gwn_default_query_batch_block_size;
+    int stack_capacity = k_gwn_default_traversal_stack_capacity;
+};
+
+namespace detail {
+template <gwn_query_batch_config Config> [[nodiscard]] constexpr ::gwn_query_config gwn_c_config() {
+    return ::gwn_query_config{Config.block_size, Config.stack_capacity, -1, 0};
+}
+} // namespace detail
+
+template <gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, class Real, class Index = std::uint32_t>
+    requires(Config.block_size > 0 && Config.block_size <= 1024)
+[[nodiscard]] gwn_status gwn_compute_winding_number_exact_batch(
+    gwn_bvh_object<Width, Real, Index> const &bvh, gwn_device_span<Real const> query_x,
+    gwn_device_span<Real const> query_y, gwn_device_span<Real const> query_z, gwn_device_span<Real> output,
+    hipStream_t stream = gwn_default_stream()
+) noexcept {
+    if (query_y.size() != query_x.size() || query_z.size() != query_x.size() || output.size() != query_x.size())
+        return gwn_status(gwn_error::invalid_argument, "Query SoA and output spans must have identical lengths.");
+    constexpr ::gwn_query_config c = detail::gwn_c_config<Config>();
+    return detail::gwn_from_code(gwn_compute_winding_number_exact_batch_f32(
+        bvh.get(), query_x.data(), query_y.data(), query_z.data(), output.data(), output.size(), &c, stream
+    ));
+}
+
+template <int Order, gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, class Real,
+          class Index = std::uint32_t>
+    requires(Config.block_size > 0 && Config.block_size <= 1024 && Config.stack_capacity > 0)
+[[nodiscard]] gwn_status gwn_compute_winding_number_taylor_batch(
+    gwn_bvh_object<Width, Real, Index> const &bvh, gwn_bvh_moment_object<Width, Order, Real, Index> const &moment,
+    gwn_device_span<Real const> query_x, gwn_device_span<Real const> query_y, gwn_device_span<Real const> query_z,
+    gwn_device_span<Real> output, Real const accuracy_scale = Real(2), hipStream_t stream = gwn_default_stream()
+) noexcept {
+    if (query_y.size() != query_x.size() || query_z.size() != query_x.size() || output.size() != query_x.size())
+        return gwn_status(gwn_error::invalid_argument, "Query SoA and output spans must have identical lengths.");
+    constexpr ::gwn_query_config c = detail::gwn_c_config<Config>();
+    return detail::gwn_from_code(gwn_compute_winding_number_taylor_batch_f32(
+        bvh.get(), moment.get(), query_x.data(), query_y.data(), query_z.data(), output.data(), output.size(),
+        accuracy_scale, &c, stream
+    ));
+}
+
+} // namespace gwn

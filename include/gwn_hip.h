@@ -161,7 +161,7 @@ int gwn_compute_winding_number_taylor_batch_ex_f32(
     const gwn_query_config *config, const uint32_t *order, uint32_t *counters, hipStream_t stream);
 
 /* Morton-order permutation of query points within the BVH root bounds (30-bit keys, stable
- * onesweep sort): order[i] is the input index of the i-th query to visit.  Reusable across
+ * radix sort): order[i] is the input index of the i-th query to visit.  Reusable across
  * batches over the same points (no reference counterpart; an MI355X traversal aid). */
 int gwn_compute_query_order_f32(
     gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, uint32_t *order,
