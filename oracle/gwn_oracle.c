@@ -711,6 +711,42 @@ typedef struct {
     int near_fp64;
 } taylor_ctx;
 
+/*
+ * Precision split of one leaf triangle (no reference counterpart): within four edge lengths of
+ * v0 -- |q - v0|^2 <= 16 max(|e1|^2, |e2|^2) -- the FP64 solid angle of the exact vertices;
+ * otherwise the FP32 value of the equivalent unnormalized form
+ * tan(W/2) = a.(e1 x e2) / (|a||b||c| + (a.b)|c| + (a.c)|b| + (b.c)|a|).  Both are added to the
+ * query's FP64 sum.
+ */
+static void split_leaf(const float *r, const float *q, const double *qd, double *sum)
+{
+    float e1[3], e2[3], a[3];
+    for (int k = 0; k < 3; ++k) {
+        e1[k] = r[3 + k] - r[k];
+        e2[k] = r[6 + k] - r[k];
+        a[k] = r[k] - q[k];
+    }
+    float le1 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+    float le2 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+    float dq = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    float m = le1 < le2 ? le2 : le1;
+    if (!(dq > 16.0f * m)) {
+        double a_[3] = {r[0], r[1], r[2]}, b_[3] = {r[3], r[4], r[5]}, c_[3] = {r[6], r[7], r[8]};
+        *sum += solid_angle_f64(a_, b_, c_, qd);
+        return;
+    }
+    float b[3] = {a[0] + e1[0], a[1] + e1[1], a[2] + e1[2]};
+    float cc[3] = {a[0] + e2[0], a[1] + e2[1], a[2] + e2[2]};
+    float la = sqrtf(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    float lb = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    float lc = sqrtf(cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2]);
+    float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    float num = a[0] * n[0] + a[1] * n[1] + a[2] * n[2];
+    float den = la * lb * lc + (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) * lc +
+                (a[0] * cc[0] + a[1] * cc[1] + a[2] * cc[2]) * lb + (b[0] * cc[0] + b[1] * cc[1] + b[2] * cc[2]) * la;
+    *sum += (double)(2.0f * atan2f(num, den));
+}
+
 static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint64_t *cnt)
 {
     const float k_pi = (float)3.141592653589793238462643383279502884L;
@@ -733,9 +769,7 @@ static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint6
                 continue;                                                                         \
             const float *r_ = c->tri + 9 * (off_ + k_);                                           \
             if (c->near_fp64) {                                                                   \
-                double a_[3] = {r_[0], r_[1], r_[2]}, b_[3] = {r_[3], r_[4], r_[5]},              \
-                       c_[3] = {r_[6], r_[7], r_[8]};                                             \
-                near += solid_angle_f64(a_, b_, c_, qd);                                          \
+                split_leaf(r_, q, qd, &near);                                                     \
             } else {                                                                              \
                 float a_[3], b_[3], c_[3];                                                        \
                 record_vertices_f32(r_, a_, b_, c_);                                              \
@@ -802,7 +836,10 @@ static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint6
                 }
 #undef T
                 if (isfinite(w)) {
-                    omega += w;
+                    if (c->near_fp64)
+                        near += (double)w; /* the split accumulates every term in FP64 */
+                    else
+                        omega += w;
                     ++accepted;
                     continue;
                 }

@@ -149,8 +149,10 @@ float gwn_oracle_solid_angle_f32(const float *a, const float *b, const float *c,
  * Taylor traversal replay (FP32, reference-form leaves).  counters (optional,
  * 3 x uint64 per query or NULL): child tests, accepted expansions, leaf
  * triangle evaluations.  Returns NaN for a query whose stack would exceed
- * stack_capacity.  near_fp64 != 0 evaluates every leaf triangle in FP64 on the
- * exact vertices and accumulates it in FP64 (the precision split).
+ * stack_capacity.  near_fp64 != 0 is the precision split: a leaf triangle whose first
+ * vertex is within four edge lengths of the query is evaluated in FP64 on the exact vertices, a
+ * farther one in FP32 (unnormalized form); every term, far-field expansions included, is
+ * accumulated in FP64.
  */
 void gwn_oracle_winding_taylor_f32(
     const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,

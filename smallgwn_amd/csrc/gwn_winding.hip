@@ -14,8 +14,9 @@
 // data is one traversal record per node (compact child refs + field-SoA coefficients), so a
 // node visit touches 2-3 consecutive 128-byte lines and no separate hierarchy array.
 //
-// Precision split: far-field Taylor terms in FP32; every near-field (exact) triangle term in
-// FP64 on the exact vertices, accumulated in FP64.
+// Precision split: traversal and far-field Taylor terms in FP32; a leaf triangle within four edge
+// lengths of the query in FP64 on the exact vertices, farther leaf triangles in FP32; every term
+// accumulated in FP64.
 #include "gwn_core.hpp"
 
 #include <cmath>
@@ -215,6 +216,23 @@ __device__ inline float product_unfused(float a, float b) {
     return a * b;
 }
 
+// Precision split for one leaf triangle reached by the traversal: a triangle whose first vertex
+// is within four edge lengths of the query (the atan2-cancellation zone) is evaluated in FP64 on
+// the exact vertices; a farther one in FP32.  Both go to the query's FP64 sum.  The distance
+// test is unfused so every implementation classifies a triangle the same way.
+__device__ inline void split_leaf(tri_record const &t, float x, float y, float z, double &sum) {
+    float const e1x = t.v1x - t.v0x, e1y = t.v1y - t.v0y, e1z = t.v1z - t.v0z;
+    float const e2x = t.v2x - t.v0x, e2y = t.v2y - t.v0y, e2z = t.v2z - t.v0z;
+    float const ax = t.v0x - x, ay = t.v0y - y, az = t.v0z - z;
+    float const le1 = squared_norm_unfused(e1x, e1y, e1z);
+    float const le2 = squared_norm_unfused(e2x, e2y, e2z);
+    float const dq = squared_norm_unfused(ax, ay, az);
+    if (!(dq > product_unfused(k_split_near_scale2, fmaxf(le1, le2))))
+        sum += solid_angle_f64(t, x, y, z);
+    else
+        sum += double(solid_angle_far_f32(ax, ay, az, e1x, e1y, e1z, e2x, e2y, e2z));
+}
+
 template <int Order>
 __device__ inline float taylor_term(float const *f, int s, float qrx, float qry, float qrz, float ql2) {
 #define GWN_F(k) f[(k) * 4 + s]
@@ -297,7 +315,10 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
             if (!descend) {
                 float const w = taylor_term<Order>(fields, s, qrx, qry, qrz, ql2);
                 if (isfinite(w)) {
-                    omega += w;
+                    if constexpr (Prec == GWN_PRECISION_PLAIN)
+                        omega += w;
+                    else
+                        near += double(w);
                     ++n_accept;
                     continue;
                 }
@@ -308,7 +329,7 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
                 if constexpr (Prec == GWN_PRECISION_PLAIN)
                     omega += solid_angle_plain(t, x, y, z);
                 else
-                    near += solid_angle_f64(t, x, y, z);
+                    split_leaf(t, x, y, z, near);
                 continue;
             }
             if (next != k_invalid) {
@@ -410,7 +431,10 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_wave_kernel(
                 if (!descend) {
                     float const w = taylor_term<Order>(fields, s, qrx, qry, qrz, ql2);
                     if (isfinite(w)) {
-                        omega += w;
+                        if constexpr (Prec == GWN_PRECISION_PLAIN)
+                            omega += w;
+                        else
+                            near += double(w);
                         ++n_accept;
                     } else {
                         descend = true;
@@ -427,7 +451,7 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_wave_kernel(
                     if constexpr (Prec == GWN_PRECISION_PLAIN)
                         omega += solid_angle_plain(t, x, y, z);
                     else
-                        near += solid_angle_f64(t, x, y, z);
+                        split_leaf(t, x, y, z, near);
                 }
                 continue;
             }
@@ -489,12 +513,12 @@ __global__ __launch_bounds__(256) void leaf_root_kernel(
         if constexpr (Prec == GWN_PRECISION_PLAIN)
             omega += solid_angle_plain(tris[offset + k], x, y, z);
         else
-            near += solid_angle_f64(tris[offset + k], x, y, z);
+            split_leaf(tris[offset + k], x, y, z, near);
     }
     if constexpr (Prec == GWN_PRECISION_PLAIN)
         out[q] = omega / (4.0f * k_pi_f);
     else
-        out[q] = static_cast<float>(near / (4.0 * k_pi_d));
+        out[q] = static_cast<float>((double(omega) + near) / (4.0 * k_pi_d));
     if (counters != nullptr) {
         counters[3 * q + 0] = 0;
         counters[3 * q + 1] = 0;
