@@ -259,4 +259,82 @@ template <int Order, gwn_query_batch_config Config = gwn_query_batch_config{}, i
     ));
 }
 
+template <int Order, gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, class Real,
+          class Index = std::uint32_t>
+    requires(Config.block_size > 0 && Config.block_size <= 1024 && Config.stack_capacity > 0)
+[[nodiscard]] gwn_status gwn_compute_winding_gradient_taylor_batch(
+    gwn_bvh_object<Width, Real, Index> const &bvh, gwn_bvh_moment_object<Width, Order, Real, Index> const &moment,
+    gwn_device_span<Real const> query_x, gwn_device_span<Real const> query_y, gwn_device_span<Real const> query_z,
+    gwn_device_span<Real> output_x, gwn_device_span<Real> output_y, gwn_device_span<Real> output_z,
+    Real const accuracy_scale = Real(2), hipStream_t stream = gwn_default_stream()
+) noexcept {
+    if (query_y.size() != query_x.size() || query_z.size() != query_x.size())
+        return gwn_status(gwn_error::invalid_argument, "Query SoA spans must have identical lengths.");
+    if (output_x.size() != query_x.size() || output_y.size() != query_x.size() || output_z.size() != query_x.size())
+        return gwn_status(gwn_error::invalid_argument, "Output span sizes must match query count.");
+    constexpr ::gwn_query_config c = detail::gwn_c_config<Config>();
+    return detail::gwn_from_code(gwn_compute_winding_gradient_taylor_batch_f32(
+        bvh.get(), moment.get(), query_x.data(), query_y.data(), query_z.data(), output_x.data(), output_y.data(),
+        output_z.data(), output_x.size(), accuracy_scale, &c, stream
+    ));
+}
+
+template <gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, class Real, class Index = std::uint32_t>
+    requires(Config.block_size > 0 && Config.block_size <= 1024 && Config.stack_capacity > 0)
+[[nodiscard]] gwn_status gwn_compute_unsigned_distance_batch(
+    gwn_bvh_object<Width, Real, Index> const &bvh, gwn_device_span<Real const> query_x,
+    gwn_device_span<Real const> query_y, gwn_device_span<Real const> query_z, gwn_device_span<Real> output,
+    Real const culling_band = Real(__builtin_huge_valf()), hipStream_t stream = gwn_default_stream()
+) noexcept {
+    if (query_y.size() != query_x.size() || query_z.size() != query_x.size())
+        return gwn_status(gwn_error::invalid_argument, "Query SoA spans must have identical lengths.");
+    if (output.size() != query_x.size())
+        return gwn_status(gwn_error::invalid_argument, "Output span size must match query count.");
+    constexpr ::gwn_query_config c = detail::gwn_c_config<Config>();
+    return detail::gwn_from_code(gwn_compute_unsigned_distance_batch_f32(
+        bvh.get(), query_x.data(), query_y.data(), query_z.data(), output.data(), output.size(), culling_band, &c,
+        stream
+    ));
+}
+
+enum class gwn_ray_first_hit_status : std::uint8_t { k_miss = 0, k_hit = 1, k_overflow = 2 };
+
+// gwn_ray_first_hit_result<float, uint32_t> (include/gwn/gwn_query.cuh:74-95), 32 bytes.
+template <class Real = float, class Index = std::uint32_t> struct gwn_ray_first_hit_result {
+    Real t{Real(-1)};
+    Real u{Real(0)};
+    Real v{Real(0)};
+    Real geometric_normal_x{Real(0)};
+    Real geometric_normal_y{Real(0)};
+    Real geometric_normal_z{Real(0)};
+    Index primitive_id{Index(0xffffffffu)};
+    gwn_ray_first_hit_status status{gwn_ray_first_hit_status::k_miss};
+    [[nodiscard]] constexpr bool hit() const noexcept { return status == gwn_ray_first_hit_status::k_hit; }
+
+private:
+    std::uint8_t storage_padding_[3]{};
+};
+static_assert(sizeof(gwn_ray_first_hit_result<>) == sizeof(::gwn_ray_first_hit_result));
+
+template <gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, class Real, class Index = std::uint32_t>
+    requires(Config.block_size > 0 && Config.block_size <= 1024 && Config.stack_capacity > 0)
+[[nodiscard]] gwn_status gwn_compute_ray_first_hit_batch(
+    gwn_bvh_object<Width, Real, Index> const &bvh, gwn_device_span<Real const> ray_origin_x,
+    gwn_device_span<Real const> ray_origin_y, gwn_device_span<Real const> ray_origin_z,
+    gwn_device_span<Real const> ray_dir_x, gwn_device_span<Real const> ray_dir_y,
+    gwn_device_span<Real const> ray_dir_z, gwn_device_span<gwn_ray_first_hit_result<Real, Index>> output,
+    Real const t_min = Real(0), Real const t_max = Real(__builtin_huge_valf()),
+    hipStream_t stream = gwn_default_stream()
+) noexcept {
+    std::size_t const n = ray_origin_x.size();
+    if (ray_origin_y.size() != n || ray_origin_z.size() != n || ray_dir_x.size() != n || ray_dir_y.size() != n ||
+        ray_dir_z.size() != n || output.size() != n)
+        return gwn_status(gwn_error::invalid_argument, "ray first-hit: mismatched span sizes");
+    constexpr ::gwn_query_config c = detail::gwn_c_config<Config>();
+    return detail::gwn_from_code(gwn_compute_ray_first_hit_batch_f32(
+        bvh.get(), ray_origin_x.data(), ray_origin_y.data(), ray_origin_z.data(), ray_dir_x.data(), ray_dir_y.data(),
+        ray_dir_z.data(), reinterpret_cast<::gwn_ray_first_hit_result *>(output.data()), n, t_min, t_max, &c, stream
+    ));
+}
+
 } // namespace gwn

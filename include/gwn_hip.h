@@ -82,6 +82,23 @@ typedef struct gwn_bvh_info {
     uint64_t root_reference;
 } gwn_bvh_info;
 
+/* Ray first-hit record, byte-compatible with gwn_ray_first_hit_result<float, uint32_t>
+ * (include/gwn/gwn_query.cuh:63-95): t = -1 and primitive_id = 0xFFFFFFFF on a miss;
+ * (u, v) weight the second and third vertex; the geometric normal is unnormalized. */
+typedef enum gwn_ray_first_hit_status {
+    GWN_RAY_MISS = 0,
+    GWN_RAY_HIT = 1,
+    GWN_RAY_OVERFLOW = 2
+} gwn_ray_first_hit_status;
+
+typedef struct gwn_ray_first_hit_result {
+    float t, u, v;
+    float geometric_normal_x, geometric_normal_y, geometric_normal_z;
+    uint32_t primitive_id;
+    uint8_t status;
+    uint8_t padding[3];
+} gwn_ray_first_hit_result;
+
 int gwn_abi_version(void);
 const char *gwn_last_error_message(void);
 
@@ -159,6 +176,39 @@ int gwn_compute_winding_number_taylor_batch_ex_f32(
     gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz,
     float *out, uint64_t count, float accuracy_scale, int32_t precision,
     const gwn_query_config *config, const uint32_t *order, uint32_t *counters, hipStream_t stream);
+
+/* ---- gradient batch: replaces gwn_compute_winding_gradient_taylor_batch<Order, Config>
+ *      (include/gwn/gwn_query.cuh:684-756, detail/gwn_query_winding_impl.cuh:20-99, 217-325).
+ *      Analytic gradient of the Taylor winding number, written to gx, gy, gz. ---- */
+int gwn_compute_winding_gradient_taylor_batch_f32(
+    gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz, float *gx,
+    float *gy, float *gz, uint64_t count, float accuracy_scale, const gwn_query_config *config,
+    hipStream_t stream);
+/* Winding number (out, optional) and gradient from one traversal, with the precision split, a
+ * caller-given query order and the work counters of gwn_compute_winding_number_taylor_batch_ex_f32. */
+int gwn_compute_winding_gradient_taylor_batch_ex_f32(
+    gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz, float *out,
+    float *gx, float *gy, float *gz, uint64_t count, float accuracy_scale, int32_t precision,
+    const gwn_query_config *config, const uint32_t *order, uint32_t *counters, hipStream_t stream);
+
+/* ---- unsigned distance: replaces gwn_compute_unsigned_distance_batch<Config>
+ *      (include/gwn/gwn_query.cuh:129-180, detail/gwn_query_distance_impl.cuh:36-192).
+ *      culling_band is the initial bound and the result when no triangle is closer (negative
+ *      values clamp to 0, +inf searches unbounded); a query whose traversal would exceed the
+ *      stack capacity gets NaN. ---- */
+int gwn_compute_unsigned_distance_batch_f32(
+    gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, float *out, uint64_t count,
+    float culling_band, const gwn_query_config *config, hipStream_t stream);
+
+/* ---- ray first hit: replaces gwn_compute_ray_first_hit_batch<Config>
+ *      (include/gwn/gwn_query.cuh:205-290, detail/gwn_query_ray_impl.cuh:20-595).  Rays are
+ *      tested on the closed interval [t_min, t_max]; the traversal is the reference's packed
+ *      near-first one when packed_stack_bound fits the stack capacity, else its canonical one,
+ *      so ties between equal-t hits resolve in the reference's order. ---- */
+int gwn_compute_ray_first_hit_batch_f32(
+    gwn_bvh_t bvh, const float *ox, const float *oy, const float *oz, const float *dx, const float *dy,
+    const float *dz, gwn_ray_first_hit_result *out, uint64_t count, float t_min, float t_max,
+    const gwn_query_config *config, hipStream_t stream);
 
 /* Morton-order permutation of query points within the BVH root bounds (30-bit keys, stable
  * radix sort): order[i] is the input index of the i-th query to visit.  Reusable across

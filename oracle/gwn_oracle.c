@@ -7,6 +7,7 @@
  */
 #include "gwn_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -693,7 +694,8 @@ void gwn_oracle_winding_exact_mesh_f64(const float *vx, const float *vy, const f
 }
 
 /* ---------------------------------------------------------------------------
- * Taylor traversal replay, gwn_query_winding_impl.cuh:102-374 (ComputeGradient = false).
+ * Taylor traversal replay, gwn_query_winding_impl.cuh:102-374, with or without the gradient
+ * (ComputeGradient).
  * ------------------------------------------------------------------------- */
 
 typedef struct {
@@ -706,19 +708,64 @@ typedef struct {
     int order, F, cap;
     float beta;
     const float *qx, *qy, *qz;
-    float *out;
+    float *out;               /* winding (may be NULL) */
+    float *gx, *gy, *gz;      /* gradient (NULL: winding only) */
     uint64_t *counters;
     int near_fp64;
 } taylor_ctx;
 
 /*
- * Precision split of one leaf triangle (no reference counterpart): within four edge lengths of
- * v0 -- |q - v0|^2 <= 16 max(|e1|^2, |e2|^2) -- the FP64 solid angle of the exact vertices;
- * otherwise the FP32 value of the equivalent unnormalized form
- * tan(W/2) = a.(e1 x e2) / (|a||b||c| + (a.b)|c| + (a.c)|b| + (b.c)|a|).  Both are added to the
- * query's FP64 sum.
+ * Gradient of one triangle's solid angle (Biot-Savart form, gwn_query_winding_impl.cuh:20-61):
+ * sum over the edges (i, j) of ((g_i - g_j) . (g_i/|g_i| - g_j/|g_j|)) / |g_i x g_j|^2 (g_i x g_j)
+ * with g = vertex - q; zero when q is on a vertex or an edge is seen end-on.
  */
-static void split_leaf(const float *r, const float *q, const double *qd, double *sum)
+#define GWN_GRAD_TRI(NAME, REALT, SQRT)                                                               \
+    static void NAME(const REALT *a, const REALT *b, const REALT *c, const REALT *q, REALT *out)      \
+    {                                                                                                 \
+        REALT g[3][3], h[3][3], l[3];                                                                 \
+        const REALT *v[3] = {a, b, c};                                                                \
+        out[0] = out[1] = out[2] = (REALT)0;                                                          \
+        for (int i = 0; i < 3; ++i) {                                                                 \
+            for (int k = 0; k < 3; ++k)                                                               \
+                g[i][k] = v[i][k] - q[k];                                                             \
+            l[i] = SQRT(g[i][0] * g[i][0] + g[i][1] * g[i][1] + g[i][2] * g[i][2]);                 \
+        }                                                                                             \
+        if (l[0] == (REALT)0 || l[1] == (REALT)0 || l[2] == (REALT)0)                                 \
+            return;                                                                                   \
+        for (int i = 0; i < 3; ++i)                                                                   \
+            for (int k = 0; k < 3; ++k)                                                               \
+                h[i][k] = g[i][k] / l[i];                                                             \
+        for (int e = 0; e < 3; ++e) {                                                                 \
+            int i = e, j = (e + 1) % 3;                                                               \
+            REALT cr[3] = {g[i][1] * g[j][2] - g[i][2] * g[j][1], g[i][2] * g[j][0] - g[i][0] * g[j][2], \
+                           g[i][0] * g[j][1] - g[i][1] * g[j][0]};                                    \
+            REALT cs = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];                                 \
+            REALT t[3] = {(REALT)0, (REALT)0, (REALT)0};                                              \
+            if (cs != (REALT)0) {                                                                     \
+                REALT d[3] = {g[i][0] - g[j][0], g[i][1] - g[j][1], g[i][2] - g[j][2]};              \
+                REALT dh[3] = {h[i][0] - h[j][0], h[i][1] - h[j][1], h[i][2] - h[j][2]};             \
+                REALT sc = (d[0] * dh[0] + d[1] * dh[1] + d[2] * dh[2]) / cs;                         \
+                t[0] = sc * cr[0]; t[1] = sc * cr[1]; t[2] = sc * cr[2];                              \
+            }                                                                                         \
+            if (e == 0) {                                                                             \
+                out[0] = t[0]; out[1] = t[1]; out[2] = t[2];                                          \
+            } else {                                                                                  \
+                out[0] += t[0]; out[1] += t[1]; out[2] += t[2];                                       \
+            }                                                                                         \
+        }                                                                                             \
+    }
+GWN_GRAD_TRI(grad_tri_f32, float, sqrtf)
+GWN_GRAD_TRI(grad_tri_f64, double, sqrt)
+#undef GWN_GRAD_TRI
+
+/*
+ * Precision split of one leaf triangle (no reference counterpart): within four edge lengths of
+ * v0 -- |q - v0|^2 <= 16 max(|e1|^2, |e2|^2) -- the FP64 solid angle (and gradient) of the exact
+ * vertices; otherwise FP32 values: the solid angle from the equivalent unnormalized form
+ * tan(W/2) = a.(e1 x e2) / (|a||b||c| + (a.b)|c| + (a.c)|b| + (b.c)|a|), the gradient from the
+ * exact vertices.  Every value is added to the query's FP64 sums.
+ */
+static void split_leaf(const float *r, const float *q, const double *qd, double *sum, double *gsum)
 {
     float e1[3], e2[3], a[3];
     for (int k = 0; k < 3; ++k) {
@@ -733,6 +780,11 @@ static void split_leaf(const float *r, const float *q, const double *qd, double 
     if (!(dq > 16.0f * m)) {
         double a_[3] = {r[0], r[1], r[2]}, b_[3] = {r[3], r[4], r[5]}, c_[3] = {r[6], r[7], r[8]};
         *sum += solid_angle_f64(a_, b_, c_, qd);
+        if (gsum) {
+            double g[3];
+            grad_tri_f64(a_, b_, c_, qd, g);
+            gsum[0] += g[0]; gsum[1] += g[1]; gsum[2] += g[2];
+        }
         return;
     }
     float b[3] = {a[0] + e1[0], a[1] + e1[1], a[2] + e1[2]};
@@ -745,21 +797,29 @@ static void split_leaf(const float *r, const float *q, const double *qd, double 
     float den = la * lb * lc + (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) * lc +
                 (a[0] * cc[0] + a[1] * cc[1] + a[2] * cc[2]) * lb + (b[0] * cc[0] + b[1] * cc[1] + b[2] * cc[2]) * la;
     *sum += (double)(2.0f * atan2f(num, den));
+    if (gsum) {
+        float g[3];
+        grad_tri_f32(r, r + 3, r + 6, q, g);
+        gsum[0] += (double)g[0]; gsum[1] += (double)g[1]; gsum[2] += (double)g[2];
+    }
 }
 
-static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint64_t *cnt)
+/* One query; writes the winding number and, if grad != NULL, its gradient. */
+static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint64_t *cnt, float *grad)
 {
     const float k_pi = (float)3.141592653589793238462643383279502884L;
     const double k_pi_d = 3.141592653589793238462643383279502884;
+    const float k_inv_4pi = 1.0f / (4.0f * k_pi);
     float q[3] = {qx, qy, qz};
     double qd[3] = {qx, qy, qz};
     float omega = 0.0f;
     double near = 0.0;
+    float gs[3] = {0.0f, 0.0f, 0.0f};
+    double gd[3] = {0.0, 0.0, 0.0};
     float beta2 = c->beta * c->beta;
     uint32_t stack[1024];
     int sp = 0;
     uint64_t tests = 0, accepted = 0, leaves = 0;
-    /* Precision split: leaves in FP64 on the exact vertices, accumulated in FP64. */
 #define LEAF(child_ptr)                                                                            \
     do {                                                                                           \
         uint64_t off_ = (child_ptr)->reference & GWN_ORACLE_OFFSET_MASK;                          \
@@ -769,20 +829,32 @@ static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint6
                 continue;                                                                         \
             const float *r_ = c->tri + 9 * (off_ + k_);                                           \
             if (c->near_fp64) {                                                                   \
-                split_leaf(r_, q, qd, &near);                                                     \
+                split_leaf(r_, q, qd, &near, grad ? gd : NULL);                                   \
             } else {                                                                              \
                 float a_[3], b_[3], c_[3];                                                        \
                 record_vertices_f32(r_, a_, b_, c_);                                              \
                 omega += solid_angle_f32(a_, b_, c_, q);                                          \
+                if (grad) {                                                                       \
+                    float g_[3];                                                                  \
+                    grad_tri_f32(a_, b_, c_, q, g_);                                              \
+                    gs[0] += g_[0]; gs[1] += g_[1]; gs[2] += g_[2];                               \
+                }                                                                                 \
             }                                                                                     \
             ++leaves;                                                                             \
         }                                                                                         \
     } while (0)
-#define RESULT() (c->near_fp64 ? (float)(((double)omega + near) / (4.0 * k_pi_d)) : omega / (4.0f * k_pi))
+#define FINISH()                                                                                   \
+    do {                                                                                           \
+        if (cnt) { cnt[0] = tests; cnt[1] = accepted; cnt[2] = leaves; }                          \
+        if (grad) {                                                                                \
+            for (int k_ = 0; k_ < 3; ++k_)                                                         \
+                grad[k_] = c->near_fp64 ? (float)(gd[k_] * (1.0 / (4.0 * k_pi_d))) : gs[k_] * k_inv_4pi; \
+        }                                                                                          \
+        return c->near_fp64 ? (float)(((double)omega + near) / (4.0 * k_pi_d)) : omega / (4.0f * k_pi); \
+    } while (0)
     if ((c->root.reference & GWN_ORACLE_VALID_BIT) && (c->root.reference >> 48) != 0) {
         LEAF(&c->root);
-        if (cnt) { cnt[0] = tests; cnt[1] = accepted; cnt[2] = leaves; }
-        return RESULT();
+        FINISH();
     }
     uint32_t node = (uint32_t)(c->root.reference & GWN_ORACLE_OFFSET_MASK);
     for (;;) {
@@ -810,6 +882,12 @@ static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint6
                 float Nx = T(4), Ny = T(5), Nz = T(6);
                 float rdotN = qnx * Nx + qny * Ny + qnz * Nz;
                 float w = -m2 * rdotN;
+                float g[3] = {0.0f, 0.0f, 0.0f};
+                if (grad) {
+                    g[0] = m3 * (3.0f * qnx * rdotN - Nx);
+                    g[1] = m3 * (3.0f * qny * rdotN - Ny);
+                    g[2] = m3 * (3.0f * qnz * rdotN - Nz);
+                }
                 if (c->order >= 1) {
                     float qxx = qnx * qnx, qyy = qny * qny, qzz = qnz * qnz;
                     float Txx = T(7), Tyy = T(8), Tzz = T(9);
@@ -818,6 +896,17 @@ static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint6
                     float S = qxx * Txx + qyy * Tyy + qzz * Tzz + qnx * qny * Sxy +
                               qnx * qnz * Szx + qny * qnz * Syz;
                     w += m3 * (Tr - 3.0f * S);
+                    if (grad) {
+                        /* gwn_query_winding_impl.cuh:247-257 */
+                        float m4 = m2 * m2;
+                        float vx = 2.0f * Txx * qnx + Sxy * qny + Szx * qnz;
+                        float vy = Sxy * qnx + 2.0f * Tyy * qny + Syz * qnz;
+                        float vz = Szx * qnx + Syz * qny + 2.0f * Tzz * qnz;
+                        float coeff = 15.0f * S - 3.0f * Tr;
+                        g[0] += m4 * (coeff * qnx - 3.0f * vx);
+                        g[1] += m4 * (coeff * qny - 3.0f * vy);
+                        g[2] += m4 * (coeff * qnz - 3.0f * vz);
+                    }
                 }
                 if (c->order >= 2) {
                     float m4 = m2 * m2;
@@ -833,13 +922,37 @@ static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint6
                                        qnz * (3.0f * zzz + t0z)) -
                                7.5f * (qnx3 * xxx + qny3 * yyy + qnz3 * zzz + qnx * qny * qnz * spn +
                                        qnx2 * t1x + qny2 * t1y + qnz2 * t1z));
+                    if (grad) {
+                        /* gwn_query_winding_impl.cuh:295-323 */
+                        float m5 = m4 * m1;
+                        float Px = 3.0f * xxx + Ayx + Azx;
+                        float Py = 3.0f * yyy + Azy + Axy;
+                        float Pz = 3.0f * zzz + Axz + Ayz;
+                        float L2 = qnx * Px + qny * Py + qnz * Pz;
+                        float C2 = qnx3 * xxx + qny3 * yyy + qnz3 * zzz + qnx * qny * qnz * spn +
+                                   qnx2 * (qny * Axy + qnz * Axz) + qny2 * (qnz * Ayz + qnx * Ayx) +
+                                   qnz2 * (qnx * Azx + qny * Azy);
+                        float Rx = 3.0f * xxx * qnx2 + 2.0f * Axy * qnx * qny + 2.0f * Axz * qnx * qnz +
+                                   Ayx * qny2 + Azx * qnz2 + spn * qny * qnz;
+                        float Ry = 3.0f * yyy * qny2 + 2.0f * Ayx * qnx * qny + 2.0f * Ayz * qny * qnz +
+                                   Axy * qnx2 + Azy * qnz2 + spn * qnx * qnz;
+                        float Rz = 3.0f * zzz * qnz2 + 2.0f * Azx * qnx * qnz + 2.0f * Azy * qny * qnz +
+                                   Axz * qnx2 + Ayz * qny2 + spn * qnx * qny;
+                        float h = 52.5f * C2;
+                        g[0] += m5 * (1.5f * Px - 7.5f * (qnx * L2 + Rx) + qnx * h);
+                        g[1] += m5 * (1.5f * Py - 7.5f * (qny * L2 + Ry) + qny * h);
+                        g[2] += m5 * (1.5f * Pz - 7.5f * (qnz * L2 + Rz) + qnz * h);
+                    }
                 }
 #undef T
                 if (isfinite(w)) {
-                    if (c->near_fp64)
+                    if (c->near_fp64) {
                         near += (double)w; /* the split accumulates every term in FP64 */
-                    else
+                        gd[0] += (double)g[0]; gd[1] += (double)g[1]; gd[2] += (double)g[2];
+                    } else {
                         omega += w;
+                        gs[0] += g[0]; gs[1] += g[1]; gs[2] += g[2];
+                    }
                     ++accepted;
                     continue;
                 }
@@ -849,6 +962,8 @@ static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint6
                 if (next != GWN_ORACLE_INVALID_INDEX) {
                     if (sp >= c->cap || sp >= 1024) {
                         if (cnt) { cnt[0] = tests; cnt[1] = accepted; cnt[2] = leaves; }
+                        if (grad)
+                            grad[0] = grad[1] = grad[2] = NAN;
                         return NAN;
                     }
                     stack[sp++] = next;
@@ -867,17 +982,25 @@ static float taylor_one(const taylor_ctx *c, float qx, float qy, float qz, uint6
         node = stack[--sp];
     }
 #undef LEAF
-    if (cnt) { cnt[0] = tests; cnt[1] = accepted; cnt[2] = leaves; }
-    return RESULT();
-#undef RESULT
+    FINISH();
+#undef FINISH
 }
 
 static void taylor_range(void *vctx, uint64_t begin, uint64_t end)
 {
     taylor_ctx *c = (taylor_ctx *)vctx;
-    for (uint64_t q = begin; q < end; ++q)
-        c->out[q] = taylor_one(c, c->qx[q], c->qy[q], c->qz[q],
-                               c->counters ? c->counters + 3 * q : NULL);
+    for (uint64_t q = begin; q < end; ++q) {
+        float g[3];
+        float w = taylor_one(c, c->qx[q], c->qy[q], c->qz[q], c->counters ? c->counters + 3 * q : NULL,
+                             c->gx ? g : NULL);
+        if (c->out)
+            c->out[q] = w;
+        if (c->gx) {
+            c->gx[q] = g[0];
+            c->gy[q] = g[1];
+            c->gz[q] = g[2];
+        }
+    }
 }
 
 void gwn_oracle_winding_taylor_f32(
@@ -889,6 +1012,592 @@ void gwn_oracle_winding_taylor_f32(
 {
     taylor_ctx ctx = {nodes, node_count, root, triangles, triangle_count, moments, order,
                       gwn_oracle_moment_field_count(order), stack_capacity, accuracy_scale,
-                      qx, qy, qz, out, counters, near_fp64};
+                      qx, qy, qz, out, NULL, NULL, NULL, counters, near_fp64};
     parallel_ranges(taylor_range, &ctx, query_count, threads);
+}
+
+void gwn_oracle_winding_gradient_taylor_f32(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
+    const float *triangles, uint64_t triangle_count, const float *moments, int order,
+    float accuracy_scale, int stack_capacity, const float *qx, const float *qy,
+    const float *qz, uint64_t query_count, float *out, float *gx, float *gy, float *gz,
+    uint64_t *counters, int near_fp64, int threads)
+{
+    taylor_ctx ctx = {nodes, node_count, root, triangles, triangle_count, moments, order,
+                      gwn_oracle_moment_field_count(order), stack_capacity, accuracy_scale,
+                      qx, qy, qz, out, gx, gy, gz, counters, near_fp64};
+    parallel_ranges(taylor_range, &ctx, query_count, threads);
+}
+
+/* Exact gradient of the winding number over all triangles in FP64 (exact vertices). */
+typedef struct {
+    const float *vx, *vy, *vz;
+    const uint32_t *i0, *i1, *i2;
+    uint64_t nt;
+    const float *qx, *qy, *qz;
+    double *gx, *gy, *gz;
+} grad_mesh_ctx;
+
+static void grad_mesh_range(void *vctx, uint64_t begin, uint64_t end)
+{
+    grad_mesh_ctx *c = (grad_mesh_ctx *)vctx;
+    const double k_pi_d = 3.141592653589793238462643383279502884;
+    for (uint64_t q = begin; q < end; ++q) {
+        double qq[3] = {c->qx[q], c->qy[q], c->qz[q]};
+        double s[3] = {0.0, 0.0, 0.0};
+        for (uint64_t t = 0; t < c->nt; ++t) {
+            double a[3] = {c->vx[c->i0[t]], c->vy[c->i0[t]], c->vz[c->i0[t]]};
+            double b[3] = {c->vx[c->i1[t]], c->vy[c->i1[t]], c->vz[c->i1[t]]};
+            double cc[3] = {c->vx[c->i2[t]], c->vy[c->i2[t]], c->vz[c->i2[t]]};
+            double g[3];
+            grad_tri_f64(a, b, cc, qq, g);
+            s[0] += g[0]; s[1] += g[1]; s[2] += g[2];
+        }
+        c->gx[q] = s[0] / (4.0 * k_pi_d);
+        c->gy[q] = s[1] / (4.0 * k_pi_d);
+        c->gz[q] = s[2] / (4.0 * k_pi_d);
+    }
+}
+
+void gwn_oracle_winding_gradient_exact_mesh_f64(
+    const float *vx, const float *vy, const float *vz, const uint32_t *i0, const uint32_t *i1,
+    const uint32_t *i2, uint64_t triangle_count, const float *qx, const float *qy,
+    const float *qz, uint64_t query_count, double *gx, double *gy, double *gz, int threads)
+{
+    grad_mesh_ctx ctx = {vx, vy, vz, i0, i1, i2, triangle_count, qx, qy, qz, gx, gy, gz};
+    parallel_ranges(grad_mesh_range, &ctx, query_count, threads);
+}
+
+/* ---------------------------------------------------------------------------
+ * Unsigned distance, include/gwn/detail/gwn_query_distance_impl.cuh:20-192 and
+ * gwn_query_geometry_impl.cuh:46-114.
+ * ------------------------------------------------------------------------- */
+
+static float dot3f(const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* Squared distance from p to the closed triangle (a, b, c): Voronoi regions (Ericson). */
+static float point_triangle_distance2_f32(const float *p, const float *a, const float *b, const float *c)
+{
+    float ab[3], ac[3], ap[3], bp[3], cp[3], r[3];
+    for (int k = 0; k < 3; ++k) {
+        ab[k] = b[k] - a[k];
+        ac[k] = c[k] - a[k];
+        ap[k] = p[k] - a[k];
+    }
+    float d1 = dot3f(ab, ap), d2 = dot3f(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f)
+        return dot3f(ap, ap);
+    for (int k = 0; k < 3; ++k)
+        bp[k] = p[k] - b[k];
+    float d3 = dot3f(ab, bp), d4 = dot3f(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3)
+        return dot3f(bp, bp);
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        float v = d1 / (d1 - d3);
+        for (int k = 0; k < 3; ++k)
+            r[k] = p[k] - (a[k] + v * ab[k]);
+        return dot3f(r, r);
+    }
+    for (int k = 0; k < 3; ++k)
+        cp[k] = p[k] - c[k];
+    float d5 = dot3f(ab, cp), d6 = dot3f(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6)
+        return dot3f(cp, cp);
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        float w = d2 / (d2 - d6);
+        for (int k = 0; k < 3; ++k)
+            r[k] = p[k] - (a[k] + w * ac[k]);
+        return dot3f(r, r);
+    }
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int k = 0; k < 3; ++k)
+            r[k] = p[k] - (b[k] + w * (c[k] - b[k]));
+        return dot3f(r, r);
+    }
+    float denom = 1.0f / (va + vb + vc);
+    float v = vb * denom, w = vc * denom;
+    for (int k = 0; k < 3; ++k)
+        r[k] = p[k] - (a[k] + v * ab[k] + w * ac[k]);
+    return dot3f(r, r);
+}
+
+static float fmax0(float x) { return (x < 0.0f) ? 0.0f : x; } /* std::max(x, 0) */
+
+static float aabb_distance2_f32(const float *q, const float *b)
+{
+    float dx = fmax0(b[0] - q[0]) + fmax0(q[0] - b[3]);
+    float dy = fmax0(b[1] - q[1]) + fmax0(q[1] - b[4]);
+    float dz = fmax0(b[2] - q[2]) + fmax0(q[2] - b[5]);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+typedef struct {
+    const gwn_oracle_node4_f32 *nodes;
+    uint64_t node_count;
+    gwn_oracle_child_f32 root;
+    const float *tri;
+    uint64_t nt;
+    const float *qx, *qy, *qz;
+    float band;
+    int cap;
+    float *out;
+} distance_ctx;
+
+/* Returns 0 when a leaf reaches distance 0 (the global lower bound), like visit_leaf. */
+static int distance_leaf(const distance_ctx *c, uint64_t ref, const float *q, float *best)
+{
+    uint64_t off = ref & GWN_ORACLE_OFFSET_MASK;
+    uint32_t n = (uint32_t)(ref >> 48);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (off + k >= c->nt)
+            continue;
+        float a[3], b[3], cc[3];
+        record_vertices_f32(c->tri + 9 * (off + k), a, b, cc);
+        float d2 = point_triangle_distance2_f32(q, a, b, cc);
+        if (d2 < *best)
+            *best = d2;
+    }
+    return *best > 0.0f;
+}
+
+static float distance_one(const distance_ctx *c, float qx, float qy, float qz)
+{
+    float q[3] = {qx, qy, qz};
+    float band = c->band < 0.0f ? 0.0f : c->band; /* std::max(culling_band, 0) */
+    float best = band * band;
+    uint64_t stack[1024];
+    int sp = 0;
+    uint64_t reference = c->root.reference;
+    for (;;) {
+        uint64_t next_ref = 0;
+        int valid = (reference & GWN_ORACLE_VALID_BIT) != 0;
+        uint32_t count = (uint32_t)(reference >> 48);
+        uint64_t off = reference & GWN_ORACLE_OFFSET_MASK;
+        if (valid && count != 0) {
+            if (!distance_leaf(c, reference, q, &best))
+                return 0.0f;
+        } else if (valid && count == 0 && off < c->node_count) {
+            const gwn_oracle_node4_f32 *nd = &c->nodes[off];
+            float cd[4];
+            uint64_t cr[4];
+            int m = 0;
+            for (int s = 0; s < 4; ++s) {
+                const gwn_oracle_child_f32 *ch = &nd->children[s];
+                if (!(ch->reference & GWN_ORACLE_VALID_BIT))
+                    continue;
+                float d2 = aabb_distance2_f32(q, ch->bounds);
+                if (d2 >= best)
+                    continue;
+                cd[m] = d2;
+                cr[m] = ch->reference;
+                ++m;
+            }
+            for (int s = m; s < 4; ++s) {
+                cd[s] = INFINITY;
+                cr[s] = 0;
+            }
+#define CSWAP(i, j)                                                                                \
+            if (cd[i] > cd[j]) {                                                                   \
+                float td = cd[i]; cd[i] = cd[j]; cd[j] = td;                                       \
+                uint64_t tr = cr[i]; cr[i] = cr[j]; cr[j] = tr;                                    \
+            }
+            CSWAP(0, 1) CSWAP(2, 3) CSWAP(0, 2) CSWAP(1, 3) CSWAP(1, 2)
+#undef CSWAP
+            for (int s = 0; s < m; ++s) {
+                if ((cr[s] & GWN_ORACLE_VALID_BIT) && (cr[s] >> 48) != 0 && cd[s] < best &&
+                    !distance_leaf(c, cr[s], q, &best))
+                    return 0.0f;
+            }
+            for (int s = m - 1; s >= 0; --s) {
+                if (!((cr[s] & GWN_ORACLE_VALID_BIT) && (cr[s] >> 48) == 0) || cd[s] >= best)
+                    continue;
+                if (next_ref != 0) {
+                    if (sp >= c->cap || sp >= 1024)
+                        return NAN;
+                    stack[sp++] = next_ref;
+                }
+                next_ref = cr[s];
+            }
+        }
+        if (next_ref != 0) {
+            reference = next_ref;
+            continue;
+        }
+        if (sp == 0)
+            break;
+        reference = stack[--sp];
+    }
+    return sqrtf(best);
+}
+
+static void distance_range(void *vctx, uint64_t begin, uint64_t end)
+{
+    distance_ctx *c = (distance_ctx *)vctx;
+    for (uint64_t q = begin; q < end; ++q)
+        c->out[q] = distance_one(c, c->qx[q], c->qy[q], c->qz[q]);
+}
+
+void gwn_oracle_unsigned_distance_f32(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
+    const float *triangles, uint64_t triangle_count, const float *qx, const float *qy,
+    const float *qz, uint64_t query_count, float culling_band, int stack_capacity, float *out,
+    int threads)
+{
+    distance_ctx ctx = {nodes, node_count, root, triangles, triangle_count, qx, qy, qz,
+                        culling_band, stack_capacity, out};
+    parallel_ranges(distance_range, &ctx, query_count, threads);
+}
+
+float gwn_oracle_point_triangle_distance2_f32(const float *p, const float *a, const float *b, const float *c)
+{
+    return point_triangle_distance2_f32(p, a, b, c);
+}
+
+/* ---------------------------------------------------------------------------
+ * Ray first hit, include/gwn/detail/gwn_query_ray_impl.cuh:20-595 and the batch dispatch of
+ * include/gwn/gwn_query.cuh:214-290 (packed near-first traversal when the root is internal, the
+ * node count is below 2^29 and packed_stack_bound fits the stack; canonical traversal otherwise).
+ * ------------------------------------------------------------------------- */
+
+static float fmin_std(float a, float b) { return (b < a) ? b : a; } /* std::min */
+static float fmax_std(float a, float b) { return (a < b) ? b : a; } /* std::max */
+
+typedef struct {
+    float inv[3];
+    int zero[3];
+} ray_dir_pre;
+
+static int ray_axis_update(float o, float lo, float hi, float inv, int zero, float *t_near, float *t_far)
+{
+    if (zero)
+        return o >= lo && o <= hi;
+    float t0 = (lo - o) * inv;
+    float t1 = (hi - o) * inv;
+    *t_near = fmax_std(*t_near, fmin_std(t0, t1));
+    *t_far = fmin_std(*t_far, fmax_std(t0, t1));
+    return *t_near <= *t_far;
+}
+
+/* gwn_ray_aabb_intersect_interval_impl (:100-140) */
+static int ray_box_canonical(const float *o, const ray_dir_pre *d, const float *b, float t_min, float t_max,
+                             float *t_near_out)
+{
+    if (!(t_max >= t_min))
+        return 0;
+    float t_near = t_min, t_far = t_max;
+    for (int k = 0; k < 3; ++k)
+        if (!ray_axis_update(o[k], b[k], b[3 + k], d->inv[k], d->zero[k], &t_near, &t_far))
+            return 0;
+    if (t_far < t_near)
+        return 0;
+    *t_near_out = t_near;
+    return 1;
+}
+
+/* gwn_stable_triangle_normal_impl (:143-172) */
+static void stable_normal(const float *a, const float *b, const float *c, float *n)
+{
+    float abx = a[2] * b[1], aby = a[0] * b[2], abz = a[1] * b[0];
+    float bcx = b[2] * c[1], bcy = b[0] * c[2], bcz = b[1] * c[0];
+    float cab[3] = {a[1] * b[2] - abx, a[2] * b[0] - aby, a[0] * b[1] - abz};
+    float cbc[3] = {b[1] * c[2] - bcx, b[2] * c[0] - bcy, b[0] * c[1] - bcz};
+    n[0] = cbc[0]; n[1] = cbc[1]; n[2] = cbc[2];
+    if (fabsf(abx) < fabsf(bcx)) n[0] = cab[0];
+    if (fabsf(aby) < fabsf(bcy)) n[1] = cab[1];
+    if (fabsf(abz) < fabsf(bcz)) n[2] = cab[2];
+}
+
+static void cross3f(const float *a, const float *b, float *r)
+{
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* gwn_ray_triangle_intersect_impl (:175-239): oriented Pluecker edge functions. */
+static int ray_triangle(const float *o, const float *dir, const float *v0, const float *v1, const float *v2,
+                        float t_min, float t_max, float *t_out, float *u_out, float *v_out, float *n_out)
+{
+    float p0[3], p1[3], p2[3], e0[3], e1[3], e2[3], s[3], c[3];
+    for (int k = 0; k < 3; ++k) {
+        p0[k] = v0[k] - o[k];
+        p1[k] = v1[k] - o[k];
+        p2[k] = v2[k] - o[k];
+    }
+    for (int k = 0; k < 3; ++k) {
+        e0[k] = p2[k] - p0[k];
+        e1[k] = p0[k] - p1[k];
+        e2[k] = p1[k] - p2[k];
+    }
+    for (int k = 0; k < 3; ++k) s[k] = p2[k] + p0[k];
+    cross3f(e0, s, c);
+    float u = dot3f(c, dir);
+    for (int k = 0; k < 3; ++k) s[k] = p0[k] + p1[k];
+    cross3f(e1, s, c);
+    float v = dot3f(c, dir);
+    for (int k = 0; k < 3; ++k) s[k] = p1[k] + p2[k];
+    cross3f(e2, s, c);
+    float w = dot3f(c, dir);
+    float uvw = u + v + w;
+    float eps = FLT_EPSILON * fabsf(uvw);
+    float mn = fmin_std(u, fmin_std(v, w));
+    float mx = fmax_std(u, fmax_std(v, w));
+    if (!(mn >= -eps || mx <= eps))
+        return 0;
+    float ng[3];
+    stable_normal(e0, e1, e2, ng);
+    float den = 2.0f * dot3f(ng, dir);
+    if (den == 0.0f)
+        return 0;
+    float t_num = 2.0f * dot3f(p0, ng);
+    float t = t_num / den;
+    if (!(t_min <= t && t <= t_max))
+        return 0;
+    if (fabsf(uvw) >= FLT_MIN) {
+        float inv = 1.0f / uvw;
+        *u_out = u * inv;
+        *v_out = v * inv;
+    } else {
+        if (uvw == 0.0f)
+            return 0;
+        *u_out = u / uvw;
+        *v_out = v / uvw;
+    }
+    *t_out = t;
+    n_out[0] = ng[0]; n_out[1] = ng[1]; n_out[2] = ng[2];
+    return 1;
+}
+
+typedef struct {
+    float t, u, v, n[3];
+    uint32_t prim;
+    int found;
+} ray_best;
+
+typedef struct {
+    const gwn_oracle_node4_f32 *nodes;
+    uint64_t node_count;
+    gwn_oracle_child_f32 root;
+    const float *tri;
+    const uint32_t *prim;
+    uint64_t nt;
+    uint32_t packed_stack_bound;
+    const float *ox, *oy, *oz, *dx, *dy, *dz;
+    float t_min, t_max;
+    int cap;
+    int mode; /* -1: the batch's dispatch, 0: canonical, 1: packed */
+    gwn_oracle_ray_hit *out;
+} ray_ctx;
+
+static void ray_visit_leaf(const ray_ctx *c, uint64_t ref, const float *o, const float *dir, float t_min,
+                           ray_best *b)
+{
+    uint64_t off = ref & GWN_ORACLE_OFFSET_MASK;
+    uint32_t n = (uint32_t)(ref >> 48);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (off + k >= c->nt)
+            continue;
+        float a[3], bb[3], cc[3], t, u, v, ng[3];
+        record_vertices_f32(c->tri + 9 * (off + k), a, bb, cc);
+        if (!ray_triangle(o, dir, a, bb, cc, t_min, b->t, &t, &u, &v, ng))
+            continue;
+        if (b->found && !(t < b->t))
+            continue;
+        b->t = t; b->u = u; b->v = v;
+        b->n[0] = ng[0]; b->n[1] = ng[1]; b->n[2] = ng[2];
+        b->prim = c->prim[off + k];
+        b->found = 1;
+    }
+}
+
+#define CSWAPT(i, j)                                                                               \
+    if (et[i] > et[j]) {                                                                           \
+        float tt = et[i]; et[i] = et[j]; et[j] = tt;                                               \
+        uint64_t tr = er[i]; er[i] = er[j]; er[j] = tr;                                            \
+    }
+
+/* Packed traversal (:242-358); returns 1 on stack overflow. */
+static int ray_packed(const ray_ctx *c, const float *o, const float *dir, const ray_dir_pre *d, int parallel,
+                      float t_min, ray_best *b)
+{
+    const uint32_t leaf_bit = 1u << 31;
+    uint32_t stack[1024];
+    int sp = 0;
+    uint32_t payload = (uint32_t)(c->root.reference & GWN_ORACLE_OFFSET_MASK);
+    for (;;) {
+        if (payload & leaf_bit) {
+            int slot = (int)(payload & 3u);
+            uint32_t parent = (payload & ~leaf_bit) >> 2;
+            ray_visit_leaf(c, c->nodes[parent].children[slot].reference, o, dir, t_min, b);
+        } else if (payload < c->node_count) {
+            const gwn_oracle_node4_f32 *nd = &c->nodes[payload];
+            float et[4];
+            uint64_t er[4];
+            for (int s = 0; s < 4; ++s) {
+                et[s] = INFINITY;
+                er[s] = 0;
+                const gwn_oracle_child_f32 *ch = &nd->children[s];
+                uint64_t ref = ch->reference;
+                if (!(ref & GWN_ORACLE_VALID_BIT))
+                    continue;
+                int leaf = (ref >> 48) != 0;
+                if (!leaf && (ref & GWN_ORACLE_OFFSET_MASK) >= c->node_count)
+                    continue;
+                const float *bx = ch->bounds;
+                float t_near, t_far;
+                int hit;
+                if (!parallel) {
+                    float tx0 = (bx[0] - o[0]) * d->inv[0], tx1 = (bx[3] - o[0]) * d->inv[0];
+                    float ty0 = (bx[1] - o[1]) * d->inv[1], ty1 = (bx[4] - o[1]) * d->inv[1];
+                    float tz0 = (bx[2] - o[2]) * d->inv[2], tz1 = (bx[5] - o[2]) * d->inv[2];
+                    t_near = fmax_std(t_min, fmax_std(fmin_std(tx0, tx1),
+                                                      fmax_std(fmin_std(ty0, ty1), fmin_std(tz0, tz1))));
+                    t_far = fmin_std(b->t, fmin_std(fmax_std(tx0, tx1),
+                                                    fmin_std(fmax_std(ty0, ty1), fmax_std(tz0, tz1))));
+                    hit = t_near <= t_far;
+                } else {
+                    t_near = t_min;
+                    t_far = b->t;
+                    hit = 1;
+                    for (int k = 0; k < 3 && hit; ++k)
+                        hit = ray_axis_update(o[k], bx[k], bx[3 + k], d->inv[k], d->zero[k], &t_near, &t_far);
+                }
+                if (!hit)
+                    continue;
+                et[s] = t_near;
+                er[s] = leaf ? (leaf_bit | (payload << 2) | (uint32_t)s) : (uint32_t)(ref & GWN_ORACLE_OFFSET_MASK);
+            }
+            CSWAPT(0, 1) CSWAPT(2, 3) CSWAPT(0, 2) CSWAPT(1, 3) CSWAPT(1, 2)
+            for (int s = 3; s > 0; --s) {
+                if (et[s] == INFINITY)
+                    continue;
+                if (sp >= c->cap || sp >= 1024)
+                    return 1;
+                stack[sp++] = (uint32_t)er[s];
+            }
+            if (et[0] != INFINITY) {
+                payload = (uint32_t)er[0];
+                continue;
+            }
+        }
+        if (sp == 0)
+            return 0;
+        payload = stack[--sp];
+    }
+}
+
+/* Canonical traversal (:485-560); returns 1 on stack overflow. */
+static int ray_canonical(const ray_ctx *c, const float *o, const float *dir, const ray_dir_pre *d, float t_min,
+                         ray_best *b)
+{
+    uint64_t stack[1024];
+    int sp = 0;
+    uint64_t reference = c->root.reference;
+    for (;;) {
+        uint64_t next = 0;
+        int valid = (reference & GWN_ORACLE_VALID_BIT) != 0;
+        uint32_t count = (uint32_t)(reference >> 48);
+        uint64_t off = reference & GWN_ORACLE_OFFSET_MASK;
+        if (valid && count != 0) {
+            ray_visit_leaf(c, reference, o, dir, t_min, b);
+        } else if (valid && count == 0 && off < c->node_count) {
+            const gwn_oracle_node4_f32 *nd = &c->nodes[off];
+            float et[4];
+            uint64_t er[4];
+            int m = 0;
+            for (int s = 0; s < 4; ++s) {
+                const gwn_oracle_child_f32 *ch = &nd->children[s];
+                if (!(ch->reference & GWN_ORACLE_VALID_BIT))
+                    continue;
+                float t_near;
+                if (!ray_box_canonical(o, d, ch->bounds, t_min, b->t, &t_near))
+                    continue;
+                et[m] = t_near;
+                er[m] = ch->reference;
+                ++m;
+            }
+            for (int s = m; s < 4; ++s) {
+                et[s] = INFINITY;
+                er[s] = 0;
+            }
+            CSWAPT(0, 1) CSWAPT(2, 3) CSWAPT(0, 2) CSWAPT(1, 3) CSWAPT(1, 2)
+            for (int s = 0; s < m; ++s)
+                if ((er[s] & GWN_ORACLE_VALID_BIT) && (er[s] >> 48) != 0 && et[s] <= b->t)
+                    ray_visit_leaf(c, er[s], o, dir, t_min, b);
+            for (int s = m - 1; s >= 0; --s) {
+                if (!((er[s] & GWN_ORACLE_VALID_BIT) && (er[s] >> 48) == 0) || et[s] > b->t)
+                    continue;
+                if (next != 0) {
+                    if (sp >= c->cap || sp >= 1024)
+                        return 1;
+                    stack[sp++] = next;
+                }
+                next = er[s];
+            }
+        }
+        if (next != 0) {
+            reference = next;
+            continue;
+        }
+        if (sp == 0)
+            return 0;
+        reference = stack[--sp];
+    }
+}
+#undef CSWAPT
+
+static void ray_one(const ray_ctx *c, uint64_t i, gwn_oracle_ray_hit *r)
+{
+    float o[3] = {c->ox[i], c->oy[i], c->oz[i]};
+    float dir[3] = {c->dx[i], c->dy[i], c->dz[i]};
+    memset(r, 0, sizeof(*r));
+    r->t = -1.0f;
+    r->primitive_id = 0xffffffffu;
+    r->status = 0;
+    float len2 = dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2];
+    if (!(len2 > 0.0f))
+        return;
+    ray_dir_pre d;
+    for (int k = 0; k < 3; ++k) {
+        d.zero[k] = dir[k] == 0.0f;
+        d.inv[k] = d.zero[k] ? 0.0f : 1.0f / dir[k];
+    }
+    ray_best b = {c->t_max, 0.0f, 0.0f, {0.0f, 0.0f, 0.0f}, 0xffffffffu, 0};
+    int packed = (c->root.reference & GWN_ORACLE_VALID_BIT) && (c->root.reference >> 48) == 0 &&
+                 c->node_count < (1ull << 29) && (int)c->packed_stack_bound <= c->cap;
+    if (c->mode >= 0)
+        packed = c->mode == 1 && (c->root.reference >> 48) == 0;
+    int overflow = packed ? ray_packed(c, o, dir, &d, d.zero[0] || d.zero[1] || d.zero[2], c->t_min, &b)
+                          : ray_canonical(c, o, dir, &d, c->t_min, &b);
+    if (b.found) {
+        r->t = b.t; r->u = b.u; r->v = b.v;
+        r->nx = b.n[0]; r->ny = b.n[1]; r->nz = b.n[2];
+        r->primitive_id = b.prim;
+        r->status = 1;
+    }
+    if (overflow)
+        r->status = 2;
+}
+
+static void ray_range(void *vctx, uint64_t begin, uint64_t end)
+{
+    ray_ctx *c = (ray_ctx *)vctx;
+    for (uint64_t i = begin; i < end; ++i)
+        ray_one(c, i, &c->out[i]);
+}
+
+void gwn_oracle_ray_first_hit_f32(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
+    const float *triangles, const uint32_t *primitive_indices, uint64_t triangle_count,
+    uint32_t packed_stack_bound, const float *ox, const float *oy, const float *oz, const float *dx,
+    const float *dy, const float *dz, uint64_t ray_count, float t_min, float t_max, int stack_capacity,
+    int traversal, gwn_oracle_ray_hit *out, int threads)
+{
+    ray_ctx ctx = {nodes, node_count, root, triangles, primitive_indices, triangle_count, packed_stack_bound,
+                   ox, oy, oz, dx, dy, dz, t_min, t_max, stack_capacity, traversal, out};
+    parallel_ranges(ray_range, &ctx, ray_count, threads);
 }

@@ -21,6 +21,10 @@
  *   - solid angle: include/gwn/detail/gwn_query_geometry_impl.cuh:16-45
  *   - exact winding: include/gwn/detail/gwn_query_winding_impl.cuh:377-389
  *   - Taylor traversal: include/gwn/detail/gwn_query_winding_impl.cuh:102-374
+ *   - solid-angle gradient and Taylor gradient terms: gwn_query_winding_impl.cuh:20-99, 217-325
+ *   - ray first hit: include/gwn/detail/gwn_query_ray_impl.cuh:20-595, gwn_query.cuh:214-290
+ *   - unsigned distance: include/gwn/detail/gwn_query_distance_impl.cuh:20-192,
+ *     gwn_query_geometry_impl.cuh:46-114
  *
  * Deliberate, documented differences from the reference that the HIP path
  * shares:
@@ -160,6 +164,59 @@ void gwn_oracle_winding_taylor_f32(
     float accuracy_scale, int stack_capacity, const float *qx, const float *qy,
     const float *qz, uint64_t query_count, float *out, uint64_t *counters, int near_fp64,
     int threads);
+
+/*
+ * The same replay with the analytic gradient (gwn_query_winding_impl.cuh:20-99, 217-325):
+ * out (optional) receives the winding number, gx/gy/gz its gradient.
+ */
+void gwn_oracle_winding_gradient_taylor_f32(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
+    const float *triangles, uint64_t triangle_count, const float *moments, int order,
+    float accuracy_scale, int stack_capacity, const float *qx, const float *qy,
+    const float *qz, uint64_t query_count, float *out, float *gx, float *gy, float *gz,
+    uint64_t *counters, int near_fp64, int threads);
+
+/* Exact winding-number gradient in FP64 over the original mesh (sum of triangle gradients). */
+void gwn_oracle_winding_gradient_exact_mesh_f64(
+    const float *vx, const float *vy, const float *vz, const uint32_t *i0, const uint32_t *i1,
+    const uint32_t *i2, uint64_t triangle_count, const float *qx, const float *qy,
+    const float *qz, uint64_t query_count, double *gx, double *gy, double *gz, int threads);
+
+/*
+ * Unsigned point-to-mesh distance by near-first culled traversal
+ * (gwn_query_distance_impl.cuh:36-166): the culling band (clamped at 0) is the initial bound
+ * and the result when nothing is closer; NaN when the stack would exceed stack_capacity.
+ */
+void gwn_oracle_unsigned_distance_f32(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
+    const float *triangles, uint64_t triangle_count, const float *qx, const float *qy,
+    const float *qz, uint64_t query_count, float culling_band, int stack_capacity, float *out,
+    int threads);
+
+/* Ray first-hit record, byte-compatible with gwn_ray_first_hit_result<float, uint32_t>
+ * (include/gwn/gwn_query.cuh:74-95): status 0 miss, 1 hit, 2 stack overflow. */
+typedef struct {
+    float t, u, v, nx, ny, nz;
+    uint32_t primitive_id;
+    uint8_t status;
+    uint8_t pad[3];
+} gwn_oracle_ray_hit;
+
+/*
+ * Nearest ray-triangle hit on [t_min, t_max] (gwn_query_ray_impl.cuh:20-595) with the batch's
+ * traversal dispatch (gwn_query.cuh:276-289): packed near-first when the root is internal, the
+ * node count is below 2^29 and packed_stack_bound <= stack_capacity, canonical otherwise
+ * (traversal = -1); traversal = 0 / 1 forces the canonical / packed traversal.
+ */
+void gwn_oracle_ray_first_hit_f32(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
+    const float *triangles, const uint32_t *primitive_indices, uint64_t triangle_count,
+    uint32_t packed_stack_bound, const float *ox, const float *oy, const float *oz, const float *dx,
+    const float *dy, const float *dz, uint64_t ray_count, float t_min, float t_max, int stack_capacity,
+    int traversal, gwn_oracle_ray_hit *out, int threads);
+
+/* Squared point-triangle distance (gwn_query_geometry_impl.cuh:46-97). */
+float gwn_oracle_point_triangle_distance2_f32(const float *p, const float *a, const float *b, const float *c);
 
 #ifdef __cplusplus
 }

@@ -91,6 +91,18 @@ def lib():
         L.gwn_oracle_winding_taylor_f32.argtypes = [
             vp, u64, _Child, vp, u64, vp, i32, f32, i32, vp, vp, vp, u64, vp, vp, i32, i32,
         ]
+        L.gwn_oracle_winding_gradient_taylor_f32.argtypes = [
+            vp, u64, _Child, vp, u64, vp, i32, f32, i32, vp, vp, vp, u64, vp, vp, vp, vp, vp, i32, i32,
+        ]
+        L.gwn_oracle_unsigned_distance_f32.argtypes = [vp, u64, _Child, vp, u64, vp, vp, vp, u64, f32, i32, vp, i32]
+        L.gwn_oracle_ray_first_hit_f32.argtypes = [
+            vp, u64, _Child, vp, vp, u64, ctypes.c_uint32, vp, vp, vp, vp, vp, vp, u64, f32, f32, i32, i32, vp, i32,
+        ]
+        L.gwn_oracle_point_triangle_distance2_f32.restype = f32
+        L.gwn_oracle_point_triangle_distance2_f32.argtypes = [vp, vp, vp, vp]
+        L.gwn_oracle_winding_gradient_exact_mesh_f64.argtypes = [
+            vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, u64, vp, vp, vp, i32,
+        ]
         _lib = L
     return _lib
 
@@ -277,6 +289,97 @@ def winding_taylor_f32(bvh: OracleBvh, moments: np.ndarray, order: int, qx, qy, 
         _ptr(cnt) if cnt is not None else None, 1 if near_fp64 else 0, threads or default_threads(),
     )
     return (out, cnt) if counters else out
+
+
+def winding_gradient_taylor_f32(bvh: OracleBvh, moments: np.ndarray, order: int, qx, qy, qz,
+                                accuracy_scale: float = 2.0, stack_capacity: int = 64,
+                                counters: bool = False, threads: int | None = None,
+                                near_fp64: bool = False):
+    """Replay of the Taylor traversal with the analytic gradient: (winding, gx, gy, gz[, counters])."""
+    nodes = _c(bvh.nodes, NODE_DTYPE)
+    tris = _c(bvh.triangles, np.float32)
+    mom = _c(moments, np.float32)
+    qx, qy, qz = _c(qx, np.float32), _c(qy, np.float32), _c(qz, np.float32)
+    n = qx.shape[0]
+    out = np.empty(n, dtype=np.float32)
+    g = [np.empty(n, dtype=np.float32) for _ in range(3)]
+    cnt = np.zeros((n, 3), dtype=np.uint64) if counters else None
+    root = _Child()
+    for k in range(6):
+        root.bounds[k] = float(bvh.root_bounds[k])
+    root.reference = bvh.root_reference
+    lib().gwn_oracle_winding_gradient_taylor_f32(
+        _ptr(nodes), bvh.node_count, root, _ptr(tris), tris.shape[0], _ptr(mom), order,
+        accuracy_scale, stack_capacity, _ptr(qx), _ptr(qy), _ptr(qz), n, _ptr(out),
+        _ptr(g[0]), _ptr(g[1]), _ptr(g[2]), _ptr(cnt) if cnt is not None else None,
+        1 if near_fp64 else 0, threads or default_threads(),
+    )
+    return (out, *g, cnt) if counters else (out, *g)
+
+
+def winding_gradient_exact_mesh_f64(mesh, qx, qy, qz, threads: int | None = None):
+    """Exact gradient of the winding number in FP64 over all triangles: (gx, gy, gz)."""
+    vx, vy, vz = _c(mesh.vx, np.float32), _c(mesh.vy, np.float32), _c(mesh.vz, np.float32)
+    i0, i1, i2 = _c(mesh.i0, np.uint32), _c(mesh.i1, np.uint32), _c(mesh.i2, np.uint32)
+    qx, qy, qz = _c(qx, np.float32), _c(qy, np.float32), _c(qz, np.float32)
+    n = qx.shape[0]
+    g = [np.empty(n, dtype=np.float64) for _ in range(3)]
+    lib().gwn_oracle_winding_gradient_exact_mesh_f64(
+        _ptr(vx), _ptr(vy), _ptr(vz), _ptr(i0), _ptr(i1), _ptr(i2), i0.shape[0],
+        _ptr(qx), _ptr(qy), _ptr(qz), n, _ptr(g[0]), _ptr(g[1]), _ptr(g[2]), threads or default_threads(),
+    )
+    return tuple(g)
+
+
+def unsigned_distance_f32(bvh: OracleBvh, qx, qy, qz, culling_band: float = float("inf"),
+                          stack_capacity: int = 64, threads: int | None = None) -> np.ndarray:
+    """Replay of the near-first unsigned-distance traversal in FP32."""
+    nodes = _c(bvh.nodes, NODE_DTYPE)
+    tris = _c(bvh.triangles, np.float32)
+    qx, qy, qz = _c(qx, np.float32), _c(qy, np.float32), _c(qz, np.float32)
+    out = np.empty(qx.shape[0], dtype=np.float32)
+    root = _Child()
+    for k in range(6):
+        root.bounds[k] = float(bvh.root_bounds[k])
+    root.reference = bvh.root_reference
+    lib().gwn_oracle_unsigned_distance_f32(
+        _ptr(nodes), bvh.node_count, root, _ptr(tris), tris.shape[0], _ptr(qx), _ptr(qy), _ptr(qz),
+        qx.shape[0], culling_band, stack_capacity, _ptr(out), threads or default_threads(),
+    )
+    return out
+
+
+# gwn_ray_first_hit_result<float, uint32_t> (include/gwn/gwn_query.cuh:74-95), 32 bytes.
+RAY_HIT_DTYPE = np.dtype([("t", np.float32), ("u", np.float32), ("v", np.float32), ("nx", np.float32),
+                          ("ny", np.float32), ("nz", np.float32), ("primitive_id", np.uint32),
+                          ("status", np.uint8), ("pad", np.uint8, (3,))])
+assert RAY_HIT_DTYPE.itemsize == 32
+
+
+def ray_first_hit_f32(bvh: OracleBvh, ox, oy, oz, dx, dy, dz, t_min: float = 0.0, t_max: float = float("inf"),
+                      stack_capacity: int = 64, threads: int | None = None, traversal: str = "auto") -> np.ndarray:
+    """Replay of the batch ray first-hit query: a RAY_HIT_DTYPE record per ray.  ``traversal``:
+    "auto" (the batch's dispatch), "canonical" or "packed"."""
+    nodes = _c(bvh.nodes, NODE_DTYPE)
+    tris = _c(bvh.triangles, np.float32)
+    prim = _c(bvh.primitive_indices, np.uint32)
+    arrs = [_c(a, np.float32) for a in (ox, oy, oz, dx, dy, dz)]
+    out = np.zeros(arrs[0].shape[0], dtype=RAY_HIT_DTYPE)
+    root = _Child()
+    for k in range(6):
+        root.bounds[k] = float(bvh.root_bounds[k])
+    root.reference = bvh.root_reference
+    lib().gwn_oracle_ray_first_hit_f32(
+        _ptr(nodes), bvh.node_count, root, _ptr(tris), _ptr(prim), tris.shape[0], bvh.packed_stack_bound,
+        *[_ptr(a) for a in arrs], arrs[0].shape[0], t_min, t_max, stack_capacity,
+        {"auto": -1, "canonical": 0, "packed": 1}[traversal], _ptr(out), threads or default_threads(),
+    )
+    return out
+
+
+def point_triangle_distance2_f32(p, a, b, c) -> float:
+    p, a, b, c = (_c(np.asarray(v), np.float32) for v in (p, a, b, c))
+    return float(lib().gwn_oracle_point_triangle_distance2_f32(_ptr(p), _ptr(a), _ptr(b), _ptr(c)))
 
 
 def canonical_dfs(nodes: np.ndarray, root_reference: int):

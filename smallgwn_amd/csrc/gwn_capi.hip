@@ -410,10 +410,12 @@ int gwn_compute_winding_number_exact_batch_ex_f32(
     });
 }
 
+// Shared validation of the Taylor winding and gradient batches, in the reference's order
+// (gwn_query.cuh:409-428, 711-737).  `out` may be null only when the gradient is requested.
 static void taylor_common(
     gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz, float *out,
-    uint64_t count, float accuracy_scale, int32_t precision, const gwn_query_config *config,
-    const uint32_t *order, uint32_t *counters, hipStream_t stream
+    float *gx, float *gy, float *gz, uint64_t count, float accuracy_scale, int32_t precision,
+    const gwn_query_config *config, const uint32_t *order, uint32_t *counters, hipStream_t stream
 ) {
     gwn_query_config const c = resolve_config(config);
     validate_config(c, true);
@@ -424,10 +426,17 @@ static void taylor_common(
         gwn::throw_invalid("Taylor accuracy scale must be nonnegative.");
     if (precision != GWN_PRECISION_PLAIN && precision != GWN_PRECISION_SPLIT)
         gwn::throw_invalid("Taylor winding supports the plain and split precision modes.");
-    require_query_storage(count, qx, qy, qz, out);
+    if (gx != nullptr || gy != nullptr || gz != nullptr) {
+        require_query_storage(count, qx, qy, qz, gx);
+        if (count != 0 && (gy == nullptr || gz == nullptr))
+            gwn::throw_invalid("Query/output spans must use non-null storage when non-empty.");
+    } else {
+        require_query_storage(count, qx, qy, qz, out);
+    }
     if (count == 0)
         return;
-    gwn::winding_taylor(*bvh, *moment, qx, qy, qz, out, count, accuracy_scale, precision, to_launch(c), order, counters, stream);
+    gwn::winding_taylor(*bvh, *moment, qx, qy, qz, out, gx, gy, gz, count, accuracy_scale, precision, to_launch(c),
+                        order, counters, stream);
 }
 
 int gwn_compute_winding_number_taylor_batch_f32(
@@ -435,7 +444,8 @@ int gwn_compute_winding_number_taylor_batch_f32(
     uint64_t count, float accuracy_scale, const gwn_query_config *config, hipStream_t stream
 ) {
     return guarded("gwn_compute_winding_number_taylor_batch", [&] {
-        taylor_common(bvh, moment, qx, qy, qz, out, count, accuracy_scale, GWN_PRECISION_PLAIN, config, nullptr, nullptr, stream);
+        taylor_common(bvh, moment, qx, qy, qz, out, nullptr, nullptr, nullptr, count, accuracy_scale,
+                      GWN_PRECISION_PLAIN, config, nullptr, nullptr, stream);
     });
 }
 
@@ -445,7 +455,71 @@ int gwn_compute_winding_number_taylor_batch_ex_f32(
     const uint32_t *order, uint32_t *counters, hipStream_t stream
 ) {
     return guarded("gwn_compute_winding_number_taylor_batch", [&] {
-        taylor_common(bvh, moment, qx, qy, qz, out, count, accuracy_scale, precision, config, order, counters, stream);
+        taylor_common(bvh, moment, qx, qy, qz, out, nullptr, nullptr, nullptr, count, accuracy_scale, precision,
+                      config, order, counters, stream);
+    });
+}
+
+int gwn_compute_winding_gradient_taylor_batch_f32(
+    gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz, float *gx,
+    float *gy, float *gz, uint64_t count, float accuracy_scale, const gwn_query_config *config,
+    hipStream_t stream
+) {
+    return guarded("gwn_compute_winding_gradient_taylor_batch", [&] {
+        if (gx == nullptr && gy == nullptr && gz == nullptr && count != 0)
+            gwn::throw_invalid("Query/output spans must use non-null storage when non-empty.");
+        taylor_common(bvh, moment, qx, qy, qz, nullptr, gx, gy, gz, count, accuracy_scale, GWN_PRECISION_PLAIN,
+                      config, nullptr, nullptr, stream);
+    });
+}
+
+int gwn_compute_winding_gradient_taylor_batch_ex_f32(
+    gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz, float *out,
+    float *gx, float *gy, float *gz, uint64_t count, float accuracy_scale, int32_t precision,
+    const gwn_query_config *config, const uint32_t *order, uint32_t *counters, hipStream_t stream
+) {
+    return guarded("gwn_compute_winding_gradient_taylor_batch", [&] {
+        if (gx == nullptr && gy == nullptr && gz == nullptr && count != 0)
+            gwn::throw_invalid("Query/output spans must use non-null storage when non-empty.");
+        taylor_common(bvh, moment, qx, qy, qz, out, gx, gy, gz, count, accuracy_scale, precision, config, order,
+                      counters, stream);
+    });
+}
+
+int gwn_compute_unsigned_distance_batch_f32(
+    gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, float *out, uint64_t count,
+    float culling_band, const gwn_query_config *config, hipStream_t stream
+) {
+    return guarded("gwn_compute_unsigned_distance_batch", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, true);
+        require_bvh(bvh);
+        require_stack(*bvh, c.stack_capacity);
+        require_query_storage(count, qx, qy, qz, out);
+        if (count == 0)
+            return;
+        gwn::unsigned_distance(*bvh, qx, qy, qz, out, count, culling_band, to_launch(c), stream);
+    });
+}
+
+int gwn_compute_ray_first_hit_batch_f32(
+    gwn_bvh_t bvh, const float *ox, const float *oy, const float *oz, const float *dx, const float *dy,
+    const float *dz, gwn_ray_first_hit_result *out, uint64_t count, float t_min, float t_max,
+    const gwn_query_config *config, hipStream_t stream
+) {
+    return guarded("gwn_compute_ray_first_hit_batch", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, true);
+        require_bvh(bvh);
+        require_stack(*bvh, c.stack_capacity);
+        if (count != 0 && (ox == nullptr || oy == nullptr || oz == nullptr || dx == nullptr || dy == nullptr ||
+                           dz == nullptr || out == nullptr))
+            gwn::throw_invalid("ray first-hit: ray/output spans must use non-null storage when non-empty.");
+        if (!(t_max >= t_min))
+            gwn::throw_invalid("ray first-hit: requires t_max >= t_min.");
+        if (count == 0)
+            return;
+        gwn::ray_first_hit(*bvh, ox, oy, oz, dx, dy, dz, out, count, t_min, t_max, to_launch(c), stream);
     });
 }
 

@@ -260,10 +260,25 @@ void winding_exact(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, void *out,
     std::size_t n, int precision, query_launch const &cfg, hipStream_t stream
 );
+// Taylor batch: winding numbers into `out` (may be null) and, when gx is non-null, their
+// gradient into gx, gy, gz.
 void winding_taylor(
     gwn_bvh_handle const &bvh, gwn_moment_handle const &moment, float const *qx, float const *qy,
-    float const *qz, float *out, std::size_t n, float accuracy_scale, int precision,
-    query_launch const &cfg, std::uint32_t const *order, std::uint32_t *counters, hipStream_t stream
+    float const *qz, float *out, float *gx, float *gy, float *gz, std::size_t n, float accuracy_scale,
+    int precision, query_launch const &cfg, std::uint32_t const *order, std::uint32_t *counters,
+    hipStream_t stream
+);
+// Unsigned point-to-mesh distance with a culling band (queries Morton-sorted when cfg says so).
+void unsigned_distance(
+    gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, float *out, std::size_t n,
+    float culling_band, query_launch const &cfg, hipStream_t stream
+);
+// Nearest ray-triangle hits on [t_min, t_max]; `out` holds 32-byte
+// gwn_ray_first_hit_result<float, uint32_t> records.
+void ray_first_hit(
+    gwn_bvh_handle const &bvh, float const *ox, float const *oy, float const *oz, float const *dx,
+    float const *dy, float const *dz, void *out, std::size_t n, float t_min, float t_max,
+    query_launch const &cfg, hipStream_t stream
 );
 // Morton-order visiting permutation of query points within the BVH root bounds.
 void query_order(

@@ -216,25 +216,101 @@ __device__ inline float product_unfused(float a, float b) {
     return a * b;
 }
 
+// Gradient of one triangle's solid angle for the corner vectors g_i = v_i - q (Biot-Savart form,
+// gwn_query_winding_impl.cuh:20-61): the sum over the edges (i, j) = (0,1), (1,2), (2,0) of
+// ((g_i - g_j) . (g_i/|g_i| - g_j/|g_j|)) / |g_i x g_j|^2 * (g_i x g_j).  Zero when q sits on a
+// vertex; an edge seen end-on contributes nothing.
+template <class R>
+__device__ inline void solid_angle_gradient(R const (&g)[3][3], R (&out)[3]) {
+    R l[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        l[i] = sqrt(g[i][0] * g[i][0] + g[i][1] * g[i][1] + g[i][2] * g[i][2]);
+    out[0] = out[1] = out[2] = R(0);
+    if (l[0] == R(0) || l[1] == R(0) || l[2] == R(0))
+        return;
+    R h[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            h[i][k] = g[i][k] / l[i];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        int const i = e, j = (e + 1) % 3;
+        R const cx = g[i][1] * g[j][2] - g[i][2] * g[j][1];
+        R const cy = g[i][2] * g[j][0] - g[i][0] * g[j][2];
+        R const cz = g[i][0] * g[j][1] - g[i][1] * g[j][0];
+        R const cs = cx * cx + cy * cy + cz * cz;
+        R tx = R(0), ty = R(0), tz = R(0);
+        if (cs != R(0)) {
+            R const sc = ((g[i][0] - g[j][0]) * (h[i][0] - h[j][0]) + (g[i][1] - g[j][1]) * (h[i][1] - h[j][1]) +
+                          (g[i][2] - g[j][2]) * (h[i][2] - h[j][2])) / cs;
+            tx = sc * cx;
+            ty = sc * cy;
+            tz = sc * cz;
+        }
+        if (e == 0) {
+            out[0] = tx; out[1] = ty; out[2] = tz;
+        } else {
+            out[0] += tx; out[1] += ty; out[2] += tz;
+        }
+    }
+}
+
+// Reference-form FP32 gradient: corners rebuilt from (v0, e1 = v1 - v0, e2 = v2 - v0).
+__device__ inline void gradient_plain(tri_record const &t, float x, float y, float z, float (&out)[3]) {
+    float const e1x = t.v1x - t.v0x, e1y = t.v1y - t.v0y, e1z = t.v1z - t.v0z;
+    float const e2x = t.v2x - t.v0x, e2y = t.v2y - t.v0y, e2z = t.v2z - t.v0z;
+    float const g[3][3] = {{t.v0x - x, t.v0y - y, t.v0z - z},
+                           {(t.v0x + e1x) - x, (t.v0y + e1y) - y, (t.v0z + e1z) - z},
+                           {(t.v0x + e2x) - x, (t.v0y + e2y) - y, (t.v0z + e2z) - z}};
+    solid_angle_gradient<float>(g, out);
+}
+
 // Precision split for one leaf triangle reached by the traversal: a triangle whose first vertex
 // is within four edge lengths of the query (the atan2-cancellation zone) is evaluated in FP64 on
-// the exact vertices; a farther one in FP32.  Both go to the query's FP64 sum.  The distance
-// test is unfused so every implementation classifies a triangle the same way.
-__device__ inline void split_leaf(tri_record const &t, float x, float y, float z, double &sum) {
+// the exact vertices; a farther one in FP32 (the gradient from the exact vertices).  Everything
+// goes to the query's FP64 sums.  The distance test is unfused so every implementation
+// classifies a triangle the same way.
+template <bool Grad>
+__device__ inline void split_leaf(tri_record const &t, float x, float y, float z, double &sum, double (&gsum)[3]) {
     float const e1x = t.v1x - t.v0x, e1y = t.v1y - t.v0y, e1z = t.v1z - t.v0z;
     float const e2x = t.v2x - t.v0x, e2y = t.v2y - t.v0y, e2z = t.v2z - t.v0z;
     float const ax = t.v0x - x, ay = t.v0y - y, az = t.v0z - z;
     float const le1 = squared_norm_unfused(e1x, e1y, e1z);
     float const le2 = squared_norm_unfused(e2x, e2y, e2z);
     float const dq = squared_norm_unfused(ax, ay, az);
-    if (!(dq > product_unfused(k_split_near_scale2, fmaxf(le1, le2))))
+    if (!(dq > product_unfused(k_split_near_scale2, fmaxf(le1, le2)))) {
         sum += solid_angle_f64(t, x, y, z);
-    else
+        if constexpr (Grad) {
+            double const dx = x, dy = y, dz = z;
+            double const g[3][3] = {{double(t.v0x) - dx, double(t.v0y) - dy, double(t.v0z) - dz},
+                                    {double(t.v1x) - dx, double(t.v1y) - dy, double(t.v1z) - dz},
+                                    {double(t.v2x) - dx, double(t.v2y) - dy, double(t.v2z) - dz}};
+            double o[3];
+            solid_angle_gradient<double>(g, o);
+            gsum[0] += o[0];
+            gsum[1] += o[1];
+            gsum[2] += o[2];
+        }
+    } else {
         sum += double(solid_angle_far_f32(ax, ay, az, e1x, e1y, e1z, e2x, e2y, e2z));
+        if constexpr (Grad) {
+            float const g[3][3] = {{ax, ay, az}, {t.v1x - x, t.v1y - y, t.v1z - z}, {t.v2x - x, t.v2y - y, t.v2z - z}};
+            float o[3];
+            solid_angle_gradient<float>(g, o);
+            gsum[0] += double(o[0]);
+            gsum[1] += double(o[1]);
+            gsum[2] += double(o[2]);
+        }
+    }
 }
 
-template <int Order>
-__device__ inline float taylor_term(float const *f, int s, float qrx, float qry, float qrz, float ql2) {
+// One accepted far-field expansion (gwn_query_winding_impl.cuh:186-325): returns the solid-angle
+// term and, with Grad, writes its gradient.
+template <int Order, bool Grad>
+__device__ inline float taylor_term(float const *f, int s, float qrx, float qry, float qrz, float ql2, float (&g)[3]) {
 #define GWN_F(k) f[(k) * 4 + s]
     float const m1 = __builtin_amdgcn_rsqf(ql2);
     float const m2 = m1 * m1;
@@ -243,6 +319,11 @@ __device__ inline float taylor_term(float const *f, int s, float qrx, float qry,
     float const Nx = GWN_F(4), Ny = GWN_F(5), Nz = GWN_F(6);
     float const rdotN = qnx * Nx + qny * Ny + qnz * Nz;
     float w = -m2 * rdotN;
+    if constexpr (Grad) {
+        g[0] = m3 * (3.0f * qnx * rdotN - Nx);
+        g[1] = m3 * (3.0f * qny * rdotN - Ny);
+        g[2] = m3 * (3.0f * qnz * rdotN - Nz);
+    }
     if constexpr (Order >= 1) {
         float const qxx = qnx * qnx, qyy = qny * qny, qzz = qnz * qnz;
         float const Txx = GWN_F(7), Tyy = GWN_F(8), Tzz = GWN_F(9);
@@ -250,6 +331,16 @@ __device__ inline float taylor_term(float const *f, int s, float qrx, float qry,
         float const Tr = Txx + Tyy + Tzz;
         float const S = qxx * Txx + qyy * Tyy + qzz * Tzz + qnx * qny * Sxy + qnx * qnz * Szx + qny * qnz * Syz;
         w += m3 * (Tr - 3.0f * S);
+        if constexpr (Grad) {
+            float const m4 = m2 * m2;
+            float const vx = 2.0f * Txx * qnx + Sxy * qny + Szx * qnz;
+            float const vy = Sxy * qnx + 2.0f * Tyy * qny + Syz * qnz;
+            float const vz = Szx * qnx + Syz * qny + 2.0f * Tzz * qnz;
+            float const coeff = 15.0f * S - 3.0f * Tr;
+            g[0] += m4 * (coeff * qnx - 3.0f * vx);
+            g[1] += m4 * (coeff * qny - 3.0f * vy);
+            g[2] += m4 * (coeff * qnz - 3.0f * vz);
+        }
     }
     if constexpr (Order >= 2) {
         float const m4 = m2 * m2;
@@ -264,17 +355,111 @@ __device__ inline float taylor_term(float const *f, int s, float qrx, float qry,
         w += m4 * (1.5f * (qnx * (3.0f * xxx + t0x) + qny * (3.0f * yyy + t0y) + qnz * (3.0f * zzz + t0z)) -
                    7.5f * (qnx3 * xxx + qny3 * yyy + qnz3 * zzz + qnx * qny * qnz * spn + qnx2 * t1x +
                            qny2 * t1y + qnz2 * t1z));
+        if constexpr (Grad) {
+            float const m5 = m4 * m1;
+            float const Px = 3.0f * xxx + Ayx + Azx;
+            float const Py = 3.0f * yyy + Azy + Axy;
+            float const Pz = 3.0f * zzz + Axz + Ayz;
+            float const L2 = qnx * Px + qny * Py + qnz * Pz;
+            float const C2 = qnx3 * xxx + qny3 * yyy + qnz3 * zzz + qnx * qny * qnz * spn +
+                             qnx2 * (qny * Axy + qnz * Axz) + qny2 * (qnz * Ayz + qnx * Ayx) +
+                             qnz2 * (qnx * Azx + qny * Azy);
+            float const Rx = 3.0f * xxx * qnx2 + 2.0f * Axy * qnx * qny + 2.0f * Axz * qnx * qnz + Ayx * qny2 +
+                             Azx * qnz2 + spn * qny * qnz;
+            float const Ry = 3.0f * yyy * qny2 + 2.0f * Ayx * qnx * qny + 2.0f * Ayz * qny * qnz + Axy * qnx2 +
+                             Azy * qnz2 + spn * qnx * qnz;
+            float const Rz = 3.0f * zzz * qnz2 + 2.0f * Azx * qnx * qnz + 2.0f * Azy * qny * qnz + Axz * qnx2 +
+                             Ayz * qny2 + spn * qnx * qny;
+            float const h = 52.5f * C2;
+            g[0] += m5 * (1.5f * Px - 7.5f * (qnx * L2 + Rx) + qnx * h);
+            g[1] += m5 * (1.5f * Py - 7.5f * (qny * L2 + Ry) + qny * h);
+            g[2] += m5 * (1.5f * Pz - 7.5f * (qnz * L2 + Rz) + qnz * h);
+        }
     }
 #undef GWN_F
     return w;
 }
 
-template <int Order, int Prec, int StackCap>
+constexpr float k_inv_4pi_f = 1.0f / (4.0f * k_pi_f);
+
+// Per-query sums of one traversal: the reference's float sums (plain) or FP64 sums (split).
+template <int Prec, bool Grad> struct taylor_sums {
+    float omega = 0.0f;
+    float gf[3] = {0.0f, 0.0f, 0.0f};
+    double near = 0.0;
+    double gd[3] = {0.0, 0.0, 0.0};
+
+    __device__ void far_term(float w, float const (&g)[3]) {
+        if constexpr (Prec == GWN_PRECISION_PLAIN) {
+            omega += w;
+            if constexpr (Grad) {
+                gf[0] += g[0];
+                gf[1] += g[1];
+                gf[2] += g[2];
+            }
+        } else {
+            near += double(w);
+            if constexpr (Grad) {
+                gd[0] += double(g[0]);
+                gd[1] += double(g[1]);
+                gd[2] += double(g[2]);
+            }
+        }
+    }
+    __device__ void leaf(tri_record const &t, float x, float y, float z) {
+        if constexpr (Prec == GWN_PRECISION_PLAIN) {
+            omega += solid_angle_plain(t, x, y, z);
+            if constexpr (Grad) {
+                float o[3];
+                gradient_plain(t, x, y, z, o);
+                gf[0] += o[0];
+                gf[1] += o[1];
+                gf[2] += o[2];
+            }
+        } else {
+            split_leaf<Grad>(t, x, y, z, near, gd);
+        }
+    }
+    // Winding = sum / 4 pi; gradient = sum * (1 / 4 pi) (gwn_query_winding_impl.cuh:130-139).
+    __device__ void store(bool overflow, std::size_t q, float *out, float *gx, float *gy, float *gz) const {
+        float const nan = __builtin_nanf("");
+        float w;
+        float g[3];
+        if (overflow) {
+            w = g[0] = g[1] = g[2] = nan;
+        } else if constexpr (Prec == GWN_PRECISION_PLAIN) {
+            w = omega / (4.0f * k_pi_f);
+            g[0] = gf[0] * k_inv_4pi_f;
+            g[1] = gf[1] * k_inv_4pi_f;
+            g[2] = gf[2] * k_inv_4pi_f;
+        } else {
+            w = static_cast<float>((double(omega) + near) / (4.0 * k_pi_d));
+            double const inv = 1.0 / (4.0 * k_pi_d);
+            g[0] = static_cast<float>(gd[0] * inv);
+            g[1] = static_cast<float>(gd[1] * inv);
+            g[2] = static_cast<float>(gd[2] * inv);
+        }
+        if (out != nullptr)
+            out[q] = w;
+        if constexpr (Grad) {
+            gx[q] = g[0];
+            gy[q] = g[1];
+            gz[q] = g[2];
+        }
+    }
+};
+
+struct taylor_outputs {
+    float *out;                 // winding numbers (may be null when only the gradient is asked)
+    float *gx, *gy, *gz;        // gradient (used when the kernel computes it)
+    std::uint32_t *counters;    // optional 3 x uint32 per query
+};
+
+template <int Order, int Prec, bool Grad, int StackCap>
 __global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
     float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node,
     float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
-    std::uint32_t const *__restrict__ order, float *__restrict__ out, std::size_t n, float beta2,
-    std::uint32_t *__restrict__ counters
+    std::uint32_t const *__restrict__ order, taylor_outputs o, std::size_t n, float beta2
 ) {
     constexpr int RV = moment_record_vec4(Order);
     std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_taylor_block + threadIdx.x;
@@ -284,8 +469,7 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
     float const x = qx[q], y = qy[q], z = qz[q];
     std::uint32_t stack[StackCap];
     int sp = 0;
-    float omega = 0.0f;
-    double near = 0.0;
+    taylor_sums<Prec, Grad> acc;
     std::uint32_t n_tests = 0, n_accept = 0, n_leaf = 0;
     std::uint32_t node = root_node;
     bool overflow = false;
@@ -313,23 +497,17 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
             float const ql2 = squared_norm_unfused(qrx, qry, qrz);
             bool const descend = !(ql2 > 0.0f) || ql2 <= product_unfused(mpd[s], beta2);
             if (!descend) {
-                float const w = taylor_term<Order>(fields, s, qrx, qry, qrz, ql2);
+                float g[3];
+                float const w = taylor_term<Order, Grad>(fields, s, qrx, qry, qrz, ql2, g);
                 if (isfinite(w)) {
-                    if constexpr (Prec == GWN_PRECISION_PLAIN)
-                        omega += w;
-                    else
-                        near += double(w);
+                    acc.far_term(w, g);
                     ++n_accept;
                     continue;
                 }
             }
             if (c & k_cref_leaf) {
-                tri_record const t = tris[c & ~k_cref_leaf];
                 ++n_leaf;
-                if constexpr (Prec == GWN_PRECISION_PLAIN)
-                    omega += solid_angle_plain(t, x, y, z);
-                else
-                    split_leaf(t, x, y, z, near);
+                acc.leaf(tris[c & ~k_cref_leaf], x, y, z);
                 continue;
             }
             if (next != k_invalid) {
@@ -351,18 +529,11 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
             break;
         node = stack[--sp];
     }
-    float result;
-    if (overflow)
-        result = __builtin_nanf("");
-    else if constexpr (Prec == GWN_PRECISION_PLAIN)
-        result = omega / (4.0f * k_pi_f);
-    else
-        result = static_cast<float>((double(omega) + near) / (4.0 * k_pi_d));
-    out[q] = result;
-    if (counters != nullptr) {
-        counters[3 * q + 0] = n_tests;
-        counters[3 * q + 1] = n_accept;
-        counters[3 * q + 2] = n_leaf;
+    acc.store(overflow, q, o.out, o.gx, o.gy, o.gz);
+    if (o.counters != nullptr) {
+        o.counters[3 * q + 0] = n_tests;
+        o.counters[3 * q + 1] = n_accept;
+        o.counters[3 * q + 2] = n_leaf;
     }
 }
 
@@ -373,12 +544,11 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
 // and entered in the same slot / last-in-first-out order as in the per-lane kernel, so every lane
 // forms its sum over exactly the same terms in exactly the same order; the stack depth is bounded
 // by internal_stack_bound for the same reason.
-template <int Order, int Prec, int StackCap>
+template <int Order, int Prec, bool Grad, int StackCap>
 __global__ __launch_bounds__(k_taylor_block) void taylor_wave_kernel(
     float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node,
     float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
-    std::uint32_t const *__restrict__ order, float *__restrict__ out, std::size_t n, float beta2,
-    std::uint32_t *__restrict__ counters
+    std::uint32_t const *__restrict__ order, taylor_outputs o, std::size_t n, float beta2
 ) {
     constexpr int RV = moment_record_vec4(Order);
     constexpr int k_waves = k_taylor_block / 64;
@@ -396,8 +566,7 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_wave_kernel(
     if (mask == 0ull)
         return;
     std::uint64_t const my_bit = 1ull << lane;
-    float omega = 0.0f;
-    double near = 0.0;
+    taylor_sums<Prec, Grad> acc;
     std::uint32_t n_tests = 0, n_accept = 0, n_leaf = 0;
     std::uint32_t node = root_node;
     int sp = 0;
@@ -429,12 +598,10 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_wave_kernel(
                 float const ql2 = squared_norm_unfused(qrx, qry, qrz);
                 descend = !(ql2 > 0.0f) || ql2 <= product_unfused(mpd[s], beta2);
                 if (!descend) {
-                    float const w = taylor_term<Order>(fields, s, qrx, qry, qrz, ql2);
+                    float g[3];
+                    float const w = taylor_term<Order, Grad>(fields, s, qrx, qry, qrz, ql2, g);
                     if (isfinite(w)) {
-                        if constexpr (Prec == GWN_PRECISION_PLAIN)
-                            omega += w;
-                        else
-                            near += double(w);
+                        acc.far_term(w, g);
                         ++n_accept;
                     } else {
                         descend = true;
@@ -448,10 +615,7 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_wave_kernel(
                 tri_record const t = tris[c & ~k_cref_leaf];
                 if (dmask & my_bit) {
                     ++n_leaf;
-                    if constexpr (Prec == GWN_PRECISION_PLAIN)
-                        omega += solid_angle_plain(t, x, y, z);
-                    else
-                        split_leaf(t, x, y, z, near);
+                    acc.leaf(t, x, y, z);
                 }
                 continue;
             }
@@ -482,90 +646,70 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_wave_kernel(
     }
     if (!active)
         return;
-    float result;
-    if (overflow)
-        result = __builtin_nanf("");
-    else if constexpr (Prec == GWN_PRECISION_PLAIN)
-        result = omega / (4.0f * k_pi_f);
-    else
-        result = static_cast<float>((double(omega) + near) / (4.0 * k_pi_d));
-    out[q] = result;
-    if (counters != nullptr) {
-        counters[3 * q + 0] = n_tests;
-        counters[3 * q + 1] = n_accept;
-        counters[3 * q + 2] = n_leaf;
+    acc.store(overflow, q, o.out, o.gx, o.gy, o.gz);
+    if (o.counters != nullptr) {
+        o.counters[3 * q + 0] = n_tests;
+        o.counters[3 * q + 1] = n_accept;
+        o.counters[3 * q + 2] = n_leaf;
     }
 }
 
-template <int Prec>
+template <int Prec, bool Grad>
 __global__ __launch_bounds__(256) void leaf_root_kernel(
     tri_record const *__restrict__ tris, std::uint64_t offset, std::uint32_t count,
     float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
-    float *__restrict__ out, std::size_t n, std::uint32_t *__restrict__ counters
+    taylor_outputs o, std::size_t n
 ) {
     std::size_t const q = static_cast<std::size_t>(blockIdx.x) * 256 + threadIdx.x;
     if (q >= n)
         return;
     float const x = qx[q], y = qy[q], z = qz[q];
-    float omega = 0.0f;
-    double near = 0.0;
-    for (std::uint32_t k = 0; k < count; ++k) {
-        if constexpr (Prec == GWN_PRECISION_PLAIN)
-            omega += solid_angle_plain(tris[offset + k], x, y, z);
-        else
-            split_leaf(tris[offset + k], x, y, z, near);
-    }
-    if constexpr (Prec == GWN_PRECISION_PLAIN)
-        out[q] = omega / (4.0f * k_pi_f);
-    else
-        out[q] = static_cast<float>((double(omega) + near) / (4.0 * k_pi_d));
-    if (counters != nullptr) {
-        counters[3 * q + 0] = 0;
-        counters[3 * q + 1] = 0;
-        counters[3 * q + 2] = count;
+    taylor_sums<Prec, Grad> acc;
+    for (std::uint32_t k = 0; k < count; ++k)
+        acc.leaf(tris[offset + k], x, y, z);
+    acc.store(false, q, o.out, o.gx, o.gy, o.gz);
+    if (o.counters != nullptr) {
+        o.counters[3 * q + 0] = 0;
+        o.counters[3 * q + 1] = 0;
+        o.counters[3 * q + 2] = count;
     }
 }
 
-template <int Order, int Prec, int Cap>
+template <int Order, int Prec, bool Grad, int Cap>
 void launch_taylor_cap(
     gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy,
-    float const *qz, std::uint32_t const *order, float *out, std::size_t n, float beta2,
-    std::uint32_t *counters, hipStream_t stream
+    float const *qz, std::uint32_t const *order, taylor_outputs const &o, std::size_t n, float beta2,
+    hipStream_t stream
 ) {
     std::uint32_t const root = static_cast<std::uint32_t>(ref_offset(bvh.root.reference));
     if (order != nullptr) {
         // Morton-ordered queries: neighbouring lanes share most of their paths.
-        taylor_wave_kernel<Order, Prec, Cap><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(
-            m.records.ptr, bvh.triangles.ptr, root, qx, qy, qz, order, out, n, beta2, counters
+        taylor_wave_kernel<Order, Prec, Grad, Cap><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(
+            m.records.ptr, bvh.triangles.ptr, root, qx, qy, qz, order, o, n, beta2
         );
     } else {
-        taylor_kernel<Order, Prec, Cap><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(
-            m.records.ptr, bvh.triangles.ptr, root, qx, qy, qz, order, out, n, beta2, counters
+        taylor_kernel<Order, Prec, Grad, Cap><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(
+            m.records.ptr, bvh.triangles.ptr, root, qx, qy, qz, order, o, n, beta2
         );
     }
     GWN_LAUNCH_CHECK();
 }
 
-template <int Order, int Prec>
+template <int Order, int Prec, bool Grad>
 void launch_taylor(
     gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy,
-    float const *qz, std::uint32_t const *order, float *out, std::size_t n, float beta2, int cap,
-    std::uint32_t *counters, hipStream_t stream
+    float const *qz, std::uint32_t const *order, taylor_outputs const &o, std::size_t n, float beta2,
+    hipStream_t stream
 ) {
     // The stack only ever holds internal_stack_bound entries; the smallest compiled capacity that
-    // covers both the bound and nothing more keeps the per-lane footprint small.
+    // covers the bound keeps the LDS / register footprint small.
     int const need = static_cast<int>(bvh.internal_stack_bound);
-    (void)cap;
     if (need <= 32)
-        launch_taylor_cap<Order, Prec, 32>(bvh, m, qx, qy, qz, order, out, n, beta2, counters, stream);
-    else if (need <= 48)
-        launch_taylor_cap<Order, Prec, 48>(bvh, m, qx, qy, qz, order, out, n, beta2, counters, stream);
+        launch_taylor_cap<Order, Prec, Grad, 32>(bvh, m, qx, qy, qz, order, o, n, beta2, stream);
     else if (need <= 64)
-        launch_taylor_cap<Order, Prec, 64>(bvh, m, qx, qy, qz, order, out, n, beta2, counters, stream);
-    else if (need <= 96)
-        launch_taylor_cap<Order, Prec, 96>(bvh, m, qx, qy, qz, order, out, n, beta2, counters, stream);
+        launch_taylor_cap<Order, Prec, Grad, 64>(bvh, m, qx, qy, qz, order, o, n, beta2, stream);
     else if (need <= 128)
-        launch_taylor_cap<Order, Prec, 128>(bvh, m, qx, qy, qz, order, out, n, beta2, counters, stream);
+        launch_taylor_cap<Order, Prec, Grad, 128>(bvh, m, qx, qy, qz, order, o, n, beta2, stream);
     else
         throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
 }
@@ -617,21 +761,26 @@ void query_order(
 
 void winding_taylor(
     gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy,
-    float const *qz, float *out, std::size_t n, float accuracy_scale, int precision,
-    query_launch const &cfg, std::uint32_t const *given_order, std::uint32_t *counters, hipStream_t stream
+    float const *qz, float *out, float *gx, float *gy, float *gz, std::size_t n, float accuracy_scale,
+    int precision, query_launch const &cfg, std::uint32_t const *given_order, std::uint32_t *counters,
+    hipStream_t stream
 ) {
     if (n == 0)
         return;
     if (precision != GWN_PRECISION_PLAIN && precision != GWN_PRECISION_SPLIT)
         throw_invalid("Taylor winding supports the plain and split precision modes.");
+    bool const grad = gx != nullptr;
+    taylor_outputs const o{out, gx, gy, gz, counters};
     if (ref_leaf(bvh.root.reference)) {
         unsigned const grid = grid_for(n, 256);
+        std::uint64_t const off = ref_offset(bvh.root.reference);
+        std::uint32_t const cnt = ref_count(bvh.root.reference);
+#define GWN_LEAF_ROOT(P, G) leaf_root_kernel<P, G><<<grid, 256, 0, stream>>>(bvh.triangles.ptr, off, cnt, qx, qy, qz, o, n)
         if (precision == GWN_PRECISION_PLAIN)
-            leaf_root_kernel<GWN_PRECISION_PLAIN><<<grid, 256, 0, stream>>>(
-                bvh.triangles.ptr, ref_offset(bvh.root.reference), ref_count(bvh.root.reference), qx, qy, qz, out, n, counters);
+            grad ? GWN_LEAF_ROOT(GWN_PRECISION_PLAIN, true) : GWN_LEAF_ROOT(GWN_PRECISION_PLAIN, false);
         else
-            leaf_root_kernel<GWN_PRECISION_SPLIT><<<grid, 256, 0, stream>>>(
-                bvh.triangles.ptr, ref_offset(bvh.root.reference), ref_count(bvh.root.reference), qx, qy, qz, out, n, counters);
+            grad ? GWN_LEAF_ROOT(GWN_PRECISION_SPLIT, true) : GWN_LEAF_ROOT(GWN_PRECISION_SPLIT, false);
+#undef GWN_LEAF_ROOT
         GWN_LAUNCH_CHECK();
         return;
     }
@@ -645,21 +794,27 @@ void winding_taylor(
         order = own_order.ptr;
     }
     float const beta2 = accuracy_scale * accuracy_scale;
-    int const cap = cfg.stack_capacity;
-#define GWN_TAYLOR_CASE(O, P) launch_taylor<O, P>(bvh, m, qx, qy, qz, order, out, n, beta2, cap, counters, stream)
+#define GWN_TAYLOR_CASE(O, P, G) launch_taylor<O, P, G>(bvh, m, qx, qy, qz, order, o, n, beta2, stream)
+#define GWN_TAYLOR_ORDERS(P, G)                                                                     \
+    switch (m.order) {                                                                              \
+    case 0: GWN_TAYLOR_CASE(0, P, G); break;                                                        \
+    case 1: GWN_TAYLOR_CASE(1, P, G); break;                                                        \
+    default: GWN_TAYLOR_CASE(2, P, G); break;                                                       \
+    }
     if (precision == GWN_PRECISION_PLAIN) {
-        switch (m.order) {
-        case 0: GWN_TAYLOR_CASE(0, GWN_PRECISION_PLAIN); break;
-        case 1: GWN_TAYLOR_CASE(1, GWN_PRECISION_PLAIN); break;
-        default: GWN_TAYLOR_CASE(2, GWN_PRECISION_PLAIN); break;
+        if (grad) {
+            GWN_TAYLOR_ORDERS(GWN_PRECISION_PLAIN, true)
+        } else {
+            GWN_TAYLOR_ORDERS(GWN_PRECISION_PLAIN, false)
         }
     } else {
-        switch (m.order) {
-        case 0: GWN_TAYLOR_CASE(0, GWN_PRECISION_SPLIT); break;
-        case 1: GWN_TAYLOR_CASE(1, GWN_PRECISION_SPLIT); break;
-        default: GWN_TAYLOR_CASE(2, GWN_PRECISION_SPLIT); break;
+        if (grad) {
+            GWN_TAYLOR_ORDERS(GWN_PRECISION_SPLIT, true)
+        } else {
+            GWN_TAYLOR_ORDERS(GWN_PRECISION_SPLIT, false)
         }
     }
+#undef GWN_TAYLOR_ORDERS
 #undef GWN_TAYLOR_CASE
 }
 

@@ -36,6 +36,8 @@ EXPORTED_SYMBOLS = (
     "gwn_moment_order",
     "gwn_compute_winding_number_exact_batch_f32", "gwn_compute_winding_number_taylor_batch_f32",
     "gwn_compute_winding_number_exact_batch_ex_f32", "gwn_compute_winding_number_taylor_batch_ex_f32",
+    "gwn_compute_winding_gradient_taylor_batch_f32", "gwn_compute_winding_gradient_taylor_batch_ex_f32",
+    "gwn_compute_unsigned_distance_batch_f32", "gwn_compute_ray_first_hit_batch_f32",
     "gwn_compute_query_order_f32", "gwn_radix_sort_pairs_u64", "gwn_radix_sort_pairs_u32",
 )
 
@@ -148,6 +150,11 @@ def _load():
         "gwn_compute_winding_number_taylor_batch_f32": [vp, vp, vp, vp, vp, vp, u64, f32, vp, vp],
         "gwn_compute_winding_number_exact_batch_ex_f32": [vp, vp, vp, vp, vp, u64, i32, vp, vp],
         "gwn_compute_winding_number_taylor_batch_ex_f32": [vp, vp, vp, vp, vp, vp, u64, f32, i32, vp, vp, vp, vp],
+        "gwn_compute_winding_gradient_taylor_batch_f32": [vp, vp, vp, vp, vp, vp, vp, vp, u64, f32, vp, vp],
+        "gwn_compute_winding_gradient_taylor_batch_ex_f32": [
+            vp, vp, vp, vp, vp, vp, vp, vp, vp, u64, f32, i32, vp, vp, vp, vp],
+        "gwn_compute_unsigned_distance_batch_f32": [vp, vp, vp, vp, vp, u64, f32, vp, vp],
+        "gwn_compute_ray_first_hit_batch_f32": [vp, vp, vp, vp, vp, vp, vp, vp, u64, f32, f32, vp, vp],
         "gwn_compute_query_order_f32": [vp, vp, vp, vp, vp, u64, vp],
         "gwn_radix_sort_pairs_u64": [vp, vp, u64, i32, i32, vp],
         "gwn_radix_sort_pairs_u32": [vp, vp, u64, i32, i32, vp],
@@ -420,6 +427,74 @@ def gwn_compute_winding_number_taylor_batch(bvh: BvhObject, moment: MomentObject
         bvh.handle, moment.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx),
         float(accuracy_scale), int(precision), ctypes.byref(cfg), _dptr(order), _dptr(counters),
         _stream(stream)))
+
+
+def gwn_compute_winding_gradient_taylor_batch(bvh: BvhObject, moment: MomentObject, qx, qy, qz, gx, gy, gz,
+                                              accuracy_scale: float = 2.0, config: QueryConfig | None = None,
+                                              stream=None, precision: Precision = Precision.PLAIN,
+                                              out=None, counters=None, order=None) -> Status:
+    """gwn_compute_winding_gradient_taylor_batch<Order> (include/gwn/gwn_query.cuh:684-756).
+
+    Writes the analytic gradient of the Taylor winding number to gx, gy, gz; ``out`` (optional)
+    receives the winding number of the same traversal.  ``precision``, ``order`` and ``counters``
+    as for ``gwn_compute_winding_number_taylor_batch``.
+    """
+    bad = _check_soa(qx, qy, qz, [gx, gy, gz] + ([out] if out is not None else []))
+    if bad:
+        return bad
+    if order is not None and _numel(order) != _numel(qx):
+        return Status(Error.INVALID_ARGUMENT, "Query order span size must match query count.")
+    cfg = _config(config)
+    if precision == Precision.PLAIN and out is None and counters is None and order is None:
+        return _status(_lib.gwn_compute_winding_gradient_taylor_batch_f32(
+            bvh.handle, moment.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(gx), _dptr(gy), _dptr(gz),
+            _numel(qx), float(accuracy_scale), ctypes.byref(cfg), _stream(stream)))
+    return _status(_lib.gwn_compute_winding_gradient_taylor_batch_ex_f32(
+        bvh.handle, moment.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _dptr(gx), _dptr(gy), _dptr(gz),
+        _numel(qx), float(accuracy_scale), int(precision), ctypes.byref(cfg), _dptr(order), _dptr(counters),
+        _stream(stream)))
+
+
+def gwn_compute_unsigned_distance_batch(bvh: BvhObject, qx, qy, qz, out, culling_band: float = float("inf"),
+                                        config: QueryConfig | None = None, stream=None) -> Status:
+    """gwn_compute_unsigned_distance_batch (include/gwn/gwn_query.cuh:129-180)."""
+    bad = _check_soa(qx, qy, qz, [out])
+    if bad:
+        return bad
+    cfg = _config(config)
+    return _status(_lib.gwn_compute_unsigned_distance_batch_f32(
+        bvh.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx), float(culling_band),
+        ctypes.byref(cfg), _stream(stream)))
+
+
+# gwn_ray_first_hit_result<float, uint32_t> as a numpy dtype (32 bytes per record).
+RAY_HIT_DTYPE = np.dtype([("t", np.float32), ("u", np.float32), ("v", np.float32), ("nx", np.float32),
+                          ("ny", np.float32), ("nz", np.float32), ("primitive_id", np.uint32),
+                          ("status", np.uint8), ("pad", np.uint8, (3,))])
+
+
+class RayStatus(enum.IntEnum):
+    """gwn_ray_first_hit_status (include/gwn/gwn_query.cuh:63-68)."""
+
+    MISS = 0
+    HIT = 1
+    OVERFLOW = 2
+
+
+def gwn_compute_ray_first_hit_batch(bvh: BvhObject, ox, oy, oz, dx, dy, dz, out, t_min: float = 0.0,
+                                    t_max: float = float("inf"), config: QueryConfig | None = None,
+                                    stream=None) -> Status:
+    """gwn_compute_ray_first_hit_batch (include/gwn/gwn_query.cuh:205-290).
+
+    ``out`` is a device buffer of 32 bytes per ray (e.g. a uint8 tensor of 32 * count or an int32
+    tensor of 8 * count); view it on the host with ``RAY_HIT_DTYPE``."""
+    n = _numel(ox)
+    if any(_numel(a) != n for a in (oy, oz, dx, dy, dz)) or int(out.numel() * out.element_size()) != 32 * n:
+        return Status(Error.INVALID_ARGUMENT, "ray first-hit: mismatched span sizes")
+    cfg = _config(config)
+    return _status(_lib.gwn_compute_ray_first_hit_batch_f32(
+        bvh.handle, _dptr(ox), _dptr(oy), _dptr(oz), _dptr(dx), _dptr(dy), _dptr(dz), _dptr(out), n,
+        float(t_min), float(t_max), ctypes.byref(cfg), _stream(stream)))
 
 
 def gwn_compute_query_order(bvh: BvhObject, qx, qy, qz, order, stream=None) -> Status:

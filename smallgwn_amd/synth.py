@@ -348,6 +348,17 @@ def closed_cube() -> Mesh:
     return _mesh_from_arrays(np.stack([vx, vy, vz], axis=1), np.stack([i0, i1, i2], axis=1))
 
 
+def layered_triangles(count: int) -> Mesh:
+    """`count` copies of the triangle (0,0,z), (1,0,z), (0,1,z) at z = 0, 1, ..., count - 1
+    (the layered fixture of the reference's distance and ray tests)."""
+    v = []
+    f = []
+    for k in range(count):
+        v += [[0.0, 0.0, float(k)], [1.0, 0.0, float(k)], [0.0, 1.0, float(k)]]
+        f.append([3 * k, 3 * k + 1, 3 * k + 2])
+    return _mesh_from_arrays(np.asarray(v, dtype=np.float32), np.asarray(f, dtype=np.int64))
+
+
 def open_cube() -> Mesh:
     vx = [-1, 1, 1, -1, -1, 1, 1, -1]
     vy = [-1, -1, 1, 1, -1, -1, 1, 1]

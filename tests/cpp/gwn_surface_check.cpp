@@ -45,6 +45,10 @@ int main() {
     expect(st.error() == gwn::gwn_error::invalid_argument, "Taylor batch on empty BVH");
     expect(st.message().find("BVH") != std::string::npos, "Taylor batch message names the BVH");
 
+    st = gwn::gwn_compute_winding_gradient_taylor_batch<1>(moved, moment, empty_in, empty_in, empty_in, empty_out,
+                                                          empty_out, empty_out);
+    expect(st.error() == gwn::gwn_error::invalid_argument, "gradient batch on empty BVH");
+
     // Length mismatch is caught by the header before the call.
     Real dummy[4] = {0, 0, 0, 0};
     gwn::gwn_device_span<Real const> const four(dummy, 4);

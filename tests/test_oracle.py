@@ -378,3 +378,350 @@ def test_split_replay_descending_everything_equals_fp64_exact(orc):
     assert np.abs(split - ref).max() <= 2e-7
     # The plain FP32 form is measurably worse near the surface (the split is not a no-op).
     assert np.abs(plain - ref).max() > 4 * np.abs(split - ref).max()
+
+
+# ---------------------------------------------------------------------------------------------
+# Gradients
+# ---------------------------------------------------------------------------------------------
+
+def _open_mesh():
+    """Open displaced sphere (30 % of faces deleted): its winding number varies smoothly off the
+    surface, so gradient errors are visible.  On a closed mesh the winding number is piecewise
+    constant and the exact gradient vanishes, which would hide a wrong sign or factor."""
+    return synth.open_soup(16, 8, seed=11)
+
+
+def _shell_queries(n, radius, seed):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    q = d / np.linalg.norm(d, axis=1, keepdims=True) * radius
+    q = np.round(q * 256) / 256  # dyadic, so q +- 2^-10 is exact in float32
+    return tuple(q[:, k].astype(np.float32) for k in range(3))
+
+
+def _fd_exact_gradient(orc, mesh, qx, qy, qz, h):
+    """Central differences of the FP64 exact winding number."""
+    out = []
+    for axis in range(3):
+        p = [np.array(v, dtype=np.float64) for v in (qx, qy, qz)]
+        m = [np.array(v, dtype=np.float64) for v in (qx, qy, qz)]
+        p[axis] = p[axis] + h
+        m[axis] = m[axis] - h
+        wp = orc.winding_exact_mesh_f64(mesh, *[v.astype(np.float32) for v in p])
+        wm = orc.winding_exact_mesh_f64(mesh, *[v.astype(np.float32) for v in m])
+        out.append((wp - wm) / (2 * h))
+    return out
+
+
+def test_exact_gradient_matches_finite_differences(orc):
+    """Biot-Savart triangle gradients summed in FP64 = central differences of the FP64 exact
+    winding number (h = 2^-10) on an open mesh, inside and outside."""
+    mesh = _open_mesh()
+    for radius, seed in ((0.5, 1), (1.7, 2)):
+        qx, qy, qz = _shell_queries(48, radius, seed)
+        g = orc.winding_gradient_exact_mesh_f64(mesh, qx, qy, qz)
+        fd = _fd_exact_gradient(orc, mesh, qx, qy, qz, 2.0 ** -10)
+        scale = max(np.abs(f).max() for f in fd)
+        assert scale > 1e-2  # the open mesh gives a nonvanishing field
+        for k in range(3):
+            assert np.abs(g[k] - fd[k]).max() <= 1e-5 * scale
+
+
+def test_closed_mesh_gradient_vanishes(orc):
+    """Off the surface of a closed mesh the winding number is constant: zero exact gradient."""
+    mesh = synth.asymmetric_octahedron()
+    qx, qy, qz = _shell_queries(16, 2.5, 3)
+    g = orc.winding_gradient_exact_mesh_f64(mesh, qx, qy, qz)
+    assert max(np.abs(v).max() for v in g) < 1e-9
+
+
+def test_taylor_gradient_descending_everything_equals_exact(orc):
+    """tests/unit_gwn_query_gradient.cu:211-240 on an open mesh: beta = 1e6 descends every leaf;
+    the replay's reference-form float gradient equals the FP64 exact gradient to 3e-3, the split
+    form to FP32-term accuracy."""
+    mesh = _open_mesh()
+    bvh = orc.build_bvh4(mesh)
+    mom = orc.refit_moment(bvh, 2)
+    qx, qy, qz = synth.near_surface_log_queries(mesh, 64, seed=8, lo=1e-3, hi=1e-1)
+    ref = orc.winding_gradient_exact_mesh_f64(mesh, qx, qy, qz)
+    _, *g = orc.winding_gradient_taylor_f32(bvh, mom, 2, qx, qy, qz, accuracy_scale=1e6)
+    _, *gs = orc.winding_gradient_taylor_f32(bvh, mom, 2, qx, qy, qz, accuracy_scale=1e6, near_fp64=True)
+    for k in range(3):
+        scale = 1 + np.abs(ref[k])
+        assert (np.abs(g[k] - ref[k]) / scale).max() <= 3e-3
+        assert (np.abs(gs[k] - ref[k]) / scale).max() <= 1e-5
+
+
+@pytest.mark.parametrize("meshname", ["asym_octahedron", "open"])
+def test_taylor_gradient_order_progression(orc, meshname):
+    """tests/unit_gwn_query_gradient.cu:153-199 (the reference's closed octahedron and queries),
+    and the same on an open mesh: at beta = 0 the mean error vs the exact gradient falls with
+    each order; on the octahedron order 2 is below 3e-3."""
+    mesh = synth.asymmetric_octahedron() if meshname == "asym_octahedron" else _open_mesh()
+    bvh = orc.build_bvh4(mesh)
+    if meshname == "asym_octahedron":
+        qx = np.float32([2.9, -2.7, 0.4, -0.6, 3.2, -2.8, 1.8, -2.1])
+        qy = np.float32([0.4, -0.8, 3.1, -2.9, 2.2, 1.7, -2.6, 2.4])
+        qz = np.float32([1.1, -0.5, 0.8, -1.2, -2.3, 2.5, 2.8, -2.5])
+    else:
+        qx, qy, qz = _shell_queries(16, 4.0, 5)
+    ref = np.stack(orc.winding_gradient_exact_mesh_f64(mesh, qx, qy, qz), 1)
+    errs = []
+    for order in (0, 1, 2):
+        _, *g = orc.winding_gradient_taylor_f32(bvh, orc.refit_moment(bvh, order), order, qx, qy, qz,
+                                                accuracy_scale=0.0)
+        errs.append(np.linalg.norm(np.stack(g, 1) - ref, axis=1).mean())
+    assert errs[1] < errs[0] and errs[2] < errs[1]
+    if meshname == "asym_octahedron":
+        assert errs[2] < 3e-3
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_taylor_gradient_is_the_derivative_of_the_taylor_winding(orc, order):
+    """At beta = 0 the replay sums the root's expansions, a smooth function of q: its analytic
+    gradient must equal central differences of the replayed winding itself (pins every far-field
+    gradient term, including the order-1 and order-2 ones, against the winding terms)."""
+    mesh = _open_mesh()
+    bvh = orc.build_bvh4(mesh)
+    mom = orc.refit_moment(bvh, order)
+    qx, qy, qz = _shell_queries(32, 2.5, 6)
+    _, *g = orc.winding_gradient_taylor_f32(bvh, mom, order, qx, qy, qz, accuracy_scale=0.0)
+    h = 2.0 ** -7
+    for axis in range(3):
+        p = [qx.copy(), qy.copy(), qz.copy()]
+        m = [qx.copy(), qy.copy(), qz.copy()]
+        p[axis] += np.float32(h)
+        m[axis] -= np.float32(h)
+        wp = orc.winding_taylor_f32(bvh, mom, order, *p, accuracy_scale=0.0).astype(np.float64)
+        wm = orc.winding_taylor_f32(bvh, mom, order, *m, accuracy_scale=0.0).astype(np.float64)
+        fd = (wp - wm) / (2 * h)
+        scale = np.abs(g[axis]).max()
+        assert scale > 1e-3
+        assert np.abs(g[axis] - fd).max() <= 2e-3 * scale + 2e-6
+
+
+def test_taylor_gradient_nan_query(orc):
+    """tests/unit_gwn_query_gradient.cu:269-288: a NaN coordinate gives an all-NaN gradient."""
+    mesh = synth.asymmetric_octahedron()
+    bvh = orc.build_bvh4(mesh)
+    mom = orc.refit_moment(bvh, 2)
+    _, *g = orc.winding_gradient_taylor_f32(bvh, mom, 2, np.float32([np.nan]), np.float32([0.25]),
+                                            np.float32([3.0]), accuracy_scale=0.0)
+    assert all(np.isnan(v[0]) for v in g)
+
+
+# ---------------------------------------------------------------------------------------------
+# Unsigned distance
+# ---------------------------------------------------------------------------------------------
+
+def _segment_distance2(p, a, b):
+    ab = b - a
+    t = np.clip(np.dot(p - a, ab) / np.dot(ab, ab), 0.0, 1.0) if np.dot(ab, ab) > 0 else 0.0
+    d = p - (a + t * ab)
+    return float(np.dot(d, d))
+
+
+def _triangle_distance2_candidates(p, a, b, c):
+    """Independent FP64 restatement: the closest point is the plane projection when it falls
+    inside the triangle, else the closest point of one of the three edges."""
+    n = np.cross(b - a, c - a)
+    best = min(_segment_distance2(p, a, b), _segment_distance2(p, b, c), _segment_distance2(p, c, a))
+    nn = np.dot(n, n)
+    if nn > 0:
+        t = np.dot(p - a, n) / nn
+        x = p - t * n
+        # barycentric sign test
+        s1 = np.dot(np.cross(b - a, x - a), n)
+        s2 = np.dot(np.cross(c - b, x - b), n)
+        s3 = np.dot(np.cross(a - c, x - c), n)
+        if s1 >= 0 and s2 >= 0 and s3 >= 0:
+            best = min(best, float(np.dot(p - x, p - x)))
+    return best
+
+
+def test_point_triangle_distance_matches_independent_form(orc):
+    rng = np.random.default_rng(12)
+    for _ in range(400):
+        a, b, c = rng.normal(size=(3, 3))
+        p = rng.normal(size=3) * 2
+        got = orc.point_triangle_distance2_f32(p, a, b, c)
+        a32, b32, c32, p32 = (v.astype(np.float32).astype(np.float64) for v in (a, b, c, p))
+        ref = _triangle_distance2_candidates(p32, a32, b32, c32)
+        assert abs(got - ref) <= 1e-5 * (1 + ref)
+    # Regions: vertex, edge and face of the unit right triangle.
+    a, b, c = np.float32([0, 0, 0]), np.float32([1, 0, 0]), np.float32([0, 1, 0])
+    assert orc.point_triangle_distance2_f32(np.float32([-1, -1, 0]), a, b, c) == 2.0
+    assert orc.point_triangle_distance2_f32(np.float32([0.5, -2, 0]), a, b, c) == 4.0
+    assert orc.point_triangle_distance2_f32(np.float32([0.25, 0.25, 3]), a, b, c) == 9.0
+
+
+def test_distance_layered_kats(orc):
+    """tests/unit_gwn_query_distance_ray.cu:222-270, 350-381 and 555-599."""
+    one = orc.build_bvh4(synth.layered_triangles(1))
+    f = np.float32
+    assert orc.unsigned_distance_f32(one, f([0.25]), f([0.25]), f([2]))[0] == 2.0
+    assert orc.unsigned_distance_f32(one, f([2]), f([0]), f([0]))[0] == 1.0
+    assert orc.unsigned_distance_f32(one, f([0.25]), f([0.25]), f([2]), culling_band=0.5)[0] == 0.5
+    assert orc.unsigned_distance_f32(one, f([0.25]), f([0.25]), f([2]), culling_band=-4.0)[0] == 0.0
+    twelve = orc.build_bvh4(synth.layered_triangles(12))
+    d = orc.unsigned_distance_f32(twelve, f([0.25, 0.25, 0.0]), f([0.25, 0.25, 0.25]), f([10.25, 30, 10.25]),
+                                  culling_band=2.0)
+    assert d.tolist() == [0.25, 2.0, 0.25]
+    four = orc.build_bvh4(synth.layered_triangles(4))
+    assert four.internal_stack_bound == 0 and four.packed_stack_bound == 3
+    assert orc.unsigned_distance_f32(four, f([0.25]), f([0.25]), f([3.5]), stack_capacity=1)[0] == 0.5
+
+
+@pytest.mark.parametrize("name", ["torus", "soup"])
+def test_distance_traversal_equals_brute_force(orc, name):
+    if name == "torus":
+        mesh = synth.torus(24, 12)
+    else:
+        rng = np.random.default_rng(8)
+        v = rng.normal(size=(300, 3)).astype(np.float32)
+        mesh = synth._mesh_from_arrays(v, rng.integers(0, 300, size=(200, 3)))
+    bvh = orc.build_bvh4(mesh)
+    qx, qy, qz = synth.uniform_box_points(4, 64, (-2.0,) * 3, (2.0,) * 3)
+    got = orc.unsigned_distance_f32(bvh, qx, qy, qz)
+    tris = bvh.triangles
+    v0 = tris[:, 0:3]
+    v1 = v0 + (tris[:, 3:6] - v0)  # the reference's (v0, e1, e2) reconstruction, in float32
+    v2 = v0 + (tris[:, 6:9] - v0)
+    for i in range(64):
+        p = np.float32([qx[i], qy[i], qz[i]])
+        best = min(orc.point_triangle_distance2_f32(p, v0[k], v1[k], v2[k]) for k in range(tris.shape[0]))
+        # Culling only skips boxes no closer than the best so far: the traversal finds the same
+        # minimum as scanning every triangle with the same float function.
+        assert got[i] == np.float32(np.sqrt(np.float32(best)))
+
+
+# ---------------------------------------------------------------------------------------------
+# Ray first hit
+# ---------------------------------------------------------------------------------------------
+
+def _ray(orc, mesh, o, d, t_min=0.0, t_max=float("inf"), traversal="auto"):
+    bvh = orc.build_bvh4(mesh)
+    o = np.asarray(o, dtype=np.float32).reshape(-1, 3)
+    d = np.asarray(d, dtype=np.float32).reshape(-1, 3)
+    return orc.ray_first_hit_f32(bvh, o[:, 0], o[:, 1], o[:, 2], d[:, 0], d[:, 1], d[:, 2], t_min, t_max,
+                                 traversal=traversal)
+
+
+def _mesh(x, y, z, i0, i1, i2):
+    return synth._mesh_from_arrays(np.stack([x, y, z], 1), np.stack([i0, i1, i2], 1))
+
+
+def test_ray_leaf_kats(orc):
+    """tests/unit_gwn_query_distance_ray.cu:222-298."""
+    one = synth.layered_triangles(1)
+    h = _ray(orc, one, [0.25, 0.25, 1], [0, 0, -1])[0]
+    assert h["status"] == 1 and h["primitive_id"] == 0 and h["t"] == 1.0
+    assert abs(h["u"] - 0.25) <= 1e-6 and abs(h["v"] - 0.25) <= 1e-6
+    assert (h["nx"], h["ny"], h["nz"]) == (0.0, 0.0, 1.0)
+    assert _ray(orc, one, [0.25, 0.25, 1], [0, 0, -1], 1.0, 1.0)[0]["t"] == 1.0
+    miss = _ray(orc, one, [0.25, 0.25, 1], [0, 0, 0], 0.0, 10.0)[0]
+    assert miss["status"] == 0 and miss["t"] == -1.0 and miss["primitive_id"] == 0xFFFFFFFF
+    h = _ray(orc, one, [0.25, 0.25, 1], [1e-30, 0, -1], 0.0, 2.0)[0]
+    assert h["status"] == 1 and abs(h["t"] - 1) <= 1e-6
+    assert _ray(orc, one, [0.25, 0.25, 0], [1, 0, 0], 0.0, 2.0)[0]["status"] == 0  # coplanar
+    h = _ray(orc, one, [0.25, 0.25, 1e-6], [0, 0, -1], 0.0, 1e-5)[0]
+    assert h["status"] == 1 and abs(h["t"] - 1e-6) <= 1e-10
+
+
+@pytest.mark.parametrize("scale,tol", [(1e-10, 1e-6), (1e-20, 1e-5)])
+def test_ray_tiny_and_subnormal_triangles(orc, scale, tol):
+    """tests/unit_gwn_query_distance_ray.cu:300-348."""
+    s = np.float32(scale)
+    mesh = _mesh([0, s, 0], [0, 0, s], [0, 0, 0], [0], [1], [2])
+    h = _ray(orc, mesh, [s * np.float32(0.25), s * np.float32(0.25), 1], [0, 0, -1], 0.0, 2.0)[0]
+    assert h["status"] == 1
+    assert abs(h["u"] - 0.25) <= tol and abs(h["v"] - 0.25) <= tol
+    if scale == 1e-20:
+        assert h["t"] == 1.0 and h["nz"] > 0
+
+
+def test_ray_layered_and_octahedron_kats(orc):
+    """tests/unit_gwn_query_distance_ray.cu:350-410, 465-482."""
+    twelve = synth.layered_triangles(12)
+    h = _ray(orc, twelve, [[0.25, 0.25, 20], [2, 2, 20], [0, 0.25, 20]], [[0, 0, -1]] * 3)
+    assert h["t"].tolist() == [9.0, -1.0, 9.0]
+    assert h["primitive_id"].tolist() == [11, 0xFFFFFFFF, 11]
+    octa = _mesh([1, -1, 0, 0, 0, 0], [0, 0, 1, -1, 0, 0], [0, 0, 0, 0, 1, -1],
+                 [0, 2, 1, 3, 2, 1, 3, 0], [2, 1, 3, 0, 0, 2, 1, 3], [4, 4, 4, 4, 5, 5, 5, 5])
+    h = _ray(orc, octa, [[2, 2, 2], [2, 0.2, 0.1], [2, 2, 2]], [[-1, -1, -1], [-1, 0, 0], [1, 1, 1]])
+    assert abs(h["t"][0] - 5 / 3) <= 1e-5 and h["primitive_id"][0] == 0
+    assert abs(h["t"][1] - 1.3) <= 1e-5 and h["primitive_id"][1] == 0
+    assert h["t"][2] == -1.0 and h["primitive_id"][2] == 0xFFFFFFFF
+    sixteen = synth.layered_triangles(16)
+    h = _ray(orc, sixteen, [0.25, 0.25, 20], [0, 0, 1], -30.0, 0.0)[0]
+    assert h["status"] == 1 and h["primitive_id"] == 0 and h["t"] == -20.0
+
+
+def test_ray_shared_edge_and_thin_bounds(orc):
+    """tests/unit_gwn_query_distance_ray.cu:412-463."""
+    quad = _mesh([0, 1, 1, 0], [0, 0, 1, 1], [0, 0, 0, 0], [0, 0], [1, 2], [2, 3])
+    h = _ray(orc, quad, [0.5, 0.5, 1], [0, 0, -1], 0.0, 2.0)[0]
+    assert h["status"] == 1 and h["t"] == 1.0 and h["primitive_id"] in (0, 1)
+    assert np.isfinite(h["u"]) and np.isfinite(h["v"])
+    thin = synth.layered_triangles(16)
+    thin = synth.Mesh((thin.vx * np.float32(1e-25)).astype(np.float32), thin.vy, thin.vz, thin.i0, thin.i1, thin.i2)
+    x = np.float32(0.25e-25)
+    packed = _ray(orc, thin, [x, 0.25, -1], [0, 0, 1], traversal="packed")[0]
+    canonical = _ray(orc, thin, [x, 0.25, -1], [0, 0, 1], traversal="canonical")[0]
+    assert packed["status"] == 1 and packed["primitive_id"] == canonical["primitive_id"]
+    assert packed["t"] == canonical["t"] == 1.0
+
+
+def _moller_trumbore_f64(o, d, a, b, c):
+    e1, e2 = b - a, c - a
+    p = np.cross(d, e2)
+    det = np.dot(e1, p)
+    if abs(det) < 1e-300:
+        return None
+    inv = 1.0 / det
+    s = o - a
+    u = np.dot(s, p) * inv
+    if u < 0 or u > 1:
+        return None
+    q = np.cross(s, e1)
+    v = np.dot(d, q) * inv
+    if v < 0 or u + v > 1:
+        return None
+    return np.dot(e2, q) * inv, u, v
+
+
+def test_ray_matches_fp64_brute_force(orc):
+    """tests/integration_gwn_static.cu:343-353: hit status and t against FP64 Moller-Trumbore
+    (relative 5e-5) over every triangle; primitive id and barycentrics where the nearest hit is
+    unambiguous."""
+    mesh = synth.displaced_sphere(24, 12, seed=4)
+    bvh = orc.build_bvh4(mesh)
+    rng = np.random.default_rng(6)
+    n = 256
+    o = rng.normal(size=(n, 3))
+    o = o / np.linalg.norm(o, axis=1, keepdims=True) * 3.0
+    tgt = rng.uniform(-0.8, 0.8, size=(n, 3))
+    d = tgt - o
+    o32, d32 = o.astype(np.float32), d.astype(np.float32)
+    hits = orc.ray_first_hit_f32(bvh, *(o32[:, k] for k in range(3)), *(d32[:, k] for k in range(3)))
+    canonical = orc.ray_first_hit_f32(bvh, *(o32[:, k] for k in range(3)), *(d32[:, k] for k in range(3)),
+                                      traversal="canonical")
+    # Both traversals find the same nearest hit (no exact ties on this mesh).
+    assert np.array_equal(hits["t"], canonical["t"]) and np.array_equal(hits["primitive_id"], canonical["primitive_id"])
+    V = np.stack([mesh.vx, mesh.vy, mesh.vz], 1).astype(np.float64)
+    F = np.stack([mesh.i0, mesh.i1, mesh.i2], 1).astype(np.int64)
+    for i in range(n):
+        oo, dd = o32[i].astype(np.float64), d32[i].astype(np.float64)
+        ts = []
+        for f, (ia, ib, ic) in enumerate(F):
+            r = _moller_trumbore_f64(oo, dd, V[ia], V[ib], V[ic])
+            if r is not None and r[0] >= 0:
+                ts.append((r[0], f, r[1], r[2]))
+        ts.sort()
+        assert hits["status"][i] == (1 if ts else 0)
+        if not ts:
+            continue
+        assert abs(hits["t"][i] - ts[0][0]) <= 5e-5 * abs(ts[0][0])
+        if len(ts) == 1 or ts[1][0] - ts[0][0] > 1e-4 * ts[0][0]:
+            assert hits["primitive_id"][i] == ts[0][1]
+            assert abs(hits["u"][i] - ts[0][2]) <= 1e-4 and abs(hits["v"][i] - ts[0][3]) <= 1e-4
