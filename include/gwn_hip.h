@@ -57,6 +57,7 @@ typedef enum gwn_precision {
 typedef struct gwn_geometry_handle *gwn_geometry_t;
 typedef struct gwn_bvh_handle *gwn_bvh_t;
 typedef struct gwn_moment_handle *gwn_moment_t;
+typedef struct gwn_boundary_handle *gwn_boundary_t;
 
 /* Reference gwn_query_batch_config (include/gwn/gwn_query.cuh:41-61): block size 256 and
  * stack capacity 64 by default.  sort_queries: -1 auto, 0 off, 1 on (queries are traversed in
@@ -143,6 +144,20 @@ int gwn_bvh_get_info(gwn_bvh_t bvh, gwn_bvh_info *info);
 int gwn_bvh_download(
     gwn_bvh_t bvh, void *nodes, uint32_t *primitive_indices, float *triangles, hipStream_t stream);
 
+/* ---- boundary chain: replaces gwn_boundary_chain_object<uint32_t> and gwn_build_boundary_chain
+ *      (include/gwn/gwn_boundary.cuh:28-125, detail/gwn_boundary_impl.cuh:21-413).  Directed rows
+ *      (start, end, multiplicity) of the algebraic boundary, in (min vertex, max vertex) order;
+ *      a closed mesh builds an empty chain.  Strong guarantee (staging + swap). ---- */
+int gwn_boundary_create(gwn_boundary_t *out);
+int gwn_boundary_destroy(gwn_boundary_t boundary);
+int gwn_build_boundary_chain(gwn_geometry_t geometry, gwn_boundary_t boundary, hipStream_t stream);
+int gwn_boundary_info(
+    gwn_boundary_t boundary, uint64_t *edge_count, uint64_t *mesh_vertex_count, uint64_t *mesh_triangle_count,
+    int32_t *is_built);
+/* Copies the rows to host memory (any pointer may be NULL) and synchronizes. */
+int gwn_boundary_download(
+    gwn_boundary_t boundary, uint32_t *start, uint32_t *end, uint64_t *multiplicity, hipStream_t stream);
+
 /* ---- Taylor moments: replaces gwn_bvh_moment_object<4, Order, float> and
  *      gwn_refit_bvh_moment<Order> (include/gwn/gwn_bvh_refit.cuh:31-51) ---- */
 int gwn_moment_create(gwn_moment_t *out, int32_t order);
@@ -209,6 +224,34 @@ int gwn_compute_ray_first_hit_batch_f32(
     gwn_bvh_t bvh, const float *ox, const float *oy, const float *oz, const float *dx, const float *dy,
     const float *dz, gwn_ray_first_hit_result *out, uint64_t count, float t_min, float t_max,
     const gwn_query_config *config, hipStream_t stream);
+
+/* ---- antipodal winding number: replaces gwn_compute_winding_number_antipodal_batch<Config>
+ *      (include/gwn/gwn_query.cuh:492-552, detail/gwn_query_winding_antipodal_impl.cuh:55-490).
+ *      Signed crossings of the positive coordinate ray with the BVH's triangles plus the boundary
+ *      chain's spherical areas over 4 pi; a singular axis retries in the order +Z, +X, +Y and a
+ *      query singular on all three (or whose traversal exceeds the stack) gets NaN.  The boundary
+ *      chain must come from geometry's topology and the BVH from the same geometry. ---- */
+int gwn_compute_winding_number_antipodal_batch_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, gwn_boundary_t boundary, const float *qx, const float *qy,
+    const float *qz, float *out, uint64_t count, const gwn_query_config *config, hipStream_t stream);
+/* Precision split (no reference counterpart): classification and crossings in FP32, each
+ * boundary area in FP64 on the exact vertices, summed in FP64. */
+int gwn_compute_winding_number_antipodal_batch_ex_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, gwn_boundary_t boundary, const float *qx, const float *qy,
+    const float *qz, float *out, uint64_t count, int32_t precision, const gwn_query_config *config,
+    hipStream_t stream);
+
+/* ---- antipodal gradient: replaces gwn_compute_winding_gradient_antipodal_batch<Config>
+ *      (include/gwn/gwn_query.cuh:582-640, detail/gwn_query_winding_antipodal_impl.cuh:257-376,
+ *      492-552).  Gradient of the boundary area term; (NaN, NaN, NaN) when all axes are
+ *      singular.  config->stack_capacity is not used. ---- */
+int gwn_compute_winding_gradient_antipodal_batch_f32(
+    gwn_geometry_t geometry, gwn_boundary_t boundary, const float *qx, const float *qy, const float *qz,
+    float *gx, float *gy, float *gz, uint64_t count, const gwn_query_config *config, hipStream_t stream);
+int gwn_compute_winding_gradient_antipodal_batch_ex_f32(
+    gwn_geometry_t geometry, gwn_boundary_t boundary, const float *qx, const float *qy, const float *qz,
+    float *gx, float *gy, float *gz, uint64_t count, int32_t precision, const gwn_query_config *config,
+    hipStream_t stream);
 
 /* Morton-order permutation of query points within the BVH root bounds (30-bit keys, stable
  * radix sort): order[i] is the input index of the i-th query to visit.  Reusable across

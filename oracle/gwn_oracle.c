@@ -1601,3 +1601,483 @@ void gwn_oracle_ray_first_hit_f32(
                    ox, oy, oz, dx, dy, dz, t_min, t_max, stack_capacity, traversal, out};
     parallel_ranges(ray_range, &ctx, ray_count, threads);
 }
+
+/* ---------------------------------------------------------------------------
+ * Boundary chain, include/gwn/detail/gwn_boundary_impl.cuh:21-354: every oriented triangle edge
+ * becomes an undirected (lo, hi) key with orientation +1 (lo -> hi), -1 (hi -> lo) or 0 (equal
+ * endpoints); keys are ordered by (lo, hi); each key group with a nonzero orientation sum emits
+ * one directed row (lo -> hi for a positive sum, hi -> lo for a negative one) with multiplicity
+ * |sum|, in key order.
+ * ------------------------------------------------------------------------- */
+
+typedef struct {
+    uint64_t key; /* lo << 32 | hi */
+    int orientation;
+} boundary_edge;
+
+static int cmp_boundary_edge(const void *pa, const void *pb)
+{
+    const boundary_edge *a = (const boundary_edge *)pa, *b = (const boundary_edge *)pb;
+    return (a->key > b->key) - (a->key < b->key);
+}
+
+int gwn_oracle_boundary_chain(
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
+    uint32_t *start, uint32_t *end, uint64_t *multiplicity, uint64_t *row_count)
+{
+    uint64_t m = 3 * triangle_count;
+    boundary_edge *e = (boundary_edge *)malloc(sizeof(boundary_edge) * (m ? m : 1));
+    if (!e)
+        return GWN_ORACLE_INTERNAL_ERROR;
+    for (uint64_t t = 0; t < triangle_count; ++t) {
+        uint32_t v[3] = {i0[t], i1[t], i2[t]};
+        for (int k = 0; k < 3; ++k) {
+            uint32_t a = v[k], b = v[(k + 1) % 3];
+            uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+            e[3 * t + k].key = ((uint64_t)lo << 32) | hi;
+            e[3 * t + k].orientation = a < b ? 1 : (b < a ? -1 : 0);
+        }
+    }
+    qsort(e, m, sizeof(boundary_edge), cmp_boundary_edge);
+    uint64_t rows = 0;
+    for (uint64_t i = 0; i < m;) {
+        uint64_t j = i;
+        long long sum = 0;
+        while (j < m && e[j].key == e[i].key)
+            sum += e[j++].orientation;
+        if (sum != 0) {
+            uint32_t lo = (uint32_t)(e[i].key >> 32), hi = (uint32_t)e[i].key;
+            if (start) {
+                start[rows] = sum > 0 ? lo : hi;
+                end[rows] = sum > 0 ? hi : lo;
+                multiplicity[rows] = (uint64_t)(sum > 0 ? sum : -sum);
+            }
+            ++rows;
+        }
+        i = j;
+    }
+    *row_count = rows;
+    free(e);
+    return GWN_ORACLE_OK;
+}
+
+/* ---------------------------------------------------------------------------
+ * Antipodal winding number and gradient, include/gwn/detail/
+ * gwn_query_winding_antipodal_impl.cuh:55-552: signed crossings of a positive coordinate ray
+ * with the triangles, plus the signed spherical areas of the boundary edges seen from the
+ * query; a singular axis (projected edge through the query, triangle through the query) restarts
+ * both terms on the next axis in the order Z, X, Y; NaN when all three are singular.
+ * near_fp64 != 0 (precision split) evaluates each boundary area / area gradient in FP64 on the
+ * exact vertices and sums in FP64; classification and crossings stay in FP32.
+ * ------------------------------------------------------------------------- */
+
+enum { AXIS_X = 0, AXIS_Y = 1, AXIS_Z = 2 };
+
+static void project2(const float *v, int axis, float *p)
+{
+    if (axis == AXIS_X) { p[0] = v[1]; p[1] = v[2]; }
+    else if (axis == AXIS_Y) { p[0] = v[2]; p[1] = v[0]; }
+    else { p[0] = v[0]; p[1] = v[1]; }
+}
+
+static int sign_of(float v) { return v > 0.0f ? 1 : (v < 0.0f ? -1 : 0); }
+
+/* (:64-110) returns the perturbed sign; *singular when the projected segment holds the query. */
+static int classify_edge(const float *s, const float *e, int axis, int *singular, float *det)
+{
+    float a[2], b[2];
+    project2(s, axis, a);
+    project2(e, axis, b);
+    float d = a[0] * b[1] - a[1] * b[0];
+    int sg = sign_of(d);
+    if (sg == 0) {
+        sg = sign_of(a[1] - b[1]);
+        if (sg == 0)
+            sg = sign_of(b[0] - a[0]);
+    }
+    *singular = d == 0.0f && (a[0] * b[0] + a[1] * b[1] <= 0.0f);
+    *det = d;
+    return sg;
+}
+
+/* (:122-176) returns 1 when singular; *crossing receives +-1 or stays 0. */
+static int triangle_crossing(const float *o, int axis, const float *a, const float *b, const float *c, float *crossing)
+{
+    float pa[3], pb[3], pc[3], u[3], v[3], n[3];
+    for (int k = 0; k < 3; ++k) {
+        pa[k] = a[k] - o[k];
+        pb[k] = b[k] - o[k];
+        pc[k] = c[k] - o[k];
+    }
+    for (int k = 0; k < 3; ++k) {
+        u[k] = pb[k] - pa[k];
+        v[k] = pc[k] - pa[k];
+    }
+    cross3f(u, v, n);
+    float na = n[axis];
+    if (na == 0.0f)
+        return 0;
+    int s1, s2, s3;
+    float d;
+    int e1 = classify_edge(pa, pb, axis, &s1, &d);
+    int e2 = classify_edge(pb, pc, axis, &s2, &d);
+    int e3 = classify_edge(pc, pa, axis, &s3, &d);
+    if (s1 || s2 || s3)
+        return 1;
+    if (e1 == 0 || e2 == 0 || e3 == 0)
+        return 1;
+    if (!((e1 >= 0 && e2 >= 0 && e3 >= 0) || (e1 <= 0 && e2 <= 0 && e3 <= 0)))
+        return 0;
+    float t = dot3f(n, pa) / na;
+    if (t == 0.0f)
+        return 1;
+    if (!(t > 0.0f))
+        return 0;
+    *crossing = na > 0.0f ? 1.0f : -1.0f;
+    return 0;
+}
+
+typedef struct {
+    const gwn_oracle_node4_f32 *nodes;
+    uint64_t node_count;
+    gwn_oracle_child_f32 root;
+    const float *tri;
+    uint64_t nt;
+    const float *vx, *vy, *vz;
+    const uint32_t *bs, *be;
+    const uint64_t *bm;
+    uint64_t n_edges;
+    const float *qx, *qy, *qz;
+    int cap;
+    int near_fp64;
+    float *out;
+    float *gx, *gy, *gz;
+} antipodal_ctx;
+
+static int positive_axis_hits_box(const float *o, int axis, const float *b)
+{
+    if (axis == AXIS_X)
+        return o[1] >= b[1] && o[1] <= b[4] && o[2] >= b[2] && o[2] <= b[5] && b[3] >= o[0];
+    if (axis == AXIS_Y)
+        return o[2] >= b[2] && o[2] <= b[5] && o[0] >= b[0] && o[0] <= b[3] && b[4] >= o[1];
+    return o[0] >= b[0] && o[0] <= b[3] && o[1] >= b[1] && o[1] <= b[4] && b[5] >= o[2];
+}
+
+/* Returns 1 when singular; *sum receives the signed crossing count (NaN on stack overflow). */
+static int leaf_crossings(const antipodal_ctx *c, uint64_t ref, const float *o, int axis, float *sum)
+{
+    uint64_t off = ref & GWN_ORACLE_OFFSET_MASK;
+    uint32_t n = (uint32_t)(ref >> 48);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (off + k >= c->nt)
+            continue;
+        float a[3], b[3], cc[3], cr = 0.0f;
+        record_vertices_f32(c->tri + 9 * (off + k), a, b, cc);
+        if (triangle_crossing(o, axis, a, b, cc, &cr))
+            return 1;
+        *sum += cr;
+    }
+    return 0;
+}
+
+static int crossing_count(const antipodal_ctx *c, const float *o, int axis, float *sum)
+{
+    *sum = 0.0f;
+    if ((c->root.reference & GWN_ORACLE_VALID_BIT) && (c->root.reference >> 48) != 0)
+        return leaf_crossings(c, c->root.reference, o, axis, sum);
+    uint32_t stack[1024];
+    int sp = 0;
+    uint32_t node = (uint32_t)(c->root.reference & GWN_ORACLE_OFFSET_MASK);
+    for (;;) {
+        uint32_t next = GWN_ORACLE_INVALID_INDEX;
+        if (node < c->node_count) {
+            const gwn_oracle_node4_f32 *nd = &c->nodes[node];
+            for (int s = 0; s < 4; ++s) {
+                const gwn_oracle_child_f32 *ch = &nd->children[s];
+                if (!(ch->reference & GWN_ORACLE_VALID_BIT))
+                    continue;
+                if (!positive_axis_hits_box(o, axis, ch->bounds))
+                    continue;
+                if ((ch->reference >> 48) != 0) {
+                    if (leaf_crossings(c, ch->reference, o, axis, sum))
+                        return 1;
+                    continue;
+                }
+                if (next != GWN_ORACLE_INVALID_INDEX) {
+                    if (sp >= c->cap || sp >= 1024) {
+                        *sum = NAN;
+                        return 0;
+                    }
+                    stack[sp++] = next;
+                }
+                next = (uint32_t)(ch->reference & GWN_ORACLE_OFFSET_MASK);
+            }
+        }
+        if (next != GWN_ORACLE_INVALID_INDEX) {
+            node = next;
+            continue;
+        }
+        if (sp == 0)
+            return 0;
+        node = stack[--sp];
+    }
+}
+
+static void minus_ray(int axis, float *m)
+{
+    m[0] = -(axis == AXIS_X ? 1.0f : 0.0f);
+    m[1] = -(axis == AXIS_Y ? 1.0f : 0.0f);
+    m[2] = -(axis == AXIS_Z ? 1.0f : 0.0f);
+}
+
+/* (:178-204) */
+static float spherical_area_f32(const float *m, const float *s, const float *e, float det)
+{
+    float sn = sqrtf(dot3f(s, s)), en = sqrtf(dot3f(e, e));
+    if (!(sn > 0.0f) || !(en > 0.0f))
+        return NAN;
+    float num = -det;
+    if (det == 0.0f) {
+        float cr[3];
+        cross3f(s, e, cr);
+        num = dot3f(m, cr);
+    }
+    float den = sn * en + dot3f(m, s) * en + dot3f(s, e) + dot3f(e, m) * sn;
+    return 2.0f * (float)atan2((double)num, (double)den);
+}
+
+static double dot3d(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static void cross3d(const double *a, const double *b, double *r)
+{
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* FP64 area of the same term (precision split); the branch sign comes from the FP32 class. */
+static double spherical_area_f64(const double *m, const double *s, const double *e)
+{
+    double sn = sqrt(dot3d(s, s)), en = sqrt(dot3d(e, e));
+    if (!(sn > 0.0) || !(en > 0.0))
+        return NAN;
+    double cr[3];
+    cross3d(s, e, cr);
+    double num = dot3d(m, cr);
+    double den = sn * en + dot3d(m, s) * en + dot3d(s, e) + dot3d(e, m) * sn;
+    return 2.0 * atan2(num, den);
+}
+
+/* (:210-255) returns 1 when singular. */
+static int boundary_term(const antipodal_ctx *c, const float *q, int axis, float *value, double *value_d)
+{
+    const float pi = (float)3.141592653589793238462643383279502884L;
+    const double pi_d = 3.141592653589793238462643383279502884;
+    float m[3];
+    minus_ray(axis, m);
+    double md[3] = {m[0], m[1], m[2]};
+    float sum = 0.0f, comp = 0.0f;
+    double sum_d = 0.0;
+    for (uint64_t i = 0; i < c->n_edges; ++i) {
+        uint32_t a = c->bs[i], b = c->be[i];
+        float s[3] = {c->vx[a] - q[0], c->vy[a] - q[1], c->vz[a] - q[2]};
+        float e[3] = {c->vx[b] - q[0], c->vy[b] - q[1], c->vz[b] - q[2]};
+        int singular;
+        float det;
+        int sg = classify_edge(s, e, axis, &singular, &det);
+        if (singular)
+            return 1;
+        if (c->near_fp64) {
+            double sd[3] = {(double)c->vx[a] - q[0], (double)c->vy[a] - q[1], (double)c->vz[a] - q[2]};
+            double ed[3] = {(double)c->vx[b] - q[0], (double)c->vy[b] - q[1], (double)c->vz[b] - q[2]};
+            double area = spherical_area_f64(md, sd, ed);
+            if (axis != AXIS_Z && sg > 0 && area < 0.0)
+                area += 2.0 * pi_d;
+            sum_d += (double)c->bm[i] * area;
+            continue;
+        }
+        float area = spherical_area_f32(m, s, e, det);
+        if (axis != AXIS_Z && sg > 0 && area < 0.0f)
+            area += 2.0f * pi;
+        float term = (float)c->bm[i] * area;
+        float ct = term - comp;
+        float next = sum + ct;
+        comp = (next - sum) - ct;
+        sum = next;
+    }
+    *value = sum / (4.0f * pi);
+    *value_d = sum_d / (4.0 * pi_d);
+    return 0;
+}
+
+static void antipodal_one(const antipodal_ctx *c, uint64_t qi)
+{
+    float q[3] = {c->qx[qi], c->qy[qi], c->qz[qi]};
+    const int order[3] = {AXIS_Z, AXIS_X, AXIS_Y};
+    float result = NAN;
+    for (int k = 0; k < 3; ++k) {
+        float crossings;
+        if (crossing_count(c, q, order[k], &crossings))
+            continue;
+        float b;
+        double bd;
+        if (boundary_term(c, q, order[k], &b, &bd))
+            continue;
+        result = c->near_fp64 ? (float)((double)crossings + bd) : crossings + b;
+        break;
+    }
+    c->out[qi] = result;
+}
+
+/* (:257-334) */
+static void area_gradient_f32(const float *m, const float *s, const float *e, float det, float *g)
+{
+    float sn = sqrtf(dot3f(s, s)), en = sqrtf(dot3f(e, e));
+    if (sn == 0.0f || en == 0.0f) {
+        g[0] = g[1] = g[2] = NAN;
+        return;
+    }
+    float is = 1.0f / sn, ie = 1.0f / en;
+    float su[3] = {s[0] * is, s[1] * is, s[2] * is};
+    float eu[3] = {e[0] * ie, e[1] * ie, e[2] * ie};
+    float num = -det;
+    if (det == 0.0f) {
+        float cr[3];
+        cross3f(s, e, cr);
+        num = dot3f(m, cr);
+    }
+    float den = sn * en + dot3f(m, s) * en + dot3f(m, e) * sn + dot3f(s, e);
+    float d2 = num * num + den * den;
+    if (d2 == 0.0f) {
+        g[0] = g[1] = g[2] = NAN;
+        return;
+    }
+    float cn = 2.0f * den / d2;
+    float cd = -2.0f * num / d2;
+    float c1[3], c2[3], gn[3], t1[3], t2[3], gd[3];
+    cross3f(e, m, c1);
+    cross3f(m, s, c2);
+    for (int k = 0; k < 3; ++k)
+        gn[k] = -(c1[k] + c2[k]);
+    float f1 = 1.0f + dot3f(m, eu);
+    float f2 = 1.0f + dot3f(m, su);
+    for (int k = 0; k < 3; ++k) {
+        t1[k] = en * (m[k] + eu[k] + f1 * su[k]);
+        t2[k] = sn * (m[k] + su[k] + f2 * eu[k]);
+        gd[k] = -(t1[k] + t2[k]);
+    }
+    for (int k = 0; k < 3; ++k)
+        g[k] = cn * gn[k] + cd * gd[k];
+}
+
+static void area_gradient_f64(const double *m, const double *s, const double *e, double *g)
+{
+    double sn = sqrt(dot3d(s, s)), en = sqrt(dot3d(e, e));
+    if (sn == 0.0 || en == 0.0) {
+        g[0] = g[1] = g[2] = NAN;
+        return;
+    }
+    double su[3] = {s[0] / sn, s[1] / sn, s[2] / sn};
+    double eu[3] = {e[0] / en, e[1] / en, e[2] / en};
+    double cr[3];
+    cross3d(s, e, cr);
+    double num = dot3d(m, cr);
+    double den = sn * en + dot3d(m, s) * en + dot3d(m, e) * sn + dot3d(s, e);
+    double d2 = num * num + den * den;
+    if (d2 == 0.0) {
+        g[0] = g[1] = g[2] = NAN;
+        return;
+    }
+    double cn = 2.0 * den / d2, cd = -2.0 * num / d2;
+    double c1[3], c2[3];
+    cross3d(e, m, c1);
+    cross3d(m, s, c2);
+    double f1 = 1.0 + dot3d(m, eu), f2 = 1.0 + dot3d(m, su);
+    for (int k = 0; k < 3; ++k) {
+        double gn = -(c1[k] + c2[k]);
+        double gdk = -(en * (m[k] + eu[k] + f1 * su[k]) + sn * (m[k] + su[k] + f2 * eu[k]));
+        g[k] = cn * gn + cd * gdk;
+    }
+}
+
+/* (:337-376) returns 1 when singular. */
+static int boundary_gradient(const antipodal_ctx *c, const float *q, int axis, float *out3)
+{
+    const float pi = (float)3.141592653589793238462643383279502884L;
+    const double pi_d = 3.141592653589793238462643383279502884;
+    float m[3];
+    minus_ray(axis, m);
+    double md[3] = {m[0], m[1], m[2]};
+    float sum[3] = {0.0f, 0.0f, 0.0f}, comp[3] = {0.0f, 0.0f, 0.0f};
+    double sum_d[3] = {0.0, 0.0, 0.0};
+    for (uint64_t i = 0; i < c->n_edges; ++i) {
+        uint32_t a = c->bs[i], b = c->be[i];
+        float s[3] = {c->vx[a] - q[0], c->vy[a] - q[1], c->vz[a] - q[2]};
+        float e[3] = {c->vx[b] - q[0], c->vy[b] - q[1], c->vz[b] - q[2]};
+        int singular;
+        float det;
+        classify_edge(s, e, axis, &singular, &det);
+        if (singular)
+            return 1;
+        if (c->near_fp64) {
+            double sd[3] = {(double)c->vx[a] - q[0], (double)c->vy[a] - q[1], (double)c->vz[a] - q[2]};
+            double ed[3] = {(double)c->vx[b] - q[0], (double)c->vy[b] - q[1], (double)c->vz[b] - q[2]};
+            double g[3];
+            area_gradient_f64(md, sd, ed, g);
+            for (int k = 0; k < 3; ++k)
+                sum_d[k] += (double)c->bm[i] * g[k];
+            continue;
+        }
+        float g[3];
+        area_gradient_f32(m, s, e, det, g);
+        float mult = (float)c->bm[i];
+        for (int k = 0; k < 3; ++k) {
+            float term = mult * g[k];
+            float ct = term - comp[k];
+            float next = sum[k] + ct;
+            comp[k] = (next - sum[k]) - ct;
+            sum[k] = next;
+        }
+    }
+    for (int k = 0; k < 3; ++k)
+        out3[k] = c->near_fp64 ? (float)(sum_d[k] / (4.0 * pi_d)) : sum[k] / (4.0f * pi);
+    return 0;
+}
+
+static void antipodal_gradient_one(const antipodal_ctx *c, uint64_t qi)
+{
+    float q[3] = {c->qx[qi], c->qy[qi], c->qz[qi]};
+    const int order[3] = {AXIS_Z, AXIS_X, AXIS_Y};
+    float g[3] = {NAN, NAN, NAN};
+    for (int k = 0; k < 3; ++k)
+        if (!boundary_gradient(c, q, order[k], g))
+            break;
+        else
+            g[0] = g[1] = g[2] = NAN;
+    c->gx[qi] = g[0];
+    c->gy[qi] = g[1];
+    c->gz[qi] = g[2];
+}
+
+static void antipodal_range(void *vctx, uint64_t begin, uint64_t end)
+{
+    antipodal_ctx *c = (antipodal_ctx *)vctx;
+    for (uint64_t q = begin; q < end; ++q) {
+        if (c->out)
+            antipodal_one(c, q);
+        if (c->gx)
+            antipodal_gradient_one(c, q);
+    }
+}
+
+void gwn_oracle_winding_antipodal_f32(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
+    const float *triangles, uint64_t triangle_count, const float *vx, const float *vy, const float *vz,
+    const uint32_t *boundary_start, const uint32_t *boundary_end, const uint64_t *boundary_multiplicity,
+    uint64_t boundary_edges, const float *qx, const float *qy, const float *qz, uint64_t query_count,
+    int stack_capacity, int near_fp64, float *out, float *gx, float *gy, float *gz, int threads)
+{
+    antipodal_ctx ctx = {nodes, node_count, root, triangles, triangle_count, vx, vy, vz, boundary_start,
+                         boundary_end, boundary_multiplicity, boundary_edges, qx, qy, qz, stack_capacity,
+                         near_fp64, out, gx, gy, gz};
+    parallel_ranges(antipodal_range, &ctx, query_count, threads);
+}

@@ -23,6 +23,8 @@
  *   - Taylor traversal: include/gwn/detail/gwn_query_winding_impl.cuh:102-374
  *   - solid-angle gradient and Taylor gradient terms: gwn_query_winding_impl.cuh:20-99, 217-325
  *   - ray first hit: include/gwn/detail/gwn_query_ray_impl.cuh:20-595, gwn_query.cuh:214-290
+ *   - boundary chain: include/gwn/detail/gwn_boundary_impl.cuh:21-354
+ *   - antipodal winding and gradient: include/gwn/detail/gwn_query_winding_antipodal_impl.cuh:55-552
  *   - unsigned distance: include/gwn/detail/gwn_query_distance_impl.cuh:20-192,
  *     gwn_query_geometry_impl.cuh:46-114
  *
@@ -214,6 +216,28 @@ void gwn_oracle_ray_first_hit_f32(
     uint32_t packed_stack_bound, const float *ox, const float *oy, const float *oz, const float *dx,
     const float *dy, const float *dz, uint64_t ray_count, float t_min, float t_max, int stack_capacity,
     int traversal, gwn_oracle_ray_hit *out, int threads);
+
+/*
+ * Algebraic boundary of the oriented triangle chain (gwn_boundary_impl.cuh:21-354): directed rows
+ * (start, end, multiplicity) in (min vertex, max vertex) key order.  Output arrays may be NULL
+ * to count rows only; otherwise they need room for 3 * triangle_count rows.
+ */
+int gwn_oracle_boundary_chain(
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
+    uint32_t *start, uint32_t *end, uint64_t *multiplicity, uint64_t *row_count);
+
+/*
+ * Antipodal winding number (out, optional) and its gradient (gx/gy/gz, optional) over the
+ * canonical BVH, the original vertices and the boundary chain
+ * (gwn_query_winding_antipodal_impl.cuh:55-552).  near_fp64 != 0: boundary areas and area
+ * gradients in FP64 on the exact vertices with FP64 sums (precision split).
+ */
+void gwn_oracle_winding_antipodal_f32(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
+    const float *triangles, uint64_t triangle_count, const float *vx, const float *vy, const float *vz,
+    const uint32_t *boundary_start, const uint32_t *boundary_end, const uint64_t *boundary_multiplicity,
+    uint64_t boundary_edges, const float *qx, const float *qy, const float *qz, uint64_t query_count,
+    int stack_capacity, int near_fp64, float *out, float *gx, float *gy, float *gz, int threads);
 
 /* Squared point-triangle distance (gwn_query_geometry_impl.cuh:46-97). */
 float gwn_oracle_point_triangle_distance2_f32(const float *p, const float *a, const float *b, const float *c);

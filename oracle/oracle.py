@@ -98,6 +98,11 @@ def lib():
         L.gwn_oracle_ray_first_hit_f32.argtypes = [
             vp, u64, _Child, vp, vp, u64, ctypes.c_uint32, vp, vp, vp, vp, vp, vp, u64, f32, f32, i32, i32, vp, i32,
         ]
+        L.gwn_oracle_boundary_chain.restype = i32
+        L.gwn_oracle_boundary_chain.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp]
+        L.gwn_oracle_winding_antipodal_f32.argtypes = [
+            vp, u64, _Child, vp, u64, vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, u64, i32, i32, vp, vp, vp, vp, i32,
+        ]
         L.gwn_oracle_point_triangle_distance2_f32.restype = f32
         L.gwn_oracle_point_triangle_distance2_f32.argtypes = [vp, vp, vp, vp]
         L.gwn_oracle_winding_gradient_exact_mesh_f64.argtypes = [
@@ -375,6 +380,47 @@ def ray_first_hit_f32(bvh: OracleBvh, ox, oy, oz, dx, dy, dz, t_min: float = 0.0
         {"auto": -1, "canonical": 0, "packed": 1}[traversal], _ptr(out), threads or default_threads(),
     )
     return out
+
+
+def boundary_chain(mesh):
+    """Directed boundary rows of the oriented triangle chain: (start, end, multiplicity)."""
+    i0, i1, i2 = _c(mesh.i0, np.uint32), _c(mesh.i1, np.uint32), _c(mesh.i2, np.uint32)
+    m = max(3 * i0.shape[0], 1)
+    start = np.zeros(m, np.uint32)
+    end = np.zeros(m, np.uint32)
+    mult = np.zeros(m, np.uint64)
+    rows = ctypes.c_uint64(0)
+    st = lib().gwn_oracle_boundary_chain(_ptr(i0), _ptr(i1), _ptr(i2), i0.shape[0], _ptr(start), _ptr(end),
+                                         _ptr(mult), ctypes.byref(rows))
+    if st != 0:
+        raise RuntimeError(f"oracle boundary chain failed with status {st}")
+    r = rows.value
+    return start[:r].copy(), end[:r].copy(), mult[:r].copy()
+
+
+def winding_antipodal_f32(bvh: OracleBvh, mesh, boundary, qx, qy, qz, gradient: bool = False,
+                          near_fp64: bool = False, stack_capacity: int = 64, threads: int | None = None):
+    """Antipodal winding number (and, with ``gradient``, (w, gx, gy, gz)) of the oriented mesh;
+    ``boundary`` is the (start, end, multiplicity) triple of ``boundary_chain``."""
+    nodes = _c(bvh.nodes, NODE_DTYPE)
+    tris = _c(bvh.triangles, np.float32)
+    vx, vy, vz = _c(mesh.vx, np.float32), _c(mesh.vy, np.float32), _c(mesh.vz, np.float32)
+    bs, be, bm = _c(boundary[0], np.uint32), _c(boundary[1], np.uint32), _c(boundary[2], np.uint64)
+    qx, qy, qz = _c(qx, np.float32), _c(qy, np.float32), _c(qz, np.float32)
+    n = qx.shape[0]
+    out = np.empty(n, np.float32)
+    g = [np.empty(n, np.float32) for _ in range(3)] if gradient else [None, None, None]
+    root = _Child()
+    for k in range(6):
+        root.bounds[k] = float(bvh.root_bounds[k])
+    root.reference = bvh.root_reference
+    lib().gwn_oracle_winding_antipodal_f32(
+        _ptr(nodes), bvh.node_count, root, _ptr(tris), tris.shape[0], _ptr(vx), _ptr(vy), _ptr(vz),
+        _ptr(bs) if bs.size else None, _ptr(be) if be.size else None, _ptr(bm) if bm.size else None, bs.shape[0],
+        _ptr(qx), _ptr(qy), _ptr(qz), n, stack_capacity, 1 if near_fp64 else 0, _ptr(out),
+        *[(_ptr(a) if a is not None else None) for a in g], threads or default_threads(),
+    )
+    return (out, *g) if gradient else out
 
 
 def point_triangle_distance2_f32(p, a, b, c) -> float:

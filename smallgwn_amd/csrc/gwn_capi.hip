@@ -319,6 +319,76 @@ int gwn_bvh_download(gwn_bvh_t bvh, void *nodes, uint32_t *primitive_indices, fl
 // moments
 // ------------------------------------------------------------------------------------------
 
+// ------------------------------------------------------------------------------------------
+// boundary chain
+// ------------------------------------------------------------------------------------------
+
+int gwn_boundary_create(gwn_boundary_t *out) {
+    return guarded("gwn_boundary_create", [&] {
+        if (out == nullptr)
+            gwn::throw_invalid("Output handle pointer is null.");
+        *out = new gwn_boundary_handle();
+    });
+}
+
+int gwn_boundary_destroy(gwn_boundary_t boundary) {
+    return guarded("gwn_boundary_destroy", [&] { delete boundary; });
+}
+
+int gwn_build_boundary_chain(gwn_geometry_t geometry, gwn_boundary_t boundary, hipStream_t stream) {
+    return guarded("gwn_build_boundary_chain", [&] {
+        if (boundary == nullptr)
+            gwn::throw_invalid("Boundary chain handle is null.");
+        if (geometry == nullptr || (geometry->triangle_count != 0 && geometry->i0.ptr == nullptr))
+            gwn::throw_invalid("Geometry object contains no queryable data.");
+        // Rows are placed by a scan with 30-bit partial counts.
+        if (geometry->triangle_count > ((1ull << 30) - 1ull) / 3ull)
+            gwn::throw_invalid("Boundary chain triangle count exceeds edge encoding.");
+        gwn_boundary_handle staging;
+        staging.stream = stream;
+        gwn::build_boundary_chain(*geometry, staging, stream);
+        std::swap(*boundary, staging);
+    });
+}
+
+int gwn_boundary_info(
+    gwn_boundary_t boundary, uint64_t *edge_count, uint64_t *mesh_vertex_count, uint64_t *mesh_triangle_count,
+    int32_t *is_built
+) {
+    return guarded("gwn_boundary_info", [&] {
+        if (boundary == nullptr)
+            gwn::throw_invalid("Boundary chain handle is null.");
+        if (edge_count)
+            *edge_count = boundary->edge_count;
+        if (mesh_vertex_count)
+            *mesh_vertex_count = boundary->mesh_vertex_count;
+        if (mesh_triangle_count)
+            *mesh_triangle_count = boundary->mesh_triangle_count;
+        if (is_built)
+            *is_built = boundary->built ? 1 : 0;
+    });
+}
+
+int gwn_boundary_download(
+    gwn_boundary_t boundary, uint32_t *start, uint32_t *end, uint64_t *multiplicity, hipStream_t stream
+) {
+    return guarded("gwn_boundary_download", [&] {
+        if (boundary == nullptr)
+            gwn::throw_invalid("Boundary chain handle is null.");
+        std::size_t const n = boundary->edge_count;
+        if (n == 0)
+            return;
+        if (start)
+            GWN_HIP(hipMemcpyAsync(start, boundary->start.ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        if (end)
+            GWN_HIP(hipMemcpyAsync(end, boundary->end.ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        if (multiplicity)
+            GWN_HIP(hipMemcpyAsync(multiplicity, boundary->multiplicity.ptr, n * sizeof(uint64_t),
+                                   hipMemcpyDeviceToHost, stream));
+        GWN_HIP(hipStreamSynchronize(stream));
+    });
+}
+
 int gwn_moment_create(gwn_moment_t *out, int32_t order) {
     return guarded("gwn_moment_create", [&] {
         if (out == nullptr)
@@ -521,6 +591,79 @@ int gwn_compute_ray_first_hit_batch_f32(
             return;
         gwn::ray_first_hit(*bvh, ox, oy, oz, dx, dy, dz, out, count, t_min, t_max, to_launch(c), stream);
     });
+}
+
+// Shared validation of the antipodal batches, in the reference's order (gwn_query.cuh:510-540,
+// 606-630).  `bvh` is null for the gradient, which reads the boundary term only.
+static void require_antipodal(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, bool needs_bvh, gwn_boundary_t boundary, int32_t precision,
+    gwn_query_config const &c
+) {
+    if (geometry == nullptr || (geometry->triangle_count != 0 && geometry->i0.ptr == nullptr))
+        gwn::throw_invalid("Geometry object contains no queryable data.");
+    if (needs_bvh) {
+        require_bvh(bvh);
+        if (bvh->triangle_count != geometry->triangle_count)
+            gwn::throw_invalid("Geometry and BVH triangle counts differ.");
+        require_stack(*bvh, c.stack_capacity);
+    }
+    if (boundary == nullptr || !boundary->built)
+        gwn::throw_invalid("Boundary chain object is unbuilt or invalid.");
+    if (boundary->mesh_vertex_count != geometry->vertex_count ||
+        boundary->mesh_triangle_count != geometry->triangle_count)
+        gwn::throw_invalid("Boundary chain belongs to a different mesh shape.");
+    if (precision != GWN_PRECISION_PLAIN && precision != GWN_PRECISION_SPLIT)
+        gwn::throw_invalid("Antipodal queries support the plain and split precision modes.");
+}
+
+int gwn_compute_winding_number_antipodal_batch_ex_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, gwn_boundary_t boundary, const float *qx, const float *qy,
+    const float *qz, float *out, uint64_t count, int32_t precision, const gwn_query_config *config,
+    hipStream_t stream
+) {
+    return guarded("gwn_compute_winding_number_antipodal_batch", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, true);
+        require_antipodal(geometry, bvh, true, boundary, precision, c);
+        require_query_storage(count, qx, qy, qz, out);
+        if (count == 0)
+            return;
+        gwn::winding_antipodal(*geometry, *bvh, *boundary, qx, qy, qz, out, count, precision, stream);
+    });
+}
+
+int gwn_compute_winding_number_antipodal_batch_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, gwn_boundary_t boundary, const float *qx, const float *qy,
+    const float *qz, float *out, uint64_t count, const gwn_query_config *config, hipStream_t stream
+) {
+    return gwn_compute_winding_number_antipodal_batch_ex_f32(geometry, bvh, boundary, qx, qy, qz, out, count,
+                                                             GWN_PRECISION_PLAIN, config, stream);
+}
+
+int gwn_compute_winding_gradient_antipodal_batch_ex_f32(
+    gwn_geometry_t geometry, gwn_boundary_t boundary, const float *qx, const float *qy, const float *qz,
+    float *gx, float *gy, float *gz, uint64_t count, int32_t precision, const gwn_query_config *config,
+    hipStream_t stream
+) {
+    return guarded("gwn_compute_winding_gradient_antipodal_batch", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, false);
+        require_antipodal(geometry, nullptr, false, boundary, precision, c);
+        require_query_storage(count, qx, qy, qz, gx);
+        if (count != 0 && (gy == nullptr || gz == nullptr))
+            gwn::throw_invalid("Query/output spans must use non-null storage when non-empty.");
+        if (count == 0)
+            return;
+        gwn::winding_gradient_antipodal(*geometry, *boundary, qx, qy, qz, gx, gy, gz, count, precision, stream);
+    });
+}
+
+int gwn_compute_winding_gradient_antipodal_batch_f32(
+    gwn_geometry_t geometry, gwn_boundary_t boundary, const float *qx, const float *qy, const float *qz,
+    float *gx, float *gy, float *gz, uint64_t count, const gwn_query_config *config, hipStream_t stream
+) {
+    return gwn_compute_winding_gradient_antipodal_batch_ex_f32(geometry, boundary, qx, qy, qz, gx, gy, gz, count,
+                                                               GWN_PRECISION_PLAIN, config, stream);
 }
 
 int gwn_compute_query_order_f32(

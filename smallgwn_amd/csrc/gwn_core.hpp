@@ -222,6 +222,16 @@ struct gwn_bvh_handle {
     }
 };
 
+struct gwn_boundary_handle {
+    gwn::device_buffer<std::uint32_t> start, end; // directed boundary edge endpoints
+    gwn::device_buffer<std::uint64_t> multiplicity;
+    std::size_t edge_count = 0;
+    std::size_t mesh_vertex_count = 0;
+    std::size_t mesh_triangle_count = 0;
+    bool built = false;
+    hipStream_t stream = nullptr;
+};
+
 struct gwn_moment_handle {
     int order = 0;
     gwn::device_buffer<float4> records; // node_count * moment_record_vec4(order)
@@ -246,6 +256,8 @@ void radix_sort_pairs_u32(
 );
 
 void build_bvh4(gwn_geometry_handle const &geometry, gwn_bvh_handle &staging, hipStream_t stream);
+// Algebraic boundary of the oriented triangle chain (directed rows in (min, max) vertex order).
+void build_boundary_chain(gwn_geometry_handle const &geometry, gwn_boundary_handle &staging, hipStream_t stream);
 void refit_moment(gwn_bvh_handle const &bvh, gwn_moment_handle &staging, hipStream_t stream);
 // In-place bounds refit: triangle records and child boxes from the geometry, root box updated.
 void refit_bounds(gwn_geometry_handle const &geometry, gwn_bvh_handle &bvh, hipStream_t stream);
@@ -279,6 +291,17 @@ void ray_first_hit(
     gwn_bvh_handle const &bvh, float const *ox, float const *oy, float const *oz, float const *dx,
     float const *dy, float const *dz, void *out, std::size_t n, float t_min, float t_max,
     query_launch const &cfg, hipStream_t stream
+);
+// Antipodal winding numbers: positive-axis crossings over the BVH triangles plus the boundary
+// chain's spherical areas; axes retried Z, X, Y, NaN when all three are singular.
+void winding_antipodal(
+    gwn_geometry_handle const &geometry, gwn_bvh_handle const &bvh, gwn_boundary_handle const &boundary,
+    float const *qx, float const *qy, float const *qz, float *out, std::size_t n, int precision, hipStream_t stream
+);
+// Antipodal winding gradient (boundary term only) into gx, gy, gz.
+void winding_gradient_antipodal(
+    gwn_geometry_handle const &geometry, gwn_boundary_handle const &boundary, float const *qx, float const *qy,
+    float const *qz, float *gx, float *gy, float *gz, std::size_t n, int precision, hipStream_t stream
 );
 // Morton-order visiting permutation of query points within the BVH root bounds.
 void query_order(

@@ -32,12 +32,16 @@ EXPORTED_SYMBOLS = (
     "gwn_update_geometry_f32", "gwn_update_geometry_device_f32", "gwn_geometry_counts",
     "gwn_bvh_create", "gwn_bvh_destroy", "gwn_build_bvh4_f32", "gwn_refit_bvh_f32", "gwn_bvh_get_info",
     "gwn_bvh_download",
+    "gwn_boundary_create", "gwn_boundary_destroy", "gwn_build_boundary_chain", "gwn_boundary_info",
+    "gwn_boundary_download",
     "gwn_moment_create", "gwn_moment_destroy", "gwn_refit_bvh_moment_f32", "gwn_moment_download",
     "gwn_moment_order",
     "gwn_compute_winding_number_exact_batch_f32", "gwn_compute_winding_number_taylor_batch_f32",
     "gwn_compute_winding_number_exact_batch_ex_f32", "gwn_compute_winding_number_taylor_batch_ex_f32",
     "gwn_compute_winding_gradient_taylor_batch_f32", "gwn_compute_winding_gradient_taylor_batch_ex_f32",
     "gwn_compute_unsigned_distance_batch_f32", "gwn_compute_ray_first_hit_batch_f32",
+    "gwn_compute_winding_number_antipodal_batch_f32", "gwn_compute_winding_number_antipodal_batch_ex_f32",
+    "gwn_compute_winding_gradient_antipodal_batch_f32", "gwn_compute_winding_gradient_antipodal_batch_ex_f32",
     "gwn_compute_query_order_f32", "gwn_radix_sort_pairs_u64", "gwn_radix_sort_pairs_u32",
 )
 
@@ -141,6 +145,11 @@ def _load():
         "gwn_refit_bvh_f32": [vp, vp, vp],
         "gwn_bvh_get_info": [vp, vp],
         "gwn_bvh_download": [vp, vp, vp, vp, vp],
+        "gwn_boundary_create": [vp],
+        "gwn_boundary_destroy": [vp],
+        "gwn_build_boundary_chain": [vp, vp, vp],
+        "gwn_boundary_info": [vp, vp, vp, vp, vp],
+        "gwn_boundary_download": [vp, vp, vp, vp, vp],
         "gwn_moment_create": [vp, i32],
         "gwn_moment_destroy": [vp],
         "gwn_refit_bvh_moment_f32": [vp, vp, vp],
@@ -155,6 +164,10 @@ def _load():
             vp, vp, vp, vp, vp, vp, vp, vp, vp, u64, f32, i32, vp, vp, vp, vp],
         "gwn_compute_unsigned_distance_batch_f32": [vp, vp, vp, vp, vp, u64, f32, vp, vp],
         "gwn_compute_ray_first_hit_batch_f32": [vp, vp, vp, vp, vp, vp, vp, vp, u64, f32, f32, vp, vp],
+        "gwn_compute_winding_number_antipodal_batch_f32": [vp, vp, vp, vp, vp, vp, vp, u64, vp, vp],
+        "gwn_compute_winding_number_antipodal_batch_ex_f32": [vp, vp, vp, vp, vp, vp, vp, u64, i32, vp, vp],
+        "gwn_compute_winding_gradient_antipodal_batch_f32": [vp, vp, vp, vp, vp, vp, vp, vp, u64, vp, vp],
+        "gwn_compute_winding_gradient_antipodal_batch_ex_f32": [vp, vp, vp, vp, vp, vp, vp, vp, u64, i32, vp, vp],
         "gwn_compute_query_order_f32": [vp, vp, vp, vp, vp, u64, vp],
         "gwn_radix_sort_pairs_u64": [vp, vp, u64, i32, i32, vp],
         "gwn_radix_sort_pairs_u32": [vp, vp, u64, i32, i32, vp],
@@ -307,6 +320,35 @@ class MomentObject(_Handle):
         return out
 
 
+class BoundaryChainObject(_Handle):
+    """gwn_boundary_chain_object<uint32_t> (include/gwn/gwn_boundary.cuh:62-115)."""
+
+    _create = "gwn_boundary_create"
+    _destroy = "gwn_boundary_destroy"
+
+    def info(self) -> dict:
+        e, v, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        b = ctypes.c_int32()
+        _status(_lib.gwn_boundary_info(self._h, ctypes.byref(e), ctypes.byref(v), ctypes.byref(t),
+                                       ctypes.byref(b))).raise_if_error()
+        return {"edge_count": e.value, "mesh_vertex_count": v.value, "mesh_triangle_count": t.value,
+                "is_built": bool(b.value)}
+
+    def has_data(self) -> bool:
+        return self.info()["is_built"]
+
+    def download(self, stream=None):
+        """Rows (start, end, multiplicity) as host arrays."""
+        n = self.info()["edge_count"]
+        start = np.zeros(n, np.uint32)
+        end = np.zeros(n, np.uint32)
+        mult = np.zeros(n, np.uint64)
+        if n:
+            _status(_lib.gwn_boundary_download(self._h, start.ctypes.data, end.ctypes.data, mult.ctypes.data,
+                                               _stream(stream))).raise_if_error()
+        return start, end, mult
+
+
 # ---------------------------------------------------------------------------------------------
 # free functions with the reference's names
 # ---------------------------------------------------------------------------------------------
@@ -352,6 +394,11 @@ def gwn_build_bvh(geometry: GeometryObject, bvh: BvhObject, options: BuildOption
     """gwn_build_bvh<4, float, uint32_t> (include/gwn/gwn_bvh_build.cuh:34-39)."""
     return _status(_lib.gwn_build_bvh4_f32(
         geometry.handle, bvh.handle, int(options.method), int(options.hploc_search_radius), _stream(stream)))
+
+
+def gwn_build_boundary_chain(geometry: GeometryObject, boundary: BoundaryChainObject, stream=None) -> Status:
+    """gwn_build_boundary_chain (include/gwn/gwn_boundary.cuh:117-125)."""
+    return _status(_lib.gwn_build_boundary_chain(geometry.handle, boundary.handle, _stream(stream)))
 
 
 def gwn_refit_bvh(geometry: GeometryObject, bvh: BvhObject, stream=None) -> Status:
@@ -495,6 +542,49 @@ def gwn_compute_ray_first_hit_batch(bvh: BvhObject, ox, oy, oz, dx, dy, dz, out,
     return _status(_lib.gwn_compute_ray_first_hit_batch_f32(
         bvh.handle, _dptr(ox), _dptr(oy), _dptr(oz), _dptr(dx), _dptr(dy), _dptr(dz), _dptr(out), n,
         float(t_min), float(t_max), ctypes.byref(cfg), _stream(stream)))
+
+
+def gwn_compute_winding_number_antipodal_batch(geometry: GeometryObject, bvh: BvhObject,
+                                               boundary: BoundaryChainObject, qx, qy, qz, out,
+                                               config: QueryConfig | None = None, stream=None,
+                                               precision: Precision = Precision.PLAIN) -> Status:
+    """gwn_compute_winding_number_antipodal_batch (include/gwn/gwn_query.cuh:492-552).
+
+    Positive-axis crossings over the BVH's triangles plus the boundary chain's spherical areas;
+    axes retried +Z, +X, +Y, NaN when all three are singular.  ``precision=Precision.SPLIT``
+    evaluates the boundary areas in FP64."""
+    n = _numel(qx)
+    if _numel(qy) != n or _numel(qz) != n:
+        return Status(Error.INVALID_ARGUMENT, "Query SoA spans require identical lengths.")
+    if _numel(out) != n:
+        return Status(Error.INVALID_ARGUMENT, "Output span size differs from query count.")
+    cfg = _config(config)
+    if precision == Precision.PLAIN:
+        return _status(_lib.gwn_compute_winding_number_antipodal_batch_f32(
+            geometry.handle, bvh.handle, boundary.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), n,
+            ctypes.byref(cfg), _stream(stream)))
+    return _status(_lib.gwn_compute_winding_number_antipodal_batch_ex_f32(
+        geometry.handle, bvh.handle, boundary.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), n,
+        int(precision), ctypes.byref(cfg), _stream(stream)))
+
+
+def gwn_compute_winding_gradient_antipodal_batch(geometry: GeometryObject, boundary: BoundaryChainObject,
+                                                 qx, qy, qz, gx, gy, gz, config: QueryConfig | None = None,
+                                                 stream=None, precision: Precision = Precision.PLAIN) -> Status:
+    """gwn_compute_winding_gradient_antipodal_batch (include/gwn/gwn_query.cuh:582-640)."""
+    n = _numel(qx)
+    if _numel(qy) != n or _numel(qz) != n:
+        return Status(Error.INVALID_ARGUMENT, "Query SoA spans require identical lengths.")
+    if _numel(gx) != n or _numel(gy) != n or _numel(gz) != n:
+        return Status(Error.INVALID_ARGUMENT, "Output span sizes differ from query count.")
+    cfg = _config(config)
+    if precision == Precision.PLAIN:
+        return _status(_lib.gwn_compute_winding_gradient_antipodal_batch_f32(
+            geometry.handle, boundary.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(gx), _dptr(gy), _dptr(gz),
+            n, ctypes.byref(cfg), _stream(stream)))
+    return _status(_lib.gwn_compute_winding_gradient_antipodal_batch_ex_f32(
+        geometry.handle, boundary.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(gx), _dptr(gy), _dptr(gz),
+        n, int(precision), ctypes.byref(cfg), _stream(stream)))
 
 
 def gwn_compute_query_order(bvh: BvhObject, qx, qy, qz, order, stream=None) -> Status:

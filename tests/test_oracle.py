@@ -725,3 +725,116 @@ def test_ray_matches_fp64_brute_force(orc):
         if len(ts) == 1 or ts[1][0] - ts[0][0] > 1e-4 * ts[0][0]:
             assert hits["primitive_id"][i] == ts[0][1]
             assert abs(hits["u"][i] - ts[0][2]) <= 1e-4 and abs(hits["v"][i] - ts[0][3]) <= 1e-4
+
+
+# ---------------------------------------------------------------------------------------------
+# Boundary chain
+# ---------------------------------------------------------------------------------------------
+
+def test_boundary_chain_kats(orc):
+    """tests/unit_gwn_boundary.cu:94-187: empty, closed (cube) and open-cube chains, net
+    orientation multiplicity 2 with the opposite direction cancelled."""
+    s, e, m = orc.boundary_chain(synth._mesh_from_arrays(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int64)))
+    assert s.size == 0
+    s, e, m = orc.boundary_chain(synth.closed_cube())
+    assert s.size == 0
+    s, e, m = orc.boundary_chain(synth.open_cube())
+    rows = sorted(zip(s.tolist(), e.tolist(), m.tolist()))
+    assert rows == [(4, 7, 1), (5, 4, 1), (6, 5, 1), (7, 6, 1)]
+    mesh = synth._mesh_from_arrays(np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]]),
+                                   np.stack([[0, 1, 0, 0], [1, 0, 1, 1], [2, 3, 2, 2]], 1))
+    s, e, m = orc.boundary_chain(mesh)
+    rows = list(zip(s.tolist(), e.tolist(), m.tolist()))
+    assert (0, 1, 2) in rows and not any(r[0] == 1 and r[1] == 0 for r in rows)
+
+
+def test_boundary_chain_is_the_algebraic_boundary(orc):
+    """Closed surfaces have no boundary; deleting faces leaves exactly the edges adjacent to one
+    remaining face, oriented with it; rows come in (min vertex, max vertex) order."""
+    assert orc.boundary_chain(synth.torus(16, 8))[0].size == 0
+    assert orc.boundary_chain(synth.icosphere(2))[0].size == 0
+    soup = synth.open_soup(16, 8, seed=2)
+    s, e, m = orc.boundary_chain(soup)
+    f = np.stack([soup.i0, soup.i1, soup.i2], 1).astype(np.int64)
+    from collections import Counter
+    net = Counter()
+    for a, b, c in f:
+        for u, v in ((a, b), (b, c), (c, a)):
+            if u != v:
+                net[(min(u, v), max(u, v))] += 1 if u < v else -1
+    expect = sorted(((k[0], k[1], v) if v > 0 else (k[1], k[0], -v)) for k, v in net.items() if v != 0)
+    assert sorted(zip(s.tolist(), e.tolist(), m.tolist())) == expect
+    keys = [(min(a, b), max(a, b)) for a, b in zip(s.tolist(), e.tolist())]
+    assert keys == sorted(keys)
+
+
+# ---------------------------------------------------------------------------------------------
+# Antipodal winding number
+# ---------------------------------------------------------------------------------------------
+
+def _antipodal(orc, mesh, qx, qy, qz, **kw):
+    return orc.winding_antipodal_f32(orc.build_bvh4(mesh), mesh, orc.boundary_chain(mesh),
+                                     np.float32(qx), np.float32(qy), np.float32(qz), **kw)
+
+
+AP_Q = ([0.23, 2.2, -0.41, -2.3], [0.17, 0.3, 0.26, -0.4], [0.11, 0.2, 1.7, 0.5])
+
+
+@pytest.mark.parametrize("meshname", ["closed_cube", "open_cube"])
+def test_antipodal_matches_exact(orc, meshname):
+    """tests/unit_gwn_query_antipodal.cu:332-396 (3e-4), including collinear boundary projections."""
+    mesh = getattr(synth, meshname)()
+    for q in (AP_Q, ([2, 2, -2, -2], [-1, 1, -1, 1], [2, 2, 2, 2])):
+        got = _antipodal(orc, mesh, *q)
+        ref = orc.winding_exact_mesh_f64(mesh, *(np.float32(v) for v in q))
+        assert np.abs(got - ref).max() <= 3e-4
+
+
+def test_antipodal_open_soup_matches_exact_both_precisions(orc):
+    mesh = synth.open_soup(16, 8, seed=3)
+    qx, qy, qz = synth.uniform_box_points(2, 256, (-1.4,) * 3, (1.4,) * 3)
+    ref = orc.winding_exact_mesh_f64(mesh, qx, qy, qz)
+    plain = _antipodal(orc, mesh, qx, qy, qz)
+    split = _antipodal(orc, mesh, qx, qy, qz, near_fp64=True)
+    assert np.abs(plain - ref).max() <= 3e-4
+    assert np.abs(split - ref).max() <= 1e-6
+    # Integer crossings + boundary term: the closed part contributes exactly 0 or 1.
+    closed = _antipodal(orc, synth.icosphere(2), qx, qy, qz)
+    exact_closed = orc.winding_exact_mesh_f64(synth.icosphere(2), qx, qy, qz)
+    assert np.array_equal(closed, np.round(exact_closed).astype(np.float32))
+
+
+def test_antipodal_gradient_matches_finite_differences(orc):
+    """tests/unit_gwn_query_antipodal.cu:398-438 (6e-3) and against the exact FP64 gradient."""
+    mesh = synth.open_cube()
+    q = ([0.21, 1.3, -1.4, 0.35], [0.18, 0.25, -0.33, 1.4], [0.15, 0.2, 0.42, 1.55])
+    w, gx, gy, gz = _antipodal(orc, mesh, *q, gradient=True)
+    eps = 1e-2
+    for axis, g in enumerate((gx, gy, gz)):
+        p = [np.float32(v) for v in q]
+        m = [np.float32(v) for v in q]
+        p[axis] = p[axis] + np.float32(eps)
+        m[axis] = m[axis] - np.float32(eps)
+        fd = (_antipodal(orc, mesh, *p).astype(np.float64) - _antipodal(orc, mesh, *m)) / (2 * eps)
+        assert np.abs(g - fd).max() <= 6e-3
+    ref = orc.winding_gradient_exact_mesh_f64(mesh, *(np.float32(v) for v in q))
+    for g, r in zip((gx, gy, gz), ref):
+        assert np.abs(g - r).max() <= 1e-5
+
+
+def test_antipodal_gradient_axis_retry_symmetry(orc):
+    """tests/unit_gwn_query_antipodal.cu:440-460: queries on the x = 1 face plane project a
+    boundary edge through the query for Z; the retried axis gives mirror-symmetric gradients."""
+    mesh = synth.open_cube()
+    q = ([1, 1, 1, 1], [0.25, -0.25, 0.5, -0.5], [0.2, 0.2, 0.4, 0.4])
+    _, gx, gy, gz = _antipodal(orc, mesh, *q, gradient=True)
+    assert all(np.all(np.isfinite(g)) for g in (gx, gy, gz))
+    for a, b in ((0, 1), (2, 3)):
+        assert abs(gx[a] - gx[b]) <= 1e-6 and abs(gy[a] + gy[b]) <= 1e-6 and abs(gz[a] - gz[b]) <= 1e-6
+
+
+def test_antipodal_all_axes_singular_is_nan(orc):
+    """A query on a boundary vertex projects an edge through itself on every axis."""
+    mesh = synth.open_cube()
+    w = _antipodal(orc, mesh, [1.0], [1.0], [1.0])
+    assert np.isnan(w[0])
