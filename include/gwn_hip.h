@@ -253,6 +253,24 @@ int gwn_compute_winding_gradient_antipodal_batch_ex_f32(
     float *gx, float *gy, float *gz, uint64_t count, int32_t precision, const gwn_query_config *config,
     hipStream_t stream);
 
+/* ---- adjoint batches (no reference counterpart; the reference differentiates only with respect
+ *      to the query point, include/gwn/gwn_query.cuh:656-756).  For a loss L with upstream
+ *      gradients grad_w[q] = dL/dw_q (Taylor winding, orders 0 and 1, with the forward batch's
+ *      accept/descend decisions for the same accuracy_scale) or grad_d[p] = dL/dd_p (unsigned
+ *      distance), the batches ADD dL/dV to grad_vx/vy/vz (vertex_count floats each, in the
+ *      geometry's vertex order) and write dL/dq to grad_qx/qy/qz (or dL/dp) when those are non-null.
+ *      The BVH and moments must describe the geometry's current positions (refit after an update).
+ *      Sums use FP32 atomics: results are reproducible to rounding, not bit for bit. ---- */
+int gwn_compute_winding_taylor_adjoint_batch_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy,
+    const float *qz, const float *grad_w, uint64_t count, float accuracy_scale, float *grad_vx, float *grad_vy,
+    float *grad_vz, float *grad_qx, float *grad_qy, float *grad_qz, const gwn_query_config *config,
+    hipStream_t stream);
+int gwn_compute_unsigned_distance_adjoint_batch_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, const float *px, const float *py, const float *pz,
+    const float *grad_d, uint64_t count, float culling_band, float *grad_vx, float *grad_vy, float *grad_vz,
+    float *grad_px, float *grad_py, float *grad_pz, const gwn_query_config *config, hipStream_t stream);
+
 /* Morton-order permutation of query points within the BVH root bounds (30-bit keys, stable
  * radix sort): order[i] is the input index of the i-th query to visit.  Reusable across
  * batches over the same points (no reference counterpart; an MI355X traversal aid). */

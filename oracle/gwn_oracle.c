@@ -190,6 +190,215 @@ static void aabb_union(float *dst, const float *a, const float *b)
     }
 }
 
+/* ---------------------------------------------------------------------------
+ * H-PLOC binary build, include/gwn/detail/gwn_bvh_build_hploc.cuh:104-478, run sequentially.
+ * Every sorted leaf climbs the implicit LBVH of the Morton codes (find_parent_id, :150-156,
+ * with the (code, position) delta of :141-149); the first path to reach a boundary publishes its
+ * range and stops, the second joins both ranges (:451-470).  A joined range larger than the
+ * 16-cluster threshold, or the full range, runs PLOC on the concatenation of its two children's
+ * cluster lists (ploc_merge, :420-447): nearest neighbours by the packed key of :195-241 (area
+ * bits << 1 with the low four bits replaced by (offset - 1) << 1 and a direction parity), mutual
+ * pairs merged with the lower lane keeping the new cluster (:343-347), survivors compacted in
+ * lane order (:392-404), until at most 16 clusters remain (1 for the full range).  The list is
+ * stored back at the start of the range, 32 lanes wide as on the device (:181-187).
+ *
+ * Node ids: the reference draws them from an atomic counter, whose order depends on scheduling.
+ * Here a merge of clusters whose smallest sorted leaves are a < b creates internal node b - 1,
+ * which is unique (a leaf is the smaller-side minimum of at most one merge) and independent of
+ * scheduling.  Left child = the lower lane's cluster, as in the reference (:366-367).
+ * ------------------------------------------------------------------------- */
+
+enum { HPLOC_LANES = 32, HPLOC_THRESHOLD = 16 };
+
+typedef struct {
+    uint64_t T;
+    const uint64_t *codes;
+    const float *leaf;             /* sorted leaf boxes */
+    gwn_oracle_binary_node *nodes;
+    float *bounds;
+    int64_t *first;
+    uint8_t *made;
+    uint32_t *clusters;
+    uint32_t radius;
+    uint64_t merges;
+    int failure;
+} hploc_state;
+
+static float half_area(const float *b);
+static void aabb_union(float *dst, const float *a, const float *b);
+
+static uint64_t hploc_delta(const uint64_t *codes, uint64_t a, uint64_t b)
+{
+    uint64_t d = codes[a] ^ codes[b];
+    return d == 0 ? (a ^ b) : d;
+}
+
+static const float *hploc_box(const hploc_state *h, uint32_t c)
+{
+    return c < h->T ? h->leaf + 6 * (uint64_t)c : h->bounds + 6 * (uint64_t)(c - h->T);
+}
+
+static uint32_t hploc_first(const hploc_state *h, uint32_t c)
+{
+    return c < h->T ? c : (uint32_t)h->first[c - h->T];
+}
+
+static uint32_t hploc_area_key(const float *a, const float *b)
+{
+    float u[6];
+    aabb_union(u, a, b);
+    float area = half_area(u);
+    uint32_t bits;
+    memcpy(&bits, &area, sizeof(bits));
+    return (bits << 1) & ~15u;
+}
+
+static void hploc_merge_range(hploc_state *h, uint64_t left, uint64_t right_end, uint64_t split, int final)
+{
+    uint32_t idx[HPLOC_LANES];
+    float box[HPLOC_LANES][6];
+    for (int l = 0; l < HPLOC_LANES; ++l)
+        idx[l] = GWN_ORACLE_INVALID_INDEX;
+    uint64_t lcl = split - left < HPLOC_THRESHOLD ? split - left : HPLOC_THRESHOLD;
+    uint32_t num_left = 0;
+    for (uint64_t k = 0; k < lcl; ++k) {
+        idx[k] = h->clusters[left + k];
+        num_left += idx[k] != GWN_ORACLE_INVALID_INDEX;
+    }
+    uint64_t lcr = right_end - split < HPLOC_THRESHOLD ? right_end - split : HPLOC_THRESHOLD;
+    uint32_t num_right = 0;
+    for (uint64_t k = 0; k < lcr; ++k) {
+        idx[num_left + k] = h->clusters[split + k];
+        num_right += idx[num_left + k] != GWN_ORACLE_INVALID_INDEX;
+    }
+    uint32_t total = num_left + num_right, num = total;
+    for (uint32_t l = 0; l < num; ++l) {
+        if (idx[l] == GWN_ORACLE_INVALID_INDEX || idx[l] >= 2 * h->T - 1) {
+            h->failure = 1;
+            return;
+        }
+        memcpy(box[l], hploc_box(h, idx[l]), sizeof(float) * 6);
+    }
+    uint32_t threshold = final ? 1u : (uint32_t)HPLOC_THRESHOLD;
+    while (num > threshold) {
+        uint32_t nn[HPLOC_LANES];
+        for (uint32_t l = 0; l < num; ++l) {
+            uint32_t best = 0xffffffffu;
+            for (uint32_t r = 1; r <= h->radius; ++r) {
+                if (l + r < num) {
+                    uint32_t key = hploc_area_key(box[l], box[l + r]) | ((r - 1) << 1) | (l & 1u);
+                    best = key < best ? key : best;
+                }
+                if (l >= r) {
+                    uint32_t key = hploc_area_key(box[l - r], box[l]) | ((r - 1) << 1) | ((l & 1u) ^ 1u);
+                    best = key < best ? key : best;
+                }
+            }
+            uint32_t off = ((best & 15u) >> 1) + 1u;
+            nn[l] = (((best ^ l) & 1u) == 0u) ? l + off : l - off;
+        }
+        int keep[HPLOC_LANES];
+        uint32_t merges = 0;
+        for (uint32_t l = 0; l < HPLOC_LANES; ++l) {
+            int mutual = 0, merge = 0;
+            if (l < num) {
+                if (nn[l] >= num || nn[l] == l) {
+                    h->failure = 1;
+                    return;
+                }
+                mutual = nn[nn[l]] == l;
+                merge = mutual && l < nn[l];
+            }
+            keep[l] = merge || !mutual;
+            if (!merge)
+                continue;
+            uint32_t a = idx[l], b = idx[nn[l]];
+            uint32_t fa = hploc_first(h, a), fb = hploc_first(h, b);
+            uint32_t id = (fa > fb ? fa : fb) - 1u;
+            if (fa == fb || id >= h->T - 1 || h->made[id]) {
+                h->failure = 1;
+                return;
+            }
+            h->made[id] = 1;
+            gwn_oracle_binary_node *nd = &h->nodes[id];
+            nd->left_index = a < h->T ? a : (uint32_t)(a - h->T);
+            nd->left_kind = a < h->T ? 2u : 1u;
+            nd->right_index = b < h->T ? b : (uint32_t)(b - h->T);
+            nd->right_kind = b < h->T ? 2u : 1u;
+            aabb_union(h->bounds + 6 * (uint64_t)id, box[l], box[nn[l]]);
+            h->first[id] = fa < fb ? fa : fb;
+            idx[l] = (uint32_t)h->T + id;
+            memcpy(box[l], h->bounds + 6 * (uint64_t)id, sizeof(float) * 6);
+            ++merges;
+        }
+        if (merges == 0) {
+            h->failure = 1;
+            return;
+        }
+        uint32_t k = 0;
+        for (uint32_t l = 0; l < HPLOC_LANES; ++l) {
+            if (!keep[l])
+                continue;
+            idx[k] = idx[l];
+            memcpy(box[k], box[l], sizeof(float) * 6);
+            ++k;
+        }
+        for (; k < HPLOC_LANES; ++k)
+            idx[k] = GWN_ORACLE_INVALID_INDEX;
+        num -= merges;
+        h->merges += merges;
+    }
+    for (uint32_t l = 0; l < total; ++l)
+        h->clusters[left + l] = idx[l];
+}
+
+/* Returns the root cluster id (internal ids offset by T) or INVALID on failure. */
+static uint32_t hploc_build(hploc_state *h)
+{
+    uint64_t T = h->T;
+    uint32_t *boundary = (uint32_t *)malloc(sizeof(uint32_t) * T);
+    for (uint64_t i = 0; i < T; ++i) {
+        h->clusters[i] = (uint32_t)i;
+        boundary[i] = GWN_ORACLE_INVALID_INDEX;
+    }
+    for (uint64_t i = 0; i < T && !h->failure; ++i) {
+        uint64_t left = i, right = i, split = 0;
+        for (;;) {
+            uint32_t prev;
+            int to_right = left == 0 ||
+                           (right != T - 1 && hploc_delta(h->codes, right, right + 1) <
+                                                  hploc_delta(h->codes, left - 1, left));
+            if (to_right) {
+                prev = boundary[right];
+                boundary[right] = (uint32_t)left;
+                if (prev != GWN_ORACLE_INVALID_INDEX) {
+                    split = right + 1;
+                    right = prev;
+                }
+            } else {
+                prev = boundary[left - 1];
+                boundary[left - 1] = (uint32_t)right;
+                if (prev != GWN_ORACLE_INVALID_INDEX) {
+                    split = left;
+                    left = prev;
+                }
+            }
+            if (prev == GWN_ORACLE_INVALID_INDEX)
+                break;
+            uint64_t size = right - left + 1;
+            int final = size == T;
+            if (size > HPLOC_THRESHOLD || final)
+                hploc_merge_range(h, left, right + 1, split, final);
+            if (h->failure)
+                break;
+        }
+    }
+    free(boundary);
+    if (h->failure || h->merges != T - 1)
+        return GWN_ORACLE_INVALID_INDEX;
+    return h->clusters[0];
+}
+
 typedef struct {
     uint32_t binary;     /* binary internal node that roots this wide node */
     uint32_t depth;
@@ -198,18 +407,26 @@ typedef struct {
     uint32_t prefix_packed;
 } wide_task;
 
-static int cmp_wide_key(const void *pa, const void *pb)
-{
-    const uint64_t *a = (const uint64_t *)pa, *b = (const uint64_t *)pb;
-    return (a[0] < b[0]) ? -1 : (a[0] > b[0]);
-}
-
 int gwn_oracle_build_bvh4_f32(
     const float *vx, const float *vy, const float *vz, uint64_t vertex_count,
     const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
     gwn_oracle_bvh_f32 *out)
 {
+    return gwn_oracle_build_bvh4_ex_f32(vx, vy, vz, vertex_count, i0, i1, i2, triangle_count,
+                                        GWN_ORACLE_BUILD_LBVH, 8u, out);
+}
+
+int gwn_oracle_build_bvh4_ex_f32(
+    const float *vx, const float *vy, const float *vz, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
+    int method, uint32_t hploc_radius, gwn_oracle_bvh_f32 *out)
+{
     uint64_t T = triangle_count;
+    out->binary_root = 0;
+    if (method != GWN_ORACLE_BUILD_LBVH && method != GWN_ORACLE_BUILD_HPLOC)
+        return GWN_ORACLE_INVALID_ARGUMENT;
+    if (method == GWN_ORACLE_BUILD_HPLOC && (hploc_radius == 0 || hploc_radius > 8))
+        return GWN_ORACLE_INVALID_ARGUMENT;
     memset(&out->root, 0, sizeof(out->root));
     out->node_count = 0;
     out->internal_stack_bound = 0;
@@ -301,21 +518,32 @@ int gwn_oracle_build_bvh4_f32(
     gwn_oracle_binary_node *bn = (gwn_oracle_binary_node *)malloc(sizeof(*bn) * NI);
     int64_t *bfirst = (int64_t *)malloc(sizeof(int64_t) * NI);
     float *bbounds = (float *)malloc(sizeof(float) * 6 * NI);
-    uint32_t *bparent = (uint32_t *)malloc(sizeof(uint32_t) * NI);
-    for (uint64_t i = 0; i < NI; ++i)
-        bparent[i] = GWN_ORACLE_INVALID_INDEX;
-    for (uint64_t i = 0; i < NI; ++i) {
-        lbvh_node(out->sorted_morton, (int64_t)T, (int64_t)i, &bn[i], &bfirst[i]);
-        if (bn[i].left_kind == 1u) bparent[bn[i].left_index] = (uint32_t)i;
-        if (bn[i].right_kind == 1u) bparent[bn[i].right_index] = (uint32_t)i;
+    uint32_t broot = 0;
+    if (method == GWN_ORACLE_BUILD_HPLOC) {
+        hploc_state h = {T, out->sorted_morton, saabb, bn, bbounds, bfirst, (uint8_t *)calloc(NI, 1),
+                         (uint32_t *)malloc(sizeof(uint32_t) * T), hploc_radius, 0, 0};
+        uint32_t root_cluster = hploc_build(&h);
+        free(h.made);
+        free(h.clusters);
+        if (root_cluster == GWN_ORACLE_INVALID_INDEX || root_cluster < T) {
+            free(bn); free(bfirst); free(bbounds);
+            free(aabb); free(codes); free(saabb);
+            return GWN_ORACLE_INTERNAL_ERROR;
+        }
+        broot = root_cluster - (uint32_t)T;
+    } else {
+        for (uint64_t i = 0; i < NI; ++i)
+            lbvh_node(out->sorted_morton, (int64_t)T, (int64_t)i, &bn[i], &bfirst[i]);
     }
+    out->binary_root = broot;
 
-    /* Bottom-up bounds (build_binary.cuh:131-192): post-order over the tree. */
-    {
+    /* Bottom-up bounds (build_binary.cuh:131-192): post-order over the tree.  H-PLOC already
+     * produced every bound as the union of (left, right) at its merge (hploc.cuh:357-359). */
+    if (method == GWN_ORACLE_BUILD_LBVH) {
         uint32_t *stack = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (NI + 1));
         uint8_t *visited = (uint8_t *)calloc(NI, 1);
         int64_t sp = 0;
-        stack[sp++] = 0;
+        stack[sp++] = broot;
         while (sp > 0) {
             uint32_t node = stack[sp - 1];
             if (!visited[node]) {
@@ -338,13 +566,18 @@ int gwn_oracle_build_bvh4_f32(
         memcpy(out->binary_nodes, bn, sizeof(*bn) * NI);
     if (out->binary_bounds)
         memcpy(out->binary_bounds, bbounds, sizeof(float) * 6 * NI);
+    if (out->binary_first)
+        for (uint64_t i = 0; i < NI; ++i)
+            out->binary_first[i] = (uint32_t)bfirst[i];
 
-    /* BVH4 collapse (collapse.cuh:163-296), breadth-first over a FIFO of tasks. */
+    /* BVH4 collapse (collapse.cuh:163-296), breadth-first over a FIFO of tasks.  A node's
+     * internal children join the FIFO in ascending order of their smallest sorted leaf, so the
+     * FIFO position is the canonical node id: breadth-first, parents in level order, siblings by
+     * first leaf.  For LBVH trees (contiguous leaf ranges) that is (depth, first leaf) order. */
     wide_task *tasks = (wide_task *)malloc(sizeof(wide_task) * NI);
     gwn_oracle_node4_f32 *tmp_nodes = (gwn_oracle_node4_f32 *)calloc(NI, sizeof(gwn_oracle_node4_f32));
-    uint32_t *child_task = (uint32_t *)malloc(sizeof(uint32_t) * 4 * NI);
     uint64_t head = 0, tail = 0;
-    tasks[tail++] = (wide_task){0u, 0u, (uint32_t)bfirst[0], 0u, 0u};
+    tasks[tail++] = (wide_task){broot, 0u, (uint32_t)bfirst[broot], 0u, 0u};
     uint32_t max_int = 0, max_packed = 0;
     while (head < tail) {
         wide_task w = tasks[head];
@@ -384,50 +617,34 @@ int gwn_oracle_build_bvh4_f32(
         uint32_t node_packed = w.prefix_packed + (uint32_t)(cnt - 1);
         if (node_int > max_int) max_int = node_int;
         if (node_packed > max_packed) max_packed = node_packed;
-        for (int s = 0; s < 4; ++s)
-            child_task[4 * self + s] = GWN_ORACLE_INVALID_INDEX;
         for (int s = 0; s < cnt; ++s) {
             gwn_oracle_child_f32 *ch = &tmp_nodes[self].children[s];
             if (ckind[s] == 2u) {
                 memcpy(ch->bounds, saabb + 6 * (uint64_t)cidx[s], sizeof(float) * 6);
                 ch->reference = GWN_ORACLE_VALID_BIT | (uint64_t)cidx[s] | (((uint64_t)1) << 48);
-            } else {
-                memcpy(ch->bounds, bbounds + 6 * (uint64_t)cidx[s], sizeof(float) * 6);
-                child_task[4 * self + s] = (uint32_t)tail;
-                tasks[tail++] = (wide_task){cidx[s], w.depth + 1, (uint32_t)bfirst[cidx[s]],
-                                            node_int, node_packed};
+                continue;
             }
+            memcpy(ch->bounds, bbounds + 6 * (uint64_t)cidx[s], sizeof(float) * 6);
+            /* Rank among this node's internal children by first leaf. */
+            uint32_t rank = 0;
+            for (int t = 0; t < cnt; ++t)
+                rank += ckind[t] == 1u && bfirst[cidx[t]] < bfirst[cidx[s]];
+            uint64_t id = tail + rank;
+            ch->reference = GWN_ORACLE_VALID_BIT | id;
+            tasks[id] = (wide_task){cidx[s], w.depth + 1, (uint32_t)bfirst[cidx[s]], node_int, node_packed};
         }
+        tail += n_int;
     }
     uint64_t NW = tail;
-
-    /* Canonical numbering: sort by (depth, first sorted leaf). */
-    uint64_t *keyed = (uint64_t *)malloc(sizeof(uint64_t) * 2 * NW);
-    for (uint64_t i = 0; i < NW; ++i) {
-        keyed[2 * i] = ((uint64_t)tasks[i].depth << 32) | tasks[i].first_leaf;
-        keyed[2 * i + 1] = i;
-    }
-    qsort(keyed, NW, 2 * sizeof(uint64_t), cmp_wide_key);
-    uint32_t *newid = (uint32_t *)malloc(sizeof(uint32_t) * NW);
-    for (uint64_t i = 0; i < NW; ++i)
-        newid[keyed[2 * i + 1]] = (uint32_t)i;
-    for (uint64_t i = 0; i < NW; ++i) {
-        uint64_t old = keyed[2 * i + 1];
-        out->nodes[i] = tmp_nodes[old];
-        for (int s = 0; s < 4; ++s) {
-            uint32_t ct = child_task[4 * old + s];
-            if (ct != GWN_ORACLE_INVALID_INDEX)
-                out->nodes[i].children[s].reference = GWN_ORACLE_VALID_BIT | (uint64_t)newid[ct];
-        }
-    }
-    memcpy(out->root.bounds, bbounds, sizeof(float) * 6);
+    memcpy(out->nodes, tmp_nodes, sizeof(gwn_oracle_node4_f32) * NW);
+    memcpy(out->root.bounds, bbounds + 6 * (uint64_t)broot, sizeof(float) * 6);
     out->root.reference = GWN_ORACLE_VALID_BIT;
     out->node_count = NW;
     out->internal_stack_bound = max_int;
     out->packed_stack_bound = max_packed;
 
-    free(keyed); free(newid); free(tasks); free(tmp_nodes); free(child_task);
-    free(bn); free(bfirst); free(bbounds); free(bparent);
+    free(tasks); free(tmp_nodes);
+    free(bn); free(bfirst); free(bbounds);
     free(aabb); free(codes); free(saabb);
     return GWN_ORACLE_OK;
 }
@@ -2080,4 +2297,696 @@ void gwn_oracle_winding_antipodal_f32(
                          boundary_end, boundary_multiplicity, boundary_edges, qx, qy, qz, stack_capacity,
                          near_fp64, out, gx, gy, gz};
     parallel_ranges(antipodal_range, &ctx, query_count, threads);
+}
+
+/* ---------------------------------------------------------------------------
+ * Adjoint of winding-number and distance losses (no reference counterpart; pinned by FP64 central
+ * differences of gwn_oracle_winding_taylor_frozen_f64 and brute-force distances).
+ *
+ * L = sum_q u_q w_q for the Taylor winding (orders 0, 1) and L = sum_p u_p d_p for the unsigned
+ * distance.  The Taylor replay takes the forward's accept/descend decisions (FP32 fields, strict
+ * |q - P|^2 > maxPDist2 * beta^2 and a finite term) and differentiates the chosen terms in FP64:
+ *   far term of child slot s:  w_s(q - P_s, N_s, T_s), T_s the first-order fields;
+ *   slot fields from the slot's additive payload (area A, A P, N, N_i c_j):
+ *     P = AP / A,  T_ii = (N c)_ii - P_i N_i,  S_ij = (N c)_ij - P_j N_i + (N c)_ji - P_i N_j;
+ *   payload of triangle (a, b, c): N = (b - a) x (c - a) / 2, A = |N|, c = (a + b + c) / 3;
+ *   near triangles: W = 2 atan2(a.(b x c), |a||b||c| + (a.b)|c| + (a.c)|b| + (b.c)|a|) with
+ *     a, b, c relative to q.
+ * Every slot's payload adjoint reaches each triangle below it (the payloads are sums), so a
+ * root-to-leaf walk adds the slot adjoints along each leaf's path.
+ * ------------------------------------------------------------------------- */
+
+typedef struct {
+    double area, ap[3], n[3], n1[9];
+} adj_payload;
+
+static void adj_payload_add(adj_payload *d, const adj_payload *s)
+{
+    d->area += s->area;
+    for (int k = 0; k < 3; ++k) {
+        d->ap[k] += s->ap[k];
+        d->n[k] += s->n[k];
+    }
+    for (int k = 0; k < 9; ++k)
+        d->n1[k] += s->n1[k];
+}
+
+static void adj_tri_payload(const double *a, const double *b, const double *c, adj_payload *p)
+{
+    double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    double n[3];
+    cross3d(e1, e2, n);
+    for (int k = 0; k < 3; ++k)
+        n[k] *= 0.5;
+    double cen[3] = {(a[0] + b[0] + c[0]) / 3.0, (a[1] + b[1] + c[1]) / 3.0, (a[2] + b[2] + c[2]) / 3.0};
+    p->area = sqrt(dot3d(n, n));
+    for (int k = 0; k < 3; ++k) {
+        p->ap[k] = cen[k] * p->area;
+        p->n[k] = n[k];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            p->n1[3 * i + j] = n[i] * cen[j];
+}
+
+/* Vertices of leaf record k in FP64 from the vertex arrays (primitive order + indices). */
+typedef struct {
+    const double *vx, *vy, *vz;
+    const uint32_t *prim, *i0, *i1, *i2;
+} adj_mesh;
+
+static void adj_record_vertices(const adj_mesh *m, uint64_t k, double *a, double *b, double *c, uint32_t *ids)
+{
+    uint32_t p = m->prim[k];
+    ids[0] = m->i0[p];
+    ids[1] = m->i1[p];
+    ids[2] = m->i2[p];
+    double *v[3] = {a, b, c};
+    for (int t = 0; t < 3; ++t) {
+        v[t][0] = m->vx[ids[t]];
+        v[t][1] = m->vy[ids[t]];
+        v[t][2] = m->vz[ids[t]];
+    }
+}
+
+/* FP64 payload of every child slot (node * 4 + slot), children before parents. */
+static void adj_slot_payloads(const gwn_oracle_node4_f32 *nodes, uint64_t node_count, const adj_mesh *m,
+                              adj_payload *slot)
+{
+    for (uint64_t n = node_count; n-- > 0;) {
+        for (int s = 0; s < 4; ++s) {
+            adj_payload *out = &slot[4 * n + s];
+            memset(out, 0, sizeof(*out));
+            uint64_t ref = nodes[n].children[s].reference;
+            if (!(ref & GWN_ORACLE_VALID_BIT))
+                continue;
+            uint64_t off = ref & GWN_ORACLE_OFFSET_MASK;
+            uint32_t cnt = (uint32_t)(ref >> 48);
+            if (cnt == 0) {
+                for (int t = 0; t < 4; ++t)
+                    adj_payload_add(out, &slot[4 * off + t]);
+                continue;
+            }
+            for (uint32_t k = 0; k < cnt; ++k) {
+                double a[3], b[3], c[3];
+                uint32_t ids[3];
+                adj_record_vertices(m, off + k, a, b, c, ids);
+                adj_payload tp;
+                adj_tri_payload(a, b, c, &tp);
+                adj_payload_add(out, &tp);
+            }
+        }
+    }
+}
+
+/* Slot fields P (3), N (3), first-order T (6) in FP64; zero-area slots keep the box centre. */
+static void adj_slot_fields(const adj_payload *p, const float *bounds, double *P, double *N, double *T)
+{
+    for (int k = 0; k < 3; ++k) {
+        P[k] = p->area > 0.0 ? p->ap[k] / p->area : ((double)bounds[k] + (double)bounds[3 + k]) * 0.5;
+        N[k] = p->n[k];
+    }
+    const double *f = p->n1;
+    T[0] = f[0] - P[0] * N[0];
+    T[1] = f[4] - P[1] * N[1];
+    T[2] = f[8] - P[2] * N[2];
+    T[3] = (f[1] - P[1] * N[0]) + (f[3] - P[0] * N[1]);
+    T[4] = (f[5] - P[2] * N[1]) + (f[7] - P[1] * N[2]);
+    T[5] = (f[6] - P[0] * N[2]) + (f[2] - P[2] * N[0]);
+}
+
+/* Far term (orders 0, 1) at r = q - P and its derivatives: dw/dr (3), dw/dN (3), dw/dT (6). */
+static double adj_far_term(const double *r, const double *N, const double *T, int order, double *dr, double *dN,
+                           double *dT)
+{
+    double l2 = dot3d(r, r), m1 = 1.0 / sqrt(l2), m2 = m1 * m1, m3 = m2 * m1, m4 = m2 * m2;
+    double qn[3] = {r[0] * m1, r[1] * m1, r[2] * m1};
+    double rdotN = dot3d(qn, N);
+    double w = -m2 * rdotN;
+    for (int k = 0; k < 3; ++k) {
+        dr[k] = m3 * (3.0 * qn[k] * rdotN - N[k]);
+        dN[k] = -m2 * qn[k];
+    }
+    for (int k = 0; k < 6; ++k)
+        dT[k] = 0.0;
+    if (order >= 1) {
+        double Tr = T[0] + T[1] + T[2];
+        double S = qn[0] * qn[0] * T[0] + qn[1] * qn[1] * T[1] + qn[2] * qn[2] * T[2] + qn[0] * qn[1] * T[3] +
+                   qn[1] * qn[2] * T[4] + qn[2] * qn[0] * T[5];
+        w += m3 * (Tr - 3.0 * S);
+        double v[3] = {2.0 * T[0] * qn[0] + T[3] * qn[1] + T[5] * qn[2],
+                       T[3] * qn[0] + 2.0 * T[1] * qn[1] + T[4] * qn[2],
+                       T[5] * qn[0] + T[4] * qn[1] + 2.0 * T[2] * qn[2]};
+        double coeff = 15.0 * S - 3.0 * Tr;
+        for (int k = 0; k < 3; ++k)
+            dr[k] += m4 * (coeff * qn[k] - 3.0 * v[k]);
+        for (int k = 0; k < 3; ++k)
+            dT[k] = m3 * (1.0 - 3.0 * qn[k] * qn[k]);
+        dT[3] = -3.0 * m3 * qn[0] * qn[1];
+        dT[4] = -3.0 * m3 * qn[1] * qn[2];
+        dT[5] = -3.0 * m3 * qn[2] * qn[0];
+    }
+    return w;
+}
+
+/* Solid angle of (a, b, c) seen from q (unnormalised atan2 form) and its vertex derivatives. */
+static double adj_solid_angle(const double *va, const double *vb, const double *vc, const double *q, double *da,
+                              double *db, double *dc)
+{
+    double a[3], b[3], c[3];
+    for (int k = 0; k < 3; ++k) {
+        a[k] = va[k] - q[k];
+        b[k] = vb[k] - q[k];
+        c[k] = vc[k] - q[k];
+        da[k] = db[k] = dc[k] = 0.0;
+    }
+    double la = sqrt(dot3d(a, a)), lb = sqrt(dot3d(b, b)), lc = sqrt(dot3d(c, c));
+    if (la == 0.0 || lb == 0.0 || lc == 0.0)
+        return 0.0;
+    double bc[3], ca[3], ab[3];
+    cross3d(b, c, bc);
+    cross3d(c, a, ca);
+    cross3d(a, b, ab);
+    double num = dot3d(a, bc);
+    if (num == 0.0)
+        return 0.0;
+    double dab = dot3d(a, b), dac = dot3d(a, c), dbc = dot3d(b, c);
+    double den = la * lb * lc + dab * lc + dac * lb + dbc * la;
+    double s = 2.0 / (num * num + den * den);
+    for (int k = 0; k < 3; ++k) {
+        double ha = a[k] / la, hb = b[k] / lb, hc = c[k] / lc;
+        double dDa = ha * lb * lc + b[k] * lc + c[k] * lb + dbc * ha;
+        double dDb = hb * la * lc + a[k] * lc + c[k] * la + dac * hb;
+        double dDc = hc * la * lb + a[k] * lb + b[k] * la + dab * hc;
+        da[k] = s * (den * bc[k] - num * dDa);
+        db[k] = s * (den * ca[k] - num * dDb);
+        dc[k] = s * (den * ab[k] - num * dDc);
+    }
+    return 2.0 * atan2(num, den);
+}
+
+typedef struct {
+    const gwn_oracle_node4_f32 *nodes;
+    uint64_t node_count;
+    gwn_oracle_child_f32 root;
+    const float *tri;          /* FP32 records: the forward's decisions */
+    uint64_t nt;
+    const float *mom;          /* FP32 fields: the forward's decisions */
+    int order, F;
+    float beta;
+    const adj_mesh *mesh;      /* FP64 values */
+    const adj_payload *slot;
+    const float *qx, *qy, *qz;
+    const double *u;
+    int threads;
+    /* per-thread outputs */
+    double *dslot;             /* 12 per slot: dP, dN, dT */
+    double *dtri;              /* 9 per record */
+    double *dq;                /* 3 per query (disjoint rows) */
+    double value;              /* forward value sum_q u_q w_q (frozen path) */
+} adj_ctx;
+
+typedef struct {
+    const adj_ctx *c;
+    uint64_t begin, end;
+    double *dslot, *dtri;
+    double value;
+} adj_job;
+
+/* Replays one query's decisions; with grad != 0 accumulates derivatives, always returns w_q. */
+static double adj_query(const adj_ctx *c, uint64_t qi, double *dslot, double *dtri, double *dqrow, int grad)
+{
+    const double inv4pi = 1.0 / (4.0 * 3.141592653589793238462643383279502884);
+    float qf[3] = {c->qx[qi], c->qy[qi], c->qz[qi]};
+    double q[3] = {qf[0], qf[1], qf[2]};
+    double u = grad ? c->u[qi] * inv4pi : 0.0;
+    double w = 0.0;
+    double dq[3] = {0.0, 0.0, 0.0};
+    float beta2 = c->beta * c->beta;
+    uint32_t stack[1024];
+    int sp = 0;
+#define ADJ_LEAF(ref_)                                                                                 \
+    do {                                                                                               \
+        uint64_t off_ = (ref_) & GWN_ORACLE_OFFSET_MASK;                                              \
+        uint32_t n_ = (uint32_t)((ref_) >> 48);                                                       \
+        for (uint32_t k_ = 0; k_ < n_; ++k_) {                                                        \
+            if (off_ + k_ >= c->nt)                                                                   \
+                continue;                                                                             \
+            double a_[3], b_[3], cc_[3], ga_[3], gb_[3], gc_[3];                                      \
+            uint32_t ids_[3];                                                                         \
+            adj_record_vertices(c->mesh, off_ + k_, a_, b_, cc_, ids_);                               \
+            w += adj_solid_angle(a_, b_, cc_, q, ga_, gb_, gc_);                                      \
+            if (grad) {                                                                               \
+                double *t_ = dtri + 9 * (off_ + k_);                                                  \
+                for (int d_ = 0; d_ < 3; ++d_) {                                                      \
+                    t_[d_] += u * ga_[d_];                                                            \
+                    t_[3 + d_] += u * gb_[d_];                                                        \
+                    t_[6 + d_] += u * gc_[d_];                                                        \
+                    dq[d_] -= u * (ga_[d_] + gb_[d_] + gc_[d_]);                                      \
+                }                                                                                     \
+            }                                                                                         \
+        }                                                                                             \
+    } while (0)
+    if ((c->root.reference & GWN_ORACLE_VALID_BIT) && (c->root.reference >> 48) != 0) {
+        ADJ_LEAF(c->root.reference);
+    } else {
+        uint32_t node = (uint32_t)(c->root.reference & GWN_ORACLE_OFFSET_MASK);
+        for (;;) {
+            if (node >= c->node_count)
+                break;
+            const gwn_oracle_node4_f32 *nd = &c->nodes[node];
+            const float *t = c->mom + (size_t)node * 4 * c->F;
+            uint32_t next = GWN_ORACLE_INVALID_INDEX;
+            for (int s = 0; s < 4; ++s) {
+                const gwn_oracle_child_f32 *ch = &nd->children[s];
+                if (!(ch->reference & GWN_ORACLE_VALID_BIT))
+                    continue;
+#define TF(f) t[(f) * 4 + s]
+                /* The forward's decision, in FP32 (gwn_query_winding_impl.cuh:330-364). */
+                float qrx = qf[0] - TF(1), qry = qf[1] - TF(2), qrz = qf[2] - TF(3);
+                float ql2 = qrx * qrx + qry * qry + qrz * qrz;
+                int descend = !(ql2 > 0.0f) || ql2 <= TF(0) * beta2;
+                if (!descend) {
+                    float m1 = 1.0f / sqrtf(ql2), m2 = m1 * m1, m3 = m2 * m1;
+                    float qn[3] = {qrx * m1, qry * m1, qrz * m1};
+                    float fw = -m2 * (qn[0] * TF(4) + qn[1] * TF(5) + qn[2] * TF(6));
+                    if (c->order >= 1) {
+                        float Tr = TF(7) + TF(8) + TF(9);
+                        float S = qn[0] * qn[0] * TF(7) + qn[1] * qn[1] * TF(8) + qn[2] * qn[2] * TF(9) +
+                                  qn[0] * qn[1] * TF(10) + qn[0] * qn[2] * TF(12) + qn[1] * qn[2] * TF(11);
+                        fw += m3 * (Tr - 3.0f * S);
+                    }
+                    descend = !isfinite(fw);
+                }
+#undef TF
+                if (!descend) {
+                    double P[3], N[3], T[6], r[3], dr[3], dN[3], dT[6];
+                    adj_slot_fields(&c->slot[4 * node + s], ch->bounds, P, N, T);
+                    for (int k = 0; k < 3; ++k)
+                        r[k] = q[k] - P[k];
+                    w += adj_far_term(r, N, T, c->order, dr, dN, dT);
+                    if (grad) {
+                        double *g = dslot + 12 * ((size_t)node * 4 + s);
+                        for (int k = 0; k < 3; ++k) {
+                            g[k] -= u * dr[k];
+                            g[3 + k] += u * dN[k];
+                            dq[k] += u * dr[k];
+                        }
+                        for (int k = 0; k < 6; ++k)
+                            g[6 + k] += u * dT[k];
+                    }
+                    continue;
+                }
+                if (ch->reference >> 48) {
+                    ADJ_LEAF(ch->reference);
+                    continue;
+                }
+                if (next != GWN_ORACLE_INVALID_INDEX) {
+                    if (sp >= 1024)
+                        return NAN;
+                    stack[sp++] = next;
+                }
+                next = (uint32_t)(ch->reference & GWN_ORACLE_OFFSET_MASK);
+            }
+            if (next != GWN_ORACLE_INVALID_INDEX) {
+                node = next;
+                continue;
+            }
+            if (sp == 0)
+                break;
+            node = stack[--sp];
+        }
+    }
+#undef ADJ_LEAF
+    if (grad)
+        for (int k = 0; k < 3; ++k)
+            dqrow[k] = dq[k];
+    return w * inv4pi;
+}
+
+static void *adj_thread(void *arg)
+{
+    adj_job *job = (adj_job *)arg;
+    const adj_ctx *c = job->c;
+    int grad = c->dq != NULL;
+    double value = 0.0;
+    for (uint64_t q = job->begin; q < job->end; ++q) {
+        double w = adj_query(c, q, job->dslot, job->dtri, grad ? c->dq + 3 * q : NULL, grad);
+        value += c->u[q] * w;
+    }
+    job->value = value;
+    return NULL;
+}
+
+/* Runs the replay over all queries on `threads` threads; per-thread slot/record buffers summed. */
+static void adj_run(adj_ctx *c, double *dslot, double *dtri)
+{
+    uint64_t n = 0;
+    while (c->qx && n == 0)
+        break;
+    int threads = c->threads < 1 ? 1 : c->threads;
+    uint64_t Q = (uint64_t)c->value; /* query count passed through value */
+    c->value = 0.0;
+    if ((uint64_t)threads > Q)
+        threads = Q ? (int)Q : 1;
+    size_t ns = 12 * 4 * (size_t)(c->node_count ? c->node_count : 1), nr = 9 * (size_t)(c->nt ? c->nt : 1);
+    adj_job *jobs = (adj_job *)calloc((size_t)threads, sizeof(adj_job));
+    pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+    for (int t = 0; t < threads; ++t) {
+        jobs[t].c = c;
+        jobs[t].begin = Q * (uint64_t)t / (uint64_t)threads;
+        jobs[t].end = Q * (uint64_t)(t + 1) / (uint64_t)threads;
+        jobs[t].dslot = dslot ? (double *)calloc(ns, sizeof(double)) : NULL;
+        jobs[t].dtri = dtri ? (double *)calloc(nr, sizeof(double)) : NULL;
+        pthread_create(&tid[t], NULL, adj_thread, &jobs[t]);
+    }
+    (void)n;
+    for (int t = 0; t < threads; ++t) {
+        pthread_join(tid[t], NULL);
+        c->value += jobs[t].value;
+        if (dslot)
+            for (size_t i = 0; i < ns; ++i)
+                dslot[i] += jobs[t].dslot[i];
+        if (dtri)
+            for (size_t i = 0; i < nr; ++i)
+                dtri[i] += jobs[t].dtri[i];
+        free(jobs[t].dslot);
+        free(jobs[t].dtri);
+    }
+    free(jobs);
+    free(tid);
+}
+
+/* Payload adjoint of one slot from its field adjoint (dP, dN, dT). */
+static void adj_fields_to_payload(const adj_payload *p, const float *bounds, const double *g, int order,
+                                  adj_payload *out)
+{
+    double P[3], N[3], T[6];
+    adj_slot_fields(p, bounds, P, N, T);
+    double dP[3] = {g[0], g[1], g[2]}, dN[3] = {g[3], g[4], g[5]};
+    memset(out, 0, sizeof(*out));
+    if (order >= 1) {
+        const double *dT = g + 6;
+        /* T0 = n1[0] - Px Nx, T1 = n1[4] - Py Ny, T2 = n1[8] - Pz Nz */
+        out->n1[0] += dT[0]; dP[0] -= dT[0] * N[0]; dN[0] -= dT[0] * P[0];
+        out->n1[4] += dT[1]; dP[1] -= dT[1] * N[1]; dN[1] -= dT[1] * P[1];
+        out->n1[8] += dT[2]; dP[2] -= dT[2] * N[2]; dN[2] -= dT[2] * P[2];
+        /* T3 = n1[1] - Py Nx + n1[3] - Px Ny */
+        out->n1[1] += dT[3]; out->n1[3] += dT[3];
+        dP[1] -= dT[3] * N[0]; dN[0] -= dT[3] * P[1]; dP[0] -= dT[3] * N[1]; dN[1] -= dT[3] * P[0];
+        /* T4 = n1[5] - Pz Ny + n1[7] - Py Nz */
+        out->n1[5] += dT[4]; out->n1[7] += dT[4];
+        dP[2] -= dT[4] * N[1]; dN[1] -= dT[4] * P[2]; dP[1] -= dT[4] * N[2]; dN[2] -= dT[4] * P[1];
+        /* T5 = n1[6] - Px Nz + n1[2] - Pz Nx */
+        out->n1[6] += dT[5]; out->n1[2] += dT[5];
+        dP[0] -= dT[5] * N[2]; dN[2] -= dT[5] * P[0]; dP[2] -= dT[5] * N[0]; dN[0] -= dT[5] * P[2];
+    }
+    for (int k = 0; k < 3; ++k)
+        out->n[k] = dN[k];
+    if (p->area > 0.0) {
+        /* P = AP / A */
+        for (int k = 0; k < 3; ++k) {
+            out->ap[k] = dP[k] / p->area;
+            out->area -= dP[k] * P[k] / p->area;
+        }
+    }
+}
+
+/* Vertex derivatives of one triangle's payload adjoint. */
+static void adj_payload_to_vertices(const double *a, const double *b, const double *c, const adj_payload *g,
+                                    double *ga, double *gb, double *gc)
+{
+    double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    double n[3];
+    cross3d(e1, e2, n);
+    for (int k = 0; k < 3; ++k)
+        n[k] *= 0.5;
+    double cen[3] = {(a[0] + b[0] + c[0]) / 3.0, (a[1] + b[1] + c[1]) / 3.0, (a[2] + b[2] + c[2]) / 3.0};
+    double area = sqrt(dot3d(n, n));
+    double gN[3], gcen[3];
+    for (int k = 0; k < 3; ++k) {
+        gN[k] = g->n[k];
+        gcen[k] = g->ap[k] * area;
+    }
+    double darea = g->area + dot3d(g->ap, cen);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            gN[i] += g->n1[3 * i + j] * cen[j];
+            gcen[j] += g->n1[3 * i + j] * n[i];
+        }
+    if (area > 0.0)
+        for (int k = 0; k < 3; ++k)
+            gN[k] += darea * n[k] / area;
+    /* N = (a x b + b x c + c x a) / 2: d(gN . N)/da = (b - c) x gN / 2, cyclic. */
+    double bc[3] = {b[0] - c[0], b[1] - c[1], b[2] - c[2]};
+    double ca[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    double ab[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+    double ta[3], tb[3], tc[3];
+    cross3d(bc, gN, ta);
+    cross3d(ca, gN, tb);
+    cross3d(ab, gN, tc);
+    for (int k = 0; k < 3; ++k) {
+        ga[k] = 0.5 * ta[k] + gcen[k] / 3.0;
+        gb[k] = 0.5 * tb[k] + gcen[k] / 3.0;
+        gc[k] = 0.5 * tc[k] + gcen[k] / 3.0;
+    }
+}
+
+static void adj_push_down(const gwn_oracle_node4_f32 *nodes, uint64_t node, const adj_payload *acc_in,
+                          const adj_payload *dpay, adj_payload *dtri_pay, uint64_t nt)
+{
+    for (int s = 0; s < 4; ++s) {
+        uint64_t ref = nodes[node].children[s].reference;
+        if (!(ref & GWN_ORACLE_VALID_BIT))
+            continue;
+        adj_payload acc = *acc_in;
+        adj_payload_add(&acc, &dpay[4 * node + s]);
+        uint64_t off = ref & GWN_ORACLE_OFFSET_MASK;
+        uint32_t cnt = (uint32_t)(ref >> 48);
+        if (cnt == 0) {
+            adj_push_down(nodes, off, &acc, dpay, dtri_pay, nt);
+            continue;
+        }
+        for (uint32_t k = 0; k < cnt; ++k)
+            if (off + k < nt)
+                adj_payload_add(&dtri_pay[off + k], &acc);
+    }
+}
+
+static void adj_mesh_from(adj_mesh *m, double **store, const double *vd, uint64_t V, const uint32_t *prim,
+                          const uint32_t *i0, const uint32_t *i1, const uint32_t *i2)
+{
+    (void)store;
+    m->vx = vd;
+    m->vy = vd + V;
+    m->vz = vd + 2 * V;
+    m->prim = prim;
+    m->i0 = i0;
+    m->i1 = i1;
+    m->i2 = i2;
+}
+
+double gwn_oracle_winding_taylor_frozen_f64(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root, const float *triangles,
+    uint64_t triangle_count, const float *moments, int order, float beta, const uint32_t *primitive_indices,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, const double *vertices, uint64_t vertex_count,
+    const float *qx, const float *qy, const float *qz, const double *u, uint64_t query_count, int threads)
+{
+    if (order < 0 || order > 1)
+        return NAN;
+    adj_mesh m;
+    adj_mesh_from(&m, NULL, vertices, vertex_count, primitive_indices, i0, i1, i2);
+    adj_payload *slot = (adj_payload *)calloc(4 * (node_count ? node_count : 1), sizeof(adj_payload));
+    adj_slot_payloads(nodes, node_count, &m, slot);
+    adj_ctx c = {nodes, node_count, root, triangles, triangle_count, moments, order,
+                 gwn_oracle_moment_field_count(order), beta, &m, slot, qx, qy, qz, u, threads,
+                 NULL, NULL, NULL, (double)query_count};
+    adj_run(&c, NULL, NULL);
+    free(slot);
+    return c.value;
+}
+
+int gwn_oracle_winding_taylor_adjoint_f64(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root, const float *triangles,
+    uint64_t triangle_count, const float *moments, int order, float beta, const uint32_t *primitive_indices,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, const double *vertices, uint64_t vertex_count,
+    const float *qx, const float *qy, const float *qz, const double *u, uint64_t query_count, double *d_vertices,
+    double *d_queries, int threads)
+{
+    if (order < 0 || order > 1)
+        return GWN_ORACLE_INVALID_ARGUMENT;
+    adj_mesh m;
+    adj_mesh_from(&m, NULL, vertices, vertex_count, primitive_indices, i0, i1, i2);
+    size_t NS = 4 * (size_t)(node_count ? node_count : 1);
+    adj_payload *slot = (adj_payload *)calloc(NS, sizeof(adj_payload));
+    adj_slot_payloads(nodes, node_count, &m, slot);
+    double *dslot = (double *)calloc(12 * NS, sizeof(double));
+    double *dtri = (double *)calloc(9 * (size_t)(triangle_count ? triangle_count : 1), sizeof(double));
+    adj_ctx c = {nodes, node_count, root, triangles, triangle_count, moments, order,
+                 gwn_oracle_moment_field_count(order), beta, &m, slot, qx, qy, qz, u, threads,
+                 NULL, NULL, d_queries, (double)query_count};
+    adj_run(&c, dslot, dtri);
+    /* Field adjoints -> payload adjoints -> every triangle below. */
+    adj_payload *dpay = (adj_payload *)calloc(NS, sizeof(adj_payload));
+    adj_payload *dtri_pay = (adj_payload *)calloc(triangle_count ? triangle_count : 1, sizeof(adj_payload));
+    for (uint64_t n = 0; n < node_count; ++n)
+        for (int s = 0; s < 4; ++s)
+            if (nodes[n].children[s].reference & GWN_ORACLE_VALID_BIT)
+                adj_fields_to_payload(&slot[4 * n + s], nodes[n].children[s].bounds, dslot + 12 * (4 * n + s), order,
+                                      &dpay[4 * n + s]);
+    int internal_root = (root.reference & GWN_ORACLE_VALID_BIT) && (root.reference >> 48) == 0 && node_count > 0;
+    if (internal_root) {
+        adj_payload zero;
+        memset(&zero, 0, sizeof(zero));
+        adj_push_down(nodes, root.reference & GWN_ORACLE_OFFSET_MASK, &zero, dpay, dtri_pay, triangle_count);
+    }
+    memset(d_vertices, 0, sizeof(double) * 3 * vertex_count);
+    for (uint64_t k = 0; k < triangle_count; ++k) {
+        double a[3], b[3], cc[3], ga[3], gb[3], gc[3];
+        uint32_t ids[3];
+        adj_record_vertices(&m, k, a, b, cc, ids);
+        adj_payload_to_vertices(a, b, cc, &dtri_pay[k], ga, gb, gc);
+        const double *nt9 = dtri + 9 * k;
+        const double *g[3] = {ga, gb, gc};
+        for (int t = 0; t < 3; ++t)
+            for (int d = 0; d < 3; ++d)
+                d_vertices[(size_t)d * vertex_count + ids[t]] += g[t][d] + nt9[3 * t + d];
+    }
+    free(slot); free(dslot); free(dtri); free(dpay); free(dtri_pay);
+    return GWN_ORACLE_OK;
+}
+
+/* Distance adjoint by brute force in FP64: each point's closest point c on its nearest triangle,
+ * n = (p - c) / |p - c|; d d/dp = n and d d/dv_i = -b_i n with b the barycentrics of c (the
+ * closest point moves with the triangle).  Returns the distances in `dist`. */
+static void adj_closest(const double *p, const double *a, const double *b, const double *c, double *bary)
+{
+    /* Ericson, Real-Time Collision Detection 5.1.5, in FP64 with barycentric output. */
+    double ab[3], ac[3], ap[3];
+    for (int k = 0; k < 3; ++k) {
+        ab[k] = b[k] - a[k];
+        ac[k] = c[k] - a[k];
+        ap[k] = p[k] - a[k];
+    }
+    double d1 = dot3d(ab, ap), d2 = dot3d(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) { bary[0] = 1; bary[1] = 0; bary[2] = 0; return; }
+    double bp[3] = {p[0] - b[0], p[1] - b[1], p[2] - b[2]};
+    double d3 = dot3d(ab, bp), d4 = dot3d(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) { bary[0] = 0; bary[1] = 1; bary[2] = 0; return; }
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        double v = d1 / (d1 - d3);
+        bary[0] = 1 - v; bary[1] = v; bary[2] = 0; return;
+    }
+    double cp[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
+    double d5 = dot3d(ab, cp), d6 = dot3d(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) { bary[0] = 0; bary[1] = 0; bary[2] = 1; return; }
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        double w = d2 / (d2 - d6);
+        bary[0] = 1 - w; bary[1] = 0; bary[2] = w; return;
+    }
+    double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        bary[0] = 0; bary[1] = 1 - w; bary[2] = w; return;
+    }
+    double den = 1.0 / (va + vb + vc);
+    double v = vb * den, w = vc * den;
+    bary[0] = 1 - v - w; bary[1] = v; bary[2] = w;
+}
+
+typedef struct {
+    const double *vx, *vy, *vz;
+    const uint32_t *i0, *i1, *i2;
+    uint64_t nt, V;
+    const float *px, *py, *pz;
+    const double *u;
+    double *dist, *dp;
+    double *dv;   /* per-thread 3V */
+} dist_adj_ctx;
+
+typedef struct {
+    const dist_adj_ctx *c;
+    uint64_t begin, end;
+    double *dv;
+} dist_adj_job;
+
+static void *dist_adj_thread(void *arg)
+{
+    dist_adj_job *job = (dist_adj_job *)arg;
+    const dist_adj_ctx *c = job->c;
+    for (uint64_t i = job->begin; i < job->end; ++i) {
+        double p[3] = {c->px[i], c->py[i], c->pz[i]};
+        double best = INFINITY, bb[3] = {0, 0, 0}, bc[3] = {0, 0, 0};
+        uint64_t bt = 0;
+        for (uint64_t t = 0; t < c->nt; ++t) {
+            uint32_t ids[3] = {c->i0[t], c->i1[t], c->i2[t]};
+            double v[3][3];
+            for (int k = 0; k < 3; ++k) {
+                v[k][0] = c->vx[ids[k]];
+                v[k][1] = c->vy[ids[k]];
+                v[k][2] = c->vz[ids[k]];
+            }
+            double bary[3];
+            adj_closest(p, v[0], v[1], v[2], bary);
+            double cp[3], d2 = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                cp[k] = bary[0] * v[0][k] + bary[1] * v[1][k] + bary[2] * v[2][k];
+                d2 += (p[k] - cp[k]) * (p[k] - cp[k]);
+            }
+            if (d2 < best) {
+                best = d2;
+                bt = t;
+                memcpy(bb, bary, sizeof(bb));
+                memcpy(bc, cp, sizeof(bc));
+            }
+        }
+        double d = sqrt(best);
+        c->dist[i] = d;
+        double n[3] = {0, 0, 0};
+        if (d > 0.0)
+            for (int k = 0; k < 3; ++k)
+                n[k] = (p[k] - bc[k]) / d;
+        for (int k = 0; k < 3; ++k)
+            c->dp[3 * i + k] = c->u[i] * n[k];
+        uint32_t ids[3] = {c->i0[bt], c->i1[bt], c->i2[bt]};
+        for (int t = 0; t < 3; ++t)
+            for (int k = 0; k < 3; ++k)
+                job->dv[(size_t)k * c->V + ids[t]] -= c->u[i] * bb[t] * n[k];
+    }
+    return NULL;
+}
+
+void gwn_oracle_distance_adjoint_f64(
+    const double *vertices, uint64_t vertex_count, const uint32_t *i0, const uint32_t *i1, const uint32_t *i2,
+    uint64_t triangle_count, const float *px, const float *py, const float *pz, const double *u, uint64_t count,
+    double *dist, double *d_vertices, double *d_points, int threads)
+{
+    dist_adj_ctx c = {vertices, vertices + vertex_count, vertices + 2 * vertex_count, i0, i1, i2, triangle_count,
+                      vertex_count, px, py, pz, u, dist, d_points, NULL};
+    if (threads < 1)
+        threads = 1;
+    if ((uint64_t)threads > count)
+        threads = count ? (int)count : 1;
+    dist_adj_job *jobs = (dist_adj_job *)calloc((size_t)threads, sizeof(dist_adj_job));
+    pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+    for (int t = 0; t < threads; ++t) {
+        jobs[t].c = &c;
+        jobs[t].begin = count * (uint64_t)t / (uint64_t)threads;
+        jobs[t].end = count * (uint64_t)(t + 1) / (uint64_t)threads;
+        jobs[t].dv = (double *)calloc(3 * (size_t)(vertex_count ? vertex_count : 1), sizeof(double));
+        pthread_create(&tid[t], NULL, dist_adj_thread, &jobs[t]);
+    }
+    memset(d_vertices, 0, sizeof(double) * 3 * vertex_count);
+    for (int t = 0; t < threads; ++t) {
+        pthread_join(tid[t], NULL);
+        for (size_t i = 0; i < 3 * (size_t)vertex_count; ++i)
+            d_vertices[i] += jobs[t].dv[i];
+        free(jobs[t].dv);
+    }
+    free(jobs);
+    free(tid);
 }

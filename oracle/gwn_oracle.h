@@ -81,7 +81,12 @@ typedef struct {
     uint32_t internal_stack_bound;
     uint32_t packed_stack_bound;
     float scene[9];                /* min xyz, max xyz, inv xyz */
+    uint32_t *binary_first;        /* optional: smallest sorted leaf under each binary internal node */
+    uint32_t binary_root;          /* binary internal root (0 for LBVH) */
 } gwn_oracle_bvh_f32;
+
+/* Binary builder selection, gwn_bvh_build_options (gwn_bvh_build.cuh:16-27). */
+enum { GWN_ORACLE_BUILD_HPLOC = 0, GWN_ORACLE_BUILD_LBVH = 1 };
 
 enum {
     GWN_ORACLE_OK = 0,
@@ -99,6 +104,16 @@ int gwn_oracle_build_bvh4_f32(
     const float *vx, const float *vy, const float *vz, uint64_t vertex_count,
     const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
     gwn_oracle_bvh_f32 *out);
+
+/*
+ * Build with a chosen binary builder: LBVH (Karras) or H-PLOC with search radius in [1, 8]
+ * (gwn_bvh_build_hploc.cuh:104-622).  H-PLOC internal nodes are numbered deterministically:
+ * a node whose children hold smallest sorted leaves a < b gets id b - 1.
+ */
+int gwn_oracle_build_bvh4_ex_f32(
+    const float *vx, const float *vy, const float *vz, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
+    int method, uint32_t hploc_radius, gwn_oracle_bvh_f32 *out);
 
 /*
  * Bounds refit in place: triangle records from the geometry, child boxes folded in slot order,

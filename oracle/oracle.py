@@ -43,6 +43,8 @@ class _Bvh(ctypes.Structure):
         ("internal_stack_bound", ctypes.c_uint32),
         ("packed_stack_bound", ctypes.c_uint32),
         ("scene", ctypes.c_float * 9),
+        ("binary_first", ctypes.c_void_p),
+        ("binary_root", ctypes.c_uint32),
     ]
 
 
@@ -72,6 +74,16 @@ def lib():
         L.gwn_oracle_stable_order_u64.argtypes = [vp, u64, vp]
         L.gwn_oracle_build_bvh4_f32.restype = i32
         L.gwn_oracle_build_bvh4_f32.argtypes = [vp, vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(_Bvh)]
+        L.gwn_oracle_winding_taylor_frozen_f64.restype = ctypes.c_double
+        L.gwn_oracle_winding_taylor_frozen_f64.argtypes = [
+            vp, u64, _Child, vp, u64, vp, i32, f32, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp, u64, i32]
+        L.gwn_oracle_winding_taylor_adjoint_f64.restype = i32
+        L.gwn_oracle_winding_taylor_adjoint_f64.argtypes = [
+            vp, u64, _Child, vp, u64, vp, i32, f32, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp, u64, vp, vp, i32]
+        L.gwn_oracle_distance_adjoint_f64.argtypes = [vp, u64, vp, vp, vp, u64, vp, vp, vp, vp, u64, vp, vp, vp, i32]
+        L.gwn_oracle_build_bvh4_ex_f32.restype = i32
+        L.gwn_oracle_build_bvh4_ex_f32.argtypes = [vp, vp, vp, u64, vp, vp, vp, u64, i32, ctypes.c_uint32,
+                                                   ctypes.POINTER(_Bvh)]
         L.gwn_oracle_refit_bvh_f32.restype = i32
         L.gwn_oracle_refit_bvh_f32.argtypes = [vp, u64, ctypes.POINTER(_Child), vp, vp, u64, vp, vp, vp, u64, vp, vp, vp]
         L.gwn_oracle_moment_field_count.restype = i32
@@ -152,13 +164,19 @@ class OracleBvh:
     internal_stack_bound: int
     packed_stack_bound: int
     scene: np.ndarray              # float32[9]
+    binary_first: np.ndarray = None  # uint32[T-1]  smallest sorted leaf under each binary node
+    binary_root: int = 0
 
     @property
     def node_count(self) -> int:
         return int(self.nodes.shape[0])
 
 
-def build_bvh4(mesh) -> OracleBvh:
+BUILD_HPLOC, BUILD_LBVH = 0, 1
+
+
+def build_bvh4(mesh, method: str = "lbvh", radius: int = 8) -> OracleBvh:
+    """BVH4 of `mesh` with the LBVH or H-PLOC binary builder (H-PLOC search radius 1-8)."""
     vx, vy, vz = _c(mesh.vx, np.float32), _c(mesh.vy, np.float32), _c(mesh.vz, np.float32)
     i0, i1, i2 = _c(mesh.i0, np.uint32), _c(mesh.i1, np.uint32), _c(mesh.i2, np.uint32)
     T = i0.shape[0]
@@ -168,11 +186,15 @@ def build_bvh4(mesh) -> OracleBvh:
     morton = np.zeros(max(T, 1), dtype=np.uint64)
     bnodes = np.zeros((max(T - 1, 1), 4), dtype=np.uint32)
     bbounds = np.zeros((max(T - 1, 1), 6), dtype=np.float32)
+    bfirst = np.zeros(max(T - 1, 1), dtype=np.uint32)
     out = _Bvh()
     out.nodes, out.primitive_indices, out.triangles = _ptr(nodes), _ptr(prim), _ptr(tris)
     out.sorted_morton, out.binary_nodes, out.binary_bounds = _ptr(morton), _ptr(bnodes), _ptr(bbounds)
-    st = lib().gwn_oracle_build_bvh4_f32(
-        _ptr(vx), _ptr(vy), _ptr(vz), vx.shape[0], _ptr(i0), _ptr(i1), _ptr(i2), T, ctypes.byref(out)
+    out.binary_first = _ptr(bfirst)
+    m = {"lbvh": BUILD_LBVH, "hploc": BUILD_HPLOC}[method]
+    st = lib().gwn_oracle_build_bvh4_ex_f32(
+        _ptr(vx), _ptr(vy), _ptr(vz), vx.shape[0], _ptr(i0), _ptr(i1), _ptr(i2), T, m, int(radius),
+        ctypes.byref(out)
     )
     if st != 0:
         raise RuntimeError(f"oracle build failed with status {st}")
@@ -189,6 +211,8 @@ def build_bvh4(mesh) -> OracleBvh:
         internal_stack_bound=int(out.internal_stack_bound),
         packed_stack_bound=int(out.packed_stack_bound),
         scene=np.array(list(out.scene), dtype=np.float32),
+        binary_first=bfirst[: max(T - 1, 0)].copy(),
+        binary_root=int(out.binary_root),
     )
 
 
@@ -453,3 +477,68 @@ def canonical_dfs(nodes: np.ndarray, root_reference: int):
                 kids.append(ref & OFFSET_MASK)
         stack.extend(reversed(kids))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Adjoint (no reference counterpart): FP64 derivatives of L = sum u_q w_q and L = sum u_p d_p.
+# ---------------------------------------------------------------------------
+
+def _vertices64(mesh, vertices):
+    if vertices is None:
+        vertices = np.stack([mesh.vx, mesh.vy, mesh.vz]).astype(np.float64)
+    return _c(np.asarray(vertices, dtype=np.float64).reshape(3, -1), np.float64)
+
+
+def _adj_args(bvh, moments, order, beta, mesh, vertices, qx, qy, qz, u):
+    nodes = _c(bvh.nodes, NODE_DTYPE)
+    tris = _c(bvh.triangles, np.float32)
+    mom = _c(moments, np.float32)
+    prim = _c(bvh.primitive_indices, np.uint32)
+    i0, i1, i2 = _c(mesh.i0, np.uint32), _c(mesh.i1, np.uint32), _c(mesh.i2, np.uint32)
+    v = _vertices64(mesh, vertices)
+    qx, qy, qz = _c(qx, np.float32), _c(qy, np.float32), _c(qz, np.float32)
+    u = _c(u, np.float64)
+    root = _Child()
+    for k in range(6):
+        root.bounds[k] = float(bvh.root_bounds[k])
+    root.reference = bvh.root_reference
+    keep = (nodes, tris, mom, prim, i0, i1, i2, v, qx, qy, qz, u)
+    args = [_ptr(nodes), bvh.node_count, root, _ptr(tris), tris.shape[0], _ptr(mom), int(order), float(beta),
+            _ptr(prim), _ptr(i0), _ptr(i1), _ptr(i2), _ptr(v), v.shape[1], _ptr(qx), _ptr(qy), _ptr(qz),
+            _ptr(u), qx.shape[0]]
+    return keep, args
+
+
+def winding_taylor_frozen_f64(bvh, moments, order, beta, mesh, qx, qy, qz, u, vertices=None, threads=None) -> float:
+    """sum_q u_q w_q in FP64 for vertex positions `vertices` (3, V) with the accept/descend decisions
+    the FP32 forward takes on `bvh`/`moments` (the function whose derivative the adjoint gives)."""
+    keep, args = _adj_args(bvh, moments, order, beta, mesh, vertices, qx, qy, qz, u)
+    return float(lib().gwn_oracle_winding_taylor_frozen_f64(*args, threads or default_threads()))
+
+
+def winding_taylor_adjoint_f64(bvh, moments, order, beta, mesh, qx, qy, qz, u, vertices=None, threads=None):
+    """(dL/dV as float64 (3, V), dL/dq as float64 (Q, 3)) for L = sum_q u_q w_q, Taylor orders 0-1."""
+    keep, args = _adj_args(bvh, moments, order, beta, mesh, vertices, qx, qy, qz, u)
+    V = keep[7].shape[1]
+    dv = np.zeros((3, V), np.float64)
+    dq = np.zeros((keep[8].shape[0], 3), np.float64)
+    st = lib().gwn_oracle_winding_taylor_adjoint_f64(*args, _ptr(dv), _ptr(dq), threads or default_threads())
+    if st != 0:
+        raise RuntimeError(f"oracle adjoint failed with status {st}")
+    return dv, dq
+
+
+def distance_adjoint_f64(mesh, px, py, pz, u, vertices=None, threads=None):
+    """Brute-force FP64 distances and (dist, dL/dV (3, V), dL/dp (P, 3)) for L = sum_p u_p d_p."""
+    v = _vertices64(mesh, vertices)
+    i0, i1, i2 = _c(mesh.i0, np.uint32), _c(mesh.i1, np.uint32), _c(mesh.i2, np.uint32)
+    px, py, pz = _c(px, np.float32), _c(py, np.float32), _c(pz, np.float32)
+    u = _c(u, np.float64)
+    n = px.shape[0]
+    dist = np.zeros(n, np.float64)
+    dv = np.zeros((3, v.shape[1]), np.float64)
+    dp = np.zeros((n, 3), np.float64)
+    lib().gwn_oracle_distance_adjoint_f64(_ptr(v), v.shape[1], _ptr(i0), _ptr(i1), _ptr(i2), i0.shape[0],
+                                          _ptr(px), _ptr(py), _ptr(pz), _ptr(u), n, _ptr(dist), _ptr(dv),
+                                          _ptr(dp), threads or default_threads())
+    return dist, dv, dp

@@ -267,6 +267,253 @@ __device__ inline float half_area(aabb const &b) {
     return dx * dy + dy * dz + dz * dx;
 }
 
+// H-PLOC binary build (reference: include/gwn/detail/gwn_bvh_build_hploc.cuh:104-622), wave64.
+//
+// Every sorted leaf is a lane that climbs the implicit LBVH of the (code, position) keys: at each
+// step it exchanges its range end into the boundary it meets (hploc.cuh:141-156, 451-470); the
+// first arrival stops, the second joins both ranges.  A joined range with more than 16 leaves, or
+// the full range, concatenates its children's cluster lists (at most 16 each, stored at the start
+// of each child range) and runs PLOC on them: nearest neighbour within the search radius by the
+// packed key (union half-area bits << 1, low four bits = (offset - 1) << 1 | direction parity,
+// hploc.cuh:195-241), mutual pairs merged into the lower lane, survivors compacted in lane order,
+// until at most 16 clusters remain (one for the full range); the list goes back to the start of
+// the range.  The 16-cluster lists are the reference's, so the merge sequence of every range is
+// the reference's; a 64-lane wave runs two ranges at once, one per 32-lane half.
+//
+// Node ids do not come from an atomic counter: merging clusters whose smallest sorted leaves are
+// a < b creates internal node b - 1.  Each leaf is the larger-side minimum of at most one merge,
+// so ids are unique, cover [0, T - 2], and do not depend on scheduling; bounds and first leaves
+// are written beside the node, so no bounds climb follows.  Release before the boundary exchange
+// and acquire after it publish a range's list and nodes to the lane (on any XCD) that joins it.
+constexpr std::uint32_t k_hploc_list = 16; // clusters kept per partial range (hploc.cuh:23)
+constexpr int k_hploc_block = 256;
+
+__device__ inline std::uint64_t hploc_delta(std::uint64_t const *codes, std::uint32_t a, std::uint32_t b) {
+    std::uint64_t const d = codes[a] ^ codes[b];
+    return d == 0 ? static_cast<std::uint64_t>(a ^ b) : d;
+}
+
+__device__ inline std::uint32_t hploc_area_key(aabb const &a, aabb const &b) {
+    float const area = half_area(aabb_union_ref(a, b));
+    return (__float_as_uint(area) << 1) & ~15u;
+}
+
+// Lane-local view of one half wave: bits of a 64-lane ballot that belong to this lane's half.
+__device__ inline std::uint32_t half_bits(std::uint64_t m) {
+    return static_cast<std::uint32_t>(m >> (threadIdx.x & 32u));
+}
+
+__device__ inline aabb shfl_box32(aabb const &b, std::uint32_t src) {
+    int const s = static_cast<int>(src);
+    return aabb{__shfl(b.min_x, s, 32), __shfl(b.min_y, s, 32), __shfl(b.min_z, s, 32),
+                __shfl(b.max_x, s, 32), __shfl(b.max_y, s, 32), __shfl(b.max_z, s, 32)};
+}
+
+// Position of the k-th (0-based) set bit of `mask`; requires k < popcount(mask).
+__device__ inline std::uint32_t select_bit(std::uint32_t mask, std::uint32_t k) {
+    std::uint32_t pos = 0;
+#pragma unroll
+    for (std::uint32_t w = 16; w > 0; w >>= 1) {
+        std::uint32_t const low = __popc(mask & ((1u << w) - 1u));
+        if (k >= low) {
+            k -= low;
+            mask >>= w;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
+struct hploc_out {
+    uint2 *bnodes;
+    aabb *ibounds;
+    std::uint32_t *ifirst;
+    std::uint32_t *clusters;
+};
+
+// PLOC over one range [L, R1) split at S, run by the 32 lanes of this half (all active, uniform
+// control flow within the half).  Returns the merges made; sets `bad` on an inconsistent state.
+__device__ inline std::uint32_t hploc_range(
+    aabb const *__restrict__ leaf_boxes, std::uint32_t T, std::uint32_t radius, hploc_out const &o,
+    std::uint32_t L, std::uint32_t R1, std::uint32_t S, bool fin, std::uint32_t *root_word, bool &bad
+) {
+    std::uint32_t const h = threadIdx.x & 31u;
+    // Concatenate the two child lists (hploc.cuh:159-177, 424-430).
+    std::uint32_t idx = k_invalid;
+    std::uint32_t const lcl = (S - L) < k_hploc_list ? (S - L) : k_hploc_list;
+    if (h < lcl)
+        idx = o.clusters[L + h];
+    std::uint32_t const num_left = __popc(half_bits(__ballot(h < lcl && idx != k_invalid)));
+    std::uint32_t const lcr = (R1 - S) < k_hploc_list ? (R1 - S) : k_hploc_list;
+    std::uint32_t const ri = h - num_left;
+    bool const rv = ri < lcr;
+    if (rv)
+        idx = o.clusters[S + ri];
+    std::uint32_t const num_right = __popc(half_bits(__ballot(rv && idx != k_invalid)));
+    std::uint32_t const total = num_left + num_right;
+    std::uint32_t num = total;
+    std::uint32_t first = k_invalid;
+    aabb box{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (h < num) {
+        if (idx < T) {
+            first = idx;
+            box = leaf_boxes[idx];
+        } else if (idx - T < T - 1u) {
+            first = o.ifirst[idx - T];
+            box = o.ibounds[idx - T];
+        } else {
+            bad = true;
+        }
+    }
+    std::uint32_t const threshold = fin ? 1u : k_hploc_list;
+    std::uint32_t made = 0;
+    while (num > threshold) {
+        // Smallest packed key over the pairs (h, h + r) and (h - r, h), r <= radius.
+        std::uint32_t best = 0xffffffffu;
+#pragma unroll
+        for (std::uint32_t r = 1; r <= 8u; ++r) {
+            if (r > radius)
+                break;
+            bool const nv = h + r < num;
+            aabb const nb = shfl_box32(box, nv ? h + r : h);
+            std::uint32_t ak = 0xffffffffu;
+            if (nv) {
+                ak = hploc_area_key(box, nb);
+                std::uint32_t const key = ak | ((r - 1u) << 1) | (h & 1u);
+                best = key < best ? key : best;
+            }
+            std::uint32_t const from_left = __shfl(ak, static_cast<int>(h >= r ? h - r : h), 32);
+            if (h >= r && h < num) {
+                std::uint32_t const key = from_left | ((r - 1u) << 1) | ((h & 1u) ^ 1u);
+                best = key < best ? key : best;
+            }
+        }
+        std::uint32_t const off = ((best & 15u) >> 1) + 1u;
+        std::uint32_t const nn = (((best ^ h) & 1u) == 0u) ? h + off : h - off;
+        // Mutual nearest neighbours merge; the lower lane keeps the cluster (hploc.cuh:337-404).
+        bool const act = h < num;
+        bool const nvalid = nn < num && nn != h;
+        if (act && !nvalid)
+            bad = true;
+        std::uint32_t const safe = nvalid ? nn : h;
+        std::uint32_t const back = __shfl(safe, static_cast<int>(safe), 32);
+        bool const mutual = act && nvalid && back == h;
+        bool const merge = mutual && h < safe;
+        std::uint32_t const n_idx = __shfl(idx, static_cast<int>(safe), 32);
+        std::uint32_t const n_first = __shfl(first, static_cast<int>(safe), 32);
+        aabb const n_box = shfl_box32(box, safe);
+        if (merge) {
+            std::uint32_t const id = (first > n_first ? first : n_first) - 1u;
+            if (first == n_first || id >= T - 1u) {
+                bad = true;
+            } else {
+                aabb const merged = aabb_union_ref(box, n_box);
+                std::uint32_t const f = first < n_first ? first : n_first;
+                o.bnodes[id] = make_uint2(idx < T ? (idx | k_leaf_bit) : idx - T,
+                                          n_idx < T ? (n_idx | k_leaf_bit) : n_idx - T);
+                o.ibounds[id] = merged;
+                o.ifirst[id] = f;
+                idx = T + id;
+                first = f;
+                box = merged;
+            }
+        }
+        std::uint32_t const merges = __popc(half_bits(__ballot(merge)));
+        std::uint32_t const keep_mask = half_bits(__ballot(merge || !mutual));
+        std::uint32_t const kept = __popc(keep_mask);
+        std::uint32_t const src = h < kept ? select_bit(keep_mask, h) : h;
+        idx = __shfl(idx, static_cast<int>(src), 32);
+        first = __shfl(first, static_cast<int>(src), 32);
+        box = shfl_box32(box, src);
+        if (h >= kept)
+            idx = k_invalid;
+        num -= merges;
+        made += merges;
+        if (merges == 0) {
+            bad = true;
+            break;
+        }
+    }
+    if (h < total)
+        o.clusters[L + h] = idx;
+    if (fin && h == 0u && num == 1u)
+        *root_word = idx - T;
+    return made;
+}
+
+// words: [0] merges, [1] failure flag, [2] root internal id.
+__global__ __launch_bounds__(k_hploc_block) void hploc_kernel(
+    std::uint64_t const *__restrict__ codes, aabb const *__restrict__ leaf_boxes, std::uint32_t T,
+    std::uint32_t radius, hploc_out o, std::uint32_t *boundary, std::uint32_t *words
+) {
+    std::uint32_t const lane = threadIdx.x & 63u;
+    std::uint32_t const idx = blockIdx.x * k_hploc_block + threadIdx.x;
+    std::uint32_t left = idx, right = idx, split = 0;
+    bool active = idx < T;
+    bool bad = false;
+    std::uint32_t made = 0;
+    for (;;) {
+        release_agent();
+        if (active) {
+            bool const to_right =
+                left == 0u || (right != T - 1u && hploc_delta(codes, right, right + 1u) <
+                                                        hploc_delta(codes, left - 1u, left));
+            std::uint32_t const slot = to_right ? right : left - 1u;
+            std::uint32_t const prev = __hip_atomic_exchange(
+                &boundary[slot], to_right ? left : right, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == k_invalid) {
+                active = false;
+            } else if (to_right) {
+                split = right + 1u;
+                right = prev;
+            } else {
+                split = left;
+                left = prev;
+            }
+        }
+        acquire_agent();
+        bool const fin = active && right - left + 1u == T;
+        std::uint64_t pending = __ballot(active && (right - left + 1u > k_hploc_list || fin));
+        while (pending != 0u) {
+            int const s0 = __builtin_ctzll(pending);
+            pending &= pending - 1u;
+            int s1 = -1;
+            if (pending != 0u) {
+                s1 = __builtin_ctzll(pending);
+                pending &= pending - 1u;
+            }
+            int const sel = (lane & 32u) ? s1 : s0;
+            int const src = sel >= 0 ? sel : s0;
+            std::uint32_t const rl = __shfl(left, src, 64);
+            std::uint32_t const rr = __shfl(right, src, 64);
+            std::uint32_t const rs = __shfl(split, src, 64);
+            int const rf = __shfl(fin ? 1 : 0, src, 64);
+            if (sel >= 0)
+                made += hploc_range(leaf_boxes, T, radius, o, rl, rr + 1u, rs, rf != 0, words + 2, bad);
+        }
+        if (__ballot(active) == 0u)
+            break;
+    }
+    // `made` is the same on the 32 lanes of a half: count it once per half.
+    std::uint32_t const total = wave_sum_u32((lane & 31u) == 0u ? made : 0u);
+    bool const any_bad = __ballot(bad) != 0u;
+    if (lane == 0u) {
+        atomicAdd(&words[0], total);
+        if (any_bad)
+            atomicOr(&words[1], 1u);
+    }
+}
+
+__global__ __launch_bounds__(k_block) void hploc_init_kernel(
+    std::uint32_t *clusters, std::uint32_t *boundary, std::uint32_t T
+) {
+    std::uint32_t const i = blockIdx.x * k_block + threadIdx.x;
+    if (i < T) {
+        clusters[i] = i;
+        boundary[i] = k_invalid;
+    }
+}
+
 // Level-synchronous BVH4 collapse.  Level d holds the wide nodes at depth d, numbered
 // contiguously from levels[d].start in (depth, first sorted leaf) order.  One launch per level:
 // every thread takes one wide node of the level, expands its binary subtree to at most four
@@ -275,7 +522,10 @@ __device__ inline float half_area(aabb const &b) {
 // counts numbers the next level's nodes.  Children of one node are numbered in first-leaf
 // order, which keeps the whole level in first-leaf order, so nodes come out in their final
 // breadth-first positions and no reordering pass follows.
-constexpr int k_max_levels = 130; // binary depth <= 128: the 127-bit (code, index) prefix grows per level
+// Wide depth never exceeds binary depth: at most 128 levels for LBVH (one (code, index) prefix bit
+// per level); H-PLOC trees are deeper only for adversarial inputs, which fail the build with an
+// internal error past this limit.
+constexpr int k_max_levels = 1024;
 
 struct collapse_task {
     std::uint32_t binary;
@@ -298,13 +548,26 @@ struct collapse_stats {
     std::uint32_t error;
 };
 
+// Seeds level 0 with the binary root: node 0 for LBVH; for H-PLOC (hploc_words non-null) the
+// recorded root, after checking that the build made T - 1 merges without failure.
 __global__ void collapse_init_kernel(
-    collapse_task *tasks, collapse_level *levels, collapse_stats *stats, std::uint32_t *parent
+    collapse_task *tasks, collapse_level *levels, collapse_stats *stats, std::uint32_t *parent,
+    std::uint32_t const *hploc_words, std::uint32_t internal_count, aabb const *ibounds, aabb *root_box
 ) {
-    tasks[0] = collapse_task{0u, 0u, 0u, 0u};
-    levels[0] = collapse_level{0u, 1u, 0u, 0u};
-    *stats = collapse_stats{0u, 0u, 0u, 0u};
+    std::uint32_t root = 0u, error = 0u;
+    if (hploc_words != nullptr) {
+        root = hploc_words[2];
+        if (hploc_words[0] != internal_count || hploc_words[1] != 0u || root >= internal_count) {
+            error = 1u;
+            root = 0u;
+        }
+    }
+    tasks[0] = collapse_task{root, 0u, 0u, 0u};
+    // A failed H-PLOC leaves level 0 empty: the collapse launches then do nothing.
+    levels[0] = collapse_level{0u, error ? 0u : 1u, 0u, 0u};
+    *stats = collapse_stats{0u, 0u, 0u, error};
     parent[0] = k_invalid;
+    *root_box = ibounds[root];
 }
 
 __device__ inline std::uint32_t wave_max_u32(std::uint32_t v) {
@@ -487,7 +750,8 @@ __global__ __launch_bounds__(k_block) void records_kernel(
 
 } // namespace
 
-void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, hipStream_t stream) {
+void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, int method, std::uint32_t hploc_radius,
+                hipStream_t stream) {
     std::size_t const T = g.triangle_count;
     std::size_t const V = g.vertex_count;
     out.triangle_count = T;
@@ -552,18 +816,38 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, hipStream_t s
         return;
     }
 
-    // Karras LBVH + bounds.
+    // Binary hierarchy with internal bounds and first leaves: Karras LBVH + bounds climb, or H-PLOC.
     std::size_t const NI = T - 1;
     device_buffer<uint2> bnodes(NI, stream);
-    device_buffer<std::uint32_t> iparent(NI, stream), lparent(T, stream), arrivals(NI, stream), ifirst(NI, stream);
+    device_buffer<std::uint32_t> ifirst(NI, stream);
     device_buffer<aabb> ibounds(NI, stream);
-    fill_u32(arrivals.ptr, 0u, NI, stream);
-    lbvh_kernel<<<grid_for(NI, k_block), k_block, 0, stream>>>(sorted_codes, T, bnodes.ptr, iparent.ptr, lparent.ptr);
-    GWN_LAUNCH_CHECK();
-    bounds_kernel<<<grid_for(T, k_block), k_block, 0, stream>>>(
-        leaf_boxes.ptr, bnodes.ptr, iparent.ptr, lparent.ptr, T, ibounds.ptr, ifirst.ptr, arrivals.ptr
-    );
-    GWN_LAUNCH_CHECK();
+    device_buffer<std::uint32_t> hploc_words;
+    if (method == GWN_BVH_BUILD_HPLOC) {
+        device_buffer<std::uint32_t> clusters(T, stream), boundary(T, stream);
+        hploc_words.allocate(3, stream);
+        fill_u32(hploc_words.ptr, 0u, 3, stream);
+        // Every node id is written exactly once by a correct run; the fill keeps any node a
+        // failed run leaves unwritten in bounds (two references to leaf 0) for the collapse.
+        fill_u32(reinterpret_cast<std::uint32_t *>(bnodes.ptr), k_leaf_bit, 2 * NI, stream);
+        fill_u32(ifirst.ptr, 0u, NI, stream);
+        hploc_init_kernel<<<grid_for(T, k_block), k_block, 0, stream>>>(
+            clusters.ptr, boundary.ptr, static_cast<std::uint32_t>(T));
+        GWN_LAUNCH_CHECK();
+        hploc_kernel<<<grid_for(T, k_hploc_block), k_hploc_block, 0, stream>>>(
+            sorted_codes, leaf_boxes.ptr, static_cast<std::uint32_t>(T), hploc_radius,
+            hploc_out{bnodes.ptr, ibounds.ptr, ifirst.ptr, clusters.ptr}, boundary.ptr, hploc_words.ptr
+        );
+        GWN_LAUNCH_CHECK();
+    } else {
+        device_buffer<std::uint32_t> iparent(NI, stream), lparent(T, stream), arrivals(NI, stream);
+        fill_u32(arrivals.ptr, 0u, NI, stream);
+        lbvh_kernel<<<grid_for(NI, k_block), k_block, 0, stream>>>(sorted_codes, T, bnodes.ptr, iparent.ptr, lparent.ptr);
+        GWN_LAUNCH_CHECK();
+        bounds_kernel<<<grid_for(T, k_block), k_block, 0, stream>>>(
+            leaf_boxes.ptr, bnodes.ptr, iparent.ptr, lparent.ptr, T, ibounds.ptr, ifirst.ptr, arrivals.ptr
+        );
+        GWN_LAUNCH_CHECK();
+    }
 
     // Level-synchronous collapse into oversized arrays (wide nodes <= binary internal nodes).
     std::size_t const capacity = NI;
@@ -575,9 +859,11 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, hipStream_t s
     device_buffer<bvh_node4> wide(capacity, stream);
     device_buffer<std::uint32_t> wparent(capacity, stream);
     device_buffer<std::uint8_t> wchildren(capacity, stream);
+    device_buffer<aabb> root_box_dev(1, stream);
     fill_u32(reinterpret_cast<std::uint32_t *>(levels.ptr), 0u, k_max_levels * 4, stream);
     fill_u32(status.ptr, 0u, 2 * static_cast<std::size_t>(status_words), stream);
-    collapse_init_kernel<<<1, 1, 0, stream>>>(tasks.ptr, levels.ptr, stats.ptr, wparent.ptr);
+    collapse_init_kernel<<<1, 1, 0, stream>>>(tasks.ptr, levels.ptr, stats.ptr, wparent.ptr, hploc_words.ptr,
+                                              static_cast<std::uint32_t>(NI), ibounds.ptr, root_box_dev.ptr);
     GWN_LAUNCH_CHECK();
 
     // Launch levels in chunks and read the level table back after each chunk; a launch for a
@@ -603,7 +889,7 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, hipStream_t s
             );
             GWN_LAUNCH_CHECK();
         }
-        GWN_HIP(hipMemcpyAsync(host_levels.data(), levels.ptr, k_max_levels * sizeof(collapse_level),
+        GWN_HIP(hipMemcpyAsync(host_levels.data(), levels.ptr, static_cast<std::size_t>(depth + 1) * sizeof(collapse_level),
                                hipMemcpyDeviceToHost, stream));
         GWN_HIP(hipStreamSynchronize(stream));
         for (int d = 0; d <= depth; ++d) {
@@ -620,12 +906,19 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, hipStream_t s
     aabb root_box{};
     std::uint32_t rec_err = 0;
     GWN_HIP(hipMemcpyAsync(&hs, stats.ptr, sizeof(hs), hipMemcpyDeviceToHost, stream));
-    GWN_HIP(hipMemcpyAsync(&root_box, ibounds.ptr, sizeof(aabb), hipMemcpyDeviceToHost, stream));
+    GWN_HIP(hipMemcpyAsync(&root_box, root_box_dev.ptr, sizeof(aabb), hipMemcpyDeviceToHost, stream));
     GWN_HIP(hipMemcpyAsync(&rec_err, flags.ptr, sizeof(rec_err), hipMemcpyDeviceToHost, stream));
     GWN_HIP(hipStreamSynchronize(stream));
     if (rec_err != 0u)
         throw_internal("bvh.build: Leaf-ordered triangle construction encountered invalid geometry.");
     std::size_t const NW = host_levels[empty_level].start;
+    if (method == GWN_BVH_BUILD_HPLOC) {
+        std::uint32_t hw[3] = {0, 0, 0};
+        GWN_HIP(hipMemcpyAsync(hw, hploc_words.ptr, sizeof(hw), hipMemcpyDeviceToHost, stream));
+        GWN_HIP(hipStreamSynchronize(stream));
+        if (hw[0] != NI || hw[1] != 0u || hw[2] >= NI)
+            throw_internal("bvh.build.hploc: H-PLOC builder did not converge to a complete binary hierarchy.");
+    }
     if (hs.error != 0u || empty_level == 0 || NW == 0u || NW > capacity || hs.leaves != T)
         throw_internal("bvh.build.collapse: Canonical wide collapse reported an invalid output.");
 

@@ -241,8 +241,6 @@ int gwn_build_bvh4_f32(
             gwn::throw_invalid("bvh.build: BVH build method is invalid.");
         if (method == GWN_BVH_BUILD_HPLOC && (hploc_search_radius == 0 || hploc_search_radius > 8))
             gwn::throw_invalid("bvh.build: H-PLOC search radius must be in the range [1, 8].");
-        if (method == GWN_BVH_BUILD_HPLOC)
-            gwn::throw_invalid("bvh.build: the H-PLOC builder is not available in this library version; use the LBVH method.");
         if (geometry->triangle_count != 0 && geometry->vertex_count == 0)
             gwn::throw_invalid("bvh.build: Cannot build a BVH with triangles but no vertices.");
         // Wide-node ids and per-level counts travel in 30-bit fields (look-back status words,
@@ -251,7 +249,7 @@ int gwn_build_bvh4_f32(
             gwn::throw_invalid("bvh.build: Triangle count exceeds the native wide-collapse counter range.");
         gwn_bvh_handle staging;
         staging.stream = stream;
-        gwn::build_bvh4(*geometry, staging, stream);
+        gwn::build_bvh4(*geometry, staging, method, hploc_search_radius, stream);
         if (geometry->triangle_count != 0) {
             staging.revision = gwn::reserve_revision();
             if (staging.revision == 0)
@@ -664,6 +662,66 @@ int gwn_compute_winding_gradient_antipodal_batch_f32(
 ) {
     return gwn_compute_winding_gradient_antipodal_batch_ex_f32(geometry, boundary, qx, qy, qz, gx, gy, gz, count,
                                                                GWN_PRECISION_PLAIN, config, stream);
+}
+
+// Adjoint batches (no reference counterpart).  Validation follows the forward batches they
+// differentiate; the vertex gradient arrays are added to, so several losses can share them.
+static void require_adjoint_geometry(gwn_geometry_t geometry, gwn_bvh_t bvh, gwn_query_config const &c,
+                                     float const *gvx, float const *gvy, float const *gvz) {
+    if (geometry == nullptr || (geometry->triangle_count != 0 && geometry->i0.ptr == nullptr))
+        gwn::throw_invalid("Geometry object contains no queryable data.");
+    require_bvh(bvh);
+    if (bvh->triangle_count != geometry->triangle_count)
+        gwn::throw_invalid("Geometry and BVH triangle counts differ.");
+    require_stack(*bvh, c.stack_capacity);
+    if (geometry->vertex_count != 0 && (gvx == nullptr || gvy == nullptr || gvz == nullptr))
+        gwn::throw_invalid("Vertex gradient spans must use non-null storage.");
+}
+
+int gwn_compute_winding_taylor_adjoint_batch_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy,
+    const float *qz, const float *grad_w, uint64_t count, float accuracy_scale, float *grad_vx, float *grad_vy,
+    float *grad_vz, float *grad_qx, float *grad_qy, float *grad_qz, const gwn_query_config *config,
+    hipStream_t stream
+) {
+    return guarded("gwn_compute_winding_taylor_adjoint_batch", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, true);
+        require_adjoint_geometry(geometry, bvh, c, grad_vx, grad_vy, grad_vz);
+        require_moment(moment, *bvh);
+        if (moment->order > 1)
+            gwn::throw_invalid("The winding adjoint supports Taylor orders 0 and 1.");
+        if (!(accuracy_scale >= 0.0f))
+            gwn::throw_invalid("Taylor accuracy scale must be nonnegative.");
+        require_query_storage(count, qx, qy, qz, grad_w);
+        if ((grad_qx != nullptr || grad_qy != nullptr || grad_qz != nullptr) && count != 0 &&
+            (grad_qx == nullptr || grad_qy == nullptr || grad_qz == nullptr))
+            gwn::throw_invalid("Query gradient spans must be all null or all non-null.");
+        if (count == 0)
+            return;
+        gwn::winding_taylor_adjoint(*geometry, *bvh, *moment, qx, qy, qz, grad_w, count, accuracy_scale, grad_vx,
+                                    grad_vy, grad_vz, grad_qx, grad_qy, grad_qz, stream);
+    });
+}
+
+int gwn_compute_unsigned_distance_adjoint_batch_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, const float *px, const float *py, const float *pz,
+    const float *grad_d, uint64_t count, float culling_band, float *grad_vx, float *grad_vy, float *grad_vz,
+    float *grad_px, float *grad_py, float *grad_pz, const gwn_query_config *config, hipStream_t stream
+) {
+    return guarded("gwn_compute_unsigned_distance_adjoint_batch", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, true);
+        require_adjoint_geometry(geometry, bvh, c, grad_vx, grad_vy, grad_vz);
+        require_query_storage(count, px, py, pz, grad_d);
+        if ((grad_px != nullptr || grad_py != nullptr || grad_pz != nullptr) && count != 0 &&
+            (grad_px == nullptr || grad_py == nullptr || grad_pz == nullptr))
+            gwn::throw_invalid("Point gradient spans must be all null or all non-null.");
+        if (count == 0)
+            return;
+        gwn::unsigned_distance_adjoint(*geometry, *bvh, px, py, pz, grad_d, count, culling_band, grad_vx, grad_vy,
+                                       grad_vz, grad_px, grad_py, grad_pz, stream);
+    });
 }
 
 int gwn_compute_query_order_f32(

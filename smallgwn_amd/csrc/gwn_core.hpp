@@ -235,6 +235,7 @@ struct gwn_boundary_handle {
 struct gwn_moment_handle {
     int order = 0;
     gwn::device_buffer<float4> records; // node_count * moment_record_vec4(order)
+    gwn::device_buffer<float> slot_area; // node_count * 4: each child slot's total triangle area
     std::size_t node_count = 0;
     std::uint64_t bvh_revision = 0;
     hipStream_t stream = nullptr;
@@ -255,7 +256,9 @@ void radix_sort_pairs_u32(
     std::uint32_t **vals_out, hipStream_t stream
 );
 
-void build_bvh4(gwn_geometry_handle const &geometry, gwn_bvh_handle &staging, hipStream_t stream);
+// Canonical BVH4 from the LBVH or H-PLOC (search radius 1-8) binary hierarchy.
+void build_bvh4(gwn_geometry_handle const &geometry, gwn_bvh_handle &staging, int method,
+                std::uint32_t hploc_radius, hipStream_t stream);
 // Algebraic boundary of the oriented triangle chain (directed rows in (min, max) vertex order).
 void build_boundary_chain(gwn_geometry_handle const &geometry, gwn_boundary_handle &staging, hipStream_t stream);
 void refit_moment(gwn_bvh_handle const &bvh, gwn_moment_handle &staging, hipStream_t stream);
@@ -291,6 +294,26 @@ void ray_first_hit(
     gwn_bvh_handle const &bvh, float const *ox, float const *oy, float const *oz, float const *dx,
     float const *dy, float const *dz, void *out, std::size_t n, float t_min, float t_max,
     query_launch const &cfg, hipStream_t stream
+);
+// Adjoint traversal (internal): per-slot field adjoints (12 floats per child slot) and per-record
+// near-field vertex adjoints (9 floats per record) are accumulated atomically; dq written.
+void taylor_adjoint_traverse(
+    gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy, float const *qz,
+    float const *u, std::uint32_t const *order, std::size_t n, float accuracy_scale, float *dslot, float *dtri,
+    float *dqx, float *dqy, float *dqz, hipStream_t stream
+);
+// Adjoint of L = sum_q u_q w_q for the Taylor winding (orders 0, 1): adds dL/dV to dvx, dvy, dvz
+// (vertex arrays of `geometry`) and writes dL/dq to dqx, dqy, dqz when they are non-null.
+void winding_taylor_adjoint(
+    gwn_geometry_handle const &geometry, gwn_bvh_handle const &bvh, gwn_moment_handle const &moment,
+    float const *qx, float const *qy, float const *qz, float const *u, std::size_t n, float accuracy_scale,
+    float *dvx, float *dvy, float *dvz, float *dqx, float *dqy, float *dqz, hipStream_t stream
+);
+// Adjoint of L = sum_p u_p d_p for the unsigned distance: adds dL/dV, writes dL/dp when non-null.
+void unsigned_distance_adjoint(
+    gwn_geometry_handle const &geometry, gwn_bvh_handle const &bvh, float const *px, float const *py,
+    float const *pz, float const *u, std::size_t n, float culling_band, float *dvx, float *dvy, float *dvz,
+    float *dpx, float *dpy, float *dpz, hipStream_t stream
 );
 // Antipodal winding numbers: positive-axis crossings over the BVH triangles plus the boundary
 // chain's spherical areas; axes retried Z, X, Y, NaN when all three are singular.

@@ -84,10 +84,11 @@ __device__ inline float box_distance2(float qx, float qy, float qz, aabb const &
     return dx * dx + dy * dy + dz * dz;
 }
 
-// Leaf range: records rebuilt in the reference's (v0, v0 + e1, v0 + e2) form.
+// Leaf range: records rebuilt in the reference's (v0, v0 + e1, v0 + e2) form.  `best_rec`, when
+// non-null, receives the record of each strict improvement.
 __device__ inline bool visit_leaf(
     tri_record const *__restrict__ tris, std::size_t nt, std::uint64_t ref, float qx, float qy, float qz,
-    float &best
+    float &best, std::uint32_t *best_rec
 ) {
     std::uint64_t const off = ref_offset(ref);
     std::uint32_t const cnt = ref_count(ref);
@@ -99,38 +100,42 @@ __device__ inline bool visit_leaf(
         float const e2x = t.v2x - t.v0x, e2y = t.v2y - t.v0y, e2z = t.v2z - t.v0z;
         float const d2 = point_triangle_distance2(qx, qy, qz, t.v0x, t.v0y, t.v0z, t.v0x + e1x, t.v0y + e1y,
                                                   t.v0z + e1z, t.v0x + e2x, t.v0y + e2y, t.v0z + e2z);
-        if (d2 < best)
+        if (d2 < best) {
             best = d2;
+            if (best_rec != nullptr)
+                *best_rec = static_cast<std::uint32_t>(off + k);
+        }
     }
     return best > 0.0f;
 }
 
+struct distance_scene {
+    bvh_node4 const *nodes;
+    std::size_t node_count;
+    tri_record const *tris;
+    std::size_t nt;
+    std::uint64_t root_ref;
+};
+
+enum : int { k_dist_done = 0, k_dist_zero = 1, k_dist_overflow = 2 };
+
+// Near-first culled traversal; `best` starts at the squared band and ends at the squared
+// distance.  Returns k_dist_zero when a triangle touches the query, k_dist_overflow when the
+// stack would overflow.
 template <int StackCap>
-__global__ __launch_bounds__(k_block) void distance_kernel(
-    bvh_node4 const *__restrict__ nodes, std::size_t node_count, tri_record const *__restrict__ tris,
-    std::size_t nt, std::uint64_t root_ref, float const *__restrict__ qx, float const *__restrict__ qy,
-    float const *__restrict__ qz, std::uint32_t const *__restrict__ order, float *__restrict__ out,
-    std::size_t n, float culling_band
+__device__ inline int distance_traverse(
+    distance_scene const &sc, float x, float y, float z, float &best, std::uint32_t *best_rec
 ) {
-    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (i >= n)
-        return;
-    std::size_t const q = order ? order[i] : i;
-    float const x = qx[q], y = qy[q], z = qz[q];
-    float const band = culling_band < 0.0f ? 0.0f : culling_band;
-    float best = band * band;
     std::uint32_t stack[StackCap];
     int sp = 0;
-    float result = 0.0f;
-    bool done = false;
-    std::uint64_t reference = root_ref;
-    while (!done) {
+    std::uint64_t reference = sc.root_ref;
+    for (;;) {
         std::uint64_t next = 0;
         if (ref_leaf(reference)) {
-            if (!visit_leaf(tris, nt, reference, x, y, z, best))
-                break; // distance zero: result stays 0
-        } else if (ref_internal(reference) && ref_offset(reference) < node_count) {
-            bvh_node4 const &node = nodes[ref_offset(reference)];
+            if (!visit_leaf(sc.tris, sc.nt, reference, x, y, z, best, best_rec))
+                return k_dist_zero;
+        } else if (ref_internal(reference) && ref_offset(reference) < sc.node_count) {
+            bvh_node4 const &node = sc.nodes[ref_offset(reference)];
             float cd[4];
             std::uint64_t cr[4];
             int m = 0;
@@ -168,44 +173,151 @@ __global__ __launch_bounds__(k_block) void distance_kernel(
             cswap(0, 2);
             cswap(1, 3);
             cswap(1, 2);
-            bool zero = false;
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                if (s < m && ref_leaf(cr[s]) && cd[s] < best && !visit_leaf(tris, nt, cr[s], x, y, z, best)) {
-                    zero = true;
-                    break;
-                }
+                if (s < m && ref_leaf(cr[s]) && cd[s] < best &&
+                    !visit_leaf(sc.tris, sc.nt, cr[s], x, y, z, best, best_rec))
+                    return k_dist_zero;
             }
-            if (zero)
-                break;
 #pragma unroll
             for (int s = 3; s >= 0; --s) {
                 if (s >= m || !ref_internal(cr[s]) || cd[s] >= best)
                     continue;
                 if (next != 0) {
-                    if (sp >= StackCap) {
-                        result = __builtin_nanf("");
-                        done = true;
-                        break;
-                    }
+                    if (sp >= StackCap)
+                        return k_dist_overflow;
                     stack[sp++] = static_cast<std::uint32_t>(ref_offset(next));
                 }
                 next = cr[s];
             }
-            if (done)
-                break;
         }
         if (next != 0) {
             reference = next;
             continue;
         }
-        if (sp == 0) {
-            result = sqrtf(best);
-            break;
-        }
+        if (sp == 0)
+            return k_dist_done;
         reference = encode_internal(stack[--sp]);
     }
-    out[q] = result;
+}
+
+template <int StackCap>
+__global__ __launch_bounds__(k_block) void distance_kernel(
+    distance_scene sc, float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
+    std::uint32_t const *__restrict__ order, float *__restrict__ out, std::size_t n, float culling_band
+) {
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    if (i >= n)
+        return;
+    std::size_t const q = order ? order[i] : i;
+    float const band = culling_band < 0.0f ? 0.0f : culling_band;
+    float best = band * band;
+    int const st = distance_traverse<StackCap>(sc, qx[q], qy[q], qz[q], best, nullptr);
+    out[q] = st == k_dist_zero ? 0.0f : (st == k_dist_overflow ? __builtin_nanf("") : sqrtf(best));
+}
+
+// Barycentric coordinates (of b and c) of the closest point of triangle (a, b, c) to p: the
+// Voronoi-region cases of point_triangle_distance2.
+__device__ inline void closest_barycentrics(
+    float const (&p)[3], float const (&a)[3], float const (&b)[3], float const (&c)[3], float &v, float &w
+) {
+    float const ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    float const ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    float const ap[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+    float const d1 = dot3(ab[0], ab[1], ab[2], ap[0], ap[1], ap[2]);
+    float const d2 = dot3(ac[0], ac[1], ac[2], ap[0], ap[1], ap[2]);
+    v = w = 0.0f;
+    if (d1 <= 0.0f && d2 <= 0.0f)
+        return;
+    float const bp[3] = {p[0] - b[0], p[1] - b[1], p[2] - b[2]};
+    float const d3 = dot3(ab[0], ab[1], ab[2], bp[0], bp[1], bp[2]);
+    float const d4 = dot3(ac[0], ac[1], ac[2], bp[0], bp[1], bp[2]);
+    if (d3 >= 0.0f && d4 <= d3) {
+        v = 1.0f;
+        return;
+    }
+    float const vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        v = d1 / (d1 - d3);
+        return;
+    }
+    float const cp[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
+    float const d5 = dot3(ab[0], ab[1], ab[2], cp[0], cp[1], cp[2]);
+    float const d6 = dot3(ac[0], ac[1], ac[2], cp[0], cp[1], cp[2]);
+    if (d6 >= 0.0f && d5 <= d6) {
+        w = 1.0f;
+        return;
+    }
+    float const vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        w = d2 / (d2 - d6);
+        return;
+    }
+    float const va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        v = 1.0f - w;
+        return;
+    }
+    float const denom = 1.0f / (va + vb + vc);
+    v = vb * denom;
+    w = vc * denom;
+}
+
+// Adjoint of sum_p u_p d_p: the forward traversal (same decisions) remembers the nearest record;
+// with c = sum b_i v_i its closest point and n = (p - c) / |p - c|, d d/dp = n and
+// d d/dv_i = -b_i n (the closest point moves with the triangle).  Points on the surface, beyond the
+// band or overflowing the stack contribute nothing (NaN dp on overflow).
+template <int StackCap>
+__global__ __launch_bounds__(k_block) void distance_adjoint_kernel(
+    distance_scene sc, float const *__restrict__ px, float const *__restrict__ py, float const *__restrict__ pz,
+    float const *__restrict__ u, std::uint32_t const *__restrict__ order, std::size_t n, float culling_band,
+    std::uint32_t const *__restrict__ prim, std::uint32_t const *__restrict__ i0, std::uint32_t const *__restrict__ i1,
+    std::uint32_t const *__restrict__ i2, float *dvx, float *dvy, float *dvz, float *__restrict__ dpx,
+    float *__restrict__ dpy, float *__restrict__ dpz
+) {
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    if (i >= n)
+        return;
+    std::size_t const q = order ? order[i] : i;
+    float const p[3] = {px[q], py[q], pz[q]};
+    float const band = culling_band < 0.0f ? 0.0f : culling_band;
+    float best = band * band;
+    std::uint32_t rec = 0xffffffffu;
+    int const st = distance_traverse<StackCap>(sc, p[0], p[1], p[2], best, &rec);
+    float g[3] = {0.0f, 0.0f, 0.0f};
+    if (st == k_dist_overflow) {
+        g[0] = g[1] = g[2] = __builtin_nanf("");
+    } else if (st == k_dist_done && rec != 0xffffffffu) {
+        tri_record const t = sc.tris[rec];
+        float const a[3] = {t.v0x, t.v0y, t.v0z};
+        float const b[3] = {t.v0x + (t.v1x - t.v0x), t.v0y + (t.v1y - t.v0y), t.v0z + (t.v1z - t.v0z)};
+        float const c[3] = {t.v0x + (t.v2x - t.v0x), t.v0y + (t.v2y - t.v0y), t.v0z + (t.v2z - t.v0z)};
+        float v, w;
+        closest_barycentrics(p, a, b, c, v, w);
+        float const bary[3] = {1.0f - v - w, v, w};
+        float r[3];
+        for (int k = 0; k < 3; ++k)
+            r[k] = p[k] - (bary[0] * a[k] + bary[1] * b[k] + bary[2] * c[k]);
+        float const len = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        if (len > 0.0f) {
+            float const su = u[q] / len;
+            std::uint32_t const pr = prim[rec];
+            std::uint32_t const ids[3] = {i0[pr], i1[pr], i2[pr]};
+            for (int k = 0; k < 3; ++k)
+                g[k] = su * r[k];
+            for (int vtx = 0; vtx < 3; ++vtx) {
+                atomicAdd(dvx + ids[vtx], -bary[vtx] * g[0]);
+                atomicAdd(dvy + ids[vtx], -bary[vtx] * g[1]);
+                atomicAdd(dvz + ids[vtx], -bary[vtx] * g[2]);
+            }
+        }
+    }
+    if (dpx != nullptr) {
+        dpx[q] = g[0];
+        dpy[q] = g[1];
+        dpz[q] = g[2];
+    }
 }
 
 template <int Cap>
@@ -213,10 +325,8 @@ void launch_distance(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, std::uint32_t const *order,
     float *out, std::size_t n, float band, hipStream_t stream
 ) {
-    distance_kernel<Cap><<<grid_for(n, k_block), k_block, 0, stream>>>(
-        bvh.nodes.ptr, bvh.node_count, bvh.triangles.ptr, bvh.triangle_count, bvh.root.reference, qx, qy, qz,
-        order, out, n, band
-    );
+    distance_scene const sc{bvh.nodes.ptr, bvh.node_count, bvh.triangles.ptr, bvh.triangle_count, bvh.root.reference};
+    distance_kernel<Cap><<<grid_for(n, k_block), k_block, 0, stream>>>(sc, qx, qy, qz, order, out, n, band);
     GWN_LAUNCH_CHECK();
 }
 
@@ -243,6 +353,36 @@ void unsigned_distance(
         launch_distance<128>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, stream);
     else
         throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
+}
+
+void unsigned_distance_adjoint(
+    gwn_geometry_handle const &g, gwn_bvh_handle const &bvh, float const *px, float const *py, float const *pz,
+    float const *u, std::size_t n, float culling_band, float *dvx, float *dvy, float *dvz, float *dpx, float *dpy,
+    float *dpz, hipStream_t stream
+) {
+    if (n == 0 || bvh.triangle_count == 0)
+        return;
+    device_buffer<std::uint32_t> order;
+    if (n >= (1u << 16)) {
+        order.allocate(n, stream);
+        query_order(bvh, px, py, pz, order.ptr, n, stream);
+    }
+    distance_scene const sc{bvh.nodes.ptr, bvh.node_count, bvh.triangles.ptr, bvh.triangle_count, bvh.root.reference};
+    int const need = static_cast<int>(bvh.internal_stack_bound);
+#define GWN_DADJ(C)                                                                                            \
+    distance_adjoint_kernel<C><<<grid_for(n, k_block), k_block, 0, stream>>>(                                 \
+        sc, px, py, pz, u, order.ptr, n, culling_band, bvh.primitive_indices.ptr, g.i0.ptr, g.i1.ptr, g.i2.ptr, \
+        dvx, dvy, dvz, dpx, dpy, dpz)
+    if (need <= 32)
+        GWN_DADJ(32);
+    else if (need <= 64)
+        GWN_DADJ(64);
+    else if (need <= 128)
+        GWN_DADJ(128);
+    else
+        throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
+#undef GWN_DADJ
+    GWN_LAUNCH_CHECK();
 }
 
 } // namespace gwn

@@ -263,7 +263,7 @@ __global__ __launch_bounds__(k_block) void moment_refit_kernel(
     bvh_node4 const *__restrict__ nodes, std::size_t node_count, tri_record const *__restrict__ tris,
     std::size_t tri_count, std::uint32_t const *__restrict__ parent,
     std::uint8_t const *__restrict__ internal_children, std::uint32_t *arrivals,
-    payload<Order> *pending, float4 *__restrict__ records, std::uint32_t *error
+    payload<Order> *pending, float4 *__restrict__ records, float *__restrict__ slot_area, std::uint32_t *error
 ) {
     constexpr int RV = moment_record_vec4(Order);
     std::size_t const start = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
@@ -307,6 +307,7 @@ __global__ __launch_bounds__(k_block) void moment_refit_kernel(
                 cref[s] = k_cref_leaf | static_cast<std::uint32_t>(off);
             }
             write_fields<Order>(fields, s, ch.bounds, child);
+            slot_area[n * 4 + s] = child.v[0];
             if (init)
                 payload_add(total, child);
             else
@@ -337,6 +338,8 @@ void launch_refit(gwn_bvh_handle const &bvh, gwn_moment_handle &m, hipStream_t s
     constexpr int RV = moment_record_vec4(Order);
     m.records.allocate(N * RV, stream);
     fill_u32(reinterpret_cast<std::uint32_t *>(m.records.ptr), 0u, N * RV * 4, stream);
+    m.slot_area.allocate(N * 4, stream);
+    fill_u32(reinterpret_cast<std::uint32_t *>(m.slot_area.ptr), 0u, N * 4, stream);
     device_buffer<payload<Order>> pending(N, stream);
     // Per-call arrival counters (plus the error word) so refits of several orders may run
     // concurrently on one BVH.
@@ -345,7 +348,7 @@ void launch_refit(gwn_bvh_handle const &bvh, gwn_moment_handle &m, hipStream_t s
     std::uint32_t *error = arrivals.ptr + N;
     moment_refit_kernel<Order><<<grid_for(N, k_block), k_block, 0, stream>>>(
         bvh.nodes.ptr, N, bvh.triangles.ptr, bvh.triangle_count, bvh.parent.ptr,
-        bvh.internal_children.ptr, arrivals.ptr, pending.ptr, m.records.ptr, error
+        bvh.internal_children.ptr, arrivals.ptr, pending.ptr, m.records.ptr, m.slot_area.ptr, error
     );
     GWN_LAUNCH_CHECK();
     std::uint32_t host_error = 0;

@@ -675,6 +675,176 @@ __global__ __launch_bounds__(256) void leaf_root_kernel(
     }
 }
 
+// Derivatives of one triangle's solid angle W(a, b, c) seen from the origin of a, b, c (vertices
+// relative to the query), from the unnormalised form W = 2 atan2(a.(b x c), D) with
+// D = |a||b||c| + (a.b)|c| + (a.c)|b| + (b.c)|a| (equal to the Van Oosterom-Strackee quotient
+// scaled by |a||b||c|):  dW = 2 (D dnum - num dD) / (num^2 + D^2).  Zero where W is defined as 0.
+__device__ inline void solid_angle_vertex_grad(
+    float const (&a)[3], float const (&b)[3], float const (&c)[3], float (&da)[3], float (&db)[3], float (&dc)[3]
+) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        da[k] = db[k] = dc[k] = 0.0f;
+    float const la = sqrtf(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    float const lb = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    float const lc = sqrtf(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    if (la == 0.0f || lb == 0.0f || lc == 0.0f)
+        return;
+    float const bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+    float const ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+    float const ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    float const num = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+    if (num == 0.0f)
+        return;
+    float const dab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    float const dac = a[0] * c[0] + a[1] * c[1] + a[2] * c[2];
+    float const dbc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
+    float const den = la * lb * lc + dab * lc + dac * lb + dbc * la;
+    float const sc = 2.0f / (num * num + den * den);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float const ha = a[k] / la, hb = b[k] / lb, hc = c[k] / lc;
+        float const dDa = ha * lb * lc + b[k] * lc + c[k] * lb + dbc * ha;
+        float const dDb = hb * la * lc + a[k] * lc + c[k] * la + dac * hb;
+        float const dDc = hc * la * lb + a[k] * lb + b[k] * la + dab * hc;
+        da[k] = sc * (den * bc[k] - num * dDa);
+        db[k] = sc * (den * ca[k] - num * dDb);
+        dc[k] = sc * (den * ab[k] - num * dDc);
+    }
+}
+
+// Adds u * dW/d(a, b, c) of one near-field triangle to its record's 9 adjoint entries and
+// returns -(da + db + dc) * u (the query part) through dq.
+__device__ inline void adjoint_leaf(
+    tri_record const &t, std::size_t rec, float x, float y, float z, float su, float *__restrict__ dtri,
+    float (&dq)[3]
+) {
+    float const a[3] = {t.v0x - x, t.v0y - y, t.v0z - z};
+    float const b[3] = {t.v1x - x, t.v1y - y, t.v1z - z};
+    float const c[3] = {t.v2x - x, t.v2y - y, t.v2z - z};
+    float da[3], db[3], dc[3];
+    solid_angle_vertex_grad(a, b, c, da, db, dc);
+    float *d = dtri + rec * 9;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        atomicAdd(d + k, su * da[k]);
+        atomicAdd(d + 3 + k, su * db[k]);
+        atomicAdd(d + 6 + k, su * dc[k]);
+        dq[k] -= su * (da[k] + db[k] + dc[k]);
+    }
+}
+
+// Adjoint of sum_q u_q w_q (orders 0, 1): the per-lane traversal of taylor_kernel with the same
+// accept/descend decisions.  An accepted child adds u/(4 pi) (dw/dP, dw/dN, dw/dT) to its slot's
+// 12 entries (dw/dP = -dw/dq); a near triangle adds u/(4 pi) dW/d(a, b, c) to its record's 9
+// entries; the query gradient goes to dq.  root_node = k_invalid evaluates the leaf root instead.
+template <int Order, int StackCap>
+__global__ __launch_bounds__(k_taylor_block) void taylor_adjoint_kernel(
+    float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node,
+    std::uint64_t root_leaf, float const *__restrict__ qx, float const *__restrict__ qy,
+    float const *__restrict__ qz, float const *__restrict__ u, std::uint32_t const *__restrict__ order,
+    std::size_t n, float beta2, float *__restrict__ dslot, float *__restrict__ dtri, float *__restrict__ dqx,
+    float *__restrict__ dqy, float *__restrict__ dqz
+) {
+    static_assert(Order <= 1, "adjoint orders are 0 and 1");
+    constexpr int RV = moment_record_vec4(Order);
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_taylor_block + threadIdx.x;
+    if (i >= n)
+        return;
+    std::size_t const q = order ? order[i] : i;
+    float const x = qx[q], y = qy[q], z = qz[q];
+    float const su = u[q] * k_inv_4pi_f;
+    float dq[3] = {0.0f, 0.0f, 0.0f};
+    if (root_node == k_invalid) {
+        std::uint64_t const off = ref_offset(root_leaf);
+        std::uint32_t const cnt = ref_count(root_leaf);
+        for (std::uint32_t k = 0; k < cnt; ++k)
+            adjoint_leaf(tris[off + k], off + k, x, y, z, su, dtri, dq);
+    } else {
+        std::uint32_t stack[StackCap];
+        int sp = 0;
+        std::uint32_t node = root_node;
+        for (;;) {
+            float4 const *rec = records + static_cast<std::size_t>(node) * RV;
+            float4 const refs = rec[0];
+            float4 const mp = rec[1], px = rec[2], py = rec[3], pz = rec[4];
+            float const *fields = reinterpret_cast<float const *>(rec + 1);
+            std::uint32_t const cref[4] = {__float_as_uint(refs.x), __float_as_uint(refs.y),
+                                           __float_as_uint(refs.z), __float_as_uint(refs.w)};
+            float const mpd[4] = {mp.x, mp.y, mp.z, mp.w};
+            float const cx[4] = {px.x, px.y, px.z, px.w};
+            float const cy[4] = {py.x, py.y, py.z, py.w};
+            float const cz[4] = {pz.x, pz.y, pz.z, pz.w};
+            std::uint32_t next = k_invalid;
+            bool overflow = false;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                std::uint32_t const c = cref[s];
+                if (c == k_invalid)
+                    continue;
+                float const qrx = x - cx[s], qry = y - cy[s], qrz = z - cz[s];
+                float const ql2 = squared_norm_unfused(qrx, qry, qrz);
+                bool const descend = !(ql2 > 0.0f) || ql2 <= product_unfused(mpd[s], beta2);
+                if (!descend) {
+                    float g[3];
+                    float const w = taylor_term<Order, true>(fields, s, qrx, qry, qrz, ql2, g);
+                    if (isfinite(w)) {
+                        float const m1 = __builtin_amdgcn_rsqf(ql2);
+                        float const m2 = m1 * m1;
+                        float const qn[3] = {qrx * m1, qry * m1, qrz * m1};
+                        float *d = dslot + (static_cast<std::size_t>(node) * 4 + s) * 12;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            atomicAdd(d + k, -su * g[k]);
+                            atomicAdd(d + 3 + k, -su * m2 * qn[k]);
+                            dq[k] += su * g[k];
+                        }
+                        if constexpr (Order >= 1) {
+                            float const m3 = m2 * m1;
+                            atomicAdd(d + 6, su * m3 * (1.0f - 3.0f * qn[0] * qn[0]));
+                            atomicAdd(d + 7, su * m3 * (1.0f - 3.0f * qn[1] * qn[1]));
+                            atomicAdd(d + 8, su * m3 * (1.0f - 3.0f * qn[2] * qn[2]));
+                            atomicAdd(d + 9, -3.0f * su * m3 * qn[0] * qn[1]);
+                            atomicAdd(d + 10, -3.0f * su * m3 * qn[1] * qn[2]);
+                            atomicAdd(d + 11, -3.0f * su * m3 * qn[2] * qn[0]);
+                        }
+                        continue;
+                    }
+                }
+                if (c & k_cref_leaf) {
+                    std::uint32_t const r = c & ~k_cref_leaf;
+                    adjoint_leaf(tris[r], r, x, y, z, su, dtri, dq);
+                    continue;
+                }
+                if (next != k_invalid) {
+                    if (sp >= StackCap) {
+                        overflow = true;
+                        break;
+                    }
+                    stack[sp++] = next;
+                }
+                next = c;
+            }
+            if (overflow) {
+                dq[0] = dq[1] = dq[2] = __builtin_nanf("");
+                break;
+            }
+            if (next != k_invalid) {
+                node = next;
+                continue;
+            }
+            if (sp == 0)
+                break;
+            node = stack[--sp];
+        }
+    }
+    if (dqx != nullptr) {
+        dqx[q] = dq[0];
+        dqy[q] = dq[1];
+        dqz[q] = dq[2];
+    }
+}
+
 template <int Order, int Prec, bool Grad, int Cap>
 void launch_taylor_cap(
     gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy,
@@ -714,7 +884,47 @@ void launch_taylor(
         throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
 }
 
+template <int Order>
+void launch_taylor_adjoint(
+    gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy, float const *qz,
+    float const *u, std::uint32_t const *order, std::size_t n, float beta2, float *dslot, float *dtri,
+    float *dqx, float *dqy, float *dqz, hipStream_t stream
+) {
+    bool const leaf_root = ref_leaf(bvh.root.reference);
+    std::uint32_t const root = leaf_root ? k_invalid : static_cast<std::uint32_t>(ref_offset(bvh.root.reference));
+    int const need = leaf_root ? 0 : static_cast<int>(bvh.internal_stack_bound);
+    float4 const *records = leaf_root ? nullptr : m.records.ptr;
+#define GWN_ADJ(C)                                                                                           \
+    taylor_adjoint_kernel<Order, C><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(            \
+        records, bvh.triangles.ptr, root, bvh.root.reference, qx, qy, qz, u, order, n, beta2, dslot, dtri, dqx, \
+        dqy, dqz)
+    if (need <= 32)
+        GWN_ADJ(32);
+    else if (need <= 64)
+        GWN_ADJ(64);
+    else if (need <= 128)
+        GWN_ADJ(128);
+    else
+        throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
+#undef GWN_ADJ
+    GWN_LAUNCH_CHECK();
+}
+
 } // namespace
+
+void taylor_adjoint_traverse(
+    gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy, float const *qz,
+    float const *u, std::uint32_t const *order, std::size_t n, float accuracy_scale, float *dslot, float *dtri,
+    float *dqx, float *dqy, float *dqz, hipStream_t stream
+) {
+    float const beta2 = accuracy_scale * accuracy_scale;
+    if (m.order == 0)
+        launch_taylor_adjoint<0>(bvh, m, qx, qy, qz, u, order, n, beta2, dslot, dtri, dqx, dqy, dqz, stream);
+    else if (m.order == 1)
+        launch_taylor_adjoint<1>(bvh, m, qx, qy, qz, u, order, n, beta2, dslot, dtri, dqx, dqy, dqz, stream);
+    else
+        throw_invalid("The winding adjoint supports Taylor orders 0 and 1.");
+}
 
 void winding_exact(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, void *out,

@@ -42,6 +42,7 @@ EXPORTED_SYMBOLS = (
     "gwn_compute_unsigned_distance_batch_f32", "gwn_compute_ray_first_hit_batch_f32",
     "gwn_compute_winding_number_antipodal_batch_f32", "gwn_compute_winding_number_antipodal_batch_ex_f32",
     "gwn_compute_winding_gradient_antipodal_batch_f32", "gwn_compute_winding_gradient_antipodal_batch_ex_f32",
+    "gwn_compute_winding_taylor_adjoint_batch_f32", "gwn_compute_unsigned_distance_adjoint_batch_f32",
     "gwn_compute_query_order_f32", "gwn_radix_sort_pairs_u64", "gwn_radix_sort_pairs_u32",
 )
 
@@ -168,6 +169,10 @@ def _load():
         "gwn_compute_winding_number_antipodal_batch_ex_f32": [vp, vp, vp, vp, vp, vp, vp, u64, i32, vp, vp],
         "gwn_compute_winding_gradient_antipodal_batch_f32": [vp, vp, vp, vp, vp, vp, vp, vp, u64, vp, vp],
         "gwn_compute_winding_gradient_antipodal_batch_ex_f32": [vp, vp, vp, vp, vp, vp, vp, vp, u64, i32, vp, vp],
+        "gwn_compute_winding_taylor_adjoint_batch_f32": [
+            vp, vp, vp, vp, vp, vp, vp, u64, f32, vp, vp, vp, vp, vp, vp, vp, vp],
+        "gwn_compute_unsigned_distance_adjoint_batch_f32": [
+            vp, vp, vp, vp, vp, vp, u64, f32, vp, vp, vp, vp, vp, vp, vp, vp],
         "gwn_compute_query_order_f32": [vp, vp, vp, vp, vp, u64, vp],
         "gwn_radix_sort_pairs_u64": [vp, vp, u64, i32, i32, vp],
         "gwn_radix_sort_pairs_u32": [vp, vp, u64, i32, i32, vp],
@@ -585,6 +590,47 @@ def gwn_compute_winding_gradient_antipodal_batch(geometry: GeometryObject, bound
     return _status(_lib.gwn_compute_winding_gradient_antipodal_batch_ex_f32(
         geometry.handle, boundary.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(gx), _dptr(gy), _dptr(gz),
         n, int(precision), ctypes.byref(cfg), _stream(stream)))
+
+
+def gwn_compute_winding_taylor_adjoint_batch(geometry: GeometryObject, bvh: BvhObject, moment: MomentObject,
+                                             qx, qy, qz, grad_w, grad_vx, grad_vy, grad_vz, grad_qx=None,
+                                             grad_qy=None, grad_qz=None, accuracy_scale: float = 2.0,
+                                             config: QueryConfig | None = None, stream=None) -> Status:
+    """dL/dV (added to grad_v*) and dL/dq (written to grad_q* when given) of L = sum grad_w * w for the
+    Taylor winding of orders 0 and 1 (no reference counterpart)."""
+    n = _numel(qx)
+    if _numel(qy) != n or _numel(qz) != n or _numel(grad_w) != n:
+        return Status(Error.INVALID_ARGUMENT, "Query SoA spans require identical lengths.")
+    if grad_qx is not None and any(_numel(a) != n for a in (grad_qx, grad_qy, grad_qz)):
+        return Status(Error.INVALID_ARGUMENT, "Query gradient span sizes differ from query count.")
+    nv = geometry.vertex_count()
+    if any(_numel(a) != nv for a in (grad_vx, grad_vy, grad_vz)):
+        return Status(Error.INVALID_ARGUMENT, "Vertex gradient span sizes differ from vertex count.")
+    cfg = _config(config)
+    return _status(_lib.gwn_compute_winding_taylor_adjoint_batch_f32(
+        geometry.handle, bvh.handle, moment.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(grad_w), n,
+        float(accuracy_scale), _dptr(grad_vx), _dptr(grad_vy), _dptr(grad_vz), _dptr(grad_qx), _dptr(grad_qy),
+        _dptr(grad_qz), ctypes.byref(cfg), _stream(stream)))
+
+
+def gwn_compute_unsigned_distance_adjoint_batch(geometry: GeometryObject, bvh: BvhObject, px, py, pz, grad_d,
+                                                grad_vx, grad_vy, grad_vz, grad_px=None, grad_py=None,
+                                                grad_pz=None, culling_band: float = float("inf"),
+                                                config: QueryConfig | None = None, stream=None) -> Status:
+    """dL/dV (added) and dL/dp (written when given) of L = sum grad_d * d for the unsigned distance."""
+    n = _numel(px)
+    if _numel(py) != n or _numel(pz) != n or _numel(grad_d) != n:
+        return Status(Error.INVALID_ARGUMENT, "Query SoA spans require identical lengths.")
+    if grad_px is not None and any(_numel(a) != n for a in (grad_px, grad_py, grad_pz)):
+        return Status(Error.INVALID_ARGUMENT, "Point gradient span sizes differ from point count.")
+    nv = geometry.vertex_count()
+    if any(_numel(a) != nv for a in (grad_vx, grad_vy, grad_vz)):
+        return Status(Error.INVALID_ARGUMENT, "Vertex gradient span sizes differ from vertex count.")
+    cfg = _config(config)
+    return _status(_lib.gwn_compute_unsigned_distance_adjoint_batch_f32(
+        geometry.handle, bvh.handle, _dptr(px), _dptr(py), _dptr(pz), _dptr(grad_d), n, float(culling_band),
+        _dptr(grad_vx), _dptr(grad_vy), _dptr(grad_vz), _dptr(grad_px), _dptr(grad_py), _dptr(grad_pz),
+        ctypes.byref(cfg), _stream(stream)))
 
 
 def gwn_compute_query_order(bvh: BvhObject, qx, qy, qz, order, stream=None) -> Status:

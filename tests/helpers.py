@@ -157,7 +157,7 @@ def check_bvh4(nodes_raw, root_reference: int, primitive_indices, triangles, mes
     assert sp == packed_stack_bound
 
 
-def check_collapse_rule(nodes_raw, binary_nodes, binary_bounds):
+def check_collapse_rule(nodes_raw, binary_nodes, binary_bounds, binary_root: int = 0):
     """Largest-half-area expansion (include/gwn/detail/gwn_bvh_build_collapse.cuh:170-211).
 
     For every wide node, the binary nodes opened inside it all have half-area >= every
@@ -198,7 +198,7 @@ def check_collapse_rule(nodes_raw, binary_nodes, binary_bounds):
         return opened, cands
 
     # Walk wide nodes from the binary root and compare child order/bounds.
-    stack = [(0, 0)]  # (wide node, binary root)
+    stack = [(0, int(binary_root))]  # (wide node, binary root)
     while stack:
         w, b = stack.pop()
         opened, cands = frontier(b)

@@ -82,8 +82,8 @@ def build_gpu(gwn, mesh, method=None):
     return geom, bvh
 
 
-def assert_bvh_matches_oracle(gwn, orc, mesh, bvh):
-    ref = orc.build_bvh4(mesh)
+def assert_bvh_matches_oracle(gwn, orc, mesh, bvh, method="lbvh", radius=8):
+    ref = orc.build_bvh4(mesh, method=method, radius=radius)
     info = bvh.info()
     nodes, prim, tris = bvh.download()
     assert info["node_count"] == ref.node_count
@@ -175,6 +175,121 @@ def test_repeated_builds_and_refits_are_bit_identical(gwn, orc):
         gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
         got = mom.download(bvh.info()["node_count"])
         assert np.array_equal(got.view(np.uint32), ref_m.view(np.uint32)), f"refit {rep}: moments differ"
+
+
+# ---------------------------------------------------------------------------------------------
+# H-PLOC builder (gwn_build_bvh with method = H-PLOC)
+# ---------------------------------------------------------------------------------------------
+
+def _equal_triangles(n):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=np.float32)
+    return synth._mesh_from_arrays(v, np.tile([[0, 1, 2]], (n, 1)))
+
+
+def _hploc(gwn, mesh, radius=8, geom=None):
+    if geom is None:
+        geom = gwn.GeometryObject()
+        gwn.gwn_upload_geometry(geom, mesh.vx, mesh.vy, mesh.vz, mesh.i0, mesh.i1, mesh.i2).raise_if_error()
+    bvh = gwn.BvhObject()
+    gwn.gwn_build_bvh(geom, bvh, gwn.BuildOptions(method=gwn.BuildMethod.HPLOC,
+                                                  hploc_search_radius=radius)).raise_if_error()
+    return geom, bvh
+
+
+@pytest.mark.parametrize("radius", [1, 4, 8])
+@pytest.mark.parametrize("name", list(MESHES))
+def test_hploc_build_matches_oracle(gwn, orc, name, radius):
+    """Node arrays, primitive order, records, root and stack bounds equal the oracle's H-PLOC
+    restatement bit for bit (deterministic node ids on both sides)."""
+    print(f"[hploc] {name} radius={radius}", flush=True)
+    mesh = MESHES[name]
+    _, bvh = _hploc(gwn, mesh, radius)
+    ref = assert_bvh_matches_oracle(gwn, orc, mesh, bvh, method="hploc", radius=radius)
+    nodes, prim, tris = bvh.download()
+    helpers.check_bvh4(nodes, bvh.info()["root_reference"], prim, tris, mesh,
+                       ref.internal_stack_bound, ref.packed_stack_bound)
+
+
+@pytest.mark.parametrize("count", [2, 3, 15, 16, 17, 31, 32, 33, 63, 64, 65, 129, 513])
+def test_hploc_repeated_equal_bounds_match_oracle(gwn, orc, count):
+    """tests/integration_gwn_builder.cu:301-343: identical triangles around the list threshold,
+    the wave width and multi-block launches, for radius 1, 4 and 8."""
+    mesh = _equal_triangles(count)
+    for radius in (1, 4, 8):
+        print(f"[hploc] equal count={count} radius={radius}", flush=True)
+        _, bvh = _hploc(gwn, mesh, radius)
+        assert_bvh_matches_oracle(gwn, orc, mesh, bvh, method="hploc", radius=radius)
+
+
+def test_hploc_large_mesh_matches_oracle(gwn, orc):
+    """The 1,048,576-triangle displaced sphere built with H-PLOC (radius 8)."""
+    mesh = synth.config_large_mesh()
+    print("[hploc] large mesh", flush=True)
+    _, bvh = _hploc(gwn, mesh)
+    assert_bvh_matches_oracle(gwn, orc, mesh, bvh, method="hploc", radius=8)
+
+
+def test_hploc_repeated_builds_are_bit_identical(gwn, orc):
+    mesh = MESHES["displaced_256x128"]
+    ref = orc.build_bvh4(mesh, method="hploc")
+    geom = None
+    for rep in range(20):
+        geom, bvh = _hploc(gwn, mesh, 8, geom)
+        nodes, prim, _ = bvh.download()
+        assert nodes.tobytes() == ref.nodes.tobytes(), f"build {rep}: nodes differ"
+        assert np.array_equal(prim, ref.primitive_indices)
+
+
+def test_hploc_radius_validation(gwn):
+    mesh = _equal_triangles(8)
+    geom = gwn.GeometryObject()
+    gwn.gwn_upload_geometry(geom, mesh.vx, mesh.vy, mesh.vz, mesh.i0, mesh.i1, mesh.i2).raise_if_error()
+    bvh = gwn.BvhObject()
+    for radius in (0, 9):
+        st = gwn.gwn_build_bvh(geom, bvh, gwn.BuildOptions(method=gwn.BuildMethod.HPLOC, hploc_search_radius=radius))
+        assert st.error == gwn.Error.INVALID_ARGUMENT and "radius" in st.message
+    assert not bvh.has_data()
+
+
+def _builder_queries(mesh):
+    """tests/integration_gwn_builder.cu:120-166: the box centre, six points 1.3 extents out along
+    the axes and two interior points."""
+    v = np.stack([mesh.vx, mesh.vy, mesh.vz], 1).astype(np.float32)
+    lo, hi = v.min(axis=0), v.max(axis=0)
+    c = (lo + hi) * np.float32(0.5)
+    e = np.maximum(hi - lo, np.float32(1e-2))
+    pts = [c]
+    for axis in range(3):
+        for sign in (1, -1):
+            p = c.copy()
+            p[axis] = c[axis] + np.float32(sign * 1.3) * e[axis]
+            pts.append(p)
+    pts.append(c + np.float32([0.31, -0.27, 0.23]) * e)
+    pts.append(c + np.float32([-0.19, 0.37, -0.29]) * e)
+    q = np.stack(pts).astype(np.float32)
+    return q[:, 0].copy(), q[:, 1].copy(), q[:, 2].copy()
+
+
+@pytest.mark.parametrize("name", ["icosphere4", "torus", "displaced_256x128", "soup"])
+def test_hploc_tree_serves_queries(gwn, orc, dev, name):
+    """tests/integration_gwn_builder.cu:169-239: exact and fully descended (beta = 1e6) Taylor
+    winding on the H-PLOC tree match the FP64 exact winding within 2e-5 + 2e-5 |w|."""
+    import torch
+
+    mesh = MESHES[name]
+    _, bvh = _hploc(gwn, mesh)
+    mom = gwn.MomentObject(2)
+    gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+    qx, qy, qz = _builder_queries(mesh)
+    d = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (qx, qy, qz)]
+    out = torch.zeros(qx.shape[0], dtype=torch.float32, device=dev)
+    gwn.gwn_compute_winding_number_taylor_batch(bvh, mom, *d, out, accuracy_scale=1e6).raise_if_error()
+    ex = torch.zeros_like(out)
+    gwn.gwn_compute_winding_number_exact_batch(bvh, *d, ex).raise_if_error()
+    torch.cuda.synchronize()
+    ref = orc.winding_exact_mesh_f64(mesh, qx, qy, qz)
+    for got in (out.cpu().numpy(), ex.cpu().numpy()):
+        assert np.all(np.abs(got - ref) <= 2e-5 + 2e-5 * np.abs(ref))
 
 
 # ---------------------------------------------------------------------------------------------

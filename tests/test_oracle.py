@@ -838,3 +838,339 @@ def test_antipodal_all_axes_singular_is_nan(orc):
     mesh = synth.open_cube()
     w = _antipodal(orc, mesh, [1.0], [1.0], [1.0])
     assert np.isnan(w[0])
+
+
+# ---------------------------------------------------------------------------
+# H-PLOC binary builder (include/gwn/detail/gwn_bvh_build_hploc.cuh:104-622)
+# ---------------------------------------------------------------------------
+
+def _equal_triangles(n):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=np.float32)
+    return synth._mesh_from_arrays(v, np.tile([[0, 1, 2]], (n, 1)))
+
+
+def _children(bvh, node):
+    l, r, lk, rk = (int(x) for x in bvh.binary_nodes[node])
+    return (l, lk), (r, rk)
+
+
+def _leaf_sets(bvh):
+    """Sorted-leaf set of every binary internal node, from the node arrays alone."""
+    T = bvh.binary_nodes.shape[0] + 1
+    memo = {}
+
+    def rec(node):
+        if node in memo:
+            return memo[node]
+        out = []
+        for idx, kind in _children(bvh, node):
+            out.extend([idx] if kind == 2 else rec(idx))
+        memo[node] = sorted(out)
+        return memo[node]
+
+    import sys
+
+    sys.setrecursionlimit(max(10000, 4 * T))
+    rec(bvh.binary_root)
+    return memo
+
+
+def _check_complete_binary_tree(bvh, T):
+    """2T - 1 clusters: every leaf and every non-root internal node is a child exactly once
+    (hploc.cuh:613-619), bounds are unions of the children, first leaf is the smallest leaf."""
+    leaf_seen = np.zeros(T, np.int64)
+    node_seen = np.zeros(T - 1, np.int64)
+    for n in range(T - 1):
+        for idx, kind in _children(bvh, n):
+            assert kind in (1, 2)
+            (leaf_seen if kind == 2 else node_seen)[idx] += 1
+    assert np.all(leaf_seen == 1)
+    expect_nodes = np.ones(T - 1, np.int64)
+    expect_nodes[bvh.binary_root] = 0
+    assert np.array_equal(node_seen, expect_nodes)
+    sets = _leaf_sets(bvh)
+    assert len(sets) == T - 1 and sorted(sets[bvh.binary_root]) == list(range(T))
+    rec = np.asarray(bvh.triangles, np.float32)
+    lo = np.minimum(np.minimum(rec[:, 0:3], rec[:, 3:6]), rec[:, 6:9])
+    hi = np.maximum(np.maximum(rec[:, 0:3], rec[:, 3:6]), rec[:, 6:9])
+    for n, leaves in sets.items():
+        assert int(bvh.binary_first[n]) == leaves[0]
+        box = np.concatenate([lo[leaves].min(axis=0), hi[leaves].max(axis=0)])
+        assert np.array_equal(np.asarray(bvh.binary_bounds[n], np.float32).view(np.uint32), box.view(np.uint32))
+
+
+@pytest.mark.parametrize("radius", [1, 4, 8])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7, 9])
+def test_hploc_equal_boxes_build_the_perfect_tree(orc, k, radius):
+    """Identical boxes make every union area equal, so the packed key's offset and direction
+    parity decide (hploc.cuh:195-241): even lanes take their right neighbour, odd lanes their left
+    one, and a round pairs (0,1), (2,3), ...  Equal Morton codes fall back to positions, whose XOR
+    deltas make the implicit hierarchy the perfect tree on aligned blocks when T = 2^k.  The whole
+    tree is then the perfect tree: the node over [a, a + 2s) has children [a, a + s) and
+    [a + s, a + 2s) (lower lane on the left) and, numbered by its right child's smallest leaf,
+    id a + s - 1."""
+    T = 1 << k
+    bvh = orc.build_bvh4(_equal_triangles(T), method="hploc", radius=radius)
+    assert bvh.binary_root == T // 2 - 1
+    s = 1
+    while s < T:
+        for a in range(0, T, 2 * s):
+            left = (a, 2) if s == 1 else (a + s // 2 - 1, 1)
+            right = (a + s, 2) if s == 1 else (a + s + s // 2 - 1, 1)
+            assert _children(bvh, a + s - 1) == (left, right), (T, a, s)
+        s *= 2
+
+
+def _half_area32(lo, hi):
+    d = np.maximum(hi - lo, np.float32(0))
+    return np.float32(d[0] * d[1]) + np.float32(d[1] * d[2]) + np.float32(d[2] * d[0])
+
+
+def _ploc_by_definition(lo, hi, radius):
+    """PLOC (Meister and Bittner 2018) on clusters in sorted order: each cluster picks the
+    neighbour within `radius` positions whose union box has the smallest half-area; mutual pairs
+    merge into the lower position; survivors keep their order; rounds repeat until one cluster
+    remains.  Returns the merges as (left leaf set, right leaf set) and whether two candidate
+    areas came within 2^-19 of each other (where the device key's dropped low bits could decide)."""
+    clusters = [([i], lo[i].copy(), hi[i].copy()) for i in range(lo.shape[0])]
+    merges, near_tie = [], False
+    while len(clusters) > 1:
+        n = len(clusters)
+        nn = []
+        for i in range(n):
+            cand = []
+            for j in range(max(0, i - radius), min(n, i + radius + 1)):
+                if j != i:
+                    a = _half_area32(np.minimum(clusters[i][1], clusters[j][1]),
+                                     np.maximum(clusters[i][2], clusters[j][2]))
+                    cand.append((float(a), j))
+            cand.sort()
+            if len(cand) > 1 and cand[1][0] - cand[0][0] <= cand[0][0] * 2.0 ** -19:
+                near_tie = True
+            nn.append(cand[0][1])
+        out = []
+        for i in range(n):
+            j = nn[i]
+            if nn[j] == i:
+                if i < j:
+                    merges.append((clusters[i][0], clusters[j][0]))
+                    out.append((clusters[i][0] + clusters[j][0], np.minimum(clusters[i][1], clusters[j][1]),
+                                np.maximum(clusters[i][2], clusters[j][2])))
+            else:
+                out.append(clusters[i])
+        clusters = out
+    return merges, near_tie
+
+
+@pytest.mark.parametrize("radius", [1, 2, 4, 8])
+@pytest.mark.parametrize("T", [2, 3, 7, 12, 16])
+def test_hploc_small_inputs_equal_ploc_by_definition(orc, T, radius):
+    """Up to 16 primitives no partial range reaches the merge threshold, so H-PLOC is PLOC on the
+    whole sorted sequence (hploc.cuh:420-447, 451-476)."""
+    checked = 0
+    for seed in range(6):
+        rng = np.random.default_rng(100 + 17 * T + seed)
+        v = rng.uniform(-1, 1, size=(3 * T, 3)).astype(np.float32)
+        mesh = synth._mesh_from_arrays(v, np.arange(3 * T).reshape(T, 3))
+        bvh = orc.build_bvh4(mesh, method="hploc", radius=radius)
+        rec = np.asarray(bvh.triangles, np.float32)
+        lo = np.minimum(np.minimum(rec[:, 0:3], rec[:, 3:6]), rec[:, 6:9])
+        hi = np.maximum(np.maximum(rec[:, 0:3], rec[:, 3:6]), rec[:, 6:9])
+        merges, near_tie = _ploc_by_definition(lo, hi, radius)
+        if near_tie:
+            continue
+        sets = _leaf_sets(bvh)
+        got = set()
+        for n, leaves in sets.items():
+            (l, lk), (r, rk) = _children(bvh, n)
+            left = [l] if lk == 2 else sets[l]
+            right = [r] if rk == 2 else sets[r]
+            got.add((tuple(sorted(left)), tuple(sorted(right))))
+        assert got == {(tuple(sorted(a)), tuple(sorted(b))) for a, b in merges}
+        checked += 1
+    assert checked >= 3
+
+
+def _sah(bvh, T):
+    areas = [float(_half_area32(np.asarray(b[:3], np.float32), np.asarray(b[3:], np.float32)))
+             for b in bvh.binary_bounds]
+    return sum(areas) / areas[bvh.binary_root]
+
+
+@pytest.mark.parametrize("radius", [1, 4, 8])
+@pytest.mark.parametrize("name", ["displaced", "soup", "torus", "equal513"])
+def test_hploc_tree_is_complete_and_canonical(orc, name, radius):
+    mesh = {
+        "displaced": lambda: synth.displaced_sphere(48, 24),
+        "soup": lambda: synth.open_soup(40, 20, seed=7),
+        "torus": lambda: synth.torus(32, 16),
+        "equal513": lambda: _equal_triangles(513),
+    }[name]()
+    T = mesh.i0.shape[0]
+    bvh = orc.build_bvh4(mesh, method="hploc", radius=radius)
+    _check_complete_binary_tree(bvh, T)
+    helpers.check_bvh4(bvh.nodes, bvh.root_reference, bvh.primitive_indices, bvh.triangles, mesh,
+                       bvh.internal_stack_bound, bvh.packed_stack_bound)
+    helpers.check_collapse_rule(bvh.nodes, bvh.binary_nodes, bvh.binary_bounds, bvh.binary_root)
+    lb = orc.build_bvh4(mesh)
+    assert np.array_equal(bvh.primitive_indices, lb.primitive_indices)
+    assert np.array_equal(bvh.triangles, lb.triangles)
+    if name in ("displaced", "torus") and radius >= 4:
+        # Locally ordered clustering yields a lower surface-area cost than the Morton split tree.
+        assert _sah(bvh, T) < _sah(lb, T)
+
+
+def test_wide_numbering_is_breadth_first_by_first_leaf(orc):
+    """The canonical order: breadth-first, parents in level order, siblings by smallest sorted
+    leaf.  For LBVH, whose subtrees are contiguous leaf ranges, that is (depth, first leaf)."""
+    for method in ("lbvh", "hploc"):
+        mesh = synth.displaced_sphere(48, 24)
+        bvh = orc.build_bvh4(mesh, method=method)
+        nodes = helpers.as_nodes(bvh.nodes)
+        rec = np.asarray(bvh.triangles)
+        first = {}
+        depth = {0: 0}
+
+        def first_leaf(n):
+            if n in first:
+                return first[n]
+            best = None
+            for ch in nodes[n]["children"]:
+                ref = int(ch["reference"])
+                if not ref & helpers.VALID_BIT:
+                    continue
+                f = (ref & helpers.OFFSET_MASK) if (ref >> 48) else first_leaf(ref & helpers.OFFSET_MASK)
+                best = f if best is None else min(best, f)
+            first[n] = best
+            return best
+
+        expect = []
+        queue = [0]
+        while queue:
+            n = queue.pop(0)
+            expect.append(n)
+            kids = [int(ch["reference"]) & helpers.OFFSET_MASK for ch in nodes[n]["children"]
+                    if int(ch["reference"]) & helpers.VALID_BIT and (int(ch["reference"]) >> 48) == 0]
+            queue.extend(sorted(kids, key=first_leaf))
+        assert expect == list(range(len(nodes)))
+        if method == "lbvh":
+            for n in range(len(nodes)):
+                for ch in nodes[n]["children"]:
+                    ref = int(ch["reference"])
+                    if ref & helpers.VALID_BIT and (ref >> 48) == 0:
+                        depth[ref & helpers.OFFSET_MASK] = depth[n] + 1
+            keys = [(depth[n], first_leaf(n)) for n in range(len(nodes))]
+            assert keys == sorted(keys)
+
+
+@pytest.mark.parametrize("count", [15, 16, 17, 31, 32, 33, 63, 64, 65, 129, 513])
+def test_hploc_repeated_equal_bounds_winding(orc, count):
+    """tests/integration_gwn_builder.cu:301-343: identical triangles at counts around the list
+    threshold; every radius builds a complete tree whose fully descended Taylor winding equals the
+    exact winding (2e-5 + 2e-5 |w|)."""
+    mesh = _equal_triangles(count)
+    q = (np.float32([0.2, 0.7, -0.3]), np.float32([0.2, 0.1, 0.3]), np.float32([1, -2, 0.5]))
+    ref = orc.winding_exact_mesh_f64(mesh, *q)
+    for radius in (1, 4, 8):
+        bvh = orc.build_bvh4(mesh, method="hploc", radius=radius)
+        _check_complete_binary_tree(bvh, count)
+        w = orc.winding_taylor_f32(bvh, orc.refit_moment(bvh, 2), 2, *q, accuracy_scale=1e6)
+        assert np.all(np.abs(w - ref) <= 2e-5 + 2e-5 * np.abs(ref))
+
+
+def test_hploc_rejects_bad_radius(orc):
+    mesh = _equal_triangles(4)
+    for radius in (0, 9):
+        with pytest.raises(RuntimeError):
+            orc.build_bvh4(mesh, method="hploc", radius=radius)
+
+
+# ---------------------------------------------------------------------------
+# Adjoint of winding-number and distance losses (FP64 oracle)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("beta", [2.0, 1e6])
+@pytest.mark.parametrize("order", [0, 1])
+def test_taylor_adjoint_matches_central_differences(orc, order, beta):
+    """dL/dV of L = sum_q u_q w_q against FP64 central differences of the same Taylor sum (same
+    accept/descend decisions) on 64 random vertex coordinates; beta = 1e6 descends every leaf, so
+    that case checks the exact solid-angle derivative alone.  Queries close to the surface make the
+    difference quotient converge slowly (at h = 1e-4 one coordinate is still 5e-5 off), so the
+    step is 1e-6 with Richardson extrapolation; FP64 roundoff there is about 1e-8."""
+    mesh = synth.open_soup(12, 6, seed=3)
+    bvh = orc.build_bvh4(mesh)
+    mom = orc.refit_moment(bvh, order)
+    rng = np.random.default_rng(order)
+    qx, qy, qz = synth.uniform_box_points(5, 300, (-1.4,) * 3, (1.4,) * 3)
+    u = rng.normal(size=300)
+    dv, _ = orc.winding_taylor_adjoint_f64(bvh, mom, order, beta, mesh, qx, qy, qz, u)
+    V0 = np.stack([mesh.vx, mesh.vy, mesh.vz]).astype(np.float64)
+
+    def central(k, vi, h):
+        Vp, Vm = V0.copy(), V0.copy()
+        Vp[k, vi] += h
+        Vm[k, vi] -= h
+        return (orc.winding_taylor_frozen_f64(bvh, mom, order, beta, mesh, qx, qy, qz, u, Vp) -
+                orc.winding_taylor_frozen_f64(bvh, mom, order, beta, mesh, qx, qy, qz, u, Vm)) / (2 * h)
+
+    h = 1e-6
+    for flat in rng.choice(V0.size, 64, replace=False):
+        k, vi = divmod(int(flat), V0.shape[1])
+        # Richardson extrapolation of the h and h/2 central differences cancels the h^2 term.
+        fd = (4 * central(k, vi, h / 2) - central(k, vi, h)) / 3
+        assert abs(fd - dv[k, vi]) <= 1e-6 * (1 + abs(fd)), (k, vi, fd, dv[k, vi])
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_taylor_adjoint_query_gradient_is_the_winding_gradient(orc, order):
+    """dL/dq = u_q * grad_q w_q: the adjoint's query part equals the Taylor gradient replay."""
+    mesh = synth.open_soup(12, 6, seed=3)
+    bvh = orc.build_bvh4(mesh)
+    mom = orc.refit_moment(bvh, order)
+    qx, qy, qz = synth.uniform_box_points(6, 400, (-1.4,) * 3, (1.4,) * 3)
+    u = np.random.default_rng(7).normal(size=400)
+    _, dq = orc.winding_taylor_adjoint_f64(bvh, mom, order, 2.0, mesh, qx, qy, qz, u)
+    _, gx, gy, gz = orc.winding_gradient_taylor_f32(bvh, mom, order, qx, qy, qz, accuracy_scale=2.0)
+    g = np.stack([gx, gy, gz], 1).astype(np.float64) * u[:, None]
+    assert np.abs(dq - g).max() <= 1e-5 * (1 + np.abs(g).max())
+
+
+def test_taylor_adjoint_translation_and_closed_mesh_invariants(orc):
+    """Moving the mesh and the queries together leaves w unchanged: sum of dL/dV + sum of dL/dq = 0
+    per axis; and the exact winding of a closed mesh is locally constant away from its surface, so
+    with every leaf descended dL/dV vanishes for queries far from it."""
+    mesh = synth.open_soup(12, 6, seed=4)
+    bvh = orc.build_bvh4(mesh)
+    mom = orc.refit_moment(bvh, 1)
+    qx, qy, qz = synth.uniform_box_points(8, 300, (-1.4,) * 3, (1.4,) * 3)
+    u = np.random.default_rng(2).normal(size=300)
+    dv, dq = orc.winding_taylor_adjoint_f64(bvh, mom, 1, 2.0, mesh, qx, qy, qz, u)
+    assert np.abs(dv.sum(axis=1) + dq.sum(axis=0)).max() <= 1e-9 * (1 + np.abs(dv).sum())
+    closed = synth.icosphere(2)
+    cb = orc.build_bvh4(closed)
+    far = synth.uniform_box_points(9, 200, (2.0,) * 3, (3.0,) * 3)
+    dv2, _ = orc.winding_taylor_adjoint_f64(cb, orc.refit_moment(cb, 1), 1, 1e6, closed, *far, np.ones(200))
+    assert np.abs(dv2).max() <= 1e-10
+
+
+def test_distance_adjoint_matches_central_differences(orc):
+    """L = sum_p u_p d_p: dL/dV against FP64 central differences of brute-force distances, dL/dp
+    is u_p times a unit vector, and the distances equal the traversal oracle's."""
+    mesh = synth.open_soup(12, 6, seed=3)
+    rng = np.random.default_rng(1)
+    px, py, pz = synth.uniform_box_points(7, 400, (-1.4,) * 3, (1.4,) * 3)
+    u = rng.normal(size=400)
+    dist, dv, dp = orc.distance_adjoint_f64(mesh, px, py, pz, u)
+    V0 = np.stack([mesh.vx, mesh.vy, mesh.vz]).astype(np.float64)
+    h = 1e-6
+    for flat in rng.choice(V0.size, 64, replace=False):
+        k, vi = divmod(int(flat), V0.shape[1])
+        Vp, Vm = V0.copy(), V0.copy()
+        Vp[k, vi] += h
+        Vm[k, vi] -= h
+        fd = (np.dot(u, orc.distance_adjoint_f64(mesh, px, py, pz, u, vertices=Vp)[0]) -
+              np.dot(u, orc.distance_adjoint_f64(mesh, px, py, pz, u, vertices=Vm)[0])) / (2 * h)
+        assert abs(fd - dv[k, vi]) <= 1e-6 * (1 + abs(fd))
+    assert np.allclose(np.linalg.norm(dp / u[:, None], axis=1), 1.0, atol=1e-12)
+    ref = orc.unsigned_distance_f32(orc.build_bvh4(mesh), px, py, pz)
+    assert np.abs(dist - ref).max() <= 1e-6
