@@ -57,6 +57,8 @@ def parse_args():
     p.add_argument("--exact-queries", type=int, default=32_768, help="subsample timed for the exact form")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline duration")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation workload")
+    p.add_argument("--adjoint-steps", type=int, default=5)
     return p.parse_args()
 
 
@@ -186,10 +188,16 @@ def main() -> int:
         return (time.perf_counter() - t) / reps
 
     build_s = timed(lambda: gwn.gwn_build_bvh(geom, bvh, opts).raise_if_error(), 5)
+    hploc_bvh = gwn.BvhObject()
+    hploc_opts = gwn.BuildOptions(method=gwn.BuildMethod.HPLOC, hploc_search_radius=8)
+    hploc_s = timed(lambda: gwn.gwn_build_bvh(geom, hploc_bvh, hploc_opts).raise_if_error(), 5)
+    hploc_info = hploc_bvh.info()
+    del hploc_bvh
     mom = gwn.MomentObject(args.order)
     refit_s = timed(lambda: gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error(), 5)
     T = mesh.triangle_count
-    log(f"build {build_s * 1e3:.2f} ms ({T / build_s / 1e6:.1f} Mtri/s), refit o{args.order} {refit_s * 1e3:.2f} ms")
+    log(f"build {build_s * 1e3:.2f} ms ({T / build_s / 1e6:.1f} Mtri/s), H-PLOC {hploc_s * 1e3:.2f} ms "
+        f"({T / hploc_s / 1e6:.1f} Mtri/s), refit o{args.order} {refit_s * 1e3:.2f} ms")
 
     result = {
         "metric": METRIC,
@@ -222,7 +230,10 @@ def main() -> int:
         "stages_ms": {"query_presort": round(t_order, 4), "traversal": round(t_trav, 4)},
         "build": {"method": "lbvh", "ms": build_s * 1e3, "mtri_per_s": T / build_s / 1e6,
                   "refit_moment_ms": refit_s * 1e3, "refit_mtri_per_s": T / refit_s / 1e6,
-                  "nodes": info["node_count"], "internal_stack_bound": info["internal_stack_bound"]},
+                  "nodes": info["node_count"], "internal_stack_bound": info["internal_stack_bound"],
+                  "hploc_r8_ms": hploc_s * 1e3, "hploc_r8_mtri_per_s": T / hploc_s / 1e6,
+                  "hploc_nodes": hploc_info["node_count"],
+                  "hploc_internal_stack_bound": hploc_info["internal_stack_bound"]},
         "work": {"child_tests": tests, "accepted": accepts, "near_field_triangles": leaves},
         "roofline": {
             "bound": "valu",
@@ -238,11 +249,63 @@ def main() -> int:
         },
     }
 
+    if not args.no_adjoint:
+        del d, out, order, cnt, sub, ref, esub, eout
+        result["adjoint"] = adjoint_bench(args, dev, stream)
+
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(mesh, qx, qy, qz, args)
 
     print(json.dumps(result), flush=True)
     return 0
+
+
+def adjoint_bench(args, dev, stream) -> dict:
+    """Differentiation workload: 131,072-triangle displaced sphere, 1,048,576 labelled queries
+    (inside a radius-0.9 sphere or not), 262,144 target points on that sphere.  One step: new
+    vertex positions (a gradient step), BVH bounds and order-1 moment refit, Taylor winding and
+    distance forward, loss, adjoint backward to dL/dV."""
+    import torch
+
+    from smallgwn_amd import autograd as ag
+    from smallgwn_amd import gwn, synth
+
+    mesh = synth.displaced_sphere(256, 256)
+    faces = np.stack([mesh.i0, mesh.i1, mesh.i2], 1)
+    verts = torch.tensor(np.stack([mesh.vx, mesh.vy, mesh.vz], 1), device=dev, requires_grad=True)
+    scene = ag.MeshScene(faces, verts, order=1, method=gwn.BuildMethod.HPLOC, accuracy_scale=2.0)
+    qx, qy, qz = synth.uniform_box_points(21, 1 << 20, (-1.5,) * 3, (1.5,) * 3)
+    queries = torch.tensor(np.stack([qx, qy, qz], 1), device=dev)
+    labels = (queries.norm(dim=1) < 0.9).to(torch.float32)
+    tx, ty, tz = synth.uniform_box_points(22, 1 << 18, (-1.0,) * 3, (1.0,) * 3)
+    targets = torch.tensor(np.stack([tx, ty, tz], 1), device=dev)
+    targets = 0.9 * targets / targets.norm(dim=1, keepdim=True)
+    opt = torch.optim.SGD([verts], lr=1e-3)
+
+    def step():
+        opt.zero_grad(set_to_none=True)
+        w = ag.taylor_winding(verts, queries, scene)
+        dist = ag.unsigned_distance(verts, targets, scene)
+        loss = ((torch.sigmoid(20.0 * (w - 0.5)) - labels) ** 2).mean() + dist.mean()
+        loss.backward()
+        opt.step()
+        return loss
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(stream)
+    for _ in range(args.adjoint_steps):
+        loss = step()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / args.adjoint_steps
+    log(f"adjoint: {ms:.2f} ms/step ({1e3 / ms:.1f} steps/s), loss {float(loss):.5f}")
+    return {"steps_per_s": 1e3 / ms, "ms_per_step": ms, "triangles": mesh.triangle_count,
+            "queries": int(queries.shape[0]), "targets": int(targets.shape[0]), "taylor_order": 1,
+            "builder": "hploc_r8", "step": "update + bounds/moment refit + winding and distance forward "
+                                           "+ adjoint backward + SGD update"}
 
 
 def traffic_bytes():

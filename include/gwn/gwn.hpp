@@ -146,6 +146,25 @@ public:
     gwn_bvh_moment_object() { gwn_moment_create(&h_, Order); }
 };
 
+// gwn_boundary_chain_object<Index> (include/gwn/gwn_boundary.cuh:56-125).
+template <class Index = std::uint32_t>
+class gwn_boundary_chain_object final : public detail::gwn_handle_owner<gwn_boundary_t, gwn_boundary_destroy> {
+    static_assert(sizeof(Index) == 4 && Index(-1) > Index(0), "libgwn_hip provides Index = uint32_t");
+
+public:
+    gwn_boundary_chain_object() { gwn_boundary_create(&h_); }
+    [[nodiscard]] std::size_t edge_count() const noexcept {
+        std::uint64_t e = 0;
+        gwn_boundary_info(h_, &e, nullptr, nullptr, nullptr);
+        return e;
+    }
+    [[nodiscard]] bool is_built() const noexcept {
+        std::int32_t b = 0;
+        gwn_boundary_info(h_, nullptr, nullptr, nullptr, &b);
+        return b != 0;
+    }
+};
+
 template <class Real = float, class Index = std::uint32_t> using gwn_bvh4_object = gwn_bvh_object<4, Real, Index>;
 template <int Order, class Real = float, class Index = std::uint32_t>
 using gwn_bvh4_moment_object = gwn_bvh_moment_object<4, Order, Real, Index>;
@@ -334,6 +353,56 @@ template <gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, c
     return detail::gwn_from_code(gwn_compute_ray_first_hit_batch_f32(
         bvh.get(), ray_origin_x.data(), ray_origin_y.data(), ray_origin_z.data(), ray_dir_x.data(), ray_dir_y.data(),
         ray_dir_z.data(), reinterpret_cast<::gwn_ray_first_hit_result *>(output.data()), n, t_min, t_max, &c, stream
+    ));
+}
+
+// ---- boundary chain and antipodal batches (include/gwn/gwn_boundary.cuh:103-125,
+//      include/gwn/gwn_query.cuh:492-640) ----------------------------------------------------
+
+template <class Real, class Index>
+[[nodiscard]] gwn_status gwn_build_boundary_chain(
+    gwn_geometry_object<Real, Index> const &geometry, gwn_boundary_chain_object<Index> &chain,
+    hipStream_t stream = gwn_default_stream()
+) noexcept {
+    return detail::gwn_from_code(::gwn_build_boundary_chain(geometry.get(), chain.get(), stream));
+}
+
+template <gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, class Real, class Index = std::uint32_t>
+    requires(Config.block_size > 0 && Config.block_size <= 1024 && Config.stack_capacity > 0)
+[[nodiscard]] gwn_status gwn_compute_winding_number_antipodal_batch(
+    gwn_geometry_object<Real, Index> const &geometry, gwn_bvh_object<Width, Real, Index> const &bvh,
+    gwn_boundary_chain_object<Index> const &chain, gwn_device_span<Real const> query_x,
+    gwn_device_span<Real const> query_y, gwn_device_span<Real const> query_z, gwn_device_span<Real> output,
+    hipStream_t stream = gwn_default_stream()
+) noexcept {
+    if (query_y.size() != query_x.size() || query_z.size() != query_x.size())
+        return gwn_status(gwn_error::invalid_argument, "Query SoA spans require identical lengths.");
+    if (output.size() != query_x.size())
+        return gwn_status(gwn_error::invalid_argument, "Output span size differs from query count.");
+    constexpr ::gwn_query_config c = detail::gwn_c_config<Config>();
+    return detail::gwn_from_code(gwn_compute_winding_number_antipodal_batch_f32(
+        geometry.get(), bvh.get(), chain.get(), query_x.data(), query_y.data(), query_z.data(), output.data(),
+        output.size(), &c, stream
+    ));
+}
+
+template <gwn_query_batch_config Config = gwn_query_batch_config{}, class Real, class Index = std::uint32_t>
+    requires(Config.block_size > 0 && Config.block_size <= 1024)
+[[nodiscard]] gwn_status gwn_compute_winding_gradient_antipodal_batch(
+    gwn_geometry_object<Real, Index> const &geometry, gwn_boundary_chain_object<Index> const &chain,
+    gwn_device_span<Real const> query_x, gwn_device_span<Real const> query_y, gwn_device_span<Real const> query_z,
+    gwn_device_span<Real> output_x, gwn_device_span<Real> output_y, gwn_device_span<Real> output_z,
+    hipStream_t stream = gwn_default_stream()
+) noexcept {
+    std::size_t const n = query_x.size();
+    if (query_y.size() != n || query_z.size() != n)
+        return gwn_status(gwn_error::invalid_argument, "Query SoA spans require identical lengths.");
+    if (output_x.size() != n || output_y.size() != n || output_z.size() != n)
+        return gwn_status(gwn_error::invalid_argument, "Output span sizes differ from query count.");
+    constexpr ::gwn_query_config c = detail::gwn_c_config<Config>();
+    return detail::gwn_from_code(gwn_compute_winding_gradient_antipodal_batch_f32(
+        geometry.get(), chain.get(), query_x.data(), query_y.data(), query_z.data(), output_x.data(),
+        output_y.data(), output_z.data(), n, &c, stream
     ));
 }
 

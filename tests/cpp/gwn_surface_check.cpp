@@ -62,6 +62,18 @@ int main() {
     st = gwn::gwn_refit_bvh_moment(moved, moment);
     expect(st.error() == gwn::gwn_error::invalid_argument, "moment refit of empty BVH");
 
+    // Boundary chain and antipodal batches (tests/unit_gwn_query_antipodal.cu:577-640): an
+    // unbuilt chain and an empty geometry are rejected before any launch.
+    gwn::gwn_boundary_chain_object<Index> chain;
+    expect(!chain.is_built() && chain.edge_count() == 0, "new chain is unbuilt and empty");
+    st = gwn::gwn_compute_winding_number_antipodal_batch(geometry, moved, chain, empty_in, empty_in, empty_in,
+                                                         empty_out);
+    expect(st.error() == gwn::gwn_error::invalid_argument, "antipodal batch on empty BVH");
+    st = gwn::gwn_compute_winding_gradient_antipodal_batch(geometry, chain, empty_in, empty_in, empty_in, empty_out,
+                                                           empty_out, empty_out);
+    expect(st.error() == gwn::gwn_error::invalid_argument, "antipodal gradient with an unbuilt chain");
+    expect(st.message().find("unbuilt") != std::string::npos, "antipodal gradient names the unbuilt chain");
+
     if (failures == 0)
         std::printf("gwn.hpp surface: all checks passed\n");
     return failures == 0 ? 0 : 1;

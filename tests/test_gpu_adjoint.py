@@ -1,0 +1,120 @@
+"""GPU adjoint batches against the FP64 adjoint oracle (which central differences pin).
+
+The device re-traverses with the forward's decisions and accumulates FP32 terms with atomics, so
+it matches the FP64 oracle to FP32 rounding of sums: vertex gradients within 1e-4 of the largest
+vertex gradient magnitude, query gradients within 1e-4 relative.
+"""
+
+import numpy as np
+import pytest
+
+from smallgwn_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dev):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _objects(gwn, mesh, order):
+    geom = gwn.GeometryObject()
+    gwn.gwn_upload_geometry(geom, mesh.vx, mesh.vy, mesh.vz, mesh.i0, mesh.i1, mesh.i2).raise_if_error()
+    bvh = gwn.BvhObject()
+    gwn.gwn_build_bvh(geom, bvh, gwn.BuildOptions(method=gwn.BuildMethod.LBVH)).raise_if_error()
+    mom = gwn.MomentObject(order)
+    gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+    return geom, bvh, mom
+
+
+def _winding_adjoint(gwn, dev, geom, bvh, mom, q, u, beta):
+    import torch
+
+    V = geom.vertex_count()
+    dv = [torch.zeros(V, dtype=torch.float32, device=dev) for _ in range(3)]
+    dq = [torch.zeros(q[0].shape[0], dtype=torch.float32, device=dev) for _ in range(3)]
+    gwn.gwn_compute_winding_taylor_adjoint_batch(geom, bvh, mom, *[_t(a, dev) for a in q], _t(u, dev), *dv, *dq,
+                                                 accuracy_scale=beta).raise_if_error()
+    torch.cuda.synchronize()
+    return np.stack([a.cpu().numpy() for a in dv]), np.stack([a.cpu().numpy() for a in dq], 1)
+
+
+@pytest.mark.parametrize("beta", [2.0, 1e6])
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("name", ["open_soup", "displaced"])
+def test_winding_adjoint_matches_oracle(gwn, orc, dev, name, order, beta):
+    mesh = synth.open_soup(24, 12, seed=3) if name == "open_soup" else synth.displaced_sphere(48, 24)
+    print(f"[adjoint] winding {name} order={order} beta={beta}", flush=True)
+    geom, bvh, mom = _objects(gwn, mesh, order)
+    q = synth.mixed_queries(mesh, 4096, seed=9, band=0.02)
+    u = np.random.default_rng(order).normal(size=4096).astype(np.float32)
+    dv, dq = _winding_adjoint(gwn, dev, geom, bvh, mom, q, u, beta)
+    rb = orc.build_bvh4(mesh)
+    rdv, rdq = orc.winding_taylor_adjoint_f64(rb, orc.refit_moment(rb, order), order, beta, mesh, *q, u)
+    scale = np.abs(rdv).max()
+    assert np.abs(dv - rdv).max() <= 1e-4 * scale, f"max |dV - ref| {np.abs(dv - rdv).max():.3e} of {scale:.3e}"
+    assert np.all(np.abs(dq - rdq) <= 1e-4 * (1 + np.abs(rdq)))
+
+
+def test_winding_adjoint_large_batch_and_accumulation(gwn, orc, dev):
+    """Above 65,536 queries the batch visits them in Morton order; two calls add their gradients."""
+    import torch
+
+    mesh = synth.displaced_sphere(64, 32)
+    geom, bvh, mom = _objects(gwn, mesh, 1)
+    q = synth.mixed_queries(mesh, 70000, seed=4, band=0.02)
+    u = np.random.default_rng(5).normal(size=70000).astype(np.float32)
+    V = geom.vertex_count()
+    dv = [torch.zeros(V, dtype=torch.float32, device=dev) for _ in range(3)]
+    dq = [torch.zeros(70000, dtype=torch.float32, device=dev) for _ in range(3)]
+    args = [_t(a, dev) for a in q] + [_t(u, dev)]
+    for _ in range(2):
+        gwn.gwn_compute_winding_taylor_adjoint_batch(geom, bvh, mom, *args, *dv, *dq).raise_if_error()
+    torch.cuda.synchronize()
+    got = np.stack([a.cpu().numpy() for a in dv])
+    rb = orc.build_bvh4(mesh)
+    rdv, rdq = orc.winding_taylor_adjoint_f64(rb, orc.refit_moment(rb, 1), 1, 2.0, mesh, *q, u)
+    assert np.abs(got - 2 * rdv).max() <= 2e-4 * np.abs(rdv).max()
+    assert np.all(np.abs(np.stack([a.cpu().numpy() for a in dq], 1) - rdq) <= 1e-4 * (1 + np.abs(rdq)))
+
+
+def test_distance_adjoint_matches_oracle(gwn, orc, dev):
+    import torch
+
+    mesh = synth.open_soup(24, 12, seed=3)
+    geom, bvh, _ = _objects(gwn, mesh, 0)
+    p = synth.uniform_box_points(11, 4096, (-1.4,) * 3, (1.4,) * 3)
+    u = np.random.default_rng(3).normal(size=4096).astype(np.float32)
+    V = geom.vertex_count()
+    dv = [torch.zeros(V, dtype=torch.float32, device=dev) for _ in range(3)]
+    dp = [torch.zeros(4096, dtype=torch.float32, device=dev) for _ in range(3)]
+    gwn.gwn_compute_unsigned_distance_adjoint_batch(geom, bvh, *[_t(a, dev) for a in p], _t(u, dev), *dv,
+                                                    *dp).raise_if_error()
+    torch.cuda.synchronize()
+    _, rdv, rdp = orc.distance_adjoint_f64(mesh, *p, u)
+    got = np.stack([a.cpu().numpy() for a in dv])
+    assert np.abs(got - rdv).max() <= 1e-4 * np.abs(rdv).max()
+    assert np.abs(np.stack([a.cpu().numpy() for a in dp], 1) - rdp).max() <= 1e-4 * (1 + np.abs(rdp).max())
+
+
+def test_adjoint_leaf_root_and_validation(gwn, orc, dev):
+    import torch
+
+    one = synth._mesh_from_arrays(np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0]]), [[0, 1, 2]])
+    geom, bvh, mom = _objects(gwn, one, 1)
+    q = (np.float32([0.2, -1.0, 0.3]), np.float32([0.3, 0.5, 0.1]), np.float32([0.5, -2.0, 0.7]))
+    u = np.float32([1.0, -0.5, 2.0])
+    dv, dq = _winding_adjoint(gwn, dev, geom, bvh, mom, q, u, 2.0)
+    rb = orc.build_bvh4(one)
+    rdv, rdq = orc.winding_taylor_adjoint_f64(rb, orc.refit_moment(rb, 1), 1, 2.0, one, *q, u.astype(np.float64))
+    assert np.abs(dv - rdv).max() <= 1e-4 * np.abs(rdv).max()
+    # Order 2 moments are rejected; vertex gradient spans must match the vertex count.
+    geom2, bvh2, mom2 = _objects(gwn, one, 2)
+    z = [torch.zeros(3, dtype=torch.float32, device=dev) for _ in range(3)]
+    st = gwn.gwn_compute_winding_taylor_adjoint_batch(geom2, bvh2, mom2, *[_t(a, dev) for a in q], _t(u, dev), *z)
+    assert st.error == gwn.Error.INVALID_ARGUMENT and "orders 0 and 1" in st.message
+    st = gwn.gwn_compute_winding_taylor_adjoint_batch(geom, bvh, mom, *[_t(a, dev) for a in q], _t(u, dev),
+                                                      z[0], z[1], z[2][:2])
+    assert st.error == gwn.Error.INVALID_ARGUMENT
