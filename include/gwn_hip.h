@@ -58,6 +58,7 @@ typedef struct gwn_geometry_handle *gwn_geometry_t;
 typedef struct gwn_bvh_handle *gwn_bvh_t;
 typedef struct gwn_moment_handle *gwn_moment_t;
 typedef struct gwn_boundary_handle *gwn_boundary_t;
+typedef struct gwn_batch_handle *gwn_batch_t;
 
 /* Reference gwn_query_batch_config (include/gwn/gwn_query.cuh:41-61): block size 256 and
  * stack capacity 64 by default.  sort_queries: -1 auto, 0 off, 1 on (queries are traversed in
@@ -270,6 +271,47 @@ int gwn_compute_unsigned_distance_adjoint_batch_f32(
     gwn_geometry_t geometry, gwn_bvh_t bvh, const float *px, const float *py, const float *pz,
     const float *grad_d, uint64_t count, float culling_band, float *grad_vx, float *grad_vy, float *grad_vz,
     float *grad_px, float *grad_py, float *grad_pz, const gwn_query_config *config, hipStream_t stream);
+
+/* ---- batched mode (no reference counterpart; the reference runs one mesh per call).  A batch
+ *      fixes, for each mesh, its BVH and moment objects (all of one Taylor order) and its query
+ *      and output device pointers.  gwn_batch_create allocates everything a replay needs and
+ *      refits the moments once (synchronizes).  gwn_batch_refit_moments enqueues every mesh's
+ *      moment refit and gwn_batch_query one persistent Taylor launch over all meshes' queries
+ *      (plain precision); neither allocates nor synchronizes, so both may be captured:
+ *      gwn_batch_capture records [refits +] counter reset + query launch into a hipGraph on
+ *      the batch's own stream, gwn_batch_replay launches it.  Results equal the single-mesh
+ *      Taylor batch's bit for bit.  gwn_batch_tune times candidate launch shapes and keeps the
+ *      fastest (synchronizes; drops a captured graph).  The objects and pointers must outlive
+ *      the batch; replays of one batch must not overlap. ---- */
+typedef struct gwn_batch_item {
+    gwn_bvh_t bvh;
+    gwn_moment_t moment;
+    const float *qx, *qy, *qz;
+    float *out;
+    uint64_t count;
+} gwn_batch_item;
+
+typedef struct gwn_batch_tuning {
+    int32_t block_size;      /* 64, 128, 192 or 256 threads */
+    int32_t chunk;           /* queries per work item, a multiple of block_size */
+    int32_t blocks_per_cu;   /* persistent grid = blocks_per_cu x compute units */
+    int32_t traversal;       /* 0: one query per lane; 1: wave-cooperative over each mesh's
+                                Morton query order (computed at create) */
+    int32_t stack_capacity;  /* compiled traversal stack (read only) */
+    uint64_t meshes;         /* read only */
+    uint64_t queries;        /* read only */
+} gwn_batch_tuning;
+
+int gwn_batch_create(const gwn_batch_item *items, uint64_t item_count, float accuracy_scale, gwn_batch_t *out,
+                     hipStream_t stream);
+int gwn_batch_destroy(gwn_batch_t batch);
+int gwn_batch_refit_moments(gwn_batch_t batch, hipStream_t stream);
+int gwn_batch_query(gwn_batch_t batch, hipStream_t stream);
+int gwn_batch_set_tuning(gwn_batch_t batch, const gwn_batch_tuning *tuning, hipStream_t stream);
+int gwn_batch_get_tuning(gwn_batch_t batch, gwn_batch_tuning *tuning);
+int gwn_batch_tune(gwn_batch_t batch, hipStream_t stream, float *best_ms);
+int gwn_batch_capture(gwn_batch_t batch, int32_t include_refit);
+int gwn_batch_replay(gwn_batch_t batch, hipStream_t stream);
 
 /* Morton-order permutation of query points within the BVH root bounds (30-bit keys, stable
  * radix sort): order[i] is the input index of the i-th query to visit.  Reusable across

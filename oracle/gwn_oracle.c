@@ -2642,9 +2642,6 @@ static void *adj_thread(void *arg)
 /* Runs the replay over all queries on `threads` threads; per-thread slot/record buffers summed. */
 static void adj_run(adj_ctx *c, double *dslot, double *dtri)
 {
-    uint64_t n = 0;
-    while (c->qx && n == 0)
-        break;
     int threads = c->threads < 1 ? 1 : c->threads;
     uint64_t Q = (uint64_t)c->value; /* query count passed through value */
     c->value = 0.0;
@@ -2661,7 +2658,6 @@ static void adj_run(adj_ctx *c, double *dslot, double *dtri)
         jobs[t].dtri = dtri ? (double *)calloc(nr, sizeof(double)) : NULL;
         pthread_create(&tid[t], NULL, adj_thread, &jobs[t]);
     }
-    (void)n;
     for (int t = 0; t < threads; ++t) {
         pthread_join(tid[t], NULL);
         c->value += jobs[t].value;
@@ -2856,6 +2852,90 @@ int gwn_oracle_winding_taylor_adjoint_f64(
     }
     free(slot); free(dslot); free(dtri); free(dpay); free(dtri_pay);
     return GWN_ORACLE_OK;
+}
+
+/* Magnitude sums of the winding adjoint's terms, for FP32 error envelopes: over every triangle t
+ * and query q (no BVH), mag_dv[k][v] = sum |u_q| |d(W_t(q)/4pi)/dv_k| over the triangles t at v and
+ * mag_dq[q][k] = |u_q| sum_t |d(W_t(q)/4pi)/dq_k|.  A sum of FP32 terms of these sizes carries a
+ * rounding error of a few FP32 ulps of the magnitude sum however much the terms cancel. */
+typedef struct {
+    const double *vx, *vy, *vz;
+    const uint32_t *i0, *i1, *i2;
+    uint64_t nt;
+    const float *qx, *qy, *qz;
+    const double *u;
+    uint64_t nq;
+    double *tri9;   /* per triangle: 3 vertices x 3 components */
+    double *dq;     /* per query: 3 components */
+} adj_mag_ctx;
+
+static void adj_mag_vertices(const adj_mag_ctx *c, uint64_t t, double v[3][3])
+{
+    uint32_t ids[3] = {c->i0[t], c->i1[t], c->i2[t]};
+    for (int k = 0; k < 3; ++k) {
+        v[k][0] = c->vx[ids[k]];
+        v[k][1] = c->vy[ids[k]];
+        v[k][2] = c->vz[ids[k]];
+    }
+}
+
+static void adj_mag_tri_range(void *vctx, uint64_t begin, uint64_t end)
+{
+    const adj_mag_ctx *c = (const adj_mag_ctx *)vctx;
+    double const inv4pi = 1.0 / (4.0 * 3.141592653589793238462643383279502884);
+    for (uint64_t t = begin; t < end; ++t) {
+        double v[3][3], d[3][3];
+        adj_mag_vertices(c, t, v);
+        double *o = c->tri9 + 9 * t;
+        for (uint64_t q = 0; q < c->nq; ++q) {
+            double p[3] = {c->qx[q], c->qy[q], c->qz[q]};
+            adj_solid_angle(v[0], v[1], v[2], p, d[0], d[1], d[2]);
+            double w = fabs(c->u[q]) * inv4pi;
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k)
+                    o[3 * i + k] += w * fabs(d[i][k]);
+        }
+    }
+}
+
+static void adj_mag_query_range(void *vctx, uint64_t begin, uint64_t end)
+{
+    const adj_mag_ctx *c = (const adj_mag_ctx *)vctx;
+    double const inv4pi = 1.0 / (4.0 * 3.141592653589793238462643383279502884);
+    for (uint64_t q = begin; q < end; ++q) {
+        double p[3] = {c->qx[q], c->qy[q], c->qz[q]};
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (uint64_t t = 0; t < c->nt; ++t) {
+            double v[3][3], d[3][3];
+            adj_mag_vertices(c, t, v);
+            adj_solid_angle(v[0], v[1], v[2], p, d[0], d[1], d[2]);
+            /* Translation invariance: dW/dq = -(dW/da + dW/db + dW/dc). */
+            for (int k = 0; k < 3; ++k)
+                acc[k] += fabs(d[0][k] + d[1][k] + d[2][k]);
+        }
+        for (int k = 0; k < 3; ++k)
+            c->dq[3 * q + k] = fabs(c->u[q]) * inv4pi * acc[k];
+    }
+}
+
+void gwn_oracle_adjoint_magnitude_f64(
+    const double *vertices, uint64_t vertex_count, const uint32_t *i0, const uint32_t *i1, const uint32_t *i2,
+    uint64_t triangle_count, const float *qx, const float *qy, const float *qz, const double *u, uint64_t count,
+    double *mag_dv, double *mag_dq, int threads)
+{
+    double *tri9 = (double *)calloc(9 * (size_t)(triangle_count ? triangle_count : 1), sizeof(double));
+    adj_mag_ctx c = {vertices, vertices + vertex_count, vertices + 2 * vertex_count, i0, i1, i2, triangle_count,
+                     qx, qy, qz, u, count, tri9, mag_dq};
+    parallel_ranges(adj_mag_tri_range, &c, triangle_count, threads);
+    parallel_ranges(adj_mag_query_range, &c, count, threads);
+    memset(mag_dv, 0, sizeof(double) * 3 * vertex_count);
+    for (uint64_t t = 0; t < triangle_count; ++t) {
+        uint32_t ids[3] = {i0[t], i1[t], i2[t]};
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k)
+                mag_dv[(size_t)k * vertex_count + ids[i]] += tri9[9 * t + 3 * i + k];
+    }
+    free(tri9);
 }
 
 /* Distance adjoint by brute force in FP64: each point's closest point c on its nearest triangle,

@@ -254,6 +254,33 @@ void gwn_oracle_winding_antipodal_f32(
     uint64_t boundary_edges, const float *qx, const float *qy, const float *qz, uint64_t query_count,
     int stack_capacity, int near_fp64, float *out, float *gx, float *gy, float *gz, int threads);
 
+/*
+ * Adjoints (no reference counterpart).  vertices: FP64 (3, V) positions; the BVH, FP32 records and
+ * moments give the forward's accept/descend decisions.  L = sum_q u_q w_q (Taylor orders 0-1):
+ * the frozen-decision value in FP64, and dL/dV (3, V) and dL/dq (Q, 3).  Distance: brute-force
+ * dist, dL/dV and dL/dp for L = sum_p u_p d_p.  Magnitude: per-vertex and per-query sums of the
+ * absolute values of every triangle's exact term (error envelopes for FP32 sums).
+ */
+double gwn_oracle_winding_taylor_frozen_f64(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root, const float *triangles,
+    uint64_t triangle_count, const float *moments, int order, float beta, const uint32_t *primitive_indices,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, const double *vertices, uint64_t vertex_count,
+    const float *qx, const float *qy, const float *qz, const double *u, uint64_t query_count, int threads);
+int gwn_oracle_winding_taylor_adjoint_f64(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root, const float *triangles,
+    uint64_t triangle_count, const float *moments, int order, float beta, const uint32_t *primitive_indices,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, const double *vertices, uint64_t vertex_count,
+    const float *qx, const float *qy, const float *qz, const double *u, uint64_t query_count, double *d_vertices,
+    double *d_queries, int threads);
+void gwn_oracle_distance_adjoint_f64(
+    const double *vertices, uint64_t vertex_count, const uint32_t *i0, const uint32_t *i1, const uint32_t *i2,
+    uint64_t triangle_count, const float *px, const float *py, const float *pz, const double *u, uint64_t count,
+    double *dist, double *d_vertices, double *d_points, int threads);
+void gwn_oracle_adjoint_magnitude_f64(
+    const double *vertices, uint64_t vertex_count, const uint32_t *i0, const uint32_t *i1, const uint32_t *i2,
+    uint64_t triangle_count, const float *qx, const float *qy, const float *qz, const double *u, uint64_t count,
+    double *mag_dv, double *mag_dq, int threads);
+
 /* Squared point-triangle distance (gwn_query_geometry_impl.cuh:46-97). */
 float gwn_oracle_point_triangle_distance2_f32(const float *p, const float *a, const float *b, const float *c);
 

@@ -81,6 +81,7 @@ def lib():
         L.gwn_oracle_winding_taylor_adjoint_f64.argtypes = [
             vp, u64, _Child, vp, u64, vp, i32, f32, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp, u64, vp, vp, i32]
         L.gwn_oracle_distance_adjoint_f64.argtypes = [vp, u64, vp, vp, vp, u64, vp, vp, vp, vp, u64, vp, vp, vp, i32]
+        L.gwn_oracle_adjoint_magnitude_f64.argtypes = [vp, u64, vp, vp, vp, u64, vp, vp, vp, vp, u64, vp, vp, i32]
         L.gwn_oracle_build_bvh4_ex_f32.restype = i32
         L.gwn_oracle_build_bvh4_ex_f32.argtypes = [vp, vp, vp, u64, vp, vp, vp, u64, i32, ctypes.c_uint32,
                                                    ctypes.POINTER(_Bvh)]
@@ -526,6 +527,23 @@ def winding_taylor_adjoint_f64(bvh, moments, order, beta, mesh, qx, qy, qz, u, v
     if st != 0:
         raise RuntimeError(f"oracle adjoint failed with status {st}")
     return dv, dq
+
+
+def adjoint_magnitude_f64(mesh, qx, qy, qz, u, vertices=None, threads=None):
+    """Error envelope of the winding adjoint: (mag_dv (3, V), mag_dq (Q, 3)), the sums over all
+    triangles and queries of |u_q| |d(W_t(q) / 4 pi)/dv| at each vertex and of |u_q| |dW_t/dq| per
+    query, every term exact in FP64 (brute force, no BVH)."""
+    v = _vertices64(mesh, vertices)
+    i0, i1, i2 = _c(mesh.i0, np.uint32), _c(mesh.i1, np.uint32), _c(mesh.i2, np.uint32)
+    qx, qy, qz = _c(qx, np.float32), _c(qy, np.float32), _c(qz, np.float32)
+    u = _c(u, np.float64)
+    n = qx.shape[0]
+    mv = np.zeros((3, v.shape[1]), np.float64)
+    mq = np.zeros((n, 3), np.float64)
+    lib().gwn_oracle_adjoint_magnitude_f64(_ptr(v), v.shape[1], _ptr(i0), _ptr(i1), _ptr(i2), i0.shape[0],
+                                           _ptr(qx), _ptr(qy), _ptr(qz), _ptr(u), n, _ptr(mv), _ptr(mq),
+                                           threads or default_threads())
+    return mv, mq
 
 
 def distance_adjoint_f64(mesh, px, py, pz, u, vertices=None, threads=None):

@@ -112,58 +112,63 @@ struct crossing_scene {
 __device__ inline bool leaf_crossings(crossing_scene const &sc, std::uint64_t ref, v3 const &o, int axis, float &sum) {
     std::uint64_t const off = ref_offset(ref);
     std::uint32_t const cnt = ref_count(ref);
-    for (std::uint32_t k = 0; k < cnt; ++k) {
+    bool singular = false;
+    for (std::uint32_t k = 0; k < cnt && !singular; ++k) {
         if (off + k >= sc.nt)
             continue;
         float cr = 0.0f;
-        if (triangle_crossing(o, axis, sc.tris[off + k], cr))
-            return true;
-        sum += cr;
+        singular = triangle_crossing(o, axis, sc.tris[off + k], cr);
+        sum += singular ? 0.0f : cr;
     }
-    return false;
+    return singular;
 }
 
 // Signed crossing count (:392-490): returns true when singular; NaN sum on stack overflow.
+//
+// The loop has one exit, at its head: singular triangles and stack overflow set flags instead of
+// returning from inside the unrolled child loop.  With early returns there, the gfx950 backend
+// (ROCm 7.2 LLVM) copied the overflow path's `next` into every lane that had just pushed, so
+// those lanes revisited the pushed node and skipped the new child; the LLVM IR was correct.
 template <int Cap>
 __device__ inline bool crossing_count(crossing_scene const &sc, v3 const &o, int axis, float &sum) {
     sum = 0.0f;
     if (ref_leaf(sc.root))
         return leaf_crossings(sc, sc.root, o, axis, sum);
+    constexpr std::uint32_t k_none = 0xffffffffu;
     std::uint32_t stack[Cap];
     int sp = 0;
+    bool singular = false, overflow = false;
     std::uint32_t node = static_cast<std::uint32_t>(ref_offset(sc.root));
-    for (;;) {
-        std::uint32_t next = 0xffffffffu;
+    while (node != k_none && !singular && !overflow) {
+        std::uint32_t next = k_none;
         if (node < sc.node_count) {
             bvh_node4 const &nd = sc.nodes[node];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 bvh_child const ch = nd.children[s];
-                if (!ref_valid(ch.reference) || !axis_ray_enters(o, axis, ch.bounds))
-                    continue;
-                if (ref_leaf(ch.reference)) {
-                    if (leaf_crossings(sc, ch.reference, o, axis, sum))
-                        return true;
-                    continue;
-                }
-                if (next != 0xffffffffu) {
-                    if (sp >= Cap) {
-                        sum = __builtin_nanf("");
-                        return false;
+                bool const enter = ref_valid(ch.reference) && axis_ray_enters(o, axis, ch.bounds);
+                if (enter && ref_leaf(ch.reference)) {
+                    singular = singular || leaf_crossings(sc, ch.reference, o, axis, sum);
+                } else if (enter) {
+                    if (next != k_none) {
+                        if (sp < Cap)
+                            stack[sp++] = next;
+                        else
+                            overflow = true;
                     }
-                    stack[sp++] = next;
+                    next = static_cast<std::uint32_t>(ref_offset(ch.reference));
                 }
-                next = static_cast<std::uint32_t>(ref_offset(ch.reference));
             }
         }
-        if (next != 0xffffffffu) {
-            node = next;
-            continue;
-        }
-        if (sp == 0)
-            return false;
-        node = stack[--sp];
+        if (next == k_none && sp > 0)
+            next = stack[--sp];
+        node = next;
     }
+    if (overflow) {
+        sum = __builtin_nanf("");
+        return false;
+    }
+    return singular;
 }
 
 __device__ inline v3 minus_ray(int axis) {

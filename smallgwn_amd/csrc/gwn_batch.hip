@@ -1,0 +1,263 @@
+// gwn_batch.hip - batched mode: Taylor moment refits and Taylor winding queries of many meshes of
+// uneven size in persistent work-queue launches, replayable as one hipGraph, with an autotuner.
+//
+// No reference counterpart: the reference runs one mesh per call.  A batch holds, for every mesh,
+// its BVH and moment objects and its query / output device pointers:
+//   - refit: every mesh's moment refit is enqueued without host synchronisation (buffers are
+//     allocated once when the batch is made, so replays allocate nothing);
+//   - query: one persistent launch whose blocks take chunks of one mesh's queries from a global
+//     counter (taylor_batch_kernel), so small and large meshes share the machine instead of
+//     running 4096 launches back to back; every result equals the single-mesh batch's;
+//   - graph: the refits, the counter reset (a fill kernel, not a memset) and the query launch are
+//     captured once into a hipGraph on the batch's own stream and replayed with hipGraphLaunch;
+//   - traversal: one query per lane, or wave-cooperative (a wave walks the tree once for its 64
+//     queries) over each mesh's Morton query order, computed once when the batch is made (any
+//     order gives the same results; Morton order keeps a wave's queries close together);
+//   - tune: traversal, block size, chunk length and blocks per CU are chosen by timing
+//     candidate launches.
+#include "gwn_core.hpp"
+
+#include <algorithm>
+#include <vector>
+
+struct gwn_batch_handle {
+    std::vector<gwn_bvh_handle const *> bvhs;
+    std::vector<gwn_moment_handle *> moments;
+    std::vector<gwn::batch_mesh> host_meshes;
+    std::vector<gwn::moment_refit_work> work;
+    gwn::device_buffer<gwn::batch_mesh> meshes;
+    gwn::device_buffer<std::uint64_t> counter;
+    std::vector<gwn::device_buffer<std::uint32_t>> orders;  // per mesh Morton visiting order
+    std::uint64_t total_queries = 0;
+    std::uint64_t total_chunks = 0;
+    float accuracy_scale = 2.0f;
+    int order = 1;
+    int cap = 32;
+    int block = 256;
+    int chunk = 256;
+    int blocks_per_cu = 4;
+    bool wave = false;
+    int cu_count = 1;
+    hipStream_t own_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~gwn_batch_handle() {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        if (own_stream)
+            (void)hipStreamDestroy(own_stream);
+    }
+};
+
+namespace gwn {
+
+namespace {
+
+void layout_chunks(gwn_batch_handle &b) {
+    std::uint64_t first = 0;
+    for (auto &m : b.host_meshes) {
+        m.first_chunk = first;
+        first += (m.count + static_cast<std::uint64_t>(b.chunk) - 1) / static_cast<std::uint64_t>(b.chunk);
+    }
+    b.total_chunks = first;
+}
+
+void upload_table(gwn_batch_handle &b, hipStream_t stream) {
+    layout_chunks(b);
+    GWN_HIP(hipMemcpyAsync(b.meshes.ptr, b.host_meshes.data(), b.host_meshes.size() * sizeof(batch_mesh),
+                           hipMemcpyHostToDevice, stream));
+    GWN_HIP(hipStreamSynchronize(stream));
+}
+
+} // namespace
+
+void batch_destroy(gwn_batch_handle *b) { delete b; }
+
+void batch_refit(gwn_batch_handle &b, hipStream_t stream) {
+    for (std::size_t i = 0; i < b.bvhs.size(); ++i)
+        refit_moment_enqueue(*b.bvhs[i], *b.moments[i], b.work[i], stream);
+}
+
+void batch_query(gwn_batch_handle &b, hipStream_t stream) {
+    if (b.total_chunks == 0)
+        return;
+    fill_u32(reinterpret_cast<std::uint32_t *>(b.counter.ptr), 0u, 2, stream);
+    int const grid = std::max(1, b.blocks_per_cu * b.cu_count);
+    taylor_batch_launch(b.meshes.ptr, static_cast<std::uint32_t>(b.host_meshes.size()), b.total_chunks,
+                        static_cast<std::uint32_t>(b.chunk), b.accuracy_scale, b.counter.ptr, b.order, b.cap,
+                        b.block, grid, b.wave, stream);
+}
+
+void batch_drop_graph(gwn_batch_handle &b) {
+    if (b.exec)
+        GWN_HIP(hipGraphExecDestroy(b.exec));
+    if (b.graph)
+        GWN_HIP(hipGraphDestroy(b.graph));
+    b.exec = nullptr;
+    b.graph = nullptr;
+}
+
+gwn_batch_handle *batch_create(
+    std::vector<gwn_bvh_handle const *> const &bvhs, std::vector<gwn_moment_handle *> const &moments,
+    std::vector<batch_mesh> const &queries, float accuracy_scale, hipStream_t stream
+) {
+    auto *b = new gwn_batch_handle();
+    try {
+        b->bvhs = bvhs;
+        b->moments = moments;
+        b->host_meshes = queries;
+        b->accuracy_scale = accuracy_scale;
+        b->order = moments.empty() ? 1 : moments[0]->order;
+        b->work.resize(bvhs.size());
+        std::uint32_t bound = 0;
+        for (std::size_t i = 0; i < bvhs.size(); ++i) {
+            refit_moment_prepare(*bvhs[i], *moments[i], b->work[i], stream);
+            bound = std::max(bound, bvhs[i]->internal_stack_bound);
+        }
+        b->cap = bound <= 32 ? 32 : (bound <= 64 ? 64 : 128);
+        if (bound > 128)
+            throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
+        batch_refit(*b, stream);
+        b->orders.resize(bvhs.size());
+        for (std::size_t i = 0; i < bvhs.size(); ++i) {
+            batch_mesh &m = b->host_meshes[i];
+            bool const internal = ref_internal(bvhs[i]->root.reference);
+            m.records = internal ? moments[i]->records.ptr : nullptr;
+            m.tris = bvhs[i]->triangles.ptr;
+            m.root = bvhs[i]->root.reference;
+            m.order = nullptr;
+            if (internal && m.count > 1) {
+                b->orders[i].allocate(m.count, stream);
+                query_order(*bvhs[i], m.qx, m.qy, m.qz, b->orders[i].ptr, m.count, stream);
+                m.order = b->orders[i].ptr;
+            }
+            b->total_queries += m.count;
+        }
+        int dev = 0;
+        GWN_HIP(hipGetDevice(&dev));
+        GWN_HIP(hipDeviceGetAttribute(&b->cu_count, hipDeviceAttributeMultiprocessorCount, dev));
+        b->meshes.allocate(std::max<std::size_t>(1, b->host_meshes.size()), stream);
+        b->counter.allocate(1, stream);
+        upload_table(*b, stream);
+        std::uint32_t err = 0;
+        for (std::size_t i = 0; i < bvhs.size(); ++i) {
+            if (b->work[i].arrivals.ptr == nullptr)
+                continue;
+            std::uint32_t e = 0;
+            GWN_HIP(hipMemcpyAsync(&e, b->work[i].arrivals.ptr + bvhs[i]->node_count, sizeof(e),
+                                   hipMemcpyDeviceToHost, stream));
+            GWN_HIP(hipStreamSynchronize(stream));
+            err |= e;
+        }
+        if (err != 0u)
+            throw_internal("batch: Bottom-up moment refit did not converge.");
+    } catch (...) {
+        delete b;
+        throw;
+    }
+    return b;
+}
+
+void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_cu, bool wave, hipStream_t stream) {
+    if (block <= 0 || block > 256 || (block & 63) != 0)
+        throw_invalid("Batched query block size must be 64, 128, 192 or 256.");
+    if (chunk <= 0 || chunk % block != 0)
+        throw_invalid("Batched query chunk length must be a positive multiple of the block size.");
+    if (blocks_per_cu <= 0 || blocks_per_cu > 64)
+        throw_invalid("Batched query blocks per CU must be in [1, 64].");
+    bool const relayout = chunk != b.chunk;
+    b.block = block;
+    b.chunk = chunk;
+    b.blocks_per_cu = blocks_per_cu;
+    b.wave = wave;
+    if (relayout)
+        upload_table(b, stream);
+    batch_drop_graph(b);
+}
+
+// Times each candidate (traversal, block size, chunk length, blocks per CU) with one launch
+// after a warm-up and keeps the fastest; returns its time in milliseconds.
+float batch_tune(gwn_batch_handle &b, hipStream_t stream) {
+    struct cand {
+        int block, chunk, bpc;
+        bool wave;
+    };
+    std::vector<cand> cands;
+    for (bool wave : {false, true})
+        for (int block : {64, 128, 256})
+            for (int mult : {1, 4})
+                for (int bpc : {2, 4, 8})
+                    cands.push_back({block, block * mult, bpc, wave});
+    hipEvent_t e0, e1;
+    GWN_HIP(hipEventCreate(&e0));
+    GWN_HIP(hipEventCreate(&e1));
+    float best_ms = 0.0f;
+    cand best = {b.block, b.chunk, b.blocks_per_cu, b.wave};
+    bool have = false;
+    try {
+        for (cand const &c : cands) {
+            batch_set_tuning(b, c.block, c.chunk, c.bpc, c.wave, stream);
+            batch_query(b, stream);
+            GWN_HIP(hipEventRecord(e0, stream));
+            batch_query(b, stream);
+            GWN_HIP(hipEventRecord(e1, stream));
+            GWN_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            GWN_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (!have || ms < best_ms) {
+                best_ms = ms;
+                best = c;
+                have = true;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    GWN_HIP(hipEventDestroy(e0));
+    GWN_HIP(hipEventDestroy(e1));
+    batch_set_tuning(b, best.block, best.chunk, best.bpc, best.wave, stream);
+    return best_ms;
+}
+
+void batch_get_tuning(gwn_batch_handle const &b, gwn_batch_tuning &t) {
+    t.block_size = b.block;
+    t.chunk = b.chunk;
+    t.blocks_per_cu = b.blocks_per_cu;
+    t.traversal = b.wave ? 1 : 0;
+    t.stack_capacity = b.cap;
+    t.meshes = b.host_meshes.size();
+    t.queries = b.total_queries;
+}
+
+void batch_capture(gwn_batch_handle &b, bool include_refit) {
+    batch_drop_graph(b);
+    if (b.own_stream == nullptr)
+        GWN_HIP(hipStreamCreateWithFlags(&b.own_stream, hipStreamNonBlocking));
+    GWN_HIP(hipStreamBeginCapture(b.own_stream, hipStreamCaptureModeThreadLocal));
+    try {
+        if (include_refit)
+            batch_refit(b, b.own_stream);
+        batch_query(b, b.own_stream);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(b.own_stream, &g);
+        if (g)
+            (void)hipGraphDestroy(g);
+        throw;
+    }
+    GWN_HIP(hipStreamEndCapture(b.own_stream, &b.graph));
+    GWN_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+}
+
+void batch_replay(gwn_batch_handle &b, hipStream_t stream) {
+    if (b.exec == nullptr)
+        throw_invalid("batch: No captured graph; call gwn_batch_capture first.");
+    GWN_HIP(hipGraphLaunch(b.exec, stream));
+}
+
+} // namespace gwn

@@ -724,6 +724,111 @@ int gwn_compute_unsigned_distance_adjoint_batch_f32(
     });
 }
 
+// ------------------------------------------------------------------------------------------
+// batched mode (no reference counterpart)
+// ------------------------------------------------------------------------------------------
+
+int gwn_batch_create(const gwn_batch_item *items, uint64_t item_count, float accuracy_scale, gwn_batch_t *out,
+                     hipStream_t stream) {
+    return guarded("gwn_batch_create", [&] {
+        if (out == nullptr)
+            gwn::throw_invalid("Output handle pointer is null.");
+        if (item_count != 0 && items == nullptr)
+            gwn::throw_invalid("Batch item array is null.");
+        if (item_count > 0xffffffffull)
+            gwn::throw_invalid("Batch item count exceeds the mesh index range.");
+        if (!(accuracy_scale >= 0.0f))
+            gwn::throw_invalid("Taylor accuracy scale must be nonnegative.");
+        std::vector<gwn_bvh_handle const *> bvhs;
+        std::vector<gwn_moment_handle *> moments;
+        std::vector<gwn::batch_mesh> qs;
+        for (uint64_t i = 0; i < item_count; ++i) {
+            gwn_batch_item const &it = items[i];
+            require_bvh(it.bvh);
+            if (it.moment == nullptr)
+                gwn::throw_invalid("Moment object is not aligned with the BVH.");
+            if (i > 0 && it.moment->order != items[0].moment->order)
+                gwn::throw_invalid("batch: Every moment object must have the same Taylor order.");
+            require_query_storage(it.count, it.qx, it.qy, it.qz, it.out);
+            bvhs.push_back(it.bvh);
+            moments.push_back(it.moment);
+            gwn::batch_mesh m{};
+            m.qx = it.qx;
+            m.qy = it.qy;
+            m.qz = it.qz;
+            m.out = it.out;
+            m.count = it.count;
+            qs.push_back(m);
+        }
+        *out = gwn::batch_create(bvhs, moments, qs, accuracy_scale, stream);
+    });
+}
+
+int gwn_batch_destroy(gwn_batch_t batch) {
+    return guarded("gwn_batch_destroy", [&] { gwn::batch_destroy(batch); });
+}
+
+int gwn_batch_refit_moments(gwn_batch_t batch, hipStream_t stream) {
+    return guarded("gwn_batch_refit_moments", [&] {
+        if (batch == nullptr)
+            gwn::throw_invalid("Batch handle is null.");
+        gwn::batch_refit(*batch, stream);
+    });
+}
+
+int gwn_batch_query(gwn_batch_t batch, hipStream_t stream) {
+    return guarded("gwn_batch_query", [&] {
+        if (batch == nullptr)
+            gwn::throw_invalid("Batch handle is null.");
+        gwn::batch_query(*batch, stream);
+    });
+}
+
+int gwn_batch_set_tuning(gwn_batch_t batch, const gwn_batch_tuning *tuning, hipStream_t stream) {
+    return guarded("gwn_batch_set_tuning", [&] {
+        if (batch == nullptr || tuning == nullptr)
+            gwn::throw_invalid("Batch handle or tuning pointer is null.");
+        if (tuning->traversal != 0 && tuning->traversal != 1)
+            gwn::throw_invalid("Batched query traversal must be 0 (per lane) or 1 (wave).");
+        gwn::batch_set_tuning(*batch, tuning->block_size, tuning->chunk, tuning->blocks_per_cu,
+                              tuning->traversal == 1, stream);
+    });
+}
+
+int gwn_batch_get_tuning(gwn_batch_t batch, gwn_batch_tuning *tuning) {
+    return guarded("gwn_batch_get_tuning", [&] {
+        if (batch == nullptr || tuning == nullptr)
+            gwn::throw_invalid("Batch handle or tuning pointer is null.");
+        gwn::batch_get_tuning(*batch, *tuning);
+    });
+}
+
+int gwn_batch_tune(gwn_batch_t batch, hipStream_t stream, float *best_ms) {
+    return guarded("gwn_batch_tune", [&] {
+        if (batch == nullptr)
+            gwn::throw_invalid("Batch handle is null.");
+        float const ms = gwn::batch_tune(*batch, stream);
+        if (best_ms)
+            *best_ms = ms;
+    });
+}
+
+int gwn_batch_capture(gwn_batch_t batch, int32_t include_refit) {
+    return guarded("gwn_batch_capture", [&] {
+        if (batch == nullptr)
+            gwn::throw_invalid("Batch handle is null.");
+        gwn::batch_capture(*batch, include_refit != 0);
+    });
+}
+
+int gwn_batch_replay(gwn_batch_t batch, hipStream_t stream) {
+    return guarded("gwn_batch_replay", [&] {
+        if (batch == nullptr)
+            gwn::throw_invalid("Batch handle is null.");
+        gwn::batch_replay(*batch, stream);
+    });
+}
+
 int gwn_compute_query_order_f32(
     gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, uint32_t *order, uint64_t count,
     hipStream_t stream

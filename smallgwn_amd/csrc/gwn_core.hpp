@@ -10,6 +10,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 #include <stdexcept>
 #include <string>
 
@@ -232,6 +233,8 @@ struct gwn_boundary_handle {
     hipStream_t stream = nullptr;
 };
 
+struct gwn_batch_handle;
+
 struct gwn_moment_handle {
     int order = 0;
     gwn::device_buffer<float4> records; // node_count * moment_record_vec4(order)
@@ -262,6 +265,45 @@ void build_bvh4(gwn_geometry_handle const &geometry, gwn_bvh_handle &staging, in
 // Algebraic boundary of the oriented triangle chain (directed rows in (min, max) vertex order).
 void build_boundary_chain(gwn_geometry_handle const &geometry, gwn_boundary_handle &staging, hipStream_t stream);
 void refit_moment(gwn_bvh_handle const &bvh, gwn_moment_handle &staging, hipStream_t stream);
+// One mesh of a batched Taylor query launch (device-resident table).
+struct batch_mesh {
+    float4 const *records;       // moment traversal records (null for a leaf root)
+    tri_record const *tris;
+    std::uint64_t root;          // root child reference
+    float const *qx, *qy, *qz;
+    float *out;
+    std::uint64_t count;
+    std::uint64_t first_chunk;   // exclusive prefix of chunk counts over earlier meshes
+    std::uint32_t const *order;  // visiting order of the queries (Morton), or null
+};
+// Persistent batched Taylor winding (plain precision): `grid` blocks of `block` threads take
+// chunks of `chunk` queries from *counter (which must be zero) until total_chunks are done.
+void taylor_batch_launch(
+    batch_mesh const *meshes, std::uint32_t mesh_count, std::uint64_t total_chunks, std::uint32_t chunk,
+    float accuracy_scale, std::uint64_t *counter, int order, int cap, int block, int grid, bool wave,
+    hipStream_t stream
+);
+// Batched mode (gwn_batch.hip).
+gwn_batch_handle *batch_create(
+    std::vector<gwn_bvh_handle const *> const &bvhs, std::vector<gwn_moment_handle *> const &moments,
+    std::vector<batch_mesh> const &queries, float accuracy_scale, hipStream_t stream
+);
+void batch_destroy(gwn_batch_handle *b);
+void batch_refit(gwn_batch_handle &b, hipStream_t stream);
+void batch_query(gwn_batch_handle &b, hipStream_t stream);
+void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_cu, bool wave, hipStream_t stream);
+float batch_tune(gwn_batch_handle &b, hipStream_t stream);
+void batch_get_tuning(gwn_batch_handle const &b, gwn_batch_tuning &t);
+void batch_capture(gwn_batch_handle &b, bool include_refit);
+void batch_replay(gwn_batch_handle &b, hipStream_t stream);
+// Moment refit split into allocation and enqueue, for callers that replay it without host
+// synchronisation (the enqueue never synchronises; its error flag stays in arrivals[node_count]).
+struct moment_refit_work {
+    device_buffer<std::uint8_t> pending;    // per-node additive payloads
+    device_buffer<std::uint32_t> arrivals;  // node_count arrival counters + error flag
+};
+void refit_moment_prepare(gwn_bvh_handle const &bvh, gwn_moment_handle &m, moment_refit_work &w, hipStream_t stream);
+void refit_moment_enqueue(gwn_bvh_handle const &bvh, gwn_moment_handle &m, moment_refit_work &w, hipStream_t stream);
 // In-place bounds refit: triangle records and child boxes from the geometry, root box updated.
 void refit_bounds(gwn_geometry_handle const &geometry, gwn_bvh_handle &bvh, hipStream_t stream);
 

@@ -358,7 +358,48 @@ void launch_refit(gwn_bvh_handle const &bvh, gwn_moment_handle &m, hipStream_t s
         throw_internal("bvh.refit.moment: Bottom-up moment refit did not converge.");
 }
 
+template <int Order>
+void enqueue_refit(gwn_bvh_handle const &bvh, gwn_moment_handle &m, moment_refit_work &w, hipStream_t stream) {
+    std::size_t const N = bvh.node_count;
+    constexpr int RV = moment_record_vec4(Order);
+    fill_u32(reinterpret_cast<std::uint32_t *>(m.records.ptr), 0u, N * RV * 4, stream);
+    fill_u32(reinterpret_cast<std::uint32_t *>(m.slot_area.ptr), 0u, N * 4, stream);
+    fill_u32(w.arrivals.ptr, 0u, N + 1, stream);
+    moment_refit_kernel<Order><<<grid_for(N, k_block), k_block, 0, stream>>>(
+        bvh.nodes.ptr, N, bvh.triangles.ptr, bvh.triangle_count, bvh.parent.ptr, bvh.internal_children.ptr,
+        w.arrivals.ptr, reinterpret_cast<payload<Order> *>(w.pending.ptr), m.records.ptr, m.slot_area.ptr,
+        w.arrivals.ptr + N);
+    GWN_LAUNCH_CHECK();
+}
+
+std::size_t payload_bytes(int order) {
+    return order == 0 ? sizeof(payload<0>) : (order == 1 ? sizeof(payload<1>) : sizeof(payload<2>));
+}
+
 } // namespace
+
+void refit_moment_prepare(gwn_bvh_handle const &bvh, gwn_moment_handle &m, moment_refit_work &w, hipStream_t stream) {
+    std::size_t const N = ref_internal(bvh.root.reference) ? bvh.node_count : 0;
+    m.node_count = bvh.node_count;
+    if (N == 0)
+        return;
+    m.records.allocate(N * static_cast<std::size_t>(moment_record_vec4(m.order)), stream);
+    m.slot_area.allocate(N * 4, stream);
+    w.pending.allocate(N * payload_bytes(m.order), stream);
+    w.arrivals.allocate(N + 1, stream);
+}
+
+void refit_moment_enqueue(gwn_bvh_handle const &bvh, gwn_moment_handle &m, moment_refit_work &w, hipStream_t stream) {
+    m.node_count = bvh.node_count;
+    m.bvh_revision = bvh.revision;
+    if (!ref_internal(bvh.root.reference) || bvh.node_count == 0)
+        return;
+    switch (m.order) {
+    case 0: enqueue_refit<0>(bvh, m, w, stream); break;
+    case 1: enqueue_refit<1>(bvh, m, w, stream); break;
+    default: enqueue_refit<2>(bvh, m, w, stream); break;
+    }
+}
 
 void refit_moment(gwn_bvh_handle const &bvh, gwn_moment_handle &m, hipStream_t stream) {
     m.node_count = bvh.node_count;

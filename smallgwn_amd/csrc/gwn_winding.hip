@@ -222,6 +222,9 @@ __device__ inline float product_unfused(float a, float b) {
 // vertex; an edge seen end-on contributes nothing.
 template <class R>
 __device__ inline void solid_angle_gradient(R const (&g)[3][3], R (&out)[3]) {
+    // Unfused: near an edge the 1 / |g_i x g_j|^2 factor amplifies rounding, and the unfused
+    // sequence is the oracle's, operation for operation.
+#pragma clang fp contract(off)
     R l[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -455,25 +458,22 @@ struct taylor_outputs {
     std::uint32_t *counters;    // optional 3 x uint32 per query
 };
 
+// Per-lane Taylor traversal of one query from an internal root: children in slot order, the last
+// descended internal child entered directly, the others pushed.  Returns true on stack overflow.
 template <int Order, int Prec, bool Grad, int StackCap>
-__global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
-    float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node,
-    float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
-    std::uint32_t const *__restrict__ order, taylor_outputs o, std::size_t n, float beta2
+__device__ inline bool taylor_traverse(
+    float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node, float x,
+    float y, float z, float beta2, taylor_sums<Prec, Grad> &acc, std::uint32_t &n_tests, std::uint32_t &n_accept,
+    std::uint32_t &n_leaf
 ) {
     constexpr int RV = moment_record_vec4(Order);
-    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_taylor_block + threadIdx.x;
-    if (i >= n)
-        return;
-    std::size_t const q = order ? order[i] : i;
-    float const x = qx[q], y = qy[q], z = qz[q];
     std::uint32_t stack[StackCap];
     int sp = 0;
-    taylor_sums<Prec, Grad> acc;
-    std::uint32_t n_tests = 0, n_accept = 0, n_leaf = 0;
-    std::uint32_t node = root_node;
     bool overflow = false;
-    for (;;) {
+    std::uint32_t node = root_node;
+    // One loop exit, at the head: an early return from inside the unrolled child loop has been
+    // lowered wrongly by the gfx950 backend (ROCm 7.2 LLVM) in the same traversal shape.
+    while (node != k_invalid && !overflow) {
         float4 const *rec = records + static_cast<std::size_t>(node) * RV;
         float4 const refs = rec[0];
         float4 const mp = rec[1], px = rec[2], py = rec[3], pz = rec[4];
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             std::uint32_t const c = cref[s];
-            if (c == k_invalid)
+            if (c == k_invalid || overflow)
                 continue;
             ++n_tests;
             float const qrx = x - cx[s], qry = y - cy[s], qrz = z - cz[s];
@@ -511,29 +511,109 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
                 continue;
             }
             if (next != k_invalid) {
-                if (sp >= StackCap) {
+                if (sp < StackCap)
+                    stack[sp++] = next;
+                else
                     overflow = true;
-                    break;
-                }
-                stack[sp++] = next;
             }
             next = c;
         }
-        if (overflow)
-            break;
-        if (next != k_invalid) {
-            node = next;
-            continue;
-        }
-        if (sp == 0)
-            break;
-        node = stack[--sp];
+        if (next == k_invalid && sp > 0)
+            next = stack[--sp];
+        node = next;
     }
+    return overflow;
+}
+
+template <int Order, int Prec, bool Grad, int StackCap>
+__global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
+    float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node,
+    float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
+    std::uint32_t const *__restrict__ order, taylor_outputs o, std::size_t n, float beta2
+) {
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_taylor_block + threadIdx.x;
+    if (i >= n)
+        return;
+    std::size_t const q = order ? order[i] : i;
+    float const x = qx[q], y = qy[q], z = qz[q];
+    taylor_sums<Prec, Grad> acc;
+    std::uint32_t n_tests = 0, n_accept = 0, n_leaf = 0;
+    bool const overflow =
+        taylor_traverse<Order, Prec, Grad, StackCap>(records, tris, root_node, x, y, z, beta2, acc, n_tests, n_accept, n_leaf);
     acc.store(overflow, q, o.out, o.gx, o.gy, o.gz);
     if (o.counters != nullptr) {
         o.counters[3 * q + 0] = n_tests;
         o.counters[3 * q + 1] = n_accept;
         o.counters[3 * q + 2] = n_leaf;
+    }
+}
+
+// Batched persistent traversal: many meshes, one launch.  Work is a list of chunks of `chunk`
+// consecutive queries of one mesh; every block takes chunks from a global counter until none is
+// left (no wave waits for another), and its lanes evaluate one query each with that mesh's
+// records and the per-lane traversal above, so each result equals the single-mesh batch's.
+template <int Order, int StackCap, bool Wave>
+__global__ __launch_bounds__(256) void taylor_batch_kernel(
+    batch_mesh const *__restrict__ meshes, std::uint32_t mesh_count, std::uint64_t total_chunks, std::uint32_t chunk,
+    float beta2, std::uint64_t *counter
+) {
+    __shared__ std::uint64_t s_chunk;
+    __shared__ std::uint32_t s_node[4][Wave ? StackCap : 1];
+    __shared__ std::uint64_t s_mask[4][Wave ? StackCap : 1];
+    int const wave = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_chunk = atomicAdd(reinterpret_cast<unsigned long long *>(counter), 1ull);
+        __syncthreads();
+        std::uint64_t const c = s_chunk;
+        __syncthreads();
+        if (c >= total_chunks)
+            return;
+        // Mesh of this chunk: last mesh whose first_chunk <= c (binary search).
+        std::uint32_t lo = 0, hi = mesh_count;
+        while (hi - lo > 1) {
+            std::uint32_t const mid = (lo + hi) >> 1;
+            if (meshes[mid].first_chunk <= c)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        batch_mesh const m = meshes[lo];
+        std::uint64_t const base = (c - m.first_chunk) * chunk;
+        bool const leaf_root = ref_leaf(m.root);
+        // Whole waves step through the chunk together, so the wave traversal sees uniform meshes.
+        for (std::uint32_t t = threadIdx.x; t - (threadIdx.x & 63u) < chunk; t += blockDim.x) {
+            std::uint64_t const p = base + t;
+            bool const active = t < chunk && p < m.count;
+            std::uint64_t const q = active ? (m.order ? m.order[p] : p) : 0;
+            float const x = active ? m.qx[q] : 0.0f, y = active ? m.qy[q] : 0.0f, z = active ? m.qz[q] : 0.0f;
+            taylor_sums<GWN_PRECISION_PLAIN, false> acc;
+            std::uint32_t n_tests = 0, n_accept = 0, n_leaf = 0;
+            bool overflow = false;
+            if (leaf_root) {
+                if (!active)
+                    continue;
+                std::uint64_t const off = ref_offset(m.root);
+                std::uint32_t const cnt = ref_count(m.root);
+                for (std::uint32_t k = 0; k < cnt; ++k)
+                    acc.leaf(m.tris[off + k], x, y, z);
+            } else if constexpr (Wave) {
+                if (__ballot(active) == 0ull)
+                    continue;
+                overflow = taylor_wave_traverse<Order, GWN_PRECISION_PLAIN, false, StackCap>(
+                    m.records, m.tris, static_cast<std::uint32_t>(ref_offset(m.root)), active, x, y, z, beta2, acc,
+                    n_tests, n_accept, n_leaf, s_node[wave], s_mask[wave]);
+                if (!active)
+                    continue;
+            } else {
+                if (!active)
+                    continue;
+                overflow = taylor_traverse<Order, GWN_PRECISION_PLAIN, false, StackCap>(
+                    m.records, m.tris, static_cast<std::uint32_t>(ref_offset(m.root)), x, y, z, beta2, acc, n_tests,
+                    n_accept, n_leaf);
+            }
+            acc.store(overflow, q, m.out, nullptr, nullptr, nullptr);
+        }
     }
 }
 
@@ -544,34 +624,22 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_kernel(
 // and entered in the same slot / last-in-first-out order as in the per-lane kernel, so every lane
 // forms its sum over exactly the same terms in exactly the same order; the stack depth is bounded
 // by internal_stack_bound for the same reason.
+// The wave-cooperative traversal of one wave's queries (lanes with `active`); s_node / s_mask are
+// this wave's LDS stack.  The wave must hold at least one active lane.  Returns true on overflow.
 template <int Order, int Prec, bool Grad, int StackCap>
-__global__ __launch_bounds__(k_taylor_block) void taylor_wave_kernel(
-    float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node,
-    float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
-    std::uint32_t const *__restrict__ order, taylor_outputs o, std::size_t n, float beta2
+__device__ inline bool taylor_wave_traverse(
+    float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node, bool active,
+    float x, float y, float z, float beta2, taylor_sums<Prec, Grad> &acc, std::uint32_t &n_tests,
+    std::uint32_t &n_accept, std::uint32_t &n_leaf, std::uint32_t *s_node, std::uint64_t *s_mask
 ) {
     constexpr int RV = moment_record_vec4(Order);
-    constexpr int k_waves = k_taylor_block / 64;
-    __shared__ std::uint32_t s_node[k_waves][StackCap];
-    __shared__ std::uint64_t s_mask[k_waves][StackCap];
     int const lane = threadIdx.x & 63;
-    int const wave = threadIdx.x >> 6;
-    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_taylor_block + threadIdx.x;
-    bool const active = i < n;
-    std::size_t const q = active ? (order ? order[i] : i) : 0;
-    float const x = active ? qx[q] : 0.0f;
-    float const y = active ? qy[q] : 0.0f;
-    float const z = active ? qz[q] : 0.0f;
     std::uint64_t mask = __ballot(active);
-    if (mask == 0ull)
-        return;
     std::uint64_t const my_bit = 1ull << lane;
-    taylor_sums<Prec, Grad> acc;
-    std::uint32_t n_tests = 0, n_accept = 0, n_leaf = 0;
     std::uint32_t node = root_node;
     int sp = 0;
-    bool overflow = false;
-    for (;;) {
+    bool overflow = false;  // wave-uniform
+    while (node != k_invalid && !overflow) {
         node = __builtin_amdgcn_readfirstlane(node);
         float4 const *rec = records + static_cast<std::size_t>(node) * RV;
         float4 const refs = rec[0];
@@ -589,7 +657,7 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_wave_kernel(
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             std::uint32_t const c = __builtin_amdgcn_readfirstlane(cref[s]);
-            if (c == k_invalid)
+            if (c == k_invalid || overflow)
                 continue;
             bool descend = false;
             if (in) {
@@ -620,30 +688,50 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_wave_kernel(
                 continue;
             }
             if (next != k_invalid) {
-                if (sp >= StackCap) {
+                if (sp < StackCap) {
+                    s_node[sp] = next;
+                    s_mask[sp] = next_mask;
+                    ++sp;
+                } else {
                     overflow = true;
-                    break;
                 }
-                s_node[wave][sp] = next;
-                s_mask[wave][sp] = next_mask;
-                ++sp;
             }
             next = c;
             next_mask = dmask;
         }
-        if (overflow)
-            break;
-        if (next != k_invalid) {
-            node = next;
-            mask = next_mask;
-            continue;
+        if (next == k_invalid && sp > 0) {
+            --sp;
+            next = s_node[sp];
+            next_mask = s_mask[sp];
         }
-        if (sp == 0)
-            break;
-        --sp;
-        node = s_node[wave][sp];
-        mask = s_mask[wave][sp];
+        node = next;
+        mask = next_mask;
     }
+    return overflow;
+}
+
+template <int Order, int Prec, bool Grad, int StackCap>
+__global__ __launch_bounds__(k_taylor_block) void taylor_wave_kernel(
+    float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node,
+    float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
+    std::uint32_t const *__restrict__ order, taylor_outputs o, std::size_t n, float beta2
+) {
+    constexpr int k_waves = k_taylor_block / 64;
+    __shared__ std::uint32_t s_node[k_waves][StackCap];
+    __shared__ std::uint64_t s_mask[k_waves][StackCap];
+    int const wave = threadIdx.x >> 6;
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_taylor_block + threadIdx.x;
+    bool const active = i < n;
+    std::size_t const q = active ? (order ? order[i] : i) : 0;
+    float const x = active ? qx[q] : 0.0f;
+    float const y = active ? qy[q] : 0.0f;
+    float const z = active ? qz[q] : 0.0f;
+    if (__ballot(active) == 0ull)
+        return;
+    taylor_sums<Prec, Grad> acc;
+    std::uint32_t n_tests = 0, n_accept = 0, n_leaf = 0;
+    bool const overflow = taylor_wave_traverse<Order, Prec, Grad, StackCap>(
+        records, tris, root_node, active, x, y, z, beta2, acc, n_tests, n_accept, n_leaf, s_node[wave], s_mask[wave]);
     if (!active)
         return;
     acc.store(overflow, q, o.out, o.gx, o.gy, o.gz);
@@ -679,34 +767,38 @@ __global__ __launch_bounds__(256) void leaf_root_kernel(
 // relative to the query), from the unnormalised form W = 2 atan2(a.(b x c), D) with
 // D = |a||b||c| + (a.b)|c| + (a.c)|b| + (b.c)|a| (equal to the Van Oosterom-Strackee quotient
 // scaled by |a||b||c|):  dW = 2 (D dnum - num dD) / (num^2 + D^2).  Zero where W is defined as 0.
+// Evaluated in FP64: near the surface the terms are ~1/d^2, cancel between neighbouring
+// triangles, and lose ~|a|/d of their FP32 digits.
 __device__ inline void solid_angle_vertex_grad(
-    float const (&a)[3], float const (&b)[3], float const (&c)[3], float (&da)[3], float (&db)[3], float (&dc)[3]
+    double const (&a)[3], double const (&b)[3], double const (&c)[3], double (&da)[3], double (&db)[3],
+    double (&dc)[3]
 ) {
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-        da[k] = db[k] = dc[k] = 0.0f;
-    float const la = sqrtf(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-    float const lb = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
-    float const lc = sqrtf(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-    if (la == 0.0f || lb == 0.0f || lc == 0.0f)
+        da[k] = db[k] = dc[k] = 0.0;
+    double const la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    double const lb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    double const lc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    if (la == 0.0 || lb == 0.0 || lc == 0.0)
         return;
-    float const bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
-    float const ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
-    float const ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
-    float const num = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
-    if (num == 0.0f)
+    double const bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+    double const ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+    double const ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    double const num = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+    if (num == 0.0)
         return;
-    float const dab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-    float const dac = a[0] * c[0] + a[1] * c[1] + a[2] * c[2];
-    float const dbc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
-    float const den = la * lb * lc + dab * lc + dac * lb + dbc * la;
-    float const sc = 2.0f / (num * num + den * den);
+    double const dab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    double const dac = a[0] * c[0] + a[1] * c[1] + a[2] * c[2];
+    double const dbc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
+    double const den = la * lb * lc + dab * lc + dac * lb + dbc * la;
+    double const sc = 2.0 / (num * num + den * den);
+    double const ia = 1.0 / la, ib = 1.0 / lb, ic = 1.0 / lc;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float const ha = a[k] / la, hb = b[k] / lb, hc = c[k] / lc;
-        float const dDa = ha * lb * lc + b[k] * lc + c[k] * lb + dbc * ha;
-        float const dDb = hb * la * lc + a[k] * lc + c[k] * la + dac * hb;
-        float const dDc = hc * la * lb + a[k] * lb + b[k] * la + dab * hc;
+        double const ha = a[k] * ia, hb = b[k] * ib, hc = c[k] * ic;
+        double const dDa = ha * lb * lc + b[k] * lc + c[k] * lb + dbc * ha;
+        double const dDb = hb * la * lc + a[k] * lc + c[k] * la + dac * hb;
+        double const dDc = hc * la * lb + a[k] * lb + b[k] * la + dab * hc;
         da[k] = sc * (den * bc[k] - num * dDa);
         db[k] = sc * (den * ca[k] - num * dDb);
         dc[k] = sc * (den * ab[k] - num * dDc);
@@ -714,22 +806,24 @@ __device__ inline void solid_angle_vertex_grad(
 }
 
 // Adds u * dW/d(a, b, c) of one near-field triangle to its record's 9 adjoint entries and
-// returns -(da + db + dc) * u (the query part) through dq.
+// subtracts (da + db + dc) * u (the query part) from dq.  The corner vectors are exact FP64
+// differences of the float vertices and query.
 __device__ inline void adjoint_leaf(
-    tri_record const &t, std::size_t rec, float x, float y, float z, float su, float *__restrict__ dtri,
-    float (&dq)[3]
+    tri_record const &t, std::size_t rec, float x, float y, float z, double su, float *__restrict__ dtri,
+    double (&dq)[3]
 ) {
-    float const a[3] = {t.v0x - x, t.v0y - y, t.v0z - z};
-    float const b[3] = {t.v1x - x, t.v1y - y, t.v1z - z};
-    float const c[3] = {t.v2x - x, t.v2y - y, t.v2z - z};
-    float da[3], db[3], dc[3];
+    double const qx = x, qy = y, qz = z;
+    double const a[3] = {double(t.v0x) - qx, double(t.v0y) - qy, double(t.v0z) - qz};
+    double const b[3] = {double(t.v1x) - qx, double(t.v1y) - qy, double(t.v1z) - qz};
+    double const c[3] = {double(t.v2x) - qx, double(t.v2y) - qy, double(t.v2z) - qz};
+    double da[3], db[3], dc[3];
     solid_angle_vertex_grad(a, b, c, da, db, dc);
     float *d = dtri + rec * 9;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        atomicAdd(d + k, su * da[k]);
-        atomicAdd(d + 3 + k, su * db[k]);
-        atomicAdd(d + 6 + k, su * dc[k]);
+        atomicAdd(d + k, static_cast<float>(su * da[k]));
+        atomicAdd(d + 3 + k, static_cast<float>(su * db[k]));
+        atomicAdd(d + 6 + k, static_cast<float>(su * dc[k]));
         dq[k] -= su * (da[k] + db[k] + dc[k]);
     }
 }
@@ -754,17 +848,19 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_adjoint_kernel(
     std::size_t const q = order ? order[i] : i;
     float const x = qx[q], y = qy[q], z = qz[q];
     float const su = u[q] * k_inv_4pi_f;
-    float dq[3] = {0.0f, 0.0f, 0.0f};
+    double const sud = double(u[q]) * (1.0 / (4.0 * k_pi_d));
+    double dq[3] = {0.0, 0.0, 0.0};  // FP64 per-query sum of the far and near parts
     if (root_node == k_invalid) {
         std::uint64_t const off = ref_offset(root_leaf);
         std::uint32_t const cnt = ref_count(root_leaf);
         for (std::uint32_t k = 0; k < cnt; ++k)
-            adjoint_leaf(tris[off + k], off + k, x, y, z, su, dtri, dq);
+            adjoint_leaf(tris[off + k], off + k, x, y, z, sud, dtri, dq);
     } else {
         std::uint32_t stack[StackCap];
         int sp = 0;
+        bool overflow = false;
         std::uint32_t node = root_node;
-        for (;;) {
+        while (node != k_invalid && !overflow) {
             float4 const *rec = records + static_cast<std::size_t>(node) * RV;
             float4 const refs = rec[0];
             float4 const mp = rec[1], px = rec[2], py = rec[3], pz = rec[4];
@@ -776,11 +872,10 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_adjoint_kernel(
             float const cy[4] = {py.x, py.y, py.z, py.w};
             float const cz[4] = {pz.x, pz.y, pz.z, pz.w};
             std::uint32_t next = k_invalid;
-            bool overflow = false;
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 std::uint32_t const c = cref[s];
-                if (c == k_invalid)
+                if (c == k_invalid || overflow)
                     continue;
                 float const qrx = x - cx[s], qry = y - cy[s], qrz = z - cz[s];
                 float const ql2 = squared_norm_unfused(qrx, qry, qrz);
@@ -797,7 +892,7 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_adjoint_kernel(
                         for (int k = 0; k < 3; ++k) {
                             atomicAdd(d + k, -su * g[k]);
                             atomicAdd(d + 3 + k, -su * m2 * qn[k]);
-                            dq[k] += su * g[k];
+                            dq[k] += double(su * g[k]);
                         }
                         if constexpr (Order >= 1) {
                             float const m3 = m2 * m1;
@@ -813,35 +908,28 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_adjoint_kernel(
                 }
                 if (c & k_cref_leaf) {
                     std::uint32_t const r = c & ~k_cref_leaf;
-                    adjoint_leaf(tris[r], r, x, y, z, su, dtri, dq);
+                    adjoint_leaf(tris[r], r, x, y, z, sud, dtri, dq);
                     continue;
                 }
                 if (next != k_invalid) {
-                    if (sp >= StackCap) {
+                    if (sp < StackCap)
+                        stack[sp++] = next;
+                    else
                         overflow = true;
-                        break;
-                    }
-                    stack[sp++] = next;
                 }
                 next = c;
             }
-            if (overflow) {
-                dq[0] = dq[1] = dq[2] = __builtin_nanf("");
-                break;
-            }
-            if (next != k_invalid) {
-                node = next;
-                continue;
-            }
-            if (sp == 0)
-                break;
-            node = stack[--sp];
+            if (next == k_invalid && sp > 0)
+                next = stack[--sp];
+            node = next;
         }
+        if (overflow)
+            dq[0] = dq[1] = dq[2] = __builtin_nan("");
     }
     if (dqx != nullptr) {
-        dqx[q] = dq[0];
-        dqy[q] = dq[1];
-        dqz[q] = dq[2];
+        dqx[q] = static_cast<float>(dq[0]);
+        dqy[q] = static_cast<float>(dq[1]);
+        dqz[q] = static_cast<float>(dq[2]);
     }
 }
 
@@ -910,7 +998,46 @@ void launch_taylor_adjoint(
     GWN_LAUNCH_CHECK();
 }
 
+template <int Order, bool Wave>
+void launch_batch_cap(batch_mesh const *meshes, std::uint32_t mesh_count, std::uint64_t total_chunks,
+                      std::uint32_t chunk, float beta2, std::uint64_t *counter, int cap, int block, int grid,
+                      hipStream_t stream) {
+#define GWN_BATCH(C)                                                                                      \
+    taylor_batch_kernel<Order, C, Wave><<<grid, block, 0, stream>>>(meshes, mesh_count, total_chunks, chunk, beta2, counter)
+    if (cap <= 32)
+        GWN_BATCH(32);
+    else if (cap <= 64)
+        GWN_BATCH(64);
+    else if (cap <= 128)
+        GWN_BATCH(128);
+    else
+        throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
+#undef GWN_BATCH
+    GWN_LAUNCH_CHECK();
+}
+
 } // namespace
+
+void taylor_batch_launch(
+    batch_mesh const *meshes, std::uint32_t mesh_count, std::uint64_t total_chunks, std::uint32_t chunk,
+    float accuracy_scale, std::uint64_t *counter, int order, int cap, int block, int grid, bool wave,
+    hipStream_t stream
+) {
+    float const beta2 = accuracy_scale * accuracy_scale;
+    if (block <= 0 || block > 256 || (block & 63) != 0)
+        throw_invalid("Batched query block size must be 64, 128, 192 or 256.");
+#define GWN_BATCH_ORDER(O)                                                                                     \
+    (wave ? launch_batch_cap<O, true>(meshes, mesh_count, total_chunks, chunk, beta2, counter, cap, block, grid, \
+                                      stream)                                                                   \
+          : launch_batch_cap<O, false>(meshes, mesh_count, total_chunks, chunk, beta2, counter, cap, block, grid, \
+                                       stream))
+    switch (order) {
+    case 0: GWN_BATCH_ORDER(0); break;
+    case 1: GWN_BATCH_ORDER(1); break;
+    default: GWN_BATCH_ORDER(2); break;
+    }
+#undef GWN_BATCH_ORDER
+}
 
 void taylor_adjoint_traverse(
     gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy, float const *qz,

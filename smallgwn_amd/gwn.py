@@ -43,6 +43,8 @@ EXPORTED_SYMBOLS = (
     "gwn_compute_winding_number_antipodal_batch_f32", "gwn_compute_winding_number_antipodal_batch_ex_f32",
     "gwn_compute_winding_gradient_antipodal_batch_f32", "gwn_compute_winding_gradient_antipodal_batch_ex_f32",
     "gwn_compute_winding_taylor_adjoint_batch_f32", "gwn_compute_unsigned_distance_adjoint_batch_f32",
+    "gwn_batch_create", "gwn_batch_destroy", "gwn_batch_refit_moments", "gwn_batch_query",
+    "gwn_batch_set_tuning", "gwn_batch_get_tuning", "gwn_batch_tune", "gwn_batch_capture", "gwn_batch_replay",
     "gwn_compute_query_order_f32", "gwn_radix_sort_pairs_u64", "gwn_radix_sort_pairs_u32",
 )
 
@@ -173,6 +175,15 @@ def _load():
             vp, vp, vp, vp, vp, vp, vp, u64, f32, vp, vp, vp, vp, vp, vp, vp, vp],
         "gwn_compute_unsigned_distance_adjoint_batch_f32": [
             vp, vp, vp, vp, vp, vp, u64, f32, vp, vp, vp, vp, vp, vp, vp, vp],
+        "gwn_batch_create": [vp, u64, f32, vp, vp],
+        "gwn_batch_destroy": [vp],
+        "gwn_batch_refit_moments": [vp, vp],
+        "gwn_batch_query": [vp, vp],
+        "gwn_batch_set_tuning": [vp, vp, vp],
+        "gwn_batch_get_tuning": [vp, vp],
+        "gwn_batch_tune": [vp, vp, vp],
+        "gwn_batch_capture": [vp, i32],
+        "gwn_batch_replay": [vp, vp],
         "gwn_compute_query_order_f32": [vp, vp, vp, vp, vp, u64, vp],
         "gwn_radix_sort_pairs_u64": [vp, vp, u64, i32, i32, vp],
         "gwn_radix_sort_pairs_u32": [vp, vp, u64, i32, i32, vp],
@@ -631,6 +642,81 @@ def gwn_compute_unsigned_distance_adjoint_batch(geometry: GeometryObject, bvh: B
         geometry.handle, bvh.handle, _dptr(px), _dptr(py), _dptr(pz), _dptr(grad_d), n, float(culling_band),
         _dptr(grad_vx), _dptr(grad_vy), _dptr(grad_vz), _dptr(grad_px), _dptr(grad_py), _dptr(grad_pz),
         ctypes.byref(cfg), _stream(stream)))
+
+
+class _BatchItem(ctypes.Structure):
+    _fields_ = [("bvh", ctypes.c_void_p), ("moment", ctypes.c_void_p), ("qx", ctypes.c_void_p),
+                ("qy", ctypes.c_void_p), ("qz", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("count", ctypes.c_uint64)]
+
+
+class _BatchTuning(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_int32), ("chunk", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
+                ("traversal", ctypes.c_int32), ("stack_capacity", ctypes.c_int32), ("meshes", ctypes.c_uint64),
+                ("queries", ctypes.c_uint64)]
+
+
+class BatchObject:
+    """Batched mode (no reference counterpart): Taylor moment refits and Taylor winding queries of
+    many meshes in persistent launches, replayable as one hipGraph.  ``items`` are tuples
+    (bvh, moment, qx, qy, qz, out) of library objects and float32 device tensors; every moment has
+    the same order.  The objects and tensors are kept alive by this object."""
+
+    def __init__(self, items, accuracy_scale: float = 2.0, stream=None):
+        self._keep = list(items)
+        arr = (_BatchItem * max(1, len(self._keep)))()
+        for k, (bvh, mom, qx, qy, qz, out) in enumerate(self._keep):
+            n = _numel(qx)
+            if _numel(qy) != n or _numel(qz) != n or _numel(out) != n:
+                raise ValueError("Query SoA spans require identical lengths.")
+            arr[k] = _BatchItem(bvh.handle.value, mom.handle.value, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), n)
+        h = ctypes.c_void_p()
+        _status(_lib.gwn_batch_create(arr, len(self._keep), float(accuracy_scale), ctypes.byref(h),
+                                      _stream(stream))).raise_if_error()
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def refit_moments(self, stream=None) -> Status:
+        return _status(_lib.gwn_batch_refit_moments(self._h, _stream(stream)))
+
+    def query(self, stream=None) -> Status:
+        return _status(_lib.gwn_batch_query(self._h, _stream(stream)))
+
+    def tune(self, stream=None) -> float:
+        ms = ctypes.c_float()
+        _status(_lib.gwn_batch_tune(self._h, _stream(stream), ctypes.byref(ms))).raise_if_error()
+        return float(ms.value)
+
+    def tuning(self) -> dict:
+        t = _BatchTuning()
+        _status(_lib.gwn_batch_get_tuning(self._h, ctypes.byref(t))).raise_if_error()
+        return {k: int(getattr(t, k)) for k, _ in _BatchTuning._fields_}
+
+    def set_tuning(self, block_size: int, chunk: int, blocks_per_cu: int, traversal: int = 0,
+                   stream=None) -> Status:
+        """traversal 0: one query per lane; 1: wave-cooperative over each mesh's Morton order."""
+        t = _BatchTuning(block_size, chunk, blocks_per_cu, traversal, 0, 0, 0)
+        return _status(_lib.gwn_batch_set_tuning(self._h, ctypes.byref(t), _stream(stream)))
+
+    def capture(self, include_refit: bool = True) -> Status:
+        return _status(_lib.gwn_batch_capture(self._h, 1 if include_refit else 0))
+
+    def replay(self, stream=None) -> Status:
+        return _status(_lib.gwn_batch_replay(self._h, _stream(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.gwn_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def gwn_compute_query_order(bvh: BvhObject, qx, qy, qz, order, stream=None) -> Status:

@@ -1,8 +1,11 @@
 """GPU adjoint batches against the FP64 adjoint oracle (which central differences pin).
 
-The device re-traverses with the forward's decisions and accumulates FP32 terms with atomics, so
-it matches the FP64 oracle to FP32 rounding of sums: vertex gradients within 1e-4 of the largest
-vertex gradient magnitude, query gradients within 1e-4 relative.
+The device re-traverses with the forward's decisions and accumulates FP32 terms with atomics.
+Near the surface the terms are large (~1/d^2) and cancel (on a closed mesh the exact vertex
+gradient is zero), so the FP32 result differs from the FP64 one by the rounding of its terms, not
+of its value.  Each entry is therefore checked against
+    1e-4 * max|ref| + ENV * eps32 * (sum of |term| over all triangles and queries),
+the magnitude sum taken in FP64 by the oracle (brute force, every term exact).
 """
 
 import numpy as np
@@ -29,6 +32,22 @@ def _objects(gwn, mesh, order):
     return geom, bvh, mom
 
 
+EPS32 = float(np.finfo(np.float32).eps)
+ENV = 128.0  # ulps of the magnitude sum
+
+
+def _check_winding_adjoint(orc, mesh, q, u, dv, dq, rdv, rdq, calls=1):
+    mv, mq = orc.adjoint_magnitude_f64(mesh, *q, u)
+    scale = np.abs(rdv).max()
+    ev, eq = np.abs(dv - rdv), np.abs(dq - rdq)
+    bound_v = 1e-4 * scale + ENV * EPS32 * calls * mv
+    bound_q = 1e-4 * (1 + np.abs(rdq)) + ENV * EPS32 * calls * mq
+    print(f"  dV: max err {ev.max():.3e} of {scale:.3e}, max err/(eps mag) {(ev / (EPS32 * calls * mv + 1e-300)).max():.2f}; "
+          f"dq: max err/(eps mag) {(eq / (EPS32 * calls * mq + 1e-300)).max():.2f}", flush=True)
+    assert (ev <= bound_v).all(), f"dV: {int((ev > bound_v).sum())} entries beyond the bound"
+    assert (eq <= bound_q).all(), f"dq: {int((eq > bound_q).sum())} entries beyond the bound"
+
+
 def _winding_adjoint(gwn, dev, geom, bvh, mom, q, u, beta):
     import torch
 
@@ -53,9 +72,7 @@ def test_winding_adjoint_matches_oracle(gwn, orc, dev, name, order, beta):
     dv, dq = _winding_adjoint(gwn, dev, geom, bvh, mom, q, u, beta)
     rb = orc.build_bvh4(mesh)
     rdv, rdq = orc.winding_taylor_adjoint_f64(rb, orc.refit_moment(rb, order), order, beta, mesh, *q, u)
-    scale = np.abs(rdv).max()
-    assert np.abs(dv - rdv).max() <= 1e-4 * scale, f"max |dV - ref| {np.abs(dv - rdv).max():.3e} of {scale:.3e}"
-    assert np.all(np.abs(dq - rdq) <= 1e-4 * (1 + np.abs(rdq)))
+    _check_winding_adjoint(orc, mesh, q, u, dv, dq, rdv, rdq)
 
 
 def test_winding_adjoint_large_batch_and_accumulation(gwn, orc, dev):
@@ -76,8 +93,8 @@ def test_winding_adjoint_large_batch_and_accumulation(gwn, orc, dev):
     got = np.stack([a.cpu().numpy() for a in dv])
     rb = orc.build_bvh4(mesh)
     rdv, rdq = orc.winding_taylor_adjoint_f64(rb, orc.refit_moment(rb, 1), 1, 2.0, mesh, *q, u)
-    assert np.abs(got - 2 * rdv).max() <= 2e-4 * np.abs(rdv).max()
-    assert np.all(np.abs(np.stack([a.cpu().numpy() for a in dq], 1) - rdq) <= 1e-4 * (1 + np.abs(rdq)))
+    # dq is written (not added) by each call; dV adds both calls' terms.
+    _check_winding_adjoint(orc, mesh, q, u, got, np.stack([a.cpu().numpy() for a in dq], 1), 2 * rdv, rdq, calls=2)
 
 
 def test_distance_adjoint_matches_oracle(gwn, orc, dev):

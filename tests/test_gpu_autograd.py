@@ -52,7 +52,7 @@ def test_autograd_loss_gradient_matches_oracle(gwn, orc, dev, order):
     assert np.abs(gq - rdq).max() <= 1e-3 * (1e-12 + np.abs(rdq).max())
 
 
-def test_scene_refits_once_per_vertex_state_and_detects_stale_backward(gwn, dev):
+def test_scene_refits_once_per_vertex_state_and_detects_stale_backward(gwn, orc, dev):
     import torch
 
     from smallgwn_amd import autograd as ag
@@ -60,14 +60,19 @@ def test_scene_refits_once_per_vertex_state_and_detects_stale_backward(gwn, dev)
     mesh = synth.icosphere(2)
     faces = np.stack([mesh.i0, mesh.i1, mesh.i2], 1)
     v = torch.tensor(np.stack([mesh.vx, mesh.vy, mesh.vz], 1), device=dev, requires_grad=True)
-    scene = ag.MeshScene(faces, v, order=1)
+    scene = ag.MeshScene(faces, v, order=1)  # H-PLOC tree, search radius 8
     q = torch.tensor([[0.0, 0.0, 0.0], [2.0, 0.0, 0.0]], device=dev)
     w = ag.taylor_winding(v, q, scene)
-    assert scene.stamp == 0 and abs(float(w[0]) - 1.0) < 1e-3 and abs(float(w[1])) < 1e-3
-    # A new vertex state (scaled sphere) is refit once; the stale graph refuses backward.
+    rb = orc.build_bvh4(mesh, "hploc", 8)
+    ref = orc.winding_taylor_f32(rb, orc.refit_moment(rb, 1), 1, np.float32([0, 2]), np.float32([0, 0]),
+                                 np.float32([0, 0]), accuracy_scale=2.0)
+    got = w.detach().cpu().numpy()
+    assert scene.stamp == 0 and np.all(np.abs(got - ref) <= 1e-6 * (1 + np.abs(ref)))
+    # A new vertex state (the sphere scaled by 1/2, exact in FP32) is refit once; the Taylor
+    # winding at the centre is scale invariant, and the stale graph refuses backward.
     v2 = (v * 0.5).detach().requires_grad_(True)
     w2 = ag.taylor_winding(v2, q, scene)
-    assert scene.stamp == 1 and abs(float(w2[0]) - 1.0) < 1e-3
+    assert scene.stamp == 1 and abs(float(w2[0]) - float(w[0])) <= 1e-6
     with pytest.raises(RuntimeError):
         w.sum().backward()
     w2.sum().backward()

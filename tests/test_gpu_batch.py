@@ -1,0 +1,132 @@
+"""Batched mode (gwn_batch_*): per-mesh results equal to sequential single-mesh Taylor batches with
+the same parameters (same trees, same accept/descend tests, same term order per query), in both
+traversals, before and after tuning, and across hipGraph replays with other GPU work in between.
+The batch kernel is a separate instantiation of the same term code and the compiler contracts
+products into FMAs differently in each, so every far-field term may differ by an ulp; over the
+~1000 terms of a query that is a few 1e-6 of |w| (1e-5 * (1 + |w|) allowed)."""
+
+import numpy as np
+import pytest
+
+from smallgwn_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _affine(mesh, seed):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    scale = rng.uniform(0.5, 2.0)
+    shift = rng.uniform(-3, 3, size=3)
+    v = np.stack([mesh.vx, mesh.vy, mesh.vz], 1).astype(np.float64) @ q.T * scale + shift
+    return synth._mesh_from_arrays(v.astype(np.float32), np.stack([mesh.i0, mesh.i1, mesh.i2], 1))
+
+
+def _meshes(count, seed=0):
+    rng = np.random.default_rng(seed)
+    out = [synth._mesh_from_arrays(np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0]]), [[0, 1, 2]])]
+    for k in range(count - 1):
+        tris = int(np.exp(rng.uniform(np.log(200), np.log(20000))))
+        if k % 2 == 0:
+            nu = max(3, int(np.sqrt(tris / 2)))
+            base = synth.torus(nu, max(3, tris // (2 * nu)))
+        else:
+            nlon = max(4, int(np.sqrt(tris / 2)))
+            base = synth.displaced_sphere(nlon, max(3, tris // (2 * nlon)), seed=k)
+        out.append(_affine(base, 100 + k))
+    return out
+
+
+def _queries(mesh, n, seed):
+    v = np.stack([mesh.vx, mesh.vy, mesh.vz], 1)
+    lo, hi = v.min(0), v.max(0)
+    c, e = (lo + hi) / 2, (hi - lo) / 2 * 1.5 + 1e-3
+    return synth.uniform_box_points(seed, n, c - e, c + e)
+
+
+def _setup(gwn, dev, count, order=1, seed=0):
+    import torch
+
+    rng = np.random.default_rng(seed + 1)
+    items, refs = [], []
+    for k, mesh in enumerate(_meshes(count, seed)):
+        geom = gwn.GeometryObject()
+        gwn.gwn_upload_geometry(geom, mesh.vx, mesh.vy, mesh.vz, mesh.i0, mesh.i1, mesh.i2).raise_if_error()
+        bvh = gwn.BvhObject()
+        method = gwn.BuildMethod.HPLOC if k % 3 else gwn.BuildMethod.LBVH
+        gwn.gwn_build_bvh(geom, bvh, gwn.BuildOptions(method=method)).raise_if_error()
+        mom = gwn.MomentObject(order)
+        gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+        n = [0, 1, 63, 500, 70000][k] if k < 5 else int(np.exp(rng.uniform(np.log(64), np.log(20000))))
+        q = [torch.from_numpy(a).to(dev) for a in _queries(mesh, n, 50 + k)]
+        out = torch.full((n,), float("nan"), dtype=torch.float32, device=dev)
+        ref = torch.empty_like(out)
+        gwn.gwn_compute_winding_number_taylor_batch(bvh, mom, *q, ref, accuracy_scale=2.0).raise_if_error()
+        items.append((bvh, mom, q[0], q[1], q[2], out))
+        refs.append((geom, ref))
+    torch.cuda.synchronize()
+    return items, refs
+
+
+def _check(items, refs):
+    import torch
+
+    torch.cuda.synchronize()
+    for k, ((_, _, _, _, _, out), (_, ref)) in enumerate(zip(items, refs)):
+        got, want = out.cpu().numpy().astype(np.float64), ref.cpu().numpy().astype(np.float64)
+        assert np.isfinite(got).all() == np.isfinite(want).all(), f"mesh {k}: NaN pattern"
+        assert (np.abs(got - want) <= 1e-5 * (1 + np.abs(want))).all(), \
+            f"mesh {k}: max diff {np.abs(got - want).max():.3e}"
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_batch_matches_sequential_calls(gwn, dev, order):
+    items, refs = _setup(gwn, dev, 24, order=order)
+    batch = gwn.BatchObject(items, accuracy_scale=2.0)
+    batch.query().raise_if_error()
+    _check(items, refs)
+    t = batch.tuning()
+    assert t["meshes"] == 24 and t["queries"] == sum(int(it[2].numel()) for it in items)
+
+
+def test_batch_tuned_and_graph_replays(gwn, dev):
+    import torch
+
+    items, refs = _setup(gwn, dev, 40, order=1, seed=3)
+    batch = gwn.BatchObject(items, accuracy_scale=2.0)
+    ms = batch.tune()
+    assert ms > 0.0
+    t = batch.tuning()
+    assert t["block_size"] in (64, 128, 256) and t["chunk"] % t["block_size"] == 0 and t["traversal"] in (0, 1)
+    batch.query().raise_if_error()
+    _check(items, refs)
+    batch.capture(include_refit=True).raise_if_error()
+    for rep in range(3):
+        for it in items:
+            it[5].fill_(float("nan"))  # eager work between replays
+        torch.cuda.synchronize()
+        batch.replay().raise_if_error()
+        _check(items, refs)
+
+
+def test_batch_other_tunings_and_validation(gwn, dev):
+    items, refs = _setup(gwn, dev, 12, order=1, seed=5)
+    batch = gwn.BatchObject(items, accuracy_scale=2.0)
+    for block, chunk, bpc, trav in ((64, 64, 1, 0), (128, 512, 2, 0), (256, 1024, 8, 0),
+                                    (64, 64, 1, 1), (128, 256, 4, 1), (256, 1024, 2, 1)):
+        batch.set_tuning(block, chunk, bpc, trav).raise_if_error()
+        assert batch.tuning()["traversal"] == trav
+        for it in items:
+            it[5].fill_(float("nan"))
+        batch.query().raise_if_error()
+        _check(items, refs)
+    assert batch.set_tuning(100, 100, 1).error == gwn.Error.INVALID_ARGUMENT
+    assert batch.set_tuning(64, 96, 1).error == gwn.Error.INVALID_ARGUMENT
+    assert batch.set_tuning(64, 64, 1, 2).error == gwn.Error.INVALID_ARGUMENT
+    assert batch.replay().error == gwn.Error.INVALID_ARGUMENT  # nothing captured
+    mixed = list(items[:2])
+    other = gwn.MomentObject(2)
+    gwn.gwn_refit_bvh_moment(mixed[1][0], other).raise_if_error()
+    mixed[1] = (mixed[1][0], other) + tuple(mixed[1][2:])
+    with pytest.raises(Exception):
+        gwn.BatchObject(mixed)

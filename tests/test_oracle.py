@@ -1174,3 +1174,37 @@ def test_distance_adjoint_matches_central_differences(orc):
     assert np.allclose(np.linalg.norm(dp / u[:, None], axis=1), 1.0, atol=1e-12)
     ref = orc.unsigned_distance_f32(orc.build_bvh4(mesh), px, py, pz)
     assert np.abs(dist - ref).max() <= 1e-6
+
+
+def test_adjoint_magnitude_single_terms_and_additivity(orc):
+    """The envelope sums |u| |dW_t/dv| and |u| |dW_t/dq|: for one triangle and one query they are the
+    absolute central differences of the FP64 solid angle and the absolute exact gradient; over a
+    mesh they add triangle by triangle and scale with |u|."""
+    tri = np.float64([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.2, 0.9, 0.1]])
+    one = synth._mesh_from_arrays(tri.astype(np.float32), [[0, 1, 2]])
+    V = np.stack([one.vx, one.vy, one.vz]).astype(np.float64)
+    q = np.float64([0.3, 0.2, 0.4])
+    u = np.float64([-1.5])
+    mv, mq = orc.adjoint_magnitude_f64(one, *(np.float32([c]) for c in q), u)
+    qf = q.astype(np.float32).astype(np.float64)
+    h = 1e-6
+    for vi in range(3):
+        for k in range(3):
+            Vp, Vm = V.copy(), V.copy()
+            Vp[k, vi] += h
+            Vm[k, vi] -= h
+            fd = (orc.solid_angle_f64(*Vp.T, qf) - orc.solid_angle_f64(*Vm.T, qf)) / (2 * h) / (4 * math.pi)
+            assert abs(mv[k, vi] - 1.5 * abs(fd)) <= 1e-7 * (1 + abs(fd))
+    g = np.array(orc.winding_gradient_exact_mesh_f64(one, *(np.float32([c]) for c in q))).ravel()
+    assert np.allclose(mq[0], 1.5 * np.abs(g), rtol=1e-12, atol=0)
+    # Two disjoint copies: per-vertex sums equal the single-triangle sums, per-query sums add.
+    two = synth._mesh_from_arrays(np.concatenate([tri, tri + [0.0, 0.0, 2.0]]).astype(np.float32),
+                                  [[0, 1, 2], [3, 4, 5]])
+    q2 = synth.uniform_box_points(3, 16, (-1.0,) * 3, (1.0,) * 3)
+    u2 = np.random.default_rng(2).normal(size=16)
+    mv2, mq2 = orc.adjoint_magnitude_f64(two, *q2, u2)
+    parts = [synth._mesh_from_arrays((tri + [0.0, 0.0, 2.0 * s]).astype(np.float32), [[0, 1, 2]]) for s in (0, 1)]
+    m = [orc.adjoint_magnitude_f64(p, *q2, u2) for p in parts]
+    assert np.allclose(mv2, np.concatenate([m[0][0], m[1][0]], 1), rtol=1e-12, atol=0)
+    assert np.allclose(mq2, m[0][1] + m[1][1], rtol=1e-12, atol=0)
+    assert np.allclose(orc.adjoint_magnitude_f64(two, *q2, -3 * u2)[0], 3 * mv2, rtol=1e-12, atol=0)
