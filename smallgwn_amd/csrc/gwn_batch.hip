@@ -20,8 +20,15 @@
 #include <algorithm>
 #include <vector>
 
+// Device storage a batch was made over, per mesh: a rebuild or re-upload swaps these buffers.
+struct gwn_batch_storage {
+    void const *nodes, *triangles, *records;
+    std::size_t node_count;
+};
+
 struct gwn_batch_handle {
     std::vector<gwn_bvh_handle const *> bvhs;
+    std::vector<gwn_batch_storage> storage;
     std::vector<gwn_moment_handle *> moments;
     std::vector<gwn::batch_mesh> host_meshes;
     std::vector<gwn::moment_refit_work> work;
@@ -41,6 +48,7 @@ struct gwn_batch_handle {
     hipStream_t own_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool graph_refits = false;
 
     ~gwn_batch_handle() {
         if (exec)
@@ -72,16 +80,36 @@ void upload_table(gwn_batch_handle &b, hipStream_t stream) {
     GWN_HIP(hipStreamSynchronize(stream));
 }
 
+gwn_batch_storage storage_of(gwn_bvh_handle const &bvh, gwn_moment_handle const &m) {
+    return {bvh.nodes.ptr, bvh.triangles.ptr, m.records.ptr, bvh.node_count};
+}
+
+// Captured graphs and the mesh table hold raw device pointers: refuse to run over a mesh whose
+// BVH or moment storage was replaced (rebuilt, re-uploaded) after the batch was made.
+void check_storage(gwn_batch_handle const &b) {
+    for (std::size_t i = 0; i < b.bvhs.size(); ++i) {
+        gwn_batch_storage const now = storage_of(*b.bvhs[i], *b.moments[i]);
+        gwn_batch_storage const &was = b.storage[i];
+        if (now.nodes != was.nodes || now.triangles != was.triangles || now.records != was.records ||
+            now.node_count != was.node_count)
+            throw_invalid("batch: A mesh's BVH or moment storage changed since the batch was made; create a new batch.");
+        if (b.bvhs[i]->revision == 0)
+            throw_invalid("batch: A mesh's BVH is being refit; its bounds are not queryable yet.");
+    }
+}
+
 } // namespace
 
 void batch_destroy(gwn_batch_handle *b) { delete b; }
 
 void batch_refit(gwn_batch_handle &b, hipStream_t stream) {
+    check_storage(b);
     for (std::size_t i = 0; i < b.bvhs.size(); ++i)
         refit_moment_enqueue(*b.bvhs[i], *b.moments[i], b.work[i], stream);
 }
 
 void batch_query(gwn_batch_handle &b, hipStream_t stream) {
+    check_storage(b);
     if (b.total_chunks == 0)
         return;
     fill_u32(reinterpret_cast<std::uint32_t *>(b.counter.ptr), 0u, 2, stream);
@@ -116,6 +144,7 @@ gwn_batch_handle *batch_create(
         for (std::size_t i = 0; i < bvhs.size(); ++i) {
             refit_moment_prepare(*bvhs[i], *moments[i], b->work[i], stream);
             bound = std::max(bound, bvhs[i]->internal_stack_bound);
+            b->storage.push_back(storage_of(*bvhs[i], *moments[i]));
         }
         b->cap = bound <= 32 ? 32 : (bound <= 64 ? 64 : 128);
         if (bound > 128)
@@ -235,6 +264,7 @@ void batch_get_tuning(gwn_batch_handle const &b, gwn_batch_tuning &t) {
 }
 
 void batch_capture(gwn_batch_handle &b, bool include_refit) {
+    check_storage(b);
     batch_drop_graph(b);
     if (b.own_stream == nullptr)
         GWN_HIP(hipStreamCreateWithFlags(&b.own_stream, hipStreamNonBlocking));
@@ -252,12 +282,18 @@ void batch_capture(gwn_batch_handle &b, bool include_refit) {
     }
     GWN_HIP(hipStreamEndCapture(b.own_stream, &b.graph));
     GWN_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+    b.graph_refits = include_refit;
 }
 
 void batch_replay(gwn_batch_handle &b, hipStream_t stream) {
     if (b.exec == nullptr)
         throw_invalid("batch: No captured graph; call gwn_batch_capture first.");
+    check_storage(b);
     GWN_HIP(hipGraphLaunch(b.exec, stream));
+    // The graph's refits form every moment from its BVH's current bounds, as an eager refit does.
+    if (b.graph_refits)
+        for (std::size_t i = 0; i < b.bvhs.size(); ++i)
+            b.moments[i]->bvh_revision = b.bvhs[i]->revision;
 }
 
 } // namespace gwn

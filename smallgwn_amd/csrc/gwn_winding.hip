@@ -972,6 +972,204 @@ void launch_taylor(
         throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
 }
 
+// Sum of v over the 64 lanes of the wave, returned in every lane.  DPP steps: swap within pairs
+// and quads, half-row and row mirrors (sums of 8 and 16 lanes), then row 15 -> next row and lane 31
+// -> rows 2-3 broadcasts leave the total in lane 63.  Every lane must be active.
+template <int Ctrl, int RowMask>
+__device__ inline float dpp_f32(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), Ctrl, RowMask, 0xf, false));
+}
+
+__device__ inline float wave_sum(float v) {
+    v += dpp_f32<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_f32<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_f32<0x141, 0xf>(v);  // row_half_mirror
+    v += dpp_f32<0x140, 0xf>(v);  // row_mirror
+    v += dpp_f32<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f32<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+
+// Adds the wave's per-lane values f[0..F) into d[0..F): one atomic per value from one lane when
+// several lanes contribute, the contributing lane's own atomics when only one does.
+template <int F>
+__device__ inline void wave_atomic_add(float *__restrict__ d, float const (&f)[F], std::uint64_t mask) {
+    int const lane = threadIdx.x & 63;
+    if (__builtin_popcountll(mask) == 1) {
+        if (mask & (1ull << lane)) {
+#pragma unroll
+            for (int k = 0; k < F; ++k)
+                atomicAdd(d + k, f[k]);
+        }
+        return;
+    }
+    float sum[F];
+#pragma unroll
+    for (int k = 0; k < F; ++k)
+        sum[k] = wave_sum(f[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < F; ++k)
+            atomicAdd(d + k, sum[k]);
+    }
+}
+
+// The adjoint of taylor_adjoint_kernel, wave-cooperative: a wave walks the tree once for its 64
+// (Morton-ordered) queries with the same per-query accept/descend decisions, and the field and
+// vertex adjoints of a child slot or near triangle are summed over the wave before one lane adds
+// them, instead of every query adding its own (the top slots are shared by every query, so
+// per-query atomics serialise on them).
+template <int Order, int StackCap>
+__global__ __launch_bounds__(k_taylor_block) void taylor_adjoint_wave_kernel(
+    float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node,
+    float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
+    float const *__restrict__ u, std::uint32_t const *__restrict__ order, std::size_t n, float beta2,
+    float *__restrict__ dslot, float *__restrict__ dtri, float *__restrict__ dqx, float *__restrict__ dqy,
+    float *__restrict__ dqz
+) {
+    static_assert(Order <= 1, "adjoint orders are 0 and 1");
+    constexpr int RV = moment_record_vec4(Order);
+    constexpr int F = Order >= 1 ? 12 : 6;
+    constexpr int k_waves = k_taylor_block / 64;
+    __shared__ std::uint32_t s_node[k_waves][StackCap];
+    __shared__ std::uint64_t s_mask[k_waves][StackCap];
+    int const wave = threadIdx.x >> 6;
+    int const lane = threadIdx.x & 63;
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_taylor_block + threadIdx.x;
+    bool const active = i < n;
+    std::size_t const q = active ? (order ? order[i] : i) : 0;
+    float const x = active ? qx[q] : 0.0f;
+    float const y = active ? qy[q] : 0.0f;
+    float const z = active ? qz[q] : 0.0f;
+    float const su = active ? u[q] * k_inv_4pi_f : 0.0f;
+    double const sud = active ? double(u[q]) * (1.0 / (4.0 * k_pi_d)) : 0.0;
+    std::uint64_t mask = __ballot(active);
+    if (mask == 0ull)
+        return;
+    std::uint64_t const my_bit = 1ull << lane;
+    double dq[3] = {0.0, 0.0, 0.0};
+    std::uint32_t node = root_node;
+    int sp = 0;
+    bool overflow = false;  // wave-uniform
+    while (node != k_invalid && !overflow) {
+        node = __builtin_amdgcn_readfirstlane(node);
+        float4 const *rec = records + static_cast<std::size_t>(node) * RV;
+        float4 const refs = rec[0];
+        float4 const mp = rec[1], px = rec[2], py = rec[3], pz = rec[4];
+        float const *fields = reinterpret_cast<float const *>(rec + 1);
+        std::uint32_t const cref[4] = {__float_as_uint(refs.x), __float_as_uint(refs.y),
+                                       __float_as_uint(refs.z), __float_as_uint(refs.w)};
+        float const mpd[4] = {mp.x, mp.y, mp.z, mp.w};
+        float const cx[4] = {px.x, px.y, px.z, px.w};
+        float const cy[4] = {py.x, py.y, py.z, py.w};
+        float const cz[4] = {pz.x, pz.y, pz.z, pz.w};
+        bool const in = (mask & my_bit) != 0ull;
+        std::uint32_t next = k_invalid;
+        std::uint64_t next_mask = 0ull;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            std::uint32_t const c = __builtin_amdgcn_readfirstlane(cref[s]);
+            if (c == k_invalid || overflow)
+                continue;
+            bool descend = false, accepted = false;
+            float f[F];
+#pragma unroll
+            for (int k = 0; k < F; ++k)
+                f[k] = 0.0f;
+            if (in) {
+                float const qrx = x - cx[s], qry = y - cy[s], qrz = z - cz[s];
+                float const ql2 = squared_norm_unfused(qrx, qry, qrz);
+                descend = !(ql2 > 0.0f) || ql2 <= product_unfused(mpd[s], beta2);
+                if (!descend) {
+                    float g[3];
+                    float const w = taylor_term<Order, true>(fields, s, qrx, qry, qrz, ql2, g);
+                    if (isfinite(w)) {
+                        accepted = true;
+                        float const m1 = __builtin_amdgcn_rsqf(ql2);
+                        float const m2 = m1 * m1;
+                        float const qn[3] = {qrx * m1, qry * m1, qrz * m1};
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            f[k] = -su * g[k];
+                            f[3 + k] = -su * m2 * qn[k];
+                            dq[k] += double(su * g[k]);
+                        }
+                        if constexpr (Order >= 1) {
+                            float const m3 = m2 * m1;
+                            f[6] = su * m3 * (1.0f - 3.0f * qn[0] * qn[0]);
+                            f[7] = su * m3 * (1.0f - 3.0f * qn[1] * qn[1]);
+                            f[8] = su * m3 * (1.0f - 3.0f * qn[2] * qn[2]);
+                            f[9] = -3.0f * su * m3 * qn[0] * qn[1];
+                            f[10] = -3.0f * su * m3 * qn[1] * qn[2];
+                            f[11] = -3.0f * su * m3 * qn[2] * qn[0];
+                        }
+                    } else {
+                        descend = true;
+                    }
+                }
+            }
+            std::uint64_t const amask = __ballot(accepted);
+            if (amask != 0ull)
+                wave_atomic_add<F>(dslot + (static_cast<std::size_t>(node) * 4 + s) * 12, f, amask);
+            std::uint64_t const dmask = __ballot(descend);
+            if (dmask == 0ull)
+                continue;
+            if (c & k_cref_leaf) {
+                std::uint32_t const r = c & ~k_cref_leaf;
+                float t9[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k)
+                    t9[k] = 0.0f;
+                if (dmask & my_bit) {
+                    tri_record const t = tris[r];
+                    double const qd[3] = {x, y, z};
+                    double const a[3] = {double(t.v0x) - qd[0], double(t.v0y) - qd[1], double(t.v0z) - qd[2]};
+                    double const b[3] = {double(t.v1x) - qd[0], double(t.v1y) - qd[1], double(t.v1z) - qd[2]};
+                    double const cc[3] = {double(t.v2x) - qd[0], double(t.v2y) - qd[1], double(t.v2z) - qd[2]};
+                    double da[3], db[3], dc[3];
+                    solid_angle_vertex_grad(a, b, cc, da, db, dc);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        t9[k] = static_cast<float>(sud * da[k]);
+                        t9[3 + k] = static_cast<float>(sud * db[k]);
+                        t9[6 + k] = static_cast<float>(sud * dc[k]);
+                        dq[k] -= sud * (da[k] + db[k] + dc[k]);
+                    }
+                }
+                wave_atomic_add<9>(dtri + static_cast<std::size_t>(r) * 9, t9, dmask);
+                continue;
+            }
+            if (next != k_invalid) {
+                if (sp < StackCap) {
+                    s_node[wave][sp] = next;
+                    s_mask[wave][sp] = next_mask;
+                    ++sp;
+                } else {
+                    overflow = true;
+                }
+            }
+            next = c;
+            next_mask = dmask;
+        }
+        if (next == k_invalid && sp > 0) {
+            --sp;
+            next = s_node[wave][sp];
+            next_mask = s_mask[wave][sp];
+        }
+        node = next;
+        mask = next_mask;
+    }
+    if (!active)
+        return;
+    if (overflow)
+        dq[0] = dq[1] = dq[2] = __builtin_nan("");
+    if (dqx != nullptr) {
+        dqx[q] = static_cast<float>(dq[0]);
+        dqy[q] = static_cast<float>(dq[1]);
+        dqz[q] = static_cast<float>(dq[2]);
+    }
+}
+
 template <int Order>
 void launch_taylor_adjoint(
     gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy, float const *qz,
@@ -982,10 +1180,17 @@ void launch_taylor_adjoint(
     std::uint32_t const root = leaf_root ? k_invalid : static_cast<std::uint32_t>(ref_offset(bvh.root.reference));
     int const need = leaf_root ? 0 : static_cast<int>(bvh.internal_stack_bound);
     float4 const *records = leaf_root ? nullptr : m.records.ptr;
+    // Morton-ordered queries (large batches) take the wave-cooperative form.
 #define GWN_ADJ(C)                                                                                           \
-    taylor_adjoint_kernel<Order, C><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(            \
-        records, bvh.triangles.ptr, root, bvh.root.reference, qx, qy, qz, u, order, n, beta2, dslot, dtri, dqx, \
-        dqy, dqz)
+    do {                                                                                                     \
+        if (order != nullptr && !leaf_root)                                                                  \
+            taylor_adjoint_wave_kernel<Order, C><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>( \
+                records, bvh.triangles.ptr, root, qx, qy, qz, u, order, n, beta2, dslot, dtri, dqx, dqy, dqz);  \
+        else                                                                                                 \
+            taylor_adjoint_kernel<Order, C><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(    \
+                records, bvh.triangles.ptr, root, bvh.root.reference, qx, qy, qz, u, order, n, beta2, dslot,   \
+                dtri, dqx, dqy, dqz);                                                                        \
+    } while (0)
     if (need <= 32)
         GWN_ADJ(32);
     else if (need <= 64)

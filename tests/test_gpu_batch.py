@@ -1,9 +1,8 @@
 """Batched mode (gwn_batch_*): per-mesh results equal to sequential single-mesh Taylor batches with
 the same parameters (same trees, same accept/descend tests, same term order per query), in both
 traversals, before and after tuning, and across hipGraph replays with other GPU work in between.
-The batch kernel is a separate instantiation of the same term code and the compiler contracts
-products into FMAs differently in each, so every far-field term may differ by an ulp; over the
-~1000 terms of a query that is a few 1e-6 of |w| (1e-5 * (1 + |w|) allowed)."""
+The batch kernel is a separate instantiation of the same term code, so the comparison allows
+FP32 rounding differences of the final sum (the compiler may contract products differently)."""
 
 import numpy as np
 import pytest
@@ -75,7 +74,7 @@ def _check(items, refs):
     for k, ((_, _, _, _, _, out), (_, ref)) in enumerate(zip(items, refs)):
         got, want = out.cpu().numpy().astype(np.float64), ref.cpu().numpy().astype(np.float64)
         assert np.isfinite(got).all() == np.isfinite(want).all(), f"mesh {k}: NaN pattern"
-        assert (np.abs(got - want) <= 1e-5 * (1 + np.abs(want))).all(), \
+        assert (np.abs(got - want) <= 2e-6 * (1 + np.abs(want))).all(), \
             f"mesh {k}: max diff {np.abs(got - want).max():.3e}"
 
 
@@ -130,3 +129,25 @@ def test_batch_other_tunings_and_validation(gwn, dev):
     mixed[1] = (mixed[1][0], other) + tuple(mixed[1][2:])
     with pytest.raises(Exception):
         gwn.BatchObject(mixed)
+
+
+def test_batch_refuses_rebuilt_mesh_and_tracks_revisions(gwn, dev):
+    """A replay after a mesh was rebuilt is refused (the captured pointers would be stale); a
+    replay with refits leaves every moment usable with its BVH's current revision."""
+    import torch
+
+    items, refs = _setup(gwn, dev, 6, order=1, seed=9)
+    batch = gwn.BatchObject(items, accuracy_scale=2.0)
+    batch.capture(include_refit=True).raise_if_error()
+    geom = refs[2][0]
+    gwn.gwn_refit_bvh(geom, items[2][0]).raise_if_error()  # bounds refit: new revision, same storage
+    batch.replay().raise_if_error()
+    torch.cuda.synchronize()
+    _check(items, refs)
+    out = torch.empty_like(items[2][5])
+    gwn.gwn_compute_winding_number_taylor_batch(items[2][0], items[2][1], *items[2][2:5], out,
+                                                accuracy_scale=2.0).raise_if_error()
+    gwn.gwn_build_bvh(geom, items[2][0]).raise_if_error()  # rebuild: new storage
+    st = batch.replay()
+    assert st.error == gwn.Error.INVALID_ARGUMENT and "storage changed" in st.message
+    assert batch.query().error == gwn.Error.INVALID_ARGUMENT
