@@ -126,14 +126,17 @@ template <int StackCap>
 __device__ inline int distance_traverse(
     distance_scene const &sc, float x, float y, float z, float &best, std::uint32_t *best_rec
 ) {
+    // One loop exit, at the head (status flags instead of returns inside the unrolled loops): the
+    // gfx950 backend (ROCm 7.2) has lowered an early exit inside such a loop wrongly.
     std::uint32_t stack[StackCap];
     int sp = 0;
+    int status = k_dist_done;
     std::uint64_t reference = sc.root_ref;
-    for (;;) {
+    while (reference != 0 && status == k_dist_done) {
         std::uint64_t next = 0;
         if (ref_leaf(reference)) {
             if (!visit_leaf(sc.tris, sc.nt, reference, x, y, z, best, best_rec))
-                return k_dist_zero;
+                status = k_dist_zero;
         } else if (ref_internal(reference) && ref_offset(reference) < sc.node_count) {
             bvh_node4 const &node = sc.nodes[ref_offset(reference)];
             float cd[4];
@@ -175,30 +178,28 @@ __device__ inline int distance_traverse(
             cswap(1, 2);
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                if (s < m && ref_leaf(cr[s]) && cd[s] < best &&
+                if (status == k_dist_done && s < m && ref_leaf(cr[s]) && cd[s] < best &&
                     !visit_leaf(sc.tris, sc.nt, cr[s], x, y, z, best, best_rec))
-                    return k_dist_zero;
+                    status = k_dist_zero;
             }
 #pragma unroll
             for (int s = 3; s >= 0; --s) {
-                if (s >= m || !ref_internal(cr[s]) || cd[s] >= best)
+                if (status != k_dist_done || s >= m || !ref_internal(cr[s]) || cd[s] >= best)
                     continue;
                 if (next != 0) {
-                    if (sp >= StackCap)
-                        return k_dist_overflow;
-                    stack[sp++] = static_cast<std::uint32_t>(ref_offset(next));
+                    if (sp < StackCap)
+                        stack[sp++] = static_cast<std::uint32_t>(ref_offset(next));
+                    else
+                        status = k_dist_overflow;
                 }
                 next = cr[s];
             }
         }
-        if (next != 0) {
-            reference = next;
-            continue;
-        }
-        if (sp == 0)
-            return k_dist_done;
-        reference = encode_internal(stack[--sp]);
+        if (next == 0 && sp > 0)
+            next = encode_internal(stack[--sp]);
+        reference = next;
     }
+    return status;
 }
 
 template <int StackCap>

@@ -215,10 +215,14 @@ __device__ inline void sort4(float (&t)[4], Key (&k)[4]) {
 // Packed near-first traversal (:242-358).  Returns true on stack overflow.
 template <int Cap>
 __device__ inline bool traverse_packed(ray_scene const &sc, std::uint32_t root, ray_state const &r, float t_min, best_hit &best) {
+    // One loop exit, at the head: no return from inside the unrolled push loop (the gfx950 backend
+    // of ROCm 7.2 has lowered such early exits wrongly).
     std::uint32_t stack[Cap];
     int sp = 0;
+    bool overflow = false, more = true;
     std::uint32_t payload = root;
-    for (;;) {
+    while (more && !overflow) {
+        bool descended = false;
         if (payload & k_payload_leaf) {
             int const slot = static_cast<int>(payload & 3u);
             std::uint32_t const parent = (payload & ~k_payload_leaf) >> 2;
@@ -247,21 +251,26 @@ __device__ inline bool traverse_packed(ray_scene const &sc, std::uint32_t root, 
             sort4(et, ep);
 #pragma unroll
             for (int s = 3; s > 0; --s) {
-                if (et[s] == INFINITY)
+                if (et[s] == INFINITY || overflow)
                     continue;
-                if (sp >= Cap)
-                    return true;
-                stack[sp++] = ep[s];
+                if (sp < Cap)
+                    stack[sp++] = ep[s];
+                else
+                    overflow = true;
             }
             if (et[0] != INFINITY) {
                 payload = ep[0];
-                continue;
+                descended = true;
             }
         }
-        if (sp == 0)
-            return false;
-        payload = stack[--sp];
+        if (!descended) {
+            if (sp > 0)
+                payload = stack[--sp];
+            else
+                more = false;
+        }
     }
+    return overflow;
 }
 
 // Canonical traversal (:485-560).  Returns true on stack overflow.
@@ -269,8 +278,9 @@ template <int Cap>
 __device__ inline bool traverse_canonical(ray_scene const &sc, std::uint64_t root_ref, ray_state const &r, float t_min, best_hit &best) {
     std::uint32_t stack[Cap];
     int sp = 0;
+    bool overflow = false;
     std::uint64_t reference = root_ref;
-    for (;;) {
+    while (reference != 0 && !overflow) {
         std::uint64_t next = 0;
         if (ref_leaf(reference)) {
             visit_leaf(sc, r, reference, t_min, best);
@@ -305,24 +315,22 @@ __device__ inline bool traverse_canonical(ray_scene const &sc, std::uint64_t roo
                     visit_leaf(sc, r, er[s], t_min, best);
 #pragma unroll
             for (int s = 3; s >= 0; --s) {
-                if (s >= m || !ref_internal(er[s]) || et[s] > best.t)
+                if (s >= m || !ref_internal(er[s]) || et[s] > best.t || overflow)
                     continue;
                 if (next != 0) {
-                    if (sp >= Cap)
-                        return true;
-                    stack[sp++] = static_cast<std::uint32_t>(ref_offset(next));
+                    if (sp < Cap)
+                        stack[sp++] = static_cast<std::uint32_t>(ref_offset(next));
+                    else
+                        overflow = true;
                 }
                 next = er[s];
             }
         }
-        if (next != 0) {
-            reference = next;
-            continue;
-        }
-        if (sp == 0)
-            return false;
-        reference = encode_internal(stack[--sp]);
+        if (next == 0 && sp > 0)
+            next = encode_internal(stack[--sp]);
+        reference = next;
     }
+    return overflow;
 }
 
 template <bool Packed, int Cap>
