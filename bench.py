@@ -6,8 +6,9 @@ triangles; 16,777,216 query points, half uniform in [-1.5, 1.5]^3 and half withi
 surface along the face normal.  Seeded synthetic data (splitmix64 from 0xB3A9E5D4).
 
 One step = one Taylor order-2 winding batch over all 16M queries (Morton presort + traversal),
-FP32 traversal and far field, FP64 near-field triangle terms, at an accuracy scale that keeps
-max |dW| <= 1e-4 against the FP64 exact winding number on a seeded query subsample.  Inputs,
+FP32 traversal and far field, near-field triangle terms with FP64 numerators and denominators,
+at an accuracy scale (6.5) that keeps max |dW| <= 1e-4 against the FP64 exact winding number on a
+seeded, strided subsample of 262,144 queries.  Inputs,
 BVH and moments are resident in HBM before the timed region.
 
 Also reported (outside the timed steps): the exact winding batch with the same FP64 near field
@@ -52,8 +53,8 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--queries", type=int, default=16_777_216)
     p.add_argument("--order", type=int, default=2)
-    p.add_argument("--beta", type=float, default=7.0, help="Taylor accuracy scale")
-    p.add_argument("--check-queries", type=int, default=32_768, help="FP64 subsample for |dW|")
+    p.add_argument("--beta", type=float, default=6.5, help="Taylor accuracy scale")
+    p.add_argument("--check-queries", type=int, default=262_144, help="FP64 subsample for |dW|")
     p.add_argument("--exact-queries", type=int, default=32_768, help="subsample timed for the exact form")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline duration")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -146,9 +147,9 @@ def main() -> int:
     ).raise_if_error()
     c = cnt.view(n, 3).to(torch.int64).sum(0).cpu().numpy()
     tests, accepts, leaves = (int(v) for v in c)
-    fp32_flops = tests * FLOPS_CHILD_TEST + accepts * FLOPS_ACCEPT[args.order]
-    fp64_flops = leaves * FLOPS_TRIANGLE
-    fp32_equiv = fp32_flops + 2 * fp64_flops  # FP64 vector rate is half the FP32 rate
+    # Every near-field triangle at its FP32 weight: its FP64 part (numerator and denominator in the
+    # near zone) is not weighted up, so the figure is a lower bound on the FLOPs done.
+    fp32_equiv = tests * FLOPS_CHILD_TEST + accepts * FLOPS_ACCEPT[args.order] + leaves * FLOPS_TRIANGLE
     achieved_tflops = fp32_equiv / (t_trav * 1e-3) / 1e12
 
     # Accuracy against FP64 exact on a seeded, strided subsample (both query halves).
@@ -247,7 +248,11 @@ def main() -> int:
             "kernel": f"taylor_wave_kernel<{args.order}, split>",
             "kernel_ms": round(t_trav, 4),
             "flops_per_launch": fp32_equiv,
-            "note": "FP32-equivalent FLOPs (FP64 counted twice) from per-query work counters",
+            "valu_lane_utilization": valu_profile().get("valu_lane_utilization"),
+            "note": "FP32 FLOPs from per-query work counters (10 per child test, 79 per accepted "
+                    "order-2 expansion, 87 per near-field triangle); peak is MI355X's FP32 vector "
+                    "figure, which counts packed FP32, disabled on this platform (non-packed 78.6); "
+                    "lane utilization from the committed rocprofv3 --pmc summary",
         },
     }
 
@@ -391,6 +396,21 @@ def batched_bench(args, dev, stream) -> dict:
             "tuning": batch.tuning(), "bit_identical_to_sequential": bool(identical),
             "max_abs_diff_vs_sequential": max_diff,
             "step": "moment refit of every mesh + Taylor queries of every mesh"}
+
+
+def valu_profile() -> dict:
+    """VALU counters of the traversal kernel from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(REPO, "profiles", "taylor_wave_o2_large_valu.md")
+    out = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                for key in ("valu_lane_utilization", "valu_insts_per_wave"):
+                    if line.startswith(key + ":"):
+                        out[key] = float(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return out
 
 
 def traffic_bytes():
