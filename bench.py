@@ -61,6 +61,9 @@ def parse_args():
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation workload")
     p.add_argument("--adjoint-steps", type=int, default=5)
     p.add_argument("--no-batched", action="store_true", help="skip the batched-mode workload")
+    p.add_argument("--no-extras", action="store_true",
+                   help="skip the large-mesh extras (Taylor at scale 2, distance, rays) and the precision-split report")
+    p.add_argument("--builder", choices=("hploc", "lbvh"), default="hploc", help="BVH builder of the large mesh")
     p.add_argument("--batched-meshes", type=int, default=256)
     return p.parse_args()
 
@@ -87,7 +90,10 @@ def main() -> int:
     geom = gwn.GeometryObject()
     gwn.gwn_upload_geometry(geom, mesh.vx, mesh.vy, mesh.vz, mesh.i0, mesh.i1, mesh.i2).raise_if_error()
     bvh = gwn.BvhObject()
-    opts = gwn.BuildOptions(method=gwn.BuildMethod.LBVH)
+    if args.builder == "hploc":
+        opts = gwn.BuildOptions(method=gwn.BuildMethod.HPLOC, hploc_search_radius=8)
+    else:
+        opts = gwn.BuildOptions(method=gwn.BuildMethod.LBVH)
     gwn.gwn_build_bvh(geom, bvh, opts).raise_if_error()
     mom = gwn.MomentObject(args.order)
     gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
@@ -190,17 +196,21 @@ def main() -> int:
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / reps
 
-    build_s = timed(lambda: gwn.gwn_build_bvh(geom, bvh, opts).raise_if_error(), 5)
-    hploc_bvh = gwn.BvhObject()
+    # Both builders on their own objects (the large mesh's tree is rebuilt deterministically).
+    lbvh_bvh, hploc_bvh = gwn.BvhObject(), gwn.BvhObject()
+    lbvh_opts = gwn.BuildOptions(method=gwn.BuildMethod.LBVH)
     hploc_opts = gwn.BuildOptions(method=gwn.BuildMethod.HPLOC, hploc_search_radius=8)
+    lbvh_s = timed(lambda: gwn.gwn_build_bvh(geom, lbvh_bvh, lbvh_opts).raise_if_error(), 5)
     hploc_s = timed(lambda: gwn.gwn_build_bvh(geom, hploc_bvh, hploc_opts).raise_if_error(), 5)
-    hploc_info = hploc_bvh.info()
-    del hploc_bvh
-    mom = gwn.MomentObject(args.order)
-    refit_s = timed(lambda: gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error(), 5)
+    lbvh_info, hploc_info = lbvh_bvh.info(), hploc_bvh.info()
+    del lbvh_bvh, hploc_bvh
+    build_s = hploc_s if args.builder == "hploc" else lbvh_s
+    mom_t = gwn.MomentObject(args.order)
+    refit_s = timed(lambda: gwn.gwn_refit_bvh_moment(bvh, mom_t).raise_if_error(), 5)
+    del mom_t
     T = mesh.triangle_count
-    log(f"build {build_s * 1e3:.2f} ms ({T / build_s / 1e6:.1f} Mtri/s), H-PLOC {hploc_s * 1e3:.2f} ms "
-        f"({T / hploc_s / 1e6:.1f} Mtri/s), refit o{args.order} {refit_s * 1e3:.2f} ms")
+    log(f"build H-PLOC r8 {hploc_s * 1e3:.2f} ms ({T / hploc_s / 1e6:.1f} Mtri/s), LBVH {lbvh_s * 1e3:.2f} ms "
+        f"({T / lbvh_s / 1e6:.1f} Mtri/s), refit o{args.order} {refit_s * 1e3:.2f} ms")
 
     result = {
         "metric": METRIC,
@@ -222,8 +232,8 @@ def main() -> int:
             "near_surface_fraction": 0.5,
             "taylor_order": args.order,
             "accuracy_scale": args.beta,
-            "precision": "fp32 traversal/far field, fp64 near field",
-            "builder": "lbvh",
+            "precision": "fp32 traversal/far field, fp64 near-field numerator/denominator",
+            "builder": "hploc_r8" if args.builder == "hploc" else "lbvh",
             "bvh_width": 4,
         },
         "accuracy": {"max_abs_dw": max_err, "p99_abs_dw": p99_err, "checked_queries": int(idx.size),
@@ -231,12 +241,15 @@ def main() -> int:
         "exact_fp64_near_field": {"mqueries_per_s": exact_rate, "sampled_queries": me,
                                   "taylor_speedup": value / exact_rate},
         "stages_ms": {"query_presort": round(t_order, 4), "traversal": round(t_trav, 4)},
-        "build": {"method": "lbvh", "ms": build_s * 1e3, "mtri_per_s": T / build_s / 1e6,
+        "build": {"method": "hploc_r8" if args.builder == "hploc" else "lbvh", "ms": build_s * 1e3,
+                  "mtri_per_s": T / build_s / 1e6,
                   "refit_moment_ms": refit_s * 1e3, "refit_mtri_per_s": T / refit_s / 1e6,
-                  "nodes": info["node_count"], "internal_stack_bound": info["internal_stack_bound"],
                   "hploc_r8_ms": hploc_s * 1e3, "hploc_r8_mtri_per_s": T / hploc_s / 1e6,
                   "hploc_nodes": hploc_info["node_count"],
-                  "hploc_internal_stack_bound": hploc_info["internal_stack_bound"]},
+                  "hploc_internal_stack_bound": hploc_info["internal_stack_bound"],
+                  "lbvh_ms": lbvh_s * 1e3, "lbvh_mtri_per_s": T / lbvh_s / 1e6,
+                  "lbvh_nodes": lbvh_info["node_count"],
+                  "lbvh_internal_stack_bound": lbvh_info["internal_stack_bound"]},
         "work": {"child_tests": tests, "accepted": accepts, "near_field_triangles": leaves},
         "roofline": {
             "bound": "valu",
@@ -256,6 +269,10 @@ def main() -> int:
         },
     }
 
+    if not args.no_extras:
+        result["large_mesh_extras"] = large_extras(args, dev, stream, bvh, d, idx, ref)
+        result["precision_split"] = precision_report(args, dev, stream)
+
     if not args.no_adjoint:
         del d, out, order, cnt, sub, ref, esub, eout
         result["adjoint"] = adjoint_bench(args, dev, stream)
@@ -268,6 +285,113 @@ def main() -> int:
 
     print(json.dumps(result), flush=True)
     return 0
+
+
+def _event_ms(stream, fn, reps=3) -> float:
+    """Average milliseconds of fn() between HIP events on the launch stream, after one warm-up."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    e[0].record(stream)
+    for _ in range(reps):
+        fn()
+    e[1].record(stream)
+    torch.cuda.synchronize()
+    return e[0].elapsed_time(e[1]) / reps
+
+
+def large_extras(args, dev, stream, bvh, d, idx, ref) -> dict:
+    """The large mesh's other batches on the same tree and queries: Taylor orders 1 and 2 at the
+    reference's default accuracy scale 2 (plain precision, with their error on the checked
+    subsample), the unsigned distance of all 16M queries, and 16,777,216 first-hit rays (origins on a
+    radius-3 sphere, directions toward uniform points in the unit ball)."""
+    import torch
+
+    from smallgwn_amd import gwn, synth
+
+    n = d[0].numel()
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    ref_h = ref.cpu().numpy()
+    res = {}
+    for order in (1, 2):
+        mom = gwn.MomentObject(order)
+        gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+        ms = _event_ms(stream, lambda: gwn.gwn_compute_winding_number_taylor_batch(
+            bvh, mom, *d, out, accuracy_scale=2.0).raise_if_error())
+        err = np.abs(out.cpu().numpy()[idx].astype(np.float64) - ref_h)
+        res[f"taylor_o{order}_scale2"] = {"mqueries_per_s": n / (ms * 1e-3) / 1e6, "ms": ms,
+                                          "max_abs_dw": float(err.max()),
+                                          "p99_abs_dw": float(np.quantile(err, 0.99))}
+        log(f"taylor o{order} plain scale 2: {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mq/s), "
+            f"max |dW| {err.max():.3e}")
+        del mom
+    ms = _event_ms(stream, lambda: gwn.gwn_compute_unsigned_distance_batch(bvh, *d, out).raise_if_error())
+    res["distance"] = {"mqueries_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "queries": n}
+    log(f"distance: {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mq/s)")
+    del out
+    rays = [torch.from_numpy(a).to(dev) for a in synth.rays_toward_ball(n, seed=synth.DEFAULT_SEED + 3)]
+    hits = torch.empty(8 * n, dtype=torch.int32, device=dev)
+    ms = _event_ms(stream, lambda: gwn.gwn_compute_ray_first_hit_batch(bvh, *rays, hits).raise_if_error())
+    status = hits.view(torch.uint8).view(n, 32)[:, 28]
+    hit_frac = float((status == int(gwn.RayStatus.HIT)).to(torch.float32).mean())
+    res["ray_first_hit"] = {"mrays_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "rays": n, "hit_fraction": hit_frac}
+    log(f"rays: {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mrays/s), hit fraction {hit_frac:.3f}")
+    return res
+
+
+def precision_report(args, dev, stream) -> dict:
+    """Precision-split report: open soup of ~262K triangles (displaced sphere, 30% of faces deleted,
+    2% with jittered duplicate vertices), 262,144 queries at normal offsets log-uniform in
+    [1e-6, 1e-1] from random surface points.  Exact, Taylor order 2 (at the headline accuracy
+    scale) and antipodal winding, each plain (FP32) and split (FP64 near field), against the all-FP64
+    exact form: max and p99 |dW| and throughput per mode."""
+    import torch
+
+    from smallgwn_amd import gwn, synth
+
+    t0 = time.time()
+    soup = synth.open_soup(612, 306, seed=synth.DEFAULT_SEED)
+    q = synth.near_surface_log_queries(soup, 262_144, seed=synth.DEFAULT_SEED + 5, lo=1e-6, hi=1e-1)
+    n = q[0].shape[0]
+    geom = gwn.GeometryObject()
+    gwn.gwn_upload_geometry(geom, soup.vx, soup.vy, soup.vz, soup.i0, soup.i1, soup.i2).raise_if_error()
+    bvh = gwn.BvhObject()
+    gwn.gwn_build_bvh(geom, bvh, gwn.BuildOptions(method=gwn.BuildMethod.HPLOC, hploc_search_radius=8)).raise_if_error()
+    mom = gwn.MomentObject(2)
+    gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+    chain = gwn.BoundaryChainObject()
+    gwn.gwn_build_boundary_chain(geom, chain).raise_if_error()
+    d = [torch.from_numpy(a).to(dev) for a in q]
+    ref = torch.empty(n, dtype=torch.float64, device=dev)
+    gwn.gwn_compute_winding_number_exact_batch_ex(bvh, *d, ref, gwn.Precision.FP64).raise_if_error()
+    ref_h = ref.cpu().numpy()
+    o32 = torch.empty(n, dtype=torch.float32, device=dev)
+    o64 = torch.empty(n, dtype=torch.float64, device=dev)
+    modes = {
+        "exact_plain": (o32, lambda: gwn.gwn_compute_winding_number_exact_batch(bvh, *d, o32)),
+        "exact_split": (o64, lambda: gwn.gwn_compute_winding_number_exact_batch_ex(bvh, *d, o64, gwn.Precision.SPLIT)),
+        "taylor_o2_plain": (o32, lambda: gwn.gwn_compute_winding_number_taylor_batch(
+            bvh, mom, *d, o32, accuracy_scale=args.beta)),
+        "taylor_o2_split": (o32, lambda: gwn.gwn_compute_winding_number_taylor_batch(
+            bvh, mom, *d, o32, accuracy_scale=args.beta, precision=gwn.Precision.SPLIT)),
+        "antipodal_plain": (o32, lambda: gwn.gwn_compute_winding_number_antipodal_batch(geom, bvh, chain, *d, o32)),
+        "antipodal_split": (o32, lambda: gwn.gwn_compute_winding_number_antipodal_batch(
+            geom, bvh, chain, *d, o32, precision=gwn.Precision.SPLIT)),
+    }
+    res = {"triangles": soup.triangle_count, "queries": n, "accuracy_scale": args.beta}
+    for name, (buf, fn) in modes.items():
+        ms = _event_ms(stream, lambda: fn().raise_if_error(), reps=1)
+        w = buf.cpu().numpy().astype(np.float64)
+        ok = np.isfinite(w)
+        err = np.abs(w[ok] - ref_h[ok])
+        res[name] = {"max_abs_dw": float(err.max()), "p99_abs_dw": float(np.quantile(err, 0.99)),
+                     "nan_results": int((~ok).sum()), "mqueries_per_s": n / (ms * 1e-3) / 1e6}
+        log(f"precision {name}: max |dW| {err.max():.3e}, p99 {np.quantile(err, 0.99):.3e}, "
+            f"{n / (ms * 1e-3) / 1e6:.2f} Mq/s")
+    log(f"precision report: {soup.triangle_count} tris, {n} queries ({time.time() - t0:.1f}s)")
+    return res
 
 
 def adjoint_bench(args, dev, stream) -> dict:
