@@ -156,14 +156,10 @@ static_assert(sizeof(tri_record) == 36, "triangle record must be 36 bytes");
 __host__ __device__ constexpr int moment_field_count(int order) {
     return order == 0 ? 7 : (order == 1 ? 13 : 23);
 }
-// Traversal record: one float4 of compact child references, then the fields as float4 (slot-SoA):
-// 128 B / 256 B / 448 B for order 0 / 1 / 2.  Order-2 records carry four derived fields after the
-// 23 moment fields, so the expansion does not rebuild them per query: the trace of the first-order
-// tensor (field 23) and the order-2 linear vector (fields 24-26).
-constexpr int k_field_trace = 23;
-constexpr int k_field_linear2 = 24;
+// Traversal record: one float4 of compact child references, then the fields as float4 (slot-SoA),
+// padded to a whole number of 128-byte cache lines: 128 B / 256 B / 384 B for order 0 / 1 / 2.
 __host__ __device__ constexpr int moment_record_vec4(int order) {
-    return order == 0 ? 8 : (order == 1 ? 16 : 28);
+    return order == 0 ? 8 : (order == 1 ? 16 : 24);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -244,29 +244,16 @@ __device__ inline void write_fields(float *fields, int slot, aabb const &bnd, pa
             c[comp][4] = raw[4] - Py * F[2] - Pz * F[1] + Py * Pz * Nn;
             c[comp][5] = raw[5] - Pz * F[0] - Px * F[2] + Pz * Px * Nn;
         }
-        float const xxx = c[0][0], yyy = c[1][1], zzz = c[2][2];
-        float const axy = 2.0f * c[0][3] + c[1][0];
-        float const axz = 2.0f * c[0][5] + c[2][0];
-        float const ayz = 2.0f * c[1][4] + c[2][1];
-        float const ayx = 2.0f * c[1][3] + c[0][1];
-        float const azx = 2.0f * c[2][5] + c[0][2];
-        float const azy = 2.0f * c[2][4] + c[1][2];
-        GWN_FIELD(13) = xxx;
-        GWN_FIELD(14) = yyy;
-        GWN_FIELD(15) = zzz;
+        GWN_FIELD(13) = c[0][0];
+        GWN_FIELD(14) = c[1][1];
+        GWN_FIELD(15) = c[2][2];
         GWN_FIELD(16) = 2.0f * (c[0][4] + c[1][5] + c[2][3]);
-        GWN_FIELD(17) = axy;
-        GWN_FIELD(18) = axz;
-        GWN_FIELD(19) = ayz;
-        GWN_FIELD(20) = ayx;
-        GWN_FIELD(21) = azx;
-        GWN_FIELD(22) = azy;
-        // Derived traversal fields (not moments): trace of the first-order tensor and the
-        // order-2 linear vector (3 xxx + Ayx + Azx, 3 yyy + Azy + Axy, 3 zzz + Axz + Ayz).
-        GWN_FIELD(k_field_trace) = GWN_FIELD(7) + GWN_FIELD(8) + GWN_FIELD(9);
-        GWN_FIELD(k_field_linear2 + 0) = 3.0f * xxx + (ayx + azx);
-        GWN_FIELD(k_field_linear2 + 1) = 3.0f * yyy + (azy + axy);
-        GWN_FIELD(k_field_linear2 + 2) = 3.0f * zzz + (axz + ayz);
+        GWN_FIELD(17) = 2.0f * c[0][3] + c[1][0];
+        GWN_FIELD(18) = 2.0f * c[0][5] + c[2][0];
+        GWN_FIELD(19) = 2.0f * c[1][4] + c[2][1];
+        GWN_FIELD(20) = 2.0f * c[1][3] + c[0][1];
+        GWN_FIELD(21) = 2.0f * c[2][5] + c[0][2];
+        GWN_FIELD(22) = 2.0f * c[2][4] + c[1][2];
     }
 #undef GWN_FIELD
 }

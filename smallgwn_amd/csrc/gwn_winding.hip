@@ -377,7 +377,7 @@ __device__ inline float taylor_term(float const *f, int s, float qrx, float qry,
         float const qxx = qnx * qnx, qyy = qny * qny, qzz = qnz * qnz;
         float const Txx = GWN_F(7), Tyy = GWN_F(8), Tzz = GWN_F(9);
         float const Sxy = GWN_F(10), Syz = GWN_F(11), Szx = GWN_F(12);
-        float const Tr = Order >= 2 ? GWN_F(k_field_trace) : Txx + Tyy + Tzz;
+        float const Tr = Txx + Tyy + Tzz;
         float const S = qxx * Txx + qyy * Tyy + qzz * Tzz + qnx * qny * Sxy + qnx * qnz * Szx + qny * qnz * Syz;
         w += m3 * (Tr - 3.0f * S);
         if constexpr (Grad) {
@@ -397,16 +397,18 @@ __device__ inline float taylor_term(float const *f, int s, float qrx, float qry,
         float const qnx3 = qnx2 * qnx, qny3 = qny2 * qny, qnz3 = qnz2 * qnz;
         float const xxx = GWN_F(13), yyy = GWN_F(14), zzz = GWN_F(15), spn = GWN_F(16);
         float const Axy = GWN_F(17), Axz = GWN_F(18), Ayz = GWN_F(19), Ayx = GWN_F(20), Azx = GWN_F(21), Azy = GWN_F(22);
-        // Linear vector (3 xxx + Ayx + Azx, ...) precomputed in the record.
-        float const Px = GWN_F(k_field_linear2 + 0), Py = GWN_F(k_field_linear2 + 1), Pz = GWN_F(k_field_linear2 + 2);
+        float const t0x = Ayx + Azx, t0y = Azy + Axy, t0z = Axz + Ayz;
         float const t1x = qny * Axy + qnz * Axz;
         float const t1y = qnz * Ayz + qnx * Ayx;
         float const t1z = qnx * Azx + qny * Azy;
-        w += m4 * (1.5f * (qnx * Px + qny * Py + qnz * Pz) -
+        w += m4 * (1.5f * (qnx * (3.0f * xxx + t0x) + qny * (3.0f * yyy + t0y) + qnz * (3.0f * zzz + t0z)) -
                    7.5f * (qnx3 * xxx + qny3 * yyy + qnz3 * zzz + qnx * qny * qnz * spn + qnx2 * t1x +
                            qny2 * t1y + qnz2 * t1z));
         if constexpr (Grad) {
             float const m5 = m4 * m1;
+            float const Px = 3.0f * xxx + Ayx + Azx;
+            float const Py = 3.0f * yyy + Azy + Axy;
+            float const Pz = 3.0f * zzz + Axz + Ayz;
             float const L2 = qnx * Px + qny * Py + qnz * Pz;
             float const C2 = qnx3 * xxx + qny3 * yyy + qnz3 * zzz + qnx * qny * qnz * spn +
                              qnx2 * (qny * Axy + qnz * Axz) + qny2 * (qnz * Ayz + qnx * Ayx) +
