@@ -328,8 +328,14 @@ def large_extras(args, dev, stream, bvh, d, idx, ref) -> dict:
             f"max |dW| {err.max():.3e}")
         del mom
     ms = _event_ms(stream, lambda: gwn.gwn_compute_unsigned_distance_batch(bvh, *d, out).raise_if_error())
-    res["distance"] = {"mqueries_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "queries": n}
+    res["distance"] = {"mqueries_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "queries": n,
+                       "traversal": "reference order, per lane (bit-exact)"}
     log(f"distance: {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mq/s)")
+    wave_cfg = gwn.QueryConfig(sort_queries=2)
+    ms = _event_ms(stream, lambda: gwn.gwn_compute_unsigned_distance_batch(bvh, *d, out, config=wave_cfg).raise_if_error())
+    res["distance_wave"] = {"mqueries_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "queries": n,
+                            "traversal": "wave-cooperative (sort_queries=2, within FP32 ulps)"}
+    log(f"distance (wave traversal): {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mq/s)")
     del out
     rays = [torch.from_numpy(a).to(dev) for a in synth.rays_toward_ball(n, seed=synth.DEFAULT_SEED + 3)]
     hits = torch.empty(8 * n, dtype=torch.int32, device=dev)

@@ -62,7 +62,11 @@ typedef struct gwn_batch_handle *gwn_batch_t;
 
 /* Reference gwn_query_batch_config (include/gwn/gwn_query.cuh:41-61): block size 256 and
  * stack capacity 64 by default.  sort_queries: -1 auto, 0 off, 1 on (queries are traversed in
- * Morton order of their positions; results are written in input order). */
+ * Morton order of their positions; results are written in input order).  The unsigned distance
+ * batch also takes 2: Morton order with a wave-cooperative traversal (a wave walks each node once
+ * for its 64 points, each point pruning by its own best distance).  Its distances are each
+ * point's minimum over the triangles it reaches, which can differ from the reference order's in
+ * the last FP32 bits where a box distance rounds above a contained triangle's distance. */
 typedef struct gwn_query_config {
     int32_t block_size;
     int32_t stack_capacity;

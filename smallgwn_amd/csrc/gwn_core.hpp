@@ -162,6 +162,27 @@ __host__ __device__ constexpr int moment_record_vec4(int order) {
     return order == 0 ? 8 : (order == 1 ? 16 : 24);
 }
 
+// Five-comparator sorting network: orders t ascending and applies the same swaps to k.  Used
+// for near-first child ordering in traversals.
+template <class Key>
+__device__ inline void sort4(float (&t)[4], Key (&k)[4]) {
+    auto cs = [&](int a, int b) {
+        if (!(t[a] > t[b]))
+            return;
+        float const tt = t[a];
+        t[a] = t[b];
+        t[b] = tt;
+        Key const kk = k[a];
+        k[a] = k[b];
+        k[b] = kk;
+    };
+    cs(0, 1);
+    cs(2, 3);
+    cs(0, 2);
+    cs(1, 3);
+    cs(1, 2);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Cross-workgroup publication within one kernel.
 //
@@ -310,7 +331,7 @@ void refit_bounds(gwn_geometry_handle const &geometry, gwn_bvh_handle &bvh, hipS
 struct query_launch {
     int block_size = 256;
     int stack_capacity = 64;
-    int sort_queries = -1; // -1 auto, 0 off, 1 on
+    int sort_queries = -1; // -1 auto, 0 off, 1 on; 2 (distance batch): on, wave-cooperative traversal
 };
 
 void winding_exact(

@@ -10,7 +10,8 @@
 // children are queued, the nearest internal child is entered directly and the others are pushed
 // far-to-near.  A query that reaches distance zero stops (zero is the global lower bound).
 // Queries may be visited through a Morton-order permutation so that a wave's lanes walk nearby
-// paths; results land at the input positions.
+// paths; results land at the input positions.  With sort_queries = 2 the Morton-ordered batch
+// uses a wave-cooperative traversal instead (distance_wave_traverse).
 //
 // Arithmetic is unfused and the square root correctly rounded, so every result equals the CPU
 // oracle's replay bit for bit.
@@ -202,6 +203,109 @@ __device__ inline int distance_traverse(
     return status;
 }
 
+// Wave-cooperative form of distance_traverse (sort_queries = 2).  The wave walks one node at a
+// time (node records are wave-uniform loads); every lane computes its own box distances and prunes
+// by its own best, so a lane's result is its minimum over the triangles it reaches.  A child is
+// entered when any lane still needs it; leaves of a node are evaluated before its internal
+// children, children in the order of the first needing lane's box distances (near first).
+// Pruning compares FP32 box and triangle distances, and a box's distance can round above that of
+// a triangle inside it, so which of two equally near triangles survives depends on visiting
+// order: results can differ from the reference-order traversal in the last bits.  `s_node` is
+// this wave's LDS stack.  Returns the lane's status (k_dist_overflow for every lane when the
+// shared stack overflows).
+template <int StackCap>
+__device__ inline int distance_wave_traverse(
+    distance_scene const &sc, bool active, float x, float y, float z, float &best, std::uint32_t *best_rec,
+    std::uint32_t *s_node
+) {
+    int status = k_dist_done;
+    if (ref_leaf(sc.root_ref)) {
+        if (active && !visit_leaf(sc.tris, sc.nt, sc.root_ref, x, y, z, best, best_rec))
+            status = k_dist_zero;
+        return status;
+    }
+    if (!ref_internal(sc.root_ref))
+        return status;
+    constexpr std::uint32_t k_none = 0xffffffffu;
+    int sp = 0;
+    bool overflow = false;  // wave-uniform
+    std::uint32_t node = static_cast<std::uint32_t>(ref_offset(sc.root_ref));
+    while (node != k_none && !overflow) {
+        node = __builtin_amdgcn_readfirstlane(node);
+        std::uint32_t next = k_none;
+        if (node < sc.node_count) {
+            bvh_node4 const &nd = sc.nodes[node];
+            float cd[4], key[4];
+            std::uint64_t cr[4], need[4];
+            int idx[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                bvh_child const ch = nd.children[s];
+                cr[s] = ch.reference;
+                cd[s] = box_distance2(x, y, z, ch.bounds);
+                bool const want = active && status == k_dist_done && ref_valid(ch.reference) && cd[s] < best;
+                need[s] = __ballot(want);
+                key[s] = INFINITY;
+                if (need[s] != 0ull) {
+                    int const first = __builtin_ctzll(need[s]);
+                    key[s] = __builtin_bit_cast(
+                        float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cd[s]), first));
+                }
+                idx[s] = s;
+            }
+            sort4(key, idx);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                int const s = idx[k];
+                if (need[s] == 0ull || !ref_leaf(cr[s]))
+                    continue;
+                if (active && status == k_dist_done && cd[s] < best &&
+                    !visit_leaf(sc.tris, sc.nt, cr[s], x, y, z, best, best_rec))
+                    status = k_dist_zero;
+            }
+#pragma unroll
+            for (int k = 3; k >= 0; --k) {
+                int const s = idx[k];
+                if (need[s] == 0ull || !ref_internal(cr[s]) || overflow)
+                    continue;
+                // Re-check after the leaves: some lanes may no longer need this child.
+                if (__ballot(active && status == k_dist_done && cd[s] < best) == 0ull)
+                    continue;
+                if (next != k_none) {
+                    if (sp < StackCap)
+                        s_node[sp++] = next;
+                    else
+                        overflow = true;
+                }
+                next = static_cast<std::uint32_t>(ref_offset(cr[s]));
+            }
+        }
+        if (next == k_none && sp > 0)
+            next = s_node[--sp];
+        node = next;
+    }
+    return overflow ? k_dist_overflow : status;
+}
+
+template <int StackCap>
+__global__ __launch_bounds__(k_block) void distance_wave_kernel(
+    distance_scene sc, float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
+    std::uint32_t const *__restrict__ order, float *__restrict__ out, std::size_t n, float culling_band
+) {
+    __shared__ std::uint32_t s_node[k_block / 64][StackCap];
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    bool const active = i < n;
+    if (__ballot(active) == 0ull)
+        return;
+    std::size_t const q = active ? order[i] : 0;
+    float const x = active ? qx[q] : 0.0f, y = active ? qy[q] : 0.0f, z = active ? qz[q] : 0.0f;
+    float const band = culling_band < 0.0f ? 0.0f : culling_band;
+    float best = band * band;
+    int const st = distance_wave_traverse<StackCap>(sc, active, x, y, z, best, nullptr, s_node[threadIdx.x >> 6]);
+    if (active)
+        out[q] = st == k_dist_zero ? 0.0f : (st == k_dist_overflow ? __builtin_nanf("") : sqrtf(best));
+}
+
 template <int StackCap>
 __global__ __launch_bounds__(k_block) void distance_kernel(
     distance_scene sc, float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
@@ -265,6 +369,48 @@ __device__ inline void closest_barycentrics(
     w = vc * denom;
 }
 
+// Per-point part of the distance adjoint once the nearest record is known.
+__device__ inline void distance_adjoint_finish(
+    distance_scene const &sc, float const (&p)[3], std::size_t q, int st, std::uint32_t rec, float uq,
+    std::uint32_t const *__restrict__ prim, std::uint32_t const *__restrict__ i0, std::uint32_t const *__restrict__ i1,
+    std::uint32_t const *__restrict__ i2, float *dvx, float *dvy, float *dvz, float *__restrict__ dpx,
+    float *__restrict__ dpy, float *__restrict__ dpz
+) {
+    float g[3] = {0.0f, 0.0f, 0.0f};
+    if (st == k_dist_overflow) {
+        g[0] = g[1] = g[2] = __builtin_nanf("");
+    } else if (st == k_dist_done && rec != 0xffffffffu) {
+        tri_record const t = sc.tris[rec];
+        float const a[3] = {t.v0x, t.v0y, t.v0z};
+        float const b[3] = {t.v0x + (t.v1x - t.v0x), t.v0y + (t.v1y - t.v0y), t.v0z + (t.v1z - t.v0z)};
+        float const c[3] = {t.v0x + (t.v2x - t.v0x), t.v0y + (t.v2y - t.v0y), t.v0z + (t.v2z - t.v0z)};
+        float v, w;
+        closest_barycentrics(p, a, b, c, v, w);
+        float const bary[3] = {1.0f - v - w, v, w};
+        float r[3];
+        for (int k = 0; k < 3; ++k)
+            r[k] = p[k] - (bary[0] * a[k] + bary[1] * b[k] + bary[2] * c[k]);
+        float const len = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        if (len > 0.0f) {
+            float const su = uq / len;
+            std::uint32_t const pr = prim[rec];
+            std::uint32_t const ids[3] = {i0[pr], i1[pr], i2[pr]};
+            for (int k = 0; k < 3; ++k)
+                g[k] = su * r[k];
+            for (int vtx = 0; vtx < 3; ++vtx) {
+                atomicAdd(dvx + ids[vtx], -bary[vtx] * g[0]);
+                atomicAdd(dvy + ids[vtx], -bary[vtx] * g[1]);
+                atomicAdd(dvz + ids[vtx], -bary[vtx] * g[2]);
+            }
+        }
+    }
+    if (dpx != nullptr) {
+        dpx[q] = g[0];
+        dpy[q] = g[1];
+        dpz[q] = g[2];
+    }
+}
+
 // Adjoint of sum_p u_p d_p: the forward traversal (same decisions) remembers the nearest record;
 // with c = sum b_i v_i its closest point and n = (p - c) / |p - c|, d d/dp = n and
 // d d/dv_i = -b_i n (the closest point moves with the triangle).  Points on the surface, beyond the
@@ -286,48 +432,20 @@ __global__ __launch_bounds__(k_block) void distance_adjoint_kernel(
     float best = band * band;
     std::uint32_t rec = 0xffffffffu;
     int const st = distance_traverse<StackCap>(sc, p[0], p[1], p[2], best, &rec);
-    float g[3] = {0.0f, 0.0f, 0.0f};
-    if (st == k_dist_overflow) {
-        g[0] = g[1] = g[2] = __builtin_nanf("");
-    } else if (st == k_dist_done && rec != 0xffffffffu) {
-        tri_record const t = sc.tris[rec];
-        float const a[3] = {t.v0x, t.v0y, t.v0z};
-        float const b[3] = {t.v0x + (t.v1x - t.v0x), t.v0y + (t.v1y - t.v0y), t.v0z + (t.v1z - t.v0z)};
-        float const c[3] = {t.v0x + (t.v2x - t.v0x), t.v0y + (t.v2y - t.v0y), t.v0z + (t.v2z - t.v0z)};
-        float v, w;
-        closest_barycentrics(p, a, b, c, v, w);
-        float const bary[3] = {1.0f - v - w, v, w};
-        float r[3];
-        for (int k = 0; k < 3; ++k)
-            r[k] = p[k] - (bary[0] * a[k] + bary[1] * b[k] + bary[2] * c[k]);
-        float const len = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-        if (len > 0.0f) {
-            float const su = u[q] / len;
-            std::uint32_t const pr = prim[rec];
-            std::uint32_t const ids[3] = {i0[pr], i1[pr], i2[pr]};
-            for (int k = 0; k < 3; ++k)
-                g[k] = su * r[k];
-            for (int vtx = 0; vtx < 3; ++vtx) {
-                atomicAdd(dvx + ids[vtx], -bary[vtx] * g[0]);
-                atomicAdd(dvy + ids[vtx], -bary[vtx] * g[1]);
-                atomicAdd(dvz + ids[vtx], -bary[vtx] * g[2]);
-            }
-        }
-    }
-    if (dpx != nullptr) {
-        dpx[q] = g[0];
-        dpy[q] = g[1];
-        dpz[q] = g[2];
-    }
+    distance_adjoint_finish(sc, p, q, st, rec, u[q], prim, i0, i1, i2, dvx, dvy, dvz, dpx, dpy, dpz);
 }
+
 
 template <int Cap>
 void launch_distance(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, std::uint32_t const *order,
-    float *out, std::size_t n, float band, hipStream_t stream
+    float *out, std::size_t n, float band, bool wave, hipStream_t stream
 ) {
     distance_scene const sc{bvh.nodes.ptr, bvh.node_count, bvh.triangles.ptr, bvh.triangle_count, bvh.root.reference};
-    distance_kernel<Cap><<<grid_for(n, k_block), k_block, 0, stream>>>(sc, qx, qy, qz, order, out, n, band);
+    if (wave && order != nullptr)
+        distance_wave_kernel<Cap><<<grid_for(n, k_block), k_block, 0, stream>>>(sc, qx, qy, qz, order, out, n, band);
+    else
+        distance_kernel<Cap><<<grid_for(n, k_block), k_block, 0, stream>>>(sc, qx, qy, qz, order, out, n, band);
     GWN_LAUNCH_CHECK();
 }
 
@@ -340,6 +458,7 @@ void unsigned_distance(
     if (n == 0)
         return;
     bool const sort = cfg.sort_queries > 0 || (cfg.sort_queries < 0 && n >= (1u << 16));
+    bool const wave = cfg.sort_queries == 2;
     device_buffer<std::uint32_t> order;
     if (sort) {
         order.allocate(n, stream);
@@ -347,11 +466,11 @@ void unsigned_distance(
     }
     int const need = static_cast<int>(bvh.internal_stack_bound);
     if (need <= 32)
-        launch_distance<32>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, stream);
+        launch_distance<32>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, wave, stream);
     else if (need <= 64)
-        launch_distance<64>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, stream);
+        launch_distance<64>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, wave, stream);
     else if (need <= 128)
-        launch_distance<128>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, stream);
+        launch_distance<128>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, wave, stream);
     else
         throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
 }

@@ -193,24 +193,6 @@ __device__ inline void visit_leaf(ray_scene const &sc, ray_state const &r, std::
     }
 }
 
-template <class Key>
-__device__ inline void sort4(float (&t)[4], Key (&k)[4]) {
-    auto cs = [&](int a, int b) {
-        if (!(t[a] > t[b]))
-            return;
-        float const tt = t[a];
-        t[a] = t[b];
-        t[b] = tt;
-        Key const kk = k[a];
-        k[a] = k[b];
-        k[b] = kk;
-    };
-    cs(0, 1);
-    cs(2, 3);
-    cs(0, 2);
-    cs(1, 3);
-    cs(1, 2);
-}
 
 // Packed near-first traversal (:242-358).  Returns true on stack overflow.
 template <int Cap>

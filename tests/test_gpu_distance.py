@@ -61,6 +61,26 @@ def test_distance_matches_oracle_bitwise(gwn, orc, dev, name, sort):
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), f"band {band}"
 
 
+@pytest.mark.parametrize("name", ["torus", "soup", "displaced_256x128", "icosphere4"])
+def test_distance_wave_traversal_within_ulps(gwn, orc, dev, name):
+    """sort_queries = 2 (wave-cooperative traversal): every distance is within 4 FP32 ulps of the
+    reference-order replay, NaN and zero patterns equal, and at least 90% identical.  Only
+    pruning near ties of box and triangle distances depends on the visiting order."""
+    mesh = MESHES[name]
+    _, bvh = build_gpu(gwn, mesh)
+    ref_bvh = orc.build_bvh4(mesh)
+    qx, qy, qz = synth.mixed_queries(mesh, 8192, seed=43, band=0.01) if name != "soup" else \
+        synth.uniform_box_points(43, 8192, (-3.0,) * 3, (3.0,) * 3)
+    for band in (float("inf"), 0.05):
+        got = _distance(gwn, dev, bvh, qx, qy, qz, band=band, sort=2).astype(np.float64)
+        ref = orc.unsigned_distance_f32(ref_bvh, qx, qy, qz, culling_band=band).astype(np.float64)
+        assert np.array_equal(np.isnan(got), np.isnan(ref)) and np.array_equal(got == 0, ref == 0)
+        ok = ~np.isnan(ref)
+        ulp = np.spacing(ref[ok].astype(np.float32)).astype(np.float64)
+        assert np.all(np.abs(got[ok] - ref[ok]) <= 4 * ulp), f"band {band}"
+        assert np.mean(got[ok] == ref[ok]) >= 0.9
+
+
 def test_distance_on_surface_points_is_zero_or_tiny(gwn, orc, dev):
     mesh = MESHES["displaced_256x128"]
     _, bvh = build_gpu(gwn, mesh)
