@@ -809,69 +809,122 @@ __global__ __launch_bounds__(256) void leaf_root_kernel(
     }
 }
 
+// Reciprocal square root and reciprocal in FP64 from the hardware approximations and Newton
+// steps (relative error at FP64 rounding level, not correctly rounded).  x > 0 / x != 0.
+__device__ inline double rsq_f64_fast(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+    double const h = 0.5 * x;
+    r = r * fma(-h * r, r, 1.5);
+    r = r * fma(-h * r, r, 1.5);
+    return r;
+}
+__device__ inline double rcp_f64_fast(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+__device__ inline double rsq_fast(double x) { return rsq_f64_fast(x); }
+__device__ inline float rsq_fast(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline double rcp_fast(double x) { return rcp_f64_fast(x); }
+__device__ inline float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Derivatives of one triangle's solid angle W(a, b, c) seen from the origin of a, b, c (vertices
 // relative to the query), from the unnormalised form W = 2 atan2(a.(b x c), D) with
 // D = |a||b||c| + (a.b)|c| + (a.c)|b| + (b.c)|a| (equal to the Van Oosterom-Strackee quotient
 // scaled by |a||b||c|):  dW = 2 (D dnum - num dD) / (num^2 + D^2).  Zero where W is defined as 0.
-// Evaluated in FP64: near the surface the terms are ~1/d^2, cancel between neighbouring
-// triangles, and lose ~|a|/d of their FP32 digits.
+template <class R>
 __device__ inline void solid_angle_vertex_grad(
-    double const (&a)[3], double const (&b)[3], double const (&c)[3], double (&da)[3], double (&db)[3],
-    double (&dc)[3]
+    R const (&a)[3], R const (&b)[3], R const (&c)[3], R (&da)[3], R (&db)[3], R (&dc)[3]
 ) {
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-        da[k] = db[k] = dc[k] = 0.0;
-    double const la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-    double const lb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
-    double const lc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-    if (la == 0.0 || lb == 0.0 || lc == 0.0)
+        da[k] = db[k] = dc[k] = R(0);
+    R const aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    R const bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+    R const cc = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    if (aa == R(0) || bb == R(0) || cc == R(0))
         return;
-    double const bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
-    double const ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
-    double const ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
-    double const num = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
-    if (num == 0.0)
+    R const ia = rsq_fast(aa), ib = rsq_fast(bb), ic = rsq_fast(cc);
+    R const la = aa * ia, lb = bb * ib, lc = cc * ic;
+    R const bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+    R const ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+    R const ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    R const num = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+    if (num == R(0))
         return;
-    double const dab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-    double const dac = a[0] * c[0] + a[1] * c[1] + a[2] * c[2];
-    double const dbc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
-    double const den = la * lb * lc + dab * lc + dac * lb + dbc * la;
-    double const sc = 2.0 / (num * num + den * den);
-    double const ia = 1.0 / la, ib = 1.0 / lb, ic = 1.0 / lc;
+    R const dab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    R const dac = a[0] * c[0] + a[1] * c[1] + a[2] * c[2];
+    R const dbc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
+    R const den = la * lb * lc + dab * lc + dac * lb + dbc * la;
+    R const sc = R(2) * rcp_fast(num * num + den * den);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        double const ha = a[k] * ia, hb = b[k] * ib, hc = c[k] * ic;
-        double const dDa = ha * lb * lc + b[k] * lc + c[k] * lb + dbc * ha;
-        double const dDb = hb * la * lc + a[k] * lc + c[k] * la + dac * hb;
-        double const dDc = hc * la * lb + a[k] * lb + b[k] * la + dab * hc;
+        R const ha = a[k] * ia, hb = b[k] * ib, hc = c[k] * ic;
+        R const dDa = ha * lb * lc + b[k] * lc + c[k] * lb + dbc * ha;
+        R const dDb = hb * la * lc + a[k] * lc + c[k] * la + dac * hb;
+        R const dDc = hc * la * lb + a[k] * lb + b[k] * la + dab * hc;
         da[k] = sc * (den * bc[k] - num * dDa);
         db[k] = sc * (den * ca[k] - num * dDb);
         dc[k] = sc * (den * ab[k] - num * dDc);
     }
 }
 
-// Adds u * dW/d(a, b, c) of one near-field triangle to its record's 9 adjoint entries and
-// subtracts (da + db + dc) * u (the query part) from dq.  The corner vectors are exact FP64
-// differences of the float vertices and query.
+// u * dW/d(a, b, c) of one descended triangle into t9 (9 floats) and -(da + db + dc) * u into dq.
+// Split like the forward's precision split: within about four edge lengths (where the terms are
+// ~1/d^2, cancel between neighbouring triangles and lose ~|a|/d of their FP32 digits) in FP64 from
+// exact differences of the float vertices and query; beyond it in FP32.
+__device__ inline void adjoint_leaf_terms(
+    tri_record const &t, float x, float y, float z, double sud, float (&t9)[9], double (&dq)[3]
+) {
+    float const e1x = t.v1x - t.v0x, e1y = t.v1y - t.v0y, e1z = t.v1z - t.v0z;
+    float const e2x = t.v2x - t.v0x, e2y = t.v2y - t.v0y, e2z = t.v2z - t.v0z;
+    float const ax = t.v0x - x, ay = t.v0y - y, az = t.v0z - z;
+    float const le1 = squared_norm_unfused(e1x, e1y, e1z);
+    float const le2 = squared_norm_unfused(e2x, e2y, e2z);
+    float const dq2 = squared_norm_unfused(ax, ay, az);
+    if (!(dq2 > product_unfused(k_split_near_scale2, fmaxf(le1, le2)))) {
+        double const qd[3] = {x, y, z};
+        double const a[3] = {double(t.v0x) - qd[0], double(t.v0y) - qd[1], double(t.v0z) - qd[2]};
+        double const b[3] = {double(t.v1x) - qd[0], double(t.v1y) - qd[1], double(t.v1z) - qd[2]};
+        double const c[3] = {double(t.v2x) - qd[0], double(t.v2y) - qd[1], double(t.v2z) - qd[2]};
+        double da[3], db[3], dc[3];
+        solid_angle_vertex_grad<double>(a, b, c, da, db, dc);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            t9[k] = static_cast<float>(sud * da[k]);
+            t9[3 + k] = static_cast<float>(sud * db[k]);
+            t9[6 + k] = static_cast<float>(sud * dc[k]);
+            dq[k] -= sud * (da[k] + db[k] + dc[k]);
+        }
+    } else {
+        float const a[3] = {ax, ay, az};
+        float const b[3] = {t.v1x - x, t.v1y - y, t.v1z - z};
+        float const c[3] = {t.v2x - x, t.v2y - y, t.v2z - z};
+        float da[3], db[3], dc[3];
+        solid_angle_vertex_grad<float>(a, b, c, da, db, dc);
+        float const su = static_cast<float>(sud);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            t9[k] = su * da[k];
+            t9[3 + k] = su * db[k];
+            t9[6 + k] = su * dc[k];
+            dq[k] -= sud * double(da[k] + db[k] + dc[k]);
+        }
+    }
+}
+
+// Adds one descended triangle's adjoint to its record's 9 entries and its query part to dq.
 __device__ inline void adjoint_leaf(
-    tri_record const &t, std::size_t rec, float x, float y, float z, double su, float *__restrict__ dtri,
+    tri_record const &t, std::size_t rec, float x, float y, float z, double sud, float *__restrict__ dtri,
     double (&dq)[3]
 ) {
-    double const qx = x, qy = y, qz = z;
-    double const a[3] = {double(t.v0x) - qx, double(t.v0y) - qy, double(t.v0z) - qz};
-    double const b[3] = {double(t.v1x) - qx, double(t.v1y) - qy, double(t.v1z) - qz};
-    double const c[3] = {double(t.v2x) - qx, double(t.v2y) - qy, double(t.v2z) - qz};
-    double da[3], db[3], dc[3];
-    solid_angle_vertex_grad(a, b, c, da, db, dc);
+    float t9[9];
+    adjoint_leaf_terms(t, x, y, z, sud, t9, dq);
     float *d = dtri + rec * 9;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        atomicAdd(d + k, static_cast<float>(su * da[k]));
-        atomicAdd(d + 3 + k, static_cast<float>(su * db[k]));
-        atomicAdd(d + 6 + k, static_cast<float>(su * dc[k]));
-        dq[k] -= su * (da[k] + db[k] + dc[k]);
-    }
+    for (int k = 0; k < 9; ++k)
+        atomicAdd(d + k, t9[k]);
 }
 
 // Adjoint of sum_q u_q w_q (orders 0, 1): the per-lane traversal of taylor_kernel with the same
@@ -1166,22 +1219,8 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_adjoint_wave_kernel(
 #pragma unroll
                 for (int k = 0; k < 9; ++k)
                     t9[k] = 0.0f;
-                if (dmask & my_bit) {
-                    tri_record const t = tris[r];
-                    double const qd[3] = {x, y, z};
-                    double const a[3] = {double(t.v0x) - qd[0], double(t.v0y) - qd[1], double(t.v0z) - qd[2]};
-                    double const b[3] = {double(t.v1x) - qd[0], double(t.v1y) - qd[1], double(t.v1z) - qd[2]};
-                    double const cc[3] = {double(t.v2x) - qd[0], double(t.v2y) - qd[1], double(t.v2z) - qd[2]};
-                    double da[3], db[3], dc[3];
-                    solid_angle_vertex_grad(a, b, cc, da, db, dc);
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        t9[k] = static_cast<float>(sud * da[k]);
-                        t9[3 + k] = static_cast<float>(sud * db[k]);
-                        t9[6 + k] = static_cast<float>(sud * dc[k]);
-                        dq[k] -= sud * (da[k] + db[k] + dc[k]);
-                    }
-                }
+                if (dmask & my_bit)
+                    adjoint_leaf_terms(tris[r], x, y, z, sud, t9, dq);
                 wave_atomic_add<9>(dtri + static_cast<std::size_t>(r) * 9, t9, dmask);
                 continue;
             }
