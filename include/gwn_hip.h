@@ -219,6 +219,12 @@ int gwn_compute_winding_gradient_taylor_batch_ex_f32(
 int gwn_compute_unsigned_distance_batch_f32(
     gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, float *out, uint64_t count,
     float culling_band, const gwn_query_config *config, hipStream_t stream);
+/* The same, also writing each query's nearest triangle (no reference counterpart; the input of
+ * gwn_compute_unsigned_distance_adjoint_batch_ex_f32): its BVH leaf record, or 0xFFFFFFFF when the
+ * distance is the culling band or zero (no gradient), 0xFFFFFFFE on stack overflow. */
+int gwn_compute_unsigned_distance_batch_ex_f32(
+    gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, float *out, uint32_t *nearest,
+    uint64_t count, float culling_band, const gwn_query_config *config, hipStream_t stream);
 
 /* ---- ray first hit: replaces gwn_compute_ray_first_hit_batch<Config>
  *      (include/gwn/gwn_query.cuh:205-290, detail/gwn_query_ray_impl.cuh:20-595).  Rays are
@@ -275,6 +281,12 @@ int gwn_compute_unsigned_distance_adjoint_batch_f32(
     gwn_geometry_t geometry, gwn_bvh_t bvh, const float *px, const float *py, const float *pz,
     const float *grad_d, uint64_t count, float culling_band, float *grad_vx, float *grad_vy, float *grad_vz,
     float *grad_px, float *grad_py, float *grad_pz, const gwn_query_config *config, hipStream_t stream);
+/* The same from the forward's nearest records (gwn_compute_unsigned_distance_batch_ex_f32 on the
+ * same BVH and points): no traversal runs. */
+int gwn_compute_unsigned_distance_adjoint_batch_ex_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, const float *px, const float *py, const float *pz,
+    const uint32_t *nearest, const float *grad_d, uint64_t count, float *grad_vx, float *grad_vy, float *grad_vz,
+    float *grad_px, float *grad_py, float *grad_pz, hipStream_t stream);
 
 /* ---- batched mode (no reference counterpart; the reference runs one mesh per call).  A batch
  *      fixes, for each mesh, its BVH and moment objects (all of one Taylor order) and its query

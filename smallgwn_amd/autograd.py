@@ -108,15 +108,17 @@ class UnsignedDistance(torch.autograd.Function):
         scene.update(vertices)
         p = _soa(points)
         d = torch.empty(p.shape[1], dtype=torch.float32, device=points.device)
-        gwn.gwn_compute_unsigned_distance_batch(scene.bvh, p[0], p[1], p[2], d,
-                                                culling_band=scene.culling_band).raise_if_error()
+        # The nearest triangle of every point is kept for backward, which then needs no traversal.
+        nearest = torch.empty(p.shape[1], dtype=torch.int32, device=points.device)
+        gwn.gwn_compute_unsigned_distance_batch(scene.bvh, p[0], p[1], p[2], d, culling_band=scene.culling_band,
+                                                nearest=nearest).raise_if_error()
         ctx.scene, ctx.stamp = scene, scene.stamp
-        ctx.save_for_backward(p)
+        ctx.save_for_backward(p, nearest)
         return d
 
     @staticmethod
     def backward(ctx, grad_d):
-        (p,) = ctx.saved_tensors
+        p, nearest = ctx.saved_tensors
         scene = ctx.scene
         if scene.stamp != ctx.stamp:
             raise RuntimeError("mesh positions changed between forward and backward")
@@ -125,7 +127,7 @@ class UnsignedDistance(torch.autograd.Function):
         gp = torch.empty(3, p.shape[1], dtype=torch.float32, device=p.device) if want_p else None
         gwn.gwn_compute_unsigned_distance_adjoint_batch(
             scene.geometry, scene.bvh, p[0], p[1], p[2], grad_d.to(torch.float32).contiguous(), gv[0], gv[1], gv[2],
-            *(gp if want_p else (None, None, None)), culling_band=scene.culling_band).raise_if_error()
+            *(gp if want_p else (None, None, None)), nearest=nearest).raise_if_error()
         return gv.t(), (gp.t() if want_p else None), None
 
 

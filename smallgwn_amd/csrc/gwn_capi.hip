@@ -566,7 +566,25 @@ int gwn_compute_unsigned_distance_batch_f32(
         require_query_storage(count, qx, qy, qz, out);
         if (count == 0)
             return;
-        gwn::unsigned_distance(*bvh, qx, qy, qz, out, count, culling_band, to_launch(c), stream);
+        gwn::unsigned_distance(*bvh, qx, qy, qz, out, count, culling_band, to_launch(c), nullptr, stream);
+    });
+}
+
+int gwn_compute_unsigned_distance_batch_ex_f32(
+    gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, float *out, uint32_t *nearest, uint64_t count,
+    float culling_band, const gwn_query_config *config, hipStream_t stream
+) {
+    return guarded("gwn_compute_unsigned_distance_batch", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, true);
+        require_bvh(bvh);
+        require_stack(*bvh, c.stack_capacity);
+        require_query_storage(count, qx, qy, qz, out);
+        if (count != 0 && nearest == nullptr)
+            gwn::throw_invalid("Nearest-record output is null.");
+        if (count == 0)
+            return;
+        gwn::unsigned_distance(*bvh, qx, qy, qz, out, count, culling_band, to_launch(c), nearest, stream);
     });
 }
 
@@ -719,7 +737,28 @@ int gwn_compute_unsigned_distance_adjoint_batch_f32(
             gwn::throw_invalid("Point gradient spans must be all null or all non-null.");
         if (count == 0)
             return;
-        gwn::unsigned_distance_adjoint(*geometry, *bvh, px, py, pz, grad_d, count, culling_band, grad_vx, grad_vy,
+        gwn::unsigned_distance_adjoint(*geometry, *bvh, px, py, pz, grad_d, count, culling_band, nullptr, grad_vx,
+                                       grad_vy, grad_vz, grad_px, grad_py, grad_pz, stream);
+    });
+}
+
+int gwn_compute_unsigned_distance_adjoint_batch_ex_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, const float *px, const float *py, const float *pz,
+    const uint32_t *nearest, const float *grad_d, uint64_t count, float *grad_vx, float *grad_vy, float *grad_vz,
+    float *grad_px, float *grad_py, float *grad_pz, hipStream_t stream
+) {
+    return guarded("gwn_compute_unsigned_distance_adjoint_batch", [&] {
+        gwn_query_config const c = resolve_config(nullptr);
+        require_adjoint_geometry(geometry, bvh, c, grad_vx, grad_vy, grad_vz);
+        require_query_storage(count, px, py, pz, grad_d);
+        if (count != 0 && nearest == nullptr)
+            gwn::throw_invalid("Nearest-record input is null.");
+        if ((grad_px != nullptr || grad_py != nullptr || grad_pz != nullptr) && count != 0 &&
+            (grad_px == nullptr || grad_py == nullptr || grad_pz == nullptr))
+            gwn::throw_invalid("Point gradient spans must be all null or all non-null.");
+        if (count == 0)
+            return;
+        gwn::unsigned_distance_adjoint(*geometry, *bvh, px, py, pz, grad_d, count, 0.0f, nearest, grad_vx, grad_vy,
                                        grad_vz, grad_px, grad_py, grad_pz, stream);
     });
 }

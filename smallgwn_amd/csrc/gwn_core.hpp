@@ -347,9 +347,13 @@ void winding_taylor(
     hipStream_t stream
 );
 // Unsigned point-to-mesh distance with a culling band (queries Morton-sorted when cfg says so).
+// `nearest` (optional, one uint32 per query) receives the BVH record of the nearest triangle, or
+// k_nearest_none (distance is the band or zero: no gradient) or k_nearest_overflow.
+constexpr std::uint32_t k_nearest_none = 0xffffffffu;
+constexpr std::uint32_t k_nearest_overflow = 0xfffffffeu;
 void unsigned_distance(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, float *out, std::size_t n,
-    float culling_band, query_launch const &cfg, hipStream_t stream
+    float culling_band, query_launch const &cfg, std::uint32_t *nearest, hipStream_t stream
 );
 // Nearest ray-triangle hits on [t_min, t_max]; `out` holds 32-byte
 // gwn_ray_first_hit_result<float, uint32_t> records.
@@ -373,10 +377,12 @@ void winding_taylor_adjoint(
     float *dvx, float *dvy, float *dvz, float *dqx, float *dqy, float *dqz, hipStream_t stream
 );
 // Adjoint of L = sum_p u_p d_p for the unsigned distance: adds dL/dV, writes dL/dp when non-null.
+// With `nearest` (the forward's records) no traversal runs; otherwise the forward's traversal is
+// repeated to find them.
 void unsigned_distance_adjoint(
     gwn_geometry_handle const &geometry, gwn_bvh_handle const &bvh, float const *px, float const *py,
-    float const *pz, float const *u, std::size_t n, float culling_band, float *dvx, float *dvy, float *dvz,
-    float *dpx, float *dpy, float *dpz, hipStream_t stream
+    float const *pz, float const *u, std::size_t n, float culling_band, std::uint32_t const *nearest,
+    float *dvx, float *dvy, float *dvz, float *dpx, float *dpy, float *dpz, hipStream_t stream
 );
 // Antipodal winding numbers: positive-axis crossings over the BVH triangles plus the boundary
 // chain's spherical areas; axes retried Z, X, Y, NaN when all three are singular.

@@ -120,6 +120,11 @@ struct distance_scene {
 
 enum : int { k_dist_done = 0, k_dist_zero = 1, k_dist_overflow = 2 };
 
+// Nearest-record code of a finished traversal (the adjoint's input).
+__device__ inline std::uint32_t nearest_code(int st, std::uint32_t rec) {
+    return st == k_dist_overflow ? k_nearest_overflow : (st == k_dist_done ? rec : k_nearest_none);
+}
+
 // Near-first culled traversal; `best` starts at the squared band and ends at the squared
 // distance.  Returns k_dist_zero when a triangle touches the query, k_dist_overflow when the
 // stack would overflow.
@@ -290,7 +295,8 @@ __device__ inline int distance_wave_traverse(
 template <int StackCap>
 __global__ __launch_bounds__(k_block) void distance_wave_kernel(
     distance_scene sc, float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
-    std::uint32_t const *__restrict__ order, float *__restrict__ out, std::size_t n, float culling_band
+    std::uint32_t const *__restrict__ order, float *__restrict__ out, std::size_t n, float culling_band,
+    std::uint32_t *__restrict__ nearest
 ) {
     __shared__ std::uint32_t s_node[k_block / 64][StackCap];
     std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
@@ -301,15 +307,21 @@ __global__ __launch_bounds__(k_block) void distance_wave_kernel(
     float const x = active ? qx[q] : 0.0f, y = active ? qy[q] : 0.0f, z = active ? qz[q] : 0.0f;
     float const band = culling_band < 0.0f ? 0.0f : culling_band;
     float best = band * band;
-    int const st = distance_wave_traverse<StackCap>(sc, active, x, y, z, best, nullptr, s_node[threadIdx.x >> 6]);
-    if (active)
+    std::uint32_t rec = k_nearest_none;
+    int const st = distance_wave_traverse<StackCap>(sc, active, x, y, z, best, nearest ? &rec : nullptr,
+                                                    s_node[threadIdx.x >> 6]);
+    if (active) {
         out[q] = st == k_dist_zero ? 0.0f : (st == k_dist_overflow ? __builtin_nanf("") : sqrtf(best));
+        if (nearest != nullptr)
+            nearest[q] = nearest_code(st, rec);
+    }
 }
 
 template <int StackCap>
 __global__ __launch_bounds__(k_block) void distance_kernel(
     distance_scene sc, float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
-    std::uint32_t const *__restrict__ order, float *__restrict__ out, std::size_t n, float culling_band
+    std::uint32_t const *__restrict__ order, float *__restrict__ out, std::size_t n, float culling_band,
+    std::uint32_t *__restrict__ nearest
 ) {
     std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
     if (i >= n)
@@ -317,8 +329,11 @@ __global__ __launch_bounds__(k_block) void distance_kernel(
     std::size_t const q = order ? order[i] : i;
     float const band = culling_band < 0.0f ? 0.0f : culling_band;
     float best = band * band;
-    int const st = distance_traverse<StackCap>(sc, qx[q], qy[q], qz[q], best, nullptr);
+    std::uint32_t rec = k_nearest_none;
+    int const st = distance_traverse<StackCap>(sc, qx[q], qy[q], qz[q], best, nearest ? &rec : nullptr);
     out[q] = st == k_dist_zero ? 0.0f : (st == k_dist_overflow ? __builtin_nanf("") : sqrtf(best));
+    if (nearest != nullptr)
+        nearest[q] = nearest_code(st, rec);
 }
 
 // Barycentric coordinates (of b and c) of the closest point of triangle (a, b, c) to p: the
@@ -436,16 +451,37 @@ __global__ __launch_bounds__(k_block) void distance_adjoint_kernel(
 }
 
 
+// Distance adjoint from the forward's nearest records: no traversal.
+__global__ __launch_bounds__(k_block) void distance_adjoint_nearest_kernel(
+    distance_scene sc, float const *__restrict__ px, float const *__restrict__ py, float const *__restrict__ pz,
+    float const *__restrict__ u, std::uint32_t const *__restrict__ nearest, std::size_t n,
+    std::uint32_t const *__restrict__ prim, std::uint32_t const *__restrict__ i0, std::uint32_t const *__restrict__ i1,
+    std::uint32_t const *__restrict__ i2, float *dvx, float *dvy, float *dvz, float *__restrict__ dpx,
+    float *__restrict__ dpy, float *__restrict__ dpz
+) {
+    std::size_t const q = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    if (q >= n)
+        return;
+    std::uint32_t const code = nearest[q];
+    bool const valid = code < sc.nt;
+    int const st = code == k_nearest_overflow ? k_dist_overflow : (valid ? k_dist_done : k_dist_zero);
+    float const p[3] = {px[q], py[q], pz[q]};
+    distance_adjoint_finish(sc, p, q, st, valid ? code : k_nearest_none, u[q], prim, i0, i1, i2, dvx, dvy, dvz,
+                            dpx, dpy, dpz);
+}
+
 template <int Cap>
 void launch_distance(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, std::uint32_t const *order,
-    float *out, std::size_t n, float band, bool wave, hipStream_t stream
+    float *out, std::size_t n, float band, bool wave, std::uint32_t *nearest, hipStream_t stream
 ) {
     distance_scene const sc{bvh.nodes.ptr, bvh.node_count, bvh.triangles.ptr, bvh.triangle_count, bvh.root.reference};
     if (wave && order != nullptr)
-        distance_wave_kernel<Cap><<<grid_for(n, k_block), k_block, 0, stream>>>(sc, qx, qy, qz, order, out, n, band);
+        distance_wave_kernel<Cap><<<grid_for(n, k_block), k_block, 0, stream>>>(sc, qx, qy, qz, order, out, n, band,
+                                                                                  nearest);
     else
-        distance_kernel<Cap><<<grid_for(n, k_block), k_block, 0, stream>>>(sc, qx, qy, qz, order, out, n, band);
+        distance_kernel<Cap><<<grid_for(n, k_block), k_block, 0, stream>>>(sc, qx, qy, qz, order, out, n, band,
+                                                                             nearest);
     GWN_LAUNCH_CHECK();
 }
 
@@ -453,7 +489,7 @@ void launch_distance(
 
 void unsigned_distance(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, float *out, std::size_t n,
-    float culling_band, query_launch const &cfg, hipStream_t stream
+    float culling_band, query_launch const &cfg, std::uint32_t *nearest, hipStream_t stream
 ) {
     if (n == 0)
         return;
@@ -466,22 +502,31 @@ void unsigned_distance(
     }
     int const need = static_cast<int>(bvh.internal_stack_bound);
     if (need <= 32)
-        launch_distance<32>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, wave, stream);
+        launch_distance<32>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, wave, nearest, stream);
     else if (need <= 64)
-        launch_distance<64>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, wave, stream);
+        launch_distance<64>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, wave, nearest, stream);
     else if (need <= 128)
-        launch_distance<128>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, wave, stream);
+        launch_distance<128>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, wave, nearest, stream);
     else
         throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
 }
 
 void unsigned_distance_adjoint(
     gwn_geometry_handle const &g, gwn_bvh_handle const &bvh, float const *px, float const *py, float const *pz,
-    float const *u, std::size_t n, float culling_band, float *dvx, float *dvy, float *dvz, float *dpx, float *dpy,
-    float *dpz, hipStream_t stream
+    float const *u, std::size_t n, float culling_band, std::uint32_t const *nearest, float *dvx, float *dvy,
+    float *dvz, float *dpx, float *dpy, float *dpz, hipStream_t stream
 ) {
     if (n == 0 || bvh.triangle_count == 0)
         return;
+    if (nearest != nullptr) {
+        distance_scene const sc{bvh.nodes.ptr, bvh.node_count, bvh.triangles.ptr, bvh.triangle_count,
+                                bvh.root.reference};
+        distance_adjoint_nearest_kernel<<<grid_for(n, k_block), k_block, 0, stream>>>(
+            sc, px, py, pz, u, nearest, n, bvh.primitive_indices.ptr, g.i0.ptr, g.i1.ptr, g.i2.ptr, dvx, dvy, dvz,
+            dpx, dpy, dpz);
+        GWN_LAUNCH_CHECK();
+        return;
+    }
     device_buffer<std::uint32_t> order;
     if (n >= (1u << 16)) {
         order.allocate(n, stream);

@@ -43,6 +43,7 @@ EXPORTED_SYMBOLS = (
     "gwn_compute_winding_number_antipodal_batch_f32", "gwn_compute_winding_number_antipodal_batch_ex_f32",
     "gwn_compute_winding_gradient_antipodal_batch_f32", "gwn_compute_winding_gradient_antipodal_batch_ex_f32",
     "gwn_compute_winding_taylor_adjoint_batch_f32", "gwn_compute_unsigned_distance_adjoint_batch_f32",
+    "gwn_compute_unsigned_distance_batch_ex_f32", "gwn_compute_unsigned_distance_adjoint_batch_ex_f32",
     "gwn_batch_create", "gwn_batch_destroy", "gwn_batch_refit_moments", "gwn_batch_query",
     "gwn_batch_set_tuning", "gwn_batch_get_tuning", "gwn_batch_tune", "gwn_batch_capture", "gwn_batch_replay",
     "gwn_compute_query_order_f32", "gwn_radix_sort_pairs_u64", "gwn_radix_sort_pairs_u32",
@@ -175,6 +176,9 @@ def _load():
             vp, vp, vp, vp, vp, vp, vp, u64, f32, vp, vp, vp, vp, vp, vp, vp, vp],
         "gwn_compute_unsigned_distance_adjoint_batch_f32": [
             vp, vp, vp, vp, vp, vp, u64, f32, vp, vp, vp, vp, vp, vp, vp, vp],
+        "gwn_compute_unsigned_distance_batch_ex_f32": [vp, vp, vp, vp, vp, vp, u64, f32, vp, vp],
+        "gwn_compute_unsigned_distance_adjoint_batch_ex_f32": [
+            vp, vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp],
         "gwn_batch_create": [vp, u64, f32, vp, vp],
         "gwn_batch_destroy": [vp],
         "gwn_batch_refit_moments": [vp, vp],
@@ -519,12 +523,20 @@ def gwn_compute_winding_gradient_taylor_batch(bvh: BvhObject, moment: MomentObje
 
 
 def gwn_compute_unsigned_distance_batch(bvh: BvhObject, qx, qy, qz, out, culling_band: float = float("inf"),
-                                        config: QueryConfig | None = None, stream=None) -> Status:
-    """gwn_compute_unsigned_distance_batch (include/gwn/gwn_query.cuh:129-180)."""
-    bad = _check_soa(qx, qy, qz, [out])
+                                        config: QueryConfig | None = None, stream=None, nearest=None) -> Status:
+    """gwn_compute_unsigned_distance_batch (include/gwn/gwn_query.cuh:129-180).
+
+    ``nearest`` (optional uint32/int32 device array, one entry per query) receives each query's
+    nearest triangle record for ``gwn_compute_unsigned_distance_adjoint_batch(..., nearest=...)``:
+    0xFFFFFFFF where the distance is the band or zero, 0xFFFFFFFE on stack overflow."""
+    bad = _check_soa(qx, qy, qz, [out] + ([nearest] if nearest is not None else []))
     if bad:
         return bad
     cfg = _config(config)
+    if nearest is not None:
+        return _status(_lib.gwn_compute_unsigned_distance_batch_ex_f32(
+            bvh.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _dptr(nearest), _numel(qx),
+            float(culling_band), ctypes.byref(cfg), _stream(stream)))
     return _status(_lib.gwn_compute_unsigned_distance_batch_f32(
         bvh.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx), float(culling_band),
         ctypes.byref(cfg), _stream(stream)))
@@ -627,8 +639,12 @@ def gwn_compute_winding_taylor_adjoint_batch(geometry: GeometryObject, bvh: BvhO
 def gwn_compute_unsigned_distance_adjoint_batch(geometry: GeometryObject, bvh: BvhObject, px, py, pz, grad_d,
                                                 grad_vx, grad_vy, grad_vz, grad_px=None, grad_py=None,
                                                 grad_pz=None, culling_band: float = float("inf"),
-                                                config: QueryConfig | None = None, stream=None) -> Status:
-    """dL/dV (added) and dL/dp (written when given) of L = sum grad_d * d for the unsigned distance."""
+                                                config: QueryConfig | None = None, stream=None,
+                                                nearest=None) -> Status:
+    """dL/dV (added) and dL/dp (written when given) of L = sum grad_d * d for the unsigned distance.
+
+    With ``nearest`` (the forward's records from ``gwn_compute_unsigned_distance_batch``) no
+    traversal runs; ``culling_band`` and ``config`` are then unused."""
     n = _numel(px)
     if _numel(py) != n or _numel(pz) != n or _numel(grad_d) != n:
         return Status(Error.INVALID_ARGUMENT, "Query SoA spans require identical lengths.")
@@ -637,6 +653,13 @@ def gwn_compute_unsigned_distance_adjoint_batch(geometry: GeometryObject, bvh: B
     nv = geometry.vertex_count()
     if any(_numel(a) != nv for a in (grad_vx, grad_vy, grad_vz)):
         return Status(Error.INVALID_ARGUMENT, "Vertex gradient span sizes differ from vertex count.")
+    if nearest is not None:
+        if _numel(nearest) != n:
+            return Status(Error.INVALID_ARGUMENT, "Nearest-record span size differs from point count.")
+        return _status(_lib.gwn_compute_unsigned_distance_adjoint_batch_ex_f32(
+            geometry.handle, bvh.handle, _dptr(px), _dptr(py), _dptr(pz), _dptr(nearest), _dptr(grad_d), n,
+            _dptr(grad_vx), _dptr(grad_vy), _dptr(grad_vz), _dptr(grad_px), _dptr(grad_py), _dptr(grad_pz),
+            _stream(stream)))
     cfg = _config(config)
     return _status(_lib.gwn_compute_unsigned_distance_adjoint_batch_f32(
         geometry.handle, bvh.handle, _dptr(px), _dptr(py), _dptr(pz), _dptr(grad_d), n, float(culling_band),

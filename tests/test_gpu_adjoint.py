@@ -141,3 +141,47 @@ def test_adjoint_leaf_root_and_validation(gwn, orc, dev):
     st = gwn.gwn_compute_winding_taylor_adjoint_batch(geom, bvh, mom, *[_t(a, dev) for a in q], _t(u, dev),
                                                       z[0], z[1], z[2][:2])
     assert st.error == gwn.Error.INVALID_ARGUMENT
+
+
+def test_distance_adjoint_from_forward_records(gwn, orc, dev):
+    """The forward's nearest records (gwn_compute_unsigned_distance_batch_ex_f32) name a triangle at
+    exactly the returned distance, and the adjoint from them (no traversal) equals the re-traversal
+    adjoint: dL/dp bit for bit, dL/dV up to atomic summation order.  Morton-ordered (70,000 points)
+    and input-ordered batches; a culling band makes some points record nothing."""
+    import torch
+
+    mesh = synth.open_soup(24, 12, seed=3)
+    geom, bvh, _ = _objects(gwn, mesh, 0)
+    _, _, tris = bvh.download()
+    tris = np.asarray(tris, np.float32).reshape(-1, 9)
+    V = geom.vertex_count()
+    for n, band in ((4096, float("inf")), (70000, 0.3)):
+        p = synth.uniform_box_points(13, n, (-1.4,) * 3, (1.4,) * 3)
+        u = np.random.default_rng(n).normal(size=n).astype(np.float32)
+        d = [_t(a, dev) for a in p]
+        dist = torch.empty(n, dtype=torch.float32, device=dev)
+        plain = torch.empty_like(dist)
+        near = torch.empty(n, dtype=torch.int32, device=dev)
+        gwn.gwn_compute_unsigned_distance_batch(bvh, *d, dist, culling_band=band, nearest=near).raise_if_error()
+        gwn.gwn_compute_unsigned_distance_batch(bvh, *d, plain, culling_band=band).raise_if_error()
+        outs = []
+        for use_records in (False, True):
+            dv = [torch.zeros(V, dtype=torch.float32, device=dev) for _ in range(3)]
+            dp = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
+            gwn.gwn_compute_unsigned_distance_adjoint_batch(
+                geom, bvh, *d, _t(u, dev), *dv, *dp, culling_band=band,
+                nearest=near if use_records else None).raise_if_error()
+            torch.cuda.synchronize()
+            outs.append((np.stack([a.cpu().numpy() for a in dv]), np.stack([a.cpu().numpy() for a in dp])))
+        dist, plain, rec = dist.cpu().numpy(), plain.cpu().numpy(), near.cpu().numpy().view(np.uint32)
+        assert np.array_equal(dist.view(np.uint32), plain.view(np.uint32))
+        valid = rec < tris.shape[0]
+        assert valid.any() and (~valid).any() == np.isfinite(band)
+        for i in np.flatnonzero(valid)[:256]:
+            t = tris[rec[i]]
+            a, e1, e2 = t[0:3], t[3:6] - t[0:3], t[6:9] - t[0:3]
+            d2 = orc.point_triangle_distance2_f32(np.float32([p[0][i], p[1][i], p[2][i]]), a, a + e1, a + e2)
+            assert np.float32(np.sqrt(np.float32(d2))) == dist[i]
+        (v0, p0), (v1, p1) = outs
+        assert np.array_equal(p0.view(np.uint32), p1.view(np.uint32))
+        assert np.abs(v0 - v1).max() <= 1e-5 * max(1.0, np.abs(v0).max())
