@@ -17,9 +17,15 @@
 // Precision split: traversal and far-field Taylor terms in FP32; a leaf triangle within four edge
 // lengths of the query in FP64 on the exact vertices, farther leaf triangles in FP32; every term
 // accumulated in FP64.
+//
+// Products are contracted into FMAs only within one source expression (contract(on), not the
+// default across statements), so every kernel that evaluates a term -- per-lane, wave, gradient,
+// batched -- rounds it identically and their results agree bit for bit.
 #include "gwn_core.hpp"
 
 #include <cmath>
+
+#pragma clang fp contract(on)
 
 namespace gwn {
 namespace {

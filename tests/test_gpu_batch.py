@@ -1,9 +1,8 @@
-"""Batched mode (gwn_batch_*): per-mesh results equal to sequential single-mesh Taylor batches with
-the same parameters (same trees, same accept/descend tests, same term order per query), in both
-traversals, before and after tuning, and across hipGraph replays with other GPU work in between.
-The batch kernel is a separate instantiation of the same term code and the compiler contracts
-products into FMAs differently in each, so every far-field term may differ by an ulp; over the
-~1000 terms of a query that is a few 1e-6 of |w| (1e-5 * (1 + |w|) allowed)."""
+"""Batched mode (gwn_batch_*): per-mesh results bit-identical to sequential single-mesh Taylor
+batches with the same parameters (same trees, same accept/descend tests, same term order per
+query, and terms contracted into FMAs only within each source expression, so every kernel rounds
+them alike), in both traversals, before and after tuning, and across hipGraph replays with other
+GPU work in between."""
 
 import numpy as np
 import pytest
@@ -73,10 +72,9 @@ def _check(items, refs):
 
     torch.cuda.synchronize()
     for k, ((_, _, _, _, _, out), (_, ref)) in enumerate(zip(items, refs)):
-        got, want = out.cpu().numpy().astype(np.float64), ref.cpu().numpy().astype(np.float64)
-        assert np.isfinite(got).all() == np.isfinite(want).all(), f"mesh {k}: NaN pattern"
-        assert (np.abs(got - want) <= 1e-5 * (1 + np.abs(want))).all(), \
-            f"mesh {k}: max diff {np.abs(got - want).max():.3e}"
+        got, want = out.cpu().numpy(), ref.cpu().numpy()
+        diff = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+        assert diff.size == 0, f"mesh {k}: {diff.size} results differ, max {np.abs(got - want).max():.3e}"
 
 
 @pytest.mark.parametrize("order", [0, 1, 2])
