@@ -7,8 +7,8 @@ surface along the face normal.  Seeded synthetic data (splitmix64 from 0xB3A9E5D
 
 One step = one Taylor order-2 winding batch over all 16M queries (Morton presort + traversal),
 FP32 traversal and far field, near-field triangle terms with FP64 numerators and denominators,
-at an accuracy scale (6.5) that keeps max |dW| <= 1e-4 against the FP64 exact winding number on a
-seeded, strided subsample of 262,144 queries.  Inputs,
+at an accuracy scale (6.25) that keeps max |dW| <= 1e-4 against the FP64 exact winding number on a
+seeded, strided subsample of 1,048,576 queries.  Inputs,
 BVH and moments are resident in HBM before the timed region.
 
 Also reported (outside the timed steps): the exact winding batch with the same FP64 near field
@@ -53,8 +53,8 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--queries", type=int, default=16_777_216)
     p.add_argument("--order", type=int, default=2)
-    p.add_argument("--beta", type=float, default=6.5, help="Taylor accuracy scale")
-    p.add_argument("--check-queries", type=int, default=262_144, help="FP64 subsample for |dW|")
+    p.add_argument("--beta", type=float, default=6.25, help="Taylor accuracy scale")
+    p.add_argument("--check-queries", type=int, default=1_048_576, help="FP64 subsample for |dW|")
     p.add_argument("--exact-queries", type=int, default=32_768, help="subsample timed for the exact form")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline duration")
     p.add_argument("--no-cpu-baseline", action="store_true")
