@@ -1077,28 +1077,53 @@ void launch_taylor(
         throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
 }
 
-// Sum of v over the 64 lanes of the wave, returned in every lane.  DPP steps: swap within pairs
-// and quads, half-row and row mirrors (sums of 8 and 16 lanes), then row 15 -> next row and lane 31
-// -> rows 2-3 broadcasts leave the total in lane 63.  Every lane must be active.
+
 template <int Ctrl, int RowMask>
 __device__ inline float dpp_f32(float x) {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), Ctrl, RowMask, 0xf, false));
 }
 
-__device__ inline float wave_sum(float v) {
-    v += dpp_f32<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
-    v += dpp_f32<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
-    v += dpp_f32<0x141, 0xf>(v);  // row_half_mirror
-    v += dpp_f32<0x140, 0xf>(v);  // row_mirror
-    v += dpp_f32<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-    v += dpp_f32<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+// The two results of a v_permlane{16,32}_swap, forced into registers before use: summing them
+// straight from the builtin's pair, hipcc (ROCm 7.2) emitted `v_add_f32 vA, vA, vA` -- the first
+// result twice -- after the swap; the empty asm keeps both operands distinct.
+__device__ inline float swap_pair_sum(int x, int y) {
+    asm volatile("" : "+v"(x), "+v"(y));
+    return __builtin_bit_cast(float, x) + __builtin_bit_cast(float, y);
 }
 
-// Adds the wave's per-lane values f[0..F) into d[0..F): one atomic per value from one lane when
-// several lanes contribute, the contributing lane's own atomics when only one does.
+// Wave totals of 12 per-lane values with a transposing butterfly (gfx950): v_permlane32_swap adds
+// the upper half-wave of value k + 6 to the lower half of value k (and the reverse), leaving value k
+// in lanes 0-31 and k + 6 in lanes 32-63; v_permlane16_swap does the same between the rows of each
+// half; DPP then sums within each 16-lane row.  On return every lane of row r (lanes 16r..16r+15)
+// holds in out[k] the wave total of value k + 3r.  About 30 VALU operations instead of 12 DPP
+// reductions (84).  Every lane must be active.
+__device__ inline void wave_sum12(float const (&f)[12], float (&out)[3]) {
+    float g[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        auto const p = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(int, f[k]),
+                                                        __builtin_bit_cast(int, f[k + 6]), false, false);
+        g[k] = swap_pair_sum(p[0], p[1]);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        auto const p = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(int, g[k]),
+                                                        __builtin_bit_cast(int, g[k + 3]), false, false);
+        float h = swap_pair_sum(p[0], p[1]);
+        h += dpp_f32<0xb1, 0xf>(h);   // quad_perm [1, 0, 3, 2]
+        h += dpp_f32<0x4e, 0xf>(h);   // quad_perm [2, 3, 0, 1]
+        h += dpp_f32<0x141, 0xf>(h);  // row_half_mirror
+        h += dpp_f32<0x140, 0xf>(h);  // row_mirror
+        out[k] = h;
+    }
+}
+
+// Adds the wave's per-lane values f[0..F) (F <= 12) into d[0..F): the contributing lane's own
+// atomics when a single lane contributes, otherwise the wave totals, one atomic per value, issued
+// by the first lane of each row.
 template <int F>
 __device__ inline void wave_atomic_add(float *__restrict__ d, float const (&f)[F], std::uint64_t mask) {
+    static_assert(F <= 12, "at most 12 values per wave sum");
     int const lane = threadIdx.x & 63;
     if (__builtin_popcountll(mask) == 1) {
         if (mask & (1ull << lane)) {
@@ -1108,14 +1133,20 @@ __device__ inline void wave_atomic_add(float *__restrict__ d, float const (&f)[F
         }
         return;
     }
-    float sum[F];
+    float v[12];
 #pragma unroll
-    for (int k = 0; k < F; ++k)
-        sum[k] = wave_sum(f[k]);
-    if (lane == 0) {
+    for (int k = 0; k < 12; ++k)
+        v[k] = k < F ? f[k] : 0.0f;
+    float out[3];
+    wave_sum12(v, out);
+    if ((lane & 15) == 0) {
+        int const row = lane >> 4;
 #pragma unroll
-        for (int k = 0; k < F; ++k)
-            atomicAdd(d + k, sum[k]);
+        for (int k = 0; k < 3; ++k) {
+            int const idx = k + 3 * row;
+            if (idx < F)
+                atomicAdd(d + idx, out[k]);
+        }
     }
 }
 
