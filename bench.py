@@ -329,8 +329,13 @@ def large_extras(args, dev, stream, bvh, d, idx, ref) -> dict:
         del mom
     ms = _event_ms(stream, lambda: gwn.gwn_compute_unsigned_distance_batch(bvh, *d, out).raise_if_error())
     res["distance"] = {"mqueries_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "queries": n,
-                       "traversal": "reference order, per lane (bit-exact)"}
+                       "traversal": "reference order, four lanes per point (bit-exact)"}
     log(f"distance: {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mq/s)")
+    lane_cfg = gwn.QueryConfig(lanes_per_point=1)
+    ms = _event_ms(stream, lambda: gwn.gwn_compute_unsigned_distance_batch(bvh, *d, out, config=lane_cfg).raise_if_error())
+    res["distance_one_lane"] = {"mqueries_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "queries": n,
+                                "traversal": "reference order, one lane per point (bit-exact)"}
+    log(f"distance (one lane per point): {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mq/s)")
     wave_cfg = gwn.QueryConfig(sort_queries=2)
     ms = _event_ms(stream, lambda: gwn.gwn_compute_unsigned_distance_batch(bvh, *d, out, config=wave_cfg).raise_if_error())
     res["distance_wave"] = {"mqueries_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "queries": n,

@@ -66,12 +66,16 @@ typedef struct gwn_batch_handle *gwn_batch_t;
  * batch also takes 2: Morton order with a wave-cooperative traversal (a wave walks each node once
  * for its 64 points, each point pruning by its own best distance).  Its distances are each
  * point's minimum over the triangles it reaches, which can differ from the reference order's in
- * the last FP32 bits where a box distance rounds above a contained triangle's distance. */
+ * the last FP32 bits where a box distance rounds above a contained triangle's distance.
+ * lanes_per_point (unsigned distance batch only; other batches ignore it): 0 (default: four),
+ * 1, or 4.  With four, the lanes of a quad share each BVH4 node, one child slot each; distances
+ * and nearest records equal the one-lane traversal's bit for bit.  Ignored with
+ * sort_queries = 2. */
 typedef struct gwn_query_config {
     int32_t block_size;
     int32_t stack_capacity;
     int32_t sort_queries;
-    int32_t reserved;
+    int32_t lanes_per_point;
 } gwn_query_config;
 
 /* Host-side description of a built BVH (the scalar members of the reference's

@@ -57,7 +57,13 @@ gwn::query_launch to_launch(gwn_query_config const &c) {
     q.block_size = c.block_size;
     q.stack_capacity = c.stack_capacity;
     q.sort_queries = c.sort_queries;
+    q.lanes_per_point = c.lanes_per_point;
     return q;
+}
+
+void validate_lanes(gwn_query_config const &c) {
+    if (c.lanes_per_point != 0 && c.lanes_per_point != 1 && c.lanes_per_point != 4)
+        gwn::throw_invalid("Lanes per point must be 0 (auto), 1 or 4.");
 }
 
 void require_bvh(gwn_bvh_handle const *bvh) {
@@ -561,6 +567,7 @@ int gwn_compute_unsigned_distance_batch_f32(
     return guarded("gwn_compute_unsigned_distance_batch", [&] {
         gwn_query_config const c = resolve_config(config);
         validate_config(c, true);
+        validate_lanes(c);
         require_bvh(bvh);
         require_stack(*bvh, c.stack_capacity);
         require_query_storage(count, qx, qy, qz, out);
@@ -577,6 +584,7 @@ int gwn_compute_unsigned_distance_batch_ex_f32(
     return guarded("gwn_compute_unsigned_distance_batch", [&] {
         gwn_query_config const c = resolve_config(config);
         validate_config(c, true);
+        validate_lanes(c);
         require_bvh(bvh);
         require_stack(*bvh, c.stack_capacity);
         require_query_storage(count, qx, qy, qz, out);

@@ -332,6 +332,7 @@ struct query_launch {
     int block_size = 256;
     int stack_capacity = 64;
     int sort_queries = -1; // -1 auto, 0 off, 1 on; 2 (distance batch): on, wave-cooperative traversal
+    int lanes_per_point = 0; // distance batch: 0 auto, 1 or 4 lanes per point (same results)
 };
 
 void winding_exact(

@@ -109,11 +109,12 @@ class QueryConfig:
     block_size: int = DEFAULT_BLOCK_SIZE
     stack_capacity: int = DEFAULT_STACK_CAPACITY
     sort_queries: int = -1
+    lanes_per_point: int = 0  # unsigned distance: 0 (four) or 1 or 4 lanes per point, same results
 
 
 class _CConfig(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int32), ("stack_capacity", ctypes.c_int32),
-                ("sort_queries", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("sort_queries", ctypes.c_int32), ("lanes_per_point", ctypes.c_int32)]
 
 
 class _CBvhInfo(ctypes.Structure):
@@ -233,7 +234,7 @@ def _dptr(t) -> int | None:
 
 def _config(cfg: QueryConfig | None) -> _CConfig:
     cfg = cfg or QueryConfig()
-    return _CConfig(cfg.block_size, cfg.stack_capacity, cfg.sort_queries, 0)
+    return _CConfig(cfg.block_size, cfg.stack_capacity, cfg.sort_queries, cfg.lanes_per_point)
 
 
 class _Handle:

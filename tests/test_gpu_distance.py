@@ -19,44 +19,50 @@ def _t(a, dev):
     return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
 
-def _distance(gwn, dev, bvh, qx, qy, qz, band=float("inf"), sort=-1, stack=64):
+def _distance(gwn, dev, bvh, qx, qy, qz, band=float("inf"), sort=-1, stack=64, lanes=0, nearest=False):
     import torch
 
     out = torch.full((qx.shape[0],), -7.0, dtype=torch.float32, device=dev)
+    near = torch.full((qx.shape[0],), 7, dtype=torch.int32, device=dev) if nearest else None
     gwn.gwn_compute_unsigned_distance_batch(
-        bvh, _t(qx, dev), _t(qy, dev), _t(qz, dev), out, culling_band=band,
-        config=gwn.QueryConfig(sort_queries=sort, stack_capacity=stack)).raise_if_error()
+        bvh, _t(qx, dev), _t(qy, dev), _t(qz, dev), out, culling_band=band, nearest=near,
+        config=gwn.QueryConfig(sort_queries=sort, stack_capacity=stack, lanes_per_point=lanes)).raise_if_error()
     torch.cuda.synchronize()
+    if nearest:
+        return out.cpu().numpy(), near.cpu().numpy().view(np.uint32)
     return out.cpu().numpy()
 
 
-def test_distance_layered_kats(gwn, dev):
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_distance_layered_kats(gwn, dev, lanes):
     """tests/unit_gwn_query_distance_ray.cu:222-270, 350-381 and 555-599."""
     f = np.float32
     _, one = build_gpu(gwn, synth.layered_triangles(1))
-    assert _distance(gwn, dev, one, f([0.25]), f([0.25]), f([2]))[0] == 2.0
-    assert _distance(gwn, dev, one, f([2]), f([0]), f([0]))[0] == 1.0
-    assert _distance(gwn, dev, one, f([0.25]), f([0.25]), f([2]), band=0.5)[0] == 0.5
-    assert _distance(gwn, dev, one, f([0.25]), f([0.25]), f([2]), band=-4.0)[0] == 0.0
+    assert _distance(gwn, dev, one, f([0.25]), f([0.25]), f([2]), lanes=lanes)[0] == 2.0
+    assert _distance(gwn, dev, one, f([2]), f([0]), f([0]), lanes=lanes)[0] == 1.0
+    assert _distance(gwn, dev, one, f([0.25]), f([0.25]), f([2]), band=0.5, lanes=lanes)[0] == 0.5
+    assert _distance(gwn, dev, one, f([0.25]), f([0.25]), f([2]), band=-4.0, lanes=lanes)[0] == 0.0
     _, twelve = build_gpu(gwn, synth.layered_triangles(12))
-    d = _distance(gwn, dev, twelve, f([0.25, 0.25, 0.0]), f([0.25, 0.25, 0.25]), f([10.25, 30, 10.25]), band=2.0)
+    d = _distance(gwn, dev, twelve, f([0.25, 0.25, 0.0]), f([0.25, 0.25, 0.25]), f([10.25, 30, 10.25]), band=2.0,
+                  lanes=lanes)
     assert d.tolist() == [0.25, 2.0, 0.25]
     _, four = build_gpu(gwn, synth.layered_triangles(4))
     info = four.info()
     assert info["internal_stack_bound"] == 0 and info["packed_stack_bound"] == 3
-    assert _distance(gwn, dev, four, f([0.25]), f([0.25]), f([3.5]), stack=1)[0] == 0.5
+    assert _distance(gwn, dev, four, f([0.25]), f([0.25]), f([3.5]), stack=1, lanes=lanes)[0] == 0.5
 
 
+@pytest.mark.parametrize("lanes", [1, 4], ids=["lane", "quad"])
 @pytest.mark.parametrize("sort", [0, 1], ids=["input_order", "morton_order"])
 @pytest.mark.parametrize("name", ["torus", "soup", "displaced_256x128", "icosphere4"])
-def test_distance_matches_oracle_bitwise(gwn, orc, dev, name, sort):
+def test_distance_matches_oracle_bitwise(gwn, orc, dev, name, sort, lanes):
     mesh = MESHES[name]
     _, bvh = build_gpu(gwn, mesh)
     ref_bvh = orc.build_bvh4(mesh)
     qx, qy, qz = synth.mixed_queries(mesh, 8192, seed=41, band=0.01) if name != "soup" else \
         synth.uniform_box_points(41, 8192, (-3.0,) * 3, (3.0,) * 3)
     for band in (float("inf"), 0.05):
-        got = _distance(gwn, dev, bvh, qx, qy, qz, band=band, sort=sort)
+        got = _distance(gwn, dev, bvh, qx, qy, qz, band=band, sort=sort, lanes=lanes)
         ref = orc.unsigned_distance_f32(ref_bvh, qx, qy, qz, culling_band=band)
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), f"band {band}"
 
@@ -94,17 +100,45 @@ def test_distance_on_surface_points_is_zero_or_tiny(gwn, orc, dev):
 
 def test_distance_large_mesh_subsample(gwn, orc, dev):
     """1,048,576-triangle mesh, 262,144 queries through the Morton-ordered path; a strided
-    subsample checked bit for bit against the oracle."""
-    import torch
-
+    subsample checked bit for bit against the oracle, and the one- and four-lane traversals equal
+    at every point (distances and nearest records)."""
     mesh = synth.config_large_mesh()
     _, bvh = build_gpu(gwn, mesh)
     qx, qy, qz = synth.mixed_queries(mesh, 262_144, seed=5, band=0.01)
-    got = _distance(gwn, dev, bvh, qx, qy, qz)
+    got, rec = _distance(gwn, dev, bvh, qx, qy, qz, lanes=4, nearest=True)
     idx = np.arange(0, 262_144, 64)
     ref = orc.unsigned_distance_f32(orc.build_bvh4(mesh), qx[idx], qy[idx], qz[idx])
     assert np.array_equal(got[idx].view(np.uint32), ref.view(np.uint32))
-    assert np.all(np.isfinite(got)) and torch.cuda.is_available()
+    assert np.all(np.isfinite(got))
+    lane, lane_rec = _distance(gwn, dev, bvh, qx, qy, qz, lanes=1, nearest=True)
+    assert np.array_equal(got.view(np.uint32), lane.view(np.uint32))
+    assert np.array_equal(rec, lane_rec)
+
+
+@pytest.mark.parametrize("name", ["torus", "soup", "displaced_256x128"])
+def test_distance_quad_records_equal_lane_records(gwn, dev, name):
+    """Four lanes per point against one lane per point, in input and Morton order, with and
+    without a band: identical distances and nearest records (first triangle attaining the minimum
+    in the traversal's order), including zero, band and overflow codes."""
+    mesh = MESHES[name]
+    _, bvh = build_gpu(gwn, mesh)
+    qx, qy, qz = synth.mixed_queries(mesh, 20_000, seed=47, band=0.01) if name != "soup" else \
+        synth.uniform_box_points(47, 20_000, (-3.0,) * 3, (3.0,) * 3)
+    if name == "torus":  # points on vertices: distance zero
+        qx[:64], qy[:64], qz[:64] = mesh.vx[:64], mesh.vy[:64], mesh.vz[:64]
+    for sort in (0, 1):
+        for band in (float("inf"), 0.05, 0.0):
+            d1, r1 = _distance(gwn, dev, bvh, qx, qy, qz, band=band, sort=sort, lanes=1, nearest=True)
+            d4, r4 = _distance(gwn, dev, bvh, qx, qy, qz, band=band, sort=sort, lanes=4, nearest=True)
+            assert np.array_equal(d1.view(np.uint32), d4.view(np.uint32)), (sort, band)
+            assert np.array_equal(r1, r4), (sort, band)
+    info = bvh.info()
+    small = int(info["internal_stack_bound"]) - 1
+    if small >= 1:  # a stack below the bound is rejected up front: no overflow path to compare
+        st = gwn.gwn_compute_unsigned_distance_batch(
+            bvh, *[_t(a, dev) for a in (qx, qy, qz)], _t(np.zeros_like(qx), dev),
+            config=gwn.QueryConfig(stack_capacity=small, lanes_per_point=4))
+        assert st.error == gwn.Error.INVALID_ARGUMENT
 
 
 def test_distance_validation_contract(gwn, dev):
@@ -120,5 +154,7 @@ def test_distance_validation_contract(gwn, dev):
     assert st.error == gwn.Error.INVALID_ARGUMENT and "Output span size" in st.message
     st = gwn.gwn_compute_unsigned_distance_batch(bvh, *q, out, config=gwn.QueryConfig(stack_capacity=1))
     assert st.error == gwn.Error.INVALID_ARGUMENT and "Traversal stack capacity" in st.message
+    st = gwn.gwn_compute_unsigned_distance_batch(bvh, *q, out, config=gwn.QueryConfig(lanes_per_point=2))
+    assert st.error == gwn.Error.INVALID_ARGUMENT and "Lanes per point" in st.message
     empty = torch.zeros(0, device=dev)
     assert gwn.gwn_compute_unsigned_distance_batch(bvh, empty, empty, empty, empty).is_ok()
