@@ -1118,36 +1118,24 @@ __device__ inline void wave_sum12(float const (&f)[12], float (&out)[3]) {
     }
 }
 
-// Adds the wave's per-lane values f[0..F) (F <= 12) into d[0..F): the contributing lane's own
-// atomics when a single lane contributes, otherwise the wave totals, one atomic per value, issued
-// by the first lane of each row.
+// Adds the wave's per-lane values f[0..F) (F <= 12) into d[0..F): wave totals from wave_sum12, then
+// one atomic instruction whose active lanes 16r + k (k < 3) add total k + 3r, so the F adds of a
+// call go to F consecutive floats in one request.  A lone contributing lane goes the same way (its
+// values plus zeros are its values exactly): one request instead of F.
 template <int F>
-__device__ inline void wave_atomic_add(float *__restrict__ d, float const (&f)[F], std::uint64_t mask) {
+__device__ inline void wave_atomic_add(float *__restrict__ d, float const (&f)[F]) {
     static_assert(F <= 12, "at most 12 values per wave sum");
-    int const lane = threadIdx.x & 63;
-    if (__builtin_popcountll(mask) == 1) {
-        if (mask & (1ull << lane)) {
-#pragma unroll
-            for (int k = 0; k < F; ++k)
-                atomicAdd(d + k, f[k]);
-        }
-        return;
-    }
     float v[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k)
         v[k] = k < F ? f[k] : 0.0f;
     float out[3];
     wave_sum12(v, out);
-    if ((lane & 15) == 0) {
-        int const row = lane >> 4;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            int const idx = k + 3 * row;
-            if (idx < F)
-                atomicAdd(d + idx, out[k]);
-        }
-    }
+    int const lane = threadIdx.x & 63;
+    int const k = lane & 15;
+    int const idx = k + 3 * (lane >> 4);
+    if (k < 3 && idx < F)
+        atomicAdd(d + idx, k == 0 ? out[0] : (k == 1 ? out[1] : out[2]));
 }
 
 // The adjoint of taylor_adjoint_kernel, wave-cooperative: a wave walks the tree once for its 64
@@ -1246,7 +1234,7 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_adjoint_wave_kernel(
             }
             std::uint64_t const amask = __ballot(accepted);
             if (amask != 0ull)
-                wave_atomic_add<F>(dslot + (static_cast<std::size_t>(node) * 4 + s) * 12, f, amask);
+                wave_atomic_add<F>(dslot + (static_cast<std::size_t>(node) * 4 + s) * 12, f);
             std::uint64_t const dmask = __ballot(descend);
             if (dmask == 0ull)
                 continue;
@@ -1258,7 +1246,7 @@ __global__ __launch_bounds__(k_taylor_block) void taylor_adjoint_wave_kernel(
                     t9[k] = 0.0f;
                 if (dmask & my_bit)
                     adjoint_leaf_terms(tris[r], x, y, z, sud, t9, dq);
-                wave_atomic_add<9>(dtri + static_cast<std::size_t>(r) * 9, t9, dmask);
+                wave_atomic_add<9>(dtri + static_cast<std::size_t>(r) * 9, t9);
                 continue;
             }
             if (next != k_invalid) {
