@@ -1140,9 +1140,9 @@ __device__ inline void wave_atomic_add(float *__restrict__ d, float const (&f)[F
 
 // The adjoint of taylor_adjoint_kernel, wave-cooperative: a wave walks the tree once for its 64
 // (Morton-ordered) queries with the same per-query accept/descend decisions, and the field and
-// vertex adjoints of a child slot or near triangle are summed over the wave before one lane adds
-// them, instead of every query adding its own (the top slots are shared by every query, so
-// per-query atomics serialise on them).
+// vertex adjoints of a child slot or near triangle are summed over the wave and added with one
+// atomic instruction over consecutive floats, instead of every query adding its own (the top
+// slots are shared by every query, so per-query atomics serialise on them).
 template <int Order, int StackCap>
 __global__ __launch_bounds__(k_taylor_block) void taylor_adjoint_wave_kernel(
     float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node,
