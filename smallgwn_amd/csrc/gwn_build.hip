@@ -592,10 +592,19 @@ __global__ __launch_bounds__(k_block) void collapse_level_kernel(
     __shared__ std::uint32_t s_wave[k_block / 64];
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
+    // Only the first ceil(count / block) blocks take part (the launch is sized for the level's
+    // bound, not its count): they claim exactly the tiles 0..tiles-1, each claim made by a running
+    // block, so the look-back below never waits on an unclaimed tile.  A level past the last has
+    // count 0 (the level table starts zeroed) and every block returns at once; no launch reads
+    // the next status array then.
+    std::uint64_t const level_tiles = (static_cast<std::uint64_t>(levels[depth].count) + k_block - 1) / k_block;
+    if (blockIdx.x >= level_tiles)
+        return;
+    std::uint32_t const live = level_tiles < gridDim.x ? static_cast<std::uint32_t>(level_tiles) : gridDim.x;
     if (threadIdx.x == 0)
         s_tile = atomicAdd(&levels[depth].tiles_claimed, 1u);
-    // The next level scans over the other status array; clear it for that launch.
-    for (std::uint32_t i = blockIdx.x * k_block + threadIdx.x; i < status_words; i += gridDim.x * k_block)
+    // The next level scans over the other status array; the live blocks clear it for that launch.
+    for (std::uint32_t i = blockIdx.x * k_block + threadIdx.x; i < status_words; i += live * k_block)
         status_next[i] = 0u;
     __syncthreads();
     std::uint32_t const start = levels[depth].start;
