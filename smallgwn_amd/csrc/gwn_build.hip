@@ -42,7 +42,7 @@ __device__ inline float float_from_order_key(std::uint32_t k) {
     return __uint_as_float(u);
 }
 
-// Per-axis min/max over all vertices; one atomic per wave per extremum.
+// Per-axis min/max over all vertices; one atomic per block per extremum.
 __global__ __launch_bounds__(k_block) void scene_bounds_kernel(
     float const *__restrict__ x, float const *__restrict__ y, float const *__restrict__ z,
     std::size_t nv, std::uint32_t *__restrict__ ordered
@@ -66,7 +66,27 @@ __global__ __launch_bounds__(k_block) void scene_bounds_kernel(
             hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], d));
         }
     }
+    // One atomic per bound and block (the waves meet in LDS first): same-address atomics from
+    // every wave of every block had serialised.
+    __shared__ float s_b[k_block / 64][6];
+    int const wave = static_cast<int>(threadIdx.x >> 6);
     if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s_b[wave][k] = lo[k];
+            s_b[wave][3 + k] = hi[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 1; w < k_block / 64; ++w) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = fminf(lo[k], s_b[w][k]);
+                hi[k] = fmaxf(hi[k], s_b[w][3 + k]);
+            }
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             atomicMin(&ordered[k], float_order_key(lo[k]));
@@ -775,7 +795,7 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, int method, s
         scene_init_kernel<<<1, 1, 0, stream>>>(scene_words.ptr);
         GWN_LAUNCH_CHECK();
         unsigned grid = grid_for(V, k_block);
-        grid = std::min(grid, 1024u);
+        grid = std::min(grid, 512u);
         scene_bounds_kernel<<<grid, k_block, 0, stream>>>(g.x.ptr, g.y.ptr, g.z.ptr, V, scene_words.ptr);
         GWN_LAUNCH_CHECK();
         scene_finalize_kernel<<<1, 1, 0, stream>>>(scene_words.ptr, scene);
