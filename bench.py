@@ -249,7 +249,8 @@ def main() -> int:
                   "hploc_internal_stack_bound": hploc_info["internal_stack_bound"],
                   "lbvh_ms": lbvh_s * 1e3, "lbvh_mtri_per_s": T / lbvh_s / 1e6,
                   "lbvh_nodes": lbvh_info["node_count"],
-                  "lbvh_internal_stack_bound": lbvh_info["internal_stack_bound"]},
+                  "lbvh_internal_stack_bound": lbvh_info["internal_stack_bound"],
+                  "roofline": build_roofline(T, hploc_s, lbvh_s, refit_s, info["node_count"], args.order)},
         "work": {"child_tests": tests, "accepted": accepts, "near_field_triangles": leaves},
         "roofline": {
             "bound": "valu",
@@ -300,6 +301,30 @@ def _event_ms(stream, fn, reps=3) -> float:
     e[1].record(stream)
     torch.cuda.synchronize()
     return e[0].elapsed_time(e[1]) / reps
+
+
+# Compulsory HBM bytes of a build: about 800 per triangle (keys, ids and boxes through the sort
+# passes, binary nodes, the collapse and the records; LBVH with u64 keys).  A moment refit reads
+# each wide node (128 B) and its triangles (36 B each), and reads and writes the node's payload
+# (136 B) and its record (order 0/1/2: 128/256/384 B).
+BUILD_BYTES_PER_TRIANGLE = 800
+HBM_PEAK_GBPS = 8000.0
+
+
+def build_roofline(tris: int, hploc_s: float, lbvh_s: float, refit_s: float, nodes: int, order: int) -> dict:
+    """HBM roofline of the builds and the moment refit against their compulsory bytes.  All three
+    are latency-bound climbs and level launches, so the fractions are small; they say how far the
+    chains are from the bandwidth they would need."""
+    record = {0: 128, 1: 256, 2: 384}[order]
+    refit_bytes = nodes * (128 + 136 + record) + tris * 36
+    build_bytes = BUILD_BYTES_PER_TRIANGLE * tris
+    out = {"bound": "hbm", "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+           "bytes_per_triangle_build": BUILD_BYTES_PER_TRIANGLE, "refit_bytes": refit_bytes}
+    for name, nbytes, sec in (("hploc_r8", build_bytes, hploc_s), ("lbvh", build_bytes, lbvh_s),
+                              ("refit_moment", refit_bytes, refit_s)):
+        gbps = nbytes / sec / 1e9
+        out[name] = {"achieved": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+    return out
 
 
 def large_extras(args, dev, stream, bvh, d, idx, ref) -> dict:
