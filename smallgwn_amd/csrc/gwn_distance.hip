@@ -592,16 +592,23 @@ __global__ __launch_bounds__(k_block) void distance_adjoint_kernel(
     std::uint32_t const *__restrict__ i2, float *dvx, float *dvy, float *dvz, float *__restrict__ dpx,
     float *__restrict__ dpy, float *__restrict__ dpz
 ) {
-    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    // Four lanes per point (distance_quad_traverse: the nearest record of distance_traverse); the
+    // quad's first lane finishes the point.  The grid covers 4 n lanes.
+    constexpr int k_quads = k_block / 4;
+    __shared__ std::uint32_t s_stack[StackCap * k_quads];
+    std::size_t const i = (static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x) >> 2;
     if (i >= n)
         return;
+    int const j = static_cast<int>(threadIdx.x & 3);
     std::size_t const q = order ? order[i] : i;
     float const p[3] = {px[q], py[q], pz[q]};
     float const band = culling_band < 0.0f ? 0.0f : culling_band;
     float best = band * band;
     std::uint32_t rec = 0xffffffffu;
-    int const st = distance_traverse<StackCap>(sc, p[0], p[1], p[2], best, &rec);
-    distance_adjoint_finish(sc, p, q, st, rec, u[q], prim, i0, i1, i2, dvx, dvy, dvz, dpx, dpy, dpz);
+    int const st = distance_quad_traverse<StackCap>(sc, j, p[0], p[1], p[2], best, rec,
+                                                    s_stack + (threadIdx.x >> 2), k_quads);
+    if (j == 0)
+        distance_adjoint_finish(sc, p, q, st, rec, u[q], prim, i0, i1, i2, dvx, dvy, dvz, dpx, dpy, dpz);
 }
 
 
@@ -695,7 +702,7 @@ void unsigned_distance_adjoint(
     distance_scene const sc{bvh.nodes.ptr, bvh.node_count, bvh.triangles.ptr, bvh.triangle_count, bvh.root.reference};
     int const need = static_cast<int>(bvh.internal_stack_bound);
 #define GWN_DADJ(C)                                                                                            \
-    distance_adjoint_kernel<C><<<grid_for(n, k_block), k_block, 0, stream>>>(                                 \
+    distance_adjoint_kernel<C><<<grid_for(4 * n, k_block), k_block, 0, stream>>>(                             \
         sc, px, py, pz, u, order.ptr, n, culling_band, bvh.primitive_indices.ptr, g.i0.ptr, g.i1.ptr, g.i2.ptr, \
         dvx, dvy, dvz, dpx, dpy, dpz)
     if (need <= 32)
