@@ -3,8 +3,9 @@
 //
 // No reference counterpart: the reference runs one mesh per call.  A batch holds, for every mesh,
 // its BVH and moment objects and its query / output device pointers:
-//   - refit: every mesh's moment refit is enqueued without host synchronisation (buffers are
-//     allocated once when the batch is made, so replays allocate nothing);
+//   - refit: every mesh's moment refit runs in two launches over all meshes' nodes, without host
+//     synchronisation (buffers are allocated once when the batch is made, so replays allocate
+//     nothing);
 //   - query: one persistent launch whose blocks take chunks of one mesh's queries from a global
 //     counter (taylor_batch_kernel), so small and large meshes share the machine instead of
 //     running 4096 launches back to back; every result equals the single-mesh batch's;
@@ -32,6 +33,9 @@ struct gwn_batch_handle {
     std::vector<gwn_moment_handle *> moments;
     std::vector<gwn::batch_mesh> host_meshes;
     std::vector<gwn::moment_refit_work> work;
+    std::vector<gwn::moment_refit_mesh> host_refits;  // meshes with an internal root
+    gwn::device_buffer<gwn::moment_refit_mesh> refits;
+    std::uint64_t refit_nodes = 0;
     gwn::device_buffer<gwn::batch_mesh> meshes;
     gwn::device_buffer<std::uint64_t> counter;
     std::vector<gwn::device_buffer<std::uint32_t>> orders;  // per mesh Morton visiting order
@@ -102,10 +106,37 @@ void check_storage(gwn_batch_handle const &b) {
 
 void batch_destroy(gwn_batch_handle *b) { delete b; }
 
+// Every mesh's moment refit in two launches over all meshes' nodes (refit_moment_batch); the
+// records equal those of per-mesh refit_moment_enqueue calls.
 void batch_refit(gwn_batch_handle &b, hipStream_t stream) {
     check_storage(b);
-    for (std::size_t i = 0; i < b.bvhs.size(); ++i)
-        refit_moment_enqueue(*b.bvhs[i], *b.moments[i], b.work[i], stream);
+    for (std::size_t i = 0; i < b.bvhs.size(); ++i) {
+        b.moments[i]->node_count = b.bvhs[i]->node_count;
+        b.moments[i]->bvh_revision = b.bvhs[i]->revision;
+    }
+    refit_moment_batch(b.refits.ptr, static_cast<std::uint32_t>(b.host_refits.size()), b.refit_nodes, b.order,
+                       stream);
+}
+
+void upload_refit_table(gwn_batch_handle &b, hipStream_t stream) {
+    b.host_refits.clear();
+    b.refit_nodes = 0;
+    for (std::size_t i = 0; i < b.bvhs.size(); ++i) {
+        gwn_bvh_handle const &bvh = *b.bvhs[i];
+        if (!ref_internal(bvh.root.reference) || bvh.node_count == 0)
+            continue;
+        b.host_refits.push_back(moment_refit_mesh{
+            bvh.nodes.ptr, bvh.triangles.ptr, bvh.parent.ptr, bvh.internal_children.ptr, b.work[i].arrivals.ptr,
+            b.work[i].pending.ptr, b.moments[i]->records.ptr, b.moments[i]->slot_area.ptr, bvh.node_count,
+            bvh.triangle_count, b.refit_nodes});
+        b.refit_nodes += bvh.node_count;
+    }
+    b.refits.allocate(std::max<std::size_t>(1, b.host_refits.size()), stream);
+    if (!b.host_refits.empty()) {
+        GWN_HIP(hipMemcpyAsync(b.refits.ptr, b.host_refits.data(), b.host_refits.size() * sizeof(moment_refit_mesh),
+                               hipMemcpyHostToDevice, stream));
+        GWN_HIP(hipStreamSynchronize(stream));
+    }
 }
 
 void batch_query(gwn_batch_handle &b, hipStream_t stream) {
@@ -142,10 +173,13 @@ gwn_batch_handle *batch_create(
         b->work.resize(bvhs.size());
         std::uint32_t bound = 0;
         for (std::size_t i = 0; i < bvhs.size(); ++i) {
+            if (moments[i]->order != b->order)
+                throw_invalid("batch: Every mesh's moment must have the same Taylor order.");
             refit_moment_prepare(*bvhs[i], *moments[i], b->work[i], stream);
             bound = std::max(bound, bvhs[i]->internal_stack_bound);
             b->storage.push_back(storage_of(*bvhs[i], *moments[i]));
         }
+        upload_refit_table(*b, stream);
         b->cap = bound <= 32 ? 32 : (bound <= 64 ? 64 : 128);
         if (bound > 128)
             throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");

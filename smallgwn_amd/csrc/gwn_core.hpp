@@ -325,6 +325,23 @@ struct moment_refit_work {
 };
 void refit_moment_prepare(gwn_bvh_handle const &bvh, gwn_moment_handle &m, moment_refit_work &w, hipStream_t stream);
 void refit_moment_enqueue(gwn_bvh_handle const &bvh, gwn_moment_handle &m, moment_refit_work &w, hipStream_t stream);
+// One mesh of a batched moment refit (device table entry): the BVH's arrays, the moment's record
+// and slot-area arrays, the refit work buffers, and the mesh's first node in the batch numbering.
+struct moment_refit_mesh {
+    bvh_node4 const *nodes;
+    tri_record const *tris;
+    std::uint32_t const *parent;
+    std::uint8_t const *internal_children;
+    std::uint32_t *arrivals;  // node_count counters + error word
+    void *pending;
+    float4 *records;
+    float *slot_area;
+    std::uint64_t node_count, tri_count, first_node;
+};
+// The refits of every mesh in `meshes` (internal roots only, first_node ascending from 0, all of
+// one order) in two launches, each mesh's records equal to refit_moment_enqueue's.
+void refit_moment_batch(moment_refit_mesh const *meshes, std::uint32_t count, std::uint64_t total_nodes, int order,
+                        hipStream_t stream);
 // In-place bounds refit: triangle records and child boxes from the geometry, root box updated.
 void refit_bounds(gwn_geometry_handle const &geometry, gwn_bvh_handle &bvh, hipStream_t stream);
 

@@ -258,16 +258,17 @@ __device__ inline void write_fields(float *fields, int slot, aabb const &bnd, pa
 #undef GWN_FIELD
 }
 
+// One thread's bottom-up pass: from node `start` when it has no internal children, then up the
+// parent links while this thread is the last to arrive at the parent.
 template <int Order>
-__global__ __launch_bounds__(k_block) void moment_refit_kernel(
-    bvh_node4 const *__restrict__ nodes, std::size_t node_count, tri_record const *__restrict__ tris,
+__device__ inline void moment_refit_from(
+    std::size_t start, bvh_node4 const *__restrict__ nodes, std::size_t node_count, tri_record const *__restrict__ tris,
     std::size_t tri_count, std::uint32_t const *__restrict__ parent,
     std::uint8_t const *__restrict__ internal_children, std::uint32_t *arrivals,
     payload<Order> *pending, float4 *__restrict__ records, float *__restrict__ slot_area, std::uint32_t *error
 ) {
     constexpr int RV = moment_record_vec4(Order);
-    std::size_t const start = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (start >= node_count || internal_children[start] != 0)
+    if (internal_children[start] != 0)
         return;
     std::size_t n = start;
     for (;;) {
@@ -330,6 +331,77 @@ __global__ __launch_bounds__(k_block) void moment_refit_kernel(
         arrivals[pn] = 0u;
         n = pn;
     }
+}
+
+template <int Order>
+__global__ __launch_bounds__(k_block) void moment_refit_kernel(
+    bvh_node4 const *__restrict__ nodes, std::size_t node_count, tri_record const *__restrict__ tris,
+    std::size_t tri_count, std::uint32_t const *__restrict__ parent,
+    std::uint8_t const *__restrict__ internal_children, std::uint32_t *arrivals,
+    payload<Order> *pending, float4 *__restrict__ records, float *__restrict__ slot_area, std::uint32_t *error
+) {
+    std::size_t const start = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    if (start >= node_count)
+        return;
+    moment_refit_from<Order>(start, nodes, node_count, tris, tri_count, parent, internal_children, arrivals,
+                             pending, records, slot_area, error);
+}
+
+// Batched refits: one thread per node of every mesh in the table (mesh found by binary search over
+// the node prefixes), so many meshes' refits take two launches instead of four each.
+__device__ inline std::uint32_t refit_mesh_of(moment_refit_mesh const *meshes, std::uint32_t count, std::uint64_t g) {
+    std::uint32_t lo = 0, hi = count;  // meshes[lo].first_node <= g, and g < meshes[hi].first_node
+    while (hi - lo > 1) {
+        std::uint32_t const mid = lo + (hi - lo) / 2;
+        if (meshes[mid].first_node <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Zeroes what moment_refit_enqueue's fills zero: each node's record and slot areas, its arrival
+// counter, and the mesh's error word.
+template <int Order>
+__global__ __launch_bounds__(k_block) void moment_refit_batch_clear_kernel(
+    moment_refit_mesh const *__restrict__ meshes, std::uint32_t count, std::uint64_t total
+) {
+    std::uint64_t const g = static_cast<std::uint64_t>(blockIdx.x) * k_block + threadIdx.x;
+    if (g >= total)
+        return;
+    moment_refit_mesh const m = meshes[refit_mesh_of(meshes, count, g)];
+    std::uint64_t const n = g - m.first_node;
+    constexpr int RV = moment_record_vec4(Order);
+    for (int k = 0; k < RV; ++k)
+        m.records[n * RV + k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int s = 0; s < 4; ++s)
+        m.slot_area[n * 4 + s] = 0.0f;
+    m.arrivals[n] = 0u;
+    if (n == 0)
+        m.arrivals[m.node_count] = 0u;
+}
+
+template <int Order>
+__global__ __launch_bounds__(k_block) void moment_refit_batch_kernel(
+    moment_refit_mesh const *__restrict__ meshes, std::uint32_t count, std::uint64_t total
+) {
+    std::uint64_t const g = static_cast<std::uint64_t>(blockIdx.x) * k_block + threadIdx.x;
+    if (g >= total)
+        return;
+    moment_refit_mesh const m = meshes[refit_mesh_of(meshes, count, g)];
+    moment_refit_from<Order>(g - m.first_node, m.nodes, m.node_count, m.tris, m.tri_count, m.parent,
+                             m.internal_children, m.arrivals, static_cast<payload<Order> *>(m.pending), m.records,
+                             m.slot_area, m.arrivals + m.node_count);
+}
+
+template <int Order>
+void launch_refit_batch(moment_refit_mesh const *meshes, std::uint32_t count, std::uint64_t total, hipStream_t stream) {
+    unsigned const grid = grid_for(total, k_block);
+    moment_refit_batch_clear_kernel<Order><<<grid, k_block, 0, stream>>>(meshes, count, total);
+    GWN_LAUNCH_CHECK();
+    moment_refit_batch_kernel<Order><<<grid, k_block, 0, stream>>>(meshes, count, total);
+    GWN_LAUNCH_CHECK();
 }
 
 template <int Order>
@@ -398,6 +470,18 @@ void refit_moment_enqueue(gwn_bvh_handle const &bvh, gwn_moment_handle &m, momen
     case 0: enqueue_refit<0>(bvh, m, w, stream); break;
     case 1: enqueue_refit<1>(bvh, m, w, stream); break;
     default: enqueue_refit<2>(bvh, m, w, stream); break;
+    }
+}
+
+void refit_moment_batch(moment_refit_mesh const *meshes, std::uint32_t count, std::uint64_t total_nodes, int order,
+                        hipStream_t stream) {
+    if (count == 0 || total_nodes == 0)
+        return;
+    switch (order) {
+    case 0: launch_refit_batch<0>(meshes, count, total_nodes, stream); break;
+    case 1: launch_refit_batch<1>(meshes, count, total_nodes, stream); break;
+    case 2: launch_refit_batch<2>(meshes, count, total_nodes, stream); break;
+    default: throw_invalid("Taylor order must be 0, 1, or 2.");
     }
 }
 
