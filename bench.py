@@ -8,21 +8,30 @@ surface along the face normal.  Seeded synthetic data (splitmix64 from 0xB3A9E5D
 One step = one Taylor order-2 winding batch over all 16M queries (Morton presort + traversal),
 FP32 traversal and far field, near-field triangle terms with FP64 numerators and denominators,
 at an accuracy scale (6.25) that keeps max |dW| <= 1e-4 against the FP64 exact winding number on a
-seeded, strided subsample of 1,048,576 queries.  Inputs,
-BVH and moments are resident in HBM before the timed region.
+seeded, strided subsample of 1,048,576 queries.  Inputs, BVH and moments are resident in HBM
+before the timed region.
 
-Also reported (outside the timed steps): the exact winding batch with the same FP64 near field
-(the plain form the Taylor path is compared with), BVH build and moment refit throughput, the
-roofline of the traversal kernel from its own work counters, and the CPU oracle's replay of the
-same algorithm on a bounded query sample.
+Also reported (outside the timed steps), each with its roofline where the kernel has one:
+- the exact winding batch with the same FP64 near field on 524,288 queries (enough to fill the
+  GPU), the plain form the Taylor path is compared with;
+- BVH build (H-PLOC radius 8, LBVH) and moment refit throughput against compulsory HBM bytes;
+- the small configuration (icosphere 4x, 65,536 queries and rays);
+- the large mesh's other batches (Taylor at scale 2, unsigned distance, first-hit rays);
+- the precision-split report (262K-triangle open soup, 4,194,304 log-offset queries);
+- the differentiation step (131,072 triangles, 1,048,576 queries, 262,144 targets);
+- the batched mode (4,096 meshes of 200 to 200,000 triangles and 64 to 1,048,576 queries);
+- the CPU baseline: the oracle's C replay of the same traversal on the same H-PLOC tree, timed on
+  the host's cores, plus the small configuration on the CPU and the accuracy of the
+  differentiation step's gradient against the FP64 oracle adjoint.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
+import platform
+import statistics
 import sys
 import time
 
@@ -38,8 +47,16 @@ METRIC = "Mqueries/s Taylor vs exact at |dW|<=1e-4 vs FP64; adjoint steps/s; bui
 FLOPS_CHILD_TEST = 10
 FLOPS_ACCEPT = {0: 14, 1: 34, 2: 79}
 FLOPS_TRIANGLE = 87            # solid angle: 80 FLOP + 3 sqrt + 3 div + 1 atan2
+FLOPS_DIST_BOX = 12            # point-box distance per child
+FLOPS_DIST_TRI = 45            # point-triangle distance
+FLOPS_RAY_SLAB = 14            # ray-box slab test per child
+FLOPS_RAY_TRI = 60             # oriented Pluecker ray-triangle test
+# Adjoint extras on top of the forward's work with the same decisions: 12 field adjoints of an
+# accepted order-1 slot (dw/dP, dw/dN, dw/dT), and the 9 vertex adjoints of a near triangle.
+FLOPS_ADJ_ACCEPT = 60
+FLOPS_ADJ_TRIANGLE = 120
 PEAK_FP32_TFLOPS = 157.3       # MI355X dense FP32 vector
-PEAK_HBM_GBS = 8000.0
+HBM_PEAK_GBPS = 8000.0
 
 
 def log(msg: str) -> None:
@@ -55,24 +72,47 @@ def parse_args():
     p.add_argument("--order", type=int, default=2)
     p.add_argument("--beta", type=float, default=6.25, help="Taylor accuracy scale")
     p.add_argument("--check-queries", type=int, default=1_048_576, help="FP64 subsample for |dW|")
-    p.add_argument("--exact-queries", type=int, default=32_768, help="subsample timed for the exact form")
-    p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline duration")
+    p.add_argument("--exact-queries", type=int, default=524_288, help="queries timed for the exact form")
+    p.add_argument("--cpu-seconds", type=float, default=3.0, help="target duration of one CPU baseline run")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation workload")
     p.add_argument("--adjoint-steps", type=int, default=5)
     p.add_argument("--no-batched", action="store_true", help="skip the batched-mode workload")
     p.add_argument("--no-extras", action="store_true",
-                   help="skip the large-mesh extras (Taylor at scale 2, distance, rays) and the precision-split report")
+                   help="skip the small configuration, the large-mesh extras and the precision-split report")
     p.add_argument("--builder", choices=("hploc", "lbvh"), default="hploc", help="BVH builder of the large mesh")
-    p.add_argument("--batched-meshes", type=int, default=256)
+    p.add_argument("--batched-meshes", type=int, default=4096)
+    p.add_argument("--precision-queries", type=int, default=4_194_304)
     return p.parse_args()
+
+
+def _event_ms(stream, fn, reps=3) -> float:
+    """Average milliseconds of fn() between HIP events on the launch stream, after one warm-up."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    e[0].record(stream)
+    for _ in range(reps):
+        fn()
+    e[1].record(stream)
+    torch.cuda.synchronize()
+    return e[0].elapsed_time(e[1]) / reps
+
+
+def _valu_roofline(flops: float, ms: float, kernel: str, note: str) -> dict:
+    tflops = flops / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / PEAK_FP32_TFLOPS, 4), "kernel": kernel, "kernel_ms": round(ms, 4),
+            "flops_per_launch": float(flops), "note": note}
 
 
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (queries shard by "
-              "contiguous ranges with a replicated BVH, see DESIGN.md)", file=sys.stderr)
+              "contiguous ranges with a replicated BVH; nothing here uses a second GPU)", file=sys.stderr)
         return 2
 
     import torch
@@ -153,6 +193,7 @@ def main() -> int:
     ).raise_if_error()
     c = cnt.view(n, 3).to(torch.int64).sum(0).cpu().numpy()
     tests, accepts, leaves = (int(v) for v in c)
+    del cnt
     # Every near-field triangle at its FP32 weight: its FP64 part (numerator and denominator in the
     # near zone) is not weighted up, so the figure is a lower bound on the FLOPs done.
     fp32_equiv = tests * FLOPS_CHILD_TEST + accepts * FLOPS_ACCEPT[args.order] + leaves * FLOPS_TRIANGLE
@@ -170,21 +211,18 @@ def main() -> int:
     max_err, p99_err = float(err.max()), float(np.quantile(err, 0.99))
     log(f"accuracy on {idx.size} queries: max |dW| {max_err:.3e}, p99 {p99_err:.3e}")
 
-    # The exact form with the same FP64 near field, on a subsample (cost is linear in queries).
+    # The exact form with the same FP64 near field, on enough queries to fill the GPU (cost is
+    # linear in the queries once every SIMD holds several waves).
     me = min(args.exact_queries, n)
     eidx = torch.from_numpy(np.linspace(0, n - 1, me).astype(np.int64)).to(dev)
     esub = [t[eidx].contiguous() for t in d]
     eout = torch.empty(me, dtype=torch.float64, device=dev)
-    gwn.gwn_compute_winding_number_exact_batch_ex(bvh, *esub, eout, gwn.Precision.SPLIT).raise_if_error()
-    torch.cuda.synchronize()
-    ee = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    ee[0].record(stream)
-    gwn.gwn_compute_winding_number_exact_batch_ex(bvh, *esub, eout, gwn.Precision.SPLIT).raise_if_error()
-    ee[1].record(stream)
-    torch.cuda.synchronize()
-    exact_ms = ee[0].elapsed_time(ee[1])
+    exact_ms = _event_ms(stream, lambda: gwn.gwn_compute_winding_number_exact_batch_ex(
+        bvh, *esub, eout, gwn.Precision.SPLIT).raise_if_error(), reps=1)
     exact_rate = me / (exact_ms * 1e-3) / 1e6
+    exact_flops = float(me) * mesh.triangle_count * FLOPS_TRIANGLE
     log(f"exact split: {exact_rate:.4f} Mq/s on {me} queries ({exact_ms:.1f} ms)")
+    del esub, eout
 
     # Build and refit throughput.
     def timed(fn, reps):
@@ -208,10 +246,15 @@ def main() -> int:
     mom_t = gwn.MomentObject(args.order)
     refit_s = timed(lambda: gwn.gwn_refit_bvh_moment(bvh, mom_t).raise_if_error(), 5)
     del mom_t
+    bounds_s = timed(lambda: gwn.gwn_refit_bvh(geom, bvh).raise_if_error(), 5)
+    gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()  # moments of the refit (unchanged) bounds
     T = mesh.triangle_count
     log(f"build H-PLOC r8 {hploc_s * 1e3:.2f} ms ({T / hploc_s / 1e6:.1f} Mtri/s), LBVH {lbvh_s * 1e3:.2f} ms "
-        f"({T / lbvh_s / 1e6:.1f} Mtri/s), refit o{args.order} {refit_s * 1e3:.2f} ms")
+        f"({T / lbvh_s / 1e6:.1f} Mtri/s), refit o{args.order} {refit_s * 1e3:.2f} ms, bounds refit "
+        f"{bounds_s * 1e3:.2f} ms")
 
+    traffic = traffic_profile()
+    valu = valu_profile()
     result = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -238,19 +281,25 @@ def main() -> int:
         },
         "accuracy": {"max_abs_dw": max_err, "p99_abs_dw": p99_err, "checked_queries": int(idx.size),
                      "target": 1e-4, "meets_target": bool(max_err <= 1e-4)},
-        "exact_fp64_near_field": {"mqueries_per_s": exact_rate, "sampled_queries": me,
-                                  "taylor_speedup": value / exact_rate},
+        "exact_fp64_near_field": {
+            "mqueries_per_s": exact_rate, "timed_queries": me, "ms": exact_ms,
+            "taylor_speedup": value / exact_rate,
+            "roofline": _valu_roofline(exact_flops, exact_ms, "exact_kernel<split>",
+                                       "query x triangle pairs at the FP32 solid-angle weight (87); the "
+                                       "near-zone FP64 part is not weighted up")},
         "stages_ms": {"query_presort": round(t_order, 4), "traversal": round(t_trav, 4)},
         "build": {"method": "hploc_r8" if args.builder == "hploc" else "lbvh", "ms": build_s * 1e3,
                   "mtri_per_s": T / build_s / 1e6,
                   "refit_moment_ms": refit_s * 1e3, "refit_mtri_per_s": T / refit_s / 1e6,
+                  "refit_bounds_ms": bounds_s * 1e3,
                   "hploc_r8_ms": hploc_s * 1e3, "hploc_r8_mtri_per_s": T / hploc_s / 1e6,
                   "hploc_nodes": hploc_info["node_count"],
                   "hploc_internal_stack_bound": hploc_info["internal_stack_bound"],
                   "lbvh_ms": lbvh_s * 1e3, "lbvh_mtri_per_s": T / lbvh_s / 1e6,
                   "lbvh_nodes": lbvh_info["node_count"],
                   "lbvh_internal_stack_bound": lbvh_info["internal_stack_bound"],
-                  "roofline": build_roofline(T, hploc_s, lbvh_s, refit_s, info["node_count"], args.order)},
+                  "roofline": build_roofline(T, hploc_s, lbvh_s, refit_s, bounds_s, info["node_count"],
+                                             args.order)},
         "work": {"child_tests": tests, "accepted": accepts, "near_field_triangles": leaves},
         "roofline": {
             "bound": "valu",
@@ -258,87 +307,142 @@ def main() -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
-            "traffic": traffic_bytes(),
+            "traffic": traffic.get("hbm_bytes_per_launch"),
+            "traffic_source": traffic.get("source"),
             "kernel": f"taylor_wave_kernel<{args.order}, split>",
             "kernel_ms": round(t_trav, 4),
             "flops_per_launch": fp32_equiv,
-            "valu_lane_utilization": valu_profile().get("valu_lane_utilization"),
+            "valu_lane_utilization": valu.get("valu_lane_utilization"),
+            "valu_source": valu.get("source"),
             "note": "FP32 FLOPs from per-query work counters (10 per child test, 79 per accepted "
                     "order-2 expansion, 87 per near-field triangle); peak is MI355X's FP32 vector "
                     "figure, which counts packed FP32, disabled on this platform (non-packed 78.6); "
-                    "lane utilization from the committed rocprofv3 --pmc summary",
+                    "traffic and lane utilization come from the rocprofv3 --pmc summaries named in "
+                    "traffic_source / valu_source, measured on the commit they state",
         },
     }
+    del sub, ref
 
     if not args.no_extras:
-        result["large_mesh_extras"] = large_extras(args, dev, stream, bvh, d, idx, ref)
+        result["config_small"] = small_config(dev, stream)
+        result["large_mesh_extras"] = large_extras(args, dev, stream, bvh, d, idx)
         result["precision_split"] = precision_report(args, dev, stream)
 
+    del d, out, order
+    adjoint_check = None
     if not args.no_adjoint:
-        del d, out, order, cnt, sub, ref, esub, eout
-        result["adjoint"] = adjoint_bench(args, dev, stream)
+        result["adjoint"], adjoint_check = adjoint_bench(args, dev, stream)
 
     if not args.no_batched:
         result["batched"] = batched_bench(args, dev, stream)
 
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(mesh, qx, qy, qz, args)
+        result["cpu_baseline"] = cpu_baseline(mesh, qx, qy, qz, args, value)
+        if not args.no_extras:
+            result["config_small"]["cpu"] = cpu_small()
+        if adjoint_check is not None:
+            result["adjoint"]["gradient_check"] = adjoint_gradient_check(adjoint_check)
+            result["adjoint"]["gradient_rel_error"] = result["adjoint"]["gradient_check"]["rel_error"]
 
     print(json.dumps(result), flush=True)
     return 0
 
 
-def _event_ms(stream, fn, reps=3) -> float:
-    """Average milliseconds of fn() between HIP events on the launch stream, after one warm-up."""
-    import torch
-
-    fn()
-    torch.cuda.synchronize()
-    e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    e[0].record(stream)
-    for _ in range(reps):
-        fn()
-    e[1].record(stream)
-    torch.cuda.synchronize()
-    return e[0].elapsed_time(e[1]) / reps
-
-
 # Compulsory HBM bytes of a build: about 800 per triangle (keys, ids and boxes through the sort
 # passes, binary nodes, the collapse and the records; LBVH with u64 keys).  A moment refit reads
-# each wide node (128 B) and its triangles (36 B each), and reads and writes the node's payload
-# (136 B) and its record (order 0/1/2: 128/256/384 B).
+# each wide node (128 B) and its triangles (36 B each), and writes and reads the node's payload
+# (136 B) and writes its record (order 0/1/2: 128/256/384 B).  A bounds refit reads each triangle's
+# indices and vertices (12 + 36 B) and writes its record (36 B), and reads and writes each node
+# (128 + 128 B) and its 24-byte box.
 BUILD_BYTES_PER_TRIANGLE = 800
-HBM_PEAK_GBPS = 8000.0
 
 
-def build_roofline(tris: int, hploc_s: float, lbvh_s: float, refit_s: float, nodes: int, order: int) -> dict:
-    """HBM roofline of the builds and the moment refit against their compulsory bytes.  All three
-    are latency-bound climbs and level launches, so the fractions are small; they say how far the
-    chains are from the bandwidth they would need."""
+def build_roofline(tris: int, hploc_s: float, lbvh_s: float, refit_s: float, bounds_s: float, nodes: int,
+                   order: int) -> dict:
+    """HBM roofline of the builds and refits against their compulsory bytes."""
     record = {0: 128, 1: 256, 2: 384}[order]
-    refit_bytes = nodes * (128 + 136 + record) + tris * 36
+    refit_bytes = nodes * (128 + 136 + 136 + record) + tris * 36
+    bounds_bytes = tris * (12 + 36 + 36) + nodes * (128 + 128 + 24 + 24)
     build_bytes = BUILD_BYTES_PER_TRIANGLE * tris
     out = {"bound": "hbm", "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-           "bytes_per_triangle_build": BUILD_BYTES_PER_TRIANGLE, "refit_bytes": refit_bytes}
+           "bytes_per_triangle_build": BUILD_BYTES_PER_TRIANGLE, "refit_bytes": refit_bytes,
+           "refit_bounds_bytes": bounds_bytes}
     for name, nbytes, sec in (("hploc_r8", build_bytes, hploc_s), ("lbvh", build_bytes, lbvh_s),
-                              ("refit_moment", refit_bytes, refit_s)):
+                              ("refit_moment", refit_bytes, refit_s), ("refit_bounds", bounds_bytes, bounds_s)):
         gbps = nbytes / sec / 1e9
         out[name] = {"achieved": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4)}
     return out
 
 
-def large_extras(args, dev, stream, bvh, d, idx, ref) -> dict:
+def small_config(dev, stream) -> dict:
+    """The small configuration on the GPU: icosphere 4x (5,120 triangles), 65,536 queries in
+    [-1.5, 1.5]^3 and 65,536 rays, LBVH, BVH4, Taylor order 1 at scale 2; exact and Taylor winding,
+    unsigned distance and first-hit rays."""
+    import torch
+
+    from smallgwn_amd import gwn, synth
+
+    mesh, q, rays = synth.config_small()
+    geom = gwn.GeometryObject()
+    gwn.gwn_upload_geometry(geom, mesh.vx, mesh.vy, mesh.vz, mesh.i0, mesh.i1, mesh.i2).raise_if_error()
+    bvh = gwn.BvhObject()
+    lbvh = gwn.BuildOptions(method=gwn.BuildMethod.LBVH)
+    gwn.gwn_build_bvh(geom, bvh, lbvh).raise_if_error()
+    mom = gwn.MomentObject(1)
+    gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+    d = [torch.from_numpy(a).to(dev) for a in q]
+    r = [torch.from_numpy(a).to(dev) for a in rays]
+    n = d[0].numel()
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    hits = torch.empty(8 * n, dtype=torch.int32, device=dev)
+    t = time.perf_counter()
+    for _ in range(10):
+        gwn.gwn_build_bvh(geom, bvh, lbvh).raise_if_error()
+        gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+    build_ms = (time.perf_counter() - t) / 10 * 1e3
+    res = {"triangles": mesh.triangle_count, "queries": n, "rays": n, "builder": "lbvh", "taylor_order": 1,
+           "accuracy_scale": 2.0, "build_and_refit_ms": build_ms}
+    for name, fn in (
+            ("exact", lambda: gwn.gwn_compute_winding_number_exact_batch(bvh, *d, out)),
+            ("taylor_o1", lambda: gwn.gwn_compute_winding_number_taylor_batch(bvh, mom, *d, out, accuracy_scale=2.0)),
+            ("distance", lambda: gwn.gwn_compute_unsigned_distance_batch(bvh, *d, out)),
+            ("ray_first_hit", lambda: gwn.gwn_compute_ray_first_hit_batch(bvh, *r, hits))):
+        ms = _event_ms(stream, lambda: fn().raise_if_error(), reps=5)
+        res[name] = {"ms": ms, "mqueries_per_s": n / (ms * 1e-3) / 1e6}
+    log(f"small config: " + ", ".join(f"{k} {v['mqueries_per_s']:.0f} Mq/s" for k, v in res.items()
+                                      if isinstance(v, dict)))
+    return res
+
+
+def _distance_ray_work(dev, bvh, d, rays, gwn):
+    import torch
+
+    n = d[0].numel()
+    work = torch.zeros(2 * n, dtype=torch.int32, device=dev)
+    gwn.gwn_compute_unsigned_distance_work(bvh, *d, work).raise_if_error()
+    dw = work.view(n, 2).to(torch.int64).sum(0).cpu().numpy()
+    gwn.gwn_compute_ray_first_hit_work(bvh, *rays, work).raise_if_error()
+    rw = work.view(n, 2).to(torch.int64).sum(0).cpu().numpy()
+    return (int(dw[0]), int(dw[1])), (int(rw[0]), int(rw[1]))
+
+
+def large_extras(args, dev, stream, bvh, d, idx) -> dict:
     """The large mesh's other batches on the same tree and queries: Taylor orders 1 and 2 at the
     reference's default accuracy scale 2 (plain precision, with their error on the checked
     subsample), the unsigned distance of all 16M queries, and 16,777,216 first-hit rays (origins on a
-    radius-3 sphere, directions toward uniform points in the unit ball)."""
+    radius-3 sphere, directions toward uniform points in the unit ball), with the distance and ray
+    rooflines from their traversals' work counters."""
     import torch
 
     from smallgwn_amd import gwn, synth
 
     n = d[0].numel()
     out = torch.empty(n, dtype=torch.float32, device=dev)
+    sub = [t[torch.from_numpy(idx).to(dev)].contiguous() for t in d]
+    ref = torch.empty(idx.size, dtype=torch.float64, device=dev)
+    gwn.gwn_compute_winding_number_exact_batch_ex(bvh, *sub, ref, gwn.Precision.FP64).raise_if_error()
     ref_h = ref.cpu().numpy()
+    del sub, ref
     res = {}
     for order in (1, 2):
         mom = gwn.MomentObject(order)
@@ -352,44 +456,54 @@ def large_extras(args, dev, stream, bvh, d, idx, ref) -> dict:
         log(f"taylor o{order} plain scale 2: {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mq/s), "
             f"max |dW| {err.max():.3e}")
         del mom
+    rays = [torch.from_numpy(a).to(dev) for a in synth.rays_toward_ball(n, seed=synth.DEFAULT_SEED + 3)]
+    (dbox, dtri), (rbox, rtri) = _distance_ray_work(dev, bvh, d, rays, gwn)
+    dist_flops = dbox * FLOPS_DIST_BOX + dtri * FLOPS_DIST_TRI
+    ray_flops = rbox * FLOPS_RAY_SLAB + rtri * FLOPS_RAY_TRI
     ms = _event_ms(stream, lambda: gwn.gwn_compute_unsigned_distance_batch(bvh, *d, out).raise_if_error())
     res["distance"] = {"mqueries_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "queries": n,
-                       "traversal": "reference order, four lanes per point (bit-exact)"}
-    log(f"distance: {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mq/s)")
+                       "traversal": "reference order, four lanes per point (bit-exact)",
+                       "work": {"box_tests": dbox, "triangle_tests": dtri},
+                       "roofline": _valu_roofline(dist_flops, ms, "distance_quad_kernel",
+                                                  "12 FLOP per point-box test, 45 per point-triangle distance, "
+                                                  "counted by the one-lane traversal the quad replays")}
+    log(f"distance: {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mq/s), frac {res['distance']['roofline']['frac']}")
     lane_cfg = gwn.QueryConfig(lanes_per_point=1)
     ms = _event_ms(stream, lambda: gwn.gwn_compute_unsigned_distance_batch(bvh, *d, out, config=lane_cfg).raise_if_error())
     res["distance_one_lane"] = {"mqueries_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "queries": n,
                                 "traversal": "reference order, one lane per point (bit-exact)"}
-    log(f"distance (one lane per point): {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mq/s)")
     wave_cfg = gwn.QueryConfig(sort_queries=2)
     ms = _event_ms(stream, lambda: gwn.gwn_compute_unsigned_distance_batch(bvh, *d, out, config=wave_cfg).raise_if_error())
     res["distance_wave"] = {"mqueries_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "queries": n,
                             "traversal": "wave-cooperative (sort_queries=2, within FP32 ulps)"}
-    log(f"distance (wave traversal): {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mq/s)")
     del out
-    rays = [torch.from_numpy(a).to(dev) for a in synth.rays_toward_ball(n, seed=synth.DEFAULT_SEED + 3)]
     hits = torch.empty(8 * n, dtype=torch.int32, device=dev)
     ms = _event_ms(stream, lambda: gwn.gwn_compute_ray_first_hit_batch(bvh, *rays, hits).raise_if_error())
     status = hits.view(torch.uint8).view(n, 32)[:, 28]
     hit_frac = float((status == int(gwn.RayStatus.HIT)).to(torch.float32).mean())
-    res["ray_first_hit"] = {"mrays_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "rays": n, "hit_fraction": hit_frac}
-    log(f"rays: {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mrays/s), hit fraction {hit_frac:.3f}")
+    res["ray_first_hit"] = {"mrays_per_s": n / (ms * 1e-3) / 1e6, "ms": ms, "rays": n, "hit_fraction": hit_frac,
+                            "work": {"slab_tests": rbox, "triangle_tests": rtri},
+                            "roofline": _valu_roofline(ray_flops, ms, "ray_kernel<packed>",
+                                                       "14 FLOP per slab test, 60 per Pluecker triangle test")}
+    log(f"rays: {ms:.2f} ms ({n / (ms * 1e-3) / 1e6:.1f} Mrays/s), hit fraction {hit_frac:.3f}, "
+        f"frac {res['ray_first_hit']['roofline']['frac']}")
     return res
 
 
 def precision_report(args, dev, stream) -> dict:
-    """Precision-split report: open soup of ~262K triangles (displaced sphere, 30% of faces deleted,
-    2% with jittered duplicate vertices), 262,144 queries at normal offsets log-uniform in
-    [1e-6, 1e-1] from random surface points.  Exact, Taylor order 2 (at the headline accuracy
-    scale) and antipodal winding, each plain (FP32) and split (FP64 near field), against the all-FP64
-    exact form: max and p99 |dW| and throughput per mode."""
+    """Precision-split configuration: open soup of ~262K triangles (displaced sphere, 30% of faces
+    deleted, 2% with jittered duplicate vertices), 4,194,304 queries at normal offsets log-uniform in
+    [1e-6, 1e-1] from random surface points.  Taylor order 2 (at the headline accuracy scale) runs on
+    all queries; the exact and antipodal batches, plain (FP32) and split (FP64 near field), on a
+    strided 262,144-query subsample; max and p99 |dW| against the all-FP64 exact form on that
+    subsample, and throughput per mode."""
     import torch
 
     from smallgwn_amd import gwn, synth
 
     t0 = time.time()
     soup = synth.open_soup(612, 306, seed=synth.DEFAULT_SEED)
-    q = synth.near_surface_log_queries(soup, 262_144, seed=synth.DEFAULT_SEED + 5, lo=1e-6, hi=1e-1)
+    q = synth.near_surface_log_queries(soup, args.precision_queries, seed=synth.DEFAULT_SEED + 5, lo=1e-6, hi=1e-1)
     n = q[0].shape[0]
     geom = gwn.GeometryObject()
     gwn.gwn_upload_geometry(geom, soup.vx, soup.vy, soup.vz, soup.i0, soup.i1, soup.i2).raise_if_error()
@@ -400,41 +514,54 @@ def precision_report(args, dev, stream) -> dict:
     chain = gwn.BoundaryChainObject()
     gwn.gwn_build_boundary_chain(geom, chain).raise_if_error()
     d = [torch.from_numpy(a).to(dev) for a in q]
-    ref = torch.empty(n, dtype=torch.float64, device=dev)
-    gwn.gwn_compute_winding_number_exact_batch_ex(bvh, *d, ref, gwn.Precision.FP64).raise_if_error()
+    m = min(262_144, n)
+    idx = np.linspace(0, n - 1, m).astype(np.int64)
+    didx = torch.from_numpy(idx).to(dev)
+    sub = [a[didx].contiguous() for a in d]
+    ref = torch.empty(m, dtype=torch.float64, device=dev)
+    gwn.gwn_compute_winding_number_exact_batch_ex(bvh, *sub, ref, gwn.Precision.FP64).raise_if_error()
     ref_h = ref.cpu().numpy()
-    o32 = torch.empty(n, dtype=torch.float32, device=dev)
-    o64 = torch.empty(n, dtype=torch.float64, device=dev)
+    full = torch.empty(n, dtype=torch.float32, device=dev)
+    o32 = torch.empty(m, dtype=torch.float32, device=dev)
+    o64 = torch.empty(m, dtype=torch.float64, device=dev)
+    stack = gwn.QueryConfig(stack_capacity=max(64, bvh.info()["internal_stack_bound"]))
     modes = {
-        "exact_plain": (o32, lambda: gwn.gwn_compute_winding_number_exact_batch(bvh, *d, o32)),
-        "exact_split": (o64, lambda: gwn.gwn_compute_winding_number_exact_batch_ex(bvh, *d, o64, gwn.Precision.SPLIT)),
-        "taylor_o2_plain": (o32, lambda: gwn.gwn_compute_winding_number_taylor_batch(
-            bvh, mom, *d, o32, accuracy_scale=args.beta)),
-        "taylor_o2_split": (o32, lambda: gwn.gwn_compute_winding_number_taylor_batch(
-            bvh, mom, *d, o32, accuracy_scale=args.beta, precision=gwn.Precision.SPLIT)),
-        "antipodal_plain": (o32, lambda: gwn.gwn_compute_winding_number_antipodal_batch(geom, bvh, chain, *d, o32)),
-        "antipodal_split": (o32, lambda: gwn.gwn_compute_winding_number_antipodal_batch(
-            geom, bvh, chain, *d, o32, precision=gwn.Precision.SPLIT)),
+        "exact_plain": (o32, m, lambda: gwn.gwn_compute_winding_number_exact_batch(bvh, *sub, o32)),
+        "exact_split": (o64, m, lambda: gwn.gwn_compute_winding_number_exact_batch_ex(bvh, *sub, o64, gwn.Precision.SPLIT)),
+        "taylor_o2_plain": (full, n, lambda: gwn.gwn_compute_winding_number_taylor_batch(
+            bvh, mom, *d, full, accuracy_scale=args.beta, config=stack)),
+        "taylor_o2_split": (full, n, lambda: gwn.gwn_compute_winding_number_taylor_batch(
+            bvh, mom, *d, full, accuracy_scale=args.beta, precision=gwn.Precision.SPLIT, config=stack)),
+        "antipodal_plain": (o32, m, lambda: gwn.gwn_compute_winding_number_antipodal_batch(
+            geom, bvh, chain, *sub, o32, config=stack)),
+        "antipodal_split": (o32, m, lambda: gwn.gwn_compute_winding_number_antipodal_batch(
+            geom, bvh, chain, *sub, o32, precision=gwn.Precision.SPLIT, config=stack)),
     }
-    res = {"triangles": soup.triangle_count, "queries": n, "accuracy_scale": args.beta}
-    for name, (buf, fn) in modes.items():
+    res = {"triangles": soup.triangle_count, "queries": n, "checked_queries": m, "accuracy_scale": args.beta,
+           "boundary_edges": chain.info()["edge_count"]}
+    for name, (buf, count, fn) in modes.items():
         ms = _event_ms(stream, lambda: fn().raise_if_error(), reps=1)
         w = buf.cpu().numpy().astype(np.float64)
+        if count == n:
+            w = w[idx]
         ok = np.isfinite(w)
         err = np.abs(w[ok] - ref_h[ok])
         res[name] = {"max_abs_dw": float(err.max()), "p99_abs_dw": float(np.quantile(err, 0.99)),
-                     "nan_results": int((~ok).sum()), "mqueries_per_s": n / (ms * 1e-3) / 1e6}
+                     "nan_results": int((~ok).sum()), "timed_queries": count,
+                     "mqueries_per_s": count / (ms * 1e-3) / 1e6}
         log(f"precision {name}: max |dW| {err.max():.3e}, p99 {np.quantile(err, 0.99):.3e}, "
-            f"{n / (ms * 1e-3) / 1e6:.2f} Mq/s")
+            f"{count / (ms * 1e-3) / 1e6:.2f} Mq/s on {count}")
     log(f"precision report: {soup.triangle_count} tris, {n} queries ({time.time() - t0:.1f}s)")
     return res
 
 
-def adjoint_bench(args, dev, stream) -> dict:
+def adjoint_bench(args, dev, stream):
     """Differentiation workload: 131,072-triangle displaced sphere, 1,048,576 labelled queries
     (inside a radius-0.9 sphere or not), 262,144 target points on that sphere.  One step: new
     vertex positions (a gradient step), BVH bounds and order-1 moment refit, Taylor winding and
-    distance forward, loss, adjoint backward to dL/dV."""
+    distance forward, loss, adjoint backward to dL/dV.  Also returns the inputs of the gradient
+    check (the winding adjoint on a strided query subsample), which the CPU leg compares with the
+    FP64 oracle adjoint."""
     import torch
 
     from smallgwn_amd import autograd as ag
@@ -442,7 +569,8 @@ def adjoint_bench(args, dev, stream) -> dict:
 
     mesh = synth.displaced_sphere(256, 256)
     faces = np.stack([mesh.i0, mesh.i1, mesh.i2], 1)
-    verts = torch.tensor(np.stack([mesh.vx, mesh.vy, mesh.vz], 1), device=dev, requires_grad=True)
+    initial = np.stack([mesh.vx, mesh.vy, mesh.vz], 1).astype(np.float32)
+    verts = torch.tensor(initial, device=dev, requires_grad=True)
     scene = ag.MeshScene(faces, verts, order=1, method=gwn.BuildMethod.HPLOC, accuracy_scale=2.0)
     qx, qy, qz = synth.uniform_box_points(21, 1 << 20, (-1.5,) * 3, (1.5,) * 3)
     queries = torch.tensor(np.stack([qx, qy, qz], 1), device=dev)
@@ -472,22 +600,69 @@ def adjoint_bench(args, dev, stream) -> dict:
     torch.cuda.synchronize()
     ms = ev[0].elapsed_time(ev[1]) / args.adjoint_steps
     log(f"adjoint: {ms:.2f} ms/step ({1e3 / ms:.1f} steps/s), loss {float(loss):.5f}")
-    return {"steps_per_s": 1e3 / ms, "ms_per_step": ms, "triangles": mesh.triangle_count,
-            "queries": int(queries.shape[0]), "targets": int(targets.shape[0]), "taylor_order": 1,
-            "builder": "hploc_r8", "step": "update + bounds/moment refit + winding and distance forward "
-                                           "+ adjoint backward + SGD update"}
+
+    # The winding adjoint kernel alone on the current positions, with its forward's work counters
+    # (the adjoint takes the same accept/descend decisions) for its roofline.
+    scene.update(verts)
+    V = mesh.vertex_count
+    q = [queries[:, k].contiguous() for k in range(3)]
+    w = torch.empty(queries.shape[0], dtype=torch.float32, device=dev)
+    nq = queries.shape[0]
+    cnt = torch.zeros(3 * nq, dtype=torch.int32, device=dev)
+    gwn.gwn_compute_winding_number_taylor_batch(scene.bvh, scene.moment, *q, w, accuracy_scale=2.0,
+                                                counters=cnt).raise_if_error()
+    tests, accepts, leaves = (int(v) for v in cnt.view(nq, 3).to(torch.int64).sum(0).cpu().numpy())
+    sig = torch.sigmoid(20.0 * (w - 0.5))
+    u = (2.0 * (sig - labels) * sig * (1.0 - sig) * 20.0 / nq).contiguous()
+    dv = [torch.zeros(V, dtype=torch.float32, device=dev) for _ in range(3)]
+
+    def adj():
+        gwn.gwn_compute_winding_taylor_adjoint_batch(scene.geometry, scene.bvh, scene.moment, *q, u, *dv,
+                                                     accuracy_scale=2.0).raise_if_error()
+
+    adj_ms = _event_ms(stream, adj, reps=3)
+    fwd_ms = _event_ms(stream, lambda: gwn.gwn_compute_winding_number_taylor_batch(
+        scene.bvh, scene.moment, *q, w, accuracy_scale=2.0).raise_if_error(), reps=3)
+    adj_flops = (tests * FLOPS_CHILD_TEST + accepts * (FLOPS_ACCEPT[1] + FLOPS_ADJ_ACCEPT)
+                 + leaves * (FLOPS_TRIANGLE + FLOPS_ADJ_TRIANGLE))
+    res = {"steps_per_s": 1e3 / ms, "ms_per_step": ms, "triangles": mesh.triangle_count,
+           "queries": nq, "targets": int(targets.shape[0]), "taylor_order": 1,
+           "builder": "hploc_r8", "step": "update + bounds/moment refit + winding and distance forward "
+                                          "+ adjoint backward + SGD update",
+           "winding_forward_ms": fwd_ms, "winding_adjoint_ms": adj_ms,
+           "work": {"child_tests": tests, "accepted": accepts, "near_field_triangles": leaves},
+           "roofline": _valu_roofline(adj_flops, adj_ms, "winding adjoint batch (Morton order, wave traversal)",
+                                      "forward weights (10 / 34 / 87) plus 60 FLOP per accepted slot's 12 field "
+                                      "adjoints and 120 per near triangle's 9 vertex adjoints; work counted by "
+                                      "the forward with the same decisions; includes the Morton presort")}
+    log(f"adjoint kernel {adj_ms:.2f} ms (forward {fwd_ms:.2f} ms), frac {res['roofline']['frac']}")
+    # Gradient check inputs: the adjoint on 4,096 strided queries with the step's upstream gradient.
+    sidx = np.linspace(0, nq - 1, 4096).astype(np.int64)
+    ts = torch.from_numpy(sidx).to(dev)
+    gs = [torch.zeros(V, dtype=torch.float32, device=dev) for _ in range(3)]
+    gwn.gwn_compute_winding_taylor_adjoint_batch(scene.geometry, scene.bvh, scene.moment,
+                                                 *[a[ts].contiguous() for a in q], u[ts].contiguous(), *gs,
+                                                 accuracy_scale=2.0).raise_if_error()
+    torch.cuda.synchronize()
+    vpos = verts.detach().cpu().numpy().astype(np.float32)
+    check = {"faces": faces, "initial": initial, "vertices": vpos, "queries": [np.ascontiguousarray(a[ts].cpu().numpy()) for a in q],
+             "u": u[ts].double().cpu().numpy(), "gpu": np.stack([g.double().cpu().numpy() for g in gs])}
+    return res, check
 
 
 def batched_bench(args, dev, stream) -> dict:
     """Batched uneven instances: meshes with triangle counts log-uniform in [200, 200,000]
     (tori and displaced spheres under random affine maps) and query counts log-uniform in
-    [64, 1,048,576] uniform in each mesh's 1.5x box; Taylor order 1, scale 2.  Sequential
-    single-mesh moment refits + query batches against the tuned batch replayed as one graph."""
+    [64, 1,048,576] uniform in each mesh's 1.5x box (drawn on the device from a seeded generator);
+    Taylor order 1, scale 2.  Sequential single-mesh moment refits + query batches against the
+    tuned batch replayed as one graph."""
     import torch
 
     from smallgwn_amd import gwn, synth
 
     rng = np.random.default_rng(2024)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(2024)
     t0 = time.time()
     items, tris, nq = [], 0, 0
     geoms = []
@@ -511,12 +686,15 @@ def batched_bench(args, dev, stream) -> dict:
         n = int(np.exp(rng.uniform(np.log(64), np.log(1 << 20))))
         lo, hi = v.min(0), v.max(0)
         c, e = (lo + hi) / 2, (hi - lo) / 2 * 1.5
-        q = [torch.from_numpy(a).to(dev) for a in synth.uniform_box_points(1000 + k, n, c - e, c + e)]
+        u = torch.rand((3, n), generator=gen, device=dev)
+        q = [(torch.tensor(float(c[j] - e[j]), device=dev) + u[j] * float(2 * e[j])).contiguous() for j in range(3)]
         out = torch.empty(n, dtype=torch.float32, device=dev)
         items.append((bvh, mom, q[0], q[1], q[2], out))
         geoms.append(geom)
         tris += base.triangle_count
         nq += n
+        if (k + 1) % 1024 == 0:
+            log(f"batched setup: {k + 1} meshes ({time.time() - t0:.1f}s)")
     torch.cuda.synchronize()
     log(f"batched: {len(items)} meshes, {tris} tris, {nq} queries ({time.time() - t0:.1f}s setup)")
 
@@ -527,13 +705,22 @@ def batched_bench(args, dev, stream) -> dict:
 
     sequential()
     torch.cuda.synchronize()
-    reps = 3
+    reps = 2
     t = time.perf_counter()
     for _ in range(reps):
         sequential()
     torch.cuda.synchronize()
     seq_ms = (time.perf_counter() - t) / reps * 1e3
     ref = [it[5].clone() for it in items]
+
+    # Work of every mesh's queries (the batch takes the single-mesh decisions) for the roofline.
+    c_tot = np.zeros(3, np.int64)
+    for bvh, mom, x, y, z, out in items:
+        cnt = torch.zeros(3 * x.numel(), dtype=torch.int32, device=dev)
+        gwn.gwn_compute_winding_number_taylor_batch(bvh, mom, x, y, z, out, accuracy_scale=2.0,
+                                                    counters=cnt).raise_if_error()
+        c_tot += cnt.view(-1, 3).to(torch.int64).sum(0).cpu().numpy()
+    flops = float(c_tot[0] * FLOPS_CHILD_TEST + c_tot[1] * FLOPS_ACCEPT[1] + c_tot[2] * FLOPS_TRIANGLE)
 
     batch = gwn.BatchObject(items, accuracy_scale=2.0)
     tune_ms = batch.tune()
@@ -545,6 +732,7 @@ def batched_bench(args, dev, stream) -> dict:
         batch.replay().raise_if_error()
     torch.cuda.synchronize()
     graph_ms = (time.perf_counter() - t) / reps * 1e3
+    st = batch.status()
     identical = all(torch.equal(a, it[5]) for a, it in zip(ref, items))
     max_diff = max(float((a.double() - it[5].double()).abs().max()) if a.numel() else 0.0
                    for a, it in zip(ref, items))
@@ -554,67 +742,175 @@ def batched_bench(args, dev, stream) -> dict:
             "sequential_ms": seq_ms, "graph_ms": graph_ms, "speedup": seq_ms / graph_ms,
             "mqueries_per_s": nq / (graph_ms * 1e-3) / 1e6, "best_query_launch_ms": tune_ms,
             "tuning": batch.tuning(), "bit_identical_to_sequential": bool(identical),
-            "max_abs_diff_vs_sequential": max_diff,
+            "max_abs_diff_vs_sequential": max_diff, "refit_status_ok": st.is_ok(),
+            "work": {"child_tests": int(c_tot[0]), "accepted": int(c_tot[1]), "near_field_triangles": int(c_tot[2])},
+            "roofline": _valu_roofline(flops, tune_ms, "taylor_batch_kernel<1> (persistent, tuned)",
+                                       "per-query work counters of every mesh (10 / 34 / 87 FLOP), "
+                                       "over the tuned query launch"),
             "step": "moment refit of every mesh + Taylor queries of every mesh"}
+
+
+def _profile_lines(name: str) -> list[str]:
+    try:
+        with open(os.path.join(REPO, "profiles", name)) as f:
+            return f.readlines()
+    except OSError:
+        return []
+
+
+def _profile_commit(lines) -> str | None:
+    for line in lines:
+        if line.startswith("measured_at_commit:"):
+            return line.split(":", 1)[1].strip()
+    return None
 
 
 def valu_profile() -> dict:
     """VALU counters of the traversal kernel from the committed rocprofv3 --pmc summary."""
-    path = os.path.join(REPO, "profiles", "taylor_wave_o2_large_valu.md")
-    out = {}
-    try:
-        with open(path) as f:
-            for line in f:
-                for key in ("valu_lane_utilization", "valu_insts_per_wave"):
-                    if line.startswith(key + ":"):
-                        out[key] = float(line.split(":", 1)[1].strip())
-    except OSError:
-        pass
+    name = "taylor_wave_o2_large_valu.md"
+    lines = _profile_lines(name)
+    out = {"source": f"profiles/{name} (commit {_profile_commit(lines)})"} if lines else {}
+    for line in lines:
+        for key in ("valu_lane_utilization", "valu_insts_per_wave"):
+            if line.startswith(key + ":"):
+                out[key] = float(line.split(":", 1)[1].strip())
     return out
 
 
-def traffic_bytes():
+def traffic_profile() -> dict:
     """HBM bytes per traversal launch, from the committed rocprofv3 --pmc summary of this
     workload (FETCH_SIZE + WRITE_SIZE of the traversal kernel, scaled by the same counters on a
-    256 MiB copy); None when that summary is absent."""
-    path = os.path.join(REPO, "profiles", "taylor_wave_o2_large_traffic.md")
+    256 MiB copy)."""
+    name = "taylor_wave_o2_large_traffic.md"
+    lines = _profile_lines(name)
+    out = {"source": f"profiles/{name} (commit {_profile_commit(lines)})"} if lines else {}
+    for line in lines:
+        if line.startswith("hbm_bytes_per_launch:"):
+            out["hbm_bytes_per_launch"] = int(line.split(":", 1)[1].strip())
+    return out
+
+
+def _cpu_threads() -> int:
+    """Host threads for the CPU baseline: the cores this process may run on, at most 16 (the host
+    share of one GPU on the MI355X boxes)."""
     try:
-        with open(path) as f:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("hbm_bytes_per_launch:"):
-                    return int(line.split(":", 1)[1].strip())
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return None
+    return platform.processor() or "unknown"
 
 
-def cpu_baseline(mesh, qx, qy, qz, args) -> dict:
-    """The oracle's C replay of the same split Taylor traversal, on a bounded query sample."""
+def _median_runs(fn, runs=5):
+    fn()  # warm-up
+    times = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t)
+    return statistics.median(times), times
+
+
+def cpu_baseline(mesh, qx, qy, qz, args, gpu_value) -> dict:
+    """The oracle's C replay of the headline batch on the same H-PLOC radius-8 tree: Taylor order 2,
+    the same accuracy scale, FP64 near-field triangles, over a strided query sample; one warm-up
+    and five timed runs, median."""
     from oracle import oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    bvh = oracle.build_bvh4(mesh)
+    threads = _cpu_threads()
+    t = time.perf_counter()
+    bvh = oracle.build_bvh4(mesh, "hploc", 8)
     mom = oracle.refit_moment(bvh, args.order)
-    rng_idx = np.linspace(0, qx.shape[0] - 1, 4096).astype(np.int64)
+    build_s = time.perf_counter() - t
+    stack = max(64, bvh.internal_stack_bound)
 
     def run(idx):
-        t = time.perf_counter()
         oracle.winding_taylor_f32(bvh, mom, args.order, qx[idx], qy[idx], qz[idx], accuracy_scale=args.beta,
-                                  stack_capacity=1024, threads=threads, near_fp64=True)
-        return time.perf_counter() - t
+                                  stack_capacity=stack, threads=threads, near_fp64=True)
 
-    probe = run(rng_idx)
-    m = int(min(qx.shape[0], max(4096, 4096 * args.cpu_seconds / max(probe, 1e-3))))
+    probe_idx = np.linspace(0, qx.shape[0] - 1, 8192).astype(np.int64)
+    t = time.perf_counter()
+    run(probe_idx)
+    probe = time.perf_counter() - t
+    m = int(min(qx.shape[0], max(8192, 8192 * args.cpu_seconds / max(probe, 1e-3))))
     idx = np.linspace(0, qx.shape[0] - 1, m).astype(np.int64)
-    sec = run(idx)
-    log(f"cpu baseline: {m} queries in {sec:.1f}s on {threads} threads")
+    med, times = _median_runs(lambda: run(idx))
+    value = m / med / 1e6
+    log(f"cpu baseline: {m} queries, median {med:.2f}s of {len(times)} runs on {threads} threads "
+        f"({value:.4f} Mq/s; GPU/CPU {gpu_value / value:.0f}x)")
     return {
-        "value": round(m / sec / 1e6, 5),
+        "value": round(value, 5),
         "unit": "Mqueries/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{m} strided queries of the same workload, Taylor o{args.order} split, beta={args.beta}",
+        "cpu_model": _cpu_model(),
+        "host_logical_cpus": os.cpu_count(),
+        "runs_s": [round(x, 3) for x in times],
+        "sample": f"{m} strided queries of the same 16,777,216-query workload on the same H-PLOC radius-8 "
+                  f"tree (built by the oracle in {build_s:.1f}s, bit-identical to the GPU's), Taylor "
+                  f"o{args.order}, beta={args.beta}, near-field triangles in FP64 (atan2 included); one "
+                  f"warm-up and {len(times)} timed runs, median",
     }
+
+
+def cpu_small() -> dict:
+    """The small configuration fully on the CPU (the oracle): LBVH build and order-1 moments, exact
+    winding (5,120 x 65,536 triangle evaluations), Taylor order 1 at scale 2, unsigned distance and
+    65,536 first-hit rays; one warm-up and three timed runs each, median."""
+    from oracle import oracle
+
+    from smallgwn_amd import synth
+
+    threads = _cpu_threads()
+    mesh, q, rays = synth.config_small()
+    bvh = oracle.build_bvh4(mesh)
+    mom = oracle.refit_moment(bvh, 1)
+    res = {"cores": threads, "kind": "port"}
+    for name, fn in (
+            ("build_and_refit", lambda: oracle.refit_moment(oracle.build_bvh4(mesh), 1)),
+            ("exact", lambda: oracle.winding_exact_records_f32(bvh.triangles, *q, threads=threads)),
+            ("taylor_o1", lambda: oracle.winding_taylor_f32(bvh, mom, 1, *q, accuracy_scale=2.0, threads=threads)),
+            ("distance", lambda: oracle.unsigned_distance_f32(bvh, *q, threads=threads)),
+            ("ray_first_hit", lambda: oracle.ray_first_hit_f32(bvh, *rays, threads=threads))):
+        med, _ = _median_runs(fn, runs=3)
+        res[name] = {"ms": med * 1e3} if name == "build_and_refit" else \
+            {"ms": med * 1e3, "mqueries_per_s": q[0].shape[0] / med / 1e6}
+    log("cpu small config: " + ", ".join(f"{k} {v['ms']:.1f} ms" for k, v in res.items() if isinstance(v, dict)))
+    return res
+
+
+def adjoint_gradient_check(check) -> dict:
+    """dL/dV of the winding loss on the differentiation step's 4,096-query subsample: the GPU
+    adjoint against the oracle's FP64 adjoint on the same tree (same decisions), relative error in
+    the 2-norm and the largest entry error relative to the largest entry."""
+    from oracle import oracle
+
+    from smallgwn_amd import synth
+
+    f = check["faces"]
+    # The GPU tree was built on the initial positions and refit (bounds, then moments) after each
+    # step: the oracle does the same, so both take the same accept/descend decisions.
+    mesh = synth._mesh_from_arrays(check["vertices"], f)
+    rb = oracle.refit_bvh(oracle.build_bvh4(synth._mesh_from_arrays(check["initial"], f), "hploc", 8), mesh)
+    rm = oracle.refit_moment(rb, 1)
+    rdv, _ = oracle.winding_taylor_adjoint_f64(rb, rm, 1, 2.0, mesh, *check["queries"], check["u"],
+                                               threads=_cpu_threads())
+    g = check["gpu"]
+    rel = float(np.linalg.norm(g - rdv) / max(np.linalg.norm(rdv), 1e-300))
+    ent = float(np.abs(g - rdv).max() / max(np.abs(rdv).max(), 1e-300))
+    log(f"adjoint gradient vs FP64 oracle: relative error {rel:.2e} (max entry {ent:.2e})")
+    return {"rel_error": rel, "max_entry_error_rel": ent, "queries": int(check["u"].shape[0]),
+            "reference": "oracle FP64 adjoint on the same refit H-PLOC tree (pinned by central differences)"}
 
 
 if __name__ == "__main__":

@@ -240,6 +240,18 @@ int gwn_compute_ray_first_hit_batch_f32(
     const float *dz, gwn_ray_first_hit_result *out, uint64_t count, float t_min, float t_max,
     const gwn_query_config *config, hipStream_t stream);
 
+/* ---- traversal work counters (no reference counterpart; roofline accounting).  Two uint32 per
+ *      query: child bounding-box tests and triangle tests of the one-lane unsigned-distance
+ *      traversal (which the four-lanes-per-point form replays step for step), or slab tests and
+ *      ray-triangle tests of the ray batch's traversal.  No distances or hits are written. ---- */
+int gwn_compute_unsigned_distance_work_f32(
+    gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, uint64_t count, float culling_band,
+    uint32_t *counters, hipStream_t stream);
+int gwn_compute_ray_first_hit_work_f32(
+    gwn_bvh_t bvh, const float *ox, const float *oy, const float *oz, const float *dx, const float *dy,
+    const float *dz, uint64_t count, float t_min, float t_max, const gwn_query_config *config, uint32_t *counters,
+    hipStream_t stream);
+
 /* ---- antipodal winding number: replaces gwn_compute_winding_number_antipodal_batch<Config>
  *      (include/gwn/gwn_query.cuh:492-552, detail/gwn_query_winding_antipodal_impl.cuh:55-490).
  *      Signed crossings of the positive coordinate ray with the BVH's triangles plus the boundary
@@ -302,7 +314,11 @@ int gwn_compute_unsigned_distance_adjoint_batch_ex_f32(
  *      the batch's own stream, gwn_batch_replay launches it.  Results equal the single-mesh
  *      Taylor batch's bit for bit.  gwn_batch_tune times candidate launch shapes and keeps the
  *      fastest (synchronizes; drops a captured graph).  The objects and pointers must outlive
- *      the batch; replays of one batch must not overlap. ---- */
+ *      the batch; replays of one batch must not overlap.  Each mesh needs its own moment object.
+ *      A batch refuses to run once a mesh's BVH or moment storage was replaced (rebuild, upload,
+ *      gwn_refit_bvh_moment_f32), and a query without refits (gwn_batch_query, or replaying a graph
+ *      captured without them) refuses moments not formed from their BVH's current revision.
+ *      gwn_batch_set_tuning and gwn_batch_tune first wait for the batch's last enqueued work. ---- */
 typedef struct gwn_batch_item {
     gwn_bvh_t bvh;
     gwn_moment_t moment;
@@ -332,6 +348,12 @@ int gwn_batch_get_tuning(gwn_batch_t batch, gwn_batch_tuning *tuning);
 int gwn_batch_tune(gwn_batch_t batch, hipStream_t stream, float *best_ms);
 int gwn_batch_capture(gwn_batch_t batch, int32_t include_refit);
 int gwn_batch_replay(gwn_batch_t batch, hipStream_t stream);
+/* Synchronizes `stream` and reports whether any moment refit of the batch (eager or in a graph
+ * replay) met an invalid tree since the batch was made or since the last gwn_batch_status call:
+ * GWN_ERROR_INTERNAL "batch: Bottom-up moment refit did not converge." (the word is then
+ * cleared), else GWN_SUCCESS.  Refits enqueue without reading the word back, so they can be
+ * captured. */
+int gwn_batch_status(gwn_batch_t batch, hipStream_t stream);
 
 /* Morton-order permutation of query points within the BVH root bounds (30-bit keys, stable
  * radix sort): order[i] is the input index of the i-th query to visit.  Reusable across

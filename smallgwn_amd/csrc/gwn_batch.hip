@@ -3,9 +3,10 @@
 //
 // No reference counterpart: the reference runs one mesh per call.  A batch holds, for every mesh,
 // its BVH and moment objects and its query / output device pointers:
-//   - refit: every mesh's moment refit runs in two launches over all meshes' nodes, without host
-//     synchronisation (buffers are allocated once when the batch is made, so replays allocate
-//     nothing);
+//   - refit: every mesh's moment refit runs in one launch per tree depth over all meshes' nodes at
+//     that depth, deepest first, without host synchronisation (buffers are allocated once when the
+//     batch is made, so replays allocate nothing); a failed refit sets a sticky device status word
+//     that gwn_batch_status reads;
 //   - query: one persistent launch whose blocks take chunks of one mesh's queries from a global
 //     counter (taylor_batch_kernel), so small and large meshes share the machine instead of
 //     running 4096 launches back to back; every result equals the single-mesh batch's;
@@ -21,10 +22,10 @@
 #include <algorithm>
 #include <vector>
 
-// Device storage a batch was made over, per mesh: a rebuild or re-upload swaps these buffers.
+// Device storage a batch was made over, per mesh: every build, upload and moment refit that
+// swaps in new buffers takes a new storage generation.
 struct gwn_batch_storage {
-    void const *nodes, *triangles, *records;
-    std::size_t node_count;
+    std::uint64_t bvh_generation, moment_generation;
 };
 
 struct gwn_batch_handle {
@@ -35,7 +36,9 @@ struct gwn_batch_handle {
     std::vector<gwn::moment_refit_work> work;
     std::vector<gwn::moment_refit_mesh> host_refits;  // meshes with an internal root
     gwn::device_buffer<gwn::moment_refit_mesh> refits;
-    std::uint64_t refit_nodes = 0;
+    std::vector<gwn::refit_level> refit_levels;       // per depth, deepest last
+    gwn::device_buffer<gwn::refit_segment> segments;
+    gwn::device_buffer<std::uint32_t> status;         // sticky refit error word
     gwn::device_buffer<gwn::batch_mesh> meshes;
     gwn::device_buffer<std::uint64_t> counter;
     std::vector<gwn::device_buffer<std::uint32_t>> orders;  // per mesh Morton visiting order
@@ -53,14 +56,22 @@ struct gwn_batch_handle {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_refits = false;
+    // Recorded after every enqueue of batch work, on the stream it went to: a re-upload of the
+    // mesh table waits for it, so it never races a query or replay still reading the table.
+    hipEvent_t last_work = nullptr;
+    bool have_work = false;
 
     ~gwn_batch_handle() {
+        if (last_work)
+            (void)hipEventSynchronize(last_work);
         if (exec)
             (void)hipGraphExecDestroy(exec);
         if (graph)
             (void)hipGraphDestroy(graph);
         if (own_stream)
             (void)hipStreamDestroy(own_stream);
+        if (last_work)
+            (void)hipEventDestroy(last_work);
     }
 };
 
@@ -77,7 +88,19 @@ void layout_chunks(gwn_batch_handle &b) {
     b.total_chunks = first;
 }
 
+// Waits for the batch work enqueued last (queries or replays on any stream).
+void wait_last_work(gwn_batch_handle &b) {
+    if (b.have_work)
+        GWN_HIP(hipEventSynchronize(b.last_work));
+}
+
+void note_work(gwn_batch_handle &b, hipStream_t stream) {
+    GWN_HIP(hipEventRecord(b.last_work, stream));
+    b.have_work = true;
+}
+
 void upload_table(gwn_batch_handle &b, hipStream_t stream) {
+    wait_last_work(b);
     layout_chunks(b);
     GWN_HIP(hipMemcpyAsync(b.meshes.ptr, b.host_meshes.data(), b.host_meshes.size() * sizeof(batch_mesh),
                            hipMemcpyHostToDevice, stream));
@@ -85,62 +108,39 @@ void upload_table(gwn_batch_handle &b, hipStream_t stream) {
 }
 
 gwn_batch_storage storage_of(gwn_bvh_handle const &bvh, gwn_moment_handle const &m) {
-    return {bvh.nodes.ptr, bvh.triangles.ptr, m.records.ptr, bvh.node_count};
+    return {bvh.storage_generation, m.storage_generation};
 }
 
-// Captured graphs and the mesh table hold raw device pointers: refuse to run over a mesh whose
-// BVH or moment storage was replaced (rebuilt, re-uploaded) after the batch was made.
-void check_storage(gwn_batch_handle const &b) {
+// Captured graphs and the device tables hold raw device pointers: refuse to run over a mesh whose
+// BVH or moment storage was replaced (rebuilt, re-uploaded, refit into new buffers) after the
+// batch was made.  Work that queries without refitting first also needs every moment formed
+// from its BVH's current revision, as the single-mesh batches require.
+void check_storage(gwn_batch_handle const &b, bool query_without_refit) {
     for (std::size_t i = 0; i < b.bvhs.size(); ++i) {
         gwn_batch_storage const now = storage_of(*b.bvhs[i], *b.moments[i]);
         gwn_batch_storage const &was = b.storage[i];
-        if (now.nodes != was.nodes || now.triangles != was.triangles || now.records != was.records ||
-            now.node_count != was.node_count)
+        if (now.bvh_generation != was.bvh_generation || now.moment_generation != was.moment_generation)
             throw_invalid("batch: A mesh's BVH or moment storage changed since the batch was made; create a new batch.");
         if (b.bvhs[i]->revision == 0)
             throw_invalid("batch: A mesh's BVH is being refit; its bounds are not queryable yet.");
+        if (query_without_refit && b.moments[i]->bvh_revision != b.bvhs[i]->revision)
+            throw_invalid("batch: Moment object is not aligned with the BVH; refit the batch's moments first.");
     }
 }
 
-} // namespace
-
-void batch_destroy(gwn_batch_handle *b) { delete b; }
-
-// Every mesh's moment refit in two launches over all meshes' nodes (refit_moment_batch); the
-// records equal those of per-mesh refit_moment_enqueue calls.
-void batch_refit(gwn_batch_handle &b, hipStream_t stream) {
-    check_storage(b);
+// Every mesh's moment is formed from its BVH's current bounds once enqueued refits have run.
+void stamp_revisions(gwn_batch_handle &b) {
     for (std::size_t i = 0; i < b.bvhs.size(); ++i) {
         b.moments[i]->node_count = b.bvhs[i]->node_count;
         b.moments[i]->bvh_revision = b.bvhs[i]->revision;
     }
-    refit_moment_batch(b.refits.ptr, static_cast<std::uint32_t>(b.host_refits.size()), b.refit_nodes, b.order,
-                       stream);
 }
 
-void upload_refit_table(gwn_batch_handle &b, hipStream_t stream) {
-    b.host_refits.clear();
-    b.refit_nodes = 0;
-    for (std::size_t i = 0; i < b.bvhs.size(); ++i) {
-        gwn_bvh_handle const &bvh = *b.bvhs[i];
-        if (!ref_internal(bvh.root.reference) || bvh.node_count == 0)
-            continue;
-        b.host_refits.push_back(moment_refit_mesh{
-            bvh.nodes.ptr, bvh.triangles.ptr, bvh.parent.ptr, bvh.internal_children.ptr, b.work[i].arrivals.ptr,
-            b.work[i].pending.ptr, b.moments[i]->records.ptr, b.moments[i]->slot_area.ptr, bvh.node_count,
-            bvh.triangle_count, b.refit_nodes});
-        b.refit_nodes += bvh.node_count;
-    }
-    b.refits.allocate(std::max<std::size_t>(1, b.host_refits.size()), stream);
-    if (!b.host_refits.empty()) {
-        GWN_HIP(hipMemcpyAsync(b.refits.ptr, b.host_refits.data(), b.host_refits.size() * sizeof(moment_refit_mesh),
-                               hipMemcpyHostToDevice, stream));
-        GWN_HIP(hipStreamSynchronize(stream));
-    }
+void enqueue_refit(gwn_batch_handle &b, hipStream_t stream) {
+    refit_moment_batch(b.refits.ptr, b.segments.ptr, b.refit_levels, b.order, b.status.ptr, stream);
 }
 
-void batch_query(gwn_batch_handle &b, hipStream_t stream) {
-    check_storage(b);
+void enqueue_query(gwn_batch_handle &b, hipStream_t stream) {
     if (b.total_chunks == 0)
         return;
     fill_u32(reinterpret_cast<std::uint32_t *>(b.counter.ptr), 0u, 2, stream);
@@ -149,6 +149,83 @@ void batch_query(gwn_batch_handle &b, hipStream_t stream) {
                         static_cast<std::uint32_t>(b.chunk), b.accuracy_scale, b.counter.ptr, b.order, b.cap,
                         b.block, grid, b.wave, stream);
 }
+
+// The refit table: every mesh with an internal root, and for every depth the meshes' node ranges
+// at that depth (segments), so one launch per depth refits all meshes.
+void upload_refit_table(gwn_batch_handle &b, hipStream_t stream) {
+    b.host_refits.clear();
+    b.refit_levels.clear();
+    std::vector<std::vector<refit_segment>> by_depth;
+    for (std::size_t i = 0; i < b.bvhs.size(); ++i) {
+        gwn_bvh_handle const &bvh = *b.bvhs[i];
+        if (!ref_internal(bvh.root.reference) || bvh.node_count == 0)
+            continue;
+        auto const &ls = bvh.level_start;
+        if (ls.size() < 2 || ls.front() != 0u || ls.back() != bvh.node_count)
+            throw_internal("batch: A BVH level table does not cover its nodes.");
+        std::uint32_t const mesh = static_cast<std::uint32_t>(b.host_refits.size());
+        b.host_refits.push_back(moment_refit_mesh{bvh.nodes.ptr, bvh.triangles.ptr, b.work[i].pending.ptr,
+                                                  b.moments[i]->records.ptr, b.moments[i]->slot_area.ptr,
+                                                  bvh.node_count, bvh.triangle_count});
+        if (by_depth.size() < ls.size() - 1)
+            by_depth.resize(ls.size() - 1);
+        for (std::size_t d = 0; d + 1 < ls.size(); ++d)
+            if (ls[d + 1] > ls[d])
+                by_depth[d].push_back(refit_segment{mesh, ls[d], ls[d + 1] - ls[d]});  // prefix set below
+    }
+    std::vector<refit_segment> flat;
+    for (auto &segs : by_depth) {
+        refit_level lv{0, static_cast<std::uint32_t>(flat.size()), static_cast<std::uint32_t>(segs.size())};
+        for (refit_segment sg : segs) {
+            std::uint64_t const count = sg.prefix;
+            sg.prefix = lv.nodes;
+            lv.nodes += count;
+            flat.push_back(sg);
+        }
+        b.refit_levels.push_back(lv);
+    }
+    b.refits.allocate(std::max<std::size_t>(1, b.host_refits.size()), stream);
+    b.segments.allocate(std::max<std::size_t>(1, flat.size()), stream);
+    if (!b.host_refits.empty())
+        GWN_HIP(hipMemcpyAsync(b.refits.ptr, b.host_refits.data(), b.host_refits.size() * sizeof(moment_refit_mesh),
+                               hipMemcpyHostToDevice, stream));
+    if (!flat.empty())
+        GWN_HIP(hipMemcpyAsync(b.segments.ptr, flat.data(), flat.size() * sizeof(refit_segment),
+                               hipMemcpyHostToDevice, stream));
+    GWN_HIP(hipStreamSynchronize(stream));
+}
+
+std::uint32_t read_status(gwn_batch_handle &b, hipStream_t stream, bool clear) {
+    std::uint32_t e = 0;
+    GWN_HIP(hipMemcpyAsync(&e, b.status.ptr, sizeof(e), hipMemcpyDeviceToHost, stream));
+    GWN_HIP(hipStreamSynchronize(stream));
+    if (clear && e != 0u) {
+        fill_u32(b.status.ptr, 0u, 1, stream);
+        GWN_HIP(hipStreamSynchronize(stream));
+    }
+    return e;
+}
+
+} // namespace
+
+void batch_destroy(gwn_batch_handle *b) { delete b; }
+
+// Every mesh's moment refit, one launch per depth over all meshes' nodes (refit_moment_batch); the
+// records equal those of single-mesh refits.
+void batch_refit(gwn_batch_handle &b, hipStream_t stream) {
+    check_storage(b, false);
+    enqueue_refit(b, stream);
+    note_work(b, stream);
+    stamp_revisions(b);
+}
+
+void batch_query(gwn_batch_handle &b, hipStream_t stream) {
+    check_storage(b, true);
+    enqueue_query(b, stream);
+    note_work(b, stream);
+}
+
+std::uint32_t batch_status(gwn_batch_handle &b, hipStream_t stream) { return read_status(b, stream, true); }
 
 void batch_drop_graph(gwn_batch_handle &b) {
     if (b.exec)
@@ -165,6 +242,7 @@ gwn_batch_handle *batch_create(
 ) {
     auto *b = new gwn_batch_handle();
     try {
+        GWN_HIP(hipEventCreateWithFlags(&b->last_work, hipEventDisableTiming));
         b->bvhs = bvhs;
         b->moments = moments;
         b->host_meshes = queries;
@@ -175,10 +253,15 @@ gwn_batch_handle *batch_create(
         for (std::size_t i = 0; i < bvhs.size(); ++i) {
             if (moments[i]->order != b->order)
                 throw_invalid("batch: Every mesh's moment must have the same Taylor order.");
+            for (std::size_t j = 0; j < i; ++j)
+                if (moments[j] == moments[i])
+                    throw_invalid("batch: Each mesh needs its own moment object.");
             refit_moment_prepare(*bvhs[i], *moments[i], b->work[i], stream);
             bound = std::max(bound, bvhs[i]->internal_stack_bound);
             b->storage.push_back(storage_of(*bvhs[i], *moments[i]));
         }
+        b->status.allocate(1, stream);
+        fill_u32(b->status.ptr, 0u, 1, stream);
         upload_refit_table(*b, stream);
         b->cap = bound <= 32 ? 32 : (bound <= 64 ? 64 : 128);
         if (bound > 128)
@@ -205,17 +288,7 @@ gwn_batch_handle *batch_create(
         b->meshes.allocate(std::max<std::size_t>(1, b->host_meshes.size()), stream);
         b->counter.allocate(1, stream);
         upload_table(*b, stream);
-        std::uint32_t err = 0;
-        for (std::size_t i = 0; i < bvhs.size(); ++i) {
-            if (b->work[i].arrivals.ptr == nullptr)
-                continue;
-            std::uint32_t e = 0;
-            GWN_HIP(hipMemcpyAsync(&e, b->work[i].arrivals.ptr + bvhs[i]->node_count, sizeof(e),
-                                   hipMemcpyDeviceToHost, stream));
-            GWN_HIP(hipStreamSynchronize(stream));
-            err |= e;
-        }
-        if (err != 0u)
+        if (read_status(*b, stream, false) != 0u)
             throw_internal("batch: Bottom-up moment refit did not converge.");
     } catch (...) {
         delete b;
@@ -231,6 +304,8 @@ void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_
         throw_invalid("Batched query chunk length must be a positive multiple of the block size.");
     if (blocks_per_cu <= 0 || blocks_per_cu > 64)
         throw_invalid("Batched query blocks per CU must be in [1, 64].");
+    // The graph and any query in flight read the table and the counter: finish them first.
+    wait_last_work(b);
     bool const relayout = chunk != b.chunk;
     b.block = block;
     b.chunk = chunk;
@@ -254,6 +329,7 @@ float batch_tune(gwn_batch_handle &b, hipStream_t stream) {
             for (int mult : {1, 4})
                 for (int bpc : {2, 4, 8})
                     cands.push_back({block, block * mult, bpc, wave});
+    check_storage(b, true);
     hipEvent_t e0, e1;
     GWN_HIP(hipEventCreate(&e0));
     GWN_HIP(hipEventCreate(&e1));
@@ -297,16 +373,19 @@ void batch_get_tuning(gwn_batch_handle const &b, gwn_batch_tuning &t) {
     t.queries = b.total_queries;
 }
 
+// Records [refits +] counter reset + query launch.  Nothing runs yet, so no moment's revision is
+// stamped here: a replay with refits stamps after it is launched.
 void batch_capture(gwn_batch_handle &b, bool include_refit) {
-    check_storage(b);
+    check_storage(b, false);
+    wait_last_work(b);
     batch_drop_graph(b);
     if (b.own_stream == nullptr)
         GWN_HIP(hipStreamCreateWithFlags(&b.own_stream, hipStreamNonBlocking));
     GWN_HIP(hipStreamBeginCapture(b.own_stream, hipStreamCaptureModeThreadLocal));
     try {
         if (include_refit)
-            batch_refit(b, b.own_stream);
-        batch_query(b, b.own_stream);
+            enqueue_refit(b, b.own_stream);
+        enqueue_query(b, b.own_stream);
     } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(b.own_stream, &g);
@@ -322,12 +401,12 @@ void batch_capture(gwn_batch_handle &b, bool include_refit) {
 void batch_replay(gwn_batch_handle &b, hipStream_t stream) {
     if (b.exec == nullptr)
         throw_invalid("batch: No captured graph; call gwn_batch_capture first.");
-    check_storage(b);
+    check_storage(b, !b.graph_refits);
     GWN_HIP(hipGraphLaunch(b.exec, stream));
+    note_work(b, stream);
     // The graph's refits form every moment from its BVH's current bounds, as an eager refit does.
     if (b.graph_refits)
-        for (std::size_t i = 0; i < b.bvhs.size(); ++i)
-            b.moments[i]->bvh_revision = b.bvhs[i]->revision;
+        stamp_revisions(b);
 }
 
 } // namespace gwn

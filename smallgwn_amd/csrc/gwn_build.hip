@@ -964,6 +964,9 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, int method, s
     out.internal_stack_bound = hs.max_internal;
     out.packed_stack_bound = hs.max_packed;
     out.max_depth = static_cast<std::uint32_t>(empty_level - 1);
+    out.level_start.resize(static_cast<std::size_t>(empty_level) + 1);
+    for (int d = 0; d <= empty_level; ++d)
+        out.level_start[d] = host_levels[d].start;
 }
 
 } // namespace gwn

@@ -256,6 +256,7 @@ int gwn_build_bvh4_f32(
         gwn_bvh_handle staging;
         staging.stream = stream;
         gwn::build_bvh4(*geometry, staging, method, hploc_search_radius, stream);
+        staging.storage_generation = gwn::next_storage_generation();
         if (geometry->triangle_count != 0) {
             staging.revision = gwn::reserve_revision();
             if (staging.revision == 0)
@@ -426,6 +427,7 @@ int gwn_refit_bvh_moment_f32(gwn_bvh_t bvh, gwn_moment_t moment, hipStream_t str
         gwn_moment_handle staging;
         staging.order = moment->order;
         staging.stream = stream;
+        staging.storage_generation = gwn::next_storage_generation();
         gwn::refit_moment(*bvh, staging, stream);
         std::swap(*moment, staging);
     });
@@ -614,6 +616,40 @@ int gwn_compute_ray_first_hit_batch_f32(
         if (count == 0)
             return;
         gwn::ray_first_hit(*bvh, ox, oy, oz, dx, dy, dz, out, count, t_min, t_max, to_launch(c), stream);
+    });
+}
+
+int gwn_compute_unsigned_distance_work_f32(
+    gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, uint64_t count, float culling_band,
+    uint32_t *counters, hipStream_t stream
+) {
+    return guarded("gwn_compute_unsigned_distance_work", [&] {
+        require_bvh(bvh);
+        require_query_storage(count, qx, qy, qz, counters);
+        if (count == 0)
+            return;
+        gwn::unsigned_distance_work(*bvh, qx, qy, qz, count, culling_band, counters, stream);
+    });
+}
+
+int gwn_compute_ray_first_hit_work_f32(
+    gwn_bvh_t bvh, const float *ox, const float *oy, const float *oz, const float *dx, const float *dy,
+    const float *dz, uint64_t count, float t_min, float t_max, const gwn_query_config *config, uint32_t *counters,
+    hipStream_t stream
+) {
+    return guarded("gwn_compute_ray_first_hit_work", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, true);
+        require_bvh(bvh);
+        require_stack(*bvh, c.stack_capacity);
+        if (count != 0 && (ox == nullptr || oy == nullptr || oz == nullptr || dx == nullptr || dy == nullptr ||
+                           dz == nullptr || counters == nullptr))
+            gwn::throw_invalid("ray first-hit: ray/output spans must use non-null storage when non-empty.");
+        if (!(t_max >= t_min))
+            gwn::throw_invalid("ray first-hit: requires t_max >= t_min.");
+        if (count == 0)
+            return;
+        gwn::ray_first_hit_work(*bvh, ox, oy, oz, dx, dy, dz, count, t_min, t_max, to_launch(c), counters, stream);
     });
 }
 
@@ -873,6 +909,15 @@ int gwn_batch_replay(gwn_batch_t batch, hipStream_t stream) {
         if (batch == nullptr)
             gwn::throw_invalid("Batch handle is null.");
         gwn::batch_replay(*batch, stream);
+    });
+}
+
+int gwn_batch_status(gwn_batch_t batch, hipStream_t stream) {
+    return guarded("gwn_batch_status", [&] {
+        if (batch == nullptr)
+            gwn::throw_invalid("Batch handle is null.");
+        if (gwn::batch_status(*batch, stream) != 0u)
+            gwn::throw_internal("batch: Bottom-up moment refit did not converge.");
     });
 }
 

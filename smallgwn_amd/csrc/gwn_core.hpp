@@ -162,6 +162,12 @@ __host__ __device__ constexpr int moment_record_vec4(int order) {
     return order == 0 ? 8 : (order == 1 ? 16 : 24);
 }
 
+// Work tallies of one query's traversal, for roofline accounting: child bounding-box tests and
+// triangle tests (point-triangle distances or ray-triangle intersections).
+struct traversal_work {
+    std::uint32_t boxes = 0, tris = 0;
+};
+
 // Five-comparator sorting network: orders t ascending and applies the same swaps to k.  Used
 // for near-first child ordering in traversals.
 template <class Key>
@@ -208,6 +214,11 @@ __device__ inline void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE
 // ---------------------------------------------------------------------------------------------
 
 std::uint64_t reserve_revision() noexcept;
+// Process-unique identity of one allocation of an object's device storage.  Every upload, build
+// and moment refit that swaps in new buffers takes a fresh one, so holders of raw device pointers
+// (batches, captured graphs) detect replaced storage even when the pool hands back the same
+// addresses.
+std::uint64_t next_storage_generation() noexcept;
 
 } // namespace gwn
 
@@ -233,6 +244,10 @@ struct gwn_bvh_handle {
     std::uint32_t packed_stack_bound = 0;
     std::uint32_t max_depth = 0;
     std::uint64_t revision = 0;
+    std::uint64_t storage_generation = 0;
+    // Breadth-first levels of the wide tree: level d holds nodes [level_start[d], level_start[d + 1]);
+    // the last entry is node_count.  Empty for a leaf root.
+    std::vector<std::uint32_t> level_start;
     hipStream_t stream = nullptr;
 
     bool valid() const noexcept {
@@ -262,6 +277,7 @@ struct gwn_moment_handle {
     gwn::device_buffer<float> slot_area; // node_count * 4: each child slot's total triangle area
     std::size_t node_count = 0;
     std::uint64_t bvh_revision = 0;
+    std::uint64_t storage_generation = 0;
     hipStream_t stream = nullptr;
 };
 
@@ -317,30 +333,43 @@ float batch_tune(gwn_batch_handle &b, hipStream_t stream);
 void batch_get_tuning(gwn_batch_handle const &b, gwn_batch_tuning &t);
 void batch_capture(gwn_batch_handle &b, bool include_refit);
 void batch_replay(gwn_batch_handle &b, hipStream_t stream);
-// Moment refit split into allocation and enqueue, for callers that replay it without host
-// synchronisation (the enqueue never synchronises; its error flag stays in arrivals[node_count]).
+// Synchronizes `stream`, returns the batch's refit error word and clears it.
+std::uint32_t batch_status(gwn_batch_handle &b, hipStream_t stream);
+// Moment refit storage that a batch allocates once per mesh, so that its refits enqueue without
+// allocating or synchronising: the per-node additive payloads of the level passes.
 struct moment_refit_work {
-    device_buffer<std::uint8_t> pending;    // per-node additive payloads
-    device_buffer<std::uint32_t> arrivals;  // node_count arrival counters + error flag
+    device_buffer<std::uint8_t> pending;
 };
+// Allocates the moment's records and the work buffers for `bvh` (takes a new storage generation).
 void refit_moment_prepare(gwn_bvh_handle const &bvh, gwn_moment_handle &m, moment_refit_work &w, hipStream_t stream);
-void refit_moment_enqueue(gwn_bvh_handle const &bvh, gwn_moment_handle &m, moment_refit_work &w, hipStream_t stream);
-// One mesh of a batched moment refit (device table entry): the BVH's arrays, the moment's record
-// and slot-area arrays, the refit work buffers, and the mesh's first node in the batch numbering.
+std::size_t moment_payload_bytes(int order);
+// One mesh of a batched moment refit (device table entry).
 struct moment_refit_mesh {
     bvh_node4 const *nodes;
     tri_record const *tris;
-    std::uint32_t const *parent;
-    std::uint8_t const *internal_children;
-    std::uint32_t *arrivals;  // node_count counters + error word
     void *pending;
     float4 *records;
     float *slot_area;
-    std::uint64_t node_count, tri_count, first_node;
+    std::uint64_t node_count, tri_count;
 };
-// The refits of every mesh in `meshes` (internal roots only, first_node ascending from 0, all of
-// one order) in two launches, each mesh's records equal to refit_moment_enqueue's.
-void refit_moment_batch(moment_refit_mesh const *meshes, std::uint32_t count, std::uint64_t total_nodes, int order,
+// One mesh's node range at one depth of a batched level pass; `prefix` counts the nodes of the
+// depth's earlier segments.
+struct refit_segment {
+    std::uint32_t mesh;
+    std::uint32_t first_node;
+    std::uint64_t prefix;
+};
+// Segments [first_segment, first_segment + segments) of one depth, `nodes` nodes in all.
+struct refit_level {
+    std::uint64_t nodes;
+    std::uint32_t first_segment;
+    std::uint32_t segments;
+};
+// Every mesh's moment refit (all of one order): one launch per depth, deepest first, over all
+// meshes' nodes at that depth.  Records equal those of single-mesh refits; an invalid tree sets
+// *error (device word) nonzero.
+void refit_moment_batch(moment_refit_mesh const *meshes, refit_segment const *segments,
+                        std::vector<refit_level> const &levels, int order, std::uint32_t *error,
                         hipStream_t stream);
 // In-place bounds refit: triangle records and child boxes from the geometry, root box updated.
 void refit_bounds(gwn_geometry_handle const &geometry, gwn_bvh_handle &bvh, hipStream_t stream);
@@ -372,6 +401,18 @@ constexpr std::uint32_t k_nearest_overflow = 0xfffffffeu;
 void unsigned_distance(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, float *out, std::size_t n,
     float culling_band, query_launch const &cfg, std::uint32_t *nearest, hipStream_t stream
+);
+// Work counters (2 per query: box tests, triangle tests) of the one-lane distance traversal in
+// input order; distances are not written.
+void unsigned_distance_work(
+    gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, std::size_t n, float culling_band,
+    std::uint32_t *counters, hipStream_t stream
+);
+// Work counters (2 per ray: slab tests, triangle tests) of the batch's ray traversal.
+void ray_first_hit_work(
+    gwn_bvh_handle const &bvh, float const *ox, float const *oy, float const *oz, float const *dx,
+    float const *dy, float const *dz, std::size_t n, float t_min, float t_max, query_launch const &cfg,
+    std::uint32_t *counters, hipStream_t stream
 );
 // Nearest ray-triangle hits on [t_min, t_max]; `out` holds 32-byte
 // gwn_ray_first_hit_result<float, uint32_t> records.

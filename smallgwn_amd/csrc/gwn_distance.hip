@@ -90,13 +90,15 @@ __device__ inline float box_distance2(float qx, float qy, float qz, aabb const &
 // non-null, receives the record of each strict improvement.
 __device__ inline bool visit_leaf(
     tri_record const *__restrict__ tris, std::size_t nt, std::uint64_t ref, float qx, float qy, float qz,
-    float &best, std::uint32_t *best_rec
+    float &best, std::uint32_t *best_rec, traversal_work *work = nullptr
 ) {
     std::uint64_t const off = ref_offset(ref);
     std::uint32_t const cnt = ref_count(ref);
     for (std::uint32_t k = 0; k < cnt; ++k) {
         if (off + k >= nt)
             continue;
+        if (work != nullptr)
+            ++work->tris;
         tri_record const t = tris[off + k];
         float const e1x = t.v1x - t.v0x, e1y = t.v1y - t.v0y, e1z = t.v1z - t.v0z;
         float const e2x = t.v2x - t.v0x, e2y = t.v2y - t.v0y, e2z = t.v2z - t.v0z;
@@ -131,7 +133,8 @@ __device__ inline std::uint32_t nearest_code(int st, std::uint32_t rec) {
 // stack would overflow.
 template <int StackCap>
 __device__ inline int distance_traverse(
-    distance_scene const &sc, float x, float y, float z, float &best, std::uint32_t *best_rec
+    distance_scene const &sc, float x, float y, float z, float &best, std::uint32_t *best_rec,
+    traversal_work *work = nullptr
 ) {
     // One loop exit, at the head (status flags instead of returns inside the unrolled loops): the
     // gfx950 backend (ROCm 7.2) has lowered an early exit inside such a loop wrongly.
@@ -142,7 +145,7 @@ __device__ inline int distance_traverse(
     while (reference != 0 && status == k_dist_done) {
         std::uint64_t next = 0;
         if (ref_leaf(reference)) {
-            if (!visit_leaf(sc.tris, sc.nt, reference, x, y, z, best, best_rec))
+            if (!visit_leaf(sc.tris, sc.nt, reference, x, y, z, best, best_rec, work))
                 status = k_dist_zero;
         } else if (ref_internal(reference) && ref_offset(reference) < sc.node_count) {
             bvh_node4 const &node = sc.nodes[ref_offset(reference)];
@@ -154,6 +157,8 @@ __device__ inline int distance_traverse(
                 bvh_child const ch = node.children[s];
                 if (!ref_valid(ch.reference))
                     continue;
+                if (work != nullptr)
+                    ++work->boxes;
                 float const d2 = box_distance2(x, y, z, ch.bounds);
                 if (d2 >= best)
                     continue;
@@ -186,7 +191,7 @@ __device__ inline int distance_traverse(
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 if (status == k_dist_done && s < m && ref_leaf(cr[s]) && cd[s] < best &&
-                    !visit_leaf(sc.tris, sc.nt, cr[s], x, y, z, best, best_rec))
+                    !visit_leaf(sc.tris, sc.nt, cr[s], x, y, z, best, best_rec, work))
                     status = k_dist_zero;
             }
 #pragma unroll
@@ -490,6 +495,24 @@ __global__ __launch_bounds__(k_block) void distance_kernel(
         nearest[q] = nearest_code(st, rec);
 }
 
+// The one-lane traversal in input order, writing only its work counters (box tests, triangle
+// tests) per query.
+template <int StackCap>
+__global__ __launch_bounds__(k_block) void distance_work_kernel(
+    distance_scene sc, float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
+    std::size_t n, float culling_band, std::uint32_t *__restrict__ counters
+) {
+    std::size_t const q = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    if (q >= n)
+        return;
+    float const band = culling_band < 0.0f ? 0.0f : culling_band;
+    float best = band * band;
+    traversal_work w;
+    (void)distance_traverse<StackCap>(sc, qx[q], qy[q], qz[q], best, nullptr, &w);
+    counters[2 * q] = w.boxes;
+    counters[2 * q + 1] = w.tris;
+}
+
 // Barycentric coordinates (of b and c) of the closest point of triangle (a, b, c) to p: the
 // Voronoi-region cases of point_triangle_distance2.
 __device__ inline void closest_barycentrics(
@@ -676,6 +699,26 @@ void unsigned_distance(
         launch_distance<128>(bvh, qx, qy, qz, order.ptr, out, n, culling_band, wave, quad, nearest, stream);
     else
         throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
+}
+
+void unsigned_distance_work(
+    gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, std::size_t n, float culling_band,
+    std::uint32_t *counters, hipStream_t stream
+) {
+    if (n == 0)
+        return;
+    distance_scene const sc{bvh.nodes.ptr, bvh.node_count, bvh.triangles.ptr, bvh.triangle_count, bvh.root.reference};
+    int const need = static_cast<int>(bvh.internal_stack_bound);
+    unsigned const grid = grid_for(n, k_block);
+    if (need <= 32)
+        distance_work_kernel<32><<<grid, k_block, 0, stream>>>(sc, qx, qy, qz, n, culling_band, counters);
+    else if (need <= 64)
+        distance_work_kernel<64><<<grid, k_block, 0, stream>>>(sc, qx, qy, qz, n, culling_band, counters);
+    else if (need <= 128)
+        distance_work_kernel<128><<<grid, k_block, 0, stream>>>(sc, qx, qy, qz, n, culling_band, counters);
+    else
+        throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
+    GWN_LAUNCH_CHECK();
 }
 
 void unsigned_distance_adjoint(

@@ -3,13 +3,13 @@
 // Reference: include/gwn/detail/gwn_bvh_refit_moment_impl.cuh:17-279 and
 // include/gwn/detail/gwn_bvh_refit_common.cuh:14-609.
 //
-// One kernel.  A thread starts at every wide node without internal children; it computes the
-// additive payload of each child slot (leaf payloads straight from the triangle records, so
-// leaves need no pending storage), writes the node's recentred coefficients, stores the node's
-// total payload and climbs: the last internal child to arrive at a parent (acquire/release
-// arrival counter) continues with the parent.  Children are always summed in slot order from the
-// first valid slot, exactly as the reference's combine_node, so the result does not depend on
-// arrival order.  Arrival counters reset themselves, so a refit needs no clearing pass.
+// One launch per breadth-first level of the wide tree, deepest first.  A thread takes one node
+// of the level; it computes the additive payload of each child slot (leaf payloads straight from
+// the triangle records; an internal child's total was stored by the previous, deeper launch),
+// writes the node's recentred coefficients and stores the node's total.  Children are always
+// summed in slot order from the first valid slot, exactly as the reference's combine_node, and
+// the kernel boundary between levels is the only ordering needed: no arrival counters, atomics or
+// cross-workgroup fences.
 //
 // Output: one traversal record per node -- a uint4 of compact child references followed by the
 // reference's field-SoA gwn_bvh_moment_node<4, Order, float> coefficients as float4 (one float
@@ -44,52 +44,56 @@ template <int Order> __device__ inline void payload_add(payload<Order> &d, paylo
         d.v[i] += s.v[i];
 }
 
-// HDK sorted-vertex split integrals for one axis (gwn_bvh_refit_common.cuh:227-281).
+// Second-moment integrals of one triangle along axis i (HDK's sorted-vertex split,
+// gwn_bvh_refit_common.cuh:227-281).  The vertices are ordered along axis i as lo, mid, hi; the
+// line through `mid` perpendicular to axis i cuts the triangle into a lower piece (lo side) and an
+// upper piece (hi side), each integrated in closed form about the triangle centroid P.  Writes the
+// (i, i) integral and, when requested, the (i, j) and (i, k) cross integrals.
 __device__ inline void axis_integrals(
-    float const (&verts)[3][3], float const (&P)[3], int va, int vb, int vc, float *out_ii,
+    float const (&verts)[3][3], float const (&P)[3], int lo, int mid, int hi, float *out_ii,
     float *out_ij, float *out_ik, int i
 ) {
     int const j = (i == 2) ? 0 : (i + 1);
     int const k = (j == 2) ? 0 : (j + 1);
-    float const oab_i = verts[vb][i] - verts[va][i];
-    float const oab_j = verts[vb][j] - verts[va][j];
-    float const oab_k = verts[vb][k] - verts[va][k];
-    float const oac_i = verts[vc][i] - verts[va][i];
-    float const oac_j = verts[vc][j] - verts[va][j];
-    float const oac_k = verts[vc][k] - verts[va][k];
-    float const ocb_i = verts[vb][i] - verts[vc][i];
-    float const ocb_j = verts[vb][j] - verts[vc][j];
-    float const ocb_k = verts[vb][k] - verts[vc][k];
-    float const t = (oac_i > 0.0f) ? (oab_i / oac_i) : 0.0f;
-    float const jdiff = t * oac_j - oab_j;
-    float const kdiff = t * oac_k - oab_k;
-    float const ca0 = jdiff * oab_k - kdiff * oab_j;
-    float const ca1 = kdiff * oab_i;
-    float const ca2 = jdiff * oab_i;
-    float const cc0 = jdiff * ocb_k - kdiff * ocb_j;
-    float const cc1 = kdiff * ocb_i;
-    float const cc2 = jdiff * ocb_i;
-    float const sa = sqrtf(fmax_ref(ca0 * ca0 + ca1 * ca1 + ca2 * ca2, 0.0f));
-    float const sc = sqrtf(fmax_ref(cc0 * cc0 + cc1 * cc1 + cc2 * cc2, 0.0f));
-    float const Pai = verts[va][i] - P[i];
-    float const Pci = verts[vc][i] - P[i];
-    *out_ii = sa * (0.5f * Pai * Pai + float(2.0 / 3.0) * Pai * oab_i + 0.25f * oab_i * oab_i) +
-              sc * (0.5f * Pci * Pci + float(2.0 / 3.0) * Pci * ocb_i + 0.25f * ocb_i * ocb_i);
+    float const lm_i = verts[mid][i] - verts[lo][i];
+    float const lm_j = verts[mid][j] - verts[lo][j];
+    float const lm_k = verts[mid][k] - verts[lo][k];
+    float const lh_i = verts[hi][i] - verts[lo][i];
+    float const lh_j = verts[hi][j] - verts[lo][j];
+    float const lh_k = verts[hi][k] - verts[lo][k];
+    float const hm_i = verts[mid][i] - verts[hi][i];
+    float const hm_j = verts[mid][j] - verts[hi][j];
+    float const hm_k = verts[mid][k] - verts[hi][k];
+    float const frac = (lh_i > 0.0f) ? (lm_i / lh_i) : 0.0f;
+    float const cut_j = frac * lh_j - lm_j;
+    float const cut_k = frac * lh_k - lm_k;
+    float const wl0 = cut_j * lm_k - cut_k * lm_j;
+    float const wl1 = cut_k * lm_i;
+    float const wl2 = cut_j * lm_i;
+    float const wu0 = cut_j * hm_k - cut_k * hm_j;
+    float const wu1 = cut_k * hm_i;
+    float const wu2 = cut_j * hm_i;
+    float const area_lo = sqrtf(fmax_ref(wl0 * wl0 + wl1 * wl1 + wl2 * wl2, 0.0f));
+    float const area_hi = sqrtf(fmax_ref(wu0 * wu0 + wu1 * wu1 + wu2 * wu2, 0.0f));
+    float const rlo_i = verts[lo][i] - P[i];
+    float const rhi_i = verts[hi][i] - P[i];
+    *out_ii = area_lo * (0.5f * rlo_i * rlo_i + float(2.0 / 3.0) * rlo_i * lm_i + 0.25f * lm_i * lm_i) +
+              area_hi * (0.5f * rhi_i * rhi_i + float(2.0 / 3.0) * rhi_i * hm_i + 0.25f * hm_i * hm_i);
     for (int pass = 0; pass < 2; ++pass) {
         float *out = pass == 0 ? out_ij : out_ik;
         int const jk = pass == 0 ? j : k;
-        float const diff = pass == 0 ? jdiff : kdiff;
+        float const cut = pass == 0 ? cut_j : cut_k;
         if (out == nullptr)
             continue;
-        float const obmidj = verts[vb][jk] + 0.5f * diff;
-        float const oabmidj = obmidj - verts[va][jk];
-        float const ocbmidj = obmidj - verts[vc][jk];
-        float const Paj = verts[va][jk] - P[jk];
-        float const Pcj = verts[vc][jk] - P[jk];
-        *out = sa * (0.5f * Pai * Paj + float(1.0 / 3.0) * Pai * oabmidj +
-                     float(1.0 / 3.0) * Paj * oab_i + 0.25f * oab_i * oabmidj) +
-               sc * (0.5f * Pci * Pcj + float(1.0 / 3.0) * Pci * ocbmidj +
-                     float(1.0 / 3.0) * Pcj * ocb_i + 0.25f * ocb_i * ocbmidj);
+        float const split = verts[mid][jk] + 0.5f * cut;
+        float const lo_split = split - verts[lo][jk];
+        float const hi_split = split - verts[hi][jk];
+        float const rlo_j = verts[lo][jk] - P[jk];
+        float const rhi_j = verts[hi][jk] - P[jk];
+        *out = area_lo * (0.5f * rlo_i * rlo_j + float(1.0 / 3.0) * rlo_i * lo_split +
+                          float(1.0 / 3.0) * rlo_j * lm_i + 0.25f * lm_i * lo_split) +
+               area_hi * (0.5f * rhi_i * rhi_j + float(1.0 / 3.0) * rhi_i * hi_split +
+                          float(1.0 / 3.0) * rhi_j * hm_i + 0.25f * hm_i * hi_split);
     }
 }
 
@@ -258,102 +262,94 @@ __device__ inline void write_fields(float *fields, int slot, aabb const &bnd, pa
 #undef GWN_FIELD
 }
 
-// One thread's bottom-up pass: from node `start` when it has no internal children, then up the
-// parent links while this thread is the last to arrive at the parent.
+// One node of a level pass: the payload of each child slot (leaf payloads straight from the
+// triangle records, internal ones from the child's total written by the previous, deeper level),
+// the node's recentred coefficients and slot areas, and the node's total in slot order from the
+// first valid slot (the reference's combine_node, gwn_bvh_refit_moment_impl.cuh:17-99).  The whole
+// record is written, empty slots and padding as zeros, so the output needs no clearing pass.
 template <int Order>
-__device__ inline void moment_refit_from(
-    std::size_t start, bvh_node4 const *__restrict__ nodes, std::size_t node_count, tri_record const *__restrict__ tris,
-    std::size_t tri_count, std::uint32_t const *__restrict__ parent,
-    std::uint8_t const *__restrict__ internal_children, std::uint32_t *arrivals,
-    payload<Order> *pending, float4 *__restrict__ records, float *__restrict__ slot_area, std::uint32_t *error
+__device__ inline void moment_refit_node(
+    std::size_t n, bvh_node4 const *__restrict__ nodes, std::size_t node_count, tri_record const *__restrict__ tris,
+    std::size_t tri_count, payload<Order> *__restrict__ pending, float4 *__restrict__ records,
+    float *__restrict__ slot_area, std::uint32_t *error
 ) {
     constexpr int RV = moment_record_vec4(Order);
-    if (internal_children[start] != 0)
-        return;
-    std::size_t n = start;
-    for (;;) {
-        float4 *rec = records + n * RV;
-        float *fields = reinterpret_cast<float *>(rec + 1);
-        std::uint32_t cref[4] = {k_invalid, k_invalid, k_invalid, k_invalid};
-        payload<Order> total;
-        payload_zero(total);
-        bool init = false;
-        for (int s = 0; s < 4; ++s) {
-            bvh_child const ch = nodes[n].children[s];
-            std::uint64_t const ref = ch.reference;
-            if (!ref_valid(ref))
-                continue;
-            payload<Order> child;
-            if (ref_internal(ref)) {
-                std::uint64_t const c = ref_offset(ref);
-                if (c >= node_count || c <= n) {
-                    atomicOr(error, 1u);
-                    return;
-                }
-                child = pending[c];
-                cref[s] = static_cast<std::uint32_t>(c);
-            } else {
-                // A leaf payload starts from zero and adds each primitive in order
-                // (gwn_bvh_refit_common.cuh:452-468).
-                std::uint64_t const off = ref_offset(ref);
-                std::uint32_t const cnt = ref_count(ref);
-                if (off + cnt > tri_count || cnt != 1u) {
-                    atomicOr(error, 1u);
-                    return;
-                }
-                payload_zero(child);
-                payload<Order> tri;
-                triangle_payload<Order>(tris[off], tri);
-                payload_add(child, tri);
-                cref[s] = k_cref_leaf | static_cast<std::uint32_t>(off);
-            }
-            write_fields<Order>(fields, s, ch.bounds, child);
-            slot_area[n * 4 + s] = child.v[0];
-            if (init)
-                payload_add(total, child);
-            else
-                total = child;
-            init = true;
+    constexpr int F = moment_field_count(Order);
+    float4 *rec = records + n * RV;
+    float *fields = reinterpret_cast<float *>(rec + 1);
+    std::uint32_t cref[4] = {k_invalid, k_invalid, k_invalid, k_invalid};
+    payload<Order> total;
+    payload_zero(total);
+    bool init = false;
+    for (int s = 0; s < 4; ++s) {
+        bvh_child const ch = nodes[n].children[s];
+        std::uint64_t const ref = ch.reference;
+        if (!ref_valid(ref)) {
+            for (int f = 0; f < F; ++f)
+                fields[f * 4 + s] = 0.0f;
+            slot_area[n * 4 + s] = 0.0f;
+            continue;
         }
-        rec[0] = make_float4(__uint_as_float(cref[0]), __uint_as_float(cref[1]),
-                             __uint_as_float(cref[2]), __uint_as_float(cref[3]));
-        std::uint32_t const link = parent[n];
-        if (link == k_invalid)
-            return;
-        pending[n] = total;
-        std::size_t const pn = link >> 2;
-        release_agent();
-        std::uint32_t const prior =
-            __hip_atomic_fetch_add(&arrivals[pn], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (prior + 1u < internal_children[pn])
-            return;
-        acquire_agent();
-        arrivals[pn] = 0u;
-        n = pn;
+        payload<Order> child;
+        if (ref_internal(ref)) {
+            std::uint64_t const c = ref_offset(ref);
+            if (c >= node_count || c <= n) {
+                atomicOr(error, 1u);
+                return;
+            }
+            child = pending[c];
+            cref[s] = static_cast<std::uint32_t>(c);
+        } else {
+            // A leaf payload starts from zero and adds each primitive in order
+            // (gwn_bvh_refit_common.cuh:452-468).
+            std::uint64_t const off = ref_offset(ref);
+            std::uint32_t const cnt = ref_count(ref);
+            if (off + cnt > tri_count || cnt != 1u) {
+                atomicOr(error, 1u);
+                return;
+            }
+            payload_zero(child);
+            payload<Order> tri;
+            triangle_payload<Order>(tris[off], tri);
+            payload_add(child, tri);
+            cref[s] = k_cref_leaf | static_cast<std::uint32_t>(off);
+        }
+        write_fields<Order>(fields, s, ch.bounds, child);
+        slot_area[n * 4 + s] = child.v[0];
+        if (init)
+            payload_add(total, child);
+        else
+            total = child;
+        init = true;
     }
+    for (int k = 4 + F * 4; k < RV * 4; ++k)
+        reinterpret_cast<float *>(rec)[k] = 0.0f;
+    rec[0] = make_float4(__uint_as_float(cref[0]), __uint_as_float(cref[1]), __uint_as_float(cref[2]),
+                         __uint_as_float(cref[3]));
+    pending[n] = total;
 }
 
+// One breadth-first level [first, first + count) of one tree; levels run deepest first, so every
+// internal child's total is complete (a kernel boundary orders the levels: no atomics, no fences).
 template <int Order>
-__global__ __launch_bounds__(k_block) void moment_refit_kernel(
+__global__ __launch_bounds__(k_block) void moment_level_kernel(
     bvh_node4 const *__restrict__ nodes, std::size_t node_count, tri_record const *__restrict__ tris,
-    std::size_t tri_count, std::uint32_t const *__restrict__ parent,
-    std::uint8_t const *__restrict__ internal_children, std::uint32_t *arrivals,
-    payload<Order> *pending, float4 *__restrict__ records, float *__restrict__ slot_area, std::uint32_t *error
+    std::size_t tri_count, std::size_t first, std::size_t count, payload<Order> *__restrict__ pending,
+    float4 *__restrict__ records, float *__restrict__ slot_area, std::uint32_t *error
 ) {
-    std::size_t const start = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (start >= node_count)
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    if (i >= count)
         return;
-    moment_refit_from<Order>(start, nodes, node_count, tris, tri_count, parent, internal_children, arrivals,
-                             pending, records, slot_area, error);
+    moment_refit_node<Order>(first + i, nodes, node_count, tris, tri_count, pending, records, slot_area, error);
 }
 
-// Batched refits: one thread per node of every mesh in the table (mesh found by binary search over
-// the node prefixes), so many meshes' refits take two launches instead of four each.
-__device__ inline std::uint32_t refit_mesh_of(moment_refit_mesh const *meshes, std::uint32_t count, std::uint64_t g) {
-    std::uint32_t lo = 0, hi = count;  // meshes[lo].first_node <= g, and g < meshes[hi].first_node
+// Batched refits: one launch per depth over that depth's nodes in every mesh.  Thread g finds its
+// segment (one mesh's node range at this depth) by binary search over the segments' prefixes.
+__device__ inline std::uint32_t segment_of(refit_segment const *seg, std::uint32_t count, std::uint64_t g) {
+    std::uint32_t lo = 0, hi = count;  // seg[lo].prefix <= g < seg[hi].prefix
     while (hi - lo > 1) {
         std::uint32_t const mid = lo + (hi - lo) / 2;
-        if (meshes[mid].first_node <= g)
+        if (seg[mid].prefix <= g)
             lo = mid;
         else
             hi = mid;
@@ -361,91 +357,62 @@ __device__ inline std::uint32_t refit_mesh_of(moment_refit_mesh const *meshes, s
     return lo;
 }
 
-// Zeroes what moment_refit_enqueue's fills zero: each node's record and slot areas, its arrival
-// counter, and the mesh's error word.
 template <int Order>
-__global__ __launch_bounds__(k_block) void moment_refit_batch_clear_kernel(
-    moment_refit_mesh const *__restrict__ meshes, std::uint32_t count, std::uint64_t total
+__global__ __launch_bounds__(k_block) void moment_level_batch_kernel(
+    moment_refit_mesh const *__restrict__ meshes, refit_segment const *__restrict__ seg, std::uint32_t seg_count,
+    std::uint64_t total, std::uint32_t *error
 ) {
     std::uint64_t const g = static_cast<std::uint64_t>(blockIdx.x) * k_block + threadIdx.x;
     if (g >= total)
         return;
-    moment_refit_mesh const m = meshes[refit_mesh_of(meshes, count, g)];
-    std::uint64_t const n = g - m.first_node;
-    constexpr int RV = moment_record_vec4(Order);
-    for (int k = 0; k < RV; ++k)
-        m.records[n * RV + k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (int s = 0; s < 4; ++s)
-        m.slot_area[n * 4 + s] = 0.0f;
-    m.arrivals[n] = 0u;
-    if (n == 0)
-        m.arrivals[m.node_count] = 0u;
-}
-
-template <int Order>
-__global__ __launch_bounds__(k_block) void moment_refit_batch_kernel(
-    moment_refit_mesh const *__restrict__ meshes, std::uint32_t count, std::uint64_t total
-) {
-    std::uint64_t const g = static_cast<std::uint64_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (g >= total)
+    refit_segment const sg = seg[segment_of(seg, seg_count, g)];
+    moment_refit_mesh const m = meshes[sg.mesh];
+    std::size_t const n = static_cast<std::size_t>(sg.first_node) + static_cast<std::size_t>(g - sg.prefix);
+    if (n >= m.node_count) {
+        atomicOr(error, 1u);
         return;
-    moment_refit_mesh const m = meshes[refit_mesh_of(meshes, count, g)];
-    moment_refit_from<Order>(g - m.first_node, m.nodes, m.node_count, m.tris, m.tri_count, m.parent,
-                             m.internal_children, m.arrivals, static_cast<payload<Order> *>(m.pending), m.records,
-                             m.slot_area, m.arrivals + m.node_count);
+    }
+    moment_refit_node<Order>(n, m.nodes, m.node_count, m.tris, m.tri_count, static_cast<payload<Order> *>(m.pending),
+                             m.records, m.slot_area, error);
 }
 
 template <int Order>
-void launch_refit_batch(moment_refit_mesh const *meshes, std::uint32_t count, std::uint64_t total, hipStream_t stream) {
-    unsigned const grid = grid_for(total, k_block);
-    moment_refit_batch_clear_kernel<Order><<<grid, k_block, 0, stream>>>(meshes, count, total);
-    GWN_LAUNCH_CHECK();
-    moment_refit_batch_kernel<Order><<<grid, k_block, 0, stream>>>(meshes, count, total);
-    GWN_LAUNCH_CHECK();
-}
-
-template <int Order>
-void launch_refit(gwn_bvh_handle const &bvh, gwn_moment_handle &m, hipStream_t stream) {
+void launch_levels(gwn_bvh_handle const &bvh, gwn_moment_handle &m, void *pending, std::uint32_t *error,
+                   hipStream_t stream) {
     std::size_t const N = bvh.node_count;
-    constexpr int RV = moment_record_vec4(Order);
-    m.records.allocate(N * RV, stream);
-    fill_u32(reinterpret_cast<std::uint32_t *>(m.records.ptr), 0u, N * RV * 4, stream);
-    m.slot_area.allocate(N * 4, stream);
-    fill_u32(reinterpret_cast<std::uint32_t *>(m.slot_area.ptr), 0u, N * 4, stream);
-    device_buffer<payload<Order>> pending(N, stream);
-    // Per-call arrival counters (plus the error word) so refits of several orders may run
-    // concurrently on one BVH.
-    device_buffer<std::uint32_t> arrivals(N + 1, stream);
-    fill_u32(arrivals.ptr, 0u, N + 1, stream);
-    std::uint32_t *error = arrivals.ptr + N;
-    moment_refit_kernel<Order><<<grid_for(N, k_block), k_block, 0, stream>>>(
-        bvh.nodes.ptr, N, bvh.triangles.ptr, bvh.triangle_count, bvh.parent.ptr,
-        bvh.internal_children.ptr, arrivals.ptr, pending.ptr, m.records.ptr, m.slot_area.ptr, error
-    );
-    GWN_LAUNCH_CHECK();
-    std::uint32_t host_error = 0;
-    GWN_HIP(hipMemcpyAsync(&host_error, error, sizeof(host_error), hipMemcpyDeviceToHost, stream));
-    GWN_HIP(hipStreamSynchronize(stream));
-    if (host_error != 0u)
-        throw_internal("bvh.refit.moment: Bottom-up moment refit did not converge.");
+    auto const &ls = bvh.level_start;
+    for (std::size_t d = ls.size() - 1; d-- > 0;) {
+        std::size_t const first = ls[d], count = ls[d + 1] - ls[d];
+        if (count == 0)
+            continue;
+        moment_level_kernel<Order><<<grid_for(count, k_block), k_block, 0, stream>>>(
+            bvh.nodes.ptr, N, bvh.triangles.ptr, bvh.triangle_count, first, count,
+            static_cast<payload<Order> *>(pending), m.records.ptr, m.slot_area.ptr, error);
+        GWN_LAUNCH_CHECK();
+    }
 }
 
-template <int Order>
-void enqueue_refit(gwn_bvh_handle const &bvh, gwn_moment_handle &m, moment_refit_work &w, hipStream_t stream) {
-    std::size_t const N = bvh.node_count;
-    constexpr int RV = moment_record_vec4(Order);
-    fill_u32(reinterpret_cast<std::uint32_t *>(m.records.ptr), 0u, N * RV * 4, stream);
-    fill_u32(reinterpret_cast<std::uint32_t *>(m.slot_area.ptr), 0u, N * 4, stream);
-    fill_u32(w.arrivals.ptr, 0u, N + 1, stream);
-    moment_refit_kernel<Order><<<grid_for(N, k_block), k_block, 0, stream>>>(
-        bvh.nodes.ptr, N, bvh.triangles.ptr, bvh.triangle_count, bvh.parent.ptr, bvh.internal_children.ptr,
-        w.arrivals.ptr, reinterpret_cast<payload<Order> *>(w.pending.ptr), m.records.ptr, m.slot_area.ptr,
-        w.arrivals.ptr + N);
-    GWN_LAUNCH_CHECK();
+void check_levels(gwn_bvh_handle const &bvh) {
+    auto const &ls = bvh.level_start;
+    if (ls.size() < 2 || ls.front() != 0u || ls.back() != bvh.node_count)
+        throw_internal("bvh.refit.moment: BVH level table does not cover its nodes.");
+    for (std::size_t d = 1; d < ls.size(); ++d)
+        if (ls[d] < ls[d - 1])
+            throw_internal("bvh.refit.moment: BVH level table is not ascending.");
 }
 
 std::size_t payload_bytes(int order) {
     return order == 0 ? sizeof(payload<0>) : (order == 1 ? sizeof(payload<1>) : sizeof(payload<2>));
+}
+
+void enqueue_levels(gwn_bvh_handle const &bvh, gwn_moment_handle &m, void *pending, std::uint32_t *error,
+                    hipStream_t stream) {
+    switch (m.order) {
+    case 0: launch_levels<0>(bvh, m, pending, error, stream); break;
+    case 1: launch_levels<1>(bvh, m, pending, error, stream); break;
+    case 2: launch_levels<2>(bvh, m, pending, error, stream); break;
+    default: throw_invalid("Taylor order must be 0, 1, or 2.");
+    }
 }
 
 } // namespace
@@ -453,49 +420,56 @@ std::size_t payload_bytes(int order) {
 void refit_moment_prepare(gwn_bvh_handle const &bvh, gwn_moment_handle &m, moment_refit_work &w, hipStream_t stream) {
     std::size_t const N = ref_internal(bvh.root.reference) ? bvh.node_count : 0;
     m.node_count = bvh.node_count;
+    m.storage_generation = next_storage_generation();
     if (N == 0)
         return;
+    check_levels(bvh);
     m.records.allocate(N * static_cast<std::size_t>(moment_record_vec4(m.order)), stream);
     m.slot_area.allocate(N * 4, stream);
     w.pending.allocate(N * payload_bytes(m.order), stream);
-    w.arrivals.allocate(N + 1, stream);
 }
 
-void refit_moment_enqueue(gwn_bvh_handle const &bvh, gwn_moment_handle &m, moment_refit_work &w, hipStream_t stream) {
-    m.node_count = bvh.node_count;
-    m.bvh_revision = bvh.revision;
-    if (!ref_internal(bvh.root.reference) || bvh.node_count == 0)
-        return;
-    switch (m.order) {
-    case 0: enqueue_refit<0>(bvh, m, w, stream); break;
-    case 1: enqueue_refit<1>(bvh, m, w, stream); break;
-    default: enqueue_refit<2>(bvh, m, w, stream); break;
-    }
-}
-
-void refit_moment_batch(moment_refit_mesh const *meshes, std::uint32_t count, std::uint64_t total_nodes, int order,
+void refit_moment_batch(moment_refit_mesh const *meshes, refit_segment const *segments,
+                        std::vector<refit_level> const &levels, int order, std::uint32_t *error,
                         hipStream_t stream) {
-    if (count == 0 || total_nodes == 0)
-        return;
-    switch (order) {
-    case 0: launch_refit_batch<0>(meshes, count, total_nodes, stream); break;
-    case 1: launch_refit_batch<1>(meshes, count, total_nodes, stream); break;
-    case 2: launch_refit_batch<2>(meshes, count, total_nodes, stream); break;
-    default: throw_invalid("Taylor order must be 0, 1, or 2.");
+    for (std::size_t d = levels.size(); d-- > 0;) {
+        refit_level const &lv = levels[d];
+        if (lv.nodes == 0)
+            continue;
+        unsigned const grid = grid_for(lv.nodes, k_block);
+        refit_segment const *seg = segments + lv.first_segment;
+        switch (order) {
+        case 0: moment_level_batch_kernel<0><<<grid, k_block, 0, stream>>>(meshes, seg, lv.segments, lv.nodes, error); break;
+        case 1: moment_level_batch_kernel<1><<<grid, k_block, 0, stream>>>(meshes, seg, lv.segments, lv.nodes, error); break;
+        case 2: moment_level_batch_kernel<2><<<grid, k_block, 0, stream>>>(meshes, seg, lv.segments, lv.nodes, error); break;
+        default: throw_invalid("Taylor order must be 0, 1, or 2.");
+        }
+        GWN_LAUNCH_CHECK();
     }
 }
+
+std::size_t moment_payload_bytes(int order) { return payload_bytes(order); }
 
 void refit_moment(gwn_bvh_handle const &bvh, gwn_moment_handle &m, hipStream_t stream) {
     m.node_count = bvh.node_count;
     m.bvh_revision = bvh.revision;
     if (!ref_internal(bvh.root.reference) || bvh.node_count == 0)
         return; // a leaf root has no moment nodes (gwn_bvh.cuh:290-291)
-    switch (m.order) {
-    case 0: launch_refit<0>(bvh, m, stream); break;
-    case 1: launch_refit<1>(bvh, m, stream); break;
-    case 2: launch_refit<2>(bvh, m, stream); break;
-    default: throw_invalid("Taylor order must be 0, 1, or 2.");
-    }
+    if (m.order < 0 || m.order > 2)
+        throw_invalid("Taylor order must be 0, 1, or 2.");
+    check_levels(bvh);
+    std::size_t const N = bvh.node_count;
+    m.records.allocate(N * static_cast<std::size_t>(moment_record_vec4(m.order)), stream);
+    m.slot_area.allocate(N * 4, stream);
+    device_buffer<std::uint8_t> pending(N * payload_bytes(m.order), stream);
+    device_buffer<std::uint32_t> error(1, stream);
+    fill_u32(error.ptr, 0u, 1, stream);
+    enqueue_levels(bvh, m, pending.ptr, error.ptr, stream);
+    std::uint32_t host_error = 0;
+    GWN_HIP(hipMemcpyAsync(&host_error, error.ptr, sizeof(host_error), hipMemcpyDeviceToHost, stream));
+    GWN_HIP(hipStreamSynchronize(stream));
+    if (host_error != 0u)
+        throw_internal("bvh.refit.moment: Bottom-up moment refit did not converge.");
 }
 
 } // namespace gwn

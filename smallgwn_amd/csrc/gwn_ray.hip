@@ -183,12 +183,15 @@ struct ray_scene {
     std::size_t nt;
 };
 
-__device__ inline void visit_leaf(ray_scene const &sc, ray_state const &r, std::uint64_t ref, float t_min, best_hit &best) {
+__device__ inline void visit_leaf(ray_scene const &sc, ray_state const &r, std::uint64_t ref, float t_min, best_hit &best,
+                                  traversal_work *work = nullptr) {
     std::uint64_t const off = ref_offset(ref);
     std::uint32_t const cnt = ref_count(ref);
     for (std::uint32_t k = 0; k < cnt; ++k) {
         if (off + k >= sc.nt)
             continue;
+        if (work != nullptr)
+            ++work->tris;
         intersect_triangle(r, sc.tris[off + k], sc.prim[off + k], t_min, best);
     }
 }
@@ -196,7 +199,8 @@ __device__ inline void visit_leaf(ray_scene const &sc, ray_state const &r, std::
 
 // Packed near-first traversal (:242-358).  Returns true on stack overflow.
 template <int Cap>
-__device__ inline bool traverse_packed(ray_scene const &sc, std::uint32_t root, ray_state const &r, float t_min, best_hit &best) {
+__device__ inline bool traverse_packed(ray_scene const &sc, std::uint32_t root, ray_state const &r, float t_min, best_hit &best,
+                                       traversal_work *work = nullptr) {
     // One loop exit, at the head: no return from inside the unrolled push loop (the gfx950 backend
     // of ROCm 7.2 has lowered such early exits wrongly).
     std::uint32_t stack[Cap];
@@ -208,7 +212,7 @@ __device__ inline bool traverse_packed(ray_scene const &sc, std::uint32_t root, 
         if (payload & k_payload_leaf) {
             int const slot = static_cast<int>(payload & 3u);
             std::uint32_t const parent = (payload & ~k_payload_leaf) >> 2;
-            visit_leaf(sc, r, sc.nodes[parent].children[slot].reference, t_min, best);
+            visit_leaf(sc, r, sc.nodes[parent].children[slot].reference, t_min, best, work);
         } else if (payload < sc.node_count) {
             bvh_node4 const &node = sc.nodes[payload];
             float et[4];
@@ -223,6 +227,8 @@ __device__ inline bool traverse_packed(ray_scene const &sc, std::uint32_t root, 
                 bool const leaf = ref_leaf(ch.reference);
                 if (!leaf && ref_offset(ch.reference) >= sc.node_count)
                     continue;
+                if (work != nullptr)
+                    ++work->boxes;
                 float t_near;
                 if (!box_packed(r, ch.bounds, t_min, best.t, t_near))
                     continue;
@@ -257,7 +263,8 @@ __device__ inline bool traverse_packed(ray_scene const &sc, std::uint32_t root, 
 
 // Canonical traversal (:485-560).  Returns true on stack overflow.
 template <int Cap>
-__device__ inline bool traverse_canonical(ray_scene const &sc, std::uint64_t root_ref, ray_state const &r, float t_min, best_hit &best) {
+__device__ inline bool traverse_canonical(ray_scene const &sc, std::uint64_t root_ref, ray_state const &r, float t_min,
+                                          best_hit &best, traversal_work *work = nullptr) {
     std::uint32_t stack[Cap];
     int sp = 0;
     bool overflow = false;
@@ -265,7 +272,7 @@ __device__ inline bool traverse_canonical(ray_scene const &sc, std::uint64_t roo
     while (reference != 0 && !overflow) {
         std::uint64_t next = 0;
         if (ref_leaf(reference)) {
-            visit_leaf(sc, r, reference, t_min, best);
+            visit_leaf(sc, r, reference, t_min, best, work);
         } else if (ref_internal(reference) && ref_offset(reference) < sc.node_count) {
             bvh_node4 const &node = sc.nodes[ref_offset(reference)];
             float et[4];
@@ -276,6 +283,8 @@ __device__ inline bool traverse_canonical(ray_scene const &sc, std::uint64_t roo
                 bvh_child const ch = node.children[s];
                 if (!ref_valid(ch.reference))
                     continue;
+                if (work != nullptr)
+                    ++work->boxes;
                 float t_near;
                 if (!box_canonical(r, ch.bounds, t_min, best.t, t_near))
                     continue;
@@ -294,7 +303,7 @@ __device__ inline bool traverse_canonical(ray_scene const &sc, std::uint64_t roo
 #pragma unroll
             for (int s = 0; s < 4; ++s)
                 if (s < m && ref_leaf(er[s]) && et[s] <= best.t)
-                    visit_leaf(sc, r, er[s], t_min, best);
+                    visit_leaf(sc, r, er[s], t_min, best, work);
 #pragma unroll
             for (int s = 3; s >= 0; --s) {
                 if (s >= m || !ref_internal(er[s]) || et[s] > best.t || overflow)
@@ -315,11 +324,13 @@ __device__ inline bool traverse_canonical(ray_scene const &sc, std::uint64_t roo
     return overflow;
 }
 
-template <bool Packed, int Cap>
+// With Count, only the work counters (slab tests, triangle tests) are written, two per ray.
+template <bool Packed, int Cap, bool Count = false>
 __global__ __launch_bounds__(k_block) void ray_kernel(
     ray_scene sc, std::uint64_t root_ref, float const *__restrict__ ox, float const *__restrict__ oy,
     float const *__restrict__ oz, float const *__restrict__ dx, float const *__restrict__ dy,
-    float const *__restrict__ dz, ray_hit *__restrict__ out, std::size_t n, float t_min, float t_max
+    float const *__restrict__ dz, ray_hit *__restrict__ out, std::size_t n, float t_min, float t_max,
+    std::uint32_t *__restrict__ counters = nullptr
 ) {
     std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
     if (i >= n)
@@ -343,11 +354,18 @@ __global__ __launch_bounds__(k_block) void ray_kernel(
         }
         r.parallel = r.zero[0] || r.zero[1] || r.zero[2];
         best_hit best{t_max, 0.0f, 0.0f, {0.0f, 0.0f, 0.0f}, 0xffffffffu, false};
+        traversal_work w;
+        traversal_work *wp = Count ? &w : nullptr;
         bool overflow;
         if constexpr (Packed)
-            overflow = traverse_packed<Cap>(sc, static_cast<std::uint32_t>(ref_offset(root_ref)), r, t_min, best);
+            overflow = traverse_packed<Cap>(sc, static_cast<std::uint32_t>(ref_offset(root_ref)), r, t_min, best, wp);
         else
-            overflow = traverse_canonical<Cap>(sc, root_ref, r, t_min, best);
+            overflow = traverse_canonical<Cap>(sc, root_ref, r, t_min, best, wp);
+        if constexpr (Count) {
+            counters[2 * i] = w.boxes;
+            counters[2 * i + 1] = w.tris;
+            return;
+        }
         if (best.found) {
             res.t = best.t;
             res.u = best.u;
@@ -360,35 +378,63 @@ __global__ __launch_bounds__(k_block) void ray_kernel(
         }
         if (overflow)
             res.status = 2;
+    } else if constexpr (Count) {
+        counters[2 * i] = 0u;
+        counters[2 * i + 1] = 0u;
+        return;
     }
-    out[i] = res;
+    if constexpr (!Count)
+        out[i] = res;
 }
 
 template <bool Packed, int Cap>
 void launch_ray(
     gwn_bvh_handle const &bvh, float const *ox, float const *oy, float const *oz, float const *dx,
-    float const *dy, float const *dz, void *out, std::size_t n, float t_min, float t_max, hipStream_t stream
+    float const *dy, float const *dz, void *out, std::size_t n, float t_min, float t_max,
+    std::uint32_t *counters, hipStream_t stream
 ) {
     ray_scene const sc{bvh.nodes.ptr, bvh.node_count, bvh.triangles.ptr, bvh.primitive_indices.ptr, bvh.triangle_count};
-    ray_kernel<Packed, Cap><<<grid_for(n, k_block), k_block, 0, stream>>>(
-        sc, bvh.root.reference, ox, oy, oz, dx, dy, dz, static_cast<ray_hit *>(out), n, t_min, t_max
-    );
+    if (counters != nullptr)
+        ray_kernel<Packed, Cap, true><<<grid_for(n, k_block), k_block, 0, stream>>>(
+            sc, bvh.root.reference, ox, oy, oz, dx, dy, dz, nullptr, n, t_min, t_max, counters);
+    else
+        ray_kernel<Packed, Cap><<<grid_for(n, k_block), k_block, 0, stream>>>(
+            sc, bvh.root.reference, ox, oy, oz, dx, dy, dz, static_cast<ray_hit *>(out), n, t_min, t_max);
     GWN_LAUNCH_CHECK();
 }
 
 template <bool Packed>
 void launch_ray_cap(
     gwn_bvh_handle const &bvh, int need, float const *ox, float const *oy, float const *oz, float const *dx,
-    float const *dy, float const *dz, void *out, std::size_t n, float t_min, float t_max, hipStream_t stream
+    float const *dy, float const *dz, void *out, std::size_t n, float t_min, float t_max, std::uint32_t *counters,
+    hipStream_t stream
 ) {
     if (need <= 32)
-        launch_ray<Packed, 32>(bvh, ox, oy, oz, dx, dy, dz, out, n, t_min, t_max, stream);
+        launch_ray<Packed, 32>(bvh, ox, oy, oz, dx, dy, dz, out, n, t_min, t_max, counters, stream);
     else if (need <= 64)
-        launch_ray<Packed, 64>(bvh, ox, oy, oz, dx, dy, dz, out, n, t_min, t_max, stream);
+        launch_ray<Packed, 64>(bvh, ox, oy, oz, dx, dy, dz, out, n, t_min, t_max, counters, stream);
     else if (need <= 128)
-        launch_ray<Packed, 128>(bvh, ox, oy, oz, dx, dy, dz, out, n, t_min, t_max, stream);
+        launch_ray<Packed, 128>(bvh, ox, oy, oz, dx, dy, dz, out, n, t_min, t_max, counters, stream);
     else
         throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
+}
+
+void dispatch_ray(
+    gwn_bvh_handle const &bvh, float const *ox, float const *oy, float const *oz, float const *dx,
+    float const *dy, float const *dz, void *out, std::size_t n, float t_min, float t_max,
+    query_launch const &cfg, std::uint32_t *counters, hipStream_t stream
+) {
+    if (n == 0)
+        return;
+    // gwn_query.cuh:276-289: packed payloads hold a 29-bit parent offset.
+    bool const packed = ref_internal(bvh.root.reference) && bvh.node_count < (std::size_t(1) << 29) &&
+                        static_cast<int>(bvh.packed_stack_bound) <= cfg.stack_capacity;
+    if (packed)
+        launch_ray_cap<true>(bvh, static_cast<int>(bvh.packed_stack_bound), ox, oy, oz, dx, dy, dz, out, n, t_min,
+                             t_max, counters, stream);
+    else
+        launch_ray_cap<false>(bvh, static_cast<int>(bvh.internal_stack_bound), ox, oy, oz, dx, dy, dz, out, n,
+                              t_min, t_max, counters, stream);
 }
 
 } // namespace
@@ -398,17 +444,15 @@ void ray_first_hit(
     float const *dy, float const *dz, void *out, std::size_t n, float t_min, float t_max,
     query_launch const &cfg, hipStream_t stream
 ) {
-    if (n == 0)
-        return;
-    // gwn_query.cuh:276-289: packed payloads hold a 29-bit parent offset.
-    bool const packed = ref_internal(bvh.root.reference) && bvh.node_count < (std::size_t(1) << 29) &&
-                        static_cast<int>(bvh.packed_stack_bound) <= cfg.stack_capacity;
-    if (packed)
-        launch_ray_cap<true>(bvh, static_cast<int>(bvh.packed_stack_bound), ox, oy, oz, dx, dy, dz, out, n, t_min,
-                             t_max, stream);
-    else
-        launch_ray_cap<false>(bvh, static_cast<int>(bvh.internal_stack_bound), ox, oy, oz, dx, dy, dz, out, n,
-                              t_min, t_max, stream);
+    dispatch_ray(bvh, ox, oy, oz, dx, dy, dz, out, n, t_min, t_max, cfg, nullptr, stream);
+}
+
+void ray_first_hit_work(
+    gwn_bvh_handle const &bvh, float const *ox, float const *oy, float const *oz, float const *dx,
+    float const *dy, float const *dz, std::size_t n, float t_min, float t_max, query_launch const &cfg,
+    std::uint32_t *counters, hipStream_t stream
+) {
+    dispatch_ray(bvh, ox, oy, oz, dx, dy, dz, nullptr, n, t_min, t_max, cfg, counters, stream);
 }
 
 } // namespace gwn

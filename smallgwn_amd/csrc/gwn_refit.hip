@@ -6,11 +6,13 @@
 // its child boxes in slot order, bounds = union(bounds, child) (combine_node, :40-51), and leaf
 // boxes come from the same vertex loads as the records (gwn_bvh_triangle_impl.cuh:56-66).
 //
-// One thread starts at every node without internal children.  It writes its own child slots,
-// publishes its node box and arrives at the parent; the last arrival continues with the parent,
-// so every node is combined exactly once after all of its children.  Child boxes are written
-// straight into the node array (the reference stages them and commits after convergence; the
-// BVH is marked unqueryable for the duration in both designs).
+// One launch per breadth-first level of the wide tree, deepest first.  A thread takes one node of
+// the level: it rewrites the records of its leaf slots, writes every child box (a leaf's from its
+// triangles, an internal child's from the fold the previous, deeper launch stored) and stores its
+// own fold for its parent's level.  The kernel boundary between levels orders the passes: no
+// arrival counters or cross-workgroup fences.  Child boxes are written straight into the node
+// array (the reference stages them and commits after convergence; the BVH is marked unqueryable
+// for the duration in both designs).
 #include "gwn_core.hpp"
 
 // Box folds must match the oracle bit for bit; there is no arithmetic to contract here, but keep
@@ -21,7 +23,6 @@ namespace gwn {
 namespace {
 
 constexpr int k_block = 256;
-constexpr std::uint32_t k_invalid = 0xffffffffu;
 
 __device__ inline float min_ref(float a, float b) { return (b < a) ? b : a; } // std::min(a, b)
 __device__ inline float max_ref(float a, float b) { return (a < b) ? b : a; } // std::max(a, b)
@@ -63,60 +64,41 @@ __device__ inline bool refit_leaf_range(
     return true;
 }
 
-__global__ __launch_bounds__(k_block) void refit_bounds_kernel(
-    refit_geometry g, bvh_node4 *nodes, std::size_t node_count, std::uint32_t const *__restrict__ parent,
-    std::uint8_t const *__restrict__ internal_children, std::uint32_t *arrivals, aabb *pending,
-    std::uint32_t *error
+__global__ __launch_bounds__(k_block) void refit_level_kernel(
+    refit_geometry g, bvh_node4 *nodes, std::size_t node_count, std::size_t first, std::size_t count,
+    aabb *pending, std::uint32_t *error
 ) {
-    std::size_t const start = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (start >= node_count || internal_children[start] != 0)
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    if (i >= count)
         return;
-    std::size_t n = start;
-    for (;;) {
-        aabb acc{};
-        bool init = false;
-        for (int s = 0; s < 4; ++s) {
-            std::uint64_t const ref = nodes[n].children[s].reference;
-            if (!ref_valid(ref))
-                continue;
-            aabb box;
-            if (ref_internal(ref)) {
-                std::uint64_t const c = ref_offset(ref);
-                if (c >= node_count || c <= n) {
-                    atomicOr(error, 1u);
-                    return;
-                }
-                box = pending[c];
-            } else if (!refit_leaf_range(g, ref_offset(ref), ref_count(ref), box)) {
+    std::size_t const n = first + i;
+    aabb acc{};
+    bool init = false;
+    for (int s = 0; s < 4; ++s) {
+        std::uint64_t const ref = nodes[n].children[s].reference;
+        if (!ref_valid(ref))
+            continue;
+        aabb box;
+        if (ref_internal(ref)) {
+            std::uint64_t const c = ref_offset(ref);
+            if (c >= node_count || c <= n) {
                 atomicOr(error, 1u);
                 return;
             }
-            nodes[n].children[s].bounds = box;
-            acc = init ? union_ref(acc, box) : box;
-            init = true;
-        }
-        if (!init) {
+            box = pending[c];
+        } else if (!refit_leaf_range(g, ref_offset(ref), ref_count(ref), box)) {
             atomicOr(error, 1u);
             return;
         }
-        pending[n] = acc;
-        std::uint32_t const link = parent[n];
-        if (link == k_invalid)
-            return;
-        std::size_t const pn = link >> 2;
-        if (pn >= n) {
-            atomicOr(error, 1u);
-            return;
-        }
-        release_agent();
-        std::uint32_t const prior =
-            __hip_atomic_fetch_add(&arrivals[pn], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (prior + 1u < internal_children[pn])
-            return;
-        acquire_agent();
-        arrivals[pn] = 0u;
-        n = pn;
+        nodes[n].children[s].bounds = box;
+        acc = init ? union_ref(acc, box) : box;
+        init = true;
     }
+    if (!init) {
+        atomicOr(error, 1u);
+        return;
+    }
+    pending[n] = acc;
 }
 
 // A leaf root owns the whole (at most 65535-primitive) range: one thread, no climb.
@@ -136,22 +118,28 @@ void refit_bounds(gwn_geometry_handle const &geo, gwn_bvh_handle &bvh, hipStream
     refit_geometry const g{geo.x.ptr, geo.y.ptr, geo.z.ptr, geo.i0.ptr, geo.i1.ptr, geo.i2.ptr,
                            geo.vertex_count, geo.triangle_count, bvh.primitive_indices.ptr,
                            bvh.triangles.ptr};
-    // Scratch: arrivals[N] + error word, then the node boxes.
-    device_buffer<std::uint32_t> words(N + 1, stream);
-    fill_u32(words.ptr, 0u, N + 1, stream);
-    std::uint32_t *error = words.ptr + N;
+    device_buffer<std::uint32_t> error(1, stream);
+    fill_u32(error.ptr, 0u, 1, stream);
     device_buffer<aabb> boxes(N > 0 ? N : 1, stream);
     if (ref_internal(bvh.root.reference)) {
-        refit_bounds_kernel<<<grid_for(N, k_block), k_block, 0, stream>>>(
-            g, bvh.nodes.ptr, N, bvh.parent.ptr, bvh.internal_children.ptr, words.ptr, boxes.ptr, error
-        );
+        auto const &ls = bvh.level_start;
+        if (ls.size() < 2 || ls.front() != 0u || ls.back() != N)
+            throw_internal("bvh.refit: BVH level table does not cover its nodes.");
+        for (std::size_t d = ls.size() - 1; d-- > 0;) {
+            std::size_t const first = ls[d], count = ls[d + 1] - ls[d];
+            if (count == 0)
+                continue;
+            refit_level_kernel<<<grid_for(count, k_block), k_block, 0, stream>>>(
+                g, bvh.nodes.ptr, N, first, count, boxes.ptr, error.ptr);
+            GWN_LAUNCH_CHECK();
+        }
     } else {
-        refit_leaf_root_kernel<<<1, 1, 0, stream>>>(g, bvh.root.reference, boxes.ptr, error);
+        refit_leaf_root_kernel<<<1, 1, 0, stream>>>(g, bvh.root.reference, boxes.ptr, error.ptr);
+        GWN_LAUNCH_CHECK();
     }
-    GWN_LAUNCH_CHECK();
     std::uint32_t host_error = 0;
     aabb root_box{};
-    GWN_HIP(hipMemcpyAsync(&host_error, error, sizeof(host_error), hipMemcpyDeviceToHost, stream));
+    GWN_HIP(hipMemcpyAsync(&host_error, error.ptr, sizeof(host_error), hipMemcpyDeviceToHost, stream));
     GWN_HIP(hipMemcpyAsync(&root_box, boxes.ptr, sizeof(aabb), hipMemcpyDeviceToHost, stream));
     GWN_HIP(hipStreamSynchronize(stream));
     if (host_error != 0u)

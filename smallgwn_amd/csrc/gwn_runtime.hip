@@ -62,4 +62,9 @@ std::uint64_t reserve_revision() noexcept {
     }
 }
 
+std::uint64_t next_storage_generation() noexcept {
+    static std::atomic<std::uint64_t> next{1};
+    return next.fetch_add(1, std::memory_order_relaxed);
+}
+
 } // namespace gwn

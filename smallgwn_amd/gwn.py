@@ -46,6 +46,7 @@ EXPORTED_SYMBOLS = (
     "gwn_compute_unsigned_distance_batch_ex_f32", "gwn_compute_unsigned_distance_adjoint_batch_ex_f32",
     "gwn_batch_create", "gwn_batch_destroy", "gwn_batch_refit_moments", "gwn_batch_query",
     "gwn_batch_set_tuning", "gwn_batch_get_tuning", "gwn_batch_tune", "gwn_batch_capture", "gwn_batch_replay",
+    "gwn_batch_status", "gwn_compute_unsigned_distance_work_f32", "gwn_compute_ray_first_hit_work_f32",
     "gwn_compute_query_order_f32", "gwn_radix_sort_pairs_u64", "gwn_radix_sort_pairs_u32",
 )
 
@@ -189,6 +190,9 @@ def _load():
         "gwn_batch_tune": [vp, vp, vp],
         "gwn_batch_capture": [vp, i32],
         "gwn_batch_replay": [vp, vp],
+        "gwn_batch_status": [vp, vp],
+        "gwn_compute_unsigned_distance_work_f32": [vp, vp, vp, vp, u64, f32, vp, vp],
+        "gwn_compute_ray_first_hit_work_f32": [vp, vp, vp, vp, vp, vp, vp, u64, f32, f32, vp, vp, vp],
         "gwn_compute_query_order_f32": [vp, vp, vp, vp, vp, u64, vp],
         "gwn_radix_sort_pairs_u64": [vp, vp, u64, i32, i32, vp],
         "gwn_radix_sort_pairs_u32": [vp, vp, u64, i32, i32, vp],
@@ -573,6 +577,30 @@ def gwn_compute_ray_first_hit_batch(bvh: BvhObject, ox, oy, oz, dx, dy, dz, out,
         float(t_min), float(t_max), ctypes.byref(cfg), _stream(stream)))
 
 
+def gwn_compute_unsigned_distance_work(bvh: BvhObject, qx, qy, qz, counters, culling_band: float = float("inf"),
+                                       stream=None) -> Status:
+    """Work counters of the unsigned-distance traversal: counters[2q] child box tests, counters[2q+1]
+    point-triangle tests of query q (int32/uint32 device array of 2 x count; no reference counterpart)."""
+    n = _numel(qx)
+    if _numel(qy) != n or _numel(qz) != n or _numel(counters) != 2 * n:
+        return Status(Error.INVALID_ARGUMENT, "Query spans and counters (2 per query) must match.")
+    return _status(_lib.gwn_compute_unsigned_distance_work_f32(bvh.handle, _dptr(qx), _dptr(qy), _dptr(qz), n,
+                                                               float(culling_band), _dptr(counters), _stream(stream)))
+
+
+def gwn_compute_ray_first_hit_work(bvh: BvhObject, ox, oy, oz, dx, dy, dz, counters, t_min: float = 0.0,
+                                   t_max: float = float("inf"), config: QueryConfig | None = None,
+                                   stream=None) -> Status:
+    """Work counters of the ray traversal: counters[2r] slab tests, counters[2r+1] triangle tests."""
+    n = _numel(ox)
+    if any(_numel(a) != n for a in (oy, oz, dx, dy, dz)) or _numel(counters) != 2 * n:
+        return Status(Error.INVALID_ARGUMENT, "Ray spans and counters (2 per ray) must match.")
+    c = _config(config)
+    return _status(_lib.gwn_compute_ray_first_hit_work_f32(
+        bvh.handle, _dptr(ox), _dptr(oy), _dptr(oz), _dptr(dx), _dptr(dy), _dptr(dz), n, float(t_min), float(t_max),
+        ctypes.byref(c), _dptr(counters), _stream(stream)))
+
+
 def gwn_compute_winding_number_antipodal_batch(geometry: GeometryObject, bvh: BvhObject,
                                                boundary: BoundaryChainObject, qx, qy, qz, out,
                                                config: QueryConfig | None = None, stream=None,
@@ -730,6 +758,10 @@ class BatchObject:
 
     def replay(self, stream=None) -> Status:
         return _status(_lib.gwn_batch_replay(self._h, _stream(stream)))
+
+    def status(self, stream=None) -> Status:
+        """Synchronizes the stream; INTERNAL_ERROR if a refit met an invalid tree since the last call."""
+        return _status(_lib.gwn_batch_status(self._h, _stream(stream)))
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -150,3 +150,72 @@ def test_batch_refuses_rebuilt_mesh_and_tracks_revisions(gwn, dev):
     st = batch.replay()
     assert st.error == gwn.Error.INVALID_ARGUMENT and "storage changed" in st.message
     assert batch.query().error == gwn.Error.INVALID_ARGUMENT
+
+
+def test_batch_refuses_stale_moments_without_refit(gwn, dev):
+    """After a bounds refit the moments no longer match their BVH's revision: an eager batch query
+    and a replay of a graph captured without refits are refused; an eager batch refit, or a replay
+    with refits, makes them usable again."""
+    import torch
+
+    items, refs = _setup(gwn, dev, 5, order=1, seed=11)
+    batch = gwn.BatchObject(items, accuracy_scale=2.0)
+    batch.capture(include_refit=False).raise_if_error()
+    batch.replay().raise_if_error()
+    _check(items, refs)
+    gwn.gwn_refit_bvh(refs[3][0], items[3][0]).raise_if_error()
+    st = batch.query()
+    assert st.error == gwn.Error.INVALID_ARGUMENT and "not aligned" in st.message
+    st = batch.replay()
+    assert st.error == gwn.Error.INVALID_ARGUMENT and "not aligned" in st.message
+    batch.refit_moments().raise_if_error()
+    batch.query().raise_if_error()
+    batch.replay().raise_if_error()
+    _check(items, refs)
+    assert batch.status().is_ok()
+
+
+def test_batch_capture_does_not_stamp_revisions(gwn, dev):
+    """Capturing refits records them without running them: until a replay runs, the moments stay
+    formed from the old bounds and a single-mesh query refuses them."""
+    import torch
+
+    items, refs = _setup(gwn, dev, 4, order=1, seed=13)
+    batch = gwn.BatchObject(items, accuracy_scale=2.0)
+    bvh, mom = items[2][0], items[2][1]
+    gwn.gwn_refit_bvh(refs[2][0], bvh).raise_if_error()
+    batch.capture(include_refit=True).raise_if_error()
+    out = torch.empty_like(items[2][5])
+    st = gwn.gwn_compute_winding_number_taylor_batch(bvh, mom, *items[2][2:5], out, accuracy_scale=2.0)
+    assert st.error == gwn.Error.INVALID_ARGUMENT and "not aligned" in st.message
+    batch.replay().raise_if_error()
+    gwn.gwn_compute_winding_number_taylor_batch(bvh, mom, *items[2][2:5], out, accuracy_scale=2.0).raise_if_error()
+    torch.cuda.synchronize()
+    _check(items, refs)
+
+
+def test_batch_refuses_storage_reused_at_the_same_addresses(gwn, dev):
+    """The pool may hand a rebuilt BVH or refit moment the addresses it had before; the batch still
+    refuses it, because every swap of storage takes a new generation."""
+    items, refs = _setup(gwn, dev, 3, order=1, seed=17)
+    batch = gwn.BatchObject(items, accuracy_scale=2.0)
+    geom = refs[1][0]
+    for _ in range(2):
+        gwn.gwn_build_bvh(geom, items[1][0]).raise_if_error()
+    st = batch.query()
+    assert st.error == gwn.Error.INVALID_ARGUMENT and "storage changed" in st.message
+    items2, refs2 = _setup(gwn, dev, 3, order=2, seed=19)
+    batch2 = gwn.BatchObject(items2, accuracy_scale=2.0)
+    for _ in range(2):
+        gwn.gwn_refit_bvh_moment(items2[2][0], items2[2][1]).raise_if_error()
+    st = batch2.query()
+    assert st.error == gwn.Error.INVALID_ARGUMENT and "storage changed" in st.message
+    assert batch2.capture(include_refit=True).error == gwn.Error.INVALID_ARGUMENT
+
+
+def test_batch_rejects_shared_moment_objects(gwn, dev):
+    items, _ = _setup(gwn, dev, 3, order=1, seed=23)
+    shared = list(items)
+    shared[2] = (shared[1][0], shared[1][1]) + tuple(shared[2][2:])
+    with pytest.raises(Exception, match="own moment"):
+        gwn.BatchObject(shared, accuracy_scale=2.0)
