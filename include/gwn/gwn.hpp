@@ -3,8 +3,11 @@
 // Header-only.  Each template below keeps the name, template parameters and argument order of
 // the reference function it replaces and forwards to one extern "C" entry point of
 // include/gwn_hip.h, so code written against the reference's public API compiles against this
-// header (Real = float, Index = std::uint32_t, Width = 4; other instantiations are rejected at
-// compile time).  Needs C++20 (class-type template parameters, as the reference uses).
+// header.  Objects take Real = float or double, Index = std::uint32_t or std::uint64_t and
+// Width = 2, 3 or 4.  Every query batch takes Real = float; Real = double objects are uploaded,
+// built, refit, moment-refit and queried by the exact and Taylor winding batches (the reference's
+// FP64 workflow); the other double batches and the device accessors of double objects are
+// rejected at compile time.  Needs C++20 (class-type template parameters, as the reference uses).
 #pragma once
 
 #include <cstddef>
@@ -13,6 +16,7 @@
 #include <utility>
 
 #include "../gwn_hip.h"
+#include "gwn_accessor.hpp"
 
 namespace gwn {
 
@@ -40,6 +44,13 @@ private:
 };
 
 namespace detail {
+template <class Real> inline constexpr bool gwn_is_real_v =
+    (sizeof(Real) == 4 || sizeof(Real) == 8) && Real(0.5) != Real(0);
+template <class Real> inline constexpr bool gwn_is_double_v = gwn_is_real_v<Real> && sizeof(Real) == 8;
+template <class Index> inline constexpr bool gwn_is_index_v =
+    (sizeof(Index) == 4 || sizeof(Index) == 8) && Index(-1) > Index(0);
+template <class Real> inline constexpr bool gwn_float_only_v = sizeof(Real) == 4;
+
 [[nodiscard]] inline gwn_status gwn_from_code(int code) {
     if (code == GWN_SUCCESS)
         return gwn_status::ok();
@@ -82,7 +93,14 @@ private:
 // ---- objects: move-only owners of device state (gwn_geometry.cuh:98-149, gwn_bvh.cuh:331-429) --
 
 namespace detail {
-template <class Handle, int (*Destroy)(Handle)> class gwn_handle_owner {
+// Move-only owner of one library handle with the reference's stream mixin
+// (include/gwn/gwn_utils.cuh:59-80): stream() is the bound stream, set_stream() rebinds without
+// synchronizing, clear() releases the storage on the bound stream, and swap exchanges storage and
+// binding together.  The binding lives in the library object, so it follows the operations that
+// replace or write the storage exactly as the C ABI documents.
+template <class Handle, int (*Destroy)(Handle), int (*GetStream)(Handle, hipStream_t *),
+          int (*SetStream)(Handle, hipStream_t), int (*Clear)(Handle)>
+class gwn_handle_owner {
 public:
     gwn_handle_owner() = default;
     gwn_handle_owner(gwn_handle_owner const &) = delete;
@@ -97,6 +115,14 @@ public:
             Destroy(h_);
     }
     [[nodiscard]] Handle get() const noexcept { return h_; }
+    [[nodiscard]] hipStream_t stream() const noexcept {
+        hipStream_t s = nullptr;
+        GetStream(h_, &s);
+        return s;
+    }
+    void set_stream(hipStream_t stream) noexcept { SetStream(h_, stream); }
+    void clear() noexcept { Clear(h_); }
+    friend void swap(gwn_handle_owner &a, gwn_handle_owner &b) noexcept { std::swap(a.h_, b.h_); }
 
 protected:
     Handle h_ = nullptr;
@@ -104,9 +130,11 @@ protected:
 } // namespace detail
 
 template <class Real, class Index>
-class gwn_geometry_object final : public detail::gwn_handle_owner<gwn_geometry_t, gwn_geometry_destroy> {
-    static_assert(sizeof(Real) == 4 && Real(0.5) != Real(0), "libgwn_hip provides Real = float");
-    static_assert(sizeof(Index) == 4 && Index(-1) > Index(0), "libgwn_hip provides Index = uint32_t");
+class gwn_geometry_object final
+    : public detail::gwn_handle_owner<gwn_geometry_t, gwn_geometry_destroy, gwn_geometry_stream,
+                                      gwn_geometry_set_stream, gwn_geometry_clear> {
+    static_assert(detail::gwn_is_real_v<Real>, "libgwn_hip provides Real = float or double");
+    static_assert(detail::gwn_is_index_v<Index>, "libgwn_hip provides Index = uint32_t or uint64_t");
 
 public:
     gwn_geometry_object() { gwn_geometry_create(&h_); }
@@ -120,12 +148,21 @@ public:
         gwn_geometry_counts(h_, &v, &t);
         return t;
     }
+    [[nodiscard]] gwn_geometry_accessor<Real, Index> accessor() const noexcept {
+        gwn_geometry_accessor<Real, Index> a{};
+        gwn_geometry_get_accessor(h_, &a);
+        return a;
+    }
+    [[nodiscard]] bool has_data() const noexcept { return vertex_count() != 0 || triangle_count() != 0; }
 };
 
 template <int Width, class Real, class Index>
-class gwn_bvh_object final : public detail::gwn_handle_owner<gwn_bvh_t, gwn_bvh_destroy> {
-    static_assert(Width == 4, "libgwn_hip builds width-4 BVHs");
-    static_assert(sizeof(Real) == 4 && Real(0.5) != Real(0), "libgwn_hip provides Real = float");
+class gwn_bvh_object final
+    : public detail::gwn_handle_owner<gwn_bvh_t, gwn_bvh_destroy, gwn_bvh_stream, gwn_bvh_set_stream,
+                                      gwn_bvh_clear> {
+    static_assert(Width >= 2 && Width <= 4, "libgwn_hip builds BVHs of width 2, 3 or 4");
+    static_assert(detail::gwn_is_real_v<Real>, "libgwn_hip provides Real = float or double");
+    static_assert(detail::gwn_is_index_v<Index>, "libgwn_hip provides Index = uint32_t or uint64_t");
 
 public:
     gwn_bvh_object() { gwn_bvh_create(&h_); }
@@ -134,21 +171,49 @@ public:
         gwn_bvh_get_info(h_, &i);
         return i;
     }
+    [[nodiscard]] gwn_bvh_accessor<Width, Real, Index> accessor() const noexcept {
+        gwn_bvh_accessor<Width, Real, Index> a{};
+        gwn_bvh_get_accessor(h_, &a);
+        return a;
+    }
     [[nodiscard]] bool has_data() const noexcept { return info().triangle_count != 0; }
 };
 
 template <int Width, int Order, class Real, class Index>
-class gwn_bvh_moment_object final : public detail::gwn_handle_owner<gwn_moment_t, gwn_moment_destroy> {
-    static_assert(Width == 4, "libgwn_hip builds width-4 BVHs");
+class gwn_bvh_moment_object final
+    : public detail::gwn_handle_owner<gwn_moment_t, gwn_moment_destroy, gwn_moment_stream, gwn_moment_set_stream,
+                                      gwn_moment_clear> {
+    static_assert(Width >= 2 && Width <= 4, "libgwn_hip builds BVHs of width 2, 3 or 4");
     static_assert(Order >= 0 && Order <= 2, "Taylor order must be 0, 1, or 2");
 
 public:
     gwn_bvh_moment_object() { gwn_moment_create(&h_, Order); }
+    [[nodiscard]] gwn_bvh_moment_accessor<Width, Order, Real, Index> accessor() const noexcept {
+        gwn_bvh_moment_accessor<Width, Order, Real, Index> a{};
+        gwn_moment_get_accessor(h_, &a);
+        return a;
+    }
+    // Valid for `bvh` when formed from its current revision (include/gwn/gwn_bvh.cuh:286-293): a
+    // queryable BVH, the same revision, and moment nodes unless the root is a leaf.
+    [[nodiscard]] bool is_valid_for(gwn_bvh_object<Width, Real, Index> const &bvh) const noexcept {
+        std::uint64_t nodes = 0, revision = 0;
+        gwn_moment_info(h_, &nodes, &revision);
+        gwn_bvh_info const i = bvh.info();
+        bool const leaf_root = (i.root_reference >> 47 & 1u) != 0 && (i.root_reference >> 48) != 0;
+        return revision != 0 && revision == i.revision && i.triangle_count != 0 && (nodes != 0 || leaf_root);
+    }
+    [[nodiscard]] bool has_data() const noexcept {
+        std::uint64_t nodes = 0;
+        gwn_moment_info(h_, &nodes, nullptr);
+        return nodes != 0;
+    }
 };
 
 // gwn_boundary_chain_object<Index> (include/gwn/gwn_boundary.cuh:56-125).
 template <class Index = std::uint32_t>
-class gwn_boundary_chain_object final : public detail::gwn_handle_owner<gwn_boundary_t, gwn_boundary_destroy> {
+class gwn_boundary_chain_object final
+    : public detail::gwn_handle_owner<gwn_boundary_t, gwn_boundary_destroy, gwn_boundary_stream,
+                                      gwn_boundary_set_stream, gwn_boundary_clear> {
     static_assert(sizeof(Index) == 4 && Index(-1) > Index(0), "libgwn_hip provides Index = uint32_t");
 
 public:
@@ -162,6 +227,12 @@ public:
         std::int32_t b = 0;
         gwn_boundary_info(h_, nullptr, nullptr, nullptr, &b);
         return b != 0;
+    }
+    [[nodiscard]] bool has_data() const noexcept { return is_built(); }
+    [[nodiscard]] gwn_boundary_chain_accessor<Index> accessor() const noexcept {
+        gwn_boundary_chain_accessor<Index> a{};
+        gwn_boundary_get_accessor(h_, &a);
+        return a;
     }
 };
 
@@ -179,9 +250,28 @@ template <class Real, class Index>
 ) noexcept {
     if (y.size() != x.size() || z.size() != x.size() || i1.size() != i0.size() || i2.size() != i0.size())
         return gwn_status(gwn_error::invalid_argument, "Geometry SoA spans must have identical lengths.");
-    return detail::gwn_from_code(gwn_upload_geometry_f32(
-        object.get(), x.data(), y.data(), z.data(), x.size(), i0.data(), i1.data(), i2.data(), i0.size(), stream
-    ));
+    int code;
+    if constexpr (detail::gwn_is_double_v<Real> && sizeof(Index) == 8)
+        code = gwn_upload_geometry_f64_u64(object.get(), x.data(), y.data(), z.data(), x.size(),
+                                           reinterpret_cast<std::uint64_t const *>(i0.data()),
+                                           reinterpret_cast<std::uint64_t const *>(i1.data()),
+                                           reinterpret_cast<std::uint64_t const *>(i2.data()), i0.size(), stream);
+    else if constexpr (detail::gwn_is_double_v<Real>)
+        code = gwn_upload_geometry_f64(object.get(), x.data(), y.data(), z.data(), x.size(),
+                                       reinterpret_cast<std::uint32_t const *>(i0.data()),
+                                       reinterpret_cast<std::uint32_t const *>(i1.data()),
+                                       reinterpret_cast<std::uint32_t const *>(i2.data()), i0.size(), stream);
+    else if constexpr (sizeof(Index) == 8)
+        code = gwn_upload_geometry_f32_u64(object.get(), x.data(), y.data(), z.data(), x.size(),
+                                           reinterpret_cast<std::uint64_t const *>(i0.data()),
+                                           reinterpret_cast<std::uint64_t const *>(i1.data()),
+                                           reinterpret_cast<std::uint64_t const *>(i2.data()), i0.size(), stream);
+    else
+        code = gwn_upload_geometry_f32(object.get(), x.data(), y.data(), z.data(), x.size(),
+                                       reinterpret_cast<std::uint32_t const *>(i0.data()),
+                                       reinterpret_cast<std::uint32_t const *>(i1.data()),
+                                       reinterpret_cast<std::uint32_t const *>(i2.data()), i0.size(), stream);
+    return detail::gwn_from_code(code);
 }
 
 template <class Real, class Index>
@@ -191,7 +281,10 @@ template <class Real, class Index>
 ) noexcept {
     if (y.size() != x.size() || z.size() != x.size())
         return gwn_status(gwn_error::invalid_argument, "Vertex SoA spans must have identical lengths.");
-    return detail::gwn_from_code(gwn_update_geometry_f32(object.get(), x.data(), y.data(), z.data(), x.size(), stream));
+    if constexpr (detail::gwn_is_double_v<Real>)
+        return detail::gwn_from_code(gwn_update_geometry_f64(object.get(), x.data(), y.data(), z.data(), x.size(), stream));
+    else
+        return detail::gwn_from_code(gwn_update_geometry_f32(object.get(), x.data(), y.data(), z.data(), x.size(), stream));
 }
 
 // ---- build and refit (include/gwn/gwn_bvh_build.cuh:16-39, gwn_bvh_refit.cuh:25-51) ----------
@@ -209,9 +302,17 @@ template <int Width, class Real, class Index, class MortonCode = std::uint64_t>
     gwn_bvh_build_options const options = {}, hipStream_t stream = gwn_default_stream()
 ) noexcept {
     int const method = options.method == gwn_bvh_build_method::k_hploc ? GWN_BVH_BUILD_HPLOC : GWN_BVH_BUILD_LBVH;
-    return detail::gwn_from_code(
-        gwn_build_bvh4_f32(geometry.get(), bvh.get(), method, options.hploc_search_radius, stream)
-    );
+    // MortonCode: codes are always 63-bit (uint64_t, the reference default); a uint32_t request
+    // builds the same tree order the library supports and is accepted for source compatibility.
+    static_assert(sizeof(MortonCode) == 4 || sizeof(MortonCode) == 8, "MortonCode must be uint32_t or uint64_t");
+    if constexpr (detail::gwn_is_double_v<Real>)
+        return detail::gwn_from_code(
+            gwn_build_bvh_f64(geometry.get(), bvh.get(), Width, method, options.hploc_search_radius, stream)
+        );
+    else
+        return detail::gwn_from_code(
+            gwn_build_bvh_f32(geometry.get(), bvh.get(), Width, method, options.hploc_search_radius, stream)
+        );
 }
 
 template <int Width, class Real, class Index>
@@ -219,7 +320,10 @@ template <int Width, class Real, class Index>
     gwn_geometry_object<Real, Index> const &geometry, gwn_bvh_object<Width, Real, Index> &bvh,
     hipStream_t stream = gwn_default_stream()
 ) noexcept {
-    return detail::gwn_from_code(gwn_refit_bvh_f32(geometry.get(), bvh.get(), stream));
+    if constexpr (detail::gwn_is_double_v<Real>)
+        return detail::gwn_from_code(gwn_refit_bvh_f64(geometry.get(), bvh.get(), stream));
+    else
+        return detail::gwn_from_code(gwn_refit_bvh_f32(geometry.get(), bvh.get(), stream));
 }
 
 template <int Order, int Width, class Real, class Index>
@@ -227,13 +331,15 @@ template <int Order, int Width, class Real, class Index>
     gwn_bvh_object<Width, Real, Index> const &bvh, gwn_bvh_moment_object<Width, Order, Real, Index> &moment,
     hipStream_t stream = gwn_default_stream()
 ) noexcept {
-    return detail::gwn_from_code(gwn_refit_bvh_moment_f32(bvh.get(), moment.get(), stream));
+    if constexpr (detail::gwn_is_double_v<Real>)
+        return detail::gwn_from_code(gwn_refit_bvh_moment_f64(bvh.get(), moment.get(), stream));
+    else
+        return detail::gwn_from_code(gwn_refit_bvh_moment_f32(bvh.get(), moment.get(), stream));
 }
 
 // ---- query batches (include/gwn/gwn_query.cuh:41-61, 296-445) --------------------------------
 
 inline constexpr int k_gwn_default_query_batch_block_size = 256;
-inline constexpr int k_gwn_default_traversal_stack_capacity = 64;
 
 struct gwn_query_batch_config {
     int block_size = k_gwn_default_query_batch_block_size;
@@ -256,9 +362,14 @@ template <gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, c
     if (query_y.size() != query_x.size() || query_z.size() != query_x.size() || output.size() != query_x.size())
         return gwn_status(gwn_error::invalid_argument, "Query SoA and output spans must have identical lengths.");
     constexpr ::gwn_query_config c = detail::gwn_c_config<Config>();
-    return detail::gwn_from_code(gwn_compute_winding_number_exact_batch_f32(
-        bvh.get(), query_x.data(), query_y.data(), query_z.data(), output.data(), output.size(), &c, stream
-    ));
+    if constexpr (detail::gwn_is_double_v<Real>)
+        return detail::gwn_from_code(gwn_compute_winding_number_exact_batch_f64(
+            bvh.get(), query_x.data(), query_y.data(), query_z.data(), output.data(), output.size(), &c, stream
+        ));
+    else
+        return detail::gwn_from_code(gwn_compute_winding_number_exact_batch_f32(
+            bvh.get(), query_x.data(), query_y.data(), query_z.data(), output.data(), output.size(), &c, stream
+        ));
 }
 
 template <int Order, gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, class Real,
@@ -272,10 +383,16 @@ template <int Order, gwn_query_batch_config Config = gwn_query_batch_config{}, i
     if (query_y.size() != query_x.size() || query_z.size() != query_x.size() || output.size() != query_x.size())
         return gwn_status(gwn_error::invalid_argument, "Query SoA and output spans must have identical lengths.");
     constexpr ::gwn_query_config c = detail::gwn_c_config<Config>();
-    return detail::gwn_from_code(gwn_compute_winding_number_taylor_batch_f32(
-        bvh.get(), moment.get(), query_x.data(), query_y.data(), query_z.data(), output.data(), output.size(),
-        accuracy_scale, &c, stream
-    ));
+    if constexpr (detail::gwn_is_double_v<Real>)
+        return detail::gwn_from_code(gwn_compute_winding_number_taylor_batch_f64(
+            bvh.get(), moment.get(), query_x.data(), query_y.data(), query_z.data(), output.data(), output.size(),
+            accuracy_scale, &c, stream
+        ));
+    else
+        return detail::gwn_from_code(gwn_compute_winding_number_taylor_batch_f32(
+            bvh.get(), moment.get(), query_x.data(), query_y.data(), query_z.data(), output.data(), output.size(),
+            accuracy_scale, &c, stream
+        ));
 }
 
 template <int Order, gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, class Real,
@@ -287,6 +404,7 @@ template <int Order, gwn_query_batch_config Config = gwn_query_batch_config{}, i
     gwn_device_span<Real> output_x, gwn_device_span<Real> output_y, gwn_device_span<Real> output_z,
     Real const accuracy_scale = Real(2), hipStream_t stream = gwn_default_stream()
 ) noexcept {
+    static_assert(detail::gwn_float_only_v<Real>, "libgwn_hip provides this batch for Real = float");
     if (query_y.size() != query_x.size() || query_z.size() != query_x.size())
         return gwn_status(gwn_error::invalid_argument, "Query SoA spans must have identical lengths.");
     if (output_x.size() != query_x.size() || output_y.size() != query_x.size() || output_z.size() != query_x.size())
@@ -305,6 +423,7 @@ template <gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, c
     gwn_device_span<Real const> query_y, gwn_device_span<Real const> query_z, gwn_device_span<Real> output,
     Real const culling_band = Real(__builtin_huge_valf()), hipStream_t stream = gwn_default_stream()
 ) noexcept {
+    static_assert(detail::gwn_float_only_v<Real>, "libgwn_hip provides this batch for Real = float");
     if (query_y.size() != query_x.size() || query_z.size() != query_x.size())
         return gwn_status(gwn_error::invalid_argument, "Query SoA spans must have identical lengths.");
     if (output.size() != query_x.size())
@@ -316,23 +435,6 @@ template <gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, c
     ));
 }
 
-enum class gwn_ray_first_hit_status : std::uint8_t { k_miss = 0, k_hit = 1, k_overflow = 2 };
-
-// gwn_ray_first_hit_result<float, uint32_t> (include/gwn/gwn_query.cuh:74-95), 32 bytes.
-template <class Real = float, class Index = std::uint32_t> struct gwn_ray_first_hit_result {
-    Real t{Real(-1)};
-    Real u{Real(0)};
-    Real v{Real(0)};
-    Real geometric_normal_x{Real(0)};
-    Real geometric_normal_y{Real(0)};
-    Real geometric_normal_z{Real(0)};
-    Index primitive_id{Index(0xffffffffu)};
-    gwn_ray_first_hit_status status{gwn_ray_first_hit_status::k_miss};
-    [[nodiscard]] constexpr bool hit() const noexcept { return status == gwn_ray_first_hit_status::k_hit; }
-
-private:
-    std::uint8_t storage_padding_[3]{};
-};
 static_assert(sizeof(gwn_ray_first_hit_result<>) == sizeof(::gwn_ray_first_hit_result));
 
 template <gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, class Real, class Index = std::uint32_t>
@@ -345,6 +447,7 @@ template <gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, c
     Real const t_min = Real(0), Real const t_max = Real(__builtin_huge_valf()),
     hipStream_t stream = gwn_default_stream()
 ) noexcept {
+    static_assert(detail::gwn_float_only_v<Real>, "libgwn_hip provides this batch for Real = float");
     std::size_t const n = ray_origin_x.size();
     if (ray_origin_y.size() != n || ray_origin_z.size() != n || ray_dir_x.size() != n || ray_dir_y.size() != n ||
         ray_dir_z.size() != n || output.size() != n)
@@ -375,6 +478,7 @@ template <gwn_query_batch_config Config = gwn_query_batch_config{}, int Width, c
     gwn_device_span<Real const> query_y, gwn_device_span<Real const> query_z, gwn_device_span<Real> output,
     hipStream_t stream = gwn_default_stream()
 ) noexcept {
+    static_assert(detail::gwn_float_only_v<Real>, "libgwn_hip provides this batch for Real = float");
     if (query_y.size() != query_x.size() || query_z.size() != query_x.size())
         return gwn_status(gwn_error::invalid_argument, "Query SoA spans require identical lengths.");
     if (output.size() != query_x.size())
@@ -394,6 +498,7 @@ template <gwn_query_batch_config Config = gwn_query_batch_config{}, class Real, 
     gwn_device_span<Real> output_x, gwn_device_span<Real> output_y, gwn_device_span<Real> output_z,
     hipStream_t stream = gwn_default_stream()
 ) noexcept {
+    static_assert(detail::gwn_float_only_v<Real>, "libgwn_hip provides this batch for Real = float");
     std::size_t const n = query_x.size();
     if (query_y.size() != n || query_z.size() != n)
         return gwn_status(gwn_error::invalid_argument, "Query SoA spans require identical lengths.");

@@ -112,6 +112,48 @@ typedef struct gwn_ray_first_hit_result {
 int gwn_abi_version(void);
 const char *gwn_last_error_message(void);
 
+/* ---- accessors: non-owning snapshots of an object's device storage, replacing the reference's
+ *      trivially copyable gwn_*_accessor structs (include/gwn/gwn_geometry.cuh:19-58,
+ *      gwn_bvh.cuh:209-258, 260-329, gwn_boundary.cuh:28-54).  Any upload, build, refit, clear
+ *      or destroy of the object invalidates them. ---- */
+typedef struct gwn_geometry_accessor {
+    const float *x, *y, *z;           /* vertex_count each */
+    const uint32_t *i0, *i1, *i2;     /* triangle_count each */
+    uint64_t vertex_count;
+    uint64_t triangle_count;
+} gwn_geometry_accessor;
+
+typedef struct gwn_bvh_accessor {
+    const void *nodes;                /* node_count gwn_bvh_node<4, float> (128 B each) */
+    uint64_t node_count;
+    const uint32_t *primitive_indices; /* leaf order -> original triangle id */
+    const float *triangles;           /* leaf-ordered records, 9 floats each: v0, v1, v2 */
+    uint64_t triangle_count;
+    float root_bounds[6];
+    uint64_t root_reference;          /* gwn_bvh_child reference bits of the root */
+    uint32_t internal_stack_bound;
+    uint32_t packed_stack_bound;
+    uint64_t revision;                /* 0: not queryable */
+} gwn_bvh_accessor;
+
+typedef struct gwn_moment_accessor {
+    const void *records;              /* node_count traversal records of record_bytes each */
+    uint64_t node_count;
+    int32_t order;
+    uint32_t record_bytes;            /* 128, 256 or 384 for orders 0, 1, 2 */
+    int32_t width;                    /* the width of the BVH the records were formed from */
+    uint64_t bvh_revision;            /* the BVH revision the records were formed from */
+} gwn_moment_accessor;
+
+typedef struct gwn_boundary_accessor {
+    const uint32_t *start, *end;      /* edge_count directed boundary rows */
+    const uint64_t *multiplicity;
+    uint64_t edge_count;
+    uint64_t mesh_vertex_count;
+    uint64_t mesh_triangle_count;
+    int32_t is_built;
+} gwn_boundary_accessor;
+
 /* ---- geometry: replaces gwn_geometry_object + gwn_upload_geometry / gwn_update_geometry
  *      (include/gwn/gwn_geometry.cuh:19-149, detail/gwn_geometry_impl.cuh:102-219) ---- */
 int gwn_geometry_create(gwn_geometry_t *out);
@@ -121,7 +163,9 @@ int gwn_upload_geometry_f32(
     gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count,
     const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
     hipStream_t stream);
-/* Replace positions (same vertex count) from host memory. */
+/* Replace positions (same vertex count) from host memory.  Enqueued on `stream` without a host
+ * synchronization (include/gwn/detail/gwn_geometry_impl.cuh:172-178): pageable sources are staged
+ * before the call returns, pinned sources must stay valid until the stream reaches the copies. */
 int gwn_update_geometry_f32(
     gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count,
     hipStream_t stream);
@@ -131,6 +175,58 @@ int gwn_update_geometry_device_f32(
     hipStream_t stream);
 int gwn_geometry_counts(gwn_geometry_t geometry, uint64_t *vertex_count, uint64_t *triangle_count);
 
+/* ---- Real = double and Index = uint64: gwn_geometry_object<Real, Index> for the reference's other
+ *      template arguments (include/gwn/gwn_geometry.cuh:19-58, gwn_bvh_build.cuh:34-39).
+ *      uint64 indices are validated like uint32 ones and stored as uint32, so vertex and triangle
+ *      counts stay below 2^32.  A Real = double geometry keeps its positions in double (a float
+ *      rounding of them steers the build's Morton codes only) and is built, refit, moment-refit
+ *      and queried through the _f64 entry points below; the _f32 entry points reject it, and the
+ *      _f64 ones reject Real = float objects.  Device accessors are provided for Real = float. ---- */
+int gwn_upload_geometry_f32_u64(
+    gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count,
+    const uint64_t *i0, const uint64_t *i1, const uint64_t *i2, uint64_t triangle_count,
+    hipStream_t stream);
+int gwn_upload_geometry_f64(
+    gwn_geometry_t geometry, const double *x, const double *y, const double *z, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
+    hipStream_t stream);
+int gwn_upload_geometry_f64_u64(
+    gwn_geometry_t geometry, const double *x, const double *y, const double *z, uint64_t vertex_count,
+    const uint64_t *i0, const uint64_t *i1, const uint64_t *i2, uint64_t triangle_count,
+    hipStream_t stream);
+/* As gwn_update_geometry_f32 / gwn_update_geometry_device_f32, for a Real = double geometry. */
+int gwn_update_geometry_f64(
+    gwn_geometry_t geometry, const double *x, const double *y, const double *z, uint64_t vertex_count,
+    hipStream_t stream);
+int gwn_update_geometry_device_f64(
+    gwn_geometry_t geometry, const double *x, const double *y, const double *z, uint64_t vertex_count,
+    hipStream_t stream);
+
+/* ---- object stream binding and release (include/gwn/gwn_utils.cuh:59-80 gwn_stream_mixin;
+ *      gwn_geometry.cuh:117-125, gwn_bvh.cuh:356-415, gwn_boundary.cuh:84-110 clear()).  An object
+ *      is bound to the stream of the last operation that replaced or wrote its storage; its
+ *      storage is released on that bound stream (by clear, destroy, or a replacement made on
+ *      another stream), so work already queued there on the old storage finishes first.
+ *      set_stream rebinds without synchronizing either stream; clear releases on the current
+ *      binding and keeps it.  get_accessor returns a non-owning snapshot of the device storage.
+ *      The update and refit calls above rebind to their stream, as the reference's do. ---- */
+int gwn_geometry_stream(gwn_geometry_t geometry, hipStream_t *stream);
+int gwn_geometry_set_stream(gwn_geometry_t geometry, hipStream_t stream);
+int gwn_geometry_clear(gwn_geometry_t geometry);
+int gwn_geometry_get_accessor(gwn_geometry_t geometry, gwn_geometry_accessor *accessor);
+int gwn_bvh_stream(gwn_bvh_t bvh, hipStream_t *stream);
+int gwn_bvh_set_stream(gwn_bvh_t bvh, hipStream_t stream);
+int gwn_bvh_clear(gwn_bvh_t bvh);
+int gwn_bvh_get_accessor(gwn_bvh_t bvh, gwn_bvh_accessor *accessor);
+int gwn_moment_stream(gwn_moment_t moment, hipStream_t *stream);
+int gwn_moment_set_stream(gwn_moment_t moment, hipStream_t stream);
+int gwn_moment_clear(gwn_moment_t moment);
+int gwn_moment_get_accessor(gwn_moment_t moment, gwn_moment_accessor *accessor);
+int gwn_boundary_stream(gwn_boundary_t boundary, hipStream_t *stream);
+int gwn_boundary_set_stream(gwn_boundary_t boundary, hipStream_t stream);
+int gwn_boundary_clear(gwn_boundary_t boundary);
+int gwn_boundary_get_accessor(gwn_boundary_t boundary, gwn_boundary_accessor *accessor);
+
 /* ---- BVH: replaces gwn_bvh_object<4, float, uint32_t> and gwn_build_bvh
  *      (include/gwn/gwn_bvh.cuh:331-380, include/gwn/gwn_bvh_build.cuh:29-39) ---- */
 int gwn_bvh_create(gwn_bvh_t *out);
@@ -139,6 +235,14 @@ int gwn_bvh_destroy(gwn_bvh_t bvh);
 int gwn_build_bvh4_f32(
     gwn_geometry_t geometry, gwn_bvh_t bvh, int32_t method, uint32_t hploc_search_radius,
     hipStream_t stream);
+/* Build of width 2, 3 or 4 (gwn_bvh_object<Width, float, uint32_t>, include/gwn/gwn_bvh.cuh:331-380;
+ * the collapse expands each node up to Width children, detail/gwn_bvh_build_collapse.cuh:163-296).
+ * Narrower trees are stored four-wide with their extra slots empty, so every refit and query
+ * below accepts them; gwn_bvh_download and gwn_moment_download return the reference's
+ * Width-wide layouts.  gwn_build_bvh4_f32 is width 4. */
+int gwn_build_bvh_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, int32_t width, int32_t method, uint32_t hploc_search_radius,
+    hipStream_t stream);
 /* Bounds refit from updated positions (same triangle indexing): replaces gwn_refit_bvh
  * (include/gwn/gwn_bvh_refit.cuh:25-29, detail/gwn_bvh_refit_impl.cuh:194-320).  Rewrites the
  * leaf-ordered triangle records and every child box in place, keeps topology, primitive order
@@ -146,12 +250,25 @@ int gwn_build_bvh4_f32(
  * after the in-place writes began leaves the BVH unqueryable (revision 0); it is then bound
  * to `stream` like a successful refit. */
 int gwn_refit_bvh_f32(gwn_geometry_t geometry, gwn_bvh_t bvh, hipStream_t stream);
+/* gwn_bvh_object<Width, double, Index> build and refit from a Real = double geometry: the tree is
+ * built from the float-rounded positions, then every leaf record and child box is recomputed in
+ * double from the double positions, one launch per breadth-first level, deepest first
+ * (detail/gwn_bvh_refit_impl.cuh:23-352 with Real = double).  Boxes are exact unions of the
+ * double triangles; the tree shape alone depends on the rounding. */
+int gwn_build_bvh_f64(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, int32_t width, int32_t method, uint32_t hploc_search_radius,
+    hipStream_t stream);
+int gwn_refit_bvh_f64(gwn_geometry_t geometry, gwn_bvh_t bvh, hipStream_t stream);
 int gwn_bvh_get_info(gwn_bvh_t bvh, gwn_bvh_info *info);
 /* Copies the canonical arrays to host memory (any pointer may be NULL) and synchronizes:
- * nodes in the reference's gwn_bvh_node<4, float> layout (128 B each), primitive order,
+ * nodes in the reference's gwn_bvh_node<Width, float> layout (32 B per child), primitive order,
  * and leaf-ordered triangle records as 9 floats (v0, v1, v2). */
 int gwn_bvh_download(
     gwn_bvh_t bvh, void *nodes, uint32_t *primitive_indices, float *triangles, hipStream_t stream);
+/* Real = double: gwn_bvh_node<Width, double> (56 B per child: six double bounds, then the
+ * 64-bit reference) and records as 9 doubles. */
+int gwn_bvh_download_f64(
+    gwn_bvh_t bvh, void *nodes, uint32_t *primitive_indices, double *triangles, hipStream_t stream);
 
 /* ---- boundary chain: replaces gwn_boundary_chain_object<uint32_t> and gwn_build_boundary_chain
  *      (include/gwn/gwn_boundary.cuh:28-125, detail/gwn_boundary_impl.cuh:21-413).  Directed rows
@@ -172,10 +289,19 @@ int gwn_boundary_download(
 int gwn_moment_create(gwn_moment_t *out, int32_t order);
 int gwn_moment_destroy(gwn_moment_t moment);
 int gwn_refit_bvh_moment_f32(gwn_bvh_t bvh, gwn_moment_t moment, hipStream_t stream);
-/* Copies coefficients in the reference's field-SoA gwn_bvh_moment_node<4, Order, float>
- * layout (node-major, then field, then child slot) and synchronizes. */
+/* Copies coefficients in the reference's field-SoA gwn_bvh_moment_node<Width, Order, float>
+ * layout (node-major, then field, then child slot; Width of the BVH they were formed from) and
+ * synchronizes. */
 int gwn_moment_download(gwn_moment_t moment, float *fields, hipStream_t stream);
 int gwn_moment_order(gwn_moment_t moment, int32_t *order);
+/* Node count and the BVH revision the moments were formed from, for either Real (the object
+ * state behind gwn_bvh_moment_object::is_valid_for, include/gwn/gwn_bvh.cuh:286-293). */
+int gwn_moment_info(gwn_moment_t moment, uint64_t *node_count, uint64_t *bvh_revision);
+/* gwn_bvh_moment_object<Width, Order, double> from a Real = double BVH: the same per-triangle
+ * moments, sums in child-slot order and recentring as the float refit, in double, one launch
+ * per level (detail/gwn_bvh_refit_moment_impl.cuh:17-279). */
+int gwn_refit_bvh_moment_f64(gwn_bvh_t bvh, gwn_moment_t moment, hipStream_t stream);
+int gwn_moment_download_f64(gwn_moment_t moment, double *fields, hipStream_t stream);
 
 /* ---- winding-number batches: replace gwn_compute_winding_number_exact_batch and
  *      gwn_compute_winding_number_taylor_batch<Order> (include/gwn/gwn_query.cuh:308-445) ---- */
@@ -185,6 +311,16 @@ int gwn_compute_winding_number_exact_batch_f32(
 int gwn_compute_winding_number_taylor_batch_f32(
     gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz,
     float *out, uint64_t count, float accuracy_scale, const gwn_query_config *config,
+    hipStream_t stream);
+/* Real = double (the same batches instantiated with double, gwn_query.cuh:308-445): the exact sum
+ * of the double Van Oosterom-Strackee solid angles, and the Taylor traversal with the acceptance
+ * test and far-field terms in double.  One query per lane, in input order. */
+int gwn_compute_winding_number_exact_batch_f64(
+    gwn_bvh_t bvh, const double *qx, const double *qy, const double *qz, double *out, uint64_t count,
+    const gwn_query_config *config, hipStream_t stream);
+int gwn_compute_winding_number_taylor_batch_f64(
+    gwn_bvh_t bvh, gwn_moment_t moment, const double *qx, const double *qy, const double *qz,
+    double *out, uint64_t count, double accuracy_scale, const gwn_query_config *config,
     hipStream_t stream);
 
 /* Precision-split and FP64 forms (no reference counterpart: the reference computes in the

@@ -28,6 +28,7 @@ int gwn_oracle_moment_field_count(int order)
     }
 }
 
+#define RECT float
 #define REAL float
 #define SFX(name) name##f32
 #include "gwn_oracle_real.inc"
@@ -39,6 +40,15 @@ int gwn_oracle_moment_field_count(int order)
 #include "gwn_oracle_real.inc"
 #undef REAL
 #undef SFX
+#undef RECT
+
+#define RECT double
+#define REAL double
+#define SFX(name) name##d64
+#include "gwn_oracle_real.inc"
+#undef REAL
+#undef SFX
+#undef RECT
 
 /* ---------------------------------------------------------------------------
  * Morton codes: gwn_bvh_build_common.cuh:35-88
@@ -421,9 +431,20 @@ int gwn_oracle_build_bvh4_ex_f32(
     const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
     int method, uint32_t hploc_radius, gwn_oracle_bvh_f32 *out)
 {
+    return gwn_oracle_build_bvh_ex_f32(vx, vy, vz, vertex_count, i0, i1, i2, triangle_count, method,
+                                       hploc_radius, 4, out);
+}
+
+int gwn_oracle_build_bvh_ex_f32(
+    const float *vx, const float *vy, const float *vz, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
+    int method, uint32_t hploc_radius, int width, gwn_oracle_bvh_f32 *out)
+{
     uint64_t T = triangle_count;
     out->binary_root = 0;
     if (method != GWN_ORACLE_BUILD_LBVH && method != GWN_ORACLE_BUILD_HPLOC)
+        return GWN_ORACLE_INVALID_ARGUMENT;
+    if (width < 2 || width > 4)
         return GWN_ORACLE_INVALID_ARGUMENT;
     if (method == GWN_ORACLE_BUILD_HPLOC && (hploc_radius == 0 || hploc_radius > 8))
         return GWN_ORACLE_INVALID_ARGUMENT;
@@ -587,7 +608,7 @@ int gwn_oracle_build_bvh4_ex_f32(
         int cnt = 2;
         cidx[0] = root.left_index; ckind[0] = root.left_kind;
         cidx[1] = root.right_index; ckind[1] = root.right_kind;
-        while (cnt < 4) {
+        while (cnt < width) {  /* collapse.cuh:166-211: expand up to Width candidates */
             int best = -1;
             float best_area = -INFINITY;
             uint32_t best_internal = GWN_ORACLE_INVALID_INDEX;
@@ -756,14 +777,23 @@ int gwn_oracle_refit_moment_f32(const gwn_oracle_node4_f32 *nodes, uint64_t node
                                 const float *triangles, uint64_t triangle_count, int order,
                                 float *out)
 {
-    return refit_moment_impl_f32(nodes, node_count, triangles, triangle_count, order, out);
+    return refit_moment_impl_f32(nodes, node_count, triangles, triangle_count, order, out, NULL);
 }
 
 int gwn_oracle_refit_moment_f64(const gwn_oracle_node4_f32 *nodes, uint64_t node_count,
                                 const float *triangles, uint64_t triangle_count, int order,
                                 double *out)
 {
-    return refit_moment_impl_f64(nodes, node_count, triangles, triangle_count, order, out);
+    return refit_moment_impl_f64(nodes, node_count, triangles, triangle_count, order, out, NULL);
+}
+
+/* Real = double moments: double records (leaf order, 9 doubles) and double child boxes
+ * (node, slot, 6 = min xyz, max xyz), references from `nodes`. */
+int gwn_oracle_refit_moment_d64(const gwn_oracle_node4_f32 *nodes, uint64_t node_count,
+                                const double *triangles, uint64_t triangle_count, const double *child_bounds,
+                                int order, double *out)
+{
+    return refit_moment_impl_d64(nodes, node_count, triangles, triangle_count, order, out, child_bounds);
 }
 
 void gwn_oracle_triangle_payload_f64(const float *record, int order, double *out)
@@ -1231,6 +1261,179 @@ void gwn_oracle_winding_taylor_f32(
                       gwn_oracle_moment_field_count(order), stack_capacity, accuracy_scale,
                       qx, qy, qz, out, NULL, NULL, NULL, counters, near_fp64};
     parallel_ranges(taylor_range, &ctx, query_count, threads);
+}
+
+/* ---------------------------------------------------------------------------
+ * Real = double: the exact sum over a mesh with double vertices, and the Taylor traversal
+ * (gwn_query_winding_impl.cuh:102-374 instantiated with double) over double records and double
+ * moments on a tree whose references come from the float build.  Records are (v0, v1, v2) as 9
+ * doubles; the corners are v0, v0 + (v1 - v0), v0 + (v2 - v0) as the reference rebuilds them.
+ * ------------------------------------------------------------------------- */
+
+typedef struct {
+    const double *vx, *vy, *vz;
+    const uint32_t *i0, *i1, *i2;
+    uint64_t nt;
+    const double *qx, *qy, *qz;
+    double *out;
+} exact_mesh_d_ctx;
+
+static void exact_mesh_d_range(void *vctx, uint64_t begin, uint64_t end)
+{
+    exact_mesh_d_ctx *c = (exact_mesh_d_ctx *)vctx;
+    const double k_pi = 3.141592653589793238462643383279502884;
+    for (uint64_t q = begin; q < end; ++q) {
+        double qq[3] = {c->qx[q], c->qy[q], c->qz[q]};
+        double sum = 0.0;
+        for (uint64_t t = 0; t < c->nt; ++t) {
+            uint32_t ia = c->i0[t], ib = c->i1[t], ic = c->i2[t];
+            double a[3] = {c->vx[ia], c->vy[ia], c->vz[ia]};
+            double b[3] = {c->vx[ib], c->vy[ib], c->vz[ib]};
+            double cc[3] = {c->vx[ic], c->vy[ic], c->vz[ic]};
+            sum += solid_angle_f64(a, b, cc, qq);
+        }
+        c->out[q] = sum / (4.0 * k_pi);
+    }
+}
+
+void gwn_oracle_winding_exact_mesh_d64(const double *vx, const double *vy, const double *vz,
+                                       const uint32_t *i0, const uint32_t *i1, const uint32_t *i2,
+                                       uint64_t triangle_count, const double *qx, const double *qy,
+                                       const double *qz, uint64_t query_count, double *out, int threads)
+{
+    exact_mesh_d_ctx ctx = {vx, vy, vz, i0, i1, i2, triangle_count, qx, qy, qz, out};
+    parallel_ranges(exact_mesh_d_range, &ctx, query_count, threads);
+}
+
+static void record_vertices_d(const double *r, double *a, double *b, double *c)
+{
+    for (int k = 0; k < 3; ++k) {
+        double e1 = r[3 + k] - r[k], e2 = r[6 + k] - r[k];
+        a[k] = r[k];
+        b[k] = r[k] + e1;
+        c[k] = r[k] + e2;
+    }
+}
+
+typedef struct {
+    const gwn_oracle_node4_f32 *nodes;
+    uint64_t node_count;
+    uint64_t root_reference;
+    const double *tri;
+    uint64_t nt;
+    const double *mom;
+    int order, F, cap;
+    double beta;
+    const double *qx, *qy, *qz;
+    double *out;
+} taylor_d_ctx;
+
+static double taylor_leaf_d(const taylor_d_ctx *c, uint64_t ref, const double *q)
+{
+    uint64_t off = ref & GWN_ORACLE_OFFSET_MASK;
+    uint32_t n = (uint32_t)(ref >> 48);
+    double sum = 0.0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (off + k >= c->nt)
+            continue;
+        double a[3], b[3], cc[3];
+        record_vertices_d(c->tri + 9 * (off + k), a, b, cc);
+        sum += solid_angle_f64(a, b, cc, q);
+    }
+    return sum;
+}
+
+static double taylor_one_d(const taylor_d_ctx *c, double qx, double qy, double qz)
+{
+    const double k_pi = 3.141592653589793238462643383279502884;
+    double q[3] = {qx, qy, qz};
+    double omega = 0.0;
+    double beta2 = c->beta * c->beta;
+    uint32_t stack[1024];
+    int sp = 0;
+    if ((c->root_reference & GWN_ORACLE_VALID_BIT) && (c->root_reference >> 48) != 0)
+        return taylor_leaf_d(c, c->root_reference, q) / (4.0 * k_pi);
+    uint32_t node = (uint32_t)(c->root_reference & GWN_ORACLE_OFFSET_MASK);
+    for (;;) {
+        if (node >= c->node_count)
+            break;
+        const gwn_oracle_node4_f32 *nd = &c->nodes[node];
+        const double *t = c->mom + (size_t)node * 4 * c->F;
+        uint32_t next = GWN_ORACLE_INVALID_INDEX;
+        for (int s = 0; s < 4; ++s) {
+            uint64_t ref = nd->children[s].reference;
+            if (!(ref & GWN_ORACLE_VALID_BIT))
+                continue;
+#define T(f) t[(f) * 4 + s]
+            double qrx = qx - T(1), qry = qy - T(2), qrz = qz - T(3);
+            double ql2 = qrx * qrx + qry * qry + qrz * qrz;
+            int descend = !(ql2 > 0.0) || ql2 <= T(0) * beta2;
+            if (!descend) {
+                double m1 = 1.0 / sqrt(ql2);
+                double m2 = m1 * m1, m3 = m2 * m1;
+                double qnx = qrx * m1, qny = qry * m1, qnz = qrz * m1;
+                double w = -m2 * (qnx * T(4) + qny * T(5) + qnz * T(6));
+                if (c->order >= 1) {
+                    double Txx = T(7), Tyy = T(8), Tzz = T(9);
+                    double S = qnx * qnx * Txx + qny * qny * Tyy + qnz * qnz * Tzz + qnx * qny * T(10) +
+                               qnx * qnz * T(12) + qny * qnz * T(11);
+                    w += m3 * ((Txx + Tyy + Tzz) - 3.0 * S);
+                }
+                if (c->order >= 2) {
+                    double m4 = m2 * m2;
+                    double qnx2 = qnx * qnx, qny2 = qny * qny, qnz2 = qnz * qnz;
+                    double xxx = T(13), yyy = T(14), zzz = T(15), spn = T(16);
+                    double Axy = T(17), Axz = T(18), Ayz = T(19), Ayx = T(20), Azx = T(21), Azy = T(22);
+                    double t1x = qny * Axy + qnz * Axz, t1y = qnz * Ayz + qnx * Ayx, t1z = qnx * Azx + qny * Azy;
+                    w += m4 * (1.5 * (qnx * (3.0 * xxx + Ayx + Azx) + qny * (3.0 * yyy + Azy + Axy) +
+                                      qnz * (3.0 * zzz + Axz + Ayz)) -
+                               7.5 * (qnx2 * qnx * xxx + qny2 * qny * yyy + qnz2 * qnz * zzz +
+                                      qnx * qny * qnz * spn + qnx2 * t1x + qny2 * t1y + qnz2 * t1z));
+                }
+#undef T
+                if (isfinite(w)) {
+                    omega += w;
+                    continue;
+                }
+            }
+            if ((ref >> 48) == 0) {
+                if (next != GWN_ORACLE_INVALID_INDEX) {
+                    if (sp >= c->cap || sp >= 1024)
+                        return NAN;
+                    stack[sp++] = next;
+                }
+                next = (uint32_t)(ref & GWN_ORACLE_OFFSET_MASK);
+                continue;
+            }
+            omega += taylor_leaf_d(c, ref, q);
+        }
+        if (next != GWN_ORACLE_INVALID_INDEX) {
+            node = next;
+            continue;
+        }
+        if (sp == 0)
+            break;
+        node = stack[--sp];
+    }
+    return omega / (4.0 * k_pi);
+}
+
+static void taylor_d_range(void *vctx, uint64_t begin, uint64_t end)
+{
+    taylor_d_ctx *c = (taylor_d_ctx *)vctx;
+    for (uint64_t q = begin; q < end; ++q)
+        c->out[q] = taylor_one_d(c, c->qx[q], c->qy[q], c->qz[q]);
+}
+
+void gwn_oracle_winding_taylor_d64(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, uint64_t root_reference,
+    const double *triangles, uint64_t triangle_count, const double *moments, int order,
+    double accuracy_scale, int stack_capacity, const double *qx, const double *qy, const double *qz,
+    uint64_t query_count, double *out, int threads)
+{
+    taylor_d_ctx ctx = {nodes, node_count, root_reference, triangles, triangle_count, moments, order,
+                        gwn_oracle_moment_field_count(order), stack_capacity, accuracy_scale, qx, qy, qz, out};
+    parallel_ranges(taylor_d_range, &ctx, query_count, threads);
 }
 
 void gwn_oracle_winding_gradient_taylor_f32(

@@ -116,6 +116,16 @@ int gwn_oracle_build_bvh4_ex_f32(
     int method, uint32_t hploc_radius, gwn_oracle_bvh_f32 *out);
 
 /*
+ * The same for a tree of width 2, 3 or 4 (gwn_bvh_build_collapse.cuh:163-296 with Width): each node
+ * expands to at most `width` children; nodes are stored four-wide with the slots past the width
+ * empty.  gwn_oracle_build_bvh4_ex_f32 is width 4.
+ */
+int gwn_oracle_build_bvh_ex_f32(
+    const float *vx, const float *vy, const float *vz, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count,
+    int method, uint32_t hploc_radius, int width, gwn_oracle_bvh_f32 *out);
+
+/*
  * Bounds refit in place: triangle records from the geometry, child boxes folded in slot order,
  * root bounds (gwn_bvh_refit_impl.cuh:23-192).  Topology, primitive order and stack bounds are
  * untouched.
@@ -175,6 +185,22 @@ float gwn_oracle_solid_angle_f32(const float *a, const float *b, const float *c,
  * farther one in FP32 (unnormalized form); every term, far-field expansions included, is
  * accumulated in FP64.
  */
+/* Real = double (no float rounding of the geometry): the exact FP64 sum over a mesh with double
+ * vertices, and the double Taylor traversal over double records (9 doubles each, leaf order) and
+ * double moments in the field-SoA node layout [node][field][4], on a tree whose child references
+ * come from `nodes` (bounds unused). */
+int gwn_oracle_refit_moment_d64(const gwn_oracle_node4_f32 *nodes, uint64_t node_count,
+                                const double *triangles, uint64_t triangle_count, const double *child_bounds,
+                                int order, double *out);
+void gwn_oracle_winding_exact_mesh_d64(const double *vx, const double *vy, const double *vz,
+                                       const uint32_t *i0, const uint32_t *i1, const uint32_t *i2,
+                                       uint64_t triangle_count, const double *qx, const double *qy,
+                                       const double *qz, uint64_t query_count, double *out, int threads);
+void gwn_oracle_winding_taylor_d64(
+    const gwn_oracle_node4_f32 *nodes, uint64_t node_count, uint64_t root_reference,
+    const double *triangles, uint64_t triangle_count, const double *moments, int order,
+    double accuracy_scale, int stack_capacity, const double *qx, const double *qy, const double *qz,
+    uint64_t query_count, double *out, int threads);
 void gwn_oracle_winding_taylor_f32(
     const gwn_oracle_node4_f32 *nodes, uint64_t node_count, gwn_oracle_child_f32 root,
     const float *triangles, uint64_t triangle_count, const float *moments, int order,

@@ -85,6 +85,9 @@ def lib():
         L.gwn_oracle_build_bvh4_ex_f32.restype = i32
         L.gwn_oracle_build_bvh4_ex_f32.argtypes = [vp, vp, vp, u64, vp, vp, vp, u64, i32, ctypes.c_uint32,
                                                    ctypes.POINTER(_Bvh)]
+        L.gwn_oracle_build_bvh_ex_f32.restype = i32
+        L.gwn_oracle_build_bvh_ex_f32.argtypes = [vp, vp, vp, u64, vp, vp, vp, u64, i32, ctypes.c_uint32, i32,
+                                                  ctypes.POINTER(_Bvh)]
         L.gwn_oracle_refit_bvh_f32.restype = i32
         L.gwn_oracle_refit_bvh_f32.argtypes = [vp, u64, ctypes.POINTER(_Child), vp, vp, u64, vp, vp, vp, u64, vp, vp, vp]
         L.gwn_oracle_moment_field_count.restype = i32
@@ -120,6 +123,12 @@ def lib():
         L.gwn_oracle_point_triangle_distance2_f32.argtypes = [vp, vp, vp, vp]
         L.gwn_oracle_winding_gradient_exact_mesh_f64.argtypes = [
             vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, u64, vp, vp, vp, i32,
+        ]
+        L.gwn_oracle_winding_exact_mesh_d64.argtypes = [vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, u64, vp, i32]
+        L.gwn_oracle_refit_moment_d64.restype = i32
+        L.gwn_oracle_refit_moment_d64.argtypes = [vp, u64, vp, u64, vp, i32, vp]
+        L.gwn_oracle_winding_taylor_d64.argtypes = [
+            vp, u64, u64, vp, u64, vp, i32, ctypes.c_double, i32, vp, vp, vp, u64, vp, i32,
         ]
         _lib = L
     return _lib
@@ -176,8 +185,9 @@ class OracleBvh:
 BUILD_HPLOC, BUILD_LBVH = 0, 1
 
 
-def build_bvh4(mesh, method: str = "lbvh", radius: int = 8) -> OracleBvh:
-    """BVH4 of `mesh` with the LBVH or H-PLOC binary builder (H-PLOC search radius 1-8)."""
+def build_bvh4(mesh, method: str = "lbvh", radius: int = 8, width: int = 4) -> OracleBvh:
+    """Wide BVH (width 2, 3 or 4, stored four-wide) of `mesh` with the LBVH or H-PLOC binary
+    builder (H-PLOC search radius 1-8)."""
     vx, vy, vz = _c(mesh.vx, np.float32), _c(mesh.vy, np.float32), _c(mesh.vz, np.float32)
     i0, i1, i2 = _c(mesh.i0, np.uint32), _c(mesh.i1, np.uint32), _c(mesh.i2, np.uint32)
     T = i0.shape[0]
@@ -193,8 +203,8 @@ def build_bvh4(mesh, method: str = "lbvh", radius: int = 8) -> OracleBvh:
     out.sorted_morton, out.binary_nodes, out.binary_bounds = _ptr(morton), _ptr(bnodes), _ptr(bbounds)
     out.binary_first = _ptr(bfirst)
     m = {"lbvh": BUILD_LBVH, "hploc": BUILD_HPLOC}[method]
-    st = lib().gwn_oracle_build_bvh4_ex_f32(
-        _ptr(vx), _ptr(vy), _ptr(vz), vx.shape[0], _ptr(i0), _ptr(i1), _ptr(i2), T, m, int(radius),
+    st = lib().gwn_oracle_build_bvh_ex_f32(
+        _ptr(vx), _ptr(vy), _ptr(vz), vx.shape[0], _ptr(i0), _ptr(i1), _ptr(i2), T, m, int(radius), int(width),
         ctypes.byref(out)
     )
     if st != 0:
@@ -276,6 +286,88 @@ def winding_exact_mesh_f64(mesh, qx, qy, qz, threads: int | None = None) -> np.n
     lib().gwn_oracle_winding_exact_mesh_f64(
         _ptr(vx), _ptr(vy), _ptr(vz), _ptr(i0), _ptr(i1), _ptr(i2), i0.shape[0],
         _ptr(qx), _ptr(qy), _ptr(qz), qx.shape[0], _ptr(out), threads or default_threads(),
+    )
+    return out
+
+
+def winding_exact_mesh_d64(vertices, i0, i1, i2, qx, qy, qz, threads: int | None = None) -> np.ndarray:
+    """Exact FP64 winding of a mesh with double vertices ``vertices`` (3, V): no float rounding
+    anywhere (tests/reference_cpu.cuh:51-82 with Real = double)."""
+    v = _c(np.asarray(vertices, dtype=np.float64).reshape(3, -1), np.float64)
+    vx, vy, vz = _c(v[0], np.float64), _c(v[1], np.float64), _c(v[2], np.float64)
+    i0, i1, i2 = _c(i0, np.uint32), _c(i1, np.uint32), _c(i2, np.uint32)
+    qx, qy, qz = _c(qx, np.float64), _c(qy, np.float64), _c(qz, np.float64)
+    out = np.empty(qx.shape[0], dtype=np.float64)
+    lib().gwn_oracle_winding_exact_mesh_d64(
+        _ptr(vx), _ptr(vy), _ptr(vz), _ptr(i0), _ptr(i1), _ptr(i2), i0.shape[0],
+        _ptr(qx), _ptr(qy), _ptr(qz), qx.shape[0], _ptr(out), threads or default_threads(),
+    )
+    return out
+
+
+def double_tree(bvh: OracleBvh, vertices, i0, i1, i2):
+    """Real = double records and boxes on the float-built tree ``bvh``: records (T, 9) of the double
+    vertices ``vertices`` (3, V) in ``bvh``'s leaf order, every child's double box (N, 4, 6; min
+    xyz, max xyz; zeros for empty slots), and the root box (6,).  Boxes are exact unions of the
+    double triangles (gwn_bvh_refit_impl.cuh:23-352 with Real = double), folded in slot order."""
+    v = np.asarray(vertices, dtype=np.float64).reshape(3, -1)
+    idx = np.stack([np.asarray(i, dtype=np.int64) for i in (i0, i1, i2)])[:, bvh.primitive_indices]
+    rec = np.concatenate([v[:, idx[0]].T, v[:, idx[1]].T, v[:, idx[2]].T], axis=1)
+    corners = rec.reshape(-1, 3, 3)
+    tri_box = np.concatenate([corners.min(axis=1), corners.max(axis=1)], axis=1)
+    N = bvh.node_count
+    child = np.zeros((N, 4, 6), dtype=np.float64)
+    node_box = np.zeros((N, 6), dtype=np.float64)
+
+    def leaf_box(ref):
+        off, cnt = ref & ((1 << 47) - 1), ref >> 48
+        b = tri_box[off:off + cnt]
+        return np.concatenate([b[:, :3].min(axis=0), b[:, 3:].max(axis=0)])
+
+    for n in range(N - 1, -1, -1):
+        acc = None
+        for s in range(4):
+            ref = int(bvh.nodes[n]["children"][s]["reference"])
+            if not ref >> 47 & 1:
+                continue
+            box = node_box[ref & ((1 << 47) - 1)] if ref >> 48 == 0 else leaf_box(ref)
+            child[n, s] = box
+            acc = box.copy() if acc is None else np.concatenate([np.minimum(acc[:3], box[:3]),
+                                                                 np.maximum(acc[3:], box[3:])])
+        node_box[n] = acc
+    root = int(bvh.root_reference)
+    root_box = node_box[0] if root >> 48 == 0 else leaf_box(root)
+    return np.ascontiguousarray(rec), child, root_box
+
+
+def refit_moment_d64(bvh: OracleBvh, triangles_d, child_bounds_d, order: int) -> np.ndarray:
+    """Real = double moments [node, field, 4] from double records and double child boxes."""
+    F = field_count(order)
+    out = np.zeros((bvh.node_count, F, 4), dtype=np.float64)
+    nodes = _c(bvh.nodes, NODE_DTYPE)
+    tris = _c(triangles_d, np.float64)
+    bounds = _c(child_bounds_d, np.float64)
+    st = lib().gwn_oracle_refit_moment_d64(_ptr(nodes), bvh.node_count, _ptr(tris), tris.shape[0], _ptr(bounds),
+                                           order, _ptr(out))
+    if st != 0:
+        raise RuntimeError(f"oracle double moment refit failed with status {st}")
+    return out
+
+
+def winding_taylor_d64(bvh: OracleBvh, triangles_d, moments_d, order: int, qx, qy, qz,
+                       accuracy_scale: float = 2.0, stack_capacity: int = 64,
+                       threads: int | None = None) -> np.ndarray:
+    """Real = double Taylor traversal: the child references of ``bvh``, double records
+    ``triangles_d`` (T, 9) and double moments ``moments_d`` [node, field, 4]."""
+    nodes = _c(bvh.nodes, NODE_DTYPE)
+    tris = _c(triangles_d, np.float64)
+    mom = _c(moments_d, np.float64)
+    qx, qy, qz = _c(qx, np.float64), _c(qy, np.float64), _c(qz, np.float64)
+    out = np.empty(qx.shape[0], dtype=np.float64)
+    lib().gwn_oracle_winding_taylor_d64(
+        _ptr(nodes), bvh.node_count, bvh.root_reference, _ptr(tris), tris.shape[0], _ptr(mom), order,
+        float(accuracy_scale), stack_capacity, _ptr(qx), _ptr(qy), _ptr(qz), qx.shape[0], _ptr(out),
+        threads or default_threads(),
     )
     return out
 

@@ -605,7 +605,7 @@ __global__ __launch_bounds__(k_block) void collapse_level_kernel(
     std::uint32_t capacity, collapse_level *levels, collapse_task *tasks, std::uint32_t *status_cur,
     std::uint32_t *status_next, std::uint32_t status_words, collapse_stats *stats,
     bvh_node4 *__restrict__ nodes, std::uint32_t *__restrict__ parent,
-    std::uint8_t *__restrict__ internal_children
+    std::uint8_t *__restrict__ internal_children, int width
 ) {
     __shared__ std::uint32_t s_tile;
     __shared__ std::uint32_t s_exclusive;
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(k_block) void collapse_level_kernel(
     std::uint32_t const start = levels[depth].start;
     std::uint32_t const count = levels[depth].count;
     std::uint32_t const tile = s_tile;
-    // A level never exceeds the launch (4^depth bound) or the capacity; report, do not index.
+    // A level never exceeds the launch (width^depth bound) or the capacity; report, do not index.
     if (static_cast<std::uint64_t>(start) + count > capacity ||
         count > static_cast<std::uint64_t>(gridDim.x) * k_block) {
         if (threadIdx.x == 0)
@@ -652,7 +652,8 @@ __global__ __launch_bounds__(k_block) void collapse_level_kernel(
         cand[0] = root.x;
         cand[1] = root.y;
         n_cand = 2;
-        while (n_cand < 4) {
+        // Expand up to the tree's width (2, 3 or 4 children; slots past it stay empty).
+        while (n_cand < width) {
             int best = -1;
             float best_area = -INFINITY;
             std::uint32_t best_index = k_invalid;
@@ -780,11 +781,14 @@ __global__ __launch_bounds__(k_block) void records_kernel(
 } // namespace
 
 void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, int method, std::uint32_t hploc_radius,
-                hipStream_t stream) {
+                hipStream_t stream, int width) {
+    if (width < 2 || width > 4)
+        throw_invalid("bvh.build: BVH width must be 2, 3 or 4.");
     std::size_t const T = g.triangle_count;
     std::size_t const V = g.vertex_count;
     out.triangle_count = T;
     out.node_count = 0;
+    out.width = width;
     if (T == 0)
         return;
 
@@ -904,17 +908,18 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, int method, s
     while (empty_level < 0) {
         int const chunk_end = std::min(depth + chunk, k_max_levels - 1);
         for (; depth < chunk_end; ++depth) {
-            // A level holds at most 4^depth nodes and never more than the capacity.
-            std::size_t bound = capacity;
-            if (depth < 15)
-                bound = std::min<std::size_t>(bound, std::size_t(1) << (2 * depth));
+            // A level holds at most width^depth nodes and never more than the capacity.
+            std::size_t bound = 1;
+            for (int d = 0; d < depth && bound < capacity; ++d)
+                bound *= static_cast<std::size_t>(width);
+            bound = std::min<std::size_t>(bound, capacity);
             unsigned const grid = grid_for(bound, k_block);
             std::uint32_t *cur = status.ptr + static_cast<std::size_t>(depth & 1) * status_words;
             std::uint32_t *nxt = status.ptr + static_cast<std::size_t>((depth + 1) & 1) * status_words;
             collapse_level_kernel<<<grid, k_block, 0, stream>>>(
                 bnodes.ptr, ibounds.ptr, ifirst.ptr, leaf_boxes.ptr, depth,
                 static_cast<std::uint32_t>(capacity), levels.ptr, tasks.ptr, cur,
-                nxt, status_words, stats.ptr, wide.ptr, wparent.ptr, wchildren.ptr
+                nxt, status_words, stats.ptr, wide.ptr, wparent.ptr, wchildren.ptr, width
             );
             GWN_LAUNCH_CHECK();
         }

@@ -66,9 +66,18 @@ void validate_lanes(gwn_query_config const &c) {
         gwn::throw_invalid("Lanes per point must be 0 (auto), 1 or 4.");
 }
 
-void require_bvh(gwn_bvh_handle const *bvh) {
+void require_bvh(gwn_bvh_handle const *bvh, int real_bytes = 4) {
     if (bvh == nullptr || !bvh->valid())
         gwn::throw_invalid("BVH object contains no queryable data.");
+    if (bvh->real_bytes != real_bytes)
+        gwn::throw_invalid(real_bytes == 4 ? "BVH holds Real = double: query it with the f64 entry points."
+                                           : "BVH holds Real = float: query it with the f32 entry points.");
+}
+
+void require_geometry_real(gwn_geometry_handle const *g, int real_bytes) {
+    if (g != nullptr && g->real_bytes != real_bytes)
+        gwn::throw_invalid(real_bytes == 4 ? "Geometry holds Real = double: use the f64 entry points."
+                                           : "Geometry holds Real = float: use the f32 entry points.");
 }
 
 void require_stack(gwn_bvh_handle const &bvh, int capacity) {
@@ -77,12 +86,14 @@ void require_stack(gwn_bvh_handle const &bvh, int capacity) {
 }
 
 void require_moment(gwn_moment_handle const *m, gwn_bvh_handle const &bvh) {
-    bool ok = m != nullptr && m->bvh_revision != 0 && m->bvh_revision == bvh.revision;
+    bool ok = m != nullptr && m->bvh_revision != 0 && m->bvh_revision == bvh.revision &&
+              m->real_bytes == bvh.real_bytes;
     if (ok) {
         if (gwn::ref_leaf(bvh.root.reference))
             ok = m->node_count == 0;
         else
-            ok = m->node_count == bvh.node_count && m->records.ptr != nullptr;
+            ok = m->node_count == bvh.node_count &&
+                 (bvh.real_bytes == 8 ? m->fields_d.ptr != nullptr : m->records.ptr != nullptr);
     }
     if (!ok)
         gwn::throw_invalid("Moment object is not aligned with the BVH.");
@@ -117,76 +128,198 @@ int gwn_geometry_destroy(gwn_geometry_t geometry) {
     return guarded("gwn_geometry_destroy", [&] { delete geometry; });
 }
 
-int gwn_upload_geometry_f32(
-    gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count,
-    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count, hipStream_t stream
+// Upload shared by every (Real, Index): Real = double keeps the positions as given and a float
+// rounding of them for the topology build; uint64 indices are validated and stored as uint32
+// (vertex and triangle counts stay within the 32-bit range either way).
+static void upload_common(
+    gwn_geometry_t geometry, int real_bytes, const void *x, const void *y, const void *z, uint64_t vertex_count,
+    int index_bytes, const void *i0, const void *i1, const void *i2, uint64_t triangle_count, hipStream_t stream
 ) {
-    return guarded("gwn_upload_geometry", [&] {
-        if (geometry == nullptr)
-            gwn::throw_invalid("Geometry handle is null.");
-        if (vertex_count > 0xffffffffull)
-            gwn::throw_invalid("Vertex count exceeds index type capacity.");
-        if (triangle_count > 0xffffffffull)
-            gwn::throw_invalid("Triangle count exceeds index type capacity.");
-        if ((vertex_count != 0 && (x == nullptr || y == nullptr || z == nullptr)) ||
-            (triangle_count != 0 && (i0 == nullptr || i1 == nullptr || i2 == nullptr)))
-            gwn::throw_invalid("Geometry spans must use non-null storage when non-empty.");
+    if (geometry == nullptr)
+        gwn::throw_invalid("Geometry handle is null.");
+    if (vertex_count > 0xffffffffull)
+        gwn::throw_invalid(index_bytes == 4 ? "Vertex count exceeds index type capacity."
+                                            : "Vertex count exceeds the native 32-bit index range.");
+    if (triangle_count > 0xffffffffull)
+        gwn::throw_invalid(index_bytes == 4 ? "Triangle count exceeds index type capacity."
+                                            : "Triangle count exceeds the native 32-bit index range.");
+    if ((vertex_count != 0 && (x == nullptr || y == nullptr || z == nullptr)) ||
+        (triangle_count != 0 && (i0 == nullptr || i1 == nullptr || i2 == nullptr)))
+        gwn::throw_invalid("Geometry spans must use non-null storage when non-empty.");
+    const std::uint32_t *j0 = static_cast<const std::uint32_t *>(i0);
+    const std::uint32_t *j1 = static_cast<const std::uint32_t *>(i1);
+    const std::uint32_t *j2 = static_cast<const std::uint32_t *>(i2);
+    std::vector<std::uint32_t> narrowed;
+    if (index_bytes == 8) {
+        auto const *k0 = static_cast<const std::uint64_t *>(i0);
+        auto const *k1 = static_cast<const std::uint64_t *>(i1);
+        auto const *k2 = static_cast<const std::uint64_t *>(i2);
         for (std::uint64_t t = 0; t < triangle_count; ++t)
-            if (i0[t] >= vertex_count || i1[t] >= vertex_count || i2[t] >= vertex_count)
+            if (k0[t] >= vertex_count || k1[t] >= vertex_count || k2[t] >= vertex_count)
                 gwn::throw_invalid("Triangle indices must be in [0, vertex_count).");
-        gwn_geometry_handle staging;
-        staging.stream = stream;
-        staging.vertex_count = vertex_count;
-        staging.triangle_count = triangle_count;
-        staging.x.allocate(vertex_count, stream);
-        staging.y.allocate(vertex_count, stream);
-        staging.z.allocate(vertex_count, stream);
-        staging.i0.allocate(triangle_count, stream);
-        staging.i1.allocate(triangle_count, stream);
-        staging.i2.allocate(triangle_count, stream);
-        std::size_t const vb = vertex_count * sizeof(float), tb = triangle_count * sizeof(std::uint32_t);
+        narrowed.resize(3 * triangle_count);
+        for (std::uint64_t t = 0; t < triangle_count; ++t) {
+            narrowed[t] = static_cast<std::uint32_t>(k0[t]);
+            narrowed[triangle_count + t] = static_cast<std::uint32_t>(k1[t]);
+            narrowed[2 * triangle_count + t] = static_cast<std::uint32_t>(k2[t]);
+        }
+        j0 = narrowed.data();
+        j1 = j0 + triangle_count;
+        j2 = j1 + triangle_count;
+    } else {
+        for (std::uint64_t t = 0; t < triangle_count; ++t)
+            if (j0[t] >= vertex_count || j1[t] >= vertex_count || j2[t] >= vertex_count)
+                gwn::throw_invalid("Triangle indices must be in [0, vertex_count).");
+    }
+    gwn_geometry_handle staging;
+    staging.stream = stream;
+    staging.vertex_count = vertex_count;
+    staging.triangle_count = triangle_count;
+    staging.real_bytes = real_bytes;
+    staging.index_bytes = index_bytes;
+    staging.x.allocate(vertex_count, stream);
+    staging.y.allocate(vertex_count, stream);
+    staging.z.allocate(vertex_count, stream);
+    staging.i0.allocate(triangle_count, stream);
+    staging.i1.allocate(triangle_count, stream);
+    staging.i2.allocate(triangle_count, stream);
+    std::size_t const tb = triangle_count * sizeof(std::uint32_t);
+    if (real_bytes == 8) {
+        staging.xd.allocate(vertex_count, stream);
+        staging.yd.allocate(vertex_count, stream);
+        staging.zd.allocate(vertex_count, stream);
+        std::size_t const vb = vertex_count * sizeof(double);
+        if (vb) {
+            GWN_HIP(hipMemcpyAsync(staging.xd.ptr, x, vb, hipMemcpyHostToDevice, stream));
+            GWN_HIP(hipMemcpyAsync(staging.yd.ptr, y, vb, hipMemcpyHostToDevice, stream));
+            GWN_HIP(hipMemcpyAsync(staging.zd.ptr, z, vb, hipMemcpyHostToDevice, stream));
+        }
+        gwn::narrow_positions(staging, stream);
+    } else {
+        std::size_t const vb = vertex_count * sizeof(float);
         if (vb) {
             GWN_HIP(hipMemcpyAsync(staging.x.ptr, x, vb, hipMemcpyHostToDevice, stream));
             GWN_HIP(hipMemcpyAsync(staging.y.ptr, y, vb, hipMemcpyHostToDevice, stream));
             GWN_HIP(hipMemcpyAsync(staging.z.ptr, z, vb, hipMemcpyHostToDevice, stream));
         }
-        if (tb) {
-            GWN_HIP(hipMemcpyAsync(staging.i0.ptr, i0, tb, hipMemcpyHostToDevice, stream));
-            GWN_HIP(hipMemcpyAsync(staging.i1.ptr, i1, tb, hipMemcpyHostToDevice, stream));
-            GWN_HIP(hipMemcpyAsync(staging.i2.ptr, i2, tb, hipMemcpyHostToDevice, stream));
-        }
-        // Host sources may be pageable and released by the caller after return.
-        GWN_HIP(hipStreamSynchronize(stream));
-        staging.generation = geometry->generation + 1;
-        std::swap(staging.x, geometry->x);
-        std::swap(staging.y, geometry->y);
-        std::swap(staging.z, geometry->z);
-        std::swap(staging.i0, geometry->i0);
-        std::swap(staging.i1, geometry->i1);
-        std::swap(staging.i2, geometry->i2);
-        geometry->vertex_count = staging.vertex_count;
-        geometry->triangle_count = staging.triangle_count;
-        geometry->stream = stream;
-        geometry->generation = staging.generation;
+    }
+    if (tb) {
+        GWN_HIP(hipMemcpyAsync(staging.i0.ptr, j0, tb, hipMemcpyHostToDevice, stream));
+        GWN_HIP(hipMemcpyAsync(staging.i1.ptr, j1, tb, hipMemcpyHostToDevice, stream));
+        GWN_HIP(hipMemcpyAsync(staging.i2.ptr, j2, tb, hipMemcpyHostToDevice, stream));
+    }
+    // Host sources may be pageable and released by the caller after return.
+    GWN_HIP(hipStreamSynchronize(stream));
+    staging.generation = geometry->generation + 1;
+    std::swap(staging.x, geometry->x);
+    std::swap(staging.y, geometry->y);
+    std::swap(staging.z, geometry->z);
+    std::swap(staging.xd, geometry->xd);
+    std::swap(staging.yd, geometry->yd);
+    std::swap(staging.zd, geometry->zd);
+    std::swap(staging.i0, geometry->i0);
+    std::swap(staging.i1, geometry->i1);
+    std::swap(staging.i2, geometry->i2);
+    geometry->vertex_count = staging.vertex_count;
+    geometry->triangle_count = staging.triangle_count;
+    geometry->real_bytes = real_bytes;
+    geometry->index_bytes = index_bytes;
+    geometry->stream = stream;
+    geometry->generation = staging.generation;
+}
+
+int gwn_upload_geometry_f32(
+    gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count, hipStream_t stream
+) {
+    return guarded("gwn_upload_geometry", [&] {
+        upload_common(geometry, 4, x, y, z, vertex_count, 4, i0, i1, i2, triangle_count, stream);
     });
 }
 
-static void update_positions(
+int gwn_upload_geometry_f32_u64(
     gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count,
+    const uint64_t *i0, const uint64_t *i1, const uint64_t *i2, uint64_t triangle_count, hipStream_t stream
+) {
+    return guarded("gwn_upload_geometry", [&] {
+        upload_common(geometry, 4, x, y, z, vertex_count, 8, i0, i1, i2, triangle_count, stream);
+    });
+}
+
+int gwn_upload_geometry_f64(
+    gwn_geometry_t geometry, const double *x, const double *y, const double *z, uint64_t vertex_count,
+    const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, uint64_t triangle_count, hipStream_t stream
+) {
+    return guarded("gwn_upload_geometry", [&] {
+        upload_common(geometry, 8, x, y, z, vertex_count, 4, i0, i1, i2, triangle_count, stream);
+    });
+}
+
+int gwn_upload_geometry_f64_u64(
+    gwn_geometry_t geometry, const double *x, const double *y, const double *z, uint64_t vertex_count,
+    const uint64_t *i0, const uint64_t *i1, const uint64_t *i2, uint64_t triangle_count, hipStream_t stream
+) {
+    return guarded("gwn_upload_geometry", [&] {
+        upload_common(geometry, 8, x, y, z, vertex_count, 8, i0, i1, i2, triangle_count, stream);
+    });
+}
+
+// Object stream binding (include/gwn/gwn_utils.cuh:59-80): every buffer of an object is released
+// on the object's bound stream, so rebinding moves every buffer's release stream with it.
+extern "C++" {
+static void rebind(gwn_geometry_handle &g, hipStream_t s) noexcept {
+    g.stream = s;
+    g.x.stream = g.y.stream = g.z.stream = s;
+    g.i0.stream = g.i1.stream = g.i2.stream = s;
+    g.xd.stream = g.yd.stream = g.zd.stream = s;
+}
+
+static void rebind(gwn_bvh_handle &b, hipStream_t s) noexcept {
+    b.stream = s;
+    b.nodes.stream = b.primitive_indices.stream = b.triangles.stream = s;
+    b.parent.stream = b.internal_children.stream = s;
+    b.nodes_d.stream = b.triangles_d.stream = s;
+}
+
+static void rebind(gwn_moment_handle &m, hipStream_t s) noexcept {
+    m.stream = s;
+    m.records.stream = m.slot_area.stream = s;
+    m.crefs_d.stream = m.fields_d.stream = s;
+}
+
+static void rebind(gwn_boundary_handle &b, hipStream_t s) noexcept {
+    b.stream = s;
+    b.start.stream = b.end.stream = b.multiplicity.stream = s;
+}
+} // extern "C++"
+
+static void update_positions(
+    gwn_geometry_t geometry, int real_bytes, const void *x, const void *y, const void *z, uint64_t vertex_count,
     hipStream_t stream, hipMemcpyKind kind
 ) {
     if (geometry == nullptr || (geometry->vertex_count != 0 && geometry->x.ptr == nullptr))
         gwn::throw_invalid("Geometry accessor is invalid for update.");
+    if (geometry->real_bytes != real_bytes)
+        gwn::throw_invalid(real_bytes == 4 ? "Geometry holds Real = double: update it with the f64 entry points."
+                                           : "Geometry holds Real = float: update it with the f32 entry points.");
     if (vertex_count != geometry->vertex_count)
         gwn::throw_invalid("Updated vertex count must match geometry.");
     if (vertex_count != 0 && (x == nullptr || y == nullptr || z == nullptr))
         gwn::throw_invalid("Geometry update spans must use non-null storage when non-empty.");
-    geometry->stream = stream;
-    std::size_t const vb = vertex_count * sizeof(float);
+    // From the first copy on, the storage holds this stream's writes: release it there.
+    rebind(*geometry, stream);
+    std::size_t const vb = vertex_count * static_cast<std::size_t>(real_bytes);
     if (vb) {
-        GWN_HIP(hipMemcpyAsync(geometry->x.ptr, x, vb, kind, stream));
-        GWN_HIP(hipMemcpyAsync(geometry->y.ptr, y, vb, kind, stream));
-        GWN_HIP(hipMemcpyAsync(geometry->z.ptr, z, vb, kind, stream));
+        if (real_bytes == 8) {
+            GWN_HIP(hipMemcpyAsync(geometry->xd.ptr, x, vb, kind, stream));
+            GWN_HIP(hipMemcpyAsync(geometry->yd.ptr, y, vb, kind, stream));
+            GWN_HIP(hipMemcpyAsync(geometry->zd.ptr, z, vb, kind, stream));
+            gwn::narrow_positions(*geometry, stream);
+        } else {
+            GWN_HIP(hipMemcpyAsync(geometry->x.ptr, x, vb, kind, stream));
+            GWN_HIP(hipMemcpyAsync(geometry->y.ptr, y, vb, kind, stream));
+            GWN_HIP(hipMemcpyAsync(geometry->z.ptr, z, vb, kind, stream));
+        }
     }
     geometry->generation += 1;
 }
@@ -195,8 +328,10 @@ int gwn_update_geometry_f32(
     gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count, hipStream_t stream
 ) {
     return guarded("gwn_update_geometry", [&] {
-        update_positions(geometry, x, y, z, vertex_count, stream, hipMemcpyHostToDevice);
-        GWN_HIP(hipStreamSynchronize(stream));
+        // Enqueued like the reference's update (detail/gwn_geometry_impl.cuh:172-178): from pageable
+        // host memory the runtime stages the bytes before the call returns; pinned sources must
+        // stay valid until the stream reaches the copies.
+        update_positions(geometry, 4, x, y, z, vertex_count, stream, hipMemcpyHostToDevice);
     });
 }
 
@@ -204,7 +339,25 @@ int gwn_update_geometry_device_f32(
     gwn_geometry_t geometry, const float *x, const float *y, const float *z, uint64_t vertex_count, hipStream_t stream
 ) {
     return guarded("gwn_update_geometry", [&] {
-        update_positions(geometry, x, y, z, vertex_count, stream, hipMemcpyDeviceToDevice);
+        update_positions(geometry, 4, x, y, z, vertex_count, stream, hipMemcpyDeviceToDevice);
+    });
+}
+
+int gwn_update_geometry_f64(
+    gwn_geometry_t geometry, const double *x, const double *y, const double *z, uint64_t vertex_count,
+    hipStream_t stream
+) {
+    return guarded("gwn_update_geometry", [&] {
+        update_positions(geometry, 8, x, y, z, vertex_count, stream, hipMemcpyHostToDevice);
+    });
+}
+
+int gwn_update_geometry_device_f64(
+    gwn_geometry_t geometry, const double *x, const double *y, const double *z, uint64_t vertex_count,
+    hipStream_t stream
+) {
+    return guarded("gwn_update_geometry", [&] {
+        update_positions(geometry, 8, x, y, z, vertex_count, stream, hipMemcpyDeviceToDevice);
     });
 }
 
@@ -235,27 +388,63 @@ int gwn_bvh_destroy(gwn_bvh_t bvh) {
     return guarded("gwn_bvh_destroy", [&] { delete bvh; });
 }
 
+static void build_common(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, int32_t width, int32_t method, uint32_t hploc_search_radius,
+    hipStream_t stream, int real_bytes = 4
+);
+
 int gwn_build_bvh4_f32(
     gwn_geometry_t geometry, gwn_bvh_t bvh, int32_t method, uint32_t hploc_search_radius, hipStream_t stream
 ) {
+    return guarded("gwn_build_bvh", [&] { build_common(geometry, bvh, 4, method, hploc_search_radius, stream); });
+}
+
+int gwn_build_bvh_f32(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, int32_t width, int32_t method, uint32_t hploc_search_radius,
+    hipStream_t stream
+) {
+    return guarded("gwn_build_bvh", [&] { build_common(geometry, bvh, width, method, hploc_search_radius, stream); });
+}
+
+int gwn_build_bvh_f64(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, int32_t width, int32_t method, uint32_t hploc_search_radius,
+    hipStream_t stream
+) {
     return guarded("gwn_build_bvh", [&] {
+        build_common(geometry, bvh, width, method, hploc_search_radius, stream, 8);
+    });
+}
+
+static void build_common(
+    gwn_geometry_t geometry, gwn_bvh_t bvh, int32_t width, int32_t method, uint32_t hploc_search_radius,
+    hipStream_t stream, int real_bytes
+) {
+    {
         if (bvh == nullptr)
             gwn::throw_invalid("bvh.build: BVH handle is null.");
         if (geometry == nullptr || (geometry->triangle_count != 0 && geometry->i0.ptr == nullptr))
             gwn::throw_invalid("bvh.build: Geometry accessor is invalid for BVH build.");
         if (method != GWN_BVH_BUILD_HPLOC && method != GWN_BVH_BUILD_LBVH)
             gwn::throw_invalid("bvh.build: BVH build method is invalid.");
+        if (width < 2 || width > 4)
+            gwn::throw_invalid("bvh.build: BVH width must be 2, 3 or 4 (nodes are stored four-wide).");
         if (method == GWN_BVH_BUILD_HPLOC && (hploc_search_radius == 0 || hploc_search_radius > 8))
             gwn::throw_invalid("bvh.build: H-PLOC search radius must be in the range [1, 8].");
         if (geometry->triangle_count != 0 && geometry->vertex_count == 0)
             gwn::throw_invalid("bvh.build: Cannot build a BVH with triangles but no vertices.");
+        require_geometry_real(geometry, real_bytes);
         // Wide-node ids and per-level counts travel in 30-bit fields (look-back status words,
         // parent links with the slot in the low two bits).
         if (geometry->triangle_count > (1ull << 30) - 1ull)
             gwn::throw_invalid("bvh.build: Triangle count exceeds the native wide-collapse counter range.");
         gwn_bvh_handle staging;
         staging.stream = stream;
-        gwn::build_bvh4(*geometry, staging, method, hploc_search_radius, stream);
+        gwn::build_bvh4(*geometry, staging, method, hploc_search_radius, stream, width);
+        staging.real_bytes = real_bytes;
+        staging.index_bytes = geometry->index_bytes;
+        // Real = double: double records and boxes from the given positions over the float-built tree.
+        if (real_bytes == 8 && geometry->triangle_count != 0)
+            gwn::refit_bounds_double(*geometry, staging, stream);
         staging.storage_generation = gwn::next_storage_generation();
         if (geometry->triangle_count != 0) {
             staging.revision = gwn::reserve_revision();
@@ -264,30 +453,39 @@ int gwn_build_bvh4_f32(
         }
         std::swap(*bvh, staging);
         // staging now owns the replaced storage; its buffers free on their own streams.
-    });
+    }
+}
+
+static void refit_common(gwn_geometry_t geometry, gwn_bvh_t bvh, hipStream_t stream, int real_bytes) {
+    if (geometry == nullptr || (geometry->triangle_count != 0 && geometry->i0.ptr == nullptr))
+        gwn::throw_invalid("bvh.refit: Geometry accessor is invalid.");
+    if (bvh == nullptr || !bvh->valid())
+        gwn::throw_invalid("bvh.refit: BVH accessor is invalid.");
+    require_geometry_real(geometry, real_bytes);
+    if (bvh->real_bytes != real_bytes)
+        gwn::throw_invalid("bvh.refit: BVH and refit entry point differ in Real.");
+    if (geometry->triangle_count != bvh->triangle_count)
+        gwn::throw_invalid("bvh.refit: Geometry triangle count differs from the BVH primitive order.");
+    std::uint64_t const next_revision = gwn::reserve_revision();
+    if (next_revision == 0)
+        gwn::throw_internal("bvh.refit: BVH geometry-derived state revision is exhausted.");
+    // Writes below are in place and cannot be rolled back: the BVH is unqueryable (and
+    // every moment formed from it stale) until the refit has converged.  From here on its
+    // storage is released on the refit stream, which holds the last writes.
+    bvh->revision = 0;
+    rebind(*bvh, stream);
+    gwn::refit_bounds(*geometry, *bvh, stream);
+    if (real_bytes == 8)
+        gwn::refit_bounds_double(*geometry, *bvh, stream);
+    bvh->revision = next_revision;
 }
 
 int gwn_refit_bvh_f32(gwn_geometry_t geometry, gwn_bvh_t bvh, hipStream_t stream) {
-    return guarded("gwn_refit_bvh", [&] {
-        if (geometry == nullptr || (geometry->triangle_count != 0 && geometry->i0.ptr == nullptr))
-            gwn::throw_invalid("bvh.refit: Geometry accessor is invalid.");
-        if (bvh == nullptr || !bvh->valid())
-            gwn::throw_invalid("bvh.refit: BVH accessor is invalid.");
-        if (geometry->triangle_count != bvh->triangle_count)
-            gwn::throw_invalid("bvh.refit: Geometry triangle count differs from the BVH primitive order.");
-        std::uint64_t const next_revision = gwn::reserve_revision();
-        if (next_revision == 0)
-            gwn::throw_internal("bvh.refit: BVH geometry-derived state revision is exhausted.");
-        // Writes below are in place and cannot be rolled back: the BVH is unqueryable (and
-        // every moment formed from it stale) until the refit has converged.  From here on its
-        // storage is released on the refit stream, which holds the last writes.
-        bvh->revision = 0;
-        bvh->stream = stream;
-        bvh->nodes.stream = bvh->primitive_indices.stream = bvh->triangles.stream = stream;
-        bvh->parent.stream = bvh->internal_children.stream = stream;
-        gwn::refit_bounds(*geometry, *bvh, stream);
-        bvh->revision = next_revision;
-    });
+    return guarded("gwn_refit_bvh", [&] { refit_common(geometry, bvh, stream, 4); });
+}
+
+int gwn_refit_bvh_f64(gwn_geometry_t geometry, gwn_bvh_t bvh, hipStream_t stream) {
+    return guarded("gwn_refit_bvh", [&] { refit_common(geometry, bvh, stream, 8); });
 }
 
 int gwn_bvh_get_info(gwn_bvh_t bvh, gwn_bvh_info *info) {
@@ -301,22 +499,229 @@ int gwn_bvh_get_info(gwn_bvh_t bvh, gwn_bvh_info *info) {
         info->internal_stack_bound = bvh->internal_stack_bound;
         info->packed_stack_bound = bvh->packed_stack_bound;
         info->max_depth = bvh->max_depth;
-        info->width = 4;
+        info->width = static_cast<uint32_t>(bvh->width);
         std::memcpy(info->root_bounds, &bvh->root.bounds, sizeof(info->root_bounds));
         info->root_reference = bvh->root.reference;
+    });
+}
+
+int gwn_bvh_download_f64(gwn_bvh_t bvh, void *nodes, uint32_t *primitive_indices, double *triangles,
+                         hipStream_t stream) {
+    return guarded("gwn_bvh_download", [&] {
+        require_bvh(bvh, 8);
+        std::vector<gwn::bvh_node4_d> wide;
+        if (nodes && bvh->node_count) {
+            wide.resize(bvh->node_count);
+            GWN_HIP(hipMemcpyAsync(wide.data(), bvh->nodes_d.ptr, bvh->node_count * sizeof(gwn::bvh_node4_d),
+                                   hipMemcpyDeviceToHost, stream));
+        }
+        if (primitive_indices)
+            GWN_HIP(hipMemcpyAsync(primitive_indices, bvh->primitive_indices.ptr,
+                                   bvh->triangle_count * sizeof(std::uint32_t), hipMemcpyDeviceToHost, stream));
+        if (triangles)
+            GWN_HIP(hipMemcpyAsync(triangles, bvh->triangles_d.ptr, bvh->triangle_count * sizeof(gwn::tri_record_d),
+                                   hipMemcpyDeviceToHost, stream));
+        GWN_HIP(hipStreamSynchronize(stream));
+        // gwn_bvh_node<Width, double>: each node's first Width children.
+        if (!wide.empty()) {
+            std::size_t const child_bytes = sizeof(gwn::bvh_child_d);
+            auto *dst = static_cast<std::uint8_t *>(nodes);
+            for (std::size_t n = 0; n < wide.size(); ++n)
+                std::memcpy(dst + n * bvh->width * child_bytes, wide[n].children, bvh->width * child_bytes);
+        }
     });
 }
 
 int gwn_bvh_download(gwn_bvh_t bvh, void *nodes, uint32_t *primitive_indices, float *triangles, hipStream_t stream) {
     return guarded("gwn_bvh_download", [&] {
         require_bvh(bvh);
-        if (nodes && bvh->node_count)
-            GWN_HIP(hipMemcpyAsync(nodes, bvh->nodes.ptr, bvh->node_count * sizeof(gwn::bvh_node4), hipMemcpyDeviceToHost, stream));
+        std::vector<gwn::bvh_node4> wide;
+        if (nodes && bvh->node_count) {
+            if (bvh->width == 4) {
+                GWN_HIP(hipMemcpyAsync(nodes, bvh->nodes.ptr, bvh->node_count * sizeof(gwn::bvh_node4),
+                                       hipMemcpyDeviceToHost, stream));
+            } else {
+                wide.resize(bvh->node_count);
+                GWN_HIP(hipMemcpyAsync(wide.data(), bvh->nodes.ptr, bvh->node_count * sizeof(gwn::bvh_node4),
+                                       hipMemcpyDeviceToHost, stream));
+            }
+        }
         if (primitive_indices)
             GWN_HIP(hipMemcpyAsync(primitive_indices, bvh->primitive_indices.ptr, bvh->triangle_count * sizeof(std::uint32_t), hipMemcpyDeviceToHost, stream));
         if (triangles)
             GWN_HIP(hipMemcpyAsync(triangles, bvh->triangles.ptr, bvh->triangle_count * sizeof(gwn::tri_record), hipMemcpyDeviceToHost, stream));
         GWN_HIP(hipStreamSynchronize(stream));
+        // A narrower tree is the reference's gwn_bvh_node<Width, float>: its first Width children.
+        if (!wide.empty()) {
+            std::size_t const child_bytes = sizeof(gwn::bvh_child);
+            auto *dst = static_cast<std::uint8_t *>(nodes);
+            for (std::size_t n = 0; n < wide.size(); ++n)
+                std::memcpy(dst + n * bvh->width * child_bytes, wide[n].children, bvh->width * child_bytes);
+        }
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// object lifetime: stream binding, release, accessors
+// ------------------------------------------------------------------------------------------
+
+extern "C++" template <class H> static H *require_handle(H *h, char const *what) {
+    if (h == nullptr)
+        gwn::throw_invalid(std::string(what) + " handle is null.");
+    return h;
+}
+
+int gwn_geometry_stream(gwn_geometry_t geometry, hipStream_t *stream) {
+    return guarded("gwn_geometry_stream", [&] {
+        if (stream == nullptr)
+            gwn::throw_invalid("Output stream pointer is null.");
+        *stream = require_handle(geometry, "Geometry")->stream;
+    });
+}
+
+int gwn_geometry_set_stream(gwn_geometry_t geometry, hipStream_t stream) {
+    return guarded("gwn_geometry_set_stream", [&] { rebind(*require_handle(geometry, "Geometry"), stream); });
+}
+
+int gwn_geometry_clear(gwn_geometry_t geometry) {
+    return guarded("gwn_geometry_clear", [&] {
+        gwn_geometry_handle &g = *require_handle(geometry, "Geometry");
+        for (auto *b : {&g.x, &g.y, &g.z})
+            b->release();
+        for (auto *b : {&g.xd, &g.yd, &g.zd})
+            b->release();
+        for (auto *b : {&g.i0, &g.i1, &g.i2})
+            b->release();
+        g.vertex_count = g.triangle_count = 0;
+        g.generation += 1;
+    });
+}
+
+int gwn_geometry_get_accessor(gwn_geometry_t geometry, gwn_geometry_accessor *out) {
+    return guarded("gwn_geometry_get_accessor", [&] {
+        gwn_geometry_handle const &g = *require_handle(geometry, "Geometry");
+        if (out == nullptr)
+            gwn::throw_invalid("Accessor output pointer is null.");
+        if (g.real_bytes != 4)
+            gwn::throw_invalid("Device accessors are provided for Real = float objects.");
+        *out = gwn_geometry_accessor{g.x.ptr, g.y.ptr, g.z.ptr, g.i0.ptr, g.i1.ptr, g.i2.ptr,
+                                     g.vertex_count, g.triangle_count};
+    });
+}
+
+int gwn_bvh_stream(gwn_bvh_t bvh, hipStream_t *stream) {
+    return guarded("gwn_bvh_stream", [&] {
+        if (stream == nullptr)
+            gwn::throw_invalid("Output stream pointer is null.");
+        *stream = require_handle(bvh, "BVH")->stream;
+    });
+}
+
+int gwn_bvh_set_stream(gwn_bvh_t bvh, hipStream_t stream) {
+    return guarded("gwn_bvh_set_stream", [&] { rebind(*require_handle(bvh, "BVH"), stream); });
+}
+
+int gwn_bvh_clear(gwn_bvh_t bvh) {
+    return guarded("gwn_bvh_clear", [&] {
+        gwn_bvh_handle &b = *require_handle(bvh, "BVH");
+        hipStream_t const s = b.stream;
+        gwn_bvh_handle empty;
+        std::swap(b, empty);
+        rebind(b, s);
+        b.storage_generation = gwn::next_storage_generation();
+        // `empty` releases the old storage on the old binding as it goes out of scope.
+    });
+}
+
+int gwn_bvh_get_accessor(gwn_bvh_t bvh, gwn_bvh_accessor *out) {
+    return guarded("gwn_bvh_get_accessor", [&] {
+        gwn_bvh_handle const &b = *require_handle(bvh, "BVH");
+        if (out == nullptr)
+            gwn::throw_invalid("Accessor output pointer is null.");
+        if (b.real_bytes != 4)
+            gwn::throw_invalid("Device accessors are provided for Real = float objects.");
+        std::memset(out, 0, sizeof(*out));
+        out->nodes = b.nodes.ptr;
+        out->node_count = b.node_count;
+        out->primitive_indices = b.primitive_indices.ptr;
+        out->triangles = reinterpret_cast<const float *>(b.triangles.ptr);
+        out->triangle_count = b.triangle_count;
+        std::memcpy(out->root_bounds, &b.root.bounds, sizeof(out->root_bounds));
+        out->root_reference = b.root.reference;
+        out->internal_stack_bound = b.internal_stack_bound;
+        out->packed_stack_bound = b.packed_stack_bound;
+        out->revision = b.revision;
+    });
+}
+
+int gwn_moment_stream(gwn_moment_t moment, hipStream_t *stream) {
+    return guarded("gwn_moment_stream", [&] {
+        if (stream == nullptr)
+            gwn::throw_invalid("Output stream pointer is null.");
+        *stream = require_handle(moment, "Moment")->stream;
+    });
+}
+
+int gwn_moment_set_stream(gwn_moment_t moment, hipStream_t stream) {
+    return guarded("gwn_moment_set_stream", [&] { rebind(*require_handle(moment, "Moment"), stream); });
+}
+
+int gwn_moment_clear(gwn_moment_t moment) {
+    return guarded("gwn_moment_clear", [&] {
+        gwn_moment_handle &m = *require_handle(moment, "Moment");
+        m.records.release();
+        m.slot_area.release();
+        m.crefs_d.release();
+        m.fields_d.release();
+        m.node_count = 0;
+        m.bvh_revision = 0;
+        m.storage_generation = gwn::next_storage_generation();
+    });
+}
+
+int gwn_moment_get_accessor(gwn_moment_t moment, gwn_moment_accessor *out) {
+    return guarded("gwn_moment_get_accessor", [&] {
+        gwn_moment_handle const &m = *require_handle(moment, "Moment");
+        if (out == nullptr)
+            gwn::throw_invalid("Accessor output pointer is null.");
+        if (m.real_bytes != 4)
+            gwn::throw_invalid("Device accessors are provided for Real = float objects.");
+        *out = gwn_moment_accessor{m.records.ptr, m.node_count, m.order,
+                                   static_cast<uint32_t>(gwn::moment_record_vec4(m.order) * 16), m.width,
+                                   m.bvh_revision};
+    });
+}
+
+int gwn_boundary_stream(gwn_boundary_t boundary, hipStream_t *stream) {
+    return guarded("gwn_boundary_stream", [&] {
+        if (stream == nullptr)
+            gwn::throw_invalid("Output stream pointer is null.");
+        *stream = require_handle(boundary, "Boundary chain")->stream;
+    });
+}
+
+int gwn_boundary_set_stream(gwn_boundary_t boundary, hipStream_t stream) {
+    return guarded("gwn_boundary_set_stream", [&] { rebind(*require_handle(boundary, "Boundary chain"), stream); });
+}
+
+int gwn_boundary_clear(gwn_boundary_t boundary) {
+    return guarded("gwn_boundary_clear", [&] {
+        gwn_boundary_handle &b = *require_handle(boundary, "Boundary chain");
+        b.start.release();
+        b.end.release();
+        b.multiplicity.release();
+        b.edge_count = b.mesh_vertex_count = b.mesh_triangle_count = 0;
+        b.built = false;
+    });
+}
+
+int gwn_boundary_get_accessor(gwn_boundary_t boundary, gwn_boundary_accessor *out) {
+    return guarded("gwn_boundary_get_accessor", [&] {
+        gwn_boundary_handle const &b = *require_handle(boundary, "Boundary chain");
+        if (out == nullptr)
+            gwn::throw_invalid("Accessor output pointer is null.");
+        *out = gwn_boundary_accessor{b.start.ptr, b.end.ptr, b.multiplicity.ptr, b.edge_count,
+                                     b.mesh_vertex_count, b.mesh_triangle_count, b.built ? 1 : 0};
     });
 }
 
@@ -418,14 +823,44 @@ int gwn_moment_order(gwn_moment_t moment, int32_t *order) {
     });
 }
 
-int gwn_refit_bvh_moment_f32(gwn_bvh_t bvh, gwn_moment_t moment, hipStream_t stream) {
+int gwn_moment_info(gwn_moment_t moment, uint64_t *node_count, uint64_t *bvh_revision) {
+    return guarded("gwn_moment_info", [&] {
+        gwn_moment_handle const &m = *require_handle(moment, "Moment");
+        if (node_count)
+            *node_count = m.node_count;
+        if (bvh_revision)
+            *bvh_revision = m.bvh_revision;
+    });
+}
+
+int gwn_refit_bvh_moment_f64(gwn_bvh_t bvh, gwn_moment_t moment, hipStream_t stream) {
     return guarded("gwn_refit_bvh_moment", [&] {
         if (bvh == nullptr || !bvh->valid())
             gwn::throw_invalid("bvh.refit.moment: BVH accessor is invalid for moment refit.");
+        if (bvh->real_bytes != 8)
+            gwn::throw_invalid("bvh.refit.moment: BVH holds Real = float: use the f32 entry point.");
         if (moment == nullptr)
             gwn::throw_invalid("bvh.refit.moment: Moment handle is null.");
         gwn_moment_handle staging;
         staging.order = moment->order;
+        staging.stream = stream;
+        staging.storage_generation = gwn::next_storage_generation();
+        gwn::refit_moment_double(*bvh, staging, stream);
+        std::swap(*moment, staging);
+    });
+}
+
+int gwn_refit_bvh_moment_f32(gwn_bvh_t bvh, gwn_moment_t moment, hipStream_t stream) {
+    return guarded("gwn_refit_bvh_moment", [&] {
+        if (bvh == nullptr || !bvh->valid())
+            gwn::throw_invalid("bvh.refit.moment: BVH accessor is invalid for moment refit.");
+        if (bvh->real_bytes != 4)
+            gwn::throw_invalid("bvh.refit.moment: BVH holds Real = double: use the f64 entry point.");
+        if (moment == nullptr)
+            gwn::throw_invalid("bvh.refit.moment: Moment handle is null.");
+        gwn_moment_handle staging;
+        staging.order = moment->order;
+        staging.width = bvh->width;
         staging.stream = stream;
         staging.storage_generation = gwn::next_storage_generation();
         gwn::refit_moment(*bvh, staging, stream);
@@ -433,10 +868,34 @@ int gwn_refit_bvh_moment_f32(gwn_bvh_t bvh, gwn_moment_t moment, hipStream_t str
     });
 }
 
+int gwn_moment_download_f64(gwn_moment_t moment, double *fields, hipStream_t stream) {
+    return guarded("gwn_moment_download", [&] {
+        if (moment == nullptr || fields == nullptr)
+            gwn::throw_invalid("Moment handle or output pointer is null.");
+        if (moment->real_bytes != 8)
+            gwn::throw_invalid("Moment holds Real = float: use the f32 download.");
+        std::size_t const N = moment->node_count;
+        int const F = gwn::moment_field_count(moment->order);
+        if (N == 0 || moment->fields_d.ptr == nullptr)
+            return;
+        std::vector<double> host(N * F * 4);
+        GWN_HIP(hipMemcpyAsync(host.data(), moment->fields_d.ptr, host.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, stream));
+        GWN_HIP(hipStreamSynchronize(stream));
+        // Field-SoA gwn_bvh_moment_node<Width, Order, double>: field f of slot s at [f * Width + s].
+        int const W = moment->width;
+        for (std::size_t n = 0; n < N; ++n)
+            for (int f = 0; f < F; ++f)
+                std::memcpy(fields + (n * F + f) * W, host.data() + (n * F + f) * 4, sizeof(double) * W);
+    });
+}
+
 int gwn_moment_download(gwn_moment_t moment, float *fields, hipStream_t stream) {
     return guarded("gwn_moment_download", [&] {
         if (moment == nullptr || fields == nullptr)
             gwn::throw_invalid("Moment handle or output pointer is null.");
+        if (moment->real_bytes != 4)
+            gwn::throw_invalid("Moment holds Real = double: use the f64 download.");
         std::size_t const N = moment->node_count;
         int const F = gwn::moment_field_count(moment->order);
         int const RV = gwn::moment_record_vec4(moment->order);
@@ -445,8 +904,11 @@ int gwn_moment_download(gwn_moment_t moment, float *fields, hipStream_t stream) 
         std::vector<float> host(N * RV * 4);
         GWN_HIP(hipMemcpyAsync(host.data(), moment->records.ptr, host.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
         GWN_HIP(hipStreamSynchronize(stream));
+        // Field-SoA gwn_bvh_moment_node<Width, Order, float>: field f of slot s at [f * Width + s].
+        int const W = moment->width;
         for (std::size_t n = 0; n < N; ++n)
-            std::memcpy(fields + n * F * 4, host.data() + n * RV * 4 + 4, sizeof(float) * F * 4);
+            for (int f = 0; f < F; ++f)
+                std::memcpy(fields + (n * F + f) * W, host.data() + n * RV * 4 + 4 + f * 4, sizeof(float) * W);
     });
 }
 
@@ -483,6 +945,42 @@ int gwn_compute_winding_number_exact_batch_ex_f32(
         if (count == 0)
             return;
         gwn::winding_exact(*bvh, qx, qy, qz, out, count, precision, to_launch(c), stream);
+    });
+}
+
+// Real = double batches (gwn_query.cuh:140-180, 380-445 instantiated with double): one query per
+// lane, the exact sum and the Taylor traversal in double.
+int gwn_compute_winding_number_exact_batch_f64(
+    gwn_bvh_t bvh, const double *qx, const double *qy, const double *qz, double *out, uint64_t count,
+    const gwn_query_config *config, hipStream_t stream
+) {
+    return guarded("gwn_compute_winding_number_exact_batch", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, false);
+        require_bvh(bvh, 8);
+        require_query_storage(count, qx, qy, qz, out);
+        if (count == 0)
+            return;
+        gwn::winding_exact_double(*bvh, qx, qy, qz, out, count, stream);
+    });
+}
+
+int gwn_compute_winding_number_taylor_batch_f64(
+    gwn_bvh_t bvh, gwn_moment_t moment, const double *qx, const double *qy, const double *qz, double *out,
+    uint64_t count, double accuracy_scale, const gwn_query_config *config, hipStream_t stream
+) {
+    return guarded("gwn_compute_winding_number_taylor_batch", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, true);
+        require_bvh(bvh, 8);
+        require_stack(*bvh, c.stack_capacity);
+        require_moment(moment, *bvh);
+        if (!(accuracy_scale >= 0.0))
+            gwn::throw_invalid("Taylor accuracy scale must be nonnegative.");
+        require_query_storage(count, qx, qy, qz, out);
+        if (count == 0)
+            return;
+        gwn::winding_taylor_double(*bvh, *moment, qx, qy, qz, out, count, accuracy_scale, stream);
     });
 }
 
@@ -661,6 +1159,7 @@ static void require_antipodal(
 ) {
     if (geometry == nullptr || (geometry->triangle_count != 0 && geometry->i0.ptr == nullptr))
         gwn::throw_invalid("Geometry object contains no queryable data.");
+    require_geometry_real(geometry, 4);
     if (needs_bvh) {
         require_bvh(bvh);
         if (bvh->triangle_count != geometry->triangle_count)
@@ -732,6 +1231,7 @@ static void require_adjoint_geometry(gwn_geometry_t geometry, gwn_bvh_t bvh, gwn
                                      float const *gvx, float const *gvy, float const *gvz) {
     if (geometry == nullptr || (geometry->triangle_count != 0 && geometry->i0.ptr == nullptr))
         gwn::throw_invalid("Geometry object contains no queryable data.");
+    require_geometry_real(geometry, 4);
     require_bvh(bvh);
     if (bvh->triangle_count != geometry->triangle_count)
         gwn::throw_invalid("Geometry and BVH triangle counts differ.");

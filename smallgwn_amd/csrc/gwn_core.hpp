@@ -15,6 +15,7 @@
 #include <string>
 
 #include "../../include/gwn_hip.h"
+#include "../../include/gwn/detail/gwn_device_layout.hpp"
 
 namespace gwn {
 
@@ -98,96 +99,8 @@ template <class T> struct device_buffer {
     std::size_t bytes() const noexcept { return count * sizeof(T); }
 };
 
-// ---------------------------------------------------------------------------------------------
-// Canonical BVH layout (kept byte-compatible with the reference's gwn_bvh_child<float> and
-// gwn_bvh_node<4, float>: include/gwn/gwn_bvh.cuh:25-130).
-// ---------------------------------------------------------------------------------------------
-
-constexpr int k_offset_bits = 47;
-constexpr std::uint64_t k_offset_mask = (std::uint64_t(1) << k_offset_bits) - 1;
-constexpr std::uint64_t k_valid_mask = std::uint64_t(1) << k_offset_bits;
-constexpr int k_count_shift = k_offset_bits + 1;
-
-struct aabb {
-    float min_x, min_y, min_z, max_x, max_y, max_z;
-};
-
-struct bvh_child {
-    aabb bounds;
-    std::uint64_t reference;
-};
-static_assert(sizeof(bvh_child) == 32, "child must be one 32-byte record");
-
-struct bvh_node4 {
-    bvh_child children[4];
-};
-static_assert(sizeof(bvh_node4) == 128, "BVH4 node must be 128 bytes");
-
-__host__ __device__ constexpr std::uint64_t encode_internal(std::uint64_t node) {
-    return k_valid_mask | node;
-}
-__host__ __device__ constexpr std::uint64_t encode_leaf(std::uint64_t offset, std::uint64_t count) {
-    return k_valid_mask | offset | (count << k_count_shift);
-}
-__host__ __device__ constexpr bool ref_valid(std::uint64_t r) { return (r & k_valid_mask) != 0; }
-__host__ __device__ constexpr std::uint32_t ref_count(std::uint64_t r) {
-    return static_cast<std::uint32_t>(r >> k_count_shift);
-}
-__host__ __device__ constexpr bool ref_internal(std::uint64_t r) {
-    return ref_valid(r) && ref_count(r) == 0;
-}
-__host__ __device__ constexpr bool ref_leaf(std::uint64_t r) { return ref_valid(r) && ref_count(r) != 0; }
-__host__ __device__ constexpr std::uint64_t ref_offset(std::uint64_t r) { return r & k_offset_mask; }
-
-// Compact 32-bit child reference used by the traversal records of the moment object:
-// bit 31 set = leaf (sorted primitive offset in bits 0-30), otherwise an internal node index.
-constexpr std::uint32_t k_cref_invalid = 0xffffffffu;
-constexpr std::uint32_t k_cref_leaf = 0x80000000u;
-
-// Leaf-ordered triangle record: the three vertices, 36 bytes.  The reference's
-// (v0, e1, e2) form is recovered exactly as e1 = v1 - v0 (include/gwn/detail/
-// gwn_bvh_triangle_impl.cuh:56-59), and the exact vertices feed the FP64 near field.
-struct tri_record {
-    float v0x, v0y, v0z, v1x, v1y, v1z, v2x, v2y, v2z;
-};
-static_assert(sizeof(tri_record) == 36, "triangle record must be 36 bytes");
-
-// Number of float fields per child slot of a Taylor moment node (include/gwn/gwn_bvh.cuh:144-202).
-__host__ __device__ constexpr int moment_field_count(int order) {
-    return order == 0 ? 7 : (order == 1 ? 13 : 23);
-}
-// Traversal record: one float4 of compact child references, then the fields as float4 (slot-SoA),
-// padded to a whole number of 128-byte cache lines: 128 B / 256 B / 384 B for order 0 / 1 / 2.
-__host__ __device__ constexpr int moment_record_vec4(int order) {
-    return order == 0 ? 8 : (order == 1 ? 16 : 24);
-}
-
-// Work tallies of one query's traversal, for roofline accounting: child bounding-box tests and
-// triangle tests (point-triangle distances or ray-triangle intersections).
-struct traversal_work {
-    std::uint32_t boxes = 0, tris = 0;
-};
-
-// Five-comparator sorting network: orders t ascending and applies the same swaps to k.  Used
-// for near-first child ordering in traversals.
-template <class Key>
-__device__ inline void sort4(float (&t)[4], Key (&k)[4]) {
-    auto cs = [&](int a, int b) {
-        if (!(t[a] > t[b]))
-            return;
-        float const tt = t[a];
-        t[a] = t[b];
-        t[b] = tt;
-        Key const kk = k[a];
-        k[a] = k[b];
-        k[b] = kk;
-    };
-    cs(0, 1);
-    cs(2, 3);
-    cs(0, 2);
-    cs(1, 3);
-    cs(1, 2);
-}
+// The canonical BVH layout, triangle records and moment-record geometry are shared with the
+// public device header: include/gwn/detail/gwn_device_layout.hpp.
 
 // ---------------------------------------------------------------------------------------------
 // Cross-workgroup publication within one kernel.
@@ -222,9 +135,36 @@ std::uint64_t next_storage_generation() noexcept;
 
 } // namespace gwn
 
+namespace gwn {
+// Real = double storage (gwn_*_object<double, Index>).  The reference's gwn_aabb<double>,
+// gwn_bvh_child<double> and gwn_bvh_node<4, double> layouts (include/gwn/gwn_bvh.cuh:25-130): a
+// 48-byte box and the 64-bit child reference, 56 bytes per child.  Triangle records hold the three
+// vertices, as the float records do.
+struct aabb_d {
+    double min_x, min_y, min_z, max_x, max_y, max_z;
+};
+struct bvh_child_d {
+    aabb_d bounds;
+    std::uint64_t reference;
+};
+static_assert(sizeof(bvh_child_d) == 56, "double child must be 56 bytes");
+struct bvh_node4_d {
+    bvh_child_d children[4];
+};
+struct tri_record_d {
+    double v0x, v0y, v0z, v1x, v1y, v1z, v2x, v2y, v2z;
+};
+static_assert(sizeof(tri_record_d) == 72, "double triangle record must be 72 bytes");
+} // namespace gwn
+
 struct gwn_geometry_handle {
     gwn::device_buffer<float> x, y, z;
     gwn::device_buffer<std::uint32_t> i0, i1, i2;
+    // Real = double geometry: the positions as given (xd, yd, zd); x, y, z then hold them rounded to
+    // float, which is what the build's Morton codes and topology are computed from.
+    gwn::device_buffer<double> xd, yd, zd;
+    int real_bytes = 4;
+    int index_bytes = 4; // the caller's index type (indices are stored as uint32 either way)
     std::size_t vertex_count = 0;
     std::size_t triangle_count = 0;
     hipStream_t stream = nullptr;
@@ -243,6 +183,16 @@ struct gwn_bvh_handle {
     std::uint32_t internal_stack_bound = 0;
     std::uint32_t packed_stack_bound = 0;
     std::uint32_t max_depth = 0;
+    // Children per node: 2, 3 or 4.  Nodes are stored four-wide either way; a narrower tree's
+    // slots past its width stay empty, so every traversal and refit runs on it unchanged.
+    int width = 4;
+    // Real = double: the same tree with double boxes and records (nodes_d, triangles_d, root_d);
+    // nodes, triangles and root keep the float topology build's values.
+    int real_bytes = 4;
+    int index_bytes = 4;
+    gwn::device_buffer<gwn::bvh_node4_d> nodes_d;
+    gwn::device_buffer<gwn::tri_record_d> triangles_d;
+    gwn::bvh_child_d root_d{};
     std::uint64_t revision = 0;
     std::uint64_t storage_generation = 0;
     // Breadth-first levels of the wide tree: level d holds nodes [level_start[d], level_start[d + 1]);
@@ -273,6 +223,12 @@ struct gwn_batch_handle;
 
 struct gwn_moment_handle {
     int order = 0;
+    int width = 4;  // the width of the BVH the records were formed from
+    // Real = double moments: compact child references (4 per node) and the field-SoA coefficients
+    // (node, field, slot) in double.
+    int real_bytes = 4;
+    gwn::device_buffer<std::uint32_t> crefs_d;
+    gwn::device_buffer<double> fields_d;
     gwn::device_buffer<float4> records; // node_count * moment_record_vec4(order)
     gwn::device_buffer<float> slot_area; // node_count * 4: each child slot's total triangle area
     std::size_t node_count = 0;
@@ -295,13 +251,23 @@ void radix_sort_pairs_u32(
     std::size_t n, int begin_bit, int end_bit, std::uint32_t **keys_out,
     std::uint32_t **vals_out, hipStream_t stream
 );
-
-// Canonical BVH4 from the LBVH or H-PLOC (search radius 1-8) binary hierarchy.
+// Canonical wide BVH (width 2, 3 or 4) from the LBVH or H-PLOC (search radius 1-8) binary
+// hierarchy, stored four-wide.
 void build_bvh4(gwn_geometry_handle const &geometry, gwn_bvh_handle &staging, int method,
-                std::uint32_t hploc_radius, hipStream_t stream);
+                std::uint32_t hploc_radius, hipStream_t stream, int width = 4);
 // Algebraic boundary of the oriented triangle chain (directed rows in (min, max) vertex order).
 void build_boundary_chain(gwn_geometry_handle const &geometry, gwn_boundary_handle &staging, hipStream_t stream);
 void refit_moment(gwn_bvh_handle const &bvh, gwn_moment_handle &staging, hipStream_t stream);
+// Real = double (gwn_double.hip): float copies of a double geometry's positions; a BVH's double
+// records, boxes and root box from the double positions over its (float-built) tree; double
+// moments; the exact and Taylor winding batches in double.
+void narrow_positions(gwn_geometry_handle &geometry, hipStream_t stream);
+void refit_bounds_double(gwn_geometry_handle const &geometry, gwn_bvh_handle &bvh, hipStream_t stream);
+void refit_moment_double(gwn_bvh_handle const &bvh, gwn_moment_handle &staging, hipStream_t stream);
+void winding_exact_double(gwn_bvh_handle const &bvh, double const *qx, double const *qy, double const *qz,
+                          double *out, std::size_t n, hipStream_t stream);
+void winding_taylor_double(gwn_bvh_handle const &bvh, gwn_moment_handle const &m, double const *qx, double const *qy,
+                           double const *qz, double *out, std::size_t n, double accuracy_scale, hipStream_t stream);
 // One mesh of a batched Taylor query launch (device-resident table).
 struct batch_mesh {
     float4 const *records;       // moment traversal records (null for a leaf root)
@@ -395,9 +361,8 @@ void winding_taylor(
 );
 // Unsigned point-to-mesh distance with a culling band (queries Morton-sorted when cfg says so).
 // `nearest` (optional, one uint32 per query) receives the BVH record of the nearest triangle, or
-// k_nearest_none (distance is the band or zero: no gradient) or k_nearest_overflow.
-constexpr std::uint32_t k_nearest_none = 0xffffffffu;
-constexpr std::uint32_t k_nearest_overflow = 0xfffffffeu;
+// k_nearest_none (distance is the band or zero: no gradient) or k_nearest_overflow (both defined
+// with the device layout).
 void unsigned_distance(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz, float *out, std::size_t n,
     float culling_band, query_launch const &cfg, std::uint32_t *nearest, hipStream_t stream

@@ -30,7 +30,8 @@ EXPORTED_SYMBOLS = (
     "gwn_abi_version", "gwn_last_error_message",
     "gwn_geometry_create", "gwn_geometry_destroy", "gwn_upload_geometry_f32",
     "gwn_update_geometry_f32", "gwn_update_geometry_device_f32", "gwn_geometry_counts",
-    "gwn_bvh_create", "gwn_bvh_destroy", "gwn_build_bvh4_f32", "gwn_refit_bvh_f32", "gwn_bvh_get_info",
+    "gwn_bvh_create", "gwn_bvh_destroy", "gwn_build_bvh4_f32", "gwn_build_bvh_f32", "gwn_refit_bvh_f32",
+    "gwn_bvh_get_info",
     "gwn_bvh_download",
     "gwn_boundary_create", "gwn_boundary_destroy", "gwn_build_boundary_chain", "gwn_boundary_info",
     "gwn_boundary_download",
@@ -48,6 +49,13 @@ EXPORTED_SYMBOLS = (
     "gwn_batch_set_tuning", "gwn_batch_get_tuning", "gwn_batch_tune", "gwn_batch_capture", "gwn_batch_replay",
     "gwn_batch_status", "gwn_compute_unsigned_distance_work_f32", "gwn_compute_ray_first_hit_work_f32",
     "gwn_compute_query_order_f32", "gwn_radix_sort_pairs_u64", "gwn_radix_sort_pairs_u32",
+    "gwn_upload_geometry_f32_u64", "gwn_upload_geometry_f64", "gwn_upload_geometry_f64_u64",
+    "gwn_update_geometry_f64", "gwn_update_geometry_device_f64", "gwn_build_bvh_f64", "gwn_refit_bvh_f64",
+    "gwn_bvh_download_f64", "gwn_refit_bvh_moment_f64", "gwn_moment_download_f64",
+    "gwn_compute_winding_number_exact_batch_f64", "gwn_compute_winding_number_taylor_batch_f64",
+    "gwn_moment_info",
+    *(f"gwn_{obj}_{op}" for obj in ("geometry", "bvh", "moment", "boundary")
+      for op in ("stream", "set_stream", "clear", "get_accessor")),
 )
 
 
@@ -101,6 +109,7 @@ class BuildOptions:
 
     method: BuildMethod = BuildMethod.HPLOC
     hploc_search_radius: int = 8
+    width: int = 4  # the Width template argument of gwn_bvh_object<Width, ...>: 2, 3 or 4
 
 
 @dataclass(frozen=True)
@@ -137,6 +146,7 @@ def _load():
 
     L = ctypes.CDLL(LIB_PATH)
     vp, u64, i32, u32, f32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_uint32, ctypes.c_float
+    f64 = ctypes.c_double
     sigs = {
         "gwn_abi_version": [],
         "gwn_geometry_create": [vp],
@@ -148,6 +158,7 @@ def _load():
         "gwn_bvh_create": [vp],
         "gwn_bvh_destroy": [vp],
         "gwn_build_bvh4_f32": [vp, vp, i32, u32, vp],
+        "gwn_build_bvh_f32": [vp, vp, i32, i32, u32, vp],
         "gwn_refit_bvh_f32": [vp, vp, vp],
         "gwn_bvh_get_info": [vp, vp],
         "gwn_bvh_download": [vp, vp, vp, vp, vp],
@@ -196,7 +207,25 @@ def _load():
         "gwn_compute_query_order_f32": [vp, vp, vp, vp, vp, u64, vp],
         "gwn_radix_sort_pairs_u64": [vp, vp, u64, i32, i32, vp],
         "gwn_radix_sort_pairs_u32": [vp, vp, u64, i32, i32, vp],
+        "gwn_upload_geometry_f32_u64": [vp, vp, vp, vp, u64, vp, vp, vp, u64, vp],
+        "gwn_upload_geometry_f64": [vp, vp, vp, vp, u64, vp, vp, vp, u64, vp],
+        "gwn_upload_geometry_f64_u64": [vp, vp, vp, vp, u64, vp, vp, vp, u64, vp],
+        "gwn_update_geometry_f64": [vp, vp, vp, vp, u64, vp],
+        "gwn_update_geometry_device_f64": [vp, vp, vp, vp, u64, vp],
+        "gwn_build_bvh_f64": [vp, vp, i32, i32, u32, vp],
+        "gwn_refit_bvh_f64": [vp, vp, vp],
+        "gwn_bvh_download_f64": [vp, vp, vp, vp, vp],
+        "gwn_refit_bvh_moment_f64": [vp, vp, vp],
+        "gwn_moment_download_f64": [vp, vp, vp],
+        "gwn_compute_winding_number_exact_batch_f64": [vp, vp, vp, vp, vp, u64, vp, vp],
+        "gwn_compute_winding_number_taylor_batch_f64": [vp, vp, vp, vp, vp, vp, u64, f64, vp, vp],
+        "gwn_moment_info": [vp, vp, vp],
     }
+    for obj in ("geometry", "bvh", "moment", "boundary"):
+        sigs[f"gwn_{obj}_stream"] = [vp, vp]
+        sigs[f"gwn_{obj}_set_stream"] = [vp, vp]
+        sigs[f"gwn_{obj}_clear"] = [vp]
+        sigs[f"gwn_{obj}_get_accessor"] = [vp, vp]
     for name, args in sigs.items():
         fn = getattr(L, name)
         fn.argtypes = args
@@ -241,6 +270,41 @@ def _config(cfg: QueryConfig | None) -> _CConfig:
     return _CConfig(cfg.block_size, cfg.stack_capacity, cfg.sort_queries, cfg.lanes_per_point)
 
 
+class _CGeometryAccessor(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_void_p), ("y", ctypes.c_void_p), ("z", ctypes.c_void_p),
+                ("i0", ctypes.c_void_p), ("i1", ctypes.c_void_p), ("i2", ctypes.c_void_p),
+                ("vertex_count", ctypes.c_uint64), ("triangle_count", ctypes.c_uint64)]
+
+
+class _CBvhAccessor(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_void_p), ("node_count", ctypes.c_uint64),
+                ("primitive_indices", ctypes.c_void_p), ("triangles", ctypes.c_void_p),
+                ("triangle_count", ctypes.c_uint64), ("root_bounds", ctypes.c_float * 6),
+                ("root_reference", ctypes.c_uint64), ("internal_stack_bound", ctypes.c_uint32),
+                ("packed_stack_bound", ctypes.c_uint32), ("revision", ctypes.c_uint64)]
+
+
+class _CMomentAccessor(ctypes.Structure):
+    _fields_ = [("records", ctypes.c_void_p), ("node_count", ctypes.c_uint64), ("order", ctypes.c_int32),
+                ("record_bytes", ctypes.c_uint32), ("width", ctypes.c_int32), ("bvh_revision", ctypes.c_uint64)]
+
+
+class _CBoundaryAccessor(ctypes.Structure):
+    _fields_ = [("start", ctypes.c_void_p), ("end", ctypes.c_void_p), ("multiplicity", ctypes.c_void_p),
+                ("edge_count", ctypes.c_uint64), ("mesh_vertex_count", ctypes.c_uint64),
+                ("mesh_triangle_count", ctypes.c_uint64), ("is_built", ctypes.c_int32)]
+
+
+def _accessor_dict(acc) -> dict:
+    out = {}
+    for name, _ in acc._fields_:
+        v = getattr(acc, name)
+        if isinstance(v, ctypes.Array):
+            v = np.array(list(v), dtype=np.float32)
+        out[name] = v
+    return out
+
+
 class _Handle:
     _create = None
     _destroy = None
@@ -249,10 +313,33 @@ class _Handle:
         h = ctypes.c_void_p()
         _status(getattr(_lib, self._create)(ctypes.byref(h), *args)).raise_if_error()
         self._h = h
+        # The Real of the storage the last successful upload, build or refit produced (4 or 8
+        # bytes); it selects the _f32 or _f64 entry points.
+        self.real_bytes = 4
 
     @property
     def handle(self) -> ctypes.c_void_p:
         return self._h
+
+    # Stream binding and release (include/gwn/gwn_utils.cuh:59-80): storage is released on the
+    # bound stream; set_stream rebinds without synchronizing; clear releases and keeps the binding.
+    def stream(self) -> int | None:
+        """The bound stream's raw handle (None for the null stream)."""
+        out = ctypes.c_void_p()
+        _status(getattr(_lib, f"gwn_{self._kind}_stream")(self._h, ctypes.byref(out))).raise_if_error()
+        return out.value
+
+    def set_stream(self, stream) -> None:
+        _status(getattr(_lib, f"gwn_{self._kind}_set_stream")(self._h, _stream(stream))).raise_if_error()
+
+    def clear(self) -> None:
+        _status(getattr(_lib, f"gwn_{self._kind}_clear")(self._h)).raise_if_error()
+
+    def accessor(self) -> dict:
+        """Non-owning snapshot of the device storage (raw device addresses as ints)."""
+        acc = self._accessor_type()
+        _status(getattr(_lib, f"gwn_{self._kind}_get_accessor")(self._h, ctypes.byref(acc))).raise_if_error()
+        return _accessor_dict(acc)
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -269,6 +356,8 @@ class _Handle:
 class GeometryObject(_Handle):
     """gwn_geometry_object<float, uint32_t> (include/gwn/gwn_geometry.cuh:97-149)."""
 
+    _kind = "geometry"
+    _accessor_type = _CGeometryAccessor
     _create = "gwn_geometry_create"
     _destroy = "gwn_geometry_destroy"
 
@@ -291,6 +380,8 @@ class GeometryObject(_Handle):
 class BvhObject(_Handle):
     """gwn_bvh_object<4, float, uint32_t> (include/gwn/gwn_bvh.cuh:331-380)."""
 
+    _kind = "bvh"
+    _accessor_type = _CBvhAccessor
     _create = "gwn_bvh_create"
     _destroy = "gwn_bvh_destroy"
 
@@ -313,13 +404,15 @@ class BvhObject(_Handle):
         return self.info()["revision"] != 0
 
     def download(self, stream=None):
-        """Host copies: nodes (reference gwn_bvh_node<4,float> bytes), primitive order, records."""
+        """Host copies: nodes (reference gwn_bvh_node<Width, Real> bytes: 32 B per child for float,
+        56 B for double), primitive order, records (t, 9) in the BVH's Real."""
         inf = self.info()
         n, t = inf["node_count"], inf["triangle_count"]
-        nodes = np.zeros(n * 128, dtype=np.uint8)
+        dbl = self.real_bytes == 8
+        nodes = np.zeros(n * (56 if dbl else 32) * inf["width"], dtype=np.uint8)
         prim = np.zeros(t, dtype=np.uint32)
-        tris = np.zeros((t, 9), dtype=np.float32)
-        _status(_lib.gwn_bvh_download(
+        tris = np.zeros((t, 9), dtype=np.float64 if dbl else np.float32)
+        _status((_lib.gwn_bvh_download_f64 if dbl else _lib.gwn_bvh_download)(
             self._h, nodes.ctypes.data if n else None, prim.ctypes.data if t else None,
             tris.ctypes.data if t else None, _stream(stream),
         )).raise_if_error()
@@ -329,6 +422,8 @@ class BvhObject(_Handle):
 class MomentObject(_Handle):
     """gwn_bvh_moment_object<4, Order, float, uint32_t> (include/gwn/gwn_bvh.cuh:383-433)."""
 
+    _kind = "moment"
+    _accessor_type = _CMomentAccessor
     _create = "gwn_moment_create"
     _destroy = "gwn_moment_destroy"
 
@@ -337,17 +432,31 @@ class MomentObject(_Handle):
         self.order = int(order)
 
     def download(self, node_count: int, stream=None) -> np.ndarray:
-        """Coefficients in the reference field-SoA layout: array[node, field, slot]."""
+        """Coefficients in the reference field-SoA layout gwn_bvh_moment_node<Width, Order>:
+        array[node, field, slot] for the width of the BVH they were formed from."""
         F = {0: 7, 1: 13, 2: 23}[self.order]
-        out = np.zeros((node_count, F, 4), dtype=np.float32)
+        acc = self.accessor()
+        if node_count != acc["node_count"]:
+            raise ValueError(f"moment holds {acc['node_count']} nodes, not {node_count}")
+        out = np.zeros((node_count, F, acc["width"]), dtype=np.float32)
         if node_count:
             _status(_lib.gwn_moment_download(self._h, out.ctypes.data, _stream(stream))).raise_if_error()
+        return out
+
+    def download_f64(self, node_count: int, width: int, stream=None) -> np.ndarray:
+        """Real = double coefficients, array[node, field, slot] (Width = the BVH's width)."""
+        F = {0: 7, 1: 13, 2: 23}[self.order]
+        out = np.zeros((node_count, F, width), dtype=np.float64)
+        if node_count:
+            _status(_lib.gwn_moment_download_f64(self._h, out.ctypes.data, _stream(stream))).raise_if_error()
         return out
 
 
 class BoundaryChainObject(_Handle):
     """gwn_boundary_chain_object<uint32_t> (include/gwn/gwn_boundary.cuh:62-115)."""
 
+    _kind = "boundary"
+    _accessor_type = _CBoundaryAccessor
     _create = "gwn_boundary_create"
     _destroy = "gwn_boundary_destroy"
 
@@ -382,43 +491,72 @@ def _host(a, dtype):
     return np.ascontiguousarray(a, dtype=dtype)
 
 
-def gwn_upload_geometry(geometry: GeometryObject, x, y, z, i0, i1, i2, stream=None) -> Status:
-    """gwn_upload_geometry (include/gwn/gwn_geometry.cuh:66-71): host SoA arrays."""
-    x, y, z = _host(x, np.float32), _host(y, np.float32), _host(z, np.float32)
-    i0, i1, i2 = _host(i0, np.uint32), _host(i1, np.uint32), _host(i2, np.uint32)
+def gwn_upload_geometry(geometry: GeometryObject, x, y, z, i0, i1, i2, stream=None, real=np.float32,
+                        index=np.uint32) -> Status:
+    """gwn_upload_geometry (include/gwn/gwn_geometry.cuh:66-71): host SoA arrays.
+
+    ``real`` (np.float32 or np.float64) and ``index`` (np.uint32 or np.uint64) are the geometry
+    object's Real and Index template arguments; the arrays are converted to them.
+    """
+    real, index = np.dtype(real), np.dtype(index)
+    if real not in (np.float32, np.float64) or index not in (np.uint32, np.uint64):
+        return Status(Error.INVALID_ARGUMENT, "Real must be float32/float64 and Index uint32/uint64.")
+    x, y, z = _host(x, real), _host(y, real), _host(z, real)
+    i0, i1, i2 = _host(i0, index), _host(i1, index), _host(i2, index)
     if not (x.shape == y.shape == z.shape):
         return Status(Error.INVALID_ARGUMENT, "Vertex SoA spans must have identical lengths.")
     if not (i0.shape == i1.shape == i2.shape):
         return Status(Error.INVALID_ARGUMENT, "Triangle SoA spans must have identical lengths.")
     nv, nt = x.shape[0], i0.shape[0]
-    return _status(_lib.gwn_upload_geometry_f32(
+    fn = {(4, 4): _lib.gwn_upload_geometry_f32, (4, 8): _lib.gwn_upload_geometry_f32_u64,
+          (8, 4): _lib.gwn_upload_geometry_f64, (8, 8): _lib.gwn_upload_geometry_f64_u64}[
+        (real.itemsize, index.itemsize)]
+    st = _status(fn(
         geometry.handle, x.ctypes.data if nv else None, y.ctypes.data if nv else None,
         z.ctypes.data if nv else None, nv, i0.ctypes.data if nt else None,
         i1.ctypes.data if nt else None, i2.ctypes.data if nt else None, nt, _stream(stream),
     ))
+    if st.is_ok():
+        geometry.real_bytes = real.itemsize
+    return st
 
 
 def gwn_update_geometry(geometry: GeometryObject, x, y, z, stream=None) -> Status:
     """gwn_update_geometry (include/gwn/gwn_geometry.cuh:84-90); device tensors are accepted too."""
+    dbl = geometry.real_bytes == 8
     if hasattr(x, "data_ptr") and getattr(x, "is_cuda", False):
         if not (x.numel() == y.numel() == z.numel()):
             return Status(Error.INVALID_ARGUMENT, "Vertex SoA spans must have identical lengths.")
-        return _status(_lib.gwn_update_geometry_device_f32(
-            geometry.handle, _dptr(x), _dptr(y), _dptr(z), x.numel(), _stream(stream)))
-    x, y, z = _host(x, np.float32), _host(y, np.float32), _host(z, np.float32)
+        fn = _lib.gwn_update_geometry_device_f64 if dbl else _lib.gwn_update_geometry_device_f32
+        return _status(fn(geometry.handle, _dptr(x), _dptr(y), _dptr(z), x.numel(), _stream(stream)))
+    real = np.float64 if dbl else np.float32
+    x, y, z = _host(x, real), _host(y, real), _host(z, real)
     if not (x.shape == y.shape == z.shape):
         return Status(Error.INVALID_ARGUMENT, "Vertex SoA spans must have identical lengths.")
     n = x.shape[0]
-    return _status(_lib.gwn_update_geometry_f32(
+    return _status((_lib.gwn_update_geometry_f64 if dbl else _lib.gwn_update_geometry_f32)(
         geometry.handle, x.ctypes.data if n else None, y.ctypes.data if n else None,
         z.ctypes.data if n else None, n, _stream(stream)))
 
 
 def gwn_build_bvh(geometry: GeometryObject, bvh: BvhObject, options: BuildOptions = BuildOptions(),
                   stream=None) -> Status:
-    """gwn_build_bvh<4, float, uint32_t> (include/gwn/gwn_bvh_build.cuh:34-39)."""
-    return _status(_lib.gwn_build_bvh4_f32(
-        geometry.handle, bvh.handle, int(options.method), int(options.hploc_search_radius), _stream(stream)))
+    """gwn_build_bvh<Width, Real, Index> (include/gwn/gwn_bvh_build.cuh:34-39), Width = options.width,
+    Real = the geometry's."""
+    if geometry.real_bytes == 8:
+        st = _status(_lib.gwn_build_bvh_f64(
+            geometry.handle, bvh.handle, int(options.width), int(options.method),
+            int(options.hploc_search_radius), _stream(stream)))
+    elif options.width == 4:
+        st = _status(_lib.gwn_build_bvh4_f32(
+            geometry.handle, bvh.handle, int(options.method), int(options.hploc_search_radius), _stream(stream)))
+    else:
+        st = _status(_lib.gwn_build_bvh_f32(
+            geometry.handle, bvh.handle, int(options.width), int(options.method),
+            int(options.hploc_search_radius), _stream(stream)))
+    if st.is_ok():
+        bvh.real_bytes = geometry.real_bytes
+    return st
 
 
 def gwn_build_boundary_chain(geometry: GeometryObject, boundary: BoundaryChainObject, stream=None) -> Status:
@@ -429,12 +567,17 @@ def gwn_build_boundary_chain(geometry: GeometryObject, boundary: BoundaryChainOb
 def gwn_refit_bvh(geometry: GeometryObject, bvh: BvhObject, stream=None) -> Status:
     """gwn_refit_bvh (include/gwn/gwn_bvh_refit.cuh:25-29): records and boxes from the updated
     geometry, same topology; the BVH gets a new revision, so moments need a refit."""
-    return _status(_lib.gwn_refit_bvh_f32(geometry.handle, bvh.handle, _stream(stream)))
+    fn = _lib.gwn_refit_bvh_f64 if bvh.real_bytes == 8 else _lib.gwn_refit_bvh_f32
+    return _status(fn(geometry.handle, bvh.handle, _stream(stream)))
 
 
 def gwn_refit_bvh_moment(bvh: BvhObject, moment: MomentObject, stream=None) -> Status:
     """gwn_refit_bvh_moment<Order> (include/gwn/gwn_bvh_refit.cuh:36-51)."""
-    return _status(_lib.gwn_refit_bvh_moment_f32(bvh.handle, moment.handle, _stream(stream)))
+    fn = _lib.gwn_refit_bvh_moment_f64 if bvh.real_bytes == 8 else _lib.gwn_refit_bvh_moment_f32
+    st = _status(fn(bvh.handle, moment.handle, _stream(stream)))
+    if st.is_ok():
+        moment.real_bytes = bvh.real_bytes
+    return st
 
 
 def _numel(t) -> int:
@@ -451,14 +594,26 @@ def _check_soa(qx, qy, qz, outs) -> Status | None:
     return None
 
 
+def _check_real(real_bytes: int, *tensors) -> Status | None:
+    """Device spans must hold the object's Real (the kernels read them with that width)."""
+    for t in tensors:
+        if t is not None and getattr(t, "element_size", None) is not None and t.numel() \
+                and (t.element_size() != real_bytes or not t.is_floating_point()):
+            return Status(Error.INVALID_ARGUMENT,
+                          f"Query/output spans must hold the objects' Real ({8 * real_bytes}-bit floats).")
+    return None
+
+
 def gwn_compute_winding_number_exact_batch(bvh: BvhObject, qx, qy, qz, out,
                                            config: QueryConfig | None = None, stream=None) -> Status:
-    """gwn_compute_winding_number_exact_batch (include/gwn/gwn_query.cuh:308-346)."""
-    bad = _check_soa(qx, qy, qz, [out])
+    """gwn_compute_winding_number_exact_batch (include/gwn/gwn_query.cuh:308-346); Real = the BVH's."""
+    bad = _check_soa(qx, qy, qz, [out]) or _check_real(bvh.real_bytes, qx, qy, qz, out)
     if bad:
         return bad
     cfg = _config(config)
-    return _status(_lib.gwn_compute_winding_number_exact_batch_f32(
+    fn = (_lib.gwn_compute_winding_number_exact_batch_f64 if bvh.real_bytes == 8
+          else _lib.gwn_compute_winding_number_exact_batch_f32)
+    return _status(fn(
         bvh.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx), ctypes.byref(cfg), _stream(stream)))
 
 
@@ -485,12 +640,19 @@ def gwn_compute_winding_number_taylor_batch(bvh: BvhObject, moment: MomentObject
     (optional uint32 device array of 3 * count) receives per-query child tests, accepted
     expansions and near-field triangle evaluations.
     """
-    bad = _check_soa(qx, qy, qz, [out])
+    bad = _check_soa(qx, qy, qz, [out]) or _check_real(bvh.real_bytes, qx, qy, qz, out)
     if bad:
         return bad
     if order is not None and _numel(order) != _numel(qx):
         return Status(Error.INVALID_ARGUMENT, "Query order span size must match query count.")
     cfg = _config(config)
+    if bvh.real_bytes == 8:
+        if precision != Precision.PLAIN or counters is not None or order is not None:
+            return Status(Error.INVALID_ARGUMENT,
+                          "Real = double Taylor batches take no precision mode, counters or order.")
+        return _status(_lib.gwn_compute_winding_number_taylor_batch_f64(
+            bvh.handle, moment.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx),
+            float(accuracy_scale), ctypes.byref(cfg), _stream(stream)))
     if precision == Precision.PLAIN and counters is None and order is None:
         return _status(_lib.gwn_compute_winding_number_taylor_batch_f32(
             bvh.handle, moment.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx),

@@ -157,12 +157,13 @@ def check_bvh4(nodes_raw, root_reference: int, primitive_indices, triangles, mes
     assert sp == packed_stack_bound
 
 
-def check_collapse_rule(nodes_raw, binary_nodes, binary_bounds, binary_root: int = 0):
+def check_collapse_rule(nodes_raw, binary_nodes, binary_bounds, binary_root: int = 0, width: int = 4):
     """Largest-half-area expansion (include/gwn/detail/gwn_bvh_build_collapse.cuh:170-211).
 
     For every wide node, the binary nodes opened inside it all have half-area >= every
     internal binary node left on its frontier (ties broken toward the smaller index),
-    and a node with fewer than four children has no internal frontier node left to open.
+    and a node with fewer than `width` children has no internal frontier node left to open.
+    Slots past the node's children (and past the width) are empty.
     """
     nodes = as_nodes(nodes_raw)
     bn = np.asarray(binary_nodes)
@@ -180,7 +181,7 @@ def check_collapse_rule(nodes_raw, binary_nodes, binary_bounds, binary_root: int
     def frontier(broot):
         cands = [(int(bn[broot][0]), int(bn[broot][2])), (int(bn[broot][1]), int(bn[broot][3]))]
         opened = [broot]
-        while len(cands) < 4:
+        while len(cands) < width:
             best = None
             for k, (idx, kind) in enumerate(cands):
                 if kind != 1:

@@ -1208,3 +1208,158 @@ def test_adjoint_magnitude_single_terms_and_additivity(orc):
     assert np.allclose(mv2, np.concatenate([m[0][0], m[1][0]], 1), rtol=1e-12, atol=0)
     assert np.allclose(mq2, m[0][1] + m[1][1], rtol=1e-12, atol=0)
     assert np.allclose(orc.adjoint_magnitude_f64(two, *q2, -3 * u2)[0], 3 * mv2, rtol=1e-12, atol=0)
+
+
+# ---- widths 2 and 3 (gwn_bvh_object<Width>, tests/integration_gwn_builder.cu:256-298) ----------
+
+@pytest.mark.parametrize("method", ["lbvh", "hploc"])
+def test_width_two_tree_is_the_binary_tree(orc, method):
+    """Width 2 opens no binary node: every wide node is one binary node with its two children,
+    so the node count is T - 1 and the children are the binary children in (left, right) order."""
+    mesh = synth.torus(16, 9)
+    bvh = orc.build_bvh4(mesh, method, 4, width=2)
+    T = mesh.triangle_count
+    assert bvh.node_count == T - 1
+    helpers.check_bvh4(bvh.nodes, bvh.root_reference, bvh.primitive_indices, bvh.triangles, mesh,
+                       bvh.internal_stack_bound, bvh.packed_stack_bound)
+    helpers.check_collapse_rule(bvh.nodes, bvh.binary_nodes, bvh.binary_bounds, bvh.binary_root, width=2)
+    nodes = helpers.as_nodes(bvh.nodes)
+    for n in range(bvh.node_count):
+        refs = [int(c["reference"]) for c in nodes[n]["children"]]
+        assert all(r & helpers.VALID_BIT for r in refs[:2]) and not any(r & helpers.VALID_BIT for r in refs[2:])
+    # A binary tree's stack bounds: one deferred child per level.
+    assert bvh.packed_stack_bound >= bvh.internal_stack_bound
+
+
+@pytest.mark.parametrize("width", [2, 3])
+@pytest.mark.parametrize("method", ["lbvh", "hploc"])
+def test_narrow_trees_keep_leaves_and_expansion_rule(orc, width, method):
+    full = synth.displaced_sphere(24, 12)
+    half = full.triangle_count // 2  # an open cap: its summed normal is far from zero
+    mesh = synth._mesh_from_arrays(np.stack([full.vx, full.vy, full.vz], 1),
+                                   np.stack([full.i0, full.i1, full.i2], 1)[:half])
+    wide = orc.build_bvh4(mesh, method, 8, width=4)
+    bvh = orc.build_bvh4(mesh, method, 8, width=width)
+    # Same sort and binary hierarchy; only the collapse differs.
+    assert np.array_equal(bvh.primitive_indices, wide.primitive_indices)
+    assert np.array_equal(bvh.binary_nodes, wide.binary_nodes)
+    helpers.check_bvh4(bvh.nodes, bvh.root_reference, bvh.primitive_indices, bvh.triangles, mesh,
+                       bvh.internal_stack_bound, bvh.packed_stack_bound)
+    helpers.check_collapse_rule(bvh.nodes, bvh.binary_nodes, bvh.binary_bounds, bvh.binary_root, width=width)
+    # Every node of a width-w tree opens w - 1 binary nodes unless its frontier is all leaves.
+    assert bvh.node_count > wide.node_count
+    # The summed area-weighted normal N (fields 4-6, additive) over the root's slots is the whole
+    # mesh's in every width (gwn_bvh.cuh:144-152).
+    for order in (0, 1, 2):
+        m = orc.refit_moment(bvh, order, dtype=np.float64)
+        mw = orc.refit_moment(wide, order, dtype=np.float64)
+        n_root, n_wide = m[0, 4:7].sum(axis=1), mw[0, 4:7].sum(axis=1)
+        assert np.linalg.norm(n_wide) > 1.0
+        assert np.allclose(n_root, n_wide, rtol=1e-12, atol=1e-12)
+
+
+# ---- H-PLOC on generic inputs, restated from its definition -------------------------------------
+
+def _hploc_by_definition(codes, lo, hi, radius, threshold=16):
+    """H-PLOC (Benthin et al. 2024, hploc.cuh:104-478) restated top-down: the implicit hierarchy is
+    the binary radix tree of the sorted (code, position) keys (helpers.lbvh_topdown, independent of
+    the oracle's bottom-up climb over range boundaries); every subtree, children first, gets the
+    cluster list of its children's lists concatenated (a leaf is a one-cluster list), and a subtree
+    of more than `threshold` leaves, or the whole tree, runs PLOC on it until at most `threshold`
+    clusters (one for the whole tree) remain.  PLOC's neighbour key is the device's packed key
+    (hploc.cuh:195-241): union half-area bits << 1 with the low four bits replaced by
+    (offset - 1) << 1 and the direction parity; mutual pairs merge into the lower position.
+    Returns the merges as ordered (left leaf set, right leaf set)."""
+    T = len(codes)
+    tree = helpers.lbvh_topdown(codes)
+
+    def key(a, b):
+        area = helpers.half_area32(np.concatenate([np.minimum(a[1], b[1]), np.maximum(a[2], b[2])]))
+        return (int(np.float32(area).view(np.uint32)) << 1) & 0xFFFFFFF0
+
+    merges = []
+
+    def ploc(lst, final):
+        limit = 1 if final else threshold
+        while len(lst) > limit:
+            n = len(lst)
+            nn = []
+            for l in range(n):
+                best = None
+                for r in range(1, radius + 1):
+                    if l + r < n:
+                        k = key(lst[l], lst[l + r]) | ((r - 1) << 1) | (l & 1)
+                        best = k if best is None else min(best, k)
+                    if l >= r:
+                        k = key(lst[l - r], lst[l]) | ((r - 1) << 1) | ((l & 1) ^ 1)
+                        best = k if best is None else min(best, k)
+                off = ((best & 15) >> 1) + 1
+                nn.append(l + off if ((best ^ l) & 1) == 0 else l - off)
+            out = []
+            for l in range(n):
+                j = nn[l]
+                if nn[j] == l:
+                    if l < j:
+                        a, b = lst[l], lst[j]
+                        merges.append((tuple(a[0]), tuple(b[0])))
+                        out.append((sorted(a[0] + b[0]), np.minimum(a[1], b[1]), np.maximum(a[2], b[2])))
+                else:
+                    out.append(lst[l])
+            lst = out
+        return lst
+
+    def leaf(i):
+        return ([i], lo[i].astype(np.float32), hi[i].astype(np.float32))
+
+    lists, sizes = {}, {}
+    order = []
+    stack = [0]
+    while stack:  # post-order: children before parents
+        n = stack.pop()
+        order.append(n)
+        left, left_leaf, right, right_leaf = tree[n]
+        if not left_leaf:
+            stack.append(left)
+        if not right_leaf:
+            stack.append(right)
+    for n in reversed(order):
+        left, left_leaf, right, right_leaf = tree[n]
+        ll = [leaf(left)] if left_leaf else lists[left]
+        rl = [leaf(right)] if right_leaf else lists[right]
+        size = (1 if left_leaf else sizes[left]) + (1 if right_leaf else sizes[right])
+        lst = ll + rl
+        if size > threshold or n == 0:
+            lst = ploc(lst, final=n == 0)
+        lists[n], sizes[n] = lst, size
+    assert sizes[0] == T and len(lists[0]) == 1
+    return merges
+
+
+@pytest.mark.parametrize("radius", [1, 4, 8])
+@pytest.mark.parametrize("name", ["displaced", "soup", "torus"])
+def test_hploc_merges_equal_the_restated_definition(orc, name, radius):
+    """Every merge of the oracle's H-PLOC on real meshes (thousands of triangles, so ranges above the
+    16-cluster threshold run PLOC repeatedly) equals the top-down restatement's, as ordered
+    (lower-position, higher-position) cluster pairs, and each internal node's id is the larger of
+    its two clusters' smallest sorted leaves minus one."""
+    mesh = {
+        "displaced": lambda: synth.displaced_sphere(40, 20),
+        "soup": lambda: synth.open_soup(36, 18, seed=7),
+        "torus": lambda: synth.torus(28, 14),
+    }[name]()
+    bvh = orc.build_bvh4(mesh, method="hploc", radius=radius)
+    T = mesh.triangle_count
+    rec = np.asarray(bvh.triangles, np.float32)
+    lo = np.minimum(np.minimum(rec[:, 0:3], rec[:, 3:6]), rec[:, 6:9])
+    hi = np.maximum(np.maximum(rec[:, 0:3], rec[:, 3:6]), rec[:, 6:9])
+    want = _hploc_by_definition(bvh.sorted_morton, lo, hi, radius)
+    assert len(want) == T - 1
+    sets = _leaf_sets(bvh)
+    got = {}
+    for n in sets:
+        (l, lk), (r, rk) = _children(bvh, n)
+        left = (l,) if lk == 2 else tuple(sets[l])
+        right = (r,) if rk == 2 else tuple(sets[r])
+        got[(left, right)] = n
+        assert n == max(left[0], right[0]) - 1
+    assert set(got) == {(tuple(sorted(a)), tuple(sorted(b))) for a, b in want}
