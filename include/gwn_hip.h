@@ -337,6 +337,19 @@ int gwn_compute_winding_number_taylor_batch_ex_f32(
     float *out, uint64_t count, float accuracy_scale, int32_t precision,
     const gwn_query_config *config, const uint32_t *order, uint32_t *counters, hipStream_t stream);
 
+/* Allocation-free Taylor winding batch (no reference counterpart).  For callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Same arguments, validation and results as
+ * gwn_compute_winding_number_taylor_batch_ex_f32 without an order or counters.  The Morton
+ * presort's scratch lives in a static per-GPU device array (about 16.3 MiB, held until the
+ * process ends) instead of stream-ordered allocations, so count is limited to 1048576 queries
+ * (larger batches return GWN_ERROR_INVALID_ARGUMENT; split them).  The array is shared by every
+ * call on one GPU: calls that overlap on one GPU (other streams or host threads) race. */
+int gwn_compute_winding_number_taylor_batch_noalloc_f32(
+    gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz,
+    float *out, uint64_t count, float accuracy_scale, int32_t precision,
+    const gwn_query_config *config, hipStream_t stream);
+
 /* ---- gradient batch: replaces gwn_compute_winding_gradient_taylor_batch<Order, Config>
  *      (include/gwn/gwn_query.cuh:684-756, detail/gwn_query_winding_impl.cuh:20-99, 217-325).
  *      Analytic gradient of the Taylor winding number, written to gx, gy, gz. ---- */

@@ -1034,6 +1034,28 @@ int gwn_compute_winding_number_taylor_batch_ex_f32(
     });
 }
 
+int gwn_compute_winding_number_taylor_batch_noalloc_f32(
+    gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz, float *out,
+    uint64_t count, float accuracy_scale, int32_t precision, const gwn_query_config *config, hipStream_t stream
+) {
+    return guarded("gwn_compute_winding_number_taylor_batch_noalloc", [&] {
+        gwn_query_config const c = resolve_config(config);
+        validate_config(c, true);
+        require_bvh(bvh);
+        require_stack(*bvh, c.stack_capacity);
+        require_moment(moment, *bvh);
+        if (!(accuracy_scale >= 0.0f))
+            gwn::throw_invalid("Taylor accuracy scale must be nonnegative.");
+        if (precision != GWN_PRECISION_PLAIN && precision != GWN_PRECISION_SPLIT)
+            gwn::throw_invalid("Taylor winding supports the plain and split precision modes.");
+        require_query_storage(count, qx, qy, qz, out);
+        if (count > gwn::k_noalloc_max_queries)
+            gwn::throw_invalid("noalloc Taylor batch: query count exceeds the static scratch capacity (1048576 queries).");
+        gwn::winding_taylor_noalloc(*bvh, *moment, qx, qy, qz, out, count, accuracy_scale, precision, to_launch(c),
+                                    stream);
+    });
+}
+
 int gwn_compute_winding_gradient_taylor_batch_f32(
     gwn_bvh_t bvh, gwn_moment_t moment, const float *qx, const float *qy, const float *qz, float *gx,
     float *gy, float *gz, uint64_t count, float accuracy_scale, const gwn_query_config *config,

@@ -251,6 +251,17 @@ void radix_sort_pairs_u32(
     std::size_t n, int begin_bit, int end_bit, std::uint32_t **keys_out,
     std::uint32_t **vals_out, hipStream_t stream
 );
+// Device words of scratch a sort of n pairs on [begin_bit, end_bit) uses (digit bases and the
+// per-tile count table).
+std::size_t radix_sort_workspace_words(std::size_t n, int begin_bit, int end_bit);
+// The same sort with its scratch in `workspace` (radix_sort_workspace_words words, caller-owned):
+// allocates nothing.
+void radix_sort_pairs_u32_ws(
+    std::uint32_t *keys, std::uint32_t *keys_alt, std::uint32_t *vals, std::uint32_t *vals_alt,
+    std::size_t n, int begin_bit, int end_bit, std::uint32_t *workspace, std::uint32_t **keys_out,
+    std::uint32_t **vals_out, hipStream_t stream
+);
+
 // Canonical wide BVH (width 2, 3 or 4) from the LBVH or H-PLOC (search radius 1-8) binary
 // hierarchy, stored four-wide.
 void build_bvh4(gwn_geometry_handle const &geometry, gwn_bvh_handle &staging, int method,
@@ -423,6 +434,15 @@ void winding_gradient_antipodal(
 void query_order(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz,
     std::uint32_t *order, std::size_t n, hipStream_t stream
+);
+// Taylor winding batch (no gradient) with the Morton presort's scratch in the library's static
+// per-device scratch array instead of stream-ordered allocations; at most
+// k_noalloc_max_queries queries.  Results equal winding_taylor's.
+constexpr std::size_t k_noalloc_max_queries = std::size_t(1) << 20;
+void winding_taylor_noalloc(
+    gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy,
+    float const *qz, float *out, std::size_t n, float accuracy_scale, int precision, query_launch const &cfg,
+    hipStream_t stream
 );
 
 // Zero-fill helper (a kernel, not hipMemsetAsync, so that captured graphs replay it safely).

@@ -262,10 +262,19 @@ __global__ __launch_bounds__(k_sort_block) void scatter_kernel(
     }
 }
 
+// Scratch of one sort: digit bases [passes][256], then the [256][tiles] tile table reused by
+// every pass.
+std::size_t sort_base_words(int begin_bit, int end_bit) {
+    return static_cast<std::size_t>((end_bit - begin_bit + 7) / 8) * k_radix;
+}
+
+std::size_t sort_table_words(std::size_t n) { return (n + k_tile - 1) / k_tile * k_radix; }
+
 template <class Key>
 void radix_sort_pairs(
     Key *keys, Key *keys_alt, std::uint32_t *vals, std::uint32_t *vals_alt, std::size_t n,
-    int begin_bit, int end_bit, Key **keys_out, std::uint32_t **vals_out, hipStream_t stream
+    int begin_bit, int end_bit, std::uint32_t *workspace, Key **keys_out, std::uint32_t **vals_out,
+    hipStream_t stream
 ) {
     *keys_out = keys;
     *vals_out = vals;
@@ -277,13 +286,16 @@ void radix_sort_pairs(
     if (passes > k_max_passes)
         throw_invalid("radix sort: bit range exceeds the key width.");
     std::size_t const tiles = (n + k_tile - 1) / k_tile;
-    // Scratch: digit bases [passes][256], then the [256][tiles] tile table reused by every pass.
-    std::size_t const table_words = tiles * k_radix;
-    std::size_t const base_words = static_cast<std::size_t>(passes) * k_radix;
-    device_buffer<std::uint32_t> scratch(base_words + table_words, stream);
-    fill_u32(scratch.ptr, 0u, base_words, stream);
-    std::uint32_t *hist = scratch.ptr;
-    std::uint32_t *table = scratch.ptr + base_words;
+    std::size_t const base_words = sort_base_words(begin_bit, end_bit);
+    device_buffer<std::uint32_t> own;
+    if (workspace == nullptr) {
+        own.allocate(base_words + sort_table_words(n), stream);
+        workspace = own.ptr;
+    }
+    // The digit bases accumulate atomically: clear them on every call (a fill kernel).
+    fill_u32(workspace, 0u, base_words, stream);
+    std::uint32_t *hist = workspace;
+    std::uint32_t *table = workspace + base_words;
 
     unsigned const hist_grid = static_cast<unsigned>(tiles < 2048 ? tiles : 2048);
     histogram_kernel<Key><<<hist_grid, k_sort_block, 0, stream>>>(keys, n, begin_bit, end_bit, passes, hist);
@@ -328,7 +340,8 @@ void radix_sort_pairs_u64(
     std::size_t n, int begin_bit, int end_bit, std::uint64_t **keys_out, std::uint32_t **vals_out,
     hipStream_t stream
 ) {
-    radix_sort_pairs<std::uint64_t>(keys, keys_alt, vals, vals_alt, n, begin_bit, end_bit, keys_out, vals_out, stream);
+    radix_sort_pairs<std::uint64_t>(keys, keys_alt, vals, vals_alt, n, begin_bit, end_bit, nullptr, keys_out, vals_out,
+                                    stream);
 }
 
 void radix_sort_pairs_u32(
@@ -336,7 +349,23 @@ void radix_sort_pairs_u32(
     std::size_t n, int begin_bit, int end_bit, std::uint32_t **keys_out, std::uint32_t **vals_out,
     hipStream_t stream
 ) {
-    radix_sort_pairs<std::uint32_t>(keys, keys_alt, vals, vals_alt, n, begin_bit, end_bit, keys_out, vals_out, stream);
+    radix_sort_pairs<std::uint32_t>(keys, keys_alt, vals, vals_alt, n, begin_bit, end_bit, nullptr, keys_out, vals_out,
+                                    stream);
+}
+
+std::size_t radix_sort_workspace_words(std::size_t n, int begin_bit, int end_bit) {
+    return sort_base_words(begin_bit, end_bit) + sort_table_words(n);
+}
+
+void radix_sort_pairs_u32_ws(
+    std::uint32_t *keys, std::uint32_t *keys_alt, std::uint32_t *vals, std::uint32_t *vals_alt,
+    std::size_t n, int begin_bit, int end_bit, std::uint32_t *workspace, std::uint32_t **keys_out,
+    std::uint32_t **vals_out, hipStream_t stream
+) {
+    if (workspace == nullptr)
+        throw_internal("radix sort: null workspace.");
+    radix_sort_pairs<std::uint32_t>(keys, keys_alt, vals, vals_alt, n, begin_bit, end_bit, workspace, keys_out,
+                                    vals_out, stream);
 }
 
 } // namespace gwn

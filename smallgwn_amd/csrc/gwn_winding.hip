@@ -21,6 +21,8 @@
 // Products are contracted into FMAs only within one source expression (contract(on), not the
 // default across statements), so every kernel that evaluates a term -- per-lane, wave, gradient,
 // batched -- rounds it identically and their results agree bit for bit.
+#include <mutex>
+
 #include "gwn_core.hpp"
 #include "../../include/gwn/detail/gwn_device_winding.hpp"
 
@@ -958,20 +960,42 @@ void winding_exact(
     GWN_LAUNCH_CHECK();
 }
 
+namespace {
+
+// Device words the Morton presort of n queries needs besides the order itself: keys, their
+// alternate buffer, the alternate id buffer and the sort's own workspace.
+std::size_t query_order_workspace_words(std::size_t n) {
+    return 3 * n + radix_sort_workspace_words(n, 0, 30);
+}
+
+// Morton presort with caller-given scratch (query_order_workspace_words(n) words): allocates
+// nothing.
+void query_order_ws(
+    gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz,
+    std::uint32_t *order, std::size_t n, std::uint32_t *ws, hipStream_t stream
+) {
+    std::uint32_t *const keys = ws;
+    std::uint32_t *const keys_alt = ws + n;
+    std::uint32_t *const ids_alt = ws + 2 * n;
+    query_morton_kernel<<<grid_for(n, 256), 256, 0, stream>>>(qx, qy, qz, n, bvh.root.bounds, keys, order);
+    GWN_LAUNCH_CHECK();
+    std::uint32_t *sk = nullptr;
+    std::uint32_t *sorted = nullptr;
+    radix_sort_pairs_u32_ws(keys, keys_alt, order, ids_alt, n, 0, 30, ws + 3 * n, &sk, &sorted, stream);
+    if (sorted != order)
+        GWN_HIP(hipMemcpyAsync(order, sorted, n * sizeof(std::uint32_t), hipMemcpyDeviceToDevice, stream));
+}
+
+} // namespace
+
 void query_order(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz,
     std::uint32_t *order, std::size_t n, hipStream_t stream
 ) {
     if (n == 0)
         return;
-    device_buffer<std::uint32_t> keys(n, stream), keys_alt(n, stream), ids_alt(n, stream);
-    query_morton_kernel<<<grid_for(n, 256), 256, 0, stream>>>(qx, qy, qz, n, bvh.root.bounds, keys.ptr, order);
-    GWN_LAUNCH_CHECK();
-    std::uint32_t *sk = nullptr;
-    std::uint32_t *sorted = nullptr;
-    radix_sort_pairs_u32(keys.ptr, keys_alt.ptr, order, ids_alt.ptr, n, 0, 30, &sk, &sorted, stream);
-    if (sorted != order)
-        GWN_HIP(hipMemcpyAsync(order, sorted, n * sizeof(std::uint32_t), hipMemcpyDeviceToDevice, stream));
+    device_buffer<std::uint32_t> ws(query_order_workspace_words(n), stream);
+    query_order_ws(bvh, qx, qy, qz, order, n, ws.ptr, stream);
 }
 
 void winding_taylor(
@@ -1031,6 +1055,76 @@ void winding_taylor(
     }
 #undef GWN_TAYLOR_ORDERS
 #undef GWN_TAYLOR_CASE
+}
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free Taylor batch.  Its only scratch is the Morton presort's (the visiting order,
+// keys, alternate buffers and the sort's digit table), which lives in one static device array per
+// GPU, sized for k_noalloc_max_queries queries.  Host code never passes the array's name to a
+// kernel: that name is a host-side stand-in.  The array's device address is looked up once per
+// device with hipGetSymbolAddress (with that device current) and that pointer is used instead.
+// Every call on one GPU shares the array and leaves the last call's contents in it; the sort
+// clears its digit bases itself, and everything else is fully overwritten before it is read.
+// Calls that overlap on one GPU (different streams, or host threads) race on the array.
+// ---------------------------------------------------------------------------------------------
+
+namespace {
+
+constexpr std::size_t k_noalloc_order_words = k_noalloc_max_queries;
+constexpr std::size_t k_noalloc_sort_words =
+    3 * k_noalloc_max_queries + 4 * 256 + (k_noalloc_max_queries + 4095) / 4096 * 256;
+constexpr std::size_t k_noalloc_scratch_words = k_noalloc_order_words + k_noalloc_sort_words;
+
+__device__ __align__(256) std::uint32_t scratch[k_noalloc_scratch_words];
+
+constexpr int k_max_devices = 64;
+std::mutex g_scratch_mutex;
+std::uint32_t *g_scratch_ptr[k_max_devices] = {};
+
+// The calling thread's current device's copy of `scratch` (resolved once per device).
+std::uint32_t *device_scratch() {
+    int dev = 0;
+    GWN_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= k_max_devices)
+        throw_internal("noalloc Taylor batch: device ordinal out of range.");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *ptr = nullptr;
+        GWN_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (ptr == nullptr)
+            throw_internal("noalloc Taylor batch: scratch symbol has no device address.");
+        g_scratch_ptr[dev] = static_cast<std::uint32_t *>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+
+} // namespace
+
+void winding_taylor_noalloc(
+    gwn_bvh_handle const &bvh, gwn_moment_handle const &m, float const *qx, float const *qy,
+    float const *qz, float *out, std::size_t n, float accuracy_scale, int precision, query_launch const &cfg,
+    hipStream_t stream
+) {
+    if (n > k_noalloc_max_queries)
+        throw_invalid("noalloc Taylor batch: query count exceeds the static scratch capacity (1048576 queries).");
+    static_assert(k_noalloc_sort_words >= 3 * k_noalloc_max_queries + 4 * 256 + (k_noalloc_max_queries + 4095) / 4096 * 256,
+                  "presort scratch must cover query_order_workspace_words(k_noalloc_max_queries)");
+    if (n == 0)
+        return;
+    bool const sort = !ref_leaf(bvh.root.reference) &&
+                      (cfg.sort_queries > 0 || (cfg.sort_queries < 0 && n >= (1u << 16)));
+    if (!sort) {
+        winding_taylor(bvh, m, qx, qy, qz, out, nullptr, nullptr, nullptr, n, accuracy_scale, precision, cfg,
+                       nullptr, nullptr, stream);
+        return;
+    }
+    if (query_order_workspace_words(n) > k_noalloc_sort_words)
+        throw_internal("noalloc Taylor batch: presort scratch too small.");
+    std::uint32_t *const base = device_scratch();
+    std::uint32_t *const order = base;
+    query_order_ws(bvh, qx, qy, qz, order, n, base + k_noalloc_order_words, stream);
+    winding_taylor(bvh, m, qx, qy, qz, out, nullptr, nullptr, nullptr, n, accuracy_scale, precision, cfg, order,
+                   nullptr, stream);
 }
 
 } // namespace gwn

@@ -39,6 +39,7 @@ EXPORTED_SYMBOLS = (
     "gwn_moment_order",
     "gwn_compute_winding_number_exact_batch_f32", "gwn_compute_winding_number_taylor_batch_f32",
     "gwn_compute_winding_number_exact_batch_ex_f32", "gwn_compute_winding_number_taylor_batch_ex_f32",
+    "gwn_compute_winding_number_taylor_batch_noalloc_f32",
     "gwn_compute_winding_gradient_taylor_batch_f32", "gwn_compute_winding_gradient_taylor_batch_ex_f32",
     "gwn_compute_unsigned_distance_batch_f32", "gwn_compute_ray_first_hit_batch_f32",
     "gwn_compute_winding_number_antipodal_batch_f32", "gwn_compute_winding_number_antipodal_batch_ex_f32",
@@ -176,6 +177,7 @@ def _load():
         "gwn_compute_winding_number_taylor_batch_f32": [vp, vp, vp, vp, vp, vp, u64, f32, vp, vp],
         "gwn_compute_winding_number_exact_batch_ex_f32": [vp, vp, vp, vp, vp, u64, i32, vp, vp],
         "gwn_compute_winding_number_taylor_batch_ex_f32": [vp, vp, vp, vp, vp, vp, u64, f32, i32, vp, vp, vp, vp],
+        "gwn_compute_winding_number_taylor_batch_noalloc_f32": [vp, vp, vp, vp, vp, vp, u64, f32, i32, vp, vp],
         "gwn_compute_winding_gradient_taylor_batch_f32": [vp, vp, vp, vp, vp, vp, vp, vp, u64, f32, vp, vp],
         "gwn_compute_winding_gradient_taylor_batch_ex_f32": [
             vp, vp, vp, vp, vp, vp, vp, vp, vp, u64, f32, i32, vp, vp, vp, vp],
@@ -661,6 +663,28 @@ def gwn_compute_winding_number_taylor_batch(bvh: BvhObject, moment: MomentObject
         bvh.handle, moment.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx),
         float(accuracy_scale), int(precision), ctypes.byref(cfg), _dptr(order), _dptr(counters),
         _stream(stream)))
+
+
+NOALLOC_MAX_QUERIES = 1 << 20
+
+
+def gwn_compute_winding_number_taylor_batch_noalloc(bvh: BvhObject, moment: MomentObject, qx, qy, qz, out,
+                                                    accuracy_scale: float = 2.0,
+                                                    config: QueryConfig | None = None, stream=None,
+                                                    precision: Precision = Precision.PLAIN) -> Status:
+    """The Taylor winding batch without device allocation (no reference counterpart).
+
+    Same results as ``gwn_compute_winding_number_taylor_batch``; the Morton presort's scratch is
+    the library's static per-GPU array, so at most ``NOALLOC_MAX_QUERIES`` queries per call, and
+    calls that overlap on one GPU race.
+    """
+    bad = _check_soa(qx, qy, qz, [out]) or _check_real(4, qx, qy, qz, out)
+    if bad:
+        return bad
+    cfg = _config(config)
+    return _status(_lib.gwn_compute_winding_number_taylor_batch_noalloc_f32(
+        bvh.handle, moment.handle, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), _numel(qx),
+        float(accuracy_scale), int(precision), ctypes.byref(cfg), _stream(stream)))
 
 
 def gwn_compute_winding_gradient_taylor_batch(bvh: BvhObject, moment: MomentObject, qx, qy, qz, gx, gy, gz,
