@@ -76,6 +76,60 @@ __host__ __device__ constexpr int moment_record_vec4(int order) {
     return order == 0 ? 8 : (order == 1 ? 16 : 24);
 }
 
+// Evaluation record: the traversal record with the expansion's wave-uniform arithmetic folded in
+// at refit time, for the value-only Taylor queries.  Its first five float4 are the traversal
+// record's (child references, then max_p_dist2 and the expansion centre per slot: what the
+// acceptance test reads); then one block of eval_slot_floats(order) floats per child slot holding
+// that slot's coefficients contiguously (N, then M, then C'; zero padded), so an accepted slot's
+// coefficients arrive in one or two scalar loads.  With q the unit direction and |q| = 1 the
+// order-1 and order-2 terms become plain forms in q:
+//   Tr - 3 q^T T q               = q^T M q,  M = Tr I - 3 T                       (fields 7-12)
+//   1.5 (q . P) - 7.5 C(q)       = C'(q) = 1.5 (q . P)(q . q) - 7.5 C(q)          (fields 13-22)
+// with P = (3 xxx + Ayx + Azx, 3 yyy + Azy + Axy, 3 zzz + Axz + Ayz) and C the reference's cubic
+// (gwn_query_winding_impl.cuh:186-325).  Field 7, 8, 9 hold M's diagonal, 10, 11, 12 the full
+// coefficients of qx qy, qy qz, qz qx; fields 13-22 the coefficients of qx^3, qy^3, qz^3,
+// qx qy qz, qx^2 qy, qx^2 qz, qy^2 qz, qy^2 qx, qz^2 qx, qz^2 qy.  `c` is one slot's traversal
+// fields (c[f] = field f), `e` receives its evaluation fields in the same numbering; a slot's
+// block holds e[4], e[5], ..., e[F - 1].
+template <int Order>
+__host__ __device__ inline void taylor_eval_fields(float const *c, float *e) {
+#pragma clang fp contract(off)
+    constexpr int F = moment_field_count(Order);
+    for (int f = 0; f < 7; ++f)
+        e[f] = c[f];
+    if constexpr (Order >= 1) {
+        float const tr = (c[7] + c[8]) + c[9];
+        e[7] = tr - 3.0f * c[7];
+        e[8] = tr - 3.0f * c[8];
+        e[9] = tr - 3.0f * c[9];
+        e[10] = -3.0f * c[10];
+        e[11] = -3.0f * c[11];
+        e[12] = -3.0f * c[12];
+    }
+    if constexpr (Order >= 2) {
+        float const xxx = c[13], yyy = c[14], zzz = c[15], spn = c[16];
+        float const axy = c[17], axz = c[18], ayz = c[19], ayx = c[20], azx = c[21], azy = c[22];
+        float const px = 1.5f * ((3.0f * xxx + ayx) + azx);
+        float const py = 1.5f * ((3.0f * yyy + azy) + axy);
+        float const pz = 1.5f * ((3.0f * zzz + axz) + ayz);
+        e[13] = px - 7.5f * xxx;
+        e[14] = py - 7.5f * yyy;
+        e[15] = pz - 7.5f * zzz;
+        e[16] = -7.5f * spn;
+        e[17] = py - 7.5f * axy;
+        e[18] = pz - 7.5f * axz;
+        e[19] = pz - 7.5f * ayz;
+        e[20] = px - 7.5f * ayx;
+        e[21] = px - 7.5f * azx;
+        e[22] = py - 7.5f * azy;
+    }
+    (void)F;
+}
+
+__host__ __device__ constexpr int eval_slot_floats(int order) { return order == 0 ? 4 : (order == 1 ? 12 : 20); }
+// float4 per evaluation record: 5 (references, acceptance fields) + 4 slot blocks.
+__host__ __device__ constexpr int eval_record_vec4(int order) { return 5 + eval_slot_floats(order); }
+
 // Nearest-triangle codes of the unsigned-distance batch's optional per-query output: the BVH
 // leaf record of the nearest triangle, or one of these.
 constexpr std::uint32_t k_nearest_none = 0xffffffffu;     // the band or zero: no gradient

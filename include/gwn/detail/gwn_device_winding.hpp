@@ -269,6 +269,39 @@ __device__ inline void split_leaf(tri_record const &t, float x, float y, float z
     }
 }
 
+// One accepted far-field expansion from a slot's evaluation block `blk` (blk[k] = evaluation
+// field k + 4, taylor_eval_fields): the value of taylor_term below with its wave-uniform arithmetic
+// folded into the record,
+//   w = m2 (m1 (q^T M q + m1 C'(q)) - q . N),  m1 = 1 / |q - P|,  q the unit direction.
+// Equal to the reference's expression up to rounding (and |q| = 1 within a few ulps).
+template <int Order>
+__device__ inline float taylor_value(float const *blk, float qrx, float qry, float qrz, float ql2) {
+#pragma clang fp contract(on)
+#define GWN_F(k) blk[(k) - 4]
+    float const m1 = __builtin_amdgcn_rsqf(ql2);
+    float const m2 = m1 * m1;
+    float const qnx = qrx * m1, qny = qry * m1, qnz = qrz * m1;
+    float const rdotN = qnx * GWN_F(4) + qny * GWN_F(5) + qnz * GWN_F(6);
+    if constexpr (Order == 0) {
+        return -m2 * rdotN;
+    } else {
+        float const qz2 = qnz * qnz;
+        float const a = qnx * GWN_F(7) + qny * GWN_F(10) + qnz * GWN_F(12);
+        float const b = qny * GWN_F(8) + qnz * GWN_F(11);
+        float t = qnx * a + qny * b + qz2 * GWN_F(9);
+        if constexpr (Order >= 2) {
+            float const qx2 = qnx * qnx, qy2 = qny * qny;
+            float const cx = qnx * GWN_F(13) + qny * GWN_F(17) + qnz * GWN_F(18);
+            float const cy = qny * GWN_F(14) + qnz * GWN_F(19) + qnx * GWN_F(20);
+            float const cz = qnz * GWN_F(15) + qnx * GWN_F(21) + qny * GWN_F(22);
+            float const cub = qx2 * cx + qy2 * cy + qz2 * cz + (qnx * qny) * (qnz * GWN_F(16));
+            t = t + m1 * cub;
+        }
+        return m2 * (m1 * t - rdotN);
+    }
+#undef GWN_F
+}
+
 // One accepted far-field expansion (gwn_query_winding_impl.cuh:186-325): returns the solid-angle
 // term and, with Grad, writes its gradient.
 template <int Order, bool Grad>
@@ -423,6 +456,8 @@ struct taylor_outputs {
 
 // Per-lane Taylor traversal of one query from an internal root: children in slot order, the last
 // descended internal child entered directly, the others pushed.  Returns true on stack overflow.
+// `records` are the moments' evaluation records for the value alone (Grad = false) and their
+// traversal records with the gradient (Grad = true).
 template <int Order, int Prec, bool Grad, int StackCap>
 __device__ inline bool taylor_traverse(
     float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node, float x,
@@ -430,7 +465,8 @@ __device__ inline bool taylor_traverse(
     std::uint32_t &n_leaf
 ) {
 #pragma clang fp contract(on)
-    constexpr int RV = moment_record_vec4(Order);
+    constexpr int RV = Grad ? moment_record_vec4(Order) : eval_record_vec4(Order);
+    constexpr int B = eval_slot_floats(Order);
     std::uint32_t stack[StackCap];
     int sp = 0;
     bool overflow = false;
@@ -462,7 +498,11 @@ __device__ inline bool taylor_traverse(
             bool const descend = !(ql2 > 0.0f) || ql2 <= product_unfused(mpd[s], beta2);
             if (!descend) {
                 float g[3];
-                float const w = taylor_term<Order, Grad>(fields, s, qrx, qry, qrz, ql2, g);
+                float w;
+                if constexpr (Grad)
+                    w = taylor_term<Order, true>(fields, s, qrx, qry, qrz, ql2, g);
+                else
+                    w = taylor_value<Order>(reinterpret_cast<float const *>(rec + 5) + s * B, qrx, qry, qrz, ql2);
                 if (isfinite(w)) {
                     acc.far_term(w, g);
                     ++n_accept;

@@ -147,8 +147,10 @@ __device__ inline winding::taylor_sums<GWN_PRECISION_PLAIN, Grad> taylor_point(
     }
     float const beta2 = accuracy_scale * accuracy_scale;
     std::uint32_t n_tests = 0, n_accept = 0, n_leaf = 0;
+    // The value alone reads the evaluation records, the gradient the traversal records, as the
+    // batches do.
     overflow = winding::taylor_traverse<Order, GWN_PRECISION_PLAIN, Grad, StackCapacity>(
-        static_cast<float4 const *>(moment.records), records_of(bvh),
+        static_cast<float4 const *>(Grad ? moment.records : moment.eval_records), records_of(bvh),
         static_cast<std::uint32_t>(bvh.root_reference & ((std::uint64_t(1) << 47) - 1)), x, y, z, beta2, acc,
         n_tests, n_accept, n_leaf);
     if (overflow)

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define GWN_HIP_ABI_VERSION 1
+#define GWN_HIP_ABI_VERSION 2
 
 typedef enum gwn_error_code {
     GWN_SUCCESS = 0,
@@ -143,6 +143,9 @@ typedef struct gwn_moment_accessor {
     uint32_t record_bytes;            /* 128, 256 or 384 for orders 0, 1, 2 */
     int32_t width;                    /* the width of the BVH the records were formed from */
     uint64_t bvh_revision;            /* the BVH revision the records were formed from */
+    const void *eval_records;         /* node_count evaluation records (144, 272 or 400 B for
+                                         orders 0, 1, 2): the coefficients the value-only Taylor
+                                         queries read, one contiguous block per child slot */
 } gwn_moment_accessor;
 
 typedef struct gwn_boundary_accessor {

@@ -38,6 +38,7 @@ struct gwn_batch_handle {
     gwn::device_buffer<gwn::moment_refit_mesh> refits;
     std::vector<gwn::refit_level> refit_levels;       // per depth, deepest last
     gwn::device_buffer<gwn::refit_segment> segments;
+    gwn::device_buffer<std::uint64_t> eval_prefix;    // slot prefix over host_refits (+ total)
     gwn::device_buffer<std::uint32_t> status;         // sticky refit error word
     gwn::device_buffer<gwn::batch_mesh> meshes;
     gwn::device_buffer<std::uint64_t> counter;
@@ -137,7 +138,8 @@ void stamp_revisions(gwn_batch_handle &b) {
 }
 
 void enqueue_refit(gwn_batch_handle &b, hipStream_t stream) {
-    refit_moment_batch(b.refits.ptr, b.segments.ptr, b.refit_levels, b.order, b.status.ptr, stream);
+    refit_moment_batch(b.refits.ptr, b.segments.ptr, b.refit_levels, b.eval_prefix.ptr,
+                       static_cast<std::uint32_t>(b.host_refits.size()), b.order, b.status.ptr, stream);
 }
 
 void enqueue_query(gwn_batch_handle &b, hipStream_t stream) {
@@ -165,8 +167,8 @@ void upload_refit_table(gwn_batch_handle &b, hipStream_t stream) {
             throw_internal("batch: A BVH level table does not cover its nodes.");
         std::uint32_t const mesh = static_cast<std::uint32_t>(b.host_refits.size());
         b.host_refits.push_back(moment_refit_mesh{bvh.nodes.ptr, bvh.triangles.ptr, b.work[i].pending.ptr,
-                                                  b.moments[i]->records.ptr, b.moments[i]->slot_area.ptr,
-                                                  bvh.node_count, bvh.triangle_count});
+                                                  b.moments[i]->records.ptr, b.moments[i]->eval_records.ptr,
+                                                  b.moments[i]->slot_area.ptr, bvh.node_count, bvh.triangle_count});
         if (by_depth.size() < ls.size() - 1)
             by_depth.resize(ls.size() - 1);
         for (std::size_t d = 0; d + 1 < ls.size(); ++d)
@@ -184,8 +186,14 @@ void upload_refit_table(gwn_batch_handle &b, hipStream_t stream) {
         }
         b.refit_levels.push_back(lv);
     }
+    std::vector<std::uint64_t> prefix(b.host_refits.size() + 1, 0);
+    for (std::size_t i = 0; i < b.host_refits.size(); ++i)
+        prefix[i + 1] = prefix[i] + 4 * b.host_refits[i].node_count;
     b.refits.allocate(std::max<std::size_t>(1, b.host_refits.size()), stream);
     b.segments.allocate(std::max<std::size_t>(1, flat.size()), stream);
+    b.eval_prefix.allocate(prefix.size(), stream);
+    GWN_HIP(hipMemcpyAsync(b.eval_prefix.ptr, prefix.data(), prefix.size() * sizeof(std::uint64_t),
+                           hipMemcpyHostToDevice, stream));
     if (!b.host_refits.empty())
         GWN_HIP(hipMemcpyAsync(b.refits.ptr, b.host_refits.data(), b.host_refits.size() * sizeof(moment_refit_mesh),
                                hipMemcpyHostToDevice, stream));
@@ -271,7 +279,7 @@ gwn_batch_handle *batch_create(
         for (std::size_t i = 0; i < bvhs.size(); ++i) {
             batch_mesh &m = b->host_meshes[i];
             bool const internal = ref_internal(bvhs[i]->root.reference);
-            m.records = internal ? moments[i]->records.ptr : nullptr;
+            m.records = internal ? moments[i]->eval_records.ptr : nullptr;
             m.tris = bvhs[i]->triangles.ptr;
             m.root = bvhs[i]->root.reference;
             m.order = nullptr;

@@ -283,7 +283,7 @@ static void rebind(gwn_bvh_handle &b, hipStream_t s) noexcept {
 
 static void rebind(gwn_moment_handle &m, hipStream_t s) noexcept {
     m.stream = s;
-    m.records.stream = m.slot_area.stream = s;
+    m.records.stream = m.eval_records.stream = m.slot_area.stream = s;
     m.crefs_d.stream = m.fields_d.stream = s;
 }
 
@@ -670,6 +670,7 @@ int gwn_moment_clear(gwn_moment_t moment) {
     return guarded("gwn_moment_clear", [&] {
         gwn_moment_handle &m = *require_handle(moment, "Moment");
         m.records.release();
+        m.eval_records.release();
         m.slot_area.release();
         m.crefs_d.release();
         m.fields_d.release();
@@ -688,7 +689,7 @@ int gwn_moment_get_accessor(gwn_moment_t moment, gwn_moment_accessor *out) {
             gwn::throw_invalid("Device accessors are provided for Real = float objects.");
         *out = gwn_moment_accessor{m.records.ptr, m.node_count, m.order,
                                    static_cast<uint32_t>(gwn::moment_record_vec4(m.order) * 16), m.width,
-                                   m.bvh_revision};
+                                   m.bvh_revision, m.eval_records.ptr};
     });
 }
 

@@ -230,6 +230,9 @@ struct gwn_moment_handle {
     gwn::device_buffer<std::uint32_t> crefs_d;
     gwn::device_buffer<double> fields_d;
     gwn::device_buffer<float4> records; // node_count * moment_record_vec4(order)
+    // The same count of evaluation records (taylor_eval_fields): what the value-only Taylor
+    // queries read; the traversal records serve the gradient, adjoint and downloads.
+    gwn::device_buffer<float4> eval_records;
     gwn::device_buffer<float> slot_area; // node_count * 4: each child slot's total triangle area
     std::size_t node_count = 0;
     std::uint64_t bvh_revision = 0;
@@ -281,7 +284,7 @@ void winding_taylor_double(gwn_bvh_handle const &bvh, gwn_moment_handle const &m
                            double const *qz, double *out, std::size_t n, double accuracy_scale, hipStream_t stream);
 // One mesh of a batched Taylor query launch (device-resident table).
 struct batch_mesh {
-    float4 const *records;       // moment traversal records (null for a leaf root)
+    float4 const *records;       // moment evaluation records (null for a leaf root)
     tri_record const *tris;
     std::uint64_t root;          // root child reference
     float const *qx, *qy, *qz;
@@ -326,6 +329,7 @@ struct moment_refit_mesh {
     tri_record const *tris;
     void *pending;
     float4 *records;
+    float4 *eval_records;
     float *slot_area;
     std::uint64_t node_count, tri_count;
 };
@@ -343,11 +347,12 @@ struct refit_level {
     std::uint32_t segments;
 };
 // Every mesh's moment refit (all of one order): one launch per depth, deepest first, over all
-// meshes' nodes at that depth.  Records equal those of single-mesh refits; an invalid tree sets
-// *error (device word) nonzero.
+// meshes' nodes at that depth, then one launch writing every mesh's evaluation records
+// (slot_prefix: device prefix of 4 x node_count over the mesh_count meshes, plus the total).
+// Records equal those of single-mesh refits; an invalid tree sets *error (device word) nonzero.
 void refit_moment_batch(moment_refit_mesh const *meshes, refit_segment const *segments,
-                        std::vector<refit_level> const &levels, int order, std::uint32_t *error,
-                        hipStream_t stream);
+                        std::vector<refit_level> const &levels, std::uint64_t const *slot_prefix,
+                        std::uint32_t mesh_count, int order, std::uint32_t *error, hipStream_t stream);
 // In-place bounds refit: triangle records and child boxes from the geometry, root box updated.
 void refit_bounds(gwn_geometry_handle const &geometry, gwn_bvh_handle &bvh, hipStream_t stream);
 

@@ -13,7 +13,9 @@
 //
 // Output: one traversal record per node -- a uint4 of compact child references followed by the
 // reference's field-SoA gwn_bvh_moment_node<4, Order, float> coefficients as float4 (one float
-// per child slot), padded to 128 / 256 / 384 bytes.
+// per child slot), padded to 128 / 256 / 384 bytes -- then, in one pass over all nodes, one
+// evaluation record of the same layout with the coefficients the value-only queries read
+// (taylor_eval_fields).
 #include "gwn_core.hpp"
 #include "gwn_moment_math.hpp"
 
@@ -141,6 +143,75 @@ __global__ __launch_bounds__(k_block) void moment_level_batch_kernel(
                              m.records, m.slot_area, error);
 }
 
+// Evaluation records from finished traversal records: one thread per (node, child slot) over
+// nodes [0, node_count) of one tree (the child references and the zero padding are copied too).
+template <int Order>
+__device__ inline void eval_slot(float4 const *__restrict__ records, float4 *__restrict__ eval_records, std::size_t n,
+                                 int s) {
+    constexpr int RV = moment_record_vec4(Order);
+    constexpr int EV = eval_record_vec4(Order);
+    constexpr int B = eval_slot_floats(Order);
+    constexpr int F = moment_field_count(Order);
+    float const *src = reinterpret_cast<float const *>(records + n * RV);
+    float *dst = reinterpret_cast<float *>(eval_records + n * EV);
+    float c[F], e[F];
+    for (int f = 0; f < F; ++f)
+        c[f] = src[4 + f * 4 + s];
+    taylor_eval_fields<Order>(c, e);
+    // References and the acceptance fields (max_p_dist2, centre) keep their slot-interleaved place.
+    dst[s] = src[s];
+    for (int f = 0; f < 4; ++f)
+        dst[4 + f * 4 + s] = e[f];
+    float *blk = dst + 20 + s * B;
+    for (int f = 4; f < F; ++f)
+        blk[f - 4] = e[f];
+    for (int k = F - 4; k < B; ++k)
+        blk[k] = 0.0f;
+}
+
+template <int Order>
+__global__ __launch_bounds__(256) void moment_eval_kernel(float4 const *__restrict__ records,
+                                                          float4 *__restrict__ eval_records, std::size_t node_count) {
+    std::size_t const g = static_cast<std::size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (g >= node_count * 4)
+        return;
+    eval_slot<Order>(records, eval_records, g >> 2, static_cast<int>(g & 3));
+}
+
+// The same over every mesh of a batched refit: thread g belongs to the last mesh whose slot prefix
+// is <= g (binary search over `slot_prefix`, mesh_count + 1 entries).
+template <int Order>
+__global__ __launch_bounds__(256) void moment_eval_batch_kernel(moment_refit_mesh const *__restrict__ meshes,
+                                                                std::uint64_t const *__restrict__ slot_prefix,
+                                                                std::uint32_t mesh_count) {
+    std::uint64_t const g = static_cast<std::uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (g >= slot_prefix[mesh_count])
+        return;
+    std::uint32_t lo = 0, hi = mesh_count;
+    while (hi - lo > 1) {
+        std::uint32_t const mid = lo + (hi - lo) / 2;
+        if (slot_prefix[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    moment_refit_mesh const m = meshes[lo];
+    std::uint64_t const k = g - slot_prefix[lo];
+    eval_slot<Order>(m.records, m.eval_records, static_cast<std::size_t>(k >> 2), static_cast<int>(k & 3));
+}
+
+void launch_eval(gwn_moment_handle &m, std::size_t node_count, hipStream_t stream) {
+    if (node_count == 0)
+        return;
+    unsigned const grid = grid_for(node_count * 4, 256);
+    switch (m.order) {
+    case 0: moment_eval_kernel<0><<<grid, 256, 0, stream>>>(m.records.ptr, m.eval_records.ptr, node_count); break;
+    case 1: moment_eval_kernel<1><<<grid, 256, 0, stream>>>(m.records.ptr, m.eval_records.ptr, node_count); break;
+    default: moment_eval_kernel<2><<<grid, 256, 0, stream>>>(m.records.ptr, m.eval_records.ptr, node_count); break;
+    }
+    GWN_LAUNCH_CHECK();
+}
+
 template <int Order>
 void launch_levels(gwn_bvh_handle const &bvh, gwn_moment_handle &m, void *pending, std::uint32_t *error,
                    hipStream_t stream) {
@@ -191,13 +262,15 @@ void refit_moment_prepare(gwn_bvh_handle const &bvh, gwn_moment_handle &m, momen
         return;
     check_levels(bvh);
     m.records.allocate(N * static_cast<std::size_t>(moment_record_vec4(m.order)), stream);
+    m.eval_records.allocate(N * static_cast<std::size_t>(eval_record_vec4(m.order)), stream);
     m.slot_area.allocate(N * 4, stream);
     w.pending.allocate(N * payload_bytes(m.order), stream);
 }
 
 void refit_moment_batch(moment_refit_mesh const *meshes, refit_segment const *segments,
-                        std::vector<refit_level> const &levels, int order, std::uint32_t *error,
-                        hipStream_t stream) {
+                        std::vector<refit_level> const &levels, std::uint64_t const *slot_prefix,
+                        std::uint32_t mesh_count, int order, std::uint32_t *error, hipStream_t stream) {
+    std::uint64_t total_nodes = 0;
     for (std::size_t d = levels.size(); d-- > 0;) {
         refit_level const &lv = levels[d];
         if (lv.nodes == 0)
@@ -211,7 +284,17 @@ void refit_moment_batch(moment_refit_mesh const *meshes, refit_segment const *se
         default: throw_invalid("Taylor order must be 0, 1, or 2.");
         }
         GWN_LAUNCH_CHECK();
+        total_nodes += lv.nodes;
     }
+    if (total_nodes == 0 || mesh_count == 0)
+        return;
+    unsigned const grid = grid_for(4 * total_nodes, 256);
+    switch (order) {
+    case 0: moment_eval_batch_kernel<0><<<grid, 256, 0, stream>>>(meshes, slot_prefix, mesh_count); break;
+    case 1: moment_eval_batch_kernel<1><<<grid, 256, 0, stream>>>(meshes, slot_prefix, mesh_count); break;
+    default: moment_eval_batch_kernel<2><<<grid, 256, 0, stream>>>(meshes, slot_prefix, mesh_count); break;
+    }
+    GWN_LAUNCH_CHECK();
 }
 
 std::size_t moment_payload_bytes(int order) { return payload_bytes(order); }
@@ -226,11 +309,13 @@ void refit_moment(gwn_bvh_handle const &bvh, gwn_moment_handle &m, hipStream_t s
     check_levels(bvh);
     std::size_t const N = bvh.node_count;
     m.records.allocate(N * static_cast<std::size_t>(moment_record_vec4(m.order)), stream);
+    m.eval_records.allocate(N * static_cast<std::size_t>(eval_record_vec4(m.order)), stream);
     m.slot_area.allocate(N * 4, stream);
     device_buffer<std::uint8_t> pending(N * payload_bytes(m.order), stream);
     device_buffer<std::uint32_t> error(1, stream);
     fill_u32(error.ptr, 0u, 1, stream);
     enqueue_levels(bvh, m, pending.ptr, error.ptr, stream);
+    launch_eval(m, N, stream);
     std::uint32_t host_error = 0;
     GWN_HIP(hipMemcpyAsync(&host_error, error.ptr, sizeof(host_error), hipMemcpyDeviceToHost, stream));
     GWN_HIP(hipStreamSynchronize(stream));

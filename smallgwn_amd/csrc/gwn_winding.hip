@@ -243,13 +243,16 @@ __global__ __launch_bounds__(256) void taylor_batch_kernel(
 // by internal_stack_bound for the same reason.
 // The wave-cooperative traversal of one wave's queries (lanes with `active`); s_node / s_mask are
 // this wave's LDS stack.  The wave must hold at least one active lane.  Returns true on overflow.
+// `records` are evaluation records for the value alone and traversal records with the gradient,
+// as for the per-lane traversal.
 template <int Order, int Prec, bool Grad, int StackCap>
 __device__ inline bool taylor_wave_traverse(
     float4 const *__restrict__ records, tri_record const *__restrict__ tris, std::uint32_t root_node, bool active,
     float x, float y, float z, float beta2, taylor_sums<Prec, Grad> &acc, std::uint32_t &n_tests,
     std::uint32_t &n_accept, std::uint32_t &n_leaf, std::uint32_t *s_node, std::uint64_t *s_mask
 ) {
-    constexpr int RV = moment_record_vec4(Order);
+    constexpr int RV = Grad ? moment_record_vec4(Order) : eval_record_vec4(Order);
+    constexpr int B = eval_slot_floats(Order);
     int const lane = threadIdx.x & 63;
     std::uint64_t mask = __ballot(active);
     std::uint64_t const my_bit = 1ull << lane;
@@ -284,7 +287,12 @@ __device__ inline bool taylor_wave_traverse(
                 descend = !(ql2 > 0.0f) || ql2 <= product_unfused(mpd[s], beta2);
                 if (!descend) {
                     float g[3];
-                    float const w = taylor_term<Order, Grad>(fields, s, qrx, qry, qrz, ql2, g);
+                    float w;
+                    if constexpr (Grad)
+                        w = taylor_term<Order, true>(fields, s, qrx, qry, qrz, ql2, g);
+                    else
+                        w = taylor_value<Order>(reinterpret_cast<float const *>(rec + 5) + s * B, qrx, qry, qrz,
+                                                ql2);
                     if (isfinite(w)) {
                         acc.far_term(w, g);
                         ++n_accept;
@@ -610,14 +618,16 @@ void launch_taylor_cap(
     hipStream_t stream
 ) {
     std::uint32_t const root = static_cast<std::uint32_t>(ref_offset(bvh.root.reference));
+    // The value alone reads the evaluation records; the gradient the traversal records.
+    float4 const *records = Grad ? m.records.ptr : m.eval_records.ptr;
     if (order != nullptr) {
         // Morton-ordered queries: neighbouring lanes share most of their paths.
         taylor_wave_kernel<Order, Prec, Grad, Cap><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(
-            m.records.ptr, bvh.triangles.ptr, root, qx, qy, qz, order, o, n, beta2
+            records, bvh.triangles.ptr, root, qx, qy, qz, order, o, n, beta2
         );
     } else {
         taylor_kernel<Order, Prec, Grad, Cap><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(
-            m.records.ptr, bvh.triangles.ptr, root, qx, qy, qz, order, o, n, beta2
+            records, bvh.triangles.ptr, root, qx, qy, qz, order, o, n, beta2
         );
     }
     GWN_LAUNCH_CHECK();

@@ -288,7 +288,8 @@ class _CBvhAccessor(ctypes.Structure):
 
 class _CMomentAccessor(ctypes.Structure):
     _fields_ = [("records", ctypes.c_void_p), ("node_count", ctypes.c_uint64), ("order", ctypes.c_int32),
-                ("record_bytes", ctypes.c_uint32), ("width", ctypes.c_int32), ("bvh_revision", ctypes.c_uint64)]
+                ("record_bytes", ctypes.c_uint32), ("width", ctypes.c_int32), ("bvh_revision", ctypes.c_uint64),
+                ("eval_records", ctypes.c_void_p)]
 
 
 class _CBoundaryAccessor(ctypes.Structure):

@@ -83,8 +83,9 @@ def test_gradient_matches_oracle_replay(gwn, orc, dev, order, beta, precision, s
 
 def test_gradient_only_output_and_winding_agree(gwn, dev):
     """The gradient-only entry point gives the same gradient as the combined form (same kernel),
-    and the combined form's winding equals the winding batch's bit for bit (products are contracted
-    only within a source expression, so both kernels round every term alike)."""
+    and the combined form's winding (the reference's expression on the traversal records) agrees
+    with the winding batch's (the same expansion folded into the evaluation records) to rounding:
+    identical decisions, terms differing by a few ulps."""
     import torch
 
     _, bvh = build_gpu(gwn, OPEN)
@@ -98,7 +99,10 @@ def test_gradient_only_output_and_winding_agree(gwn, dev):
     w = torch.empty(4096, dtype=torch.float32, device=dev)
     gwn.gwn_compute_winding_number_taylor_batch(bvh, mom, *d, w, accuracy_scale=2.0,
                                                 config=gwn.QueryConfig(sort_queries=0)).raise_if_error()
-    assert np.array_equal(w.cpu().numpy().view(np.uint32), w2.view(np.uint32))
+    w = w.cpu().numpy()
+    rel = (np.abs(w - w2) / (1 + np.abs(w2))).max()
+    print(f"[gradient] combined vs value-only winding: max rel {rel:.3e}", flush=True)
+    assert rel <= 2e-6
 
 
 def test_descended_leaf_gradient_matches_exact(gwn, orc, dev):
