@@ -5,7 +5,7 @@ spherical harmonics of degree <= 12), 1024 x 512 latitude-longitude quads = 1,04
 triangles; 16,777,216 query points, half uniform in [-1.5, 1.5]^3 and half within 0.01 of the
 surface along the face normal.  Seeded synthetic data (splitmix64 from 0xB3A9E5D4).
 
-One step = one Taylor order-2 winding batch over all 16M queries (Morton presort + traversal),
+One step = one Taylor order-2 winding batch over all 16M queries (Hilbert presort + traversal),
 FP32 traversal and far field, near-field triangle terms with FP64 numerators and denominators,
 at an accuracy scale (6.25) that keeps max |dW| <= 1e-4 against the FP64 exact winding number on a
 seeded, strided subsample of 1,048,576 queries.  Inputs, BVH and moments are resident in HBM
@@ -631,10 +631,10 @@ def adjoint_bench(args, dev, stream):
                                           "+ adjoint backward + SGD update",
            "winding_forward_ms": fwd_ms, "winding_adjoint_ms": adj_ms,
            "work": {"child_tests": tests, "accepted": accepts, "near_field_triangles": leaves},
-           "roofline": _valu_roofline(adj_flops, adj_ms, "winding adjoint batch (Morton order, wave traversal)",
+           "roofline": _valu_roofline(adj_flops, adj_ms, "winding adjoint batch (Hilbert order, wave traversal)",
                                       "forward weights (10 / 34 / 87) plus 60 FLOP per accepted slot's 12 field "
                                       "adjoints and 120 per near triangle's 9 vertex adjoints; work counted by "
-                                      "the forward with the same decisions; includes the Morton presort")}
+                                      "the forward with the same decisions; includes the Hilbert presort")}
     log(f"adjoint kernel {adj_ms:.2f} ms (forward {fwd_ms:.2f} ms), frac {res['roofline']['frac']}")
     # Gradient check inputs: the adjoint on 4,096 strided queries with the step's upstream gradient.
     sidx = np.linspace(0, nq - 1, 4096).astype(np.int64)

@@ -62,8 +62,8 @@ typedef struct gwn_batch_handle *gwn_batch_t;
 
 /* Reference gwn_query_batch_config (include/gwn/gwn_query.cuh:41-61): block size 256 and
  * stack capacity 64 by default.  sort_queries: -1 auto, 0 off, 1 on (queries are traversed in
- * Morton order of their positions; results are written in input order).  The unsigned distance
- * batch also takes 2: Morton order with a wave-cooperative traversal (a wave walks each node once
+ * Hilbert-curve order of their positions; results are written in input order).  The unsigned
+ * distance batch also takes 2: Hilbert order with a wave-cooperative traversal (a wave walks each node once
  * for its 64 points, each point pruning by its own best distance).  Its distances are each
  * point's minimum over the triangles it reaches, which can differ from the reference order's in
  * the last FP32 bits where a box distance rounds above a contained triangle's distance.
@@ -343,7 +343,7 @@ int gwn_compute_winding_number_taylor_batch_ex_f32(
 /* Allocation-free Taylor winding batch (no reference counterpart).  For callers that cannot
  * allocate device memory: code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc.  Same arguments, validation and results as
- * gwn_compute_winding_number_taylor_batch_ex_f32 without an order or counters.  The Morton
+ * gwn_compute_winding_number_taylor_batch_ex_f32 without an order or counters.  The Hilbert
  * presort's scratch lives in a static per-GPU device array (about 16.3 MiB, held until the
  * process ends) instead of stream-ordered allocations, so count is limited to 1048576 queries
  * (larger batches return GWN_ERROR_INVALID_ARGUMENT; split them).  The array is shared by every
@@ -484,7 +484,7 @@ typedef struct gwn_batch_tuning {
     int32_t chunk;           /* queries per work item, a multiple of block_size */
     int32_t blocks_per_cu;   /* persistent grid = blocks_per_cu x compute units */
     int32_t traversal;       /* 0: one query per lane; 1: wave-cooperative over each mesh's
-                                Morton query order (computed at create) */
+                                Hilbert query order (computed at create) */
     int32_t stack_capacity;  /* compiled traversal stack (read only) */
     uint64_t meshes;         /* read only */
     uint64_t queries;        /* read only */
@@ -507,8 +507,9 @@ int gwn_batch_replay(gwn_batch_t batch, hipStream_t stream);
  * captured. */
 int gwn_batch_status(gwn_batch_t batch, hipStream_t stream);
 
-/* Morton-order permutation of query points within the BVH root bounds (30-bit keys, stable
- * radix sort): order[i] is the input index of the i-th query to visit.  Reusable across
+/* Hilbert-curve-order permutation of query points within the BVH root bounds (positions on the
+ * 1024^3 grid over the root box, 30-bit keys, stable radix sort): order[i] is the input index of
+ * the i-th query to visit.  Reusable across
  * batches over the same points (no reference counterpart; an MI355X traversal aid). */
 int gwn_compute_query_order_f32(
     gwn_bvh_t bvh, const float *qx, const float *qy, const float *qz, uint32_t *order,

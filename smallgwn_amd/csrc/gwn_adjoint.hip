@@ -15,7 +15,7 @@
 //     c = (a + b + c) / 3 then gives the vertex adjoints, added with the near-field ones into the
 //     caller's vertex arrays (atomics).
 //
-// Queries are visited in Morton order for large batches.  Sums use FP32 atomics, so the result is
+// Queries are visited in Hilbert order for large batches.  Sums use FP32 atomics, so the result is
 // reproducible to rounding, not bit for bit.
 #include "gwn_core.hpp"
 

@@ -13,8 +13,8 @@
 //   - graph: the refits, the counter reset (a fill kernel, not a memset) and the query launch are
 //     captured once into a hipGraph on the batch's own stream and replayed with hipGraphLaunch;
 //   - traversal: one query per lane, or wave-cooperative (a wave walks the tree once for its 64
-//     queries) over each mesh's Morton query order, computed once when the batch is made (any
-//     order gives the same results; Morton order keeps a wave's queries close together);
+//     queries) over each mesh's Hilbert query order, computed once when the batch is made (any
+//     order gives the same results; Hilbert order keeps a wave's queries close together);
 //   - tune: traversal, block size, chunk length and blocks per CU are chosen by timing
 //     candidate launches.
 #include "gwn_core.hpp"
@@ -42,7 +42,7 @@ struct gwn_batch_handle {
     gwn::device_buffer<std::uint32_t> status;         // sticky refit error word
     gwn::device_buffer<gwn::batch_mesh> meshes;
     gwn::device_buffer<std::uint64_t> counter;
-    std::vector<gwn::device_buffer<std::uint32_t>> orders;  // per mesh Morton visiting order
+    std::vector<gwn::device_buffer<std::uint32_t>> orders;  // per mesh Hilbert visiting order
     std::uint64_t total_queries = 0;
     std::uint64_t total_chunks = 0;
     float accuracy_scale = 2.0f;

@@ -291,7 +291,7 @@ struct batch_mesh {
     float *out;
     std::uint64_t count;
     std::uint64_t first_chunk;   // exclusive prefix of chunk counts over earlier meshes
-    std::uint32_t const *order;  // visiting order of the queries (Morton), or null
+    std::uint32_t const *order;  // visiting order of the queries (Hilbert), or null
 };
 // Persistent batched Taylor winding (plain precision): `grid` blocks of `block` threads take
 // chunks of `chunk` queries from *counter (which must be zero) until total_chunks are done.
@@ -375,7 +375,7 @@ void winding_taylor(
     int precision, query_launch const &cfg, std::uint32_t const *order, std::uint32_t *counters,
     hipStream_t stream
 );
-// Unsigned point-to-mesh distance with a culling band (queries Morton-sorted when cfg says so).
+// Unsigned point-to-mesh distance with a culling band (queries Hilbert-sorted when cfg says so).
 // `nearest` (optional, one uint32 per query) receives the BVH record of the nearest triangle, or
 // k_nearest_none (distance is the band or zero: no gradient) or k_nearest_overflow (both defined
 // with the device layout).
@@ -435,12 +435,12 @@ void winding_gradient_antipodal(
     gwn_geometry_handle const &geometry, gwn_boundary_handle const &boundary, float const *qx, float const *qy,
     float const *qz, float *gx, float *gy, float *gz, std::size_t n, int precision, hipStream_t stream
 );
-// Morton-order visiting permutation of query points within the BVH root bounds.
+// Hilbert-order visiting permutation of query points within the BVH root bounds.
 void query_order(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz,
     std::uint32_t *order, std::size_t n, hipStream_t stream
 );
-// Taylor winding batch (no gradient) with the Morton presort's scratch in the library's static
+// Taylor winding batch (no gradient) with the Hilbert presort's scratch in the library's static
 // per-device scratch array instead of stream-ordered allocations; at most
 // k_noalloc_max_queries queries.  Results equal winding_taylor's.
 constexpr std::size_t k_noalloc_max_queries = std::size_t(1) << 20;

@@ -10,8 +10,8 @@
 // distance with a five-comparator network; leaves of a node are evaluated before its internal
 // children are queued, the nearest internal child is entered directly and the others are pushed
 // far-to-near.  A query that reaches distance zero stops (zero is the global lower bound).
-// Queries may be visited through a Morton-order permutation so that a wave's lanes walk nearby
-// paths; results land at the input positions.  With sort_queries = 2 the Morton-ordered batch
+// Queries may be visited through a Hilbert-order permutation so that a wave's lanes walk nearby
+// paths; results land at the input positions.  With sort_queries = 2 the Hilbert-ordered batch
 // uses a wave-cooperative traversal instead (distance_wave_traverse).
 //
 // Arithmetic is unfused and the square root correctly rounded, so every result equals the CPU

@@ -7,7 +7,7 @@
 // LDS in 256-record tiles, so every record is read from HBM/L2 once per block and broadcast to
 // all 256 lanes.  VALU bound (solid angle: 3 sqrt, 3 div, 1 atan2 per pair).
 //
-// Taylor batch: one query per lane, queries optionally visited in Morton order (results are
+// Taylor batch: one query per lane, queries optionally visited in Hilbert order (results are
 // written back in input order).  Per-lane stack of internal node ids; the last internal child
 // of a node is entered directly, the others are pushed, leaves are evaluated in slot order --
 // the reference's traversal order, so float sums are formed in the same sequence.  The node
@@ -114,7 +114,41 @@ __device__ inline std::uint32_t spread10(std::uint32_t v) {
     return x;
 }
 
-__global__ __launch_bounds__(256) void query_morton_kernel(
+// Position of a cell of the 1024^3 grid along the 3-D Hilbert curve (30 bits), by Skilling's
+// transpose method ("Programming the Hilbert curve", AIP Conf. Proc. 707, 2004): undo the
+// excess work on the axes' bits from the top down, Gray-encode, then interleave the transposed
+// bits (axis 0 most significant in each triple).  Consecutive keys are face-adjacent cells, so a
+// wave of consecutive queries is more compact than along the Morton curve, whose keys jump.
+__device__ inline std::uint32_t hilbert_key10(std::uint32_t x0, std::uint32_t x1, std::uint32_t x2) {
+    std::uint32_t X[3] = {x0 & 0x3ffu, x1 & 0x3ffu, x2 & 0x3ffu};
+    for (std::uint32_t Q = 1u << 9; Q > 1u; Q >>= 1) {
+        std::uint32_t const P = Q - 1u;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                std::uint32_t const t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];
+    X[2] ^= X[1];
+    std::uint32_t t = 0;
+    for (std::uint32_t Q = 1u << 9; Q > 1u; Q >>= 1)
+        if (X[2] & Q)
+            t ^= Q - 1u;
+    X[0] ^= t;
+    X[1] ^= t;
+    X[2] ^= t;
+    return (spread10(X[0]) << 2) | (spread10(X[1]) << 1) | spread10(X[2]);
+}
+
+// Sort keys of the query presort: each query's Hilbert position on the 1024^3 grid over the
+// BVH's root box (points outside are clamped onto it).
+__global__ __launch_bounds__(256) void query_key_kernel(
     float const *__restrict__ qx, float const *__restrict__ qy, float const *__restrict__ qz,
     std::size_t n, aabb box, std::uint32_t *__restrict__ keys, std::uint32_t *__restrict__ ids
 ) {
@@ -128,8 +162,8 @@ __global__ __launch_bounds__(256) void query_morton_kernel(
     float const fx = fminf(fmaxf((qx[i] - box.min_x) * sx, 0.0f), 1.0f) * 1023.0f;
     float const fy = fminf(fmaxf((qy[i] - box.min_y) * sy, 0.0f), 1.0f) * 1023.0f;
     float const fz = fminf(fmaxf((qz[i] - box.min_z) * sz, 0.0f), 1.0f) * 1023.0f;
-    keys[i] = (spread10(static_cast<std::uint32_t>(fx)) << 2) | (spread10(static_cast<std::uint32_t>(fy)) << 1) |
-              spread10(static_cast<std::uint32_t>(fz));
+    keys[i] = hilbert_key10(static_cast<std::uint32_t>(fx), static_cast<std::uint32_t>(fy),
+                            static_cast<std::uint32_t>(fz));
     ids[i] = static_cast<std::uint32_t>(i);
 }
 
@@ -621,7 +655,7 @@ void launch_taylor_cap(
     // The value alone reads the evaluation records; the gradient the traversal records.
     float4 const *records = Grad ? m.records.ptr : m.eval_records.ptr;
     if (order != nullptr) {
-        // Morton-ordered queries: neighbouring lanes share most of their paths.
+        // Hilbert-ordered queries: neighbouring lanes share most of their paths.
         taylor_wave_kernel<Order, Prec, Grad, Cap><<<grid_for(n, k_taylor_block), k_taylor_block, 0, stream>>>(
             records, bvh.triangles.ptr, root, qx, qy, qz, order, o, n, beta2
         );
@@ -714,7 +748,7 @@ __device__ inline void wave_atomic_add(float *__restrict__ d, float const (&f)[F
 }
 
 // The adjoint of taylor_adjoint_kernel, wave-cooperative: a wave walks the tree once for its 64
-// (Morton-ordered) queries with the same per-query accept/descend decisions, and the field and
+// (Hilbert-ordered) queries with the same per-query accept/descend decisions, and the field and
 // vertex adjoints of a child slot or near triangle are summed over the wave and added with one
 // atomic instruction over consecutive floats, instead of every query adding its own (the top
 // slots are shared by every query, so per-query atomics serialise on them).
@@ -865,7 +899,7 @@ void launch_taylor_adjoint(
     std::uint32_t const root = leaf_root ? k_invalid : static_cast<std::uint32_t>(ref_offset(bvh.root.reference));
     int const need = leaf_root ? 0 : static_cast<int>(bvh.internal_stack_bound);
     float4 const *records = leaf_root ? nullptr : m.records.ptr;
-    // Morton-ordered queries (large batches) take the wave-cooperative form.
+    // Hilbert-ordered queries (large batches) take the wave-cooperative form.
 #define GWN_ADJ(C)                                                                                           \
     do {                                                                                                     \
         if (order != nullptr && !leaf_root)                                                                  \
@@ -972,13 +1006,13 @@ void winding_exact(
 
 namespace {
 
-// Device words the Morton presort of n queries needs besides the order itself: keys, their
+// Device words the Hilbert presort of n queries needs besides the order itself: keys, their
 // alternate buffer, the alternate id buffer and the sort's own workspace.
 std::size_t query_order_workspace_words(std::size_t n) {
     return 3 * n + radix_sort_workspace_words(n, 0, 30);
 }
 
-// Morton presort with caller-given scratch (query_order_workspace_words(n) words): allocates
+// Hilbert presort with caller-given scratch (query_order_workspace_words(n) words): allocates
 // nothing.
 void query_order_ws(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz,
@@ -987,7 +1021,7 @@ void query_order_ws(
     std::uint32_t *const keys = ws;
     std::uint32_t *const keys_alt = ws + n;
     std::uint32_t *const ids_alt = ws + 2 * n;
-    query_morton_kernel<<<grid_for(n, 256), 256, 0, stream>>>(qx, qy, qz, n, bvh.root.bounds, keys, order);
+    query_key_kernel<<<grid_for(n, 256), 256, 0, stream>>>(qx, qy, qz, n, bvh.root.bounds, keys, order);
     GWN_LAUNCH_CHECK();
     std::uint32_t *sk = nullptr;
     std::uint32_t *sorted = nullptr;
@@ -1068,7 +1102,7 @@ void winding_taylor(
 }
 
 // ---------------------------------------------------------------------------------------------
-// Allocation-free Taylor batch.  Its only scratch is the Morton presort's (the visiting order,
+// Allocation-free Taylor batch.  Its only scratch is the Hilbert presort's (the visiting order,
 // keys, alternate buffers and the sort's digit table), which lives in one static device array per
 // GPU, sized for k_noalloc_max_queries queries.  Host code never passes the array's name to a
 // kernel: that name is a host-side stand-in.  The array's device address is looked up once per

@@ -675,7 +675,7 @@ def gwn_compute_winding_number_taylor_batch_noalloc(bvh: BvhObject, moment: Mome
                                                     precision: Precision = Precision.PLAIN) -> Status:
     """The Taylor winding batch without device allocation (no reference counterpart).
 
-    Same results as ``gwn_compute_winding_number_taylor_batch``; the Morton presort's scratch is
+    Same results as ``gwn_compute_winding_number_taylor_batch``; the Hilbert presort's scratch is
     the library's static per-GPU array, so at most ``NOALLOC_MAX_QUERIES`` queries per call, and
     calls that overlap on one GPU race.
     """
@@ -936,7 +936,7 @@ class BatchObject:
 
     def set_tuning(self, block_size: int, chunk: int, blocks_per_cu: int, traversal: int = 0,
                    stream=None) -> Status:
-        """traversal 0: one query per lane; 1: wave-cooperative over each mesh's Morton order."""
+        """traversal 0: one query per lane; 1: wave-cooperative over each mesh's Hilbert order."""
         t = _BatchTuning(block_size, chunk, blocks_per_cu, traversal, 0, 0, 0)
         return _status(_lib.gwn_batch_set_tuning(self._h, ctypes.byref(t), _stream(stream)))
 
@@ -963,7 +963,7 @@ class BatchObject:
 
 
 def gwn_compute_query_order(bvh: BvhObject, qx, qy, qz, order, stream=None) -> Status:
-    """Morton visiting order of query points (uint32 device array, one entry per query)."""
+    """Hilbert visiting order of query points (uint32 device array, one entry per query)."""
     bad = _check_soa(qx, qy, qz, [order])
     if bad:
         return bad

@@ -44,6 +44,53 @@ def morton_bitloop_u32(qx: int, qy: int, qz: int) -> int:
     return code
 
 
+def hilbert_key10(x, y, z) -> np.ndarray:
+    """Position of 1024^3 grid cells along the 3-D Hilbert curve, Skilling's transpose method
+    ("Programming the Hilbert curve", AIP Conf. Proc. 707, 2004), vectorised: 30-bit keys with
+    axis 0's transposed bit the most significant of each triple."""
+    X = [np.asarray(v, dtype=np.uint64) & np.uint64(0x3FF) for v in (x, y, z)]
+    Q = 1 << 9
+    while Q > 1:
+        P = np.uint64(Q - 1)
+        q = np.uint64(Q)
+        for i in range(3):
+            bit = (X[i] & q) != 0
+            t = (X[0] ^ X[i]) & P
+            x0 = np.where(bit, X[0] ^ P, X[0] ^ t)
+            if i:
+                X[i] = np.where(bit, X[i], X[i] ^ t)
+            X[0] = x0
+        Q >>= 1
+    X[1] = X[1] ^ X[0]
+    X[2] = X[2] ^ X[1]
+    t = np.zeros_like(X[0])
+    Q = 1 << 9
+    while Q > 1:
+        t = np.where((X[2] & np.uint64(Q)) != 0, t ^ np.uint64(Q - 1), t)
+        Q >>= 1
+    X = [v ^ t for v in X]
+    key = np.zeros_like(X[0])
+    for b in range(10):
+        for a in range(3):
+            key |= ((X[a] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + 2 - a)
+    return key
+
+
+def query_cells(qx, qy, qz, root_bounds) -> tuple:
+    """The presort's grid cell of each query: FP32 (q - min) * (1 / extent), clamped to [0, 1],
+    times 1023, truncated (the library's arithmetic, operation by operation)."""
+    lo = np.asarray(root_bounds[:3], np.float32)
+    hi = np.asarray(root_bounds[3:], np.float32)
+    out = []
+    for k, q in enumerate((qx, qy, qz)):
+        ext = np.float32(hi[k] - lo[k])
+        s = np.float32(1.0) / ext if ext > 0 else np.float32(1.0)
+        f = (np.asarray(q, np.float32) - lo[k]) * s
+        f = np.minimum(np.maximum(f, np.float32(0.0)), np.float32(1.0)) * np.float32(1023.0)
+        out.append(np.nan_to_num(f, nan=0.0).astype(np.uint32))
+    return tuple(out)
+
+
 def _delta(codes, i, j) -> int:
     """Common-prefix length of keys i and j, extended by the index for equal keys."""
     n = len(codes)

@@ -155,6 +155,24 @@ def test_taylor_split_matches_oracle_replay(gwn, orc, dev, order, sort):
     assert np.array_equal(cnt.cpu().numpy().reshape(-1, 3).astype(np.uint64), ref_cnt)
 
 
+def test_query_order_is_the_hilbert_order(gwn, dev):
+    """The presort visits queries in the order of their Hilbert keys on the 1024^3 grid over the
+    root box, ties in input order (a stable sort): equal to numpy's stable argsort of the keys the
+    test-side restatement computes."""
+    import torch
+
+    from tests import helpers
+
+    _, bvh = build_gpu(gwn, MESHES["displaced_256x128"])
+    qx, qy, qz = _queries(100_003, seed=41, box=1.7)  # some outside the root box: clamped
+    d = [_t(a, dev) for a in (qx, qy, qz)]
+    order = torch.empty(100_003, dtype=torch.int32, device=dev)
+    gwn.gwn_compute_query_order(bvh, *d, order).raise_if_error()
+    cells = helpers.query_cells(qx, qy, qz, bvh.info()["root_bounds"])
+    want = np.argsort(helpers.hilbert_key10(*cells), kind="stable")
+    assert np.array_equal(order.cpu().numpy(), want)
+
+
 def test_query_order_is_a_permutation_and_reusable(gwn, dev):
     import torch
 
@@ -262,7 +280,7 @@ def test_taylor_nan_query(gwn, dev):
 
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 4096, 4097, 8192, 8193, 70_000, 262_145])
 def test_query_order_permutation_repeated(gwn, dev, n):
-    """The Morton presort is a permutation on every call (each query is visited exactly once)."""
+    """The query presort is a permutation on every call (each query is visited exactly once)."""
     import torch
 
     _, bvh = build_gpu(gwn, MESHES["icosphere4"])

@@ -1363,3 +1363,39 @@ def test_hploc_merges_equal_the_restated_definition(orc, name, radius):
         got[(left, right)] = n
         assert n == max(left[0], right[0]) - 1
     assert set(got) == {(tuple(sorted(a)), tuple(sorted(b))) for a, b in want}
+
+
+# ---------------------------------------------------------------------------------------------
+# Query presort key (test-side restatement of the Hilbert curve)
+# ---------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("bits", [1, 2, 3, 4])
+def test_hilbert_key_visits_every_cell_by_face_steps(bits):
+    """On a 2^b grid (the 1024^3 grid's top b bits) the keys are a bijection onto [0, 8^b) and
+    consecutive keys are face-adjacent cells: the defining property of the Hilbert curve, which
+    neither a Morton interleave nor a transposed or mis-ordered Gray step satisfies."""
+    n = 1 << bits
+    g = np.stack(np.meshgrid(*(np.arange(n),) * 3, indexing="ij")).reshape(3, -1)
+    shift = 10 - bits
+    key = helpers.hilbert_key10(*(g << shift)) >> np.uint64(3 * shift)
+    assert np.array_equal(np.sort(key), np.arange(n ** 3, dtype=np.uint64))
+    walk = g[:, np.argsort(key)]
+    assert np.all(np.abs(np.diff(walk, axis=1)).sum(axis=0) == 1)
+    assert tuple(walk[:, 0]) == (0, 0, 0)
+
+
+def test_hilbert_key_blocks_at_full_resolution():
+    """At 10 bits, every aligned 2^L block (L = 1, 2, 3) at random places holds a contiguous run
+    of 8^L keys walked by face steps: the curve's recursion is right down to the lowest bits."""
+    rng = np.random.default_rng(3)
+    for level in (1, 2, 3):
+        m = 1 << level
+        local = np.stack(np.meshgrid(*(np.arange(m),) * 3, indexing="ij")).reshape(3, -1)
+        for _ in range(16):
+            origin = rng.integers(0, 1024 // m, size=3)[:, None] * m
+            cells = local + origin
+            key = helpers.hilbert_key10(*cells).astype(np.int64)
+            ks = np.sort(key)
+            assert np.array_equal(ks, np.arange(ks[0], ks[0] + m ** 3))
+            walk = cells[:, np.argsort(key)]
+            assert np.all(np.abs(np.diff(walk, axis=1)).sum(axis=0) == 1)
