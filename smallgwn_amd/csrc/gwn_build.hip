@@ -245,36 +245,54 @@ __device__ inline aabb aabb_union_ref(aabb const &a, aabb const &b) {
                 fmax_ref(a.max_x, b.max_x), fmax_ref(a.max_y, b.max_y), fmax_ref(a.max_z, b.max_z)};
 }
 
-// Leaf-to-root climb: the second child to arrive at a node unions both children in left/right
-// order, so the result is independent of arrival order.  Counters reset themselves to zero.
-__global__ __launch_bounds__(k_block) void bounds_kernel(
-    aabb const *__restrict__ leaf_boxes, uint2 const *__restrict__ bnodes,
-    std::uint32_t const *__restrict__ iparent, std::uint32_t const *__restrict__ lparent,
-    std::size_t n, aabb *ibounds, std::uint32_t *ifirst, std::uint32_t *arrivals
+// Depth of every internal node of the Karras tree (edges to the root), by following the parent
+// links: the parent array is small enough to stay in L2, and the tree's height bounds each walk.
+// A walk longer than the node count means a broken parent chain: error.
+__global__ __launch_bounds__(k_block) void lbvh_depth_kernel(
+    std::uint32_t const *__restrict__ iparent, std::size_t ni, std::uint32_t *__restrict__ depth,
+    std::uint32_t *__restrict__ max_depth, std::uint32_t *__restrict__ error
 ) {
-    std::size_t const leaf = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (leaf >= n || n < 2)
-        return;
-    std::uint32_t link = lparent[leaf];
-    while (link != k_invalid) {
-        std::uint32_t const node = link >> 1;
-        release_agent();
-        std::uint32_t const prior =
-            __hip_atomic_fetch_add(&arrivals[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (prior == 0u)
-            return;
-        acquire_agent();
-        arrivals[node] = 0u;
-        uint2 const bn = bnodes[node];
-        std::uint32_t const l = bn.x & ~k_leaf_bit, r = bn.y & ~k_leaf_bit;
-        aabb const lb = (bn.x & k_leaf_bit) ? leaf_boxes[l] : ibounds[l];
-        aabb const rb = (bn.y & k_leaf_bit) ? leaf_boxes[r] : ibounds[r];
-        std::uint32_t const lf = (bn.x & k_leaf_bit) ? l : ifirst[l];
-        std::uint32_t const rf = (bn.y & k_leaf_bit) ? r : ifirst[r];
-        ibounds[node] = aabb_union_ref(lb, rb);
-        ifirst[node] = lf < rf ? lf : rf;
-        link = iparent[node];
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    std::uint32_t d = 0;
+    if (i < ni) {
+        std::uint32_t link = iparent[i];
+        while (link != k_invalid) {
+            if (++d > ni || (link >> 1) >= ni) {
+                atomicOr(error, 1u);
+                d = 0;
+                break;
+            }
+            link = iparent[link >> 1];
+        }
+        depth[i] = d;
     }
+    // Wave maximum first (every lane takes part), then one atomic per wave.
+    for (int off = 32; off > 0; off >>= 1) {
+        std::uint32_t const o = __shfl_xor(d, off);
+        d = o > d ? o : d;
+    }
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(max_depth, d);
+}
+
+// Bounds of the internal nodes at one depth, deepest depth first: both children are leaves or
+// were written by an earlier launch, so each node unions them in left/right order (the
+// reference's order) with no atomics or fences; the kernel boundary orders the depths.
+__global__ __launch_bounds__(k_block) void lbvh_bounds_level_kernel(
+    aabb const *__restrict__ leaf_boxes, uint2 const *__restrict__ bnodes, std::uint32_t const *__restrict__ depth,
+    std::size_t ni, std::uint32_t level, aabb *__restrict__ ibounds, std::uint32_t *__restrict__ ifirst
+) {
+    std::size_t const node = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    if (node >= ni || depth[node] != level)
+        return;
+    uint2 const bn = bnodes[node];
+    std::uint32_t const l = bn.x & ~k_leaf_bit, r = bn.y & ~k_leaf_bit;
+    aabb const lb = (bn.x & k_leaf_bit) ? leaf_boxes[l] : ibounds[l];
+    aabb const rb = (bn.y & k_leaf_bit) ? leaf_boxes[r] : ibounds[r];
+    std::uint32_t const lf = (bn.x & k_leaf_bit) ? l : ifirst[l];
+    std::uint32_t const rf = (bn.y & k_leaf_bit) ? r : ifirst[r];
+    ibounds[node] = aabb_union_ref(lb, rb);
+    ifirst[node] = lf < rf ? lf : rf;
 }
 
 __device__ inline float half_area(aabb const &b) {
@@ -872,14 +890,25 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, int method, s
         );
         GWN_LAUNCH_CHECK();
     } else {
-        device_buffer<std::uint32_t> iparent(NI, stream), lparent(T, stream), arrivals(NI, stream);
-        fill_u32(arrivals.ptr, 0u, NI, stream);
+        device_buffer<std::uint32_t> iparent(NI, stream), lparent(T, stream), depth(NI, stream), words(2, stream);
+        fill_u32(words.ptr, 0u, 2, stream);
         lbvh_kernel<<<grid_for(NI, k_block), k_block, 0, stream>>>(sorted_codes, T, bnodes.ptr, iparent.ptr, lparent.ptr);
         GWN_LAUNCH_CHECK();
-        bounds_kernel<<<grid_for(T, k_block), k_block, 0, stream>>>(
-            leaf_boxes.ptr, bnodes.ptr, iparent.ptr, lparent.ptr, T, ibounds.ptr, ifirst.ptr, arrivals.ptr
-        );
+        // Bounds level by level, deepest first: the depths come from the parent links, and one
+        // read-back of the tree's height sizes the loop (no leaf-to-root climb, no fences).
+        lbvh_depth_kernel<<<grid_for(NI, k_block), k_block, 0, stream>>>(iparent.ptr, NI, depth.ptr, words.ptr,
+                                                                         words.ptr + 1);
         GWN_LAUNCH_CHECK();
+        std::uint32_t host_words[2] = {0, 0};
+        GWN_HIP(hipMemcpyAsync(host_words, words.ptr, sizeof(host_words), hipMemcpyDeviceToHost, stream));
+        GWN_HIP(hipStreamSynchronize(stream));
+        if (host_words[1] != 0u)
+            throw_internal("bvh.build: LBVH parent links do not form a tree.");
+        for (std::uint32_t d = host_words[0] + 1; d-- > 0;) {
+            lbvh_bounds_level_kernel<<<grid_for(NI, k_block), k_block, 0, stream>>>(
+                leaf_boxes.ptr, bnodes.ptr, depth.ptr, NI, d, ibounds.ptr, ifirst.ptr);
+            GWN_LAUNCH_CHECK();
+        }
     }
 
     // Level-synchronous collapse into oversized arrays (wide nodes <= binary internal nodes).

@@ -242,6 +242,43 @@ struct gwn_moment_handle {
 
 namespace gwn {
 
+// Level-synchronous passes over a wide tree's breadth-first levels, deepest first.  A run of
+// consecutive small levels (at most `small` nodes each -- one per thread of the fused launch --
+// and at most k_max_run_levels of them) goes to one single-workgroup launch that steps through
+// the levels with a workgroup barrier between them (the workgroup's writes are visible to it
+// after the barrier); every other level is one launch over its nodes.  Deep trees end in long
+// chains of tiny levels, whose launches would otherwise cost more than their work.
+constexpr int k_max_run_levels = 32;
+struct level_run {
+    std::uint32_t first[k_max_run_levels];
+    std::uint32_t count[k_max_run_levels];
+    int levels = 0;   // > 0: a fused run of small levels, in the order to process them
+    std::uint32_t big_first = 0, big_count = 0;  // levels == 0: one big level
+};
+// The launches for levels [0, level_start.size() - 1), deepest first.
+inline std::vector<level_run> plan_level_runs(std::vector<std::uint32_t> const &level_start, std::uint32_t small) {
+    std::vector<level_run> runs;
+    for (std::size_t d = level_start.size() - 1; d-- > 0;) {
+        std::uint32_t const first = level_start[d], count = level_start[d + 1] - level_start[d];
+        if (count == 0)
+            continue;
+        if (count <= small) {
+            if (runs.empty() || runs.back().levels == 0 || runs.back().levels == k_max_run_levels)
+                runs.emplace_back();
+            level_run &r = runs.back();
+            r.first[r.levels] = first;
+            r.count[r.levels] = count;
+            ++r.levels;
+        } else {
+            level_run r;
+            r.big_first = first;
+            r.big_count = count;
+            runs.push_back(r);
+        }
+    }
+    return runs;
+}
+
 // Radix sort of (key, value) pairs on bits [begin_bit, end_bit).  Keys are uint32 or uint64;
 // values are uint32.  The result ends in (keys_out, vals_out); the inputs are clobbered.
 void radix_sort_pairs_u64(

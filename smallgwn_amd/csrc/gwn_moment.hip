@@ -29,71 +29,102 @@ using namespace moment_math;
 constexpr int k_block = 128;
 constexpr std::uint32_t k_invalid = 0xffffffffu;
 
-// One node of a level pass: the payload of each child slot (leaf payloads straight from the
-// triangle records, internal ones from the child's total written by the previous, deeper level),
-// the node's recentred coefficients and slot areas, and the node's total in slot order from the
-// first valid slot (the reference's combine_node, gwn_bvh_refit_moment_impl.cuh:17-99).  The whole
-// record is written, empty slots and padding as zeros, so the output needs no clearing pass.
+// One child slot of a level pass, four lanes per node (lane & 3 = the slot; the lanes of a node
+// are one quad of a wave, and every lane of the wave takes part, `active` or not): the slot's
+// payload (a leaf's straight from its triangle record, an internal child's from the total the
+// previous, deeper level wrote), its recentred coefficients and area, then the node's total in
+// slot order from the first valid slot (the reference's combine_node,
+// gwn_bvh_refit_moment_impl.cuh:17-99), folded by the quad's slot-0 lane from the payloads the
+// quad leaves in `share` / `share_have` (one entry per lane of the workgroup; every lane of the
+// workgroup calls this, since it holds a workgroup barrier).  The whole record is written, empty slots and padding as
+// zeros, so the output needs no clearing pass; the four lanes write each field's four slots as
+// one 16-byte span.
 template <int Order>
-__device__ inline void moment_refit_node(
-    std::size_t n, bvh_node4 const *__restrict__ nodes, std::size_t node_count, tri_record const *__restrict__ tris,
-    std::size_t tri_count, payload<Order> *__restrict__ pending, float4 *__restrict__ records,
-    float *__restrict__ slot_area, std::uint32_t *error
+__device__ inline void moment_refit_slot(
+    std::size_t n, int s, bool active, bvh_node4 const *__restrict__ nodes, std::size_t node_count,
+    tri_record const *__restrict__ tris, std::size_t tri_count, payload<Order> *__restrict__ pending,
+    float4 *__restrict__ records, float *__restrict__ slot_area, std::uint32_t *error, payload<Order> *share,
+    std::uint32_t *share_have
 ) {
     constexpr int RV = moment_record_vec4(Order);
     constexpr int F = moment_field_count(Order);
-    float4 *rec = records + n * RV;
-    float *fields = reinterpret_cast<float *>(rec + 1);
-    std::uint32_t cref[4] = {k_invalid, k_invalid, k_invalid, k_invalid};
-    payload<Order> total;
-    payload_zero(total);
-    bool init = false;
-    for (int s = 0; s < 4; ++s) {
+    payload<Order> child;
+    payload_zero(child);
+    bool have = false;
+    if (active) {
+        float4 *rec = records + n * RV;
+        float *fields = reinterpret_cast<float *>(rec + 1);
         bvh_child const ch = nodes[n].children[s];
         std::uint64_t const ref = ch.reference;
-        if (!ref_valid(ref)) {
+        std::uint32_t cref = k_invalid;
+        bool ok = true;
+        if (ref_valid(ref)) {
+            if (ref_internal(ref)) {
+                std::uint64_t const c = ref_offset(ref);
+                if (c >= node_count || c <= n) {
+                    ok = false;
+                } else {
+                    child = pending[c];
+                    cref = static_cast<std::uint32_t>(c);
+                }
+            } else {
+                // A leaf payload starts from zero and adds each primitive in order
+                // (gwn_bvh_refit_common.cuh:452-468).
+                std::uint64_t const off = ref_offset(ref);
+                std::uint32_t const cnt = ref_count(ref);
+                if (off + cnt > tri_count || cnt != 1u) {
+                    ok = false;
+                } else {
+                    payload<Order> tri;
+                    triangle_payload<Order>(tris[off], tri);
+                    payload_add(child, tri);
+                    cref = k_cref_leaf | static_cast<std::uint32_t>(off);
+                }
+            }
+            if (!ok) {
+                atomicOr(error, 1u);
+                payload_zero(child);
+            }
+        }
+        have = ok && ref_valid(ref);
+        if (have) {
+            write_fields<Order>(fields, s, ch.bounds, child);
+        } else {
             for (int f = 0; f < F; ++f)
                 fields[f * 4 + s] = 0.0f;
-            slot_area[n * 4 + s] = 0.0f;
-            continue;
         }
-        payload<Order> child;
-        if (ref_internal(ref)) {
-            std::uint64_t const c = ref_offset(ref);
-            if (c >= node_count || c <= n) {
-                atomicOr(error, 1u);
-                return;
-            }
-            child = pending[c];
-            cref[s] = static_cast<std::uint32_t>(c);
-        } else {
-            // A leaf payload starts from zero and adds each primitive in order
-            // (gwn_bvh_refit_common.cuh:452-468).
-            std::uint64_t const off = ref_offset(ref);
-            std::uint32_t const cnt = ref_count(ref);
-            if (off + cnt > tri_count || cnt != 1u) {
-                atomicOr(error, 1u);
-                return;
-            }
-            payload_zero(child);
-            payload<Order> tri;
-            triangle_payload<Order>(tris[off], tri);
-            payload_add(child, tri);
-            cref[s] = k_cref_leaf | static_cast<std::uint32_t>(off);
-        }
-        write_fields<Order>(fields, s, ch.bounds, child);
-        slot_area[n * 4 + s] = child.v[0];
-        if (init)
-            payload_add(total, child);
-        else
-            total = child;
-        init = true;
+        slot_area[n * 4 + s] = have ? child.v[0] : 0.0f;
+        reinterpret_cast<std::uint32_t *>(rec)[s] = cref;
+        for (int k = 4 + F * 4 + s; k < RV * 4; k += 4)
+            reinterpret_cast<float *>(rec)[k] = 0.0f;
     }
-    for (int k = 4 + F * 4; k < RV * 4; ++k)
-        reinterpret_cast<float *>(rec)[k] = 0.0f;
-    rec[0] = make_float4(__uint_as_float(cref[0]), __uint_as_float(cref[1]), __uint_as_float(cref[2]),
-                         __uint_as_float(cref[3]));
-    pending[n] = total;
+    // Slot order from the first valid slot: each lane leaves its payload (zero if its slot is
+    // empty) and a flag, then the quad's lane 0 folds lanes 1..3 in.
+    std::uint32_t const lane = threadIdx.x;
+    share[lane] = child;
+    share_have[lane] = have ? 1u : 0u;
+    __syncthreads();
+    if (active && s == 0) {
+        payload<Order> total;
+        bool init = have;
+        if (have)
+            total = share[lane];
+        for (int k = 1; k < 4; ++k) {
+            if (share_have[lane + k] == 0u)
+                continue;
+            payload<Order> const &o = share[lane + k];
+            if (init) {
+                payload_add(total, o);
+            } else {
+                total = o;
+                init = true;
+            }
+        }
+        if (!init)
+            payload_zero(total);
+        pending[n] = total;
+    }
+    __syncthreads();
 }
 
 // One breadth-first level [first, first + count) of one tree; levels run deepest first, so every
@@ -104,10 +135,33 @@ __global__ __launch_bounds__(k_block) void moment_level_kernel(
     std::size_t tri_count, std::size_t first, std::size_t count, payload<Order> *__restrict__ pending,
     float4 *__restrict__ records, float *__restrict__ slot_area, std::uint32_t *error
 ) {
-    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (i >= count)
-        return;
-    moment_refit_node<Order>(first + i, nodes, node_count, tris, tri_count, pending, records, slot_area, error);
+    std::size_t const g = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    std::size_t const i = g >> 2;
+    __shared__ payload<Order> share[k_block];
+    __shared__ std::uint32_t share_have[k_block];
+    moment_refit_slot<Order>(first + i, static_cast<int>(g & 3), i < count, nodes, node_count, tris, tri_count, pending,
+                             records, slot_area, error, share, share_have);
+}
+
+// A run of small levels (at most k_fused_block nodes each) in one workgroup, a barrier between
+// levels.
+constexpr int k_fused_block = 512;
+template <int Order>
+__global__ __launch_bounds__(k_fused_block) void moment_levels_fused_kernel(
+    bvh_node4 const *__restrict__ nodes, std::size_t node_count, tri_record const *__restrict__ tris,
+    std::size_t tri_count, level_run run, payload<Order> *__restrict__ pending, float4 *__restrict__ records,
+    float *__restrict__ slot_area, std::uint32_t *error
+) {
+    __shared__ payload<Order> share[k_fused_block];
+    __shared__ std::uint32_t share_have[k_fused_block];
+    for (int k = 0; k < run.levels; ++k) {
+        for (std::uint32_t base = 0; base < run.count[k]; base += k_fused_block / 4) {
+            std::uint32_t const i = base + (threadIdx.x >> 2);
+            moment_refit_slot<Order>(run.first[k] + i, static_cast<int>(threadIdx.x & 3), i < run.count[k], nodes,
+                                     node_count, tris, tri_count, pending, records, slot_area, error, share, share_have);
+        }
+        __syncthreads();
+    }
 }
 
 // Batched refits: one launch per depth over that depth's nodes in every mesh.  Thread g finds its
@@ -129,18 +183,24 @@ __global__ __launch_bounds__(k_block) void moment_level_batch_kernel(
     moment_refit_mesh const *__restrict__ meshes, refit_segment const *__restrict__ seg, std::uint32_t seg_count,
     std::uint64_t total, std::uint32_t *error
 ) {
-    std::uint64_t const g = static_cast<std::uint64_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (g >= total)
-        return;
-    refit_segment const sg = seg[segment_of(seg, seg_count, g)];
-    moment_refit_mesh const m = meshes[sg.mesh];
-    std::size_t const n = static_cast<std::size_t>(sg.first_node) + static_cast<std::size_t>(g - sg.prefix);
-    if (n >= m.node_count) {
-        atomicOr(error, 1u);
-        return;
+    std::uint64_t const t = static_cast<std::uint64_t>(blockIdx.x) * k_block + threadIdx.x;
+    std::uint64_t const g = t >> 2;  // node of this quad within the depth's nodes of every mesh
+    bool active = g < total;
+    moment_refit_mesh m{};
+    std::size_t n = 0;
+    if (active) {
+        refit_segment const sg = seg[segment_of(seg, seg_count, g)];
+        m = meshes[sg.mesh];
+        n = static_cast<std::size_t>(sg.first_node) + static_cast<std::size_t>(g - sg.prefix);
+        if (n >= m.node_count) {
+            atomicOr(error, 1u);
+            active = false;
+        }
     }
-    moment_refit_node<Order>(n, m.nodes, m.node_count, m.tris, m.tri_count, static_cast<payload<Order> *>(m.pending),
-                             m.records, m.slot_area, error);
+    __shared__ payload<Order> share[k_block];
+    __shared__ std::uint32_t share_have[k_block];
+    moment_refit_slot<Order>(n, static_cast<int>(t & 3), active, m.nodes, m.node_count, m.tris, m.tri_count,
+                             static_cast<payload<Order> *>(m.pending), m.records, m.slot_area, error, share, share_have);
 }
 
 // Evaluation records from finished traversal records: one thread per (node, child slot) over
@@ -216,14 +276,15 @@ template <int Order>
 void launch_levels(gwn_bvh_handle const &bvh, gwn_moment_handle &m, void *pending, std::uint32_t *error,
                    hipStream_t stream) {
     std::size_t const N = bvh.node_count;
-    auto const &ls = bvh.level_start;
-    for (std::size_t d = ls.size() - 1; d-- > 0;) {
-        std::size_t const first = ls[d], count = ls[d + 1] - ls[d];
-        if (count == 0)
-            continue;
-        moment_level_kernel<Order><<<grid_for(count, k_block), k_block, 0, stream>>>(
-            bvh.nodes.ptr, N, bvh.triangles.ptr, bvh.triangle_count, first, count,
-            static_cast<payload<Order> *>(pending), m.records.ptr, m.slot_area.ptr, error);
+    for (level_run const &r : plan_level_runs(bvh.level_start, k_fused_block)) {
+        if (r.levels > 0)
+            moment_levels_fused_kernel<Order><<<1, k_fused_block, 0, stream>>>(
+                bvh.nodes.ptr, N, bvh.triangles.ptr, bvh.triangle_count, r, static_cast<payload<Order> *>(pending),
+                m.records.ptr, m.slot_area.ptr, error);
+        else
+            moment_level_kernel<Order><<<grid_for(4 * std::size_t(r.big_count), k_block), k_block, 0, stream>>>(
+                bvh.nodes.ptr, N, bvh.triangles.ptr, bvh.triangle_count, r.big_first, r.big_count,
+                static_cast<payload<Order> *>(pending), m.records.ptr, m.slot_area.ptr, error);
         GWN_LAUNCH_CHECK();
     }
 }
@@ -275,7 +336,7 @@ void refit_moment_batch(moment_refit_mesh const *meshes, refit_segment const *se
         refit_level const &lv = levels[d];
         if (lv.nodes == 0)
             continue;
-        unsigned const grid = grid_for(lv.nodes, k_block);
+        unsigned const grid = grid_for(4 * lv.nodes, k_block);
         refit_segment const *seg = segments + lv.first_segment;
         switch (order) {
         case 0: moment_level_batch_kernel<0><<<grid, k_block, 0, stream>>>(meshes, seg, lv.segments, lv.nodes, error); break;

@@ -64,14 +64,8 @@ __device__ inline bool refit_leaf_range(
     return true;
 }
 
-__global__ __launch_bounds__(k_block) void refit_level_kernel(
-    refit_geometry g, bvh_node4 *nodes, std::size_t node_count, std::size_t first, std::size_t count,
-    aabb *pending, std::uint32_t *error
-) {
-    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
-    if (i >= count)
-        return;
-    std::size_t const n = first + i;
+__device__ inline void refit_node(refit_geometry const &g, bvh_node4 *nodes, std::size_t node_count, std::size_t n,
+                                  aabb *pending, std::uint32_t *error) {
     aabb acc{};
     bool init = false;
     for (int s = 0; s < 4; ++s) {
@@ -101,6 +95,28 @@ __global__ __launch_bounds__(k_block) void refit_level_kernel(
     pending[n] = acc;
 }
 
+__global__ __launch_bounds__(k_block) void refit_level_kernel(
+    refit_geometry g, bvh_node4 *nodes, std::size_t node_count, std::size_t first, std::size_t count,
+    aabb *pending, std::uint32_t *error
+) {
+    std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
+    if (i < count)
+        refit_node(g, nodes, node_count, first + i, pending, error);
+}
+
+// A run of small levels (at most k_fused_block nodes each) in one workgroup, a barrier between
+// levels.
+constexpr int k_fused_block = 1024;
+__global__ __launch_bounds__(k_fused_block) void refit_levels_fused_kernel(
+    refit_geometry g, bvh_node4 *nodes, std::size_t node_count, level_run run, aabb *pending, std::uint32_t *error
+) {
+    for (int k = 0; k < run.levels; ++k) {
+        if (threadIdx.x < run.count[k])
+            refit_node(g, nodes, node_count, run.first[k] + threadIdx.x, pending, error);
+        __syncthreads();
+    }
+}
+
 // A leaf root owns the whole (at most 65535-primitive) range: one thread, no climb.
 __global__ void refit_leaf_root_kernel(refit_geometry g, std::uint64_t root_ref, aabb *root_box, std::uint32_t *error) {
     aabb box;
@@ -125,12 +141,13 @@ void refit_bounds(gwn_geometry_handle const &geo, gwn_bvh_handle &bvh, hipStream
         auto const &ls = bvh.level_start;
         if (ls.size() < 2 || ls.front() != 0u || ls.back() != N)
             throw_internal("bvh.refit: BVH level table does not cover its nodes.");
-        for (std::size_t d = ls.size() - 1; d-- > 0;) {
-            std::size_t const first = ls[d], count = ls[d + 1] - ls[d];
-            if (count == 0)
-                continue;
-            refit_level_kernel<<<grid_for(count, k_block), k_block, 0, stream>>>(
-                g, bvh.nodes.ptr, N, first, count, boxes.ptr, error.ptr);
+        for (level_run const &r : plan_level_runs(ls, k_fused_block)) {
+            if (r.levels > 0)
+                refit_levels_fused_kernel<<<1, k_fused_block, 0, stream>>>(g, bvh.nodes.ptr, N, r, boxes.ptr,
+                                                                           error.ptr);
+            else
+                refit_level_kernel<<<grid_for(r.big_count, k_block), k_block, 0, stream>>>(
+                    g, bvh.nodes.ptr, N, r.big_first, r.big_count, boxes.ptr, error.ptr);
             GWN_LAUNCH_CHECK();
         }
     } else {
