@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define GWN_HIP_ABI_VERSION 2
+#define GWN_HIP_ABI_VERSION 3
 
 typedef enum gwn_error_code {
     GWN_SUCCESS = 0,
@@ -485,7 +485,8 @@ typedef struct gwn_batch_tuning {
     int32_t blocks_per_cu;   /* persistent grid = blocks_per_cu x compute units */
     int32_t traversal;       /* 0: one query per lane; 1: wave-cooperative over each mesh's
                                 Hilbert query order (computed at create) */
-    int32_t stack_capacity;  /* compiled traversal stack (read only) */
+    int32_t stack_capacity;  /* compiled traversal stack: 32, 64 or 128, at least every tree's
+                                internal_stack_bound; 0 in gwn_batch_set_tuning keeps it */
     uint64_t meshes;         /* read only */
     uint64_t queries;        /* read only */
 } gwn_batch_tuning;
@@ -500,12 +501,57 @@ int gwn_batch_get_tuning(gwn_batch_t batch, gwn_batch_tuning *tuning);
 int gwn_batch_tune(gwn_batch_t batch, hipStream_t stream, float *best_ms);
 int gwn_batch_capture(gwn_batch_t batch, int32_t include_refit);
 int gwn_batch_replay(gwn_batch_t batch, hipStream_t stream);
-/* Synchronizes `stream` and reports whether any moment refit of the batch (eager or in a graph
- * replay) met an invalid tree since the batch was made or since the last gwn_batch_status call:
- * GWN_ERROR_INTERNAL "batch: Bottom-up moment refit did not converge." (the word is then
- * cleared), else GWN_SUCCESS.  Refits enqueue without reading the word back, so they can be
- * captured. */
+/* Synchronizes `stream` and reports whether any moment refit or segmented build of the batch
+ * (eager or in a graph replay) failed since the batch was made or since the last
+ * gwn_batch_status call: GWN_ERROR_INTERNAL with "batch: Bottom-up moment refit did not
+ * converge.", "batch: A segmented BVH build failed.", or (a batch over geometries whose updated
+ * geometry gave a tree past what its launches were planned for) a message naming the level
+ * launches or the stack capacity; the word is then cleared.  Else GWN_SUCCESS.  Refits and builds
+ * enqueue without reading the word back, so they can be captured. */
 int gwn_batch_status(gwn_batch_t batch, hipStream_t stream);
+
+/* ---- batched mode over geometries (no reference counterpart).  The batch builds and owns every
+ *      mesh's BVH (H-PLOC, the given width and search radius) and Taylor moments of the given
+ *      order: all meshes' builds and moment refits run in one segmented pass of a fixed sequence
+ *      of launches (scene bounds, Morton codes, one stable sort by (mesh, code), H-PLOC over all
+ *      sorted leaves, one plan + one persistent launch per wide level of the collapse, the level
+ *      refit of the moments) that allocates nothing and never reads back, so it can be captured.
+ *      Each mesh's tree, records and moments equal gwn_build_bvh_f32 + gwn_refit_bvh_moment_f32
+ *      of that mesh alone, and its query results equal the single-mesh Taylor batch's, bit for bit.
+ *      gwn_batch_create_built builds once (synchronizes) and plans the pass: the number of wide
+ *      level launches (the deepest tree + 8) and the traversal stack.  gwn_batch_build enqueues the
+ *      pass; gwn_batch_capture(batch, 1) records pass + query launch, so a replay rebuilds every tree
+ *      from the geometries' current positions (gwn_update_geometry_f32 between replays).  A tree
+ *      past the plan sets the status word (gwn_batch_status); gwn_batch_set_build re-plans.
+ *      gwn_batch_tune picks the query launch shape and stack; gwn_batch_tune_build times the pass
+ *      plus the queries over widths {4, 3, 2} x radii {8, 4, 2, 1} and keeps the fastest
+ *      (synchronizes).  gwn_batch_export copies one mesh's tree and moments into BVH and moment
+ *      objects for the single-mesh entry points.  gwn_batch_refit_moments does not apply to such
+ *      a batch.  The geometry objects (Real = float) must outlive the batch and keep their storage
+ *      (gwn_update_geometry_f32 keeps it; an upload or clear replaces it and the batch refuses to
+ *      run).  Device memory held: about 0.9 KB per triangle at order 1 (node slots for the
+ *      widest case, moments and build scratch). ---- */
+typedef struct gwn_batch_build_item {
+    gwn_geometry_t geometry;
+    const float *qx, *qy, *qz;
+    float *out;
+    uint64_t count;
+} gwn_batch_build_item;
+
+typedef struct gwn_batch_build_config {
+    int32_t method;                /* GWN_BVH_BUILD_HPLOC */
+    int32_t width;                 /* 2, 3 or 4 */
+    uint32_t hploc_search_radius;  /* 1..8 */
+    int32_t order;                 /* Taylor order 0, 1 or 2 (fixed at create) */
+} gwn_batch_build_config;
+
+int gwn_batch_create_built(const gwn_batch_build_item *items, uint64_t item_count, const gwn_batch_build_config *config,
+                           float accuracy_scale, gwn_batch_t *out, hipStream_t stream);
+int gwn_batch_build(gwn_batch_t batch, hipStream_t stream);
+int gwn_batch_set_build(gwn_batch_t batch, const gwn_batch_build_config *config, hipStream_t stream);
+int gwn_batch_get_build(gwn_batch_t batch, gwn_batch_build_config *config, int32_t *level_launches);
+int gwn_batch_tune_build(gwn_batch_t batch, hipStream_t stream, float *best_ms);
+int gwn_batch_export(gwn_batch_t batch, uint64_t mesh, gwn_bvh_t bvh, gwn_moment_t moment, hipStream_t stream);
 
 /* Hilbert-curve-order permutation of query points within the BVH root bounds (positions on the
  * 1024^3 grid over the root box, 30-bit keys, stable radix sort): order[i] is the input index of

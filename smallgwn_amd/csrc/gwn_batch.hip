@@ -15,8 +15,12 @@
 //   - traversal: one query per lane, or wave-cooperative (a wave walks the tree once for its 64
 //     queries) over each mesh's Hilbert query order, computed once when the batch is made (any
 //     order gives the same results; Hilbert order keeps a wave's queries close together);
-//   - tune: traversal, block size, chunk length and blocks per CU are chosen by timing
-//     candidate launches.
+//   - tune: traversal, block size, chunk length, blocks per CU and the compiled traversal stack
+//     are chosen by timing candidate launches;
+//   - builds: a batch made over geometries (batch_create_built) owns every mesh's tree and
+//     moments and rebuilds them all in one segmented pass (gwn_batch_build.hip) that a graph can
+//     hold too, so a replay is builds + refits + queries; its tuner also picks the BVH width and
+//     the H-PLOC search radius.
 #include "gwn_core.hpp"
 
 #include <algorithm>
@@ -32,6 +36,13 @@ struct gwn_batch_handle {
     std::vector<gwn_bvh_handle const *> bvhs;
     std::vector<gwn_batch_storage> storage;
     std::vector<gwn_moment_handle *> moments;
+    // A batch over geometries: the batch builds and owns the trees and moments (bvhs, moments and
+    // the object refit tables stay empty).
+    std::vector<gwn_geometry_handle const *> geometries;
+    std::vector<std::uint64_t> geometry_storage;
+    gwn::batch_trees_ptr trees;
+    int build_levels = 0;          // wide levels each segmented pass collapses
+    std::uint32_t stack_bound = 0; // largest internal_stack_bound of the trees
     std::vector<gwn::batch_mesh> host_meshes;
     std::vector<gwn::moment_refit_work> work;
     std::vector<gwn::moment_refit_mesh> host_refits;  // meshes with an internal root
@@ -78,6 +89,8 @@ struct gwn_batch_handle {
 
 namespace gwn {
 
+void batch_drop_graph(gwn_batch_handle &b);
+
 namespace {
 
 void layout_chunks(gwn_batch_handle &b) {
@@ -117,6 +130,9 @@ gwn_batch_storage storage_of(gwn_bvh_handle const &bvh, gwn_moment_handle const 
 // batch was made.  Work that queries without refitting first also needs every moment formed
 // from its BVH's current revision, as the single-mesh batches require.
 void check_storage(gwn_batch_handle const &b, bool query_without_refit) {
+    for (std::size_t i = 0; i < b.geometries.size(); ++i)
+        if (b.geometries[i]->storage_generation != b.geometry_storage[i])
+            throw_invalid("batch: A mesh's geometry storage changed since the batch was made; create a new batch.");
     for (std::size_t i = 0; i < b.bvhs.size(); ++i) {
         gwn_batch_storage const now = storage_of(*b.bvhs[i], *b.moments[i]);
         gwn_batch_storage const &was = b.storage[i];
@@ -131,6 +147,7 @@ void check_storage(gwn_batch_handle const &b, bool query_without_refit) {
 
 // Every mesh's moment is formed from its BVH's current bounds once enqueued refits have run.
 void stamp_revisions(gwn_batch_handle &b) {
+    // (A batch over geometries holds no BVH or moment objects: bvhs is empty.)
     for (std::size_t i = 0; i < b.bvhs.size(); ++i) {
         b.moments[i]->node_count = b.bvhs[i]->node_count;
         b.moments[i]->bvh_revision = b.bvhs[i]->revision;
@@ -138,6 +155,10 @@ void stamp_revisions(gwn_batch_handle &b) {
 }
 
 void enqueue_refit(gwn_batch_handle &b, hipStream_t stream) {
+    if (b.trees) {
+        batch_trees_enqueue(*b.trees, b.build_levels, static_cast<std::uint32_t>(b.cap), b.status.ptr, stream);
+        return;
+    }
     refit_moment_batch(b.refits.ptr, b.segments.ptr, b.refit_levels, b.eval_prefix.ptr,
                        static_cast<std::uint32_t>(b.host_refits.size()), b.order, b.status.ptr, stream);
 }
@@ -214,13 +235,48 @@ std::uint32_t read_status(gwn_batch_handle &b, hipStream_t stream, bool clear) {
     return e;
 }
 
+// Smallest compiled traversal stack that holds `bound` entries.
+int stack_for(std::uint32_t bound) {
+    if (bound > 128)
+        throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
+    return bound <= 32 ? 32 : (bound <= 64 ? 64 : 128);
+}
+
+// Builds every tree of a batch over geometries once with the largest level count, then fixes the
+// level count of later passes (the deepest tree plus 8 levels of headroom for geometry updates)
+// and the traversal stack from the trees (synchronizes).
+void plan_build(gwn_batch_handle &b, hipStream_t stream) {
+    wait_last_work(b);
+    batch_trees_enqueue(*b.trees, k_batch_build_max_levels, 128u, b.status.ptr, stream);
+    batch_trees_summary const sum = batch_trees_read(*b.trees, stream);
+    std::uint32_t const e = read_status(b, stream, true);
+    if (e != 0u)
+        throw_internal(batch_status_message(e));
+    b.build_levels = std::min<int>(k_batch_build_max_levels, static_cast<int>(sum.max_depth) + 1 + 8);
+    b.stack_bound = sum.max_internal_stack;
+    b.cap = stack_for(sum.max_internal_stack);
+    batch_drop_graph(b);
+}
+
 } // namespace
+
+std::string batch_status_message(std::uint32_t bits) {
+    if (bits & 2u)
+        return "batch: A segmented BVH build failed.";
+    if (bits & 4u)
+        return "batch: A rebuilt tree is deeper than the batch's level launches; call gwn_batch_set_build to re-plan.";
+    if (bits & 8u)
+        return "batch: A rebuilt tree's traversal stack bound exceeds the batch's stack capacity.";
+    return "batch: Bottom-up moment refit did not converge.";
+}
 
 void batch_destroy(gwn_batch_handle *b) { delete b; }
 
 // Every mesh's moment refit, one launch per depth over all meshes' nodes (refit_moment_batch); the
 // records equal those of single-mesh refits.
 void batch_refit(gwn_batch_handle &b, hipStream_t stream) {
+    if (b.trees)
+        throw_invalid("batch: A batch over geometries refits with its builds: call gwn_batch_build.");
     check_storage(b, false);
     enqueue_refit(b, stream);
     note_work(b, stream);
@@ -271,9 +327,8 @@ gwn_batch_handle *batch_create(
         b->status.allocate(1, stream);
         fill_u32(b->status.ptr, 0u, 1, stream);
         upload_refit_table(*b, stream);
-        b->cap = bound <= 32 ? 32 : (bound <= 64 ? 64 : 128);
-        if (bound > 128)
-            throw_invalid("Traversal stack bound exceeds the largest compiled stack (128).");
+        b->stack_bound = bound;
+        b->cap = stack_for(bound);
         batch_refit(*b, stream);
         b->orders.resize(bvhs.size());
         for (std::size_t i = 0; i < bvhs.size(); ++i) {
@@ -305,13 +360,19 @@ gwn_batch_handle *batch_create(
     return b;
 }
 
-void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_cu, bool wave, hipStream_t stream) {
+void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_cu, bool wave, int stack_capacity,
+                      hipStream_t stream) {
     if (block <= 0 || block > 256 || (block & 63) != 0)
         throw_invalid("Batched query block size must be 64, 128, 192 or 256.");
     if (chunk <= 0 || chunk % block != 0)
         throw_invalid("Batched query chunk length must be a positive multiple of the block size.");
     if (blocks_per_cu <= 0 || blocks_per_cu > 64)
         throw_invalid("Batched query blocks per CU must be in [1, 64].");
+    // 0 keeps the current stack; otherwise a compiled one (32, 64, 128) that holds every tree's bound.
+    if (stack_capacity != 0 && stack_capacity != 32 && stack_capacity != 64 && stack_capacity != 128)
+        throw_invalid("Batched query stack capacity must be 32, 64 or 128.");
+    if (stack_capacity != 0 && static_cast<std::uint32_t>(stack_capacity) < b.stack_bound)
+        throw_invalid("Batched query stack capacity is below a tree's traversal stack bound.");
     // The graph and any query in flight read the table and the counter: finish them first.
     wait_last_work(b);
     bool const relayout = chunk != b.chunk;
@@ -319,13 +380,16 @@ void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_
     b.chunk = chunk;
     b.blocks_per_cu = blocks_per_cu;
     b.wave = wave;
+    if (stack_capacity != 0)
+        b.cap = stack_capacity;
     if (relayout)
         upload_table(b, stream);
     batch_drop_graph(b);
 }
 
 // Times each candidate (traversal, block size, chunk length, blocks per CU) with one launch
-// after a warm-up and keeps the fastest; returns its time in milliseconds.
+// after a warm-up and keeps the fastest, then each compiled traversal stack that holds the trees'
+// bound for that shape; returns the best time in milliseconds.
 float batch_tune(gwn_batch_handle &b, hipStream_t stream) {
     struct cand {
         int block, chunk, bpc;
@@ -346,7 +410,7 @@ float batch_tune(gwn_batch_handle &b, hipStream_t stream) {
     bool have = false;
     try {
         for (cand const &c : cands) {
-            batch_set_tuning(b, c.block, c.chunk, c.bpc, c.wave, stream);
+            batch_set_tuning(b, c.block, c.chunk, c.bpc, c.wave, 0, stream);
             batch_query(b, stream);
             GWN_HIP(hipEventRecord(e0, stream));
             batch_query(b, stream);
@@ -365,9 +429,34 @@ float batch_tune(gwn_batch_handle &b, hipStream_t stream) {
         (void)hipEventDestroy(e1);
         throw;
     }
+    batch_set_tuning(b, best.block, best.chunk, best.bpc, best.wave, 0, stream);
+    // Then the compiled traversal stack for that shape: every one that holds the trees' bound.
+    int best_cap = b.cap;
+    try {
+        for (int cap : {32, 64, 128}) {
+            if (static_cast<std::uint32_t>(cap) < b.stack_bound || cap == best_cap)
+                continue;
+            batch_set_tuning(b, best.block, best.chunk, best.bpc, best.wave, cap, stream);
+            batch_query(b, stream);
+            GWN_HIP(hipEventRecord(e0, stream));
+            batch_query(b, stream);
+            GWN_HIP(hipEventRecord(e1, stream));
+            GWN_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            GWN_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_cap = cap;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
     GWN_HIP(hipEventDestroy(e0));
     GWN_HIP(hipEventDestroy(e1));
-    batch_set_tuning(b, best.block, best.chunk, best.bpc, best.wave, stream);
+    batch_set_tuning(b, best.block, best.chunk, best.bpc, best.wave, best_cap, stream);
     return best_ms;
 }
 
@@ -415,6 +504,146 @@ void batch_replay(gwn_batch_handle &b, hipStream_t stream) {
     // The graph's refits form every moment from its BVH's current bounds, as an eager refit does.
     if (b.graph_refits)
         stamp_revisions(b);
+}
+
+// A batch over geometries: builds every tree once (plan_build fixes the level launches and the
+// traversal stack), then lays out the query table over the batch's own trees and moments.
+gwn_batch_handle *batch_create_built(
+    std::vector<gwn_geometry_handle const *> const &geometries, std::vector<batch_mesh> const &queries, int width,
+    std::uint32_t hploc_radius, int order, float accuracy_scale, hipStream_t stream
+) {
+    auto *b = new gwn_batch_handle();
+    try {
+        GWN_HIP(hipEventCreateWithFlags(&b->last_work, hipEventDisableTiming));
+        for (std::size_t i = 0; i < geometries.size(); ++i)
+            for (std::size_t j = 0; j < i; ++j)
+                if (geometries[j] == geometries[i])
+                    throw_invalid("batch: Each mesh needs its own geometry object.");
+        b->geometries = geometries;
+        for (gwn_geometry_handle const *g : geometries)
+            b->geometry_storage.push_back(g->storage_generation);
+        b->host_meshes = queries;
+        b->accuracy_scale = accuracy_scale;
+        b->order = order;
+        b->trees = batch_trees_create(geometries, width, hploc_radius, order, stream);
+        b->status.allocate(1, stream);
+        fill_u32(b->status.ptr, 0u, 1, stream);
+        plan_build(*b, stream);
+        batch_trees_summary const sum = batch_trees_read(*b->trees, stream);
+        b->orders.resize(geometries.size());
+        for (std::size_t i = 0; i < geometries.size(); ++i) {
+            batch_mesh &m = b->host_meshes[i];
+            batch_trees_view(*b->trees, i, &m.records, &m.tris, &m.root);
+            m.order = nullptr;
+            if (ref_internal(m.root) && m.count > 1) {
+                b->orders[i].allocate(m.count, stream);
+                query_order_box(sum.root_boxes[i], m.qx, m.qy, m.qz, b->orders[i].ptr, m.count, stream);
+                m.order = b->orders[i].ptr;
+            }
+            b->total_queries += m.count;
+        }
+        int dev = 0;
+        GWN_HIP(hipGetDevice(&dev));
+        GWN_HIP(hipDeviceGetAttribute(&b->cu_count, hipDeviceAttributeMultiprocessorCount, dev));
+        b->meshes.allocate(std::max<std::size_t>(1, b->host_meshes.size()), stream);
+        b->counter.allocate(1, stream);
+        upload_table(*b, stream);
+    } catch (...) {
+        delete b;
+        throw;
+    }
+    return b;
+}
+
+void batch_build(gwn_batch_handle &b, hipStream_t stream) {
+    if (!b.trees)
+        throw_invalid("batch: This batch holds BVH and moment objects: refit it with gwn_batch_refit_moments.");
+    check_storage(b, false);
+    enqueue_refit(b, stream);
+    note_work(b, stream);
+}
+
+void batch_set_build(gwn_batch_handle &b, int width, std::uint32_t hploc_radius, hipStream_t stream) {
+    if (!b.trees)
+        throw_invalid("batch: Build parameters belong to batches over geometries.");
+    check_storage(b, false);
+    wait_last_work(b);
+    batch_trees_set_build(*b.trees, width, hploc_radius);
+    plan_build(b, stream);
+}
+
+void batch_get_build(gwn_batch_handle const &b, int &width, std::uint32_t &hploc_radius, int &levels, int &order) {
+    if (!b.trees)
+        throw_invalid("batch: Build parameters belong to batches over geometries.");
+    width = batch_trees_width(*b.trees);
+    hploc_radius = batch_trees_radius(*b.trees);
+    levels = b.build_levels;
+    order = b.order;
+}
+
+// For every (width, radius) candidate: rebuild and re-plan, then time one build + refit + query
+// pass after a warm-up one; keeps the fastest.  The query launch shape stays as tuned.
+float batch_tune_build(gwn_batch_handle &b, hipStream_t stream) {
+    if (!b.trees)
+        throw_invalid("batch: Build parameters belong to batches over geometries.");
+    check_storage(b, false);
+    struct cand {
+        int width;
+        std::uint32_t radius;
+    };
+    std::vector<cand> cands;
+    for (int width : {4, 3, 2})
+        for (std::uint32_t radius : {8u, 4u, 2u, 1u})
+            cands.push_back({width, radius});
+    hipEvent_t e0, e1;
+    GWN_HIP(hipEventCreate(&e0));
+    GWN_HIP(hipEventCreate(&e1));
+    float best_ms = 0.0f;
+    cand best = {batch_trees_width(*b.trees), batch_trees_radius(*b.trees)};
+    bool have = false;
+    try {
+        for (cand const &c : cands) {
+            try {
+                batch_set_build(b, c.width, c.radius, stream);
+            } catch (error const &e) {
+                if (e.code != GWN_ERROR_INTERNAL && e.code != GWN_ERROR_INVALID_ARGUMENT)
+                    throw;
+                continue; // trees deeper than the level limit or the compiled stacks
+            }
+            batch_build(b, stream);
+            batch_query(b, stream);
+            GWN_HIP(hipEventRecord(e0, stream));
+            batch_build(b, stream);
+            batch_query(b, stream);
+            GWN_HIP(hipEventRecord(e1, stream));
+            GWN_HIP(hipEventSynchronize(e1));
+            if (read_status(b, stream, true) != 0u)
+                continue; // a candidate whose trees overflow the plan is not a candidate
+            float ms = 0.0f;
+            GWN_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (!have || ms < best_ms) {
+                best_ms = ms;
+                best = c;
+                have = true;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    GWN_HIP(hipEventDestroy(e0));
+    GWN_HIP(hipEventDestroy(e1));
+    batch_set_build(b, best.width, best.radius, stream);
+    return best_ms;
+}
+
+void batch_export(gwn_batch_handle &b, std::size_t i, gwn_bvh_handle &bvh, gwn_moment_handle &moment,
+                  hipStream_t stream) {
+    if (!b.trees)
+        throw_invalid("batch: Only a batch over geometries holds trees of its own.");
+    wait_last_work(b);
+    batch_trees_export(*b.trees, i, bvh, moment, stream);
 }
 
 } // namespace gwn

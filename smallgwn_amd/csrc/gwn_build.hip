@@ -5,130 +5,37 @@
 //   triangle AABB + 63-bit Morton code                       build_common.cuh:157-214
 //   stable LSD radix sort of (code, triangle id)             build_common.cuh:325-347
 //   Karras LBVH                                              build_binary.cuh:29-129
-//   bottom-up bounds + first leaf (arrival counters)         build_binary.cuh:131-192
+//   bounds + first leaf, one launch per binary depth         build_binary.cuh:131-192
+//     (or H-PLOC, which writes them beside each node)        build_hploc.cuh:104-622
 //   BVH4 collapse, one launch per level, emitting nodes      build_collapse.cuh:98-314
 //     directly in breadth-first (depth, first sorted leaf)   (replaces reorder.cuh:61-143)
 //     order
 //   leaf-ordered triangle records                            build_impl.cuh:23-41
 // The host reads the small level table back once per chunk of levels (about three times for a
 // million triangles) to find the last level and size the node array; the reference performs
-// one such read-back of its node count (collapse.cuh:425-446).
-#include "gwn_core.hpp"
-#include "gwn_lookback.hpp"
+// one such read-back of its node count (collapse.cuh:425-446).  The device pieces shared with the
+// segmented build of the batched mode are in gwn_build_device.hpp.
+#include "gwn_build_device.hpp"
 
 #include <algorithm>
 #include <vector>
 
-// The collapse compares half-areas and the Morton path compares quantized centroids; keep
-// every float expression unfused so the CPU oracle reproduces them bit for bit.
-#pragma clang fp contract(off)
-
 namespace gwn {
 namespace {
-
-constexpr int k_block = 256;
-constexpr std::uint32_t k_invalid = 0xffffffffu;
-constexpr std::uint32_t k_leaf_bit = 0x80000000u;
-
-__device__ inline float fmin_ref(float a, float b) { return (b < a) ? b : a; } // std::min(a, b)
-__device__ inline float fmax_ref(float a, float b) { return (a < b) ? b : a; } // std::max(a, b)
-
-__device__ inline std::uint32_t float_order_key(float f) {
-    std::uint32_t const u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ inline float float_from_order_key(std::uint32_t k) {
-    std::uint32_t const u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
-    return __uint_as_float(u);
-}
 
 // Per-axis min/max over all vertices; one atomic per block per extremum.
 __global__ __launch_bounds__(k_block) void scene_bounds_kernel(
     float const *__restrict__ x, float const *__restrict__ y, float const *__restrict__ z,
     std::size_t nv, std::uint32_t *__restrict__ ordered
 ) {
-    float lo[3] = {INFINITY, INFINITY, INFINITY};
-    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    std::size_t i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
-    std::size_t const stride = static_cast<std::size_t>(gridDim.x) * k_block;
-    for (; i < nv; i += stride) {
-        float const p[3] = {x[i], y[i], z[i]};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = fminf(lo[k], p[k]);
-            hi[k] = fmaxf(hi[k], p[k]);
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        for (int d = 32; d >= 1; d >>= 1) {
-            lo[k] = fminf(lo[k], __shfl_xor(lo[k], d));
-            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], d));
-        }
-    }
-    // One atomic per bound and block (the waves meet in LDS first): same-address atomics from
-    // every wave of every block had serialised.
-    __shared__ float s_b[k_block / 64][6];
-    int const wave = static_cast<int>(threadIdx.x >> 6);
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            s_b[wave][k] = lo[k];
-            s_b[wave][3 + k] = hi[k];
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int w = 1; w < k_block / 64; ++w) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = fminf(lo[k], s_b[w][k]);
-                hi[k] = fmaxf(hi[k], s_b[w][3 + k]);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            atomicMin(&ordered[k], float_order_key(lo[k]));
-            atomicMax(&ordered[3 + k], float_order_key(hi[k]));
-        }
-    }
+    scene_bounds_range(x, y, z, static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x, nv,
+                       static_cast<std::size_t>(gridDim.x) * k_block, ordered);
 }
 
-__global__ void scene_init_kernel(std::uint32_t *ordered) {
-    for (int k = 0; k < 3; ++k) {
-        ordered[k] = ~0u;
-        ordered[3 + k] = 0u;
-    }
-}
+__global__ void scene_init_kernel(std::uint32_t *ordered) { scene_init_words(ordered); }
 
-// scene[0..8] = min xyz, max xyz, inverse extent xyz (the reference's host safe_inv).
 __global__ void scene_finalize_kernel(std::uint32_t const *ordered, float *scene) {
-    for (int k = 0; k < 3; ++k) {
-        float const lo = float_from_order_key(ordered[k]);
-        float const hi = float_from_order_key(ordered[3 + k]);
-        scene[k] = lo;
-        scene[3 + k] = hi;
-        scene[6 + k] = (hi == lo) ? 1.0f : 1.0f / (hi - lo);
-    }
-}
-
-__device__ inline std::uint64_t expand_bits_u64(std::uint64_t v) {
-    std::uint64_t x = v & 0x1fffffu;
-    x = (x | (x << 32)) & 0x1f00000000ffffull;
-    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
-    x = (x | (x << 8)) & 0x100f00f00f00f00full;
-    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
-    x = (x | (x << 2)) & 0x1249249249249249ull;
-    return x;
-}
-
-__device__ inline float clamp01(float v) {
-    if (v < 0.0f)
-        return 0.0f;
-    if (v > 1.0f)
-        return 1.0f;
-    return v;
+    scene_finalize_words(ordered, scene);
 }
 
 __global__ __launch_bounds__(k_block) void morton_kernel(
@@ -141,31 +48,13 @@ __global__ __launch_bounds__(k_block) void morton_kernel(
     if (t >= nt)
         return;
     ids[t] = static_cast<std::uint32_t>(t);
-    std::uint32_t const a = i0[t], b = i1[t], c = i2[t];
-    if (a >= nv || b >= nv || c >= nv) {
-        boxes[t] = aabb{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        codes[t] = 0;
-        return;
-    }
-    float const ax = vx[a], ay = vy[a], az = vz[a];
-    float const bx = vx[b], by = vy[b], bz = vz[b];
-    float const cx = vx[c], cy = vy[c], cz = vz[c];
-    aabb const box{
-        fmin_ref(ax, fmin_ref(bx, cx)), fmin_ref(ay, fmin_ref(by, cy)), fmin_ref(az, fmin_ref(bz, cz)),
-        fmax_ref(ax, fmax_ref(bx, cx)), fmax_ref(ay, fmax_ref(by, cy)), fmax_ref(az, fmax_ref(bz, cz)),
-    };
+    aabb box;
+    std::uint64_t code;
+    morton_triangle(vx, vy, vz, nv, i0, i1, i2, t, scene, box, code);
     boxes[t] = box;
-    float const mx = (box.min_x + box.max_x) * 0.5f;
-    float const my = (box.min_y + box.max_y) * 0.5f;
-    float const mz = (box.min_z + box.max_z) * 0.5f;
-    float const nx = (mx - scene[0]) * scene[6];
-    float const ny = (my - scene[1]) * scene[7];
-    float const nz = (mz - scene[2]) * scene[8];
-    std::uint64_t const qx = static_cast<std::uint64_t>(clamp01(nx) * 2097151.0f);
-    std::uint64_t const qy = static_cast<std::uint64_t>(clamp01(ny) * 2097151.0f);
-    std::uint64_t const qz = static_cast<std::uint64_t>(clamp01(nz) * 2097151.0f);
-    codes[t] = (expand_bits_u64(qx) << 2) | (expand_bits_u64(qy) << 1) | expand_bits_u64(qz);
+    codes[t] = code;
 }
+
 
 __global__ __launch_bounds__(k_block) void gather_boxes_kernel(
     aabb const *__restrict__ boxes, std::uint32_t const *__restrict__ order, std::size_t nt,
@@ -240,11 +129,6 @@ __global__ __launch_bounds__(k_block) void lbvh_kernel(
         iparent[0] = k_invalid;
 }
 
-__device__ inline aabb aabb_union_ref(aabb const &a, aabb const &b) {
-    return aabb{fmin_ref(a.min_x, b.min_x), fmin_ref(a.min_y, b.min_y), fmin_ref(a.min_z, b.min_z),
-                fmax_ref(a.max_x, b.max_x), fmax_ref(a.max_y, b.max_y), fmax_ref(a.max_z, b.max_z)};
-}
-
 // Depth of every internal node of the Karras tree (edges to the root), by following the parent
 // links: the parent array is small enough to stay in L2, and the tree's height bounds each walk.
 // A walk longer than the node count means a broken parent chain: error.
@@ -295,252 +179,17 @@ __global__ __launch_bounds__(k_block) void lbvh_bounds_level_kernel(
     ifirst[node] = lf < rf ? lf : rf;
 }
 
-__device__ inline float half_area(aabb const &b) {
-    float dx = b.max_x - b.min_x;
-    float dy = b.max_y - b.min_y;
-    float dz = b.max_z - b.min_z;
-    dx = dx < 0.0f ? 0.0f : dx;
-    dy = dy < 0.0f ? 0.0f : dy;
-    dz = dz < 0.0f ? 0.0f : dz;
-    return dx * dy + dy * dz + dz * dx;
-}
-
-// H-PLOC binary build (reference: include/gwn/detail/gwn_bvh_build_hploc.cuh:104-622), wave64.
-//
-// Every sorted leaf is a lane that climbs the implicit LBVH of the (code, position) keys: at each
-// step it exchanges its range end into the boundary it meets (hploc.cuh:141-156, 451-470); the
-// first arrival stops, the second joins both ranges.  A joined range with more than 16 leaves, or
-// the full range, concatenates its children's cluster lists (at most 16 each, stored at the start
-// of each child range) and runs PLOC on them: nearest neighbour within the search radius by the
-// packed key (union half-area bits << 1, low four bits = (offset - 1) << 1 | direction parity,
-// hploc.cuh:195-241), mutual pairs merged into the lower lane, survivors compacted in lane order,
-// until at most 16 clusters remain (one for the full range); the list goes back to the start of
-// the range.  The 16-cluster lists are the reference's, so the merge sequence of every range is
-// the reference's; a 64-lane wave runs two ranges at once, one per 32-lane half.
-//
-// Node ids do not come from an atomic counter: merging clusters whose smallest sorted leaves are
-// a < b creates internal node b - 1.  Each leaf is the larger-side minimum of at most one merge,
-// so ids are unique, cover [0, T - 2], and do not depend on scheduling; bounds and first leaves
-// are written beside the node, so no bounds climb follows.  Release before the boundary exchange
-// and acquire after it publish a range's list and nodes to the lane (on any XCD) that joins it.
-constexpr std::uint32_t k_hploc_list = 16; // clusters kept per partial range (hploc.cuh:23)
-constexpr int k_hploc_block = 256;
-
-__device__ inline std::uint64_t hploc_delta(std::uint64_t const *codes, std::uint32_t a, std::uint32_t b) {
-    std::uint64_t const d = codes[a] ^ codes[b];
-    return d == 0 ? static_cast<std::uint64_t>(a ^ b) : d;
-}
-
-__device__ inline std::uint32_t hploc_area_key(aabb const &a, aabb const &b) {
-    float const area = half_area(aabb_union_ref(a, b));
-    return (__float_as_uint(area) << 1) & ~15u;
-}
-
-// Lane-local view of one half wave: bits of a 64-lane ballot that belong to this lane's half.
-__device__ inline std::uint32_t half_bits(std::uint64_t m) {
-    return static_cast<std::uint32_t>(m >> (threadIdx.x & 32u));
-}
-
-__device__ inline aabb shfl_box32(aabb const &b, std::uint32_t src) {
-    int const s = static_cast<int>(src);
-    return aabb{__shfl(b.min_x, s, 32), __shfl(b.min_y, s, 32), __shfl(b.min_z, s, 32),
-                __shfl(b.max_x, s, 32), __shfl(b.max_y, s, 32), __shfl(b.max_z, s, 32)};
-}
-
-// Position of the k-th (0-based) set bit of `mask`; requires k < popcount(mask).
-__device__ inline std::uint32_t select_bit(std::uint32_t mask, std::uint32_t k) {
-    std::uint32_t pos = 0;
-#pragma unroll
-    for (std::uint32_t w = 16; w > 0; w >>= 1) {
-        std::uint32_t const low = __popc(mask & ((1u << w) - 1u));
-        if (k >= low) {
-            k -= low;
-            mask >>= w;
-            pos += w;
-        }
-    }
-    return pos;
-}
-
-struct hploc_out {
-    uint2 *bnodes;
-    aabb *ibounds;
-    std::uint32_t *ifirst;
-    std::uint32_t *clusters;
-};
-
-// PLOC over one range [L, R1) split at S, run by the 32 lanes of this half (all active, uniform
-// control flow within the half).  Returns the merges made; sets `bad` on an inconsistent state.
-__device__ inline std::uint32_t hploc_range(
-    aabb const *__restrict__ leaf_boxes, std::uint32_t T, std::uint32_t radius, hploc_out const &o,
-    std::uint32_t L, std::uint32_t R1, std::uint32_t S, bool fin, std::uint32_t *root_word, bool &bad
-) {
-    std::uint32_t const h = threadIdx.x & 31u;
-    // Concatenate the two child lists (hploc.cuh:159-177, 424-430).
-    std::uint32_t idx = k_invalid;
-    std::uint32_t const lcl = (S - L) < k_hploc_list ? (S - L) : k_hploc_list;
-    if (h < lcl)
-        idx = o.clusters[L + h];
-    std::uint32_t const num_left = __popc(half_bits(__ballot(h < lcl && idx != k_invalid)));
-    std::uint32_t const lcr = (R1 - S) < k_hploc_list ? (R1 - S) : k_hploc_list;
-    std::uint32_t const ri = h - num_left;
-    bool const rv = ri < lcr;
-    if (rv)
-        idx = o.clusters[S + ri];
-    std::uint32_t const num_right = __popc(half_bits(__ballot(rv && idx != k_invalid)));
-    std::uint32_t const total = num_left + num_right;
-    std::uint32_t num = total;
-    std::uint32_t first = k_invalid;
-    aabb box{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (h < num) {
-        if (idx < T) {
-            first = idx;
-            box = leaf_boxes[idx];
-        } else if (idx - T < T - 1u) {
-            first = o.ifirst[idx - T];
-            box = o.ibounds[idx - T];
-        } else {
-            bad = true;
-        }
-    }
-    std::uint32_t const threshold = fin ? 1u : k_hploc_list;
-    std::uint32_t made = 0;
-    while (num > threshold) {
-        // Smallest packed key over the pairs (h, h + r) and (h - r, h), r <= radius.
-        std::uint32_t best = 0xffffffffu;
-#pragma unroll
-        for (std::uint32_t r = 1; r <= 8u; ++r) {
-            if (r > radius)
-                break;
-            bool const nv = h + r < num;
-            aabb const nb = shfl_box32(box, nv ? h + r : h);
-            std::uint32_t ak = 0xffffffffu;
-            if (nv) {
-                ak = hploc_area_key(box, nb);
-                std::uint32_t const key = ak | ((r - 1u) << 1) | (h & 1u);
-                best = key < best ? key : best;
-            }
-            std::uint32_t const from_left = __shfl(ak, static_cast<int>(h >= r ? h - r : h), 32);
-            if (h >= r && h < num) {
-                std::uint32_t const key = from_left | ((r - 1u) << 1) | ((h & 1u) ^ 1u);
-                best = key < best ? key : best;
-            }
-        }
-        std::uint32_t const off = ((best & 15u) >> 1) + 1u;
-        std::uint32_t const nn = (((best ^ h) & 1u) == 0u) ? h + off : h - off;
-        // Mutual nearest neighbours merge; the lower lane keeps the cluster (hploc.cuh:337-404).
-        bool const act = h < num;
-        bool const nvalid = nn < num && nn != h;
-        if (act && !nvalid)
-            bad = true;
-        std::uint32_t const safe = nvalid ? nn : h;
-        std::uint32_t const back = __shfl(safe, static_cast<int>(safe), 32);
-        bool const mutual = act && nvalid && back == h;
-        bool const merge = mutual && h < safe;
-        std::uint32_t const n_idx = __shfl(idx, static_cast<int>(safe), 32);
-        std::uint32_t const n_first = __shfl(first, static_cast<int>(safe), 32);
-        aabb const n_box = shfl_box32(box, safe);
-        if (merge) {
-            std::uint32_t const id = (first > n_first ? first : n_first) - 1u;
-            if (first == n_first || id >= T - 1u) {
-                bad = true;
-            } else {
-                aabb const merged = aabb_union_ref(box, n_box);
-                std::uint32_t const f = first < n_first ? first : n_first;
-                o.bnodes[id] = make_uint2(idx < T ? (idx | k_leaf_bit) : idx - T,
-                                          n_idx < T ? (n_idx | k_leaf_bit) : n_idx - T);
-                o.ibounds[id] = merged;
-                o.ifirst[id] = f;
-                idx = T + id;
-                first = f;
-                box = merged;
-            }
-        }
-        std::uint32_t const merges = __popc(half_bits(__ballot(merge)));
-        std::uint32_t const keep_mask = half_bits(__ballot(merge || !mutual));
-        std::uint32_t const kept = __popc(keep_mask);
-        std::uint32_t const src = h < kept ? select_bit(keep_mask, h) : h;
-        idx = __shfl(idx, static_cast<int>(src), 32);
-        first = __shfl(first, static_cast<int>(src), 32);
-        box = shfl_box32(box, src);
-        if (h >= kept)
-            idx = k_invalid;
-        num -= merges;
-        made += merges;
-        if (merges == 0) {
-            bad = true;
-            break;
-        }
-    }
-    if (h < total)
-        o.clusters[L + h] = idx;
-    if (fin && h == 0u && num == 1u)
-        *root_word = idx - T;
-    return made;
-}
-
+// H-PLOC over one tree, one lane per sorted leaf (hploc_climb).
 // words: [0] merges, [1] failure flag, [2] root internal id.
 __global__ __launch_bounds__(k_hploc_block) void hploc_kernel(
     std::uint64_t const *__restrict__ codes, aabb const *__restrict__ leaf_boxes, std::uint32_t T,
     std::uint32_t radius, hploc_out o, std::uint32_t *boundary, std::uint32_t *words
 ) {
-    std::uint32_t const lane = threadIdx.x & 63u;
     std::uint32_t const idx = blockIdx.x * k_hploc_block + threadIdx.x;
-    std::uint32_t left = idx, right = idx, split = 0;
-    bool active = idx < T;
-    bool bad = false;
-    std::uint32_t made = 0;
-    for (;;) {
-        release_agent();
-        if (active) {
-            bool const to_right =
-                left == 0u || (right != T - 1u && hploc_delta(codes, right, right + 1u) <
-                                                        hploc_delta(codes, left - 1u, left));
-            std::uint32_t const slot = to_right ? right : left - 1u;
-            std::uint32_t const prev = __hip_atomic_exchange(
-                &boundary[slot], to_right ? left : right, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (prev == k_invalid) {
-                active = false;
-            } else if (to_right) {
-                split = right + 1u;
-                right = prev;
-            } else {
-                split = left;
-                left = prev;
-            }
-        }
-        acquire_agent();
-        bool const fin = active && right - left + 1u == T;
-        std::uint64_t pending = __ballot(active && (right - left + 1u > k_hploc_list || fin));
-        while (pending != 0u) {
-            int const s0 = __builtin_ctzll(pending);
-            pending &= pending - 1u;
-            int s1 = -1;
-            if (pending != 0u) {
-                s1 = __builtin_ctzll(pending);
-                pending &= pending - 1u;
-            }
-            int const sel = (lane & 32u) ? s1 : s0;
-            int const src = sel >= 0 ? sel : s0;
-            std::uint32_t const rl = __shfl(left, src, 64);
-            std::uint32_t const rr = __shfl(right, src, 64);
-            std::uint32_t const rs = __shfl(split, src, 64);
-            int const rf = __shfl(fin ? 1 : 0, src, 64);
-            if (sel >= 0)
-                made += hploc_range(leaf_boxes, T, radius, o, rl, rr + 1u, rs, rf != 0, words + 2, bad);
-        }
-        if (__ballot(active) == 0u)
-            break;
-    }
-    // `made` is the same on the 32 lanes of a half: count it once per half.
-    std::uint32_t const total = wave_sum_u32((lane & 31u) == 0u ? made : 0u);
-    bool const any_bad = __ballot(bad) != 0u;
-    if (lane == 0u) {
-        atomicAdd(&words[0], total);
-        if (any_bad)
-            atomicOr(&words[1], 1u);
-    }
+    hploc_climb(codes, leaf_boxes, T, radius, o, boundary, build_segment{0u, T, 0u, 0u}, idx, idx < T, words + 2,
+                words);
 }
+
 
 __global__ __launch_bounds__(k_block) void hploc_init_kernel(
     std::uint32_t *clusters, std::uint32_t *boundary, std::uint32_t T
@@ -551,40 +200,6 @@ __global__ __launch_bounds__(k_block) void hploc_init_kernel(
         boundary[i] = k_invalid;
     }
 }
-
-// Level-synchronous BVH4 collapse.  Level d holds the wide nodes at depth d, numbered
-// contiguously from levels[d].start in (depth, first sorted leaf) order.  One launch per level:
-// every thread takes one wide node of the level, expands its binary subtree to at most four
-// candidates by always opening the internal candidate with the largest half-area (ties: smaller
-// binary index; collapse.cuh:170-211), and a look-back scan over the level's internal-child
-// counts numbers the next level's nodes.  Children of one node are numbered in first-leaf
-// order, which keeps the whole level in first-leaf order, so nodes come out in their final
-// breadth-first positions and no reordering pass follows.
-// Wide depth never exceeds binary depth: at most 128 levels for LBVH (one (code, index) prefix bit
-// per level); H-PLOC trees are deeper only for adversarial inputs, which fail the build with an
-// internal error past this limit.
-constexpr int k_max_levels = 1024;
-
-struct collapse_task {
-    std::uint32_t binary;
-    std::uint32_t prefix_internal; // internal-child stack demand accumulated above this node
-    std::uint32_t prefix_packed;   // same for all valid children
-    std::uint32_t pad;
-};
-
-struct collapse_level {
-    std::uint32_t start;
-    std::uint32_t count;
-    std::uint32_t tiles_claimed;
-    std::uint32_t pad;
-};
-
-struct collapse_stats {
-    std::uint32_t max_internal;
-    std::uint32_t max_packed;
-    std::uint32_t leaves;
-    std::uint32_t error;
-};
 
 // Seeds level 0 with the binary root: node 0 for LBVH; for H-PLOC (hploc_words non-null) the
 // recorded root, after checking that the build made T - 1 merges without failure.
@@ -608,173 +223,40 @@ __global__ void collapse_init_kernel(
     *root_box = ibounds[root];
 }
 
-__device__ inline std::uint32_t wave_max_u32(std::uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        std::uint32_t const w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-
+// One launch per level of the collapse (collapse_tile).
 __global__ __launch_bounds__(k_block) void collapse_level_kernel(
-    uint2 const *__restrict__ bnodes, aabb const *__restrict__ ibounds,
-    std::uint32_t const *__restrict__ ifirst, aabb const *__restrict__ leaf_boxes, int depth,
-    std::uint32_t capacity, collapse_level *levels, collapse_task *tasks, std::uint32_t *status_cur,
-    std::uint32_t *status_next, std::uint32_t status_words, collapse_stats *stats,
-    bvh_node4 *__restrict__ nodes, std::uint32_t *__restrict__ parent,
-    std::uint8_t *__restrict__ internal_children, int width
+    collapse_tree t, int depth, std::uint32_t *status_cur, std::uint32_t *status_next, std::uint32_t status_words,
+    int width
 ) {
     __shared__ std::uint32_t s_tile;
-    __shared__ std::uint32_t s_exclusive;
-    __shared__ std::uint32_t s_wave[k_block / 64];
-    int const lane = threadIdx.x & 63;
-    int const wave = threadIdx.x >> 6;
     // Only the first ceil(count / block) blocks take part (the launch is sized for the level's
     // bound, not its count): they claim exactly the tiles 0..tiles-1, each claim made by a running
-    // block, so the look-back below never waits on an unclaimed tile.  A level past the last has
-    // count 0 (the level table starts zeroed) and every block returns at once; no launch reads
-    // the next status array then.
-    std::uint64_t const level_tiles = (static_cast<std::uint64_t>(levels[depth].count) + k_block - 1) / k_block;
+    // block, so the look-back never waits on an unclaimed tile.  A level past the last has count 0
+    // (the level table starts zeroed) and every block returns at once; no launch reads the next
+    // status array then.
+    std::uint64_t const level_tiles = (static_cast<std::uint64_t>(t.levels[depth].count) + k_block - 1) / k_block;
     if (blockIdx.x >= level_tiles)
         return;
     std::uint32_t const live = level_tiles < gridDim.x ? static_cast<std::uint32_t>(level_tiles) : gridDim.x;
     if (threadIdx.x == 0)
-        s_tile = atomicAdd(&levels[depth].tiles_claimed, 1u);
+        s_tile = atomicAdd(&t.levels[depth].tiles_claimed, 1u);
     // The next level scans over the other status array; the live blocks clear it for that launch.
     for (std::uint32_t i = blockIdx.x * k_block + threadIdx.x; i < status_words; i += live * k_block)
         status_next[i] = 0u;
     __syncthreads();
-    std::uint32_t const start = levels[depth].start;
-    std::uint32_t const count = levels[depth].count;
+    std::uint32_t const start = t.levels[depth].start;
+    std::uint32_t const count = t.levels[depth].count;
     std::uint32_t const tile = s_tile;
     // A level never exceeds the launch (width^depth bound) or the capacity; report, do not index.
-    if (static_cast<std::uint64_t>(start) + count > capacity ||
+    if (static_cast<std::uint64_t>(start) + count > t.capacity ||
         count > static_cast<std::uint64_t>(gridDim.x) * k_block) {
         if (threadIdx.x == 0)
-            atomicOr(&stats->error, 1u);
+            atomicOr(&t.stats->error, 1u);
         return;
     }
     if (tile >= (count + k_block - 1) / k_block)
         return;
-    std::uint32_t const i = tile * k_block + threadIdx.x;
-    bool const active = i < count;
-    std::uint32_t const self = start + i;
-
-    std::uint32_t cand[4] = {k_invalid, k_invalid, k_invalid, k_invalid};
-    int n_cand = 0;
-    collapse_task task{};
-    if (active) {
-        task = tasks[self];
-        uint2 const root = bnodes[task.binary];
-        cand[0] = root.x;
-        cand[1] = root.y;
-        n_cand = 2;
-        // Expand up to the tree's width (2, 3 or 4 children; slots past it stay empty).
-        while (n_cand < width) {
-            int best = -1;
-            float best_area = -INFINITY;
-            std::uint32_t best_index = k_invalid;
-            for (int s = 0; s < n_cand; ++s) {
-                if (cand[s] & k_leaf_bit)
-                    continue;
-                float const area = half_area(ibounds[cand[s]]);
-                if (best >= 0 && area < best_area)
-                    continue;
-                if (best >= 0 && area == best_area && cand[s] >= best_index)
-                    continue;
-                best = s;
-                best_area = area;
-                best_index = cand[s];
-            }
-            if (best < 0)
-                break;
-            uint2 const ex = bnodes[cand[best]];
-            cand[best] = ex.x;
-            cand[n_cand] = ex.y;
-            ++n_cand;
-        }
-    }
-    std::uint32_t first[4];
-    std::uint32_t n_internal = 0, n_leaf = 0;
-    for (int s = 0; s < 4; ++s) {
-        first[s] = k_invalid;
-        if (s >= n_cand)
-            continue;
-        if (cand[s] & k_leaf_bit) {
-            ++n_leaf;
-        } else {
-            ++n_internal;
-            first[s] = ifirst[cand[s]];
-        }
-    }
-
-    // Block-wide exclusive scan of internal-child counts, then the cross-tile look-back.
-    std::uint32_t incl = n_internal;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        std::uint32_t const v = __shfl_up(incl, o, 64);
-        if (lane >= o)
-            incl += v;
-    }
-    if (lane == 63)
-        s_wave[wave] = incl;
-    __syncthreads();
-    std::uint32_t wave_prefix = 0, block_total = 0;
-    for (int w = 0; w < k_block / 64; ++w) {
-        if (w < wave)
-            wave_prefix += s_wave[w];
-        block_total += s_wave[w];
-    }
-    if (wave == 0) {
-        std::uint32_t const ex = wave_lookback(status_cur, tile, block_total, &stats->error);
-        if (lane == 0)
-            s_exclusive = ex;
-    }
-    __syncthreads();
-    std::uint32_t const next_start = start + count;
-    if (threadIdx.x == 0 && tile == (count - 1) / k_block) {
-        levels[depth + 1].start = next_start;
-        levels[depth + 1].count = s_exclusive + block_total;
-    }
-    if (active) {
-        std::uint32_t const base = next_start + s_exclusive + wave_prefix + incl - n_internal;
-        std::uint32_t const node_internal = task.prefix_internal + (n_internal ? n_internal - 1u : 0u);
-        std::uint32_t const node_packed = task.prefix_packed + static_cast<std::uint32_t>(n_cand - 1);
-        bvh_node4 out{};
-        for (int s = 0; s < n_cand; ++s) {
-            std::uint32_t const c = cand[s] & ~k_leaf_bit;
-            if (cand[s] & k_leaf_bit) {
-                out.children[s].bounds = leaf_boxes[c];
-                out.children[s].reference = encode_leaf(c, 1u);
-                continue;
-            }
-            std::uint32_t rank = 0;
-            for (int t = 0; t < 4; ++t)
-                rank += first[t] < first[s] ? 1u : 0u;
-            std::uint32_t const id = base + rank;
-            if (id >= capacity) {
-                atomicOr(&stats->error, 1u);
-                continue;
-            }
-            out.children[s].bounds = ibounds[c];
-            out.children[s].reference = encode_internal(id);
-            tasks[id] = collapse_task{c, node_internal, node_packed, 0u};
-            parent[id] = (self << 2) | static_cast<std::uint32_t>(s);
-        }
-        nodes[self] = out;
-        internal_children[self] = static_cast<std::uint8_t>(n_internal);
-        task.prefix_internal = node_internal;
-        task.prefix_packed = node_packed;
-    }
-    std::uint32_t const w_int = wave_max_u32(active ? task.prefix_internal : 0u);
-    std::uint32_t const w_pack = wave_max_u32(active ? task.prefix_packed : 0u);
-    std::uint32_t const w_leaf = wave_sum_u32(n_leaf);
-    if (lane == 0) {
-        atomicMax(&stats->max_internal, w_int);
-        atomicMax(&stats->max_packed, w_pack);
-        atomicAdd(&stats->leaves, w_leaf);
-    }
+    collapse_tile(t, depth, start, count, tile, status_cur, width);
 }
 
 __global__ __launch_bounds__(k_block) void records_kernel(
@@ -786,13 +268,8 @@ __global__ __launch_bounds__(k_block) void records_kernel(
     std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
     if (i >= nt)
         return;
-    std::uint32_t const p = order[i];
-    std::uint32_t const a = i0[p], b = i1[p], c = i2[p];
-    if (a >= nv || b >= nv || c >= nv) {
+    if (!triangle_record(vx, vy, vz, nv, i0, i1, i2, order[i], records + i))
         atomicOr(error, 1u);
-        return;
-    }
-    records[i] = tri_record{vx[a], vy[a], vz[a], vx[b], vy[b], vz[b], vx[c], vy[c], vz[c]};
 }
 
 
@@ -928,6 +405,8 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, int method, s
                                               static_cast<std::uint32_t>(NI), ibounds.ptr, root_box_dev.ptr);
     GWN_LAUNCH_CHECK();
 
+    collapse_tree const tree{bnodes.ptr, ibounds.ptr, ifirst.ptr, leaf_boxes.ptr, levels.ptr, tasks.ptr, stats.ptr,
+                             wide.ptr, wparent.ptr, wchildren.ptr, static_cast<std::uint32_t>(capacity)};
     // Launch levels in chunks and read the level table back after each chunk; a launch for a
     // level past the last one finds an empty level and returns.
     std::vector<collapse_level> host_levels(k_max_levels);
@@ -945,11 +424,7 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, int method, s
             unsigned const grid = grid_for(bound, k_block);
             std::uint32_t *cur = status.ptr + static_cast<std::size_t>(depth & 1) * status_words;
             std::uint32_t *nxt = status.ptr + static_cast<std::size_t>((depth + 1) & 1) * status_words;
-            collapse_level_kernel<<<grid, k_block, 0, stream>>>(
-                bnodes.ptr, ibounds.ptr, ifirst.ptr, leaf_boxes.ptr, depth,
-                static_cast<std::uint32_t>(capacity), levels.ptr, tasks.ptr, cur,
-                nxt, status_words, stats.ptr, wide.ptr, wparent.ptr, wchildren.ptr, width
-            );
+            collapse_level_kernel<<<grid, k_block, 0, stream>>>(tree, depth, cur, nxt, status_words, width);
             GWN_LAUNCH_CHECK();
         }
         GWN_HIP(hipMemcpyAsync(host_levels.data(), levels.ptr, static_cast<std::size_t>(depth + 1) * sizeof(collapse_level),

@@ -226,6 +226,7 @@ static void upload_common(
     geometry->index_bytes = index_bytes;
     geometry->stream = stream;
     geometry->generation = staging.generation;
+    geometry->storage_generation = gwn::next_storage_generation();
 }
 
 int gwn_upload_geometry_f32(
@@ -594,6 +595,7 @@ int gwn_geometry_clear(gwn_geometry_t geometry) {
             b->release();
         g.vertex_count = g.triangle_count = 0;
         g.generation += 1;
+        g.storage_generation = gwn::next_storage_generation();
     });
 }
 
@@ -1397,7 +1399,7 @@ int gwn_batch_set_tuning(gwn_batch_t batch, const gwn_batch_tuning *tuning, hipS
         if (tuning->traversal != 0 && tuning->traversal != 1)
             gwn::throw_invalid("Batched query traversal must be 0 (per lane) or 1 (wave).");
         gwn::batch_set_tuning(*batch, tuning->block_size, tuning->chunk, tuning->blocks_per_cu,
-                              tuning->traversal == 1, stream);
+                              tuning->traversal == 1, tuning->stack_capacity, stream);
     });
 }
 
@@ -1439,8 +1441,98 @@ int gwn_batch_status(gwn_batch_t batch, hipStream_t stream) {
     return guarded("gwn_batch_status", [&] {
         if (batch == nullptr)
             gwn::throw_invalid("Batch handle is null.");
-        if (gwn::batch_status(*batch, stream) != 0u)
-            gwn::throw_internal("batch: Bottom-up moment refit did not converge.");
+        std::uint32_t const bits = gwn::batch_status(*batch, stream);
+        if (bits != 0u)
+            gwn::throw_internal(gwn::batch_status_message(bits));
+    });
+}
+
+int gwn_batch_create_built(const gwn_batch_build_item *items, uint64_t item_count, const gwn_batch_build_config *config,
+                           float accuracy_scale, gwn_batch_t *out, hipStream_t stream) {
+    return guarded("gwn_batch_create_built", [&] {
+        if (out == nullptr)
+            gwn::throw_invalid("Output handle pointer is null.");
+        if (config == nullptr)
+            gwn::throw_invalid("Batch build configuration pointer is null.");
+        if (item_count != 0 && items == nullptr)
+            gwn::throw_invalid("Batch item array is null.");
+        if (item_count > 0xffffffffull)
+            gwn::throw_invalid("Batch item count exceeds the mesh index range.");
+        if (!(accuracy_scale >= 0.0f))
+            gwn::throw_invalid("Taylor accuracy scale must be nonnegative.");
+        if (config->method != GWN_BVH_BUILD_HPLOC)
+            gwn::throw_invalid("batch: Segmented builds use the H-PLOC builder.");
+        std::vector<gwn_geometry_handle const *> geometries;
+        std::vector<gwn::batch_mesh> qs;
+        for (uint64_t i = 0; i < item_count; ++i) {
+            gwn_batch_build_item const &it = items[i];
+            if (it.geometry == nullptr || (it.geometry->triangle_count != 0 && it.geometry->i0.ptr == nullptr))
+                gwn::throw_invalid("bvh.build: Geometry accessor is invalid for BVH build.");
+            require_geometry_real(it.geometry, 4);
+            require_query_storage(it.count, it.qx, it.qy, it.qz, it.out);
+            geometries.push_back(it.geometry);
+            gwn::batch_mesh m{};
+            m.qx = it.qx;
+            m.qy = it.qy;
+            m.qz = it.qz;
+            m.out = it.out;
+            m.count = it.count;
+            qs.push_back(m);
+        }
+        *out = gwn::batch_create_built(geometries, qs, config->width, config->hploc_search_radius, config->order,
+                                       accuracy_scale, stream);
+    });
+}
+
+int gwn_batch_build(gwn_batch_t batch, hipStream_t stream) {
+    return guarded("gwn_batch_build", [&] {
+        if (batch == nullptr)
+            gwn::throw_invalid("Batch handle is null.");
+        gwn::batch_build(*batch, stream);
+    });
+}
+
+int gwn_batch_set_build(gwn_batch_t batch, const gwn_batch_build_config *config, hipStream_t stream) {
+    return guarded("gwn_batch_set_build", [&] {
+        if (batch == nullptr || config == nullptr)
+            gwn::throw_invalid("Batch handle or build configuration pointer is null.");
+        if (config->method != GWN_BVH_BUILD_HPLOC)
+            gwn::throw_invalid("batch: Segmented builds use the H-PLOC builder.");
+        gwn::batch_set_build(*batch, config->width, config->hploc_search_radius, stream);
+    });
+}
+
+int gwn_batch_get_build(gwn_batch_t batch, gwn_batch_build_config *config, int32_t *level_launches) {
+    return guarded("gwn_batch_get_build", [&] {
+        if (batch == nullptr || config == nullptr)
+            gwn::throw_invalid("Batch handle or build configuration pointer is null.");
+        int width = 0, levels = 0, order = 0;
+        std::uint32_t radius = 0;
+        gwn::batch_get_build(*batch, width, radius, levels, order);
+        config->method = GWN_BVH_BUILD_HPLOC;
+        config->order = order;
+        config->width = width;
+        config->hploc_search_radius = radius;
+        if (level_launches)
+            *level_launches = levels;
+    });
+}
+
+int gwn_batch_tune_build(gwn_batch_t batch, hipStream_t stream, float *best_ms) {
+    return guarded("gwn_batch_tune_build", [&] {
+        if (batch == nullptr)
+            gwn::throw_invalid("Batch handle is null.");
+        float const ms = gwn::batch_tune_build(*batch, stream);
+        if (best_ms)
+            *best_ms = ms;
+    });
+}
+
+int gwn_batch_export(gwn_batch_t batch, uint64_t mesh, gwn_bvh_t bvh, gwn_moment_t moment, hipStream_t stream) {
+    return guarded("gwn_batch_export", [&] {
+        if (batch == nullptr || bvh == nullptr || moment == nullptr)
+            gwn::throw_invalid("Batch, BVH or moment handle is null.");
+        gwn::batch_export(*batch, mesh, *bvh, *moment, stream);
     });
 }
 

@@ -10,6 +10,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <vector>
 #include <stdexcept>
 #include <string>
@@ -169,6 +170,8 @@ struct gwn_geometry_handle {
     std::size_t triangle_count = 0;
     hipStream_t stream = nullptr;
     std::uint64_t generation = 0;  // bumps on every upload/update
+    // New on every upload and clear (which replace the buffers; an in-place update keeps it).
+    std::uint64_t storage_generation = 0;
 };
 
 struct gwn_bvh_handle {
@@ -294,8 +297,13 @@ void radix_sort_pairs_u32(
 // Device words of scratch a sort of n pairs on [begin_bit, end_bit) uses (digit bases and the
 // per-tile count table).
 std::size_t radix_sort_workspace_words(std::size_t n, int begin_bit, int end_bit);
-// The same sort with its scratch in `workspace` (radix_sort_workspace_words words, caller-owned):
-// allocates nothing.
+// The same sorts with their scratch in `workspace` (radix_sort_workspace_words words,
+// caller-owned): they allocate nothing.
+void radix_sort_pairs_u64_ws(
+    std::uint64_t *keys, std::uint64_t *keys_alt, std::uint32_t *vals, std::uint32_t *vals_alt,
+    std::size_t n, int begin_bit, int end_bit, std::uint32_t *workspace, std::uint64_t **keys_out,
+    std::uint32_t **vals_out, hipStream_t stream
+);
 void radix_sort_pairs_u32_ws(
     std::uint32_t *keys, std::uint32_t *keys_alt, std::uint32_t *vals, std::uint32_t *vals_alt,
     std::size_t n, int begin_bit, int end_bit, std::uint32_t *workspace, std::uint32_t **keys_out,
@@ -337,6 +345,43 @@ void taylor_batch_launch(
     float accuracy_scale, std::uint64_t *counter, int order, int cap, int block, int grid, bool wave,
     hipStream_t stream
 );
+// Segmented builds (gwn_batch_build.hip): the H-PLOC BVH trees and Taylor moments of many meshes,
+// built side by side in one pass of a fixed sequence of launches that allocates nothing and never
+// reads back to the host, so the whole pass can be captured into a graph.  Each mesh's tree,
+// records and moments equal those of gwn_build_bvh + gwn_refit_bvh_moment on that mesh alone.
+struct batch_trees;
+struct batch_trees_deleter {
+    void operator()(batch_trees *t) const noexcept;
+};
+using batch_trees_ptr = std::unique_ptr<batch_trees, batch_trees_deleter>;
+// Largest number of wide levels one pass can collapse (the level tables hold one more entry).
+constexpr int k_batch_build_max_levels = 128;
+batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> const &geometries, int width,
+                                   std::uint32_t hploc_radius, int order, hipStream_t stream);
+void batch_trees_set_build(batch_trees &t, int width, std::uint32_t hploc_radius);
+// Enqueues every mesh's build and moment refit, collapsing `levels` wide levels at most.  Failures
+// OR bits into the device word *status: 1 moment refit, 2 build, 4 a tree deeper than `levels`,
+// 8 a tree whose traversal stack bound exceeds `stack_capacity`.
+void batch_trees_enqueue(batch_trees &t, int levels, std::uint32_t stack_capacity, std::uint32_t *status,
+                         hipStream_t stream);
+struct batch_trees_summary {
+    std::uint32_t max_depth = 0;          // deepest wide level index over all trees
+    std::uint32_t max_internal_stack = 0; // largest internal_stack_bound
+    std::vector<aabb> root_boxes;
+};
+// Synchronizes `stream` and reads the last pass's per-tree results.
+batch_trees_summary batch_trees_read(batch_trees &t, hipStream_t stream);
+// Mesh i's evaluation records, triangle records and root reference (fixed for the pass's life).
+void batch_trees_view(batch_trees const &t, std::size_t i, float4 const **eval_records, tri_record const **tris,
+                      std::uint64_t *root);
+// Copies mesh i's tree and moments into single-mesh objects (allocates them; synchronizes).
+void batch_trees_export(batch_trees &t, std::size_t i, gwn_bvh_handle &bvh, gwn_moment_handle &moment,
+                        hipStream_t stream);
+int batch_trees_width(batch_trees const &t);
+std::uint32_t batch_trees_radius(batch_trees const &t);
+// Bytes of device memory the pass holds.
+std::size_t batch_trees_bytes(batch_trees const &t);
+
 // Batched mode (gwn_batch.hip).
 gwn_batch_handle *batch_create(
     std::vector<gwn_bvh_handle const *> const &bvhs, std::vector<gwn_moment_handle *> const &moments,
@@ -345,13 +390,35 @@ gwn_batch_handle *batch_create(
 void batch_destroy(gwn_batch_handle *b);
 void batch_refit(gwn_batch_handle &b, hipStream_t stream);
 void batch_query(gwn_batch_handle &b, hipStream_t stream);
-void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_cu, bool wave, hipStream_t stream);
+// stack_capacity: 0 keeps the current compiled traversal stack, else 32, 64 or 128 (at least the
+// trees' bound).
+void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_cu, bool wave, int stack_capacity,
+                      hipStream_t stream);
 float batch_tune(gwn_batch_handle &b, hipStream_t stream);
 void batch_get_tuning(gwn_batch_handle const &b, gwn_batch_tuning &t);
 void batch_capture(gwn_batch_handle &b, bool include_refit);
 void batch_replay(gwn_batch_handle &b, hipStream_t stream);
 // Synchronizes `stream`, returns the batch's refit error word and clears it.
 std::uint32_t batch_status(gwn_batch_handle &b, hipStream_t stream);
+// The message for a nonzero status word (the most severe bit).
+std::string batch_status_message(std::uint32_t bits);
+// A batch over geometries: the batch builds every mesh's tree (H-PLOC, `hploc_radius`, `width`)
+// and Taylor moments of `order` itself (segmented builds) and owns them; queries as batch_create.
+gwn_batch_handle *batch_create_built(
+    std::vector<gwn_geometry_handle const *> const &geometries, std::vector<batch_mesh> const &queries, int width,
+    std::uint32_t hploc_radius, int order, float accuracy_scale, hipStream_t stream
+);
+// Enqueues every mesh's build and moment refit (a batch over geometries).
+void batch_build(gwn_batch_handle &b, hipStream_t stream);
+// New build parameters for a batch over geometries: rebuilds now and re-plans (synchronizes).
+void batch_set_build(gwn_batch_handle &b, int width, std::uint32_t hploc_radius, hipStream_t stream);
+void batch_get_build(gwn_batch_handle const &b, int &width, std::uint32_t &hploc_radius, int &levels, int &order);
+// Times builds + refits + queries over candidate widths and search radii and keeps the fastest
+// (a batch over geometries; synchronizes; drops a captured graph).  Returns its milliseconds.
+float batch_tune_build(gwn_batch_handle &b, hipStream_t stream);
+// Copies mesh i's tree and moments into BVH / moment objects (a batch over geometries).
+void batch_export(gwn_batch_handle &b, std::size_t i, gwn_bvh_handle &bvh, gwn_moment_handle &moment,
+                  hipStream_t stream);
 // Moment refit storage that a batch allocates once per mesh, so that its refits enqueue without
 // allocating or synchronising: the per-node additive payloads of the level passes.
 struct moment_refit_work {
@@ -390,6 +457,14 @@ struct refit_level {
 void refit_moment_batch(moment_refit_mesh const *meshes, refit_segment const *segments,
                         std::vector<refit_level> const &levels, std::uint64_t const *slot_prefix,
                         std::uint32_t mesh_count, int order, std::uint32_t *error, hipStream_t stream);
+// The same with the per-depth levels (level_count of them), their segments and the slot prefix in
+// device memory, written earlier in the stream (the segmented build): every launch has a fixed
+// grid of `grid` blocks that strides over what the table holds, so the refit can be recorded into
+// a graph before the trees it refits exist.
+void refit_moment_batch_device(moment_refit_mesh const *meshes, refit_segment const *segments,
+                               refit_level const *levels, int level_count, std::uint64_t const *slot_prefix,
+                               std::uint32_t mesh_count, int order, std::uint32_t *error, unsigned grid,
+                               hipStream_t stream);
 // In-place bounds refit: triangle records and child boxes from the geometry, root box updated.
 void refit_bounds(gwn_geometry_handle const &geometry, gwn_bvh_handle &bvh, hipStream_t stream);
 
@@ -476,6 +551,11 @@ void winding_gradient_antipodal(
 void query_order(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz,
     std::uint32_t *order, std::size_t n, hipStream_t stream
+);
+// The same within a given box (the root bounds of a tree held elsewhere).
+void query_order_box(
+    aabb const &box, float const *qx, float const *qy, float const *qz, std::uint32_t *order, std::size_t n,
+    hipStream_t stream
 );
 // Taylor winding batch (no gradient) with the Hilbert presort's scratch in the library's static
 // per-device scratch array instead of stream-ordered allocations; at most

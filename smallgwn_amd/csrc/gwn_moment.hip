@@ -178,12 +178,13 @@ __device__ inline std::uint32_t segment_of(refit_segment const *seg, std::uint32
     return lo;
 }
 
+// Thread t of a depth's quads (node t / 4, slot t % 4) over that depth's `total` nodes of every
+// mesh; the whole block calls it.
 template <int Order>
-__global__ __launch_bounds__(k_block) void moment_level_batch_kernel(
+__device__ inline void moment_level_batch_item(
     moment_refit_mesh const *__restrict__ meshes, refit_segment const *__restrict__ seg, std::uint32_t seg_count,
-    std::uint64_t total, std::uint32_t *error
+    std::uint64_t total, std::uint64_t t, std::uint32_t *error, payload<Order> *share, std::uint32_t *share_have
 ) {
-    std::uint64_t const t = static_cast<std::uint64_t>(blockIdx.x) * k_block + threadIdx.x;
     std::uint64_t const g = t >> 2;  // node of this quad within the depth's nodes of every mesh
     bool active = g < total;
     moment_refit_mesh m{};
@@ -197,10 +198,38 @@ __global__ __launch_bounds__(k_block) void moment_level_batch_kernel(
             active = false;
         }
     }
-    __shared__ payload<Order> share[k_block];
-    __shared__ std::uint32_t share_have[k_block];
     moment_refit_slot<Order>(n, static_cast<int>(t & 3), active, m.nodes, m.node_count, m.tris, m.tri_count,
                              static_cast<payload<Order> *>(m.pending), m.records, m.slot_area, error, share, share_have);
+}
+
+template <int Order>
+__global__ __launch_bounds__(k_block) void moment_level_batch_kernel(
+    moment_refit_mesh const *__restrict__ meshes, refit_segment const *__restrict__ seg, std::uint32_t seg_count,
+    std::uint64_t total, std::uint32_t *error
+) {
+    __shared__ payload<Order> share[k_block];
+    __shared__ std::uint32_t share_have[k_block];
+    moment_level_batch_item<Order>(meshes, seg, seg_count, total,
+                                   static_cast<std::uint64_t>(blockIdx.x) * k_block + threadIdx.x, error, share,
+                                   share_have);
+}
+
+// The same with the depth's segments and node count in device memory (written earlier in the
+// stream by the segmented build), so the launch can be recorded before the tree exists: a fixed
+// grid strides over the depth's quads.  The stride loop's bound is uniform over the block.
+template <int Order>
+__global__ __launch_bounds__(k_block) void moment_level_batch_grid_kernel(
+    moment_refit_mesh const *__restrict__ meshes, refit_segment const *__restrict__ segments,
+    refit_level const *__restrict__ level, std::uint32_t *error
+) {
+    __shared__ payload<Order> share[k_block];
+    __shared__ std::uint32_t share_have[k_block];
+    refit_level const lv = *level;
+    std::uint64_t const quads = 4 * lv.nodes;
+    for (std::uint64_t base = static_cast<std::uint64_t>(blockIdx.x) * k_block; base < quads;
+         base += static_cast<std::uint64_t>(gridDim.x) * k_block)
+        moment_level_batch_item<Order>(meshes, segments + lv.first_segment, lv.segments, lv.nodes,
+                                       base + threadIdx.x, error, share, share_have);
 }
 
 // Evaluation records from finished traversal records: one thread per (node, child slot) over
@@ -258,6 +287,28 @@ __global__ __launch_bounds__(256) void moment_eval_batch_kernel(moment_refit_mes
     moment_refit_mesh const m = meshes[lo];
     std::uint64_t const k = g - slot_prefix[lo];
     eval_slot<Order>(m.records, m.eval_records, static_cast<std::size_t>(k >> 2), static_cast<int>(k & 3));
+}
+
+// The same over a slot prefix in device memory (written by the segmented build), by a fixed grid.
+template <int Order>
+__global__ __launch_bounds__(256) void moment_eval_batch_grid_kernel(moment_refit_mesh const *__restrict__ meshes,
+                                                                     std::uint64_t const *__restrict__ slot_prefix,
+                                                                     std::uint32_t mesh_count) {
+    std::uint64_t const total = slot_prefix[mesh_count];
+    for (std::uint64_t g = static_cast<std::uint64_t>(blockIdx.x) * 256 + threadIdx.x; g < total;
+         g += static_cast<std::uint64_t>(gridDim.x) * 256) {
+        std::uint32_t lo = 0, hi = mesh_count;
+        while (hi - lo > 1) {
+            std::uint32_t const mid = lo + (hi - lo) / 2;
+            if (slot_prefix[mid] <= g)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        moment_refit_mesh const m = meshes[lo];
+        std::uint64_t const k = g - slot_prefix[lo];
+        eval_slot<Order>(m.records, m.eval_records, static_cast<std::size_t>(k >> 2), static_cast<int>(k & 3));
+    }
 }
 
 void launch_eval(gwn_moment_handle &m, std::size_t node_count, hipStream_t stream) {
@@ -354,6 +405,30 @@ void refit_moment_batch(moment_refit_mesh const *meshes, refit_segment const *se
     case 0: moment_eval_batch_kernel<0><<<grid, 256, 0, stream>>>(meshes, slot_prefix, mesh_count); break;
     case 1: moment_eval_batch_kernel<1><<<grid, 256, 0, stream>>>(meshes, slot_prefix, mesh_count); break;
     default: moment_eval_batch_kernel<2><<<grid, 256, 0, stream>>>(meshes, slot_prefix, mesh_count); break;
+    }
+    GWN_LAUNCH_CHECK();
+}
+
+void refit_moment_batch_device(moment_refit_mesh const *meshes, refit_segment const *segments,
+                               refit_level const *levels, int level_count, std::uint64_t const *slot_prefix,
+                               std::uint32_t mesh_count, int order, std::uint32_t *error, unsigned grid,
+                               hipStream_t stream) {
+    if (order < 0 || order > 2)
+        throw_invalid("Taylor order must be 0, 1, or 2.");
+    if (mesh_count == 0 || grid == 0)
+        return;
+    for (int d = level_count; d-- > 0;) {
+        switch (order) {
+        case 0: moment_level_batch_grid_kernel<0><<<grid, k_block, 0, stream>>>(meshes, segments, levels + d, error); break;
+        case 1: moment_level_batch_grid_kernel<1><<<grid, k_block, 0, stream>>>(meshes, segments, levels + d, error); break;
+        default: moment_level_batch_grid_kernel<2><<<grid, k_block, 0, stream>>>(meshes, segments, levels + d, error); break;
+        }
+        GWN_LAUNCH_CHECK();
+    }
+    switch (order) {
+    case 0: moment_eval_batch_grid_kernel<0><<<grid, 256, 0, stream>>>(meshes, slot_prefix, mesh_count); break;
+    case 1: moment_eval_batch_grid_kernel<1><<<grid, 256, 0, stream>>>(meshes, slot_prefix, mesh_count); break;
+    default: moment_eval_batch_grid_kernel<2><<<grid, 256, 0, stream>>>(meshes, slot_prefix, mesh_count); break;
     }
     GWN_LAUNCH_CHECK();
 }

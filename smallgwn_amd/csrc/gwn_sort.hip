@@ -344,6 +344,17 @@ void radix_sort_pairs_u64(
                                     stream);
 }
 
+void radix_sort_pairs_u64_ws(
+    std::uint64_t *keys, std::uint64_t *keys_alt, std::uint32_t *vals, std::uint32_t *vals_alt,
+    std::size_t n, int begin_bit, int end_bit, std::uint32_t *workspace, std::uint64_t **keys_out,
+    std::uint32_t **vals_out, hipStream_t stream
+) {
+    if (workspace == nullptr)
+        throw_internal("radix sort: null workspace.");
+    radix_sort_pairs<std::uint64_t>(keys, keys_alt, vals, vals_alt, n, begin_bit, end_bit, workspace, keys_out,
+                                    vals_out, stream);
+}
+
 void radix_sort_pairs_u32(
     std::uint32_t *keys, std::uint32_t *keys_alt, std::uint32_t *vals, std::uint32_t *vals_alt,
     std::size_t n, int begin_bit, int end_bit, std::uint32_t **keys_out, std::uint32_t **vals_out,

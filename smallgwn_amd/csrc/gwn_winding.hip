@@ -1015,13 +1015,13 @@ std::size_t query_order_workspace_words(std::size_t n) {
 // Hilbert presort with caller-given scratch (query_order_workspace_words(n) words): allocates
 // nothing.
 void query_order_ws(
-    gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz,
+    aabb const &box, float const *qx, float const *qy, float const *qz,
     std::uint32_t *order, std::size_t n, std::uint32_t *ws, hipStream_t stream
 ) {
     std::uint32_t *const keys = ws;
     std::uint32_t *const keys_alt = ws + n;
     std::uint32_t *const ids_alt = ws + 2 * n;
-    query_key_kernel<<<grid_for(n, 256), 256, 0, stream>>>(qx, qy, qz, n, bvh.root.bounds, keys, order);
+    query_key_kernel<<<grid_for(n, 256), 256, 0, stream>>>(qx, qy, qz, n, box, keys, order);
     GWN_LAUNCH_CHECK();
     std::uint32_t *sk = nullptr;
     std::uint32_t *sorted = nullptr;
@@ -1036,10 +1036,17 @@ void query_order(
     gwn_bvh_handle const &bvh, float const *qx, float const *qy, float const *qz,
     std::uint32_t *order, std::size_t n, hipStream_t stream
 ) {
+    query_order_box(bvh.root.bounds, qx, qy, qz, order, n, stream);
+}
+
+void query_order_box(
+    aabb const &box, float const *qx, float const *qy, float const *qz, std::uint32_t *order, std::size_t n,
+    hipStream_t stream
+) {
     if (n == 0)
         return;
     device_buffer<std::uint32_t> ws(query_order_workspace_words(n), stream);
-    query_order_ws(bvh, qx, qy, qz, order, n, ws.ptr, stream);
+    query_order_ws(box, qx, qy, qz, order, n, ws.ptr, stream);
 }
 
 void winding_taylor(
@@ -1166,7 +1173,7 @@ void winding_taylor_noalloc(
         throw_internal("noalloc Taylor batch: presort scratch too small.");
     std::uint32_t *const base = device_scratch();
     std::uint32_t *const order = base;
-    query_order_ws(bvh, qx, qy, qz, order, n, base + k_noalloc_order_words, stream);
+    query_order_ws(bvh.root.bounds, qx, qy, qz, order, n, base + k_noalloc_order_words, stream);
     winding_taylor(bvh, m, qx, qy, qz, out, nullptr, nullptr, nullptr, n, accuracy_scale, precision, cfg, order,
                    nullptr, stream);
 }

@@ -48,7 +48,8 @@ EXPORTED_SYMBOLS = (
     "gwn_compute_unsigned_distance_batch_ex_f32", "gwn_compute_unsigned_distance_adjoint_batch_ex_f32",
     "gwn_batch_create", "gwn_batch_destroy", "gwn_batch_refit_moments", "gwn_batch_query",
     "gwn_batch_set_tuning", "gwn_batch_get_tuning", "gwn_batch_tune", "gwn_batch_capture", "gwn_batch_replay",
-    "gwn_batch_status", "gwn_compute_unsigned_distance_work_f32", "gwn_compute_ray_first_hit_work_f32",
+    "gwn_batch_status", "gwn_batch_create_built", "gwn_batch_build", "gwn_batch_set_build", "gwn_batch_get_build",
+    "gwn_batch_tune_build", "gwn_batch_export", "gwn_compute_unsigned_distance_work_f32", "gwn_compute_ray_first_hit_work_f32",
     "gwn_compute_query_order_f32", "gwn_radix_sort_pairs_u64", "gwn_radix_sort_pairs_u32",
     "gwn_upload_geometry_f32_u64", "gwn_upload_geometry_f64", "gwn_upload_geometry_f64_u64",
     "gwn_update_geometry_f64", "gwn_update_geometry_device_f64", "gwn_build_bvh_f64", "gwn_refit_bvh_f64",
@@ -204,6 +205,12 @@ def _load():
         "gwn_batch_capture": [vp, i32],
         "gwn_batch_replay": [vp, vp],
         "gwn_batch_status": [vp, vp],
+        "gwn_batch_create_built": [vp, u64, vp, f32, vp, vp],
+        "gwn_batch_build": [vp, vp],
+        "gwn_batch_set_build": [vp, vp, vp],
+        "gwn_batch_get_build": [vp, vp, vp],
+        "gwn_batch_tune_build": [vp, vp, vp],
+        "gwn_batch_export": [vp, u64, vp, vp, vp],
         "gwn_compute_unsigned_distance_work_f32": [vp, vp, vp, vp, u64, f32, vp, vp],
         "gwn_compute_ray_first_hit_work_f32": [vp, vp, vp, vp, vp, vp, vp, u64, f32, f32, vp, vp, vp],
         "gwn_compute_query_order_f32": [vp, vp, vp, vp, vp, u64, vp],
@@ -935,9 +942,10 @@ class BatchObject:
         return {k: int(getattr(t, k)) for k, _ in _BatchTuning._fields_}
 
     def set_tuning(self, block_size: int, chunk: int, blocks_per_cu: int, traversal: int = 0,
-                   stream=None) -> Status:
-        """traversal 0: one query per lane; 1: wave-cooperative over each mesh's Hilbert order."""
-        t = _BatchTuning(block_size, chunk, blocks_per_cu, traversal, 0, 0, 0)
+                   stack_capacity: int = 0, stream=None) -> Status:
+        """traversal 0: one query per lane; 1: wave-cooperative over each mesh's Hilbert order.
+        stack_capacity: 0 keeps the compiled traversal stack, else 32, 64 or 128."""
+        t = _BatchTuning(block_size, chunk, blocks_per_cu, traversal, stack_capacity, 0, 0)
         return _status(_lib.gwn_batch_set_tuning(self._h, ctypes.byref(t), _stream(stream)))
 
     def capture(self, include_refit: bool = True) -> Status:
@@ -960,6 +968,65 @@ class BatchObject:
             self.close()
         except Exception:
             pass
+
+
+class _BatchBuildItem(ctypes.Structure):
+    _fields_ = [("geometry", ctypes.c_void_p), ("qx", ctypes.c_void_p), ("qy", ctypes.c_void_p),
+                ("qz", ctypes.c_void_p), ("out", ctypes.c_void_p), ("count", ctypes.c_uint64)]
+
+
+class _BatchBuildConfig(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int32), ("width", ctypes.c_int32), ("hploc_search_radius", ctypes.c_uint32),
+                ("order", ctypes.c_int32)]
+
+
+class BuiltBatchObject(BatchObject):
+    """Batched mode over geometries (no reference counterpart): the batch builds and owns every
+    mesh's H-PLOC tree and Taylor moments in one segmented pass, which a captured graph replays
+    together with the queries.  ``items`` are tuples (geometry, qx, qy, qz, out); each mesh's tree,
+    moments and results equal a single-mesh build, refit and Taylor batch of that mesh."""
+
+    def __init__(self, items, width: int = 4, hploc_radius: int = 8, order: int = 1, accuracy_scale: float = 2.0,
+                 stream=None):
+        self._keep = list(items)
+        arr = (_BatchBuildItem * max(1, len(self._keep)))()
+        for k, (geo, qx, qy, qz, out) in enumerate(self._keep):
+            n = _numel(qx)
+            if _numel(qy) != n or _numel(qz) != n or _numel(out) != n:
+                raise ValueError("Query SoA spans require identical lengths.")
+            arr[k] = _BatchBuildItem(geo.handle.value, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), n)
+        cfg = _BatchBuildConfig(int(BuildMethod.HPLOC), int(width), int(hploc_radius), int(order))
+        h = ctypes.c_void_p()
+        _status(_lib.gwn_batch_create_built(arr, len(self._keep), ctypes.byref(cfg), float(accuracy_scale),
+                                            ctypes.byref(h), _stream(stream))).raise_if_error()
+        self._h = h
+
+    def build(self, stream=None) -> Status:
+        """Enqueues every mesh's build and moment refit."""
+        return _status(_lib.gwn_batch_build(self._h, _stream(stream)))
+
+    def set_build(self, width: int, hploc_radius: int, stream=None) -> Status:
+        cfg = _BatchBuildConfig(int(BuildMethod.HPLOC), int(width), int(hploc_radius), 0)
+        return _status(_lib.gwn_batch_set_build(self._h, ctypes.byref(cfg), _stream(stream)))
+
+    def build_config(self) -> dict:
+        cfg = _BatchBuildConfig()
+        levels = ctypes.c_int32()
+        _status(_lib.gwn_batch_get_build(self._h, ctypes.byref(cfg), ctypes.byref(levels))).raise_if_error()
+        return {"width": cfg.width, "hploc_search_radius": cfg.hploc_search_radius, "order": cfg.order,
+                "level_launches": levels.value}
+
+    def tune_build(self, stream=None) -> float:
+        ms = ctypes.c_float()
+        _status(_lib.gwn_batch_tune_build(self._h, _stream(stream), ctypes.byref(ms))).raise_if_error()
+        return float(ms.value)
+
+    def export(self, mesh: int, stream=None):
+        """(BvhObject, MomentObject) holding copies of mesh ``mesh``'s tree and moments."""
+        bvh = BvhObject()
+        mom = MomentObject(self.build_config()["order"])
+        _status(_lib.gwn_batch_export(self._h, int(mesh), bvh.handle, mom.handle, _stream(stream))).raise_if_error()
+        return bvh, mom
 
 
 def gwn_compute_query_order(bvh: BvhObject, qx, qy, qz, order, stream=None) -> Status:

@@ -41,7 +41,7 @@ def test_python_mirror_lists_every_export():
 
 
 def test_abi_version(lib):
-    assert lib.gwn_abi_version() == 2
+    assert lib.gwn_abi_version() == 3
 
 
 def _err(lib):
