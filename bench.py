@@ -86,6 +86,8 @@ def parse_args():
                    help="skip the small configuration, the large-mesh extras and the precision-split report")
     p.add_argument("--builder", choices=("hploc", "lbvh"), default="hploc", help="BVH builder of the large mesh")
     p.add_argument("--batched-meshes", type=int, default=4096)
+    p.add_argument("--batched-tune-build", action="store_true",
+                   help="also time the build tuner (widths x H-PLOC radii) of the batch over geometries")
     p.add_argument("--precision-queries", type=int, default=4_194_304)
     return p.parse_args()
 
@@ -742,16 +744,85 @@ def batched_bench(args, dev, stream) -> dict:
                    for a, it in zip(ref, items))
     log(f"batched: sequential {seq_ms:.1f} ms, graph {graph_ms:.1f} ms ({seq_ms / graph_ms:.2f}x), "
         f"identical={identical}, max |diff| {max_diff:.2e}, tuning {batch.tuning()}")
+    tuning = batch.tuning()
+    batch.close()
+    with_builds = batched_builds_leg(args, dev, geoms, items, reps)
     return {"meshes": len(items), "triangles": tris, "queries": nq, "taylor_order": 1, "builder": "hploc_r8",
             "sequential_ms": seq_ms, "graph_ms": graph_ms, "speedup": seq_ms / graph_ms,
             "mqueries_per_s": nq / (graph_ms * 1e-3) / 1e6, "best_query_launch_ms": tune_ms,
-            "tuning": batch.tuning(), "bit_identical_to_sequential": bool(identical),
+            "tuning": tuning, "bit_identical_to_sequential": bool(identical),
             "max_abs_diff_vs_sequential": max_diff, "refit_status_ok": st.is_ok(),
             "work": {"child_tests": int(c_tot[0]), "accepted": int(c_tot[1]), "near_field_triangles": int(c_tot[2])},
             "roofline": _valu_roofline(flops, tune_ms, "taylor_batch_kernel<1> (persistent, tuned)",
                                        "per-query work counters of every mesh (10 / 34 / 87 FLOP), "
                                        "over the tuned query launch"),
-            "step": "moment refit of every mesh + Taylor queries of every mesh"}
+            "step": "moment refit of every mesh + Taylor queries of every mesh",
+            "with_builds": with_builds}
+
+
+def batched_builds_leg(args, dev, geoms, items, reps) -> dict:
+    """The batched step with every tree rebuilt: a batch over the same geometries owns the trees
+    and its graph replays all builds, moment refits and queries, against sequential gwn_build_bvh +
+    gwn_refit_bvh_moment + Taylor batch calls with the same builder (H-PLOC r8, then LBVH; width 4)."""
+    import torch
+
+    from smallgwn_amd import gwn
+
+    t0 = time.time()
+    outs = [torch.empty_like(it[5]) for it in items]
+    built = gwn.BuiltBatchObject([(g, it[2], it[3], it[4], o) for g, it, o in zip(geoms, items, outs)],
+                                 width=4, hploc_radius=8, order=1, accuracy_scale=2.0)
+    create_s = time.time() - t0
+    query_ms = built.tune()
+    result = {"create_s": create_s, "best_query_launch_ms": query_ms,
+              "step": "build + moment refit of every mesh + Taylor queries of every mesh (one graph)"}
+    if args.batched_tune_build:
+        best = built.tune_build()
+        cfg = built.build_config()
+        result["tuned_build"] = {"best_ms": best, "method": cfg["method"].name.lower(), "width": cfg["width"],
+                                 "hploc_search_radius": cfg["hploc_search_radius"]}
+    for name, method in (("hploc_r8", gwn.BuildMethod.HPLOC), ("lbvh", gwn.BuildMethod.LBVH)):
+        built.set_build(4, 8, method=method).raise_if_error()
+        built.capture(include_refit=True).raise_if_error()
+        built.replay().raise_if_error()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            built.replay().raise_if_error()
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - t) / reps * 1e3
+        ok = built.status().is_ok()
+        built.build().raise_if_error()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        built.build().raise_if_error()
+        torch.cuda.synchronize()
+        pass_ms = (time.perf_counter() - t) * 1e3
+        opts = gwn.BuildOptions(method=method, hploc_search_radius=8, width=4)
+
+        def sequential():
+            for g, (bvh, mom, x, y, z, out) in zip(geoms, items):
+                gwn.gwn_build_bvh(g, bvh, opts).raise_if_error()
+                gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
+                gwn.gwn_compute_winding_number_taylor_batch(bvh, mom, x, y, z, out,
+                                                            accuracy_scale=2.0).raise_if_error()
+
+        sequential()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sequential()
+        torch.cuda.synchronize()
+        seq_ms = (time.perf_counter() - t) * 1e3
+        identical = all(torch.equal(o, it[5]) for o, it in zip(outs, items))
+        levels = built.build_config()["level_launches"]
+        log(f"batched with {name} builds: sequential {seq_ms:.1f} ms, graph {graph_ms:.1f} ms "
+            f"({seq_ms / graph_ms:.2f}x), builds + refits {pass_ms:.1f} ms, identical={identical}, "
+            f"{levels} wide level launches (create {create_s:.1f}s)")
+        result[name] = {"sequential_ms": seq_ms, "graph_ms": graph_ms, "speedup": seq_ms / graph_ms,
+                        "builds_and_refits_ms": pass_ms, "level_launches": levels, "width": 4,
+                        "bit_identical_to_sequential": bool(identical), "status_ok": ok}
+    built.close()
+    return result
 
 
 def _profile_lines(name: str) -> list[str]:

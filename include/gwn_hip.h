@@ -511,20 +511,21 @@ int gwn_batch_replay(gwn_batch_t batch, hipStream_t stream);
 int gwn_batch_status(gwn_batch_t batch, hipStream_t stream);
 
 /* ---- batched mode over geometries (no reference counterpart).  The batch builds and owns every
- *      mesh's BVH (H-PLOC, the given width and search radius) and Taylor moments of the given
- *      order: all meshes' builds and moment refits run in one segmented pass of a fixed sequence
- *      of launches (scene bounds, Morton codes, one stable sort by (mesh, code), H-PLOC over all
- *      sorted leaves, one plan + one persistent launch per wide level of the collapse, the level
- *      refit of the moments) that allocates nothing and never reads back, so it can be captured.
+ *      mesh's BVH (LBVH, or H-PLOC with the given search radius; the given width) and Taylor
+ *      moments of the given order: all meshes' builds and moment refits run in one segmented pass
+ *      of a fixed sequence of launches (scene bounds, Morton codes, one stable sort by (mesh,
+ *      code), H-PLOC over all sorted leaves or Karras LBVH + one bounds launch per binary level,
+ *      one plan + one persistent launch per wide level of the collapse, the level refit of the
+ *      moments) that allocates nothing and never reads back, so it can be captured.
  *      Each mesh's tree, records and moments equal gwn_build_bvh_f32 + gwn_refit_bvh_moment_f32
  *      of that mesh alone, and its query results equal the single-mesh Taylor batch's, bit for bit.
- *      gwn_batch_create_built builds once (synchronizes) and plans the pass: the number of wide
- *      level launches (the deepest tree + 8) and the traversal stack.  gwn_batch_build enqueues the
+ *      gwn_batch_create_built builds once (synchronizes) and plans the pass: the number of level
+ *      launches (the deepest tree + 8) and the traversal stack (the largest bound + 8).  gwn_batch_build enqueues the
  *      pass; gwn_batch_capture(batch, 1) records pass + query launch, so a replay rebuilds every tree
  *      from the geometries' current positions (gwn_update_geometry_f32 between replays).  A tree
  *      past the plan sets the status word (gwn_batch_status); gwn_batch_set_build re-plans.
  *      gwn_batch_tune picks the query launch shape and stack; gwn_batch_tune_build times the pass
- *      plus the queries over widths {4, 3, 2} x radii {8, 4, 2, 1} and keeps the fastest
+ *      plus the queries over widths {4, 3, 2} x {LBVH, H-PLOC radii 8, 4, 2, 1} and keeps the fastest
  *      (synchronizes).  gwn_batch_export copies one mesh's tree and moments into BVH and moment
  *      objects for the single-mesh entry points.  gwn_batch_refit_moments does not apply to such
  *      a batch.  The geometry objects (Real = float) must outlive the batch and keep their storage
@@ -539,9 +540,9 @@ typedef struct gwn_batch_build_item {
 } gwn_batch_build_item;
 
 typedef struct gwn_batch_build_config {
-    int32_t method;                /* GWN_BVH_BUILD_HPLOC */
+    int32_t method;                /* GWN_BVH_BUILD_HPLOC or GWN_BVH_BUILD_LBVH */
     int32_t width;                 /* 2, 3 or 4 */
-    uint32_t hploc_search_radius;  /* 1..8 */
+    uint32_t hploc_search_radius;  /* 1..8 (H-PLOC) */
     int32_t order;                 /* Taylor order 0, 1 or 2 (fixed at create) */
 } gwn_batch_build_config;
 

@@ -42,7 +42,10 @@ struct gwn_batch_handle {
     std::vector<std::uint64_t> geometry_storage;
     gwn::batch_trees_ptr trees;
     int build_levels = 0;          // wide levels each segmented pass collapses
-    std::uint32_t stack_bound = 0; // largest internal_stack_bound of the trees
+    int binary_levels = 0;         // binary levels each LBVH pass bounds
+    // Least traversal stack the queries may use: the trees' largest internal_stack_bound (a batch
+    // over geometries adds 8 of headroom for rebuilt trees).
+    std::uint32_t stack_bound = 0;
     std::vector<gwn::batch_mesh> host_meshes;
     std::vector<gwn::moment_refit_work> work;
     std::vector<gwn::moment_refit_mesh> host_refits;  // meshes with an internal root
@@ -156,7 +159,8 @@ void stamp_revisions(gwn_batch_handle &b) {
 
 void enqueue_refit(gwn_batch_handle &b, hipStream_t stream) {
     if (b.trees) {
-        batch_trees_enqueue(*b.trees, b.build_levels, static_cast<std::uint32_t>(b.cap), b.status.ptr, stream);
+        batch_trees_enqueue(*b.trees, b.build_levels, b.binary_levels, static_cast<std::uint32_t>(b.cap), b.status.ptr,
+                            stream);
         return;
     }
     refit_moment_batch(b.refits.ptr, b.segments.ptr, b.refit_levels, b.eval_prefix.ptr,
@@ -242,19 +246,20 @@ int stack_for(std::uint32_t bound) {
     return bound <= 32 ? 32 : (bound <= 64 ? 64 : 128);
 }
 
-// Builds every tree of a batch over geometries once with the largest level count, then fixes the
-// level count of later passes (the deepest tree plus 8 levels of headroom for geometry updates)
-// and the traversal stack from the trees (synchronizes).
+// Builds every tree of a batch over geometries once with the largest level counts, then fixes the
+// level counts of later passes and the traversal stack from the trees, each with 8 entries of
+// headroom for trees rebuilt from updated geometry (synchronizes).
 void plan_build(gwn_batch_handle &b, hipStream_t stream) {
     wait_last_work(b);
-    batch_trees_enqueue(*b.trees, k_batch_build_max_levels, 128u, b.status.ptr, stream);
+    batch_trees_enqueue(*b.trees, k_batch_build_max_levels, k_batch_build_max_levels, 128u, b.status.ptr, stream);
     batch_trees_summary const sum = batch_trees_read(*b.trees, stream);
     std::uint32_t const e = read_status(b, stream, true);
     if (e != 0u)
         throw_internal(batch_status_message(e));
     b.build_levels = std::min<int>(k_batch_build_max_levels, static_cast<int>(sum.max_depth) + 1 + 8);
-    b.stack_bound = sum.max_internal_stack;
-    b.cap = stack_for(sum.max_internal_stack);
+    b.binary_levels = std::min<int>(k_batch_build_max_levels, static_cast<int>(sum.max_binary_depth) + 1 + 8);
+    b.stack_bound = std::min<std::uint32_t>(128u, sum.max_internal_stack + 8u);
+    b.cap = stack_for(b.stack_bound);
     batch_drop_graph(b);
 }
 
@@ -509,8 +514,8 @@ void batch_replay(gwn_batch_handle &b, hipStream_t stream) {
 // A batch over geometries: builds every tree once (plan_build fixes the level launches and the
 // traversal stack), then lays out the query table over the batch's own trees and moments.
 gwn_batch_handle *batch_create_built(
-    std::vector<gwn_geometry_handle const *> const &geometries, std::vector<batch_mesh> const &queries, int width,
-    std::uint32_t hploc_radius, int order, float accuracy_scale, hipStream_t stream
+    std::vector<gwn_geometry_handle const *> const &geometries, std::vector<batch_mesh> const &queries, int method,
+    int width, std::uint32_t hploc_radius, int order, float accuracy_scale, hipStream_t stream
 ) {
     auto *b = new gwn_batch_handle();
     try {
@@ -525,7 +530,7 @@ gwn_batch_handle *batch_create_built(
         b->host_meshes = queries;
         b->accuracy_scale = accuracy_scale;
         b->order = order;
-        b->trees = batch_trees_create(geometries, width, hploc_radius, order, stream);
+        b->trees = batch_trees_create(geometries, method, width, hploc_radius, order, stream);
         b->status.allocate(1, stream);
         fill_u32(b->status.ptr, 0u, 1, stream);
         plan_build(*b, stream);
@@ -563,48 +568,54 @@ void batch_build(gwn_batch_handle &b, hipStream_t stream) {
     note_work(b, stream);
 }
 
-void batch_set_build(gwn_batch_handle &b, int width, std::uint32_t hploc_radius, hipStream_t stream) {
+void batch_set_build(gwn_batch_handle &b, int method, int width, std::uint32_t hploc_radius, hipStream_t stream) {
     if (!b.trees)
         throw_invalid("batch: Build parameters belong to batches over geometries.");
     check_storage(b, false);
     wait_last_work(b);
-    batch_trees_set_build(*b.trees, width, hploc_radius);
+    batch_trees_set_build(*b.trees, method, width, hploc_radius);
     plan_build(b, stream);
 }
 
-void batch_get_build(gwn_batch_handle const &b, int &width, std::uint32_t &hploc_radius, int &levels, int &order) {
+void batch_get_build(gwn_batch_handle const &b, int &method, int &width, std::uint32_t &hploc_radius, int &levels,
+                     int &order) {
     if (!b.trees)
         throw_invalid("batch: Build parameters belong to batches over geometries.");
+    method = batch_trees_method(*b.trees);
     width = batch_trees_width(*b.trees);
     hploc_radius = batch_trees_radius(*b.trees);
     levels = b.build_levels;
     order = b.order;
 }
 
-// For every (width, radius) candidate: rebuild and re-plan, then time one build + refit + query
-// pass after a warm-up one; keeps the fastest.  The query launch shape stays as tuned.
+// For every (builder, width) candidate -- LBVH, or H-PLOC with each search radius -- rebuild and
+// re-plan, then time one build + refit + query pass after a warm-up one; keeps the fastest.  The
+// query launch shape stays as tuned.
 float batch_tune_build(gwn_batch_handle &b, hipStream_t stream) {
     if (!b.trees)
         throw_invalid("batch: Build parameters belong to batches over geometries.");
     check_storage(b, false);
     struct cand {
+        int method;
         int width;
         std::uint32_t radius;
     };
     std::vector<cand> cands;
-    for (int width : {4, 3, 2})
+    for (int width : {4, 3, 2}) {
+        cands.push_back({GWN_BVH_BUILD_LBVH, width, 8u});
         for (std::uint32_t radius : {8u, 4u, 2u, 1u})
-            cands.push_back({width, radius});
+            cands.push_back({GWN_BVH_BUILD_HPLOC, width, radius});
+    }
     hipEvent_t e0, e1;
     GWN_HIP(hipEventCreate(&e0));
     GWN_HIP(hipEventCreate(&e1));
     float best_ms = 0.0f;
-    cand best = {batch_trees_width(*b.trees), batch_trees_radius(*b.trees)};
+    cand best = {batch_trees_method(*b.trees), batch_trees_width(*b.trees), batch_trees_radius(*b.trees)};
     bool have = false;
     try {
         for (cand const &c : cands) {
             try {
-                batch_set_build(b, c.width, c.radius, stream);
+                batch_set_build(b, c.method, c.width, c.radius, stream);
             } catch (error const &e) {
                 if (e.code != GWN_ERROR_INTERNAL && e.code != GWN_ERROR_INVALID_ARGUMENT)
                     throw;
@@ -634,7 +645,7 @@ float batch_tune_build(gwn_batch_handle &b, hipStream_t stream) {
     }
     GWN_HIP(hipEventDestroy(e0));
     GWN_HIP(hipEventDestroy(e1));
-    batch_set_build(b, best.width, best.radius, stream);
+    batch_set_build(b, best.method, best.width, best.radius, stream);
     return best_ms;
 }
 

@@ -178,26 +178,102 @@ __global__ __launch_bounds__(k_block) void seg_hploc_init_kernel(
     }
 }
 
-// One lane per sorted leaf of every mesh; meshes of one triangle take no part.
+__device__ inline build_segment seg_of(seg_mesh const *__restrict__ meshes,
+                                      std::uint32_t const *__restrict__ leaf_prefix, std::uint32_t mesh_count,
+                                      std::uint32_t leaf) {
+    std::uint32_t const m = prefix_slot(leaf_prefix, mesh_count, leaf);
+    seg_mesh const mm = meshes[m];
+    return build_segment{mm.leaf_base, mm.leaves, mm.internal_base, m};
+}
+
+// H-PLOC chunk launch: one lane per sorted leaf of every mesh (meshes of one triangle take no
+// part); a chunk may hold several meshes, whose ends stay hard range ends.  words: [0] merges,
+// [1] failure flag, [4] parked ranges.
 __global__ __launch_bounds__(k_hploc_block) void seg_hploc_kernel(
     seg_mesh const *__restrict__ meshes, std::uint32_t const *__restrict__ leaf_prefix, std::uint32_t mesh_count,
     std::uint64_t const *__restrict__ codes, aabb const *__restrict__ leaf_boxes, std::uint32_t total,
-    std::uint32_t radius, hploc_out o, std::uint32_t *boundary, std::uint32_t *root_words, std::uint32_t *words
+    std::uint32_t radius, hploc_out o, std::uint32_t *boundary, uint2 *parked, std::uint32_t *root_words,
+    std::uint32_t *words
 ) {
     std::uint32_t const idx = blockIdx.x * k_hploc_block + threadIdx.x;
     build_segment seg{0u, 0u, 0u, 0u};
     bool active = idx < total;
     if (active) {
-        std::uint32_t const m = prefix_slot(leaf_prefix, mesh_count, idx);
-        seg_mesh const mm = meshes[m];
-        seg = build_segment{mm.leaf_base, mm.leaves, mm.internal_base, m};
-        active = mm.leaves >= 2u;
+        seg = seg_of(meshes, leaf_prefix, mesh_count, idx);
+        active = seg.leaves >= 2u;
     }
-    hploc_climb(codes, leaf_boxes, total, radius, o, boundary, seg, idx, active, root_words, words);
+    hploc_chunk_climb(codes, leaf_boxes, total, radius, o, boundary, seg, idx, active, parked, words + 4, root_words,
+                      words);
 }
 
-// Level 0 of every mesh's collapse: the H-PLOC root (a tree of two or more leaves), or an empty
-// level (one leaf: the root is that leaf).
+// H-PLOC global launch over the ranges the chunk launch parked.
+__global__ __launch_bounds__(k_hploc_block) void seg_hploc_parked_kernel(
+    seg_mesh const *__restrict__ meshes, std::uint32_t const *__restrict__ leaf_prefix, std::uint32_t mesh_count,
+    std::uint64_t const *__restrict__ codes, aabb const *__restrict__ leaf_boxes, std::uint32_t total,
+    std::uint32_t radius, hploc_out o, std::uint32_t *boundary, uint2 const *parked, std::uint32_t *root_words,
+    std::uint32_t *words
+) {
+    std::uint32_t const i = blockIdx.x * k_hploc_block + threadIdx.x;
+    bool const active = i < words[4];
+    uint2 const r = active ? parked[i] : make_uint2(0u, 0u);
+    build_segment const seg = active ? seg_of(meshes, leaf_prefix, mesh_count, r.x) : build_segment{0u, 0u, 0u, 0u};
+    hploc_climb<false>(codes, leaf_boxes, total, radius, o, boundary, seg, r.x, r.y, active, 0u, 0u, nullptr, nullptr,
+                       root_words, words);
+}
+
+// Karras LBVH over every mesh: one thread per binary internal node of the concatenation (mesh m's
+// nodes are [internal_prefix[m], internal_prefix[m + 1]); meshes of one triangle have none).
+__global__ __launch_bounds__(k_block) void seg_lbvh_kernel(
+    seg_mesh const *__restrict__ meshes, std::uint32_t const *__restrict__ internal_prefix, std::uint32_t mesh_count,
+    std::uint32_t internal_total, std::uint64_t const *__restrict__ codes, uint2 *__restrict__ bnodes,
+    std::uint32_t *__restrict__ iparent
+) {
+    std::uint32_t const j = blockIdx.x * k_block + threadIdx.x;
+    if (j >= internal_total)
+        return;
+    seg_mesh const mm = meshes[prefix_slot(internal_prefix, mesh_count, j)];
+    lbvh_node(codes + mm.leaf_base, mm.leaves, j - mm.internal_base, bnodes + mm.internal_base,
+              iparent + mm.internal_base, nullptr);
+}
+
+// Binary depth of every internal node within its mesh; the deepest into *max_depth.
+__global__ __launch_bounds__(k_block) void seg_lbvh_depth_kernel(
+    seg_mesh const *__restrict__ meshes, std::uint32_t const *__restrict__ internal_prefix, std::uint32_t mesh_count,
+    std::uint32_t internal_total, std::uint32_t const *__restrict__ iparent, std::uint32_t *__restrict__ depth,
+    std::uint32_t *max_depth, std::uint32_t *error
+) {
+    std::uint32_t const j = blockIdx.x * k_block + threadIdx.x;
+    std::uint32_t d = 0;
+    if (j < internal_total) {
+        seg_mesh const mm = meshes[prefix_slot(internal_prefix, mesh_count, j)];
+        d = lbvh_node_depth(iparent + mm.internal_base, mm.leaves - 1u, j - mm.internal_base, error);
+        depth[j] = d;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        std::uint32_t const o = __shfl_xor(d, off);
+        d = o > d ? o : d;
+    }
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(max_depth, d);
+}
+
+// Bounds of every mesh's internal nodes at binary depth `level` (launched deepest first).
+__global__ __launch_bounds__(k_block) void seg_lbvh_bounds_level_kernel(
+    seg_mesh const *__restrict__ meshes, std::uint32_t const *__restrict__ internal_prefix, std::uint32_t mesh_count,
+    std::uint32_t internal_total, std::uint32_t const *__restrict__ depth, std::uint32_t level,
+    aabb const *__restrict__ leaf_boxes, uint2 const *__restrict__ bnodes, aabb *__restrict__ ibounds,
+    std::uint32_t *__restrict__ ifirst
+) {
+    std::uint32_t const j = blockIdx.x * k_block + threadIdx.x;
+    if (j >= internal_total || depth[j] != level)
+        return;
+    seg_mesh const mm = meshes[prefix_slot(internal_prefix, mesh_count, j)];
+    lbvh_node_bounds(leaf_boxes + mm.leaf_base, bnodes + mm.internal_base, j - mm.internal_base,
+                     ibounds + mm.internal_base, ifirst + mm.internal_base);
+}
+
+// Level 0 of every mesh's collapse: the binary root (LBVH: node 0; H-PLOC: the recorded root) of a
+// tree of two or more leaves, or an empty level (one leaf: the root is that leaf).
 __global__ __launch_bounds__(k_block) void seg_collapse_init_kernel(
     seg_mesh const *__restrict__ meshes, collapse_tree const *__restrict__ trees, std::uint32_t mesh_count,
     std::uint32_t const *__restrict__ root_words, aabb *__restrict__ root_box
@@ -293,15 +369,18 @@ __global__ __launch_bounds__(k_block) void seg_collapse_level_kernel(
 
 // Per mesh after the last level launch: a failed build (2), a tree deeper than the launched levels
 // (4), a traversal stack bound above the compiled capacity (8).  Mesh 0's thread also checks the
-// H-PLOC merge count and failure flag and the records' error word.
+// words: [0] H-PLOC merges, [1] failure flag (H-PLOC or an LBVH parent chain), [2] the records'
+// error word, [3] the deepest LBVH binary level against the bounds launches.
 __global__ __launch_bounds__(k_block) void seg_check_kernel(
     seg_mesh const *__restrict__ meshes, collapse_tree const *__restrict__ trees, std::uint32_t mesh_count,
     int levels, std::uint32_t stack_capacity, std::uint32_t const *__restrict__ words, std::uint32_t expected_merges,
-    std::uint32_t *status
+    std::uint32_t binary_levels, std::uint32_t *status
 ) {
     std::uint32_t const m = blockIdx.x * k_block + threadIdx.x;
     if (m == 0u && (words[0] != expected_merges || words[1] != 0u || words[2] != 0u))
         atomicOr(status, 2u);
+    if (m == 0u && words[3] >= binary_levels)
+        atomicOr(status, 4u); // an LBVH deeper than the bounds launches
     if (m >= mesh_count)
         return;
     seg_mesh const mm = meshes[m];
@@ -381,6 +460,7 @@ struct batch_trees {
     std::uint32_t internal_total = 0;  // binary internal nodes of all meshes (= node slots)
     std::uint32_t vertex_blocks = 0;
     std::uint32_t status_words = 0;
+    int method = GWN_BVH_BUILD_HPLOC;
     int width = 4;
     std::uint32_t radius = 8;
     int order = 1;
@@ -388,7 +468,7 @@ struct batch_trees {
     unsigned grid = 1;
     std::vector<seg_mesh> host_meshes;
     device_buffer<seg_mesh> meshes;
-    device_buffer<std::uint32_t> leaf_prefix, block_prefix;
+    device_buffer<std::uint32_t> leaf_prefix, block_prefix, internal_prefix;
     // Build scratch.
     device_buffer<std::uint32_t> scene_words;
     device_buffer<float> scene;
@@ -397,7 +477,8 @@ struct batch_trees {
     device_buffer<std::uint32_t> ids, ids_alt, keys, keys_alt, pos, pos_alt, sort_ws;
     device_buffer<uint2> bnodes;
     device_buffer<aabb> ibounds;
-    device_buffer<std::uint32_t> ifirst, clusters, boundary, root_words, words;
+    device_buffer<std::uint32_t> ifirst, clusters, boundary, root_words, words, iparent, depth;
+    device_buffer<uint2> parked;
     device_buffer<collapse_task> tasks;
     device_buffer<collapse_level> levels;  // mesh_count x (k_batch_build_max_levels + 1)
     device_buffer<collapse_stats> stats;
@@ -435,14 +516,14 @@ unsigned blocks_for(std::size_t n, int block) { return grid_for(n, block); }
 
 } // namespace
 
-batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> const &geometries, int width,
-                                   std::uint32_t hploc_radius, int order, hipStream_t stream) {
+batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> const &geometries, int method,
+                                   int width, std::uint32_t hploc_radius, int order, hipStream_t stream) {
     if (order < 0 || order > 2)
         throw_invalid("Taylor order must be 0, 1, or 2.");
     batch_trees_ptr tp(new batch_trees());
     batch_trees &t = *tp;
     t.order = order;
-    batch_trees_set_build(t, width, hploc_radius);
+    batch_trees_set_build(t, method, width, hploc_radius);
     std::size_t const M = geometries.size();
     t.mesh_count = M;
     if (M == 0)
@@ -450,7 +531,7 @@ batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> cons
     if (M > (1u << 24))
         throw_invalid("batch: Too many meshes for one segmented build.");
     std::uint64_t leaves = 0, blocks = 0, status_words = 0;
-    std::vector<std::uint32_t> leaf_prefix(M + 1), block_prefix(M + 1);
+    std::vector<std::uint32_t> leaf_prefix(M + 1), block_prefix(M + 1), internal_prefix(M + 1);
     for (std::size_t m = 0; m < M; ++m) {
         gwn_geometry_handle const &g = *geometries[m];
         std::uint64_t const T = g.triangle_count;
@@ -461,6 +542,7 @@ batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> cons
         if (T > (1ull << 30) - 1ull)
             throw_invalid("bvh.build: Triangle count exceeds the native wide-collapse counter range.");
         leaf_prefix[m] = static_cast<std::uint32_t>(leaves);
+        internal_prefix[m] = static_cast<std::uint32_t>(leaves - m);
         block_prefix[m] = static_cast<std::uint32_t>(blocks);
         seg_mesh sm{};
         sm.x = g.x.ptr;
@@ -482,6 +564,7 @@ batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> cons
             throw_invalid("batch: The meshes' triangles exceed the segmented build's 2^31 - 1 range.");
     }
     leaf_prefix[M] = static_cast<std::uint32_t>(leaves);
+    internal_prefix[M] = static_cast<std::uint32_t>(leaves - M);
     block_prefix[M] = static_cast<std::uint32_t>(blocks);
     t.total = static_cast<std::uint32_t>(leaves);
     t.internal_total = static_cast<std::uint32_t>(leaves - M);
@@ -500,6 +583,7 @@ batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> cons
     alloc(t.meshes, M, stream, b);
     alloc(t.leaf_prefix, M + 1, stream, b);
     alloc(t.block_prefix, M + 1, stream, b);
+    alloc(t.internal_prefix, M + 1, stream, b);
     alloc(t.scene_words, 6 * M, stream, b);
     alloc(t.scene, 9 * M, stream, b);
     alloc(t.boxes, n, stream, b);
@@ -519,7 +603,10 @@ batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> cons
     alloc(t.clusters, n, stream, b);
     alloc(t.boundary, n, stream, b);
     alloc(t.root_words, M, stream, b);
-    alloc(t.words, 4, stream, b);
+    alloc(t.words, 8, stream, b);
+    alloc(t.parked, 2 * ((n + k_hploc_chunk - 1) / k_hploc_chunk), stream, b);
+    alloc(t.iparent, std::max<std::size_t>(1, ni), stream, b);
+    alloc(t.depth, std::max<std::size_t>(1, ni), stream, b);
     alloc(t.tasks, std::max<std::size_t>(1, ni), stream, b);
     alloc(t.levels, M * k_level_stride, stream, b);
     alloc(t.stats, M, stream, b);
@@ -563,6 +650,8 @@ batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> cons
                            hipMemcpyHostToDevice, stream));
     GWN_HIP(hipMemcpyAsync(t.block_prefix.ptr, block_prefix.data(), (M + 1) * sizeof(std::uint32_t),
                            hipMemcpyHostToDevice, stream));
+    GWN_HIP(hipMemcpyAsync(t.internal_prefix.ptr, internal_prefix.data(), (M + 1) * sizeof(std::uint32_t),
+                           hipMemcpyHostToDevice, stream));
     GWN_HIP(hipMemcpyAsync(t.trees.ptr, trees.data(), M * sizeof(collapse_tree), hipMemcpyHostToDevice, stream));
     GWN_HIP(hipMemcpyAsync(t.refit_meshes.ptr, refits.data(), M * sizeof(moment_refit_mesh), hipMemcpyHostToDevice,
                            stream));
@@ -570,22 +659,25 @@ batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> cons
     return tp;
 }
 
-void batch_trees_set_build(batch_trees &t, int width, std::uint32_t hploc_radius) {
+void batch_trees_set_build(batch_trees &t, int method, int width, std::uint32_t hploc_radius) {
+    if (method != GWN_BVH_BUILD_HPLOC && method != GWN_BVH_BUILD_LBVH)
+        throw_invalid("bvh.build: BVH build method is invalid.");
     if (width < 2 || width > 4)
         throw_invalid("bvh.build: BVH width must be 2, 3 or 4 (nodes are stored four-wide).");
-    if (hploc_radius == 0 || hploc_radius > 8)
+    if (method == GWN_BVH_BUILD_HPLOC && (hploc_radius == 0 || hploc_radius > 8))
         throw_invalid("bvh.build: H-PLOC search radius must be in the range [1, 8].");
+    t.method = method;
     t.width = width;
     t.radius = hploc_radius;
 }
 
-void batch_trees_enqueue(batch_trees &t, int levels, std::uint32_t stack_capacity, std::uint32_t *status,
-                         hipStream_t stream) {
+void batch_trees_enqueue(batch_trees &t, int levels, int binary_levels, std::uint32_t stack_capacity,
+                         std::uint32_t *status, hipStream_t stream) {
     std::uint32_t const M = static_cast<std::uint32_t>(t.mesh_count);
     if (M == 0)
         return;
-    if (levels < 1 || levels > k_batch_build_max_levels)
-        throw_invalid("batch: Level launch count must be in [1, 128].");
+    if (levels < 1 || levels > k_batch_build_max_levels || binary_levels < 1 || binary_levels > k_batch_build_max_levels)
+        throw_invalid("batch: Level launch counts must be in [1, 128].");
     std::uint32_t const n = t.total;
     // Scene bounds per mesh.
     seg_scene_init_kernel<<<blocks_for(M, k_block), k_block, 0, stream>>>(t.scene_words.ptr, M);
@@ -612,25 +704,50 @@ void batch_trees_enqueue(batch_trees &t, int levels, std::uint32_t stack_capacit
                                 &mesh_of, &pos_of, stream);
     }
     std::uint64_t *codes = codes1 == t.codes.ptr ? t.codes_alt.ptr : t.codes.ptr;
-    fill_u32(t.words.ptr, 0u, 4, stream);
+    fill_u32(t.words.ptr, 0u, 8, stream);
     seg_gather_kernel<<<blocks_for(n, k_block), k_block, 0, stream>>>(
         t.meshes.ptr, n, mesh_of, pos_of, codes1, ids1, t.boxes.ptr, codes, t.leaf_boxes.ptr, t.prim.ptr, t.tris.ptr,
         t.words.ptr + 2);
     GWN_LAUNCH_CHECK();
-    // H-PLOC over every mesh.
     std::size_t const ni = t.internal_total;
-    if (ni > 0) {
-        fill_u32(reinterpret_cast<std::uint32_t *>(t.bnodes.ptr), k_leaf_bit, 2 * ni, stream);
-        fill_u32(t.ifirst.ptr, 0u, ni, stream);
+    std::uint32_t const NI = t.internal_total;
+    if (t.method == GWN_BVH_BUILD_LBVH) {
+        // Karras LBVH, binary depths, then bounds one binary level per launch, deepest first.
+        fill_u32(t.root_words.ptr, 0u, M, stream);
+        if (ni > 0) {
+            seg_lbvh_kernel<<<blocks_for(ni, k_block), k_block, 0, stream>>>(t.meshes.ptr, t.internal_prefix.ptr, M, NI,
+                                                                             codes, t.bnodes.ptr, t.iparent.ptr);
+            GWN_LAUNCH_CHECK();
+            seg_lbvh_depth_kernel<<<blocks_for(ni, k_block), k_block, 0, stream>>>(
+                t.meshes.ptr, t.internal_prefix.ptr, M, NI, t.iparent.ptr, t.depth.ptr, t.words.ptr + 3,
+                t.words.ptr + 1);
+            GWN_LAUNCH_CHECK();
+            for (int d = binary_levels; d-- > 0;) {
+                seg_lbvh_bounds_level_kernel<<<blocks_for(ni, k_block), k_block, 0, stream>>>(
+                    t.meshes.ptr, t.internal_prefix.ptr, M, NI, t.depth.ptr, static_cast<std::uint32_t>(d),
+                    t.leaf_boxes.ptr, t.bnodes.ptr, t.ibounds.ptr, t.ifirst.ptr);
+                GWN_LAUNCH_CHECK();
+            }
+        }
+    } else {
+        // H-PLOC over every mesh.
+        if (ni > 0) {
+            fill_u32(reinterpret_cast<std::uint32_t *>(t.bnodes.ptr), k_leaf_bit, 2 * ni, stream);
+            fill_u32(t.ifirst.ptr, 0u, ni, stream);
+        }
+        fill_u32(t.root_words.ptr, k_invalid, M, stream);
+        seg_hploc_init_kernel<<<blocks_for(n, k_block), k_block, 0, stream>>>(t.clusters.ptr, t.boundary.ptr, n);
+        GWN_LAUNCH_CHECK();
+        hploc_out const out_arrays{t.bnodes.ptr, t.ibounds.ptr, t.ifirst.ptr, t.clusters.ptr};
+        seg_hploc_kernel<<<blocks_for(n, k_hploc_block), k_hploc_block, 0, stream>>>(
+            t.meshes.ptr, t.leaf_prefix.ptr, M, codes, t.leaf_boxes.ptr, n, t.radius, out_arrays, t.boundary.ptr,
+            t.parked.ptr, t.root_words.ptr, t.words.ptr);
+        GWN_LAUNCH_CHECK();
+        seg_hploc_parked_kernel<<<blocks_for(t.parked.count, k_hploc_block), k_hploc_block, 0, stream>>>(
+            t.meshes.ptr, t.leaf_prefix.ptr, M, codes, t.leaf_boxes.ptr, n, t.radius, out_arrays, t.boundary.ptr,
+            t.parked.ptr, t.root_words.ptr, t.words.ptr);
+        GWN_LAUNCH_CHECK();
     }
-    fill_u32(t.root_words.ptr, k_invalid, M, stream);
-    seg_hploc_init_kernel<<<blocks_for(n, k_block), k_block, 0, stream>>>(t.clusters.ptr, t.boundary.ptr, n);
-    GWN_LAUNCH_CHECK();
-    seg_hploc_kernel<<<blocks_for(n, k_hploc_block), k_hploc_block, 0, stream>>>(
-        t.meshes.ptr, t.leaf_prefix.ptr, M, codes, t.leaf_boxes.ptr, n, t.radius,
-        hploc_out{t.bnodes.ptr, t.ibounds.ptr, t.ifirst.ptr, t.clusters.ptr}, t.boundary.ptr, t.root_words.ptr,
-        t.words.ptr);
-    GWN_LAUNCH_CHECK();
     // Collapse, one plan + one persistent launch per wide level.
     fill_u32(reinterpret_cast<std::uint32_t *>(t.levels.ptr), 0u, static_cast<std::size_t>(M) * k_level_stride * 4,
              stream);
@@ -649,9 +766,11 @@ void batch_trees_enqueue(batch_trees &t, int levels, std::uint32_t stack_capacit
                                                                          t.width);
         GWN_LAUNCH_CHECK();
     }
+    std::uint32_t const merges = t.method == GWN_BVH_BUILD_HPLOC ? NI : 0u;
+    std::uint32_t const bin_check = t.method == GWN_BVH_BUILD_LBVH ? static_cast<std::uint32_t>(binary_levels) : ~0u;
     seg_check_kernel<<<blocks_for(M, k_block), k_block, 0, stream>>>(t.meshes.ptr, t.trees.ptr, M, levels,
-                                                                     stack_capacity, t.words.ptr,
-                                                                     t.internal_total, status);
+                                                                     stack_capacity, t.words.ptr, merges, bin_check,
+                                                                     status);
     GWN_LAUNCH_CHECK();
     // Moments.
     seg_refit_plan_kernel<<<levels + 1, k_plan_block, 0, stream>>>(t.meshes.ptr, t.trees.ptr, M, levels,
@@ -673,7 +792,10 @@ batch_trees_summary batch_trees_read(batch_trees &t, hipStream_t stream) {
     GWN_HIP(hipMemcpyAsync(lv.data(), t.levels.ptr, lv.size() * sizeof(collapse_level), hipMemcpyDeviceToHost, stream));
     GWN_HIP(hipMemcpyAsync(st.data(), t.stats.ptr, M * sizeof(collapse_stats), hipMemcpyDeviceToHost, stream));
     GWN_HIP(hipMemcpyAsync(s.root_boxes.data(), t.root_box.ptr, M * sizeof(aabb), hipMemcpyDeviceToHost, stream));
+    std::uint32_t words[4] = {0, 0, 0, 0};
+    GWN_HIP(hipMemcpyAsync(words, t.words.ptr, sizeof(words), hipMemcpyDeviceToHost, stream));
     GWN_HIP(hipStreamSynchronize(stream));
+    s.max_binary_depth = t.method == GWN_BVH_BUILD_LBVH ? words[3] : 0u;
     for (std::size_t m = 0; m < M; ++m) {
         if (t.host_meshes[m].leaves < 2u)
             continue;
@@ -779,6 +901,7 @@ void batch_trees_export(batch_trees &t, std::size_t i, gwn_bvh_handle &bvh, gwn_
 }
 
 int batch_trees_width(batch_trees const &t) { return t.width; }
+int batch_trees_method(batch_trees const &t) { return t.method; }
 std::uint32_t batch_trees_radius(batch_trees const &t) { return t.radius; }
 std::size_t batch_trees_bytes(batch_trees const &t) { return t.bytes; }
 

@@ -65,22 +65,7 @@ __global__ __launch_bounds__(k_block) void gather_boxes_kernel(
         sorted[i] = boxes[order[i]];
 }
 
-// delta(i, j) of Karras' construction with the reference's equal-code tie-break
-// (gwn_bvh_build_binary.cuh:37-57).
-__device__ inline int lbvh_delta(std::uint64_t const *codes, std::int64_t n, std::int64_t i, std::int64_t j) {
-    if (j < 0 || j >= n)
-        return -1;
-    std::uint64_t const ci = codes[i];
-    std::uint64_t const cj = codes[j];
-    if (ci == cj) {
-        std::uint64_t const diff = static_cast<std::uint64_t>(i) ^ static_cast<std::uint64_t>(j);
-        return diff == 0 ? 128 : 64 + __clzll(static_cast<long long>(diff));
-    }
-    return __clzll(static_cast<long long>(ci ^ cj));
-}
-
-// Binary node = (left, right) with bit 31 marking a leaf child.  Parent links carry the side in
-// bit 0: parent = (node << 1) | side.
+// Karras LBVH over one tree, one thread per internal node (lbvh_node).
 __global__ __launch_bounds__(k_block) void lbvh_kernel(
     std::uint64_t const *__restrict__ codes, std::size_t n, uint2 *__restrict__ bnodes,
     std::uint32_t *__restrict__ iparent, std::uint32_t *__restrict__ lparent
@@ -88,45 +73,7 @@ __global__ __launch_bounds__(k_block) void lbvh_kernel(
     std::size_t const tid = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
     if (tid + 1 >= n)
         return;
-    std::int64_t const i = static_cast<std::int64_t>(tid);
-    std::int64_t const N = static_cast<std::int64_t>(n);
-    int const dir = (lbvh_delta(codes, N, i, i + 1) - lbvh_delta(codes, N, i, i - 1) >= 0) ? 1 : -1;
-    int const dmin = lbvh_delta(codes, N, i, i - dir);
-    std::int64_t lmax = 2;
-    while (lbvh_delta(codes, N, i, i + lmax * dir) > dmin)
-        lmax <<= 1;
-    std::int64_t len = 0;
-    for (std::int64_t step = lmax >> 1; step > 0; step >>= 1)
-        if (lbvh_delta(codes, N, i, i + (len + step) * dir) > dmin)
-            len += step;
-    std::int64_t const j = i + len * dir;
-    std::int64_t const first = i < j ? i : j;
-    std::int64_t const last = i < j ? j : i;
-    int const dnode = lbvh_delta(codes, N, first, last);
-    std::int64_t split = first;
-    std::int64_t step = last - first;
-    do {
-        step = (step + 1) >> 1;
-        std::int64_t const cand = split + step;
-        if (cand < last && lbvh_delta(codes, N, first, cand) > dnode)
-            split = cand;
-    } while (step > 1);
-    bool const left_leaf = split == first;
-    bool const right_leaf = split + 1 == last;
-    std::uint32_t const l = static_cast<std::uint32_t>(split);
-    std::uint32_t const r = static_cast<std::uint32_t>(split + 1);
-    bnodes[i] = make_uint2(l | (left_leaf ? k_leaf_bit : 0u), r | (right_leaf ? k_leaf_bit : 0u));
-    std::uint32_t const me = static_cast<std::uint32_t>(i) << 1;
-    if (left_leaf)
-        lparent[l] = me;
-    else
-        iparent[l] = me;
-    if (right_leaf)
-        lparent[r] = me | 1u;
-    else
-        iparent[r] = me | 1u;
-    if (i == 0)
-        iparent[0] = k_invalid;
+    lbvh_node(codes, static_cast<std::int64_t>(n), static_cast<std::int64_t>(tid), bnodes, iparent, lparent);
 }
 
 // Depth of every internal node of the Karras tree (edges to the root), by following the parent
@@ -139,15 +86,7 @@ __global__ __launch_bounds__(k_block) void lbvh_depth_kernel(
     std::size_t const i = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
     std::uint32_t d = 0;
     if (i < ni) {
-        std::uint32_t link = iparent[i];
-        while (link != k_invalid) {
-            if (++d > ni || (link >> 1) >= ni) {
-                atomicOr(error, 1u);
-                d = 0;
-                break;
-            }
-            link = iparent[link >> 1];
-        }
+        d = lbvh_node_depth(iparent, ni, i, error);
         depth[i] = d;
     }
     // Wave maximum first (every lane takes part), then one atomic per wave.
@@ -169,25 +108,30 @@ __global__ __launch_bounds__(k_block) void lbvh_bounds_level_kernel(
     std::size_t const node = static_cast<std::size_t>(blockIdx.x) * k_block + threadIdx.x;
     if (node >= ni || depth[node] != level)
         return;
-    uint2 const bn = bnodes[node];
-    std::uint32_t const l = bn.x & ~k_leaf_bit, r = bn.y & ~k_leaf_bit;
-    aabb const lb = (bn.x & k_leaf_bit) ? leaf_boxes[l] : ibounds[l];
-    aabb const rb = (bn.y & k_leaf_bit) ? leaf_boxes[r] : ibounds[r];
-    std::uint32_t const lf = (bn.x & k_leaf_bit) ? l : ifirst[l];
-    std::uint32_t const rf = (bn.y & k_leaf_bit) ? r : ifirst[r];
-    ibounds[node] = aabb_union_ref(lb, rb);
-    ifirst[node] = lf < rf ? lf : rf;
+    lbvh_node_bounds(leaf_boxes, bnodes, node, ibounds, ifirst);
 }
 
-// H-PLOC over one tree, one lane per sorted leaf (hploc_climb).
-// words: [0] merges, [1] failure flag, [2] root internal id.
+// H-PLOC over one tree: the chunk launch, one lane per sorted leaf, then the global launch over
+// the ranges it parked (hploc_chunk_climb, hploc_climb).  words: [0] merges, [1] failure flag,
+// [2] root internal id, [3] parked ranges.
 __global__ __launch_bounds__(k_hploc_block) void hploc_kernel(
     std::uint64_t const *__restrict__ codes, aabb const *__restrict__ leaf_boxes, std::uint32_t T,
-    std::uint32_t radius, hploc_out o, std::uint32_t *boundary, std::uint32_t *words
+    std::uint32_t radius, hploc_out o, std::uint32_t *boundary, uint2 *parked, std::uint32_t *words
 ) {
     std::uint32_t const idx = blockIdx.x * k_hploc_block + threadIdx.x;
-    hploc_climb(codes, leaf_boxes, T, radius, o, boundary, build_segment{0u, T, 0u, 0u}, idx, idx < T, words + 2,
-                words);
+    hploc_chunk_climb(codes, leaf_boxes, T, radius, o, boundary, build_segment{0u, T, 0u, 0u}, idx, idx < T, parked,
+                      words + 3, words + 2, words);
+}
+
+__global__ __launch_bounds__(k_hploc_block) void hploc_parked_kernel(
+    std::uint64_t const *__restrict__ codes, aabb const *__restrict__ leaf_boxes, std::uint32_t T,
+    std::uint32_t radius, hploc_out o, std::uint32_t *boundary, uint2 const *parked, std::uint32_t *words
+) {
+    std::uint32_t const i = blockIdx.x * k_hploc_block + threadIdx.x;
+    bool const active = i < words[3];
+    uint2 const r = active ? parked[i] : make_uint2(0u, 0u);
+    hploc_climb<false>(codes, leaf_boxes, T, radius, o, boundary, build_segment{0u, T, 0u, 0u}, r.x, r.y, active, 0u,
+                       0u, nullptr, nullptr, words + 2, words);
 }
 
 
@@ -352,8 +296,11 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, int method, s
     device_buffer<std::uint32_t> hploc_words;
     if (method == GWN_BVH_BUILD_HPLOC) {
         device_buffer<std::uint32_t> clusters(T, stream), boundary(T, stream);
-        hploc_words.allocate(3, stream);
-        fill_u32(hploc_words.ptr, 0u, 3, stream);
+        // At most two parked ranges per chunk (one holding its first leaf, one its last).
+        std::size_t const chunks = (T + k_hploc_chunk - 1) / k_hploc_chunk;
+        device_buffer<uint2> parked(2 * chunks, stream);
+        hploc_words.allocate(4, stream);
+        fill_u32(hploc_words.ptr, 0u, 4, stream);
         // Every node id is written exactly once by a correct run; the fill keeps any node a
         // failed run leaves unwritten in bounds (two references to leaf 0) for the collapse.
         fill_u32(reinterpret_cast<std::uint32_t *>(bnodes.ptr), k_leaf_bit, 2 * NI, stream);
@@ -361,9 +308,15 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, int method, s
         hploc_init_kernel<<<grid_for(T, k_block), k_block, 0, stream>>>(
             clusters.ptr, boundary.ptr, static_cast<std::uint32_t>(T));
         GWN_LAUNCH_CHECK();
+        hploc_out const out_arrays{bnodes.ptr, ibounds.ptr, ifirst.ptr, clusters.ptr};
         hploc_kernel<<<grid_for(T, k_hploc_block), k_hploc_block, 0, stream>>>(
-            sorted_codes, leaf_boxes.ptr, static_cast<std::uint32_t>(T), hploc_radius,
-            hploc_out{bnodes.ptr, ibounds.ptr, ifirst.ptr, clusters.ptr}, boundary.ptr, hploc_words.ptr
+            sorted_codes, leaf_boxes.ptr, static_cast<std::uint32_t>(T), hploc_radius, out_arrays, boundary.ptr,
+            parked.ptr, hploc_words.ptr
+        );
+        GWN_LAUNCH_CHECK();
+        hploc_parked_kernel<<<grid_for(2 * chunks, k_hploc_block), k_hploc_block, 0, stream>>>(
+            sorted_codes, leaf_boxes.ptr, static_cast<std::uint32_t>(T), hploc_radius, out_arrays, boundary.ptr,
+            parked.ptr, hploc_words.ptr
         );
         GWN_LAUNCH_CHECK();
     } else {

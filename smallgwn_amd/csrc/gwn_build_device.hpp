@@ -196,6 +196,99 @@ __device__ inline float half_area(aabb const &b) {
     return dx * dy + dy * dz + dz * dx;
 }
 
+// delta(i, j) of Karras' construction with the reference's equal-code tie-break
+// (gwn_bvh_build_binary.cuh:37-57).
+__device__ inline int lbvh_delta(std::uint64_t const *codes, std::int64_t n, std::int64_t i, std::int64_t j) {
+    if (j < 0 || j >= n)
+        return -1;
+    std::uint64_t const ci = codes[i];
+    std::uint64_t const cj = codes[j];
+    if (ci == cj) {
+        std::uint64_t const diff = static_cast<std::uint64_t>(i) ^ static_cast<std::uint64_t>(j);
+        return diff == 0 ? 128 : 64 + __clzll(static_cast<long long>(diff));
+    }
+    return __clzll(static_cast<long long>(ci ^ cj));
+}
+
+// Internal node i of the Karras LBVH over the n sorted codes of one tree (tree-local indices;
+// `codes`, `bnodes`, `iparent` and `lparent` at the tree's bases).  Binary node = (left, right) with
+// bit 31 marking a leaf child.  Parent links carry the side in bit 0: parent = (node << 1) | side.
+// lparent may be null.
+__device__ inline void lbvh_node(
+    std::uint64_t const *__restrict__ codes, std::int64_t N, std::int64_t i, uint2 *__restrict__ bnodes,
+    std::uint32_t *__restrict__ iparent, std::uint32_t *__restrict__ lparent
+) {
+    int const dir = (lbvh_delta(codes, N, i, i + 1) - lbvh_delta(codes, N, i, i - 1) >= 0) ? 1 : -1;
+    int const dmin = lbvh_delta(codes, N, i, i - dir);
+    std::int64_t lmax = 2;
+    while (lbvh_delta(codes, N, i, i + lmax * dir) > dmin)
+        lmax <<= 1;
+    std::int64_t len = 0;
+    for (std::int64_t step = lmax >> 1; step > 0; step >>= 1)
+        if (lbvh_delta(codes, N, i, i + (len + step) * dir) > dmin)
+            len += step;
+    std::int64_t const j = i + len * dir;
+    std::int64_t const first = i < j ? i : j;
+    std::int64_t const last = i < j ? j : i;
+    int const dnode = lbvh_delta(codes, N, first, last);
+    std::int64_t split = first;
+    std::int64_t step = last - first;
+    do {
+        step = (step + 1) >> 1;
+        std::int64_t const cand = split + step;
+        if (cand < last && lbvh_delta(codes, N, first, cand) > dnode)
+            split = cand;
+    } while (step > 1);
+    bool const left_leaf = split == first;
+    bool const right_leaf = split + 1 == last;
+    std::uint32_t const l = static_cast<std::uint32_t>(split);
+    std::uint32_t const r = static_cast<std::uint32_t>(split + 1);
+    bnodes[i] = make_uint2(l | (left_leaf ? k_leaf_bit : 0u), r | (right_leaf ? k_leaf_bit : 0u));
+    std::uint32_t const me = static_cast<std::uint32_t>(i) << 1;
+    if (!left_leaf)
+        iparent[l] = me;
+    else if (lparent)
+        lparent[l] = me;
+    if (!right_leaf)
+        iparent[r] = me | 1u;
+    else if (lparent)
+        lparent[r] = me | 1u;
+    if (i == 0)
+        iparent[0] = k_invalid;
+}
+
+
+// Depth of internal node i (edges to the root) by following the parent links of its tree; a walk
+// longer than the node count means a broken parent chain: error, depth 0.
+__device__ inline std::uint32_t lbvh_node_depth(std::uint32_t const *__restrict__ iparent, std::size_t ni,
+                                                std::size_t i, std::uint32_t *error) {
+    std::uint32_t d = 0;
+    std::uint32_t link = iparent[i];
+    while (link != k_invalid) {
+        if (++d > ni || (link >> 1) >= ni) {
+            atomicOr(error, 1u);
+            return 0;
+        }
+        link = iparent[link >> 1];
+    }
+    return d;
+}
+
+// Bounds and first leaf of internal node `node` from its children (leaves, or nodes of a deeper
+// level written by an earlier launch), unioned in left/right order (the reference's order).
+__device__ inline void lbvh_node_bounds(aabb const *__restrict__ leaf_boxes, uint2 const *__restrict__ bnodes,
+                                        std::size_t node, aabb *__restrict__ ibounds,
+                                        std::uint32_t *__restrict__ ifirst) {
+    uint2 const bn = bnodes[node];
+    std::uint32_t const l = bn.x & ~k_leaf_bit, r = bn.y & ~k_leaf_bit;
+    aabb const lb = (bn.x & k_leaf_bit) ? leaf_boxes[l] : ibounds[l];
+    aabb const rb = (bn.y & k_leaf_bit) ? leaf_boxes[r] : ibounds[r];
+    std::uint32_t const lf = (bn.x & k_leaf_bit) ? l : ifirst[l];
+    std::uint32_t const rf = (bn.y & k_leaf_bit) ? r : ifirst[r];
+    ibounds[node] = aabb_union_ref(lb, rb);
+    ifirst[node] = lf < rf ? lf : rf;
+}
+
 // H-PLOC binary build (reference: include/gwn/detail/gwn_bvh_build_hploc.cuh:104-622), wave64.
 //
 // Every sorted leaf is a lane that climbs the implicit LBVH of the (code, position) keys: at each
@@ -386,40 +479,82 @@ __device__ inline std::uint32_t hploc_range(
     return made;
 }
 
-// The climb of one sorted leaf per lane; every lane of the wave calls it.  `active` lanes own leaf
-// `leaf` of tree `seg` (a tree of at least two leaves).  words: [0] merges, [1] failure flag;
-// root_words[tree] receives each tree's tree-local root.
+// H-PLOC runs in two launches with the same decisions and merges as one global climb:
+//   1. chunk climb: a workgroup takes the k_hploc_block sorted leaves of its chunk, one per lane,
+//      and climbs while the boundary a range meets lies between two leaves of the chunk: those
+//      boundaries live in LDS and workgroup-scope release / acquire order the ranges' lists and
+//      nodes (no L2 write-back).  A range whose next step meets a boundary outside the chunk
+//      parks; only a range holding the chunk's first or last leaf can, so at most two per chunk.
+//      At the end the chunk's boundary words go to the global array.
+//   2. global climb: the parked ranges go on from where they stopped over the global boundary
+//      array, with agent-scope ordering as the one-launch climb has.
+// Which lane arrives first at a boundary can differ from one launch to the other, but each
+// boundary is met once from each side and the second arrival joins the same two ranges, so every
+// merge is the same.  A range that covers its whole tree stops after its final PLOC.
+constexpr std::uint32_t k_hploc_chunk = static_cast<std::uint32_t>(k_hploc_block);
+
+__device__ inline void release_workgroup() {
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+}
+__device__ inline void acquire_workgroup() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
+
+// The climb of the ranges [left, right] of tree `seg` held by `active` lanes; every lane of the
+// wave calls it.  Chunk: boundaries in LDS (`boundary` indexed from chunk_lo), parking ranges
+// that leave [chunk_lo, chunk_hi) into `parked`; otherwise the global boundary array.  words:
+// [0] merges, [1] failure flag; root_words[tree] receives each tree's tree-local root.
+template <bool Chunk>
 __device__ inline void hploc_climb(
     std::uint64_t const *__restrict__ codes, aabb const *__restrict__ leaf_boxes, std::uint32_t total_leaves,
-    std::uint32_t radius, hploc_out const &o, std::uint32_t *boundary, build_segment seg, std::uint32_t leaf,
-    bool active, std::uint32_t *root_words, std::uint32_t *words
+    std::uint32_t radius, hploc_out const &o, std::uint32_t *boundary, build_segment seg, std::uint32_t left,
+    std::uint32_t right, bool active, std::uint32_t chunk_lo, std::uint32_t chunk_hi, uint2 *parked,
+    std::uint32_t *parked_count, std::uint32_t *root_words, std::uint32_t *words
 ) {
     std::uint32_t const lane = threadIdx.x & 63u;
-    std::uint32_t left = leaf, right = leaf, split = 0;
+    std::uint32_t split = 0;
     std::uint32_t const last = seg.leaf_base + seg.leaves - 1u;
     bool bad = false;
     std::uint32_t made = 0;
     for (;;) {
-        release_agent();
+        if constexpr (Chunk)
+            release_workgroup();
+        else
+            release_agent();
         if (active) {
             bool const to_right =
                 left == seg.leaf_base ||
                 (right != last && hploc_delta(codes, right, right + 1u, seg.leaf_base) <
                                       hploc_delta(codes, left - 1u, left, seg.leaf_base));
             std::uint32_t const slot = to_right ? right : left - 1u;
-            std::uint32_t const prev = __hip_atomic_exchange(
-                &boundary[slot], to_right ? left : right, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (prev == k_invalid) {
+            bool park = false;
+            if constexpr (Chunk)
+                park = slot < chunk_lo || slot + 1u >= chunk_hi;
+            if (park) {
+                parked[atomicAdd(parked_count, 1u)] = make_uint2(left, right);
                 active = false;
-            } else if (to_right) {
-                split = right + 1u;
-                right = prev;
             } else {
-                split = left;
-                left = prev;
+                std::uint32_t const want = to_right ? left : right;
+                std::uint32_t prev;
+                if constexpr (Chunk)
+                    prev = __hip_atomic_exchange(&boundary[slot - chunk_lo], want, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+                else
+                    prev = __hip_atomic_exchange(&boundary[slot], want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (prev == k_invalid) {
+                    active = false;
+                } else if (to_right) {
+                    split = right + 1u;
+                    right = prev;
+                } else {
+                    split = left;
+                    left = prev;
+                }
             }
         }
-        acquire_agent();
+        if constexpr (Chunk)
+            acquire_workgroup();
+        else
+            acquire_agent();
         bool const fin = active && right - left + 1u == seg.leaves;
         std::uint64_t pending = __ballot(active && (right - left + 1u > k_hploc_list || fin));
         while (pending != 0u) {
@@ -442,6 +577,8 @@ __device__ inline void hploc_climb(
                 made += hploc_range(leaf_boxes, total_leaves, radius, o, rseg, rl, rr + 1u, rs, rf != 0, root_words,
                                     bad);
         }
+        if (fin)
+            active = false; // the tree is complete
         if (__ballot(active) == 0u)
             break;
     }
@@ -453,6 +590,28 @@ __device__ inline void hploc_climb(
         if (any_bad)
             atomicOr(&words[1], 1u);
     }
+}
+
+// The chunk launch's body for the block's chunk of sorted leaves [chunk_lo, chunk_hi): boundary
+// words in LDS, then written back to the global array for the global launch.  `seg` / `active`
+// are this lane's leaf's tree and whether it takes part (a tree of at least two leaves).
+__device__ inline void hploc_chunk_climb(
+    std::uint64_t const *__restrict__ codes, aabb const *__restrict__ leaf_boxes, std::uint32_t total_leaves,
+    std::uint32_t radius, hploc_out const &o, std::uint32_t *boundary, build_segment seg, std::uint32_t leaf,
+    bool active, uint2 *parked, std::uint32_t *parked_count, std::uint32_t *root_words, std::uint32_t *words
+) {
+    __shared__ std::uint32_t s_boundary[k_hploc_chunk];
+    std::uint32_t const chunk_lo = blockIdx.x * k_hploc_chunk;
+    std::uint32_t const chunk_hi =
+        total_leaves - chunk_lo < k_hploc_chunk ? total_leaves : chunk_lo + k_hploc_chunk;
+    s_boundary[threadIdx.x] = k_invalid;
+    __syncthreads();
+    hploc_climb<true>(codes, leaf_boxes, total_leaves, radius, o, s_boundary, seg, leaf, leaf, active, chunk_lo,
+                      chunk_hi, parked, parked_count, root_words, words);
+    __syncthreads();
+    // Boundaries between two leaves of the chunk (the chunk's last slot faces the next chunk).
+    if (chunk_lo + threadIdx.x + 1u < chunk_hi)
+        boundary[chunk_lo + threadIdx.x] = s_boundary[threadIdx.x];
 }
 
 // Level-synchronous BVH4 collapse.  Level d holds the wide nodes at depth d, numbered

@@ -1460,8 +1460,6 @@ int gwn_batch_create_built(const gwn_batch_build_item *items, uint64_t item_coun
             gwn::throw_invalid("Batch item count exceeds the mesh index range.");
         if (!(accuracy_scale >= 0.0f))
             gwn::throw_invalid("Taylor accuracy scale must be nonnegative.");
-        if (config->method != GWN_BVH_BUILD_HPLOC)
-            gwn::throw_invalid("batch: Segmented builds use the H-PLOC builder.");
         std::vector<gwn_geometry_handle const *> geometries;
         std::vector<gwn::batch_mesh> qs;
         for (uint64_t i = 0; i < item_count; ++i) {
@@ -1479,8 +1477,8 @@ int gwn_batch_create_built(const gwn_batch_build_item *items, uint64_t item_coun
             m.count = it.count;
             qs.push_back(m);
         }
-        *out = gwn::batch_create_built(geometries, qs, config->width, config->hploc_search_radius, config->order,
-                                       accuracy_scale, stream);
+        *out = gwn::batch_create_built(geometries, qs, config->method, config->width, config->hploc_search_radius,
+                                       config->order, accuracy_scale, stream);
     });
 }
 
@@ -1496,9 +1494,7 @@ int gwn_batch_set_build(gwn_batch_t batch, const gwn_batch_build_config *config,
     return guarded("gwn_batch_set_build", [&] {
         if (batch == nullptr || config == nullptr)
             gwn::throw_invalid("Batch handle or build configuration pointer is null.");
-        if (config->method != GWN_BVH_BUILD_HPLOC)
-            gwn::throw_invalid("batch: Segmented builds use the H-PLOC builder.");
-        gwn::batch_set_build(*batch, config->width, config->hploc_search_radius, stream);
+        gwn::batch_set_build(*batch, config->method, config->width, config->hploc_search_radius, stream);
     });
 }
 
@@ -1506,10 +1502,10 @@ int gwn_batch_get_build(gwn_batch_t batch, gwn_batch_build_config *config, int32
     return guarded("gwn_batch_get_build", [&] {
         if (batch == nullptr || config == nullptr)
             gwn::throw_invalid("Batch handle or build configuration pointer is null.");
-        int width = 0, levels = 0, order = 0;
+        int method = 0, width = 0, levels = 0, order = 0;
         std::uint32_t radius = 0;
-        gwn::batch_get_build(*batch, width, radius, levels, order);
-        config->method = GWN_BVH_BUILD_HPLOC;
+        gwn::batch_get_build(*batch, method, width, radius, levels, order);
+        config->method = method;
         config->order = order;
         config->width = width;
         config->hploc_search_radius = radius;

@@ -356,16 +356,19 @@ struct batch_trees_deleter {
 using batch_trees_ptr = std::unique_ptr<batch_trees, batch_trees_deleter>;
 // Largest number of wide levels one pass can collapse (the level tables hold one more entry).
 constexpr int k_batch_build_max_levels = 128;
-batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> const &geometries, int width,
-                                   std::uint32_t hploc_radius, int order, hipStream_t stream);
-void batch_trees_set_build(batch_trees &t, int width, std::uint32_t hploc_radius);
-// Enqueues every mesh's build and moment refit, collapsing `levels` wide levels at most.  Failures
-// OR bits into the device word *status: 1 moment refit, 2 build, 4 a tree deeper than `levels`,
-// 8 a tree whose traversal stack bound exceeds `stack_capacity`.
-void batch_trees_enqueue(batch_trees &t, int levels, std::uint32_t stack_capacity, std::uint32_t *status,
-                         hipStream_t stream);
+// method: GWN_BVH_BUILD_HPLOC or GWN_BVH_BUILD_LBVH.
+batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> const &geometries, int method,
+                                   int width, std::uint32_t hploc_radius, int order, hipStream_t stream);
+void batch_trees_set_build(batch_trees &t, int method, int width, std::uint32_t hploc_radius);
+// Enqueues every mesh's build and moment refit, collapsing `levels` wide levels and (LBVH) bounding
+// `binary_levels` binary levels at most.  Failures OR bits into the device word *status: 1 moment
+// refit, 2 build, 4 a tree deeper than the launches, 8 a tree whose traversal stack bound exceeds
+// `stack_capacity`.
+void batch_trees_enqueue(batch_trees &t, int levels, int binary_levels, std::uint32_t stack_capacity,
+                         std::uint32_t *status, hipStream_t stream);
 struct batch_trees_summary {
     std::uint32_t max_depth = 0;          // deepest wide level index over all trees
+    std::uint32_t max_binary_depth = 0;   // deepest LBVH binary level (LBVH builds)
     std::uint32_t max_internal_stack = 0; // largest internal_stack_bound
     std::vector<aabb> root_boxes;
 };
@@ -378,6 +381,7 @@ void batch_trees_view(batch_trees const &t, std::size_t i, float4 const **eval_r
 void batch_trees_export(batch_trees &t, std::size_t i, gwn_bvh_handle &bvh, gwn_moment_handle &moment,
                         hipStream_t stream);
 int batch_trees_width(batch_trees const &t);
+int batch_trees_method(batch_trees const &t);
 std::uint32_t batch_trees_radius(batch_trees const &t);
 // Bytes of device memory the pass holds.
 std::size_t batch_trees_bytes(batch_trees const &t);
@@ -402,18 +406,20 @@ void batch_replay(gwn_batch_handle &b, hipStream_t stream);
 std::uint32_t batch_status(gwn_batch_handle &b, hipStream_t stream);
 // The message for a nonzero status word (the most severe bit).
 std::string batch_status_message(std::uint32_t bits);
-// A batch over geometries: the batch builds every mesh's tree (H-PLOC, `hploc_radius`, `width`)
-// and Taylor moments of `order` itself (segmented builds) and owns them; queries as batch_create.
+// A batch over geometries: the batch builds every mesh's tree (`method` LBVH or H-PLOC with
+// `hploc_radius`, `width`) and Taylor moments of `order` itself (segmented builds) and owns them;
+// queries as batch_create.
 gwn_batch_handle *batch_create_built(
-    std::vector<gwn_geometry_handle const *> const &geometries, std::vector<batch_mesh> const &queries, int width,
-    std::uint32_t hploc_radius, int order, float accuracy_scale, hipStream_t stream
+    std::vector<gwn_geometry_handle const *> const &geometries, std::vector<batch_mesh> const &queries, int method,
+    int width, std::uint32_t hploc_radius, int order, float accuracy_scale, hipStream_t stream
 );
 // Enqueues every mesh's build and moment refit (a batch over geometries).
 void batch_build(gwn_batch_handle &b, hipStream_t stream);
 // New build parameters for a batch over geometries: rebuilds now and re-plans (synchronizes).
-void batch_set_build(gwn_batch_handle &b, int width, std::uint32_t hploc_radius, hipStream_t stream);
-void batch_get_build(gwn_batch_handle const &b, int &width, std::uint32_t &hploc_radius, int &levels, int &order);
-// Times builds + refits + queries over candidate widths and search radii and keeps the fastest
+void batch_set_build(gwn_batch_handle &b, int method, int width, std::uint32_t hploc_radius, hipStream_t stream);
+void batch_get_build(gwn_batch_handle const &b, int &method, int &width, std::uint32_t &hploc_radius, int &levels,
+                     int &order);
+// Times builds + refits + queries over candidate builders, widths and search radii and keeps the fastest
 // (a batch over geometries; synchronizes; drops a captured graph).  Returns its milliseconds.
 float batch_tune_build(gwn_batch_handle &b, hipStream_t stream);
 // Copies mesh i's tree and moments into BVH / moment objects (a batch over geometries).

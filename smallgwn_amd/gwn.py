@@ -982,12 +982,12 @@ class _BatchBuildConfig(ctypes.Structure):
 
 class BuiltBatchObject(BatchObject):
     """Batched mode over geometries (no reference counterpart): the batch builds and owns every
-    mesh's H-PLOC tree and Taylor moments in one segmented pass, which a captured graph replays
+    mesh's tree (H-PLOC or LBVH) and Taylor moments in one segmented pass, which a captured graph replays
     together with the queries.  ``items`` are tuples (geometry, qx, qy, qz, out); each mesh's tree,
     moments and results equal a single-mesh build, refit and Taylor batch of that mesh."""
 
     def __init__(self, items, width: int = 4, hploc_radius: int = 8, order: int = 1, accuracy_scale: float = 2.0,
-                 stream=None):
+                 method: "BuildMethod" = None, stream=None):
         self._keep = list(items)
         arr = (_BatchBuildItem * max(1, len(self._keep)))()
         for k, (geo, qx, qy, qz, out) in enumerate(self._keep):
@@ -995,7 +995,8 @@ class BuiltBatchObject(BatchObject):
             if _numel(qy) != n or _numel(qz) != n or _numel(out) != n:
                 raise ValueError("Query SoA spans require identical lengths.")
             arr[k] = _BatchBuildItem(geo.handle.value, _dptr(qx), _dptr(qy), _dptr(qz), _dptr(out), n)
-        cfg = _BatchBuildConfig(int(BuildMethod.HPLOC), int(width), int(hploc_radius), int(order))
+        method = BuildMethod.HPLOC if method is None else method
+        cfg = _BatchBuildConfig(int(method), int(width), int(hploc_radius), int(order))
         h = ctypes.c_void_p()
         _status(_lib.gwn_batch_create_built(arr, len(self._keep), ctypes.byref(cfg), float(accuracy_scale),
                                             ctypes.byref(h), _stream(stream))).raise_if_error()
@@ -1005,16 +1006,17 @@ class BuiltBatchObject(BatchObject):
         """Enqueues every mesh's build and moment refit."""
         return _status(_lib.gwn_batch_build(self._h, _stream(stream)))
 
-    def set_build(self, width: int, hploc_radius: int, stream=None) -> Status:
-        cfg = _BatchBuildConfig(int(BuildMethod.HPLOC), int(width), int(hploc_radius), 0)
+    def set_build(self, width: int, hploc_radius: int, method: "BuildMethod" = None, stream=None) -> Status:
+        method = BuildMethod.HPLOC if method is None else method
+        cfg = _BatchBuildConfig(int(method), int(width), int(hploc_radius), 0)
         return _status(_lib.gwn_batch_set_build(self._h, ctypes.byref(cfg), _stream(stream)))
 
     def build_config(self) -> dict:
         cfg = _BatchBuildConfig()
         levels = ctypes.c_int32()
         _status(_lib.gwn_batch_get_build(self._h, ctypes.byref(cfg), ctypes.byref(levels))).raise_if_error()
-        return {"width": cfg.width, "hploc_search_radius": cfg.hploc_search_radius, "order": cfg.order,
-                "level_launches": levels.value}
+        return {"method": BuildMethod(cfg.method), "width": cfg.width, "hploc_search_radius": cfg.hploc_search_radius,
+                "order": cfg.order, "level_launches": levels.value}
 
     def tune_build(self, stream=None) -> float:
         ms = ctypes.c_float()

@@ -1,6 +1,7 @@
 """Batched mode over geometries (gwn_batch_create_built): every mesh's tree, records and moments
-come from one segmented pass over all meshes, and must equal gwn_build_bvh (H-PLOC, same width and
-radius) + gwn_refit_bvh_moment of that mesh alone bit for bit; the query results must equal the
+come from one segmented pass over all meshes, and must equal gwn_build_bvh (the same builder --
+H-PLOC with the same radius, or LBVH -- and width) + gwn_refit_bvh_moment of that mesh alone bit
+for bit; the query results must equal the
 single-mesh Taylor batch's bit for bit, eagerly and in graph replays that rebuild every tree from
 updated vertices.  The meshes cover one and two triangles, repeated triangles (equal Morton codes,
 so ties break by mesh-local position), uneven sizes, and enough meshes for a two-pass mesh sort."""
@@ -53,11 +54,12 @@ def _setup(gwn, dev, meshes, seed=0):
     return geoms, items
 
 
-def _sequential(gwn, geom, q, width, radius, order):
+def _sequential(gwn, geom, q, width, radius, order, method=None):
     import torch
 
+    method = gwn.BuildMethod.HPLOC if method is None else method
     bvh = gwn.BvhObject()
-    gwn.gwn_build_bvh(geom, bvh, gwn.BuildOptions(method=gwn.BuildMethod.HPLOC, hploc_search_radius=radius,
+    gwn.gwn_build_bvh(geom, bvh, gwn.BuildOptions(method=method, hploc_search_radius=radius,
                                                   width=width)).raise_if_error()
     mom = gwn.MomentObject(order)
     gwn.gwn_refit_bvh_moment(bvh, mom).raise_if_error()
@@ -86,31 +88,37 @@ def _same_results(k, got, want):
     assert diff.size == 0, f"mesh {k}: {diff.size} of {g.size} results differ"
 
 
-@pytest.mark.parametrize("count,width,radius,order", [(40, 4, 8, 1), (24, 2, 1, 0), (24, 3, 4, 2), (300, 4, 8, 1)])
-def test_built_batch_equals_single_mesh_builds(gwn, dev, count, width, radius, order):
+@pytest.mark.parametrize("lbvh,count,width,radius,order", [
+    (False, 40, 4, 8, 1), (False, 24, 2, 1, 0), (False, 24, 3, 4, 2), (False, 300, 4, 8, 1),
+    (True, 40, 4, 8, 1), (True, 24, 2, 8, 2), (True, 300, 3, 8, 0)])
+def test_built_batch_equals_single_mesh_builds(gwn, dev, lbvh, count, width, radius, order):
     import torch
 
+    method = gwn.BuildMethod.LBVH if lbvh else gwn.BuildMethod.HPLOC
     meshes = _meshes(count, seed=count + width, max_tris=20000 if count < 100 else 4000)
     geoms, items = _setup(gwn, dev, meshes, seed=count)
-    batch = gwn.BuiltBatchObject(items, width=width, hploc_radius=radius, order=order, accuracy_scale=2.0)
+    batch = gwn.BuiltBatchObject(items, width=width, hploc_radius=radius, order=order, accuracy_scale=2.0,
+                                 method=method)
     cfg = batch.build_config()
-    assert (cfg["width"], cfg["hploc_search_radius"], cfg["order"]) == (width, radius, order)
+    assert (cfg["method"], cfg["width"], cfg["hploc_search_radius"], cfg["order"]) == (method, width, radius, order)
     assert 1 <= cfg["level_launches"] <= 128
     batch.query().raise_if_error()
     torch.cuda.synchronize()
     for k, (geom, qx, qy, qz, out) in enumerate(items):
-        want = _sequential(gwn, geom, (qx, qy, qz), width, radius, order)
+        want = _sequential(gwn, geom, (qx, qy, qz), width, radius, order, method)
         _same_tree(k, batch.export(k), want[:2])
         _same_results(k, out, want[2])
     batch.status().raise_if_error()
 
 
-def test_built_batch_graph_rebuilds_updated_geometry(gwn, dev):
+@pytest.mark.parametrize("lbvh", [False, True])
+def test_built_batch_graph_rebuilds_updated_geometry(gwn, dev, lbvh):
     import torch
 
+    method = gwn.BuildMethod.LBVH if lbvh else gwn.BuildMethod.HPLOC
     meshes = _meshes(32, seed=9)
     geoms, items = _setup(gwn, dev, meshes, seed=9)
-    batch = gwn.BuiltBatchObject(items, width=4, hploc_radius=8, order=1, accuracy_scale=2.0)
+    batch = gwn.BuiltBatchObject(items, width=4, hploc_radius=8, order=1, accuracy_scale=2.0, method=method)
     batch.capture(include_refit=True).raise_if_error()
     rng = np.random.default_rng(5)
     for rep in range(3):
@@ -124,7 +132,7 @@ def test_built_batch_graph_rebuilds_updated_geometry(gwn, dev):
         batch.replay().raise_if_error()
         batch.status().raise_if_error()
         for k, (geom, qx, qy, qz, out) in enumerate(items):
-            want = _sequential(gwn, geom, (qx, qy, qz), 4, 8, 1)
+            want = _sequential(gwn, geom, (qx, qy, qz), 4, 8, 1, method)
             _same_results(k, out, want[2])
             if k % 7 == 0:
                 _same_tree(k, batch.export(k), want[:2])
@@ -135,7 +143,7 @@ def test_built_batch_graph_rebuilds_updated_geometry(gwn, dev):
     batch.query().raise_if_error()
     torch.cuda.synchronize()
     for k, (geom, qx, qy, qz, out) in enumerate(items):
-        _same_results(k, out, _sequential(gwn, geom, (qx, qy, qz), 4, 8, 1)[2])
+        _same_results(k, out, _sequential(gwn, geom, (qx, qy, qz), 4, 8, 1, method)[2])
 
 
 def test_built_batch_tunes_build_and_launch(gwn, dev):
@@ -148,6 +156,7 @@ def test_built_batch_tunes_build_and_launch(gwn, dev):
     assert batch.tune_build() > 0.0
     cfg = batch.build_config()
     assert cfg["width"] in (2, 3, 4) and cfg["hploc_search_radius"] in (1, 2, 4, 8)
+    assert cfg["method"] in (gwn.BuildMethod.HPLOC, gwn.BuildMethod.LBVH)
     t = batch.tuning()
     assert t["stack_capacity"] in (32, 64, 128)
     batch.capture(include_refit=True).raise_if_error()
@@ -155,7 +164,7 @@ def test_built_batch_tunes_build_and_launch(gwn, dev):
     batch.status().raise_if_error()
     torch.cuda.synchronize()
     for k, (geom, qx, qy, qz, out) in enumerate(items):
-        want = _sequential(gwn, geom, (qx, qy, qz), cfg["width"], cfg["hploc_search_radius"], 1)
+        want = _sequential(gwn, geom, (qx, qy, qz), cfg["width"], cfg["hploc_search_radius"], 1, cfg["method"])
         assert t["stack_capacity"] >= want[0].info()["internal_stack_bound"]
         _same_results(k, out, want[2])
 
