@@ -189,21 +189,21 @@ __device__ inline build_segment seg_of(seg_mesh const *__restrict__ meshes,
 // H-PLOC chunk launch: one lane per sorted leaf of every mesh (meshes of one triangle take no
 // part); a chunk may hold several meshes, whose ends stay hard range ends.  words: [0] merges,
 // [1] failure flag, [4] parked ranges.
-__global__ __launch_bounds__(k_hploc_block) void seg_hploc_kernel(
+__global__ __launch_bounds__(k_hploc_seg_chunk_block) void seg_hploc_kernel(
     seg_mesh const *__restrict__ meshes, std::uint32_t const *__restrict__ leaf_prefix, std::uint32_t mesh_count,
     std::uint64_t const *__restrict__ codes, aabb const *__restrict__ leaf_boxes, std::uint32_t total,
     std::uint32_t radius, hploc_out o, std::uint32_t *boundary, uint2 *parked, std::uint32_t *root_words,
     std::uint32_t *words
 ) {
-    std::uint32_t const idx = blockIdx.x * k_hploc_block + threadIdx.x;
+    std::uint32_t const idx = blockIdx.x * k_hploc_seg_chunk_block + threadIdx.x;
     build_segment seg{0u, 0u, 0u, 0u};
     bool active = idx < total;
     if (active) {
         seg = seg_of(meshes, leaf_prefix, mesh_count, idx);
         active = seg.leaves >= 2u;
     }
-    hploc_chunk_climb(codes, leaf_boxes, total, radius, o, boundary, seg, idx, active, parked, words + 4, root_words,
-                      words);
+    hploc_chunk_climb<k_hploc_seg_chunk_block>(codes, leaf_boxes, total, radius, o, boundary, seg, idx, active,
+                                               parked, words + 4, root_words, words);
 }
 
 // H-PLOC global launch over the ranges the chunk launch parked.
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(k_hploc_block) void seg_hploc_parked_kernel(
     std::uint32_t radius, hploc_out o, std::uint32_t *boundary, uint2 const *parked, std::uint32_t *root_words,
     std::uint32_t *words
 ) {
-    std::uint32_t const i = blockIdx.x * k_hploc_block + threadIdx.x;
+    std::uint32_t const i = blockIdx.x * k_hploc_block + threadIdx.x;  // one range per lane (they are many and small)
     bool const active = i < words[4];
     uint2 const r = active ? parked[i] : make_uint2(0u, 0u);
     build_segment const seg = active ? seg_of(meshes, leaf_prefix, mesh_count, r.x) : build_segment{0u, 0u, 0u, 0u};
@@ -604,7 +604,7 @@ batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> cons
     alloc(t.boundary, n, stream, b);
     alloc(t.root_words, M, stream, b);
     alloc(t.words, 8, stream, b);
-    alloc(t.parked, 2 * ((n + k_hploc_chunk - 1) / k_hploc_chunk), stream, b);
+    alloc(t.parked, hploc_parked_capacity(n, k_hploc_seg_chunk_block), stream, b);
     alloc(t.iparent, std::max<std::size_t>(1, ni), stream, b);
     alloc(t.depth, std::max<std::size_t>(1, ni), stream, b);
     alloc(t.tasks, std::max<std::size_t>(1, ni), stream, b);
@@ -739,7 +739,7 @@ void batch_trees_enqueue(batch_trees &t, int levels, int binary_levels, std::uin
         seg_hploc_init_kernel<<<blocks_for(n, k_block), k_block, 0, stream>>>(t.clusters.ptr, t.boundary.ptr, n);
         GWN_LAUNCH_CHECK();
         hploc_out const out_arrays{t.bnodes.ptr, t.ibounds.ptr, t.ifirst.ptr, t.clusters.ptr};
-        seg_hploc_kernel<<<blocks_for(n, k_hploc_block), k_hploc_block, 0, stream>>>(
+        seg_hploc_kernel<<<blocks_for(n, k_hploc_seg_chunk_block), k_hploc_seg_chunk_block, 0, stream>>>(
             t.meshes.ptr, t.leaf_prefix.ptr, M, codes, t.leaf_boxes.ptr, n, t.radius, out_arrays, t.boundary.ptr,
             t.parked.ptr, t.root_words.ptr, t.words.ptr);
         GWN_LAUNCH_CHECK();

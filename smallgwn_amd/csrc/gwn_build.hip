@@ -114,21 +114,21 @@ __global__ __launch_bounds__(k_block) void lbvh_bounds_level_kernel(
 // H-PLOC over one tree: the chunk launch, one lane per sorted leaf, then the global launch over
 // the ranges it parked (hploc_chunk_climb, hploc_climb).  words: [0] merges, [1] failure flag,
 // [2] root internal id, [3] parked ranges.
-__global__ __launch_bounds__(k_hploc_block) void hploc_kernel(
+__global__ __launch_bounds__(k_hploc_chunk_block) void hploc_kernel(
     std::uint64_t const *__restrict__ codes, aabb const *__restrict__ leaf_boxes, std::uint32_t T,
     std::uint32_t radius, hploc_out o, std::uint32_t *boundary, uint2 *parked, std::uint32_t *words
 ) {
-    std::uint32_t const idx = blockIdx.x * k_hploc_block + threadIdx.x;
-    hploc_chunk_climb(codes, leaf_boxes, T, radius, o, boundary, build_segment{0u, T, 0u, 0u}, idx, idx < T, parked,
-                      words + 3, words + 2, words);
+    std::uint32_t const idx = blockIdx.x * k_hploc_chunk_block + threadIdx.x;
+    hploc_chunk_climb<k_hploc_chunk_block>(codes, leaf_boxes, T, radius, o, boundary, build_segment{0u, T, 0u, 0u},
+                                           idx, idx < T, parked, words + 3, words + 2, words);
 }
 
 __global__ __launch_bounds__(k_hploc_block) void hploc_parked_kernel(
     std::uint64_t const *__restrict__ codes, aabb const *__restrict__ leaf_boxes, std::uint32_t T,
     std::uint32_t radius, hploc_out o, std::uint32_t *boundary, uint2 const *parked, std::uint32_t *words
 ) {
-    std::uint32_t const i = blockIdx.x * k_hploc_block + threadIdx.x;
-    bool const active = i < words[3];
+    std::uint32_t const i = hploc_parked_slot(blockIdx.x);
+    bool const active = i != k_invalid && i < words[3];
     uint2 const r = active ? parked[i] : make_uint2(0u, 0u);
     hploc_climb<false>(codes, leaf_boxes, T, radius, o, boundary, build_segment{0u, T, 0u, 0u}, r.x, r.y, active, 0u,
                        0u, nullptr, nullptr, words + 2, words);
@@ -297,8 +297,8 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, int method, s
     if (method == GWN_BVH_BUILD_HPLOC) {
         device_buffer<std::uint32_t> clusters(T, stream), boundary(T, stream);
         // At most two parked ranges per chunk (one holding its first leaf, one its last).
-        std::size_t const chunks = (T + k_hploc_chunk - 1) / k_hploc_chunk;
-        device_buffer<uint2> parked(2 * chunks, stream);
+        std::size_t const parked_cap = hploc_parked_capacity(T, k_hploc_chunk_block);
+        device_buffer<uint2> parked(parked_cap, stream);
         hploc_words.allocate(4, stream);
         fill_u32(hploc_words.ptr, 0u, 4, stream);
         // Every node id is written exactly once by a correct run; the fill keeps any node a
@@ -309,12 +309,12 @@ void build_bvh4(gwn_geometry_handle const &g, gwn_bvh_handle &out, int method, s
             clusters.ptr, boundary.ptr, static_cast<std::uint32_t>(T));
         GWN_LAUNCH_CHECK();
         hploc_out const out_arrays{bnodes.ptr, ibounds.ptr, ifirst.ptr, clusters.ptr};
-        hploc_kernel<<<grid_for(T, k_hploc_block), k_hploc_block, 0, stream>>>(
+        hploc_kernel<<<grid_for(T, k_hploc_chunk_block), k_hploc_chunk_block, 0, stream>>>(
             sorted_codes, leaf_boxes.ptr, static_cast<std::uint32_t>(T), hploc_radius, out_arrays, boundary.ptr,
             parked.ptr, hploc_words.ptr
         );
         GWN_LAUNCH_CHECK();
-        hploc_parked_kernel<<<grid_for(2 * chunks, k_hploc_block), k_hploc_block, 0, stream>>>(
+        hploc_parked_kernel<<<static_cast<unsigned>(hploc_parked_blocks(parked_cap)), k_hploc_block, 0, stream>>>(
             sorted_codes, leaf_boxes.ptr, static_cast<std::uint32_t>(T), hploc_radius, out_arrays, boundary.ptr,
             parked.ptr, hploc_words.ptr
         );

@@ -480,18 +480,41 @@ __device__ inline std::uint32_t hploc_range(
 }
 
 // H-PLOC runs in two launches with the same decisions and merges as one global climb:
-//   1. chunk climb: a workgroup takes the k_hploc_block sorted leaves of its chunk, one per lane,
+//   1. chunk climb: a workgroup takes the sorted leaves of its chunk, one per lane (1024-leaf
+//      chunks for one large tree; 256 for many small trees side by side, whose chunks would
+//      otherwise end in long single-lane tails),
 //      and climbs while the boundary a range meets lies between two leaves of the chunk: those
 //      boundaries live in LDS and workgroup-scope release / acquire order the ranges' lists and
 //      nodes (no L2 write-back).  A range whose next step meets a boundary outside the chunk
 //      parks; only a range holding the chunk's first or last leaf can, so at most two per chunk.
 //      At the end the chunk's boundary words go to the global array.
 //   2. global climb: the parked ranges go on from where they stopped over the global boundary
-//      array, with agent-scope ordering as the one-launch climb has.
+//      array, with agent-scope ordering as the one-launch climb has.  For one large tree every
+//      parked range is large and merges at nearly every step, so each gets a 32-lane half of its
+//      own (lane 0 of the half climbs): a wave then runs its two ranges' PLOC side by side instead
+//      of one after the other for each of up to 64 ranges.  Many small trees park many short
+//      ranges: one per lane keeps the launch small.
 // Which lane arrives first at a boundary can differ from one launch to the other, but each
 // boundary is met once from each side and the second arrival joins the same two ranges, so every
 // merge is the same.  A range that covers its whole tree stops after its final PLOC.
-constexpr std::uint32_t k_hploc_chunk = static_cast<std::uint32_t>(k_hploc_block);
+constexpr int k_hploc_chunk_block = 1024;       // one tree
+constexpr int k_hploc_seg_chunk_block = 256;    // many trees side by side
+
+// Parked range of lane `threadIdx.x` of block `block` in the global launch (k_hploc_block
+// threads per block, one range per 32-lane half), or k_invalid for the other lanes.
+__device__ inline std::uint32_t hploc_parked_slot(std::uint32_t block) {
+    if ((threadIdx.x & 31u) != 0u)
+        return k_invalid;
+    return block * (static_cast<std::uint32_t>(k_hploc_block) / 32u) + (threadIdx.x >> 5);
+}
+// Blocks of the global launch for `parked` ranges.
+constexpr std::size_t hploc_parked_blocks(std::size_t parked) {
+    return (parked + k_hploc_block / 32 - 1) / (k_hploc_block / 32);
+}
+// Most ranges a chunk launch can park: two per chunk of `chunk` leaves.
+constexpr std::size_t hploc_parked_capacity(std::size_t leaves, std::size_t chunk) {
+    return 2 * ((leaves + chunk - 1) / chunk);
+}
 
 __device__ inline void release_workgroup() {
     __builtin_amdgcn_s_waitcnt(0);
@@ -592,18 +615,19 @@ __device__ inline void hploc_climb(
     }
 }
 
-// The chunk launch's body for the block's chunk of sorted leaves [chunk_lo, chunk_hi): boundary
-// words in LDS, then written back to the global array for the global launch.  `seg` / `active`
-// are this lane's leaf's tree and whether it takes part (a tree of at least two leaves).
+// The chunk launch's body for the block's chunk of sorted leaves [chunk_lo, chunk_hi) (Chunk =
+// the block size): boundary words in LDS, then written back to the global array for the global
+// launch.  `seg` / `active` are this lane's leaf's tree and whether it takes part (a tree of at
+// least two leaves).
+template <std::uint32_t Chunk>
 __device__ inline void hploc_chunk_climb(
     std::uint64_t const *__restrict__ codes, aabb const *__restrict__ leaf_boxes, std::uint32_t total_leaves,
     std::uint32_t radius, hploc_out const &o, std::uint32_t *boundary, build_segment seg, std::uint32_t leaf,
     bool active, uint2 *parked, std::uint32_t *parked_count, std::uint32_t *root_words, std::uint32_t *words
 ) {
-    __shared__ std::uint32_t s_boundary[k_hploc_chunk];
-    std::uint32_t const chunk_lo = blockIdx.x * k_hploc_chunk;
-    std::uint32_t const chunk_hi =
-        total_leaves - chunk_lo < k_hploc_chunk ? total_leaves : chunk_lo + k_hploc_chunk;
+    __shared__ std::uint32_t s_boundary[Chunk];
+    std::uint32_t const chunk_lo = blockIdx.x * Chunk;
+    std::uint32_t const chunk_hi = total_leaves - chunk_lo < Chunk ? total_leaves : chunk_lo + Chunk;
     s_boundary[threadIdx.x] = k_invalid;
     __syncthreads();
     hploc_climb<true>(codes, leaf_boxes, total_leaves, radius, o, s_boundary, seg, leaf, leaf, active, chunk_lo,
