@@ -236,40 +236,172 @@ __global__ __launch_bounds__(k_block) void seg_lbvh_kernel(
               iparent + mm.internal_base, nullptr);
 }
 
-// Binary depth of every internal node within its mesh; the deepest into *max_depth.
-__global__ __launch_bounds__(k_block) void seg_lbvh_depth_kernel(
-    seg_mesh const *__restrict__ meshes, std::uint32_t const *__restrict__ internal_prefix, std::uint32_t mesh_count,
-    std::uint32_t internal_total, std::uint32_t const *__restrict__ iparent, std::uint32_t *__restrict__ depth,
-    std::uint32_t *max_depth, std::uint32_t *error
-) {
-    std::uint32_t const j = blockIdx.x * k_block + threadIdx.x;
-    std::uint32_t d = 0;
-    if (j < internal_total) {
-        seg_mesh const mm = meshes[prefix_slot(internal_prefix, mesh_count, j)];
-        d = lbvh_node_depth(iparent + mm.internal_base, mm.leaves - 1u, j - mm.internal_base, error);
-        depth[j] = d;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        std::uint32_t const o = __shfl_xor(d, off);
-        d = o > d ? o : d;
-    }
-    if ((threadIdx.x & 63) == 0)
-        atomicMax(max_depth, d);
+// Binary depths of every mesh's internal nodes by pointer jumping.  jump[j] packs the ancestor
+// (low word) and the edges to it (high word) of internal node j; a root is its own ancestor at
+// distance 0.  Each round replaces (a, d) by (ancestor of a,
+// d + a's distance), read as one 64-bit word, so every word stays a true (ancestor, distance)
+// pair whatever order the lanes run in, and after r rounds a node has climbed at least
+// min(depth, 2^r) edges.  Seven rounds reach 128 = k_batch_build_max_levels.
+constexpr int k_jump_rounds = 7;
+static_assert((1 << k_jump_rounds) >= k_batch_build_max_levels, "jump rounds must cover the level limit");
+
+__device__ inline std::uint64_t jump_pack(std::uint32_t ancestor, std::uint32_t distance) {
+    return static_cast<std::uint64_t>(ancestor) | (static_cast<std::uint64_t>(distance) << 32);
 }
 
-// Bounds of every mesh's internal nodes at binary depth `level` (launched deepest first).
+__global__ __launch_bounds__(k_block) void seg_jump_init_kernel(
+    seg_mesh const *__restrict__ meshes, std::uint32_t const *__restrict__ internal_prefix, std::uint32_t mesh_count,
+    std::uint32_t internal_total, std::uint32_t const *__restrict__ iparent, std::uint64_t *__restrict__ jump,
+    std::uint32_t *error
+) {
+    std::uint32_t const j = blockIdx.x * k_block + threadIdx.x;
+    if (j >= internal_total)
+        return;
+    seg_mesh const mm = meshes[prefix_slot(internal_prefix, mesh_count, j)];
+    std::uint32_t const link = iparent[j];
+    if (link == k_invalid) {
+        jump[j] = jump_pack(j, 0u);
+    } else if ((link >> 1) >= mm.leaves - 1u) {
+        atomicOr(error, 1u); // a parent outside the mesh: a broken tree
+        jump[j] = jump_pack(j, 0u);
+    } else {
+        jump[j] = jump_pack(mm.internal_base + (link >> 1), 1u);
+    }
+}
+
+__global__ __launch_bounds__(k_block) void seg_jump_kernel(std::uint32_t internal_total, std::uint64_t *jump) {
+    std::uint32_t const j = blockIdx.x * k_block + threadIdx.x;
+    if (j >= internal_total)
+        return;
+    std::uint64_t const p = __hip_atomic_load(jump + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    std::uint32_t const a = static_cast<std::uint32_t>(p);
+    if (a == j)
+        return;
+    std::uint64_t const q = __hip_atomic_load(jump + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(jump + j, jump_pack(static_cast<std::uint32_t>(q), static_cast<std::uint32_t>(p >> 32) +
+                                                                         static_cast<std::uint32_t>(q >> 32)),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The contiguous share [lo, hi) of `total` items that block blockIdx.x of a fixed grid takes.
+__device__ inline void block_share(std::uint32_t total, std::uint32_t &lo, std::uint32_t &hi) {
+    std::uint64_t const per = (static_cast<std::uint64_t>(total) + gridDim.x - 1) / gridDim.x;
+    std::uint64_t const a = per * blockIdx.x;
+    lo = static_cast<std::uint32_t>(a < total ? a : total);
+    hi = static_cast<std::uint32_t>(a + per < total ? a + per : total);
+}
+
+__device__ inline std::uint32_t depth_bin(std::uint32_t d) {
+    return d < k_batch_build_max_levels ? d : k_batch_build_max_levels - 1;
+}
+
+// Depth of every node after the rounds, the deepest into *max_depth (all ones if some node did
+// not reach its root: a tree deeper than the level limit), and the depth histogram into
+// counts[depth] (depths clamp to the last bin).  A fixed grid: each block counts its share in LDS
+// and adds it to the global counts once.
+__global__ __launch_bounds__(k_block) void seg_depth_kernel(
+    std::uint32_t internal_total, std::uint64_t const *__restrict__ jump, std::uint32_t *__restrict__ depth,
+    std::uint32_t *counts, std::uint32_t *max_depth
+) {
+    __shared__ std::uint32_t s_hist[k_batch_build_max_levels];
+    __shared__ std::uint32_t s_max;
+    for (int b = threadIdx.x; b < k_batch_build_max_levels; b += k_block)
+        s_hist[b] = 0u;
+    if (threadIdx.x == 0)
+        s_max = 0u;
+    __syncthreads();
+    std::uint32_t lo, hi;
+    block_share(internal_total, lo, hi);
+    std::uint32_t dmax = 0;
+    for (std::uint32_t j = lo + threadIdx.x; j < hi; j += k_block) {
+        std::uint64_t const p = jump[j];
+        std::uint32_t const a = static_cast<std::uint32_t>(p);
+        std::uint32_t d = static_cast<std::uint32_t>(p >> 32);
+        if (static_cast<std::uint32_t>(jump[a]) != a)
+            d = 0xffffffffu;
+        depth[j] = d;
+        dmax = d > dmax ? d : dmax;
+        atomicAdd(&s_hist[depth_bin(d)], 1u);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        std::uint32_t const o = __shfl_xor(dmax, off);
+        dmax = o > dmax ? o : dmax;
+    }
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(&s_max, dmax);
+    __syncthreads();
+    for (int b = threadIdx.x; b < k_batch_build_max_levels; b += k_block)
+        if (s_hist[b] != 0u)
+            atomicAdd(&counts[b], s_hist[b]);
+    if (threadIdx.x == 0 && hi > lo)
+        atomicMax(max_depth, s_max);
+}
+
+// Bucket starts (exclusive prefix of counts, bucket_start[k_batch_build_max_levels] = total) and
+// the scatter cursors cleared.  One block.
+__global__ __launch_bounds__(k_batch_build_max_levels) void seg_depth_scan_kernel(
+    std::uint32_t const *__restrict__ counts, std::uint32_t *__restrict__ bucket_start, std::uint32_t *cursor
+) {
+    __shared__ std::uint32_t s_c[k_batch_build_max_levels];
+    std::uint32_t const b = threadIdx.x;
+    s_c[b] = counts[b];
+    __syncthreads();
+    if (b == 0) {
+        std::uint32_t run = 0;
+        for (int k = 0; k < k_batch_build_max_levels; ++k) {
+            bucket_start[k] = run;
+            run += s_c[k];
+        }
+        bucket_start[k_batch_build_max_levels] = run;
+    }
+    cursor[b] = 0u;
+}
+
+// Node ids into their depth's bucket (any order within a bucket: each node's bounds are computed
+// on their own).  A fixed grid over the same shares: each block counts its share per depth,
+// reserves those places with one atomic per depth, then writes its nodes into them.
+__global__ __launch_bounds__(k_block) void seg_depth_scatter_kernel(
+    std::uint32_t internal_total, std::uint32_t const *__restrict__ depth, std::uint32_t const *__restrict__ bucket_start,
+    std::uint32_t *cursor, std::uint32_t *__restrict__ order
+) {
+    __shared__ std::uint32_t s_count[k_batch_build_max_levels];
+    __shared__ std::uint32_t s_base[k_batch_build_max_levels];
+    for (int b = threadIdx.x; b < k_batch_build_max_levels; b += k_block)
+        s_count[b] = 0u;
+    __syncthreads();
+    std::uint32_t lo, hi;
+    block_share(internal_total, lo, hi);
+    for (std::uint32_t j = lo + threadIdx.x; j < hi; j += k_block)
+        atomicAdd(&s_count[depth_bin(depth[j])], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < k_batch_build_max_levels; b += k_block) {
+        std::uint32_t const c = s_count[b];
+        s_base[b] = bucket_start[b] + (c != 0u ? atomicAdd(&cursor[b], c) : 0u);
+        s_count[b] = 0u;
+    }
+    __syncthreads();
+    for (std::uint32_t j = lo + threadIdx.x; j < hi; j += k_block) {
+        std::uint32_t const bin = depth_bin(depth[j]);
+        order[s_base[bin] + atomicAdd(&s_count[bin], 1u)] = j;
+    }
+}
+
+// Bounds of every mesh's internal nodes at binary depth `level` (launched deepest first): a fixed
+// grid strides over that depth's bucket.
 __global__ __launch_bounds__(k_block) void seg_lbvh_bounds_level_kernel(
     seg_mesh const *__restrict__ meshes, std::uint32_t const *__restrict__ internal_prefix, std::uint32_t mesh_count,
-    std::uint32_t internal_total, std::uint32_t const *__restrict__ depth, std::uint32_t level,
+    std::uint32_t const *__restrict__ order, std::uint32_t const *__restrict__ bucket_start, std::uint32_t level,
     aabb const *__restrict__ leaf_boxes, uint2 const *__restrict__ bnodes, aabb *__restrict__ ibounds,
     std::uint32_t *__restrict__ ifirst
 ) {
-    std::uint32_t const j = blockIdx.x * k_block + threadIdx.x;
-    if (j >= internal_total || depth[j] != level)
-        return;
-    seg_mesh const mm = meshes[prefix_slot(internal_prefix, mesh_count, j)];
-    lbvh_node_bounds(leaf_boxes + mm.leaf_base, bnodes + mm.internal_base, j - mm.internal_base,
-                     ibounds + mm.internal_base, ifirst + mm.internal_base);
+    std::uint32_t const end = bucket_start[level + 1];
+    for (std::uint32_t k = bucket_start[level] + blockIdx.x * k_block + threadIdx.x; k < end;
+         k += gridDim.x * k_block) {
+        std::uint32_t const j = order[k];
+        seg_mesh const mm = meshes[prefix_slot(internal_prefix, mesh_count, j)];
+        lbvh_node_bounds(leaf_boxes + mm.leaf_base, bnodes + mm.internal_base, j - mm.internal_base,
+                         ibounds + mm.internal_base, ifirst + mm.internal_base);
+    }
 }
 
 // Level 0 of every mesh's collapse: the binary root (LBVH: node 0; H-PLOC: the recorded root) of a
@@ -479,6 +611,8 @@ struct batch_trees {
     device_buffer<aabb> ibounds;
     device_buffer<std::uint32_t> ifirst, clusters, boundary, root_words, words, iparent, depth;
     device_buffer<uint2> parked;
+    device_buffer<std::uint64_t> jump;
+    device_buffer<std::uint32_t> by_depth, buckets;  // buckets: counts | starts (+1) | cursors
     device_buffer<collapse_task> tasks;
     device_buffer<collapse_level> levels;  // mesh_count x (k_batch_build_max_levels + 1)
     device_buffer<collapse_stats> stats;
@@ -607,6 +741,9 @@ batch_trees_ptr batch_trees_create(std::vector<gwn_geometry_handle const *> cons
     alloc(t.parked, hploc_parked_capacity(n, k_hploc_seg_chunk_block), stream, b);
     alloc(t.iparent, std::max<std::size_t>(1, ni), stream, b);
     alloc(t.depth, std::max<std::size_t>(1, ni), stream, b);
+    alloc(t.jump, std::max<std::size_t>(1, ni), stream, b);
+    alloc(t.by_depth, std::max<std::size_t>(1, ni), stream, b);
+    alloc(t.buckets, 3 * k_batch_build_max_levels + 1, stream, b);
     alloc(t.tasks, std::max<std::size_t>(1, ni), stream, b);
     alloc(t.levels, M * k_level_stride, stream, b);
     alloc(t.stats, M, stream, b);
@@ -712,19 +849,34 @@ void batch_trees_enqueue(batch_trees &t, int levels, int binary_levels, std::uin
     std::size_t const ni = t.internal_total;
     std::uint32_t const NI = t.internal_total;
     if (t.method == GWN_BVH_BUILD_LBVH) {
-        // Karras LBVH, binary depths, then bounds one binary level per launch, deepest first.
+        // Karras LBVH, binary depths by pointer jumping, node ids bucketed by depth, then bounds one
+        // binary depth per launch, deepest first.
         fill_u32(t.root_words.ptr, 0u, M, stream);
         if (ni > 0) {
             seg_lbvh_kernel<<<blocks_for(ni, k_block), k_block, 0, stream>>>(t.meshes.ptr, t.internal_prefix.ptr, M, NI,
                                                                              codes, t.bnodes.ptr, t.iparent.ptr);
             GWN_LAUNCH_CHECK();
-            seg_lbvh_depth_kernel<<<blocks_for(ni, k_block), k_block, 0, stream>>>(
-                t.meshes.ptr, t.internal_prefix.ptr, M, NI, t.iparent.ptr, t.depth.ptr, t.words.ptr + 3,
-                t.words.ptr + 1);
+            unsigned const nb = blocks_for(ni, k_block);
+            seg_jump_init_kernel<<<nb, k_block, 0, stream>>>(t.meshes.ptr, t.internal_prefix.ptr, M, NI, t.iparent.ptr,
+                                                            t.jump.ptr, t.words.ptr + 1);
+            GWN_LAUNCH_CHECK();
+            for (int r = 0; r < k_jump_rounds; ++r) {
+                seg_jump_kernel<<<nb, k_block, 0, stream>>>(NI, t.jump.ptr);
+                GWN_LAUNCH_CHECK();
+            }
+            std::uint32_t *counts = t.buckets.ptr;
+            std::uint32_t *starts = t.buckets.ptr + k_batch_build_max_levels;
+            std::uint32_t *cursor = t.buckets.ptr + 2 * k_batch_build_max_levels + 1;
+            fill_u32(counts, 0u, k_batch_build_max_levels, stream);
+            seg_depth_kernel<<<t.grid, k_block, 0, stream>>>(NI, t.jump.ptr, t.depth.ptr, counts, t.words.ptr + 3);
+            GWN_LAUNCH_CHECK();
+            seg_depth_scan_kernel<<<1, k_batch_build_max_levels, 0, stream>>>(counts, starts, cursor);
+            GWN_LAUNCH_CHECK();
+            seg_depth_scatter_kernel<<<t.grid, k_block, 0, stream>>>(NI, t.depth.ptr, starts, cursor, t.by_depth.ptr);
             GWN_LAUNCH_CHECK();
             for (int d = binary_levels; d-- > 0;) {
-                seg_lbvh_bounds_level_kernel<<<blocks_for(ni, k_block), k_block, 0, stream>>>(
-                    t.meshes.ptr, t.internal_prefix.ptr, M, NI, t.depth.ptr, static_cast<std::uint32_t>(d),
+                seg_lbvh_bounds_level_kernel<<<t.grid, k_block, 0, stream>>>(
+                    t.meshes.ptr, t.internal_prefix.ptr, M, t.by_depth.ptr, starts, static_cast<std::uint32_t>(d),
                     t.leaf_boxes.ptr, t.bnodes.ptr, t.ibounds.ptr, t.ifirst.ptr);
                 GWN_LAUNCH_CHECK();
             }
