@@ -484,9 +484,15 @@ typedef struct gwn_batch_tuning {
     int32_t chunk;           /* queries per work item, a multiple of block_size */
     int32_t blocks_per_cu;   /* persistent grid = blocks_per_cu x compute units */
     int32_t traversal;       /* 0: one query per lane; 1: wave-cooperative over each mesh's
-                                Hilbert query order (computed at create) */
+                                Hilbert query order (computed at create); 2: per mesh, wave-
+                                cooperative where the mesh has at least wave_queries_per_triangle
+                                queries per triangle, one query per lane elsewhere.  All three
+                                give the same results. */
     int32_t stack_capacity;  /* compiled traversal stack: 32, 64 or 128, at least every tree's
                                 internal_stack_bound; 0 in gwn_batch_set_tuning keeps it */
+    float wave_queries_per_triangle; /* traversal 2's density threshold (positive, finite); read
+                                only for traversal 2.  It fills what was padding, so the layout
+                                of every other field is unchanged. */
     uint64_t meshes;         /* read only */
     uint64_t queries;        /* read only */
 } gwn_batch_tuning;

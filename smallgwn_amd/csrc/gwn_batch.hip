@@ -24,6 +24,7 @@
 #include "gwn_core.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 // Device storage a batch was made over, per mesh: every build, upload and moment refit that
@@ -47,6 +48,7 @@ struct gwn_batch_handle {
     // over geometries adds 8 of headroom for rebuilt trees).
     std::uint32_t stack_bound = 0;
     std::vector<gwn::batch_mesh> host_meshes;
+    std::vector<std::uint64_t> mesh_triangles;        // per mesh, for the per-mesh traversal rule
     std::vector<gwn::moment_refit_work> work;
     std::vector<gwn::moment_refit_mesh> host_refits;  // meshes with an internal root
     gwn::device_buffer<gwn::moment_refit_mesh> refits;
@@ -65,7 +67,8 @@ struct gwn_batch_handle {
     int block = 256;
     int chunk = 256;
     int blocks_per_cu = 4;
-    bool wave = false;
+    int traversal = 0;                 // 0 per lane, 1 wave, 2 per mesh (wave_queries_per_triangle)
+    float wave_queries_per_triangle = 1.0f;
     int cu_count = 1;
     hipStream_t own_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -116,9 +119,21 @@ void note_work(gwn_batch_handle &b, hipStream_t stream) {
     b.have_work = true;
 }
 
+// Traversal 2: a mesh with a query order and at least wave_queries_per_triangle queries per
+// triangle walks its tree wave-cooperatively, the rest one query per lane.
+void set_wave_flags(gwn_batch_handle &b) {
+    for (std::size_t i = 0; i < b.host_meshes.size(); ++i) {
+        batch_mesh &m = b.host_meshes[i];
+        bool const dense = static_cast<double>(m.count) >=
+                           static_cast<double>(b.wave_queries_per_triangle) * static_cast<double>(b.mesh_triangles[i]);
+        m.wave = b.traversal == 1 || (b.traversal == 2 && m.order != nullptr && dense) ? 1u : 0u;
+    }
+}
+
 void upload_table(gwn_batch_handle &b, hipStream_t stream) {
     wait_last_work(b);
     layout_chunks(b);
+    set_wave_flags(b);
     GWN_HIP(hipMemcpyAsync(b.meshes.ptr, b.host_meshes.data(), b.host_meshes.size() * sizeof(batch_mesh),
                            hipMemcpyHostToDevice, stream));
     GWN_HIP(hipStreamSynchronize(stream));
@@ -174,7 +189,7 @@ void enqueue_query(gwn_batch_handle &b, hipStream_t stream) {
     int const grid = std::max(1, b.blocks_per_cu * b.cu_count);
     taylor_batch_launch(b.meshes.ptr, static_cast<std::uint32_t>(b.host_meshes.size()), b.total_chunks,
                         static_cast<std::uint32_t>(b.chunk), b.accuracy_scale, b.counter.ptr, b.order, b.cap,
-                        b.block, grid, b.wave, stream);
+                        b.block, grid, b.traversal, stream);
 }
 
 // The refit table: every mesh with an internal root, and for every depth the meshes' node ranges
@@ -315,6 +330,8 @@ gwn_batch_handle *batch_create(
         b->bvhs = bvhs;
         b->moments = moments;
         b->host_meshes = queries;
+        for (gwn_bvh_handle const *h : bvhs)
+            b->mesh_triangles.push_back(h->triangle_count);
         b->accuracy_scale = accuracy_scale;
         b->order = moments.empty() ? 1 : moments[0]->order;
         b->work.resize(bvhs.size());
@@ -365,8 +382,8 @@ gwn_batch_handle *batch_create(
     return b;
 }
 
-void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_cu, bool wave, int stack_capacity,
-                      hipStream_t stream) {
+void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_cu, int traversal,
+                      float wave_queries_per_triangle, int stack_capacity, hipStream_t stream) {
     if (block <= 0 || block > 256 || (block & 63) != 0)
         throw_invalid("Batched query block size must be 64, 128, 192 or 256.");
     if (chunk <= 0 || chunk % block != 0)
@@ -378,13 +395,20 @@ void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_
         throw_invalid("Batched query stack capacity must be 32, 64 or 128.");
     if (stack_capacity != 0 && static_cast<std::uint32_t>(stack_capacity) < b.stack_bound)
         throw_invalid("Batched query stack capacity is below a tree's traversal stack bound.");
+    if (traversal < 0 || traversal > 2)
+        throw_invalid("Batched query traversal must be 0 (per lane), 1 (wave) or 2 (per mesh).");
+    if (traversal == 2 && !(wave_queries_per_triangle > 0.0f && std::isfinite(wave_queries_per_triangle)))
+        throw_invalid("Batched query wave_queries_per_triangle must be positive and finite for traversal 2.");
     // The graph and any query in flight read the table and the counter: finish them first.
     wait_last_work(b);
-    bool const relayout = chunk != b.chunk;
+    bool const relayout = chunk != b.chunk || traversal != b.traversal ||
+                          (traversal == 2 && wave_queries_per_triangle != b.wave_queries_per_triangle);
     b.block = block;
     b.chunk = chunk;
     b.blocks_per_cu = blocks_per_cu;
-    b.wave = wave;
+    b.traversal = traversal;
+    if (traversal == 2)
+        b.wave_queries_per_triangle = wave_queries_per_triangle;
     if (stack_capacity != 0)
         b.cap = stack_capacity;
     if (relayout)
@@ -394,28 +418,31 @@ void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_
 
 // Times each candidate (traversal, block size, chunk length, blocks per CU) with one launch
 // after a warm-up and keeps the fastest, then each compiled traversal stack that holds the trees'
-// bound for that shape; returns the best time in milliseconds.
+// bound for that shape, then the per-mesh traversal at a few densities; returns the best time in
+// milliseconds.
 float batch_tune(gwn_batch_handle &b, hipStream_t stream) {
     struct cand {
         int block, chunk, bpc;
-        bool wave;
+        int traversal;
+        float density;
     };
     std::vector<cand> cands;
-    for (bool wave : {false, true})
+    float const density = b.wave_queries_per_triangle;
+    for (int traversal : {0, 1})
         for (int block : {64, 128, 256})
             for (int mult : {1, 4})
                 for (int bpc : {2, 4, 8})
-                    cands.push_back({block, block * mult, bpc, wave});
+                    cands.push_back({block, block * mult, bpc, traversal, density});
     check_storage(b, true);
     hipEvent_t e0, e1;
     GWN_HIP(hipEventCreate(&e0));
     GWN_HIP(hipEventCreate(&e1));
     float best_ms = 0.0f;
-    cand best = {b.block, b.chunk, b.blocks_per_cu, b.wave};
+    cand best = {b.block, b.chunk, b.blocks_per_cu, b.traversal, density};
     bool have = false;
     try {
         for (cand const &c : cands) {
-            batch_set_tuning(b, c.block, c.chunk, c.bpc, c.wave, 0, stream);
+            batch_set_tuning(b, c.block, c.chunk, c.bpc, c.traversal, c.density, 0, stream);
             batch_query(b, stream);
             GWN_HIP(hipEventRecord(e0, stream));
             batch_query(b, stream);
@@ -434,14 +461,14 @@ float batch_tune(gwn_batch_handle &b, hipStream_t stream) {
         (void)hipEventDestroy(e1);
         throw;
     }
-    batch_set_tuning(b, best.block, best.chunk, best.bpc, best.wave, 0, stream);
+    batch_set_tuning(b, best.block, best.chunk, best.bpc, best.traversal, best.density, 0, stream);
     // Then the compiled traversal stack for that shape: every one that holds the trees' bound.
     int best_cap = b.cap;
     try {
         for (int cap : {32, 64, 128}) {
             if (static_cast<std::uint32_t>(cap) < b.stack_bound || cap == best_cap)
                 continue;
-            batch_set_tuning(b, best.block, best.chunk, best.bpc, best.wave, cap, stream);
+            batch_set_tuning(b, best.block, best.chunk, best.bpc, best.traversal, best.density, cap, stream);
             batch_query(b, stream);
             GWN_HIP(hipEventRecord(e0, stream));
             batch_query(b, stream);
@@ -459,9 +486,33 @@ float batch_tune(gwn_batch_handle &b, hipStream_t stream) {
         (void)hipEventDestroy(e1);
         throw;
     }
+    // Then the per-mesh traversal (wave where a mesh has at least d queries per triangle) for that
+    // shape and stack.
+    try {
+        for (float d : {0.0625f, 0.25f, 1.0f, 4.0f, 16.0f}) {
+            batch_set_tuning(b, best.block, best.chunk, best.bpc, 2, d, best_cap, stream);
+            batch_query(b, stream);
+            GWN_HIP(hipEventRecord(e0, stream));
+            batch_query(b, stream);
+            GWN_HIP(hipEventRecord(e1, stream));
+            GWN_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            GWN_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.traversal = 2;
+                best.density = d;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
     GWN_HIP(hipEventDestroy(e0));
     GWN_HIP(hipEventDestroy(e1));
-    batch_set_tuning(b, best.block, best.chunk, best.bpc, best.wave, best_cap, stream);
+    batch_set_tuning(b, best.block, best.chunk, best.bpc, best.traversal, best.density, best_cap, stream);
+    b.wave_queries_per_triangle = best.density;  // the candidates above left their own
     return best_ms;
 }
 
@@ -469,7 +520,8 @@ void batch_get_tuning(gwn_batch_handle const &b, gwn_batch_tuning &t) {
     t.block_size = b.block;
     t.chunk = b.chunk;
     t.blocks_per_cu = b.blocks_per_cu;
-    t.traversal = b.wave ? 1 : 0;
+    t.traversal = b.traversal;
+    t.wave_queries_per_triangle = b.wave_queries_per_triangle;
     t.stack_capacity = b.cap;
     t.meshes = b.host_meshes.size();
     t.queries = b.total_queries;
@@ -528,6 +580,8 @@ gwn_batch_handle *batch_create_built(
         for (gwn_geometry_handle const *g : geometries)
             b->geometry_storage.push_back(g->storage_generation);
         b->host_meshes = queries;
+        for (gwn_geometry_handle const *g : geometries)
+            b->mesh_triangles.push_back(g->triangle_count);
         b->accuracy_scale = accuracy_scale;
         b->order = order;
         b->trees = batch_trees_create(geometries, method, width, hploc_radius, order, stream);

@@ -1396,10 +1396,8 @@ int gwn_batch_set_tuning(gwn_batch_t batch, const gwn_batch_tuning *tuning, hipS
     return guarded("gwn_batch_set_tuning", [&] {
         if (batch == nullptr || tuning == nullptr)
             gwn::throw_invalid("Batch handle or tuning pointer is null.");
-        if (tuning->traversal != 0 && tuning->traversal != 1)
-            gwn::throw_invalid("Batched query traversal must be 0 (per lane) or 1 (wave).");
-        gwn::batch_set_tuning(*batch, tuning->block_size, tuning->chunk, tuning->blocks_per_cu,
-                              tuning->traversal == 1, tuning->stack_capacity, stream);
+        gwn::batch_set_tuning(*batch, tuning->block_size, tuning->chunk, tuning->blocks_per_cu, tuning->traversal,
+                              tuning->wave_queries_per_triangle, tuning->stack_capacity, stream);
     });
 }
 

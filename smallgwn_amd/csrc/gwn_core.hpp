@@ -337,12 +337,13 @@ struct batch_mesh {
     std::uint64_t count;
     std::uint64_t first_chunk;   // exclusive prefix of chunk counts over earlier meshes
     std::uint32_t const *order;  // visiting order of the queries (Hilbert), or null
+    std::uint32_t wave;          // per-mesh traversal (traversal 2): 1 wave-cooperative, 0 per lane
 };
 // Persistent batched Taylor winding (plain precision): `grid` blocks of `block` threads take
 // chunks of `chunk` queries from *counter (which must be zero) until total_chunks are done.
 void taylor_batch_launch(
     batch_mesh const *meshes, std::uint32_t mesh_count, std::uint64_t total_chunks, std::uint32_t chunk,
-    float accuracy_scale, std::uint64_t *counter, int order, int cap, int block, int grid, bool wave,
+    float accuracy_scale, std::uint64_t *counter, int order, int cap, int block, int grid, int traversal,
     hipStream_t stream
 );
 // Segmented builds (gwn_batch_build.hip): the H-PLOC BVH trees and Taylor moments of many meshes,
@@ -396,7 +397,8 @@ void batch_refit(gwn_batch_handle &b, hipStream_t stream);
 void batch_query(gwn_batch_handle &b, hipStream_t stream);
 // stack_capacity: 0 keeps the current compiled traversal stack, else 32, 64 or 128 (at least the
 // trees' bound).
-void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_cu, bool wave, int stack_capacity,
+void batch_set_tuning(gwn_batch_handle &b, int block, int chunk, int blocks_per_cu, int traversal,
+                      float wave_queries_per_triangle, int stack_capacity,
                       hipStream_t stream);
 float batch_tune(gwn_batch_handle &b, hipStream_t stream);
 void batch_get_tuning(gwn_batch_handle const &b, gwn_batch_tuning &t);

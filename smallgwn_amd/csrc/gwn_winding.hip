@@ -203,14 +203,17 @@ __device__ inline bool taylor_wave_traverse(
 // consecutive queries of one mesh; every block takes chunks from a global counter until none is
 // left (no wave waits for another), and its lanes evaluate one query each with that mesh's
 // records and the per-lane traversal above, so each result equals the single-mesh batch's.
-template <int Order, int StackCap, bool Wave>
+// Mode 0: one query per lane; 1: wave-cooperative; 2: each mesh as its `wave` flag says (a wave
+// walks the union of its lanes' paths, which pays where a mesh's queries are dense and costs
+// where a few queries are spread over a large tree).  Every mode gives the same results.
+template <int Order, int StackCap, int Mode>
 __global__ __launch_bounds__(256) void taylor_batch_kernel(
     batch_mesh const *__restrict__ meshes, std::uint32_t mesh_count, std::uint64_t total_chunks, std::uint32_t chunk,
     float beta2, std::uint64_t *counter
 ) {
     __shared__ std::uint64_t s_chunk;
-    __shared__ std::uint32_t s_node[4][Wave ? StackCap : 1];
-    __shared__ std::uint64_t s_mask[4][Wave ? StackCap : 1];
+    __shared__ std::uint32_t s_node[4][Mode != 0 ? StackCap : 1];
+    __shared__ std::uint64_t s_mask[4][Mode != 0 ? StackCap : 1];
     int const wave = threadIdx.x >> 6;
     for (;;) {
         if (threadIdx.x == 0)
@@ -248,14 +251,16 @@ __global__ __launch_bounds__(256) void taylor_batch_kernel(
                 std::uint32_t const cnt = ref_count(m.root);
                 for (std::uint32_t k = 0; k < cnt; ++k)
                     acc.leaf(m.tris[off + k], x, y, z);
-            } else if constexpr (Wave) {
-                if (__ballot(active) == 0ull)
-                    continue;
-                overflow = taylor_wave_traverse<Order, GWN_PRECISION_PLAIN, false, StackCap>(
-                    m.records, m.tris, static_cast<std::uint32_t>(ref_offset(m.root)), active, x, y, z, beta2, acc,
-                    n_tests, n_accept, n_leaf, s_node[wave], s_mask[wave]);
-                if (!active)
-                    continue;
+            } else if (Mode == 1 || (Mode == 2 && m.wave != 0u)) {  // uniform over the block
+                if constexpr (Mode != 0) {
+                    if (__ballot(active) == 0ull)
+                        continue;
+                    overflow = taylor_wave_traverse<Order, GWN_PRECISION_PLAIN, false, StackCap>(
+                        m.records, m.tris, static_cast<std::uint32_t>(ref_offset(m.root)), active, x, y, z, beta2,
+                        acc, n_tests, n_accept, n_leaf, s_node[wave], s_mask[wave]);
+                    if (!active)
+                        continue;
+                }
             } else {
                 if (!active)
                     continue;
@@ -922,12 +927,12 @@ void launch_taylor_adjoint(
     GWN_LAUNCH_CHECK();
 }
 
-template <int Order, bool Wave>
+template <int Order, int Mode>
 void launch_batch_cap(batch_mesh const *meshes, std::uint32_t mesh_count, std::uint64_t total_chunks,
                       std::uint32_t chunk, float beta2, std::uint64_t *counter, int cap, int block, int grid,
                       hipStream_t stream) {
 #define GWN_BATCH(C)                                                                                      \
-    taylor_batch_kernel<Order, C, Wave><<<grid, block, 0, stream>>>(meshes, mesh_count, total_chunks, chunk, beta2, counter)
+    taylor_batch_kernel<Order, C, Mode><<<grid, block, 0, stream>>>(meshes, mesh_count, total_chunks, chunk, beta2, counter)
     if (cap <= 32)
         GWN_BATCH(32);
     else if (cap <= 64)
@@ -944,17 +949,21 @@ void launch_batch_cap(batch_mesh const *meshes, std::uint32_t mesh_count, std::u
 
 void taylor_batch_launch(
     batch_mesh const *meshes, std::uint32_t mesh_count, std::uint64_t total_chunks, std::uint32_t chunk,
-    float accuracy_scale, std::uint64_t *counter, int order, int cap, int block, int grid, bool wave,
+    float accuracy_scale, std::uint64_t *counter, int order, int cap, int block, int grid, int traversal,
     hipStream_t stream
 ) {
     float const beta2 = accuracy_scale * accuracy_scale;
     if (block <= 0 || block > 256 || (block & 63) != 0)
         throw_invalid("Batched query block size must be 64, 128, 192 or 256.");
-#define GWN_BATCH_ORDER(O)                                                                                     \
-    (wave ? launch_batch_cap<O, true>(meshes, mesh_count, total_chunks, chunk, beta2, counter, cap, block, grid, \
-                                      stream)                                                                   \
-          : launch_batch_cap<O, false>(meshes, mesh_count, total_chunks, chunk, beta2, counter, cap, block, grid, \
-                                       stream))
+    if (traversal < 0 || traversal > 2)
+        throw_invalid("Batched query traversal must be 0 (per lane), 1 (wave) or 2 (per mesh).");
+#define GWN_BATCH_ORDER(O)                                                                                      \
+    (traversal == 0   ? launch_batch_cap<O, 0>(meshes, mesh_count, total_chunks, chunk, beta2, counter, cap, block, \
+                                               grid, stream)                                                     \
+     : traversal == 1 ? launch_batch_cap<O, 1>(meshes, mesh_count, total_chunks, chunk, beta2, counter, cap, block, \
+                                               grid, stream)                                                     \
+                      : launch_batch_cap<O, 2>(meshes, mesh_count, total_chunks, chunk, beta2, counter, cap, block, \
+                                               grid, stream))
     switch (order) {
     case 0: GWN_BATCH_ORDER(0); break;
     case 1: GWN_BATCH_ORDER(1); break;

@@ -898,7 +898,8 @@ class _BatchItem(ctypes.Structure):
 
 class _BatchTuning(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int32), ("chunk", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
-                ("traversal", ctypes.c_int32), ("stack_capacity", ctypes.c_int32), ("meshes", ctypes.c_uint64),
+                ("traversal", ctypes.c_int32), ("stack_capacity", ctypes.c_int32),
+                ("wave_queries_per_triangle", ctypes.c_float), ("meshes", ctypes.c_uint64),
                 ("queries", ctypes.c_uint64)]
 
 
@@ -939,13 +940,17 @@ class BatchObject:
     def tuning(self) -> dict:
         t = _BatchTuning()
         _status(_lib.gwn_batch_get_tuning(self._h, ctypes.byref(t))).raise_if_error()
-        return {k: int(getattr(t, k)) for k, _ in _BatchTuning._fields_}
+        return {k: (float if k == "wave_queries_per_triangle" else int)(getattr(t, k))
+                for k, _ in _BatchTuning._fields_}
 
     def set_tuning(self, block_size: int, chunk: int, blocks_per_cu: int, traversal: int = 0,
-                   stack_capacity: int = 0, stream=None) -> Status:
-        """traversal 0: one query per lane; 1: wave-cooperative over each mesh's Hilbert order.
-        stack_capacity: 0 keeps the compiled traversal stack, else 32, 64 or 128."""
-        t = _BatchTuning(block_size, chunk, blocks_per_cu, traversal, stack_capacity, 0, 0)
+                   stack_capacity: int = 0, wave_queries_per_triangle: float = 1.0, stream=None) -> Status:
+        """traversal 0: one query per lane; 1: wave-cooperative over each mesh's Hilbert order;
+        2: wave-cooperative for meshes with at least wave_queries_per_triangle queries per
+        triangle, one query per lane for the rest.  stack_capacity: 0 keeps the compiled
+        traversal stack, else 32, 64 or 128."""
+        t = _BatchTuning(block_size, chunk, blocks_per_cu, traversal, stack_capacity, wave_queries_per_triangle,
+                         0, 0)
         return _status(_lib.gwn_batch_set_tuning(self._h, ctypes.byref(t), _stream(stream)))
 
     def capture(self, include_refit: bool = True) -> Status:

@@ -95,7 +95,7 @@ def test_batch_tuned_and_graph_replays(gwn, dev):
     ms = batch.tune()
     assert ms > 0.0
     t = batch.tuning()
-    assert t["block_size"] in (64, 128, 256) and t["chunk"] % t["block_size"] == 0 and t["traversal"] in (0, 1)
+    assert t["block_size"] in (64, 128, 256) and t["chunk"] % t["block_size"] == 0 and t["traversal"] in (0, 1, 2)
     batch.query().raise_if_error()
     _check(items, refs)
     batch.capture(include_refit=True).raise_if_error()
@@ -110,17 +110,26 @@ def test_batch_tuned_and_graph_replays(gwn, dev):
 def test_batch_other_tunings_and_validation(gwn, dev):
     items, refs = _setup(gwn, dev, 12, order=1, seed=5)
     batch = gwn.BatchObject(items, accuracy_scale=2.0)
-    for block, chunk, bpc, trav in ((64, 64, 1, 0), (128, 512, 2, 0), (256, 1024, 8, 0),
-                                    (64, 64, 1, 1), (128, 256, 4, 1), (256, 1024, 2, 1)):
-        batch.set_tuning(block, chunk, bpc, trav).raise_if_error()
-        assert batch.tuning()["traversal"] == trav
+    # Traversal 2 picks per mesh by queries per triangle: 1e-3 sends every mesh with a query order
+    # to the wave traversal, 1e6 none, and the middle values split this batch's meshes.
+    shapes = ((64, 64, 1, 0, 1.0), (128, 512, 2, 0, 1.0), (256, 1024, 8, 0, 1.0),
+              (64, 64, 1, 1, 1.0), (128, 256, 4, 1, 1.0), (256, 1024, 2, 1, 1.0),
+              (256, 256, 4, 2, 1e-3), (128, 512, 2, 2, 0.5), (256, 1024, 8, 2, 2.0), (64, 64, 1, 2, 1e6))
+    for block, chunk, bpc, trav, dens in shapes:
+        batch.set_tuning(block, chunk, bpc, trav, wave_queries_per_triangle=dens).raise_if_error()
+        t = batch.tuning()
+        assert t["traversal"] == trav
+        if trav == 2:
+            assert t["wave_queries_per_triangle"] == pytest.approx(dens)
         for it in items:
             it[5].fill_(float("nan"))
         batch.query().raise_if_error()
         _check(items, refs)
     assert batch.set_tuning(100, 100, 1).error == gwn.Error.INVALID_ARGUMENT
     assert batch.set_tuning(64, 96, 1).error == gwn.Error.INVALID_ARGUMENT
-    assert batch.set_tuning(64, 64, 1, 2).error == gwn.Error.INVALID_ARGUMENT
+    assert batch.set_tuning(64, 64, 1, 3).error == gwn.Error.INVALID_ARGUMENT
+    assert batch.set_tuning(64, 64, 1, 2, wave_queries_per_triangle=0.0).error == gwn.Error.INVALID_ARGUMENT
+    assert batch.set_tuning(64, 64, 1, 2, wave_queries_per_triangle=float("nan")).error == gwn.Error.INVALID_ARGUMENT
     assert batch.replay().error == gwn.Error.INVALID_ARGUMENT  # nothing captured
     mixed = list(items[:2])
     other = gwn.MomentObject(2)
