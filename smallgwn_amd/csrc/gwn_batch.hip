@@ -280,11 +280,14 @@ void plan_build(gwn_batch_handle &b, hipStream_t stream) {
 
 } // namespace
 
+// A tree cut short by the level launches (4) also leaves leaves unplaced, which sets the failed-build
+// bit (2) too: the re-plan message comes first, and a build that still fails after the re-plan
+// reports itself there.
 std::string batch_status_message(std::uint32_t bits) {
-    if (bits & 2u)
-        return "batch: A segmented BVH build failed.";
     if (bits & 4u)
         return "batch: A rebuilt tree is deeper than the batch's level launches; call gwn_batch_set_build to re-plan.";
+    if (bits & 2u)
+        return "batch: A segmented BVH build failed.";
     if (bits & 8u)
         return "batch: A rebuilt tree's traversal stack bound exceeds the batch's stack capacity.";
     return "batch: Bottom-up moment refit did not converge.";

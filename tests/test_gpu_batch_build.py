@@ -189,3 +189,59 @@ def test_built_batch_validation(gwn, dev):
     st = batch.build()
     assert not st.is_ok() and "geometry storage changed" in st.message
     assert not batch.query().is_ok()
+
+
+def _soup(centers, e):
+    """One small triangle (c, c + e x, c + e y) per center, as a triangle soup."""
+    c = np.asarray(centers, dtype=np.float64)
+    v = np.stack([c, c + [e, 0.0, 0.0], c + [0.0, e, 0.0]], 1).reshape(-1, 3).astype(np.float32)
+    return synth._mesh_from_arrays(v, np.arange(v.shape[0], dtype=np.int64).reshape(-1, 3))
+
+
+@pytest.mark.parametrize("lbvh", [False, True])
+def test_built_batch_flags_a_deeper_rebuild_and_replans(gwn, dev, lbvh):
+    """A geometry update whose tree is deeper than the levels planned at create: the pass flags it
+    (status), set_build re-plans, and the next pass equals the single-mesh build again."""
+    import torch
+
+    method = gwn.BuildMethod.LBVH if lbvh else gwn.BuildMethod.HPLOC
+    grid = [(i / 4.0, j / 12.0, 0.5) for i in range(5) for j in range(13)]  # 65 triangles
+    meshes = [_soup(grid, 0.01), _soup(grid[::-1], 0.02)]
+    # The same 65 triangles moved to single-axis points 2^-k (k = 1..21 on each axis) plus the
+    # corners (0,0,0) and (1,1,1): every point's Morton code has its own leading bit, so the
+    # binary tree is a chain about 64 deep, against about 8 for the grid.
+    chain = [(0.0, 0.0, 0.0), (1.0, 1.0, 1.0)]
+    for axis in range(3):
+        for k in range(1, 22):
+            p = [0.0, 0.0, 0.0]
+            p[axis] = 2.0 ** -k
+            chain.append(tuple(p))
+    deep = _soup(chain, 1e-9)
+    items = []
+    for k, mesh in enumerate(meshes):
+        geom = gwn.GeometryObject()
+        gwn.gwn_upload_geometry(geom, mesh.vx, mesh.vy, mesh.vz, mesh.i0, mesh.i1, mesh.i2).raise_if_error()
+        q = [torch.from_numpy(a).to(dev) for a in _queries(mesh, 300, 90 + k)]
+        items.append((geom, q[0], q[1], q[2], torch.full((300,), float("nan"), dtype=torch.float32, device=dev)))
+    batch = gwn.BuiltBatchObject(items, width=4, hploc_radius=8, order=1, accuracy_scale=2.0, method=method)
+    planned = batch.build_config()["level_launches"]
+    gwn.gwn_update_geometry(items[0][0], deep.vx, deep.vy, deep.vz).raise_if_error()
+    batch.build().raise_if_error()
+    st = batch.status()
+    if lbvh:  # the binary chain is far deeper than the binary-depth launches planned for the grid
+        assert not st.is_ok() and "re-plan" in st.message, st.message
+    elif not st.is_ok():
+        assert "re-plan" in st.message or "stack" in st.message, st.message
+    batch.set_build(4, 8, method).raise_if_error()
+    assert batch.build_config()["level_launches"] >= planned
+    want0 = _sequential(gwn, items[0][0], items[0][1:4], 4, 8, 1, method)
+    for it in items:
+        it[4].fill_(float("nan"))
+    batch.build().raise_if_error()
+    batch.query().raise_if_error()
+    batch.status().raise_if_error()
+    torch.cuda.synchronize()
+    for k, (geom, qx, qy, qz, out) in enumerate(items):
+        want = want0 if k == 0 else _sequential(gwn, geom, (qx, qy, qz), 4, 8, 1, method)
+        _same_tree(k, batch.export(k), want[:2])
+        _same_results(k, out, want[2])
