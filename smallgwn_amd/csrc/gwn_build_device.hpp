@@ -480,9 +480,7 @@ __device__ inline std::uint32_t hploc_range(
 }
 
 // H-PLOC runs in two launches with the same decisions and merges as one global climb:
-//   1. chunk climb: a workgroup takes the sorted leaves of its chunk, one per lane (1024-leaf
-//      chunks for one large tree; 256 for many small trees side by side, whose chunks would
-//      otherwise end in long single-lane tails),
+//   1. chunk climb: a workgroup takes the sorted leaves of its 1024-leaf chunk, one per lane,
 //      and climbs while the boundary a range meets lies between two leaves of the chunk: those
 //      boundaries live in LDS and workgroup-scope release / acquire order the ranges' lists and
 //      nodes (no L2 write-back).  A range whose next step meets a boundary outside the chunk
@@ -497,8 +495,11 @@ __device__ inline std::uint32_t hploc_range(
 // Which lane arrives first at a boundary can differ from one launch to the other, but each
 // boundary is met once from each side and the second arrival joins the same two ranges, so every
 // merge is the same.  A range that covers its whole tree stops after its final PLOC.
-constexpr int k_hploc_chunk_block = 1024;       // one tree
-constexpr int k_hploc_seg_chunk_block = 256;    // many trees side by side
+// Chunk lengths: one tree, and many trees side by side.  With one parked range per lane in the
+// global launch, 4,096 meshes of 113M triangles build (with their moments) in 116 / 102 / 101 ms
+// at 256 / 512 / 1024-leaf chunks.
+constexpr int k_hploc_chunk_block = 1024;
+constexpr int k_hploc_seg_chunk_block = 1024;
 
 // Parked range of lane `threadIdx.x` of block `block` in the global launch (k_hploc_block
 // threads per block, one range per 32-lane half), or k_invalid for the other lanes.
