@@ -28,6 +28,7 @@ Also reported (outside the timed steps), each with its roofline where the kernel
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import platform
@@ -315,16 +316,19 @@ def main() -> int:
             "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
             "traffic": traffic.get("hbm_bytes_per_launch"),
             "traffic_source": traffic.get("source"),
+            "traffic_current": traffic.get("current"),
             "kernel": f"taylor_wave_kernel<{args.order}, split>",
             "kernel_ms": round(t_trav, 4),
             "flops_per_launch": fp32_equiv,
             "valu_lane_utilization": valu.get("valu_lane_utilization"),
             "valu_source": valu.get("source"),
+            "valu_current": valu.get("current"),
             "note": "FP32 FLOPs from per-query work counters (10 per child test, 55 per accepted "
                     "order-2 expansion in its folded form, 87 per near-field triangle); peak is MI355X's FP32 vector "
                     "figure, which counts packed FP32, disabled on this platform (non-packed 78.6); "
                     "traffic and lane utilization come from the rocprofv3 --pmc summaries named in "
-                    "traffic_source / valu_source, measured on the commit they state",
+                    "traffic_source / valu_source, used only while the kernel's device-source digest "
+                    "equals the one they were measured at (traffic_current / valu_current)",
         },
     }
     del sub, ref
@@ -833,35 +837,65 @@ def _profile_lines(name: str) -> list[str]:
         return []
 
 
-def _profile_commit(lines) -> str | None:
+def _profile_field(lines, key) -> str | None:
     for line in lines:
-        if line.startswith("measured_at_commit:"):
+        if line.startswith(key + ":"):
             return line.split(":", 1)[1].strip()
     return None
 
 
+def traversal_source_digest() -> str:
+    """sha256 over the device source taylor_wave_kernel compiles from: the layout and winding
+    device headers and gwn_winding.hip from the wave traversal's comment to leaf_root_kernel.  A
+    counter summary records the digest it was measured at; a different digest means the kernel
+    changed since, and its figures are left out of the line instead of being reported stale."""
+    h = hashlib.sha256()
+    for rel in ("include/gwn/detail/gwn_device_layout.hpp", "include/gwn/detail/gwn_device_winding.hpp"):
+        with open(os.path.join(REPO, rel), encoding="utf-8") as f:
+            h.update(f.read().encode())
+    with open(os.path.join(REPO, "smallgwn_amd", "csrc", "gwn_winding.hip"), encoding="utf-8") as f:
+        text = f.read()
+    a = text.index("// Wave-cooperative traversal for spatially sorted queries.")
+    b = text.index("template <int Prec, bool Grad>\n__global__ __launch_bounds__(256) void leaf_root_kernel(")
+    h.update(text[a:b].encode())
+    return h.hexdigest()
+
+
+def _profile_source(name: str, lines) -> dict:
+    """Where a counter summary's figures come from, and whether it still describes the kernel."""
+    if not lines:
+        return {}
+    recorded = _profile_field(lines, "kernel_source_sha256")
+    current = traversal_source_digest()
+    return {"source": f"profiles/{name} (commit {_profile_field(lines, 'measured_at_commit')})",
+            "current": recorded == current}
+
+
 def valu_profile() -> dict:
-    """VALU counters of the traversal kernel from the committed rocprofv3 --pmc summary."""
+    """VALU counters of the traversal kernel from the committed rocprofv3 --pmc summary (left out
+    when the kernel's source digest differs from the one the summary was measured at)."""
     name = "taylor_wave_o2_large_valu.md"
     lines = _profile_lines(name)
-    out = {"source": f"profiles/{name} (commit {_profile_commit(lines)})"} if lines else {}
-    for line in lines:
+    out = _profile_source(name, lines)
+    if out.get("current"):
         for key in ("valu_lane_utilization", "valu_insts_per_wave"):
-            if line.startswith(key + ":"):
-                out[key] = float(line.split(":", 1)[1].strip())
+            v = _profile_field(lines, key)
+            if v is not None:
+                out[key] = float(v)
     return out
 
 
 def traffic_profile() -> dict:
     """HBM bytes per traversal launch, from the committed rocprofv3 --pmc summary of this
     workload (FETCH_SIZE + WRITE_SIZE of the traversal kernel, scaled by the same counters on a
-    256 MiB copy)."""
+    256 MiB copy); left out when the kernel's source digest differs."""
     name = "taylor_wave_o2_large_traffic.md"
     lines = _profile_lines(name)
-    out = {"source": f"profiles/{name} (commit {_profile_commit(lines)})"} if lines else {}
-    for line in lines:
-        if line.startswith("hbm_bytes_per_launch:"):
-            out["hbm_bytes_per_launch"] = int(line.split(":", 1)[1].strip())
+    out = _profile_source(name, lines)
+    if out.get("current"):
+        v = _profile_field(lines, "hbm_bytes_per_launch")
+        if v is not None:
+            out["hbm_bytes_per_launch"] = int(v)
     return out
 
 
